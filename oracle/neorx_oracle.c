@@ -1,0 +1,1218 @@
+/*
+ * neorx_oracle.c -- CPU restatement of the NEO-RXCHF geminal-integral and
+ * Fock-build path, used ONLY as the checker of the HIP library.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library.  The product path
+ * (libneorx_hip.so) never links, loads or calls it.
+ *
+ * What it restates (reference file:line in each function comment):
+ *   - Boys function with the reference's two-branch semantics
+ *     (AC_Integrals/common.f:317-365, XGAMMAF/iboys2).
+ *   - McMurchie-Davidson evaluation of every elementary geminal integral
+ *     the path uses: per-particle Hermite expansion (ghec,
+ *     CS_Integrals/XC1_int_tools.f:136-227), integration of the product of
+ *     Hermite Gaussians and Gaussian geminals (the reference does this with
+ *     Boys' lemma, SFBoys2 XC1_int_tools.f:230-270, and DGK123/DKGx
+ *     XC1_G3_MD.f:1876-2179; here the same integral is done with the
+ *     n-particle exponent matrix), Coulomb factors through R_tuv
+ *     (RTUV, XC1_int_tools.f:341-464) and the kinetic operator as the second
+ *     derivative of the ket (G3_MD_T1d XC1_G3_MD.f:2680-2868, pgi_Lapx1
+ *     AC_Integrals/g2.f).
+ *   - Term assembly of one primitive tuple: xcalc_GAM1_MD (gam_1_OMP.f:398),
+ *     xcalc_GAM2_MD (gam_2_OMP.f:546-1077), xcalc_GAM3_MD (gam_3_OMP.f:617-983).
+ *   - Contracted builders and symmetrisation: GAM1_OMP_MD (gam_1_OMP.f:2),
+ *     GAM2_IC1 + contract_omega2_conv (gam2_IC1_OMP.f:2-337,
+ *     gam2_CONV_OMP.f:388-579), GAM3_IC1 + contract_omega3_conv
+ *     (gam3_IC1_OMP.f:2-431, gam3_CONV_OMP.f:479-705), calc_GAM_ee
+ *     (gam_ee.f:2-168), GAM4_ICR/symm_gam4 (gam4_ICR_OMP.f:2-270,
+ *     gam_4_OMP.f:712-1192).
+ *   - Fock/energy contractions EG2IC1, EG3IC1, EG4ICR
+ *     (gam2_IC1_OMP.f:339-570, gam3_IC1_OMP.f:433-725, gam4_ICR_OMP.f:273-490).
+ *
+ * Array conventions follow the reference's Fortran arrays exactly: all index
+ * arrays are 64-bit, matrices are column-major (ELCAM(npebf,3) is stored as
+ * ELCAM[k*npebf+i]), cat(3,nat) as cat[3*iat+k], contracted tensors in the
+ * pack_GAM.f layouts (CS_Integrals/Spot_Check/pack_GAM.f:72-240).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+/* ------------------------------------------------------------------ */
+/* Boys function, reference semantics (common.f:317-341 XGAMMAF,       */
+/* common.f:351-365 iboys2: each order from its own series)           */
+/* ------------------------------------------------------------------ */
+static void xgammaf(double *F, double T, int M)
+{
+    const double TMAX = 20.0, ACCY = 1.0e-12;
+    const double SQRPI = 0.88622692545275801365;
+    double pt5pm = 0.5 + M;
+    if (T > TMAX) {
+        F[0] = SQRPI / sqrt(T);
+        for (int l = 1; l <= M; ++l) F[l] = F[l - 1] * (1.0 * l - 0.5) / T;
+    } else {
+        double ti2 = T * 2.0, expti = exp(-T);
+        double term = 0.5 / pt5pm, sumk = term;
+        for (int k = 1; k <= 1000; ++k) {
+            term = term * T / (pt5pm + k);
+            sumk = sumk + term;
+            if (term < ACCY) break;
+        }
+        F[M] = expti * sumk;
+        for (int l = M - 1; l >= 0; --l) F[l] = (F[l + 1] * ti2 + expti) / (2.0 * l + 1.0);
+    }
+}
+
+double orc_boys(i64 m, double T)
+{
+    double F[64];
+    xgammaf(F, T, (int)m);
+    return F[m];
+}
+
+static void underflow(double *x)
+{ /* AC_Integrals/underflow.f:2-12 */
+    if (fabs(*x) < 1.0e-100) *x = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Generic n-particle Gaussian-geminal integral (MD formulation)       */
+/* ------------------------------------------------------------------ */
+#define MAXN 4
+#define MAXT 7 /* per-dimension Hermite order per particle + 1 (l<=3) */
+
+typedef struct {
+    int l[3];
+    double a;
+    double A[3];
+} prim_t;
+
+enum { OP_OVL = 0, OP_NUC = 1, OP_COUL = 2, OP_KIN = 3 };
+
+typedef struct {
+    int n;
+    prim_t bra[MAXN], ket[MAXN];
+    double gam[MAXN][MAXN];  /* geminal exponent between particles i,j, bra+ket */
+    double gamk[MAXN][MAXN]; /* ket-side geminal exponents (kinetic operator) */
+    int op, a, b;
+    double C[3];
+    double mass;
+    double pi; /* value of pi in the normalisation constants */
+} gint_t;
+
+/* Hermite expansion coefficients, ghec (XC1_int_tools.f:136-227):
+ * G_i(x;a,A) G_j(x;b,B) = sum_t E_t Lambda_t(x;p,P), E_0^{00} = K_AB. */
+static void hermite_E(int il, int jl, double a, double A, double b, double B, double *Eout)
+{
+    double E[MAXT][MAXT][2 * MAXT + 2];
+    memset(E, 0, sizeof(E));
+    double p = a + b, q = a * b / p, Qx = A - B;
+    double x1 = 1.0 / (2.0 * p);
+    E[0][0][0] = exp(-q * Qx * Qx);
+    double xpa = -q * Qx / a;
+    for (int i = 1; i <= il; ++i) {
+        E[i][0][0] = xpa * E[i - 1][0][0] + E[i - 1][0][1];
+        for (int t = 1; t <= i; ++t)
+            E[i][0][t] = x1 * E[i - 1][0][t - 1] + xpa * E[i - 1][0][t] + (t + 1) * E[i - 1][0][t + 1];
+    }
+    double xpb = q * Qx / b;
+    for (int j = 1; j <= jl; ++j)
+        for (int i = 0; i <= il; ++i) {
+            E[i][j][0] = xpb * E[i][j - 1][0] + E[i][j - 1][1];
+            for (int t = 1; t <= i + j; ++t)
+                E[i][j][t] = x1 * E[i][j - 1][t - 1] + xpb * E[i][j - 1][t] + (t + 1) * E[i][j - 1][t + 1];
+        }
+    for (int t = 0; t <= il + jl; ++t) Eout[t] = E[il][jl][t];
+}
+
+/* small dense inverse (Gauss-Jordan with partial pivoting); returns det */
+static double inv_small(int n, const double *M, double *Mi)
+{
+    double a[MAXN][2 * MAXN];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            a[i][j] = M[i * n + j];
+            a[i][n + j] = (i == j) ? 1.0 : 0.0;
+        }
+    double det = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (piv != c) {
+            for (int k = 0; k < 2 * n; ++k) {
+                double t = a[c][k];
+                a[c][k] = a[piv][k];
+                a[piv][k] = t;
+            }
+            det = -det;
+        }
+        double d = a[c][c];
+        det *= d;
+        for (int k = 0; k < 2 * n; ++k) a[c][k] /= d;
+        for (int r = 0; r < n; ++r)
+            if (r != c) {
+                double f = a[r][c];
+                if (f != 0.0)
+                    for (int k = 0; k < 2 * n; ++k) a[r][k] -= f * a[c][k];
+            }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) Mi[i * n + j] = a[i][n + j];
+    return det;
+}
+
+static double binom(int n, int k)
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+/* R_tuv^(0) for exponent alpha and vector D, Boys values via iboys2 for each
+ * order (RTUV, XC1_int_tools.f:341-464). */
+/* R_tuv^(0) for exponent alpha and vector D, Boys values via iboys2 for each
+ * order (RTUV, XC1_int_tools.f:341-464).  R is (L+1)^3, index [t][u][v]. */
+static void rtuv_table(int L, double alpha, const double *D, double *R)
+{
+    int S = L + 2, S2 = S * S, S3 = S2 * S;
+    double *Rn = (double *)calloc((size_t)(L + 2) * S3, sizeof(double));
+#define RN(n, t, u, v) Rn[(size_t)(n) * S3 + (t) * S2 + (u) * S + (v)]
+    double T = alpha * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    for (int n = 0; n <= L; ++n) {
+        double F[128];
+        xgammaf(F, T, n);
+        RN(n, 0, 0, 0) = pow(-2.0 * alpha, (double)n) * F[n];
+    }
+    for (int n = L; n >= 0; --n) {
+        int m = L - n;
+        for (int t = 0; t <= m; ++t)
+            for (int u = 0; u <= m - t; ++u)
+                for (int v = 0; v <= m - t - u; ++v) {
+                    if (t == 0 && u == 0 && v == 0) continue;
+                    double val;
+                    if (t > 0) {
+                        val = D[0] * RN(n + 1, t - 1, u, v);
+                        if (t > 1) val += (t - 1) * RN(n + 1, t - 2, u, v);
+                    } else if (u > 0) {
+                        val = D[1] * RN(n + 1, t, u - 1, v);
+                        if (u > 1) val += (u - 1) * RN(n + 1, t, u - 2, v);
+                    } else {
+                        val = D[2] * RN(n + 1, t, u, v - 1);
+                        if (v > 1) val += (v - 1) * RN(n + 1, t, u, v - 2);
+                    }
+                    RN(n, t, u, v) = val;
+                }
+    }
+    int L1 = L + 1;
+    for (int t = 0; t <= L; ++t)
+        for (int u = 0; u <= L - t; ++u)
+            for (int v = 0; v <= L - t - u; ++v) R[(t * L1 + u) * L1 + v] = RN(0, t, u, v);
+#undef RN
+    free(Rn);
+}
+
+typedef struct {
+    int n;
+    double p[MAXN], P[3][MAXN];
+    double Mi[MAXN * MAXN], Q[MAXN * MAXN];
+    double detM;
+} gsetup_t;
+
+static void gsetup(const gint_t *g, gsetup_t *s)
+{
+    int n = g->n;
+    s->n = n;
+    double M[MAXN * MAXN];
+    for (int i = 0; i < n; ++i) {
+        double a = g->bra[i].a, b = g->ket[i].a;
+        s->p[i] = a + b;
+        for (int d = 0; d < 3; ++d) s->P[d][i] = (a * g->bra[i].A[d] + b * g->ket[i].A[d]) / (a + b);
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            if (i == j) {
+                double sgm = s->p[i];
+                for (int k = 0; k < n; ++k)
+                    if (k != i) sgm += g->gam[i][k];
+                M[i * n + j] = sgm;
+            } else
+                M[i * n + j] = -g->gam[i][j];
+        }
+    s->detM = inv_small(n, M, s->Mi);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) s->Q[i * n + j] = (i == j ? s->p[i] : 0.0) - s->p[i] * s->Mi[i * n + j] * s->p[j];
+}
+
+/* multi-index helpers over n particles with radix r[i] */
+static int mi_size(int n, const int *r)
+{
+    int s = 1;
+    for (int i = 0; i < n; ++i) s *= r[i];
+    return s;
+}
+static void mi_decode(int n, const int *r, int idx, int *t)
+{
+    for (int i = n - 1; i >= 0; --i) {
+        t[i] = idx % r[i];
+        idx /= r[i];
+    }
+}
+static int mi_encode(int n, const int *r, const int *t)
+{
+    int idx = 0;
+    for (int i = 0; i < n; ++i) idx = idx * r[i] + t[i];
+    return idx;
+}
+
+/* Hermite-Gaussian derivative polynomials: H[s] = d^s exp(-P'QP) / exp(-P'QP) */
+static void gauss_derivs(int n, const int *r, const double *Q, const double *P, double *H)
+{
+    int N = mi_size(n, r);
+    double QP[MAXN];
+    for (int i = 0; i < n; ++i) {
+        QP[i] = 0.0;
+        for (int j = 0; j < n; ++j) QP[i] += Q[i * n + j] * P[j];
+    }
+    int t[MAXN];
+    H[0] = 1.0;
+    for (int idx = 1; idx < N; ++idx) {
+        mi_decode(n, r, idx, t);
+        int i = 0;
+        while (t[i] == 0) ++i;
+        t[i] -= 1;
+        double val = -2.0 * QP[i] * H[mi_encode(n, r, t)];
+        for (int j = 0; j < n; ++j)
+            if (t[j] > 0) {
+                t[j] -= 1;
+                val -= 2.0 * Q[i * n + j] * (t[j] + 1) * H[mi_encode(n, r, t)];
+                t[j] += 1;
+            }
+        H[idx] = val;
+    }
+}
+
+/* per-dimension Hermite sums: V[k] = g * sum_t prod E_t sum_{s<=t,|s|=k} C(t,s) kap^s H[t-s] */
+static void dim_sums(const gint_t *g, const gsetup_t *s, int d, const int *lb, const int *lk, const double *kap,
+                     int kmax, double *V)
+{
+    int n = g->n, r[MAXN];
+    double E[MAXN][2 * MAXT];
+    for (int i = 0; i < n; ++i) {
+        r[i] = lb[i] + lk[i] + 1;
+        hermite_E(lb[i], lk[i], g->bra[i].a, g->bra[i].A[d], g->ket[i].a, g->ket[i].A[d], E[i]);
+    }
+    /* shift centres relative to particle 0 (translation invariance Q.1 = 0) */
+    double P[MAXN];
+    for (int i = 0; i < n; ++i) P[i] = s->P[d][i] - s->P[d][0];
+    int N = mi_size(n, r);
+    double *H = (double *)malloc(sizeof(double) * N);
+    gauss_derivs(n, r, s->Q, P, H);
+    double qf = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) qf += P[i] * s->Q[i * n + j] * P[j];
+    double gval = exp(-qf);
+    for (int k = 0; k <= kmax; ++k) V[k] = 0.0;
+    int t[MAXN], sv[MAXN], rs[MAXN], ts[MAXN];
+    for (int idx = 0; idx < N; ++idx) {
+        mi_decode(n, r, idx, t);
+        double ep = 1.0;
+        for (int i = 0; i < n; ++i) ep *= E[i][t[i]];
+        if (ep == 0.0) continue;
+        if (kmax == 0) {
+            V[0] += ep * H[idx];
+            continue;
+        }
+        for (int i = 0; i < n; ++i) rs[i] = t[i] + 1;
+        int Ns = mi_size(n, rs);
+        for (int js = 0; js < Ns; ++js) {
+            mi_decode(n, rs, js, sv);
+            int ks = 0;
+            double c = 1.0;
+            for (int i = 0; i < n; ++i) {
+                ks += sv[i];
+                c *= binom(t[i], sv[i]) * pow(kap[i], sv[i]);
+                ts[i] = t[i] - sv[i];
+            }
+            if (ks > kmax) continue;
+            V[ks] += ep * c * H[mi_encode(n, r, ts)];
+        }
+    }
+    for (int k = 0; k <= kmax; ++k) V[k] *= gval;
+    free(H);
+}
+
+static const double PI_CS = 3.14159265358979; /* XC1_G3_MD.f:125 */
+
+/* 1-D overlap with explicit bra/ket powers (per-dimension factor of the
+ * n-particle overlap; the constant pi^{n/2} det(M)^{-1/2} is included). */
+static double ovl1d(const gint_t *g, const gsetup_t *s, int d, const int *lb, const int *lk)
+{
+    for (int i = 0; i < g->n; ++i)
+        if (lk[i] < 0 || lb[i] < 0) return 0.0;
+    double V[1];
+    dim_sums(g, s, d, lb, lk, NULL, 0, V);
+    return V[0] * pow(g->pi, 0.5 * g->n) / sqrt(s->detM);
+}
+
+/* Second derivative of the ket along dimension d for particle a, written as
+ * overlaps with raised/lowered ket powers.  General form of G3_MD_T1d
+ * (XC1_G3_MD.f:2680-2868) / pgi_Lapx1 (AC_Integrals/g2.f): with
+ * X = x_a - B_a, h = beta X + sum_c g_c (x_a - x_c):
+ *   ket'' / ket = L(L-1) X^-2 - 4 L h X^-1 + 4 h^2 - 2 (beta + sum_c g_c). */
+static double kin1d(const gint_t *g, const gsetup_t *s, int d)
+{
+    int n = g->n, a = g->a;
+    int lb[MAXN], lk[MAXN];
+    for (int i = 0; i < n; ++i) {
+        lb[i] = g->bra[i].l[d];
+        lk[i] = g->ket[i].l[d];
+    }
+    int L = lk[a];
+    double beta = g->ket[a].a;
+    double gc[MAXN], dc[MAXN];
+    double alpha = beta, delta = 0.0;
+    for (int c = 0; c < n; ++c) {
+        gc[c] = (c == a) ? 0.0 : g->gamk[a][c];
+        dc[c] = g->ket[a].A[d] - g->ket[c].A[d];
+        alpha += gc[c];
+        delta += gc[c] * dc[c];
+    }
+    /* S(ea, c1, e1, c2, e2): overlap with ket power of a shifted by ea and of
+       particles c1,c2 raised by e1,e2 */
+#define SOV(ea, c1, e1, c2, e2)                                                                                    \
+    ({                                                                                                             \
+        int _k[MAXN];                                                                                              \
+        for (int _i = 0; _i < n; ++_i) _k[_i] = lk[_i];                                                            \
+        _k[a] += (ea);                                                                                             \
+        if ((c1) >= 0) _k[(c1)] += (e1);                                                                           \
+        if ((c2) >= 0) _k[(c2)] += (e2);                                                                           \
+        ovl1d(g, s, d, lb, _k);                                                                                    \
+    })
+    double ans = 0.0;
+    double s0 = SOV(0, -1, 0, -1, 0);
+    if (L >= 2) ans += (double)(L * (L - 1)) * SOV(-2, -1, 0, -1, 0);
+    if (L >= 1) {
+        double t = alpha * s0 + delta * SOV(-1, -1, 0, -1, 0);
+        for (int c = 0; c < n; ++c)
+            if (gc[c] != 0.0) t -= gc[c] * SOV(-1, c, 1, -1, 0);
+        ans -= 4.0 * L * t;
+    }
+    double h2 = alpha * alpha * SOV(2, -1, 0, -1, 0) + delta * delta * s0 + 2.0 * alpha * delta * SOV(1, -1, 0, -1, 0);
+    for (int c = 0; c < n; ++c) {
+        if (gc[c] == 0.0) continue;
+        h2 -= 2.0 * alpha * gc[c] * SOV(1, c, 1, -1, 0);
+        h2 -= 2.0 * delta * gc[c] * SOV(0, c, 1, -1, 0);
+        for (int c2 = 0; c2 < n; ++c2) {
+            if (gc[c2] == 0.0) continue;
+            if (c2 == c)
+                h2 += gc[c] * gc[c] * SOV(0, c, 2, -1, 0);
+            else
+                h2 += gc[c] * gc[c2] * SOV(0, c, 1, c2, 1);
+        }
+    }
+    ans += 4.0 * h2 - 2.0 * alpha * s0;
+#undef SOV
+    return ans;
+}
+
+static double gint_eval(const gint_t *g)
+{
+    gsetup_t s;
+    gsetup(g, &s);
+    int n = g->n;
+    if (g->op == OP_OVL || g->op == OP_KIN) {
+        double S[3], T[3];
+        for (int d = 0; d < 3; ++d) {
+            int lb[MAXN], lk[MAXN];
+            for (int i = 0; i < n; ++i) {
+                lb[i] = g->bra[i].l[d];
+                lk[i] = g->ket[i].l[d];
+            }
+            S[d] = ovl1d(g, &s, d, lb, lk);
+        }
+        if (g->op == OP_OVL) return S[0] * S[1] * S[2];
+        for (int d = 0; d < 3; ++d) T[d] = kin1d(g, &s, d);
+        double t = T[0] * S[1] * S[2] + S[0] * T[1] * S[2] + S[0] * S[1] * T[2];
+        return -0.5 * t / g->mass;
+    }
+    /* Coulomb-type: e = e_a (NUC, c0 = C) or e_a - e_b (COUL) */
+    double e[MAXN] = {0}, w[MAXN], kap[MAXN];
+    e[g->a] = 1.0;
+    if (g->op == OP_COUL) e[g->b] = -1.0;
+    double sden = 0.0;
+    for (int i = 0; i < n; ++i) {
+        w[i] = 0.0;
+        for (int j = 0; j < n; ++j) w[i] += s.Mi[i * n + j] * e[j];
+        sden += e[i] * w[i];
+    }
+    double rho = 1.0 / sden;
+    for (int i = 0; i < n; ++i) kap[i] = s.p[i] * w[i];
+    double D[3];
+    int kmax[3];
+    double V[3][4 * MAXN * MAXT];
+    for (int d = 0; d < 3; ++d) {
+        int lb[MAXN], lk[MAXN];
+        kmax[d] = 0;
+        for (int i = 0; i < n; ++i) {
+            lb[i] = g->bra[i].l[d];
+            lk[i] = g->ket[i].l[d];
+            kmax[d] += lb[i] + lk[i];
+        }
+        dim_sums(g, &s, d, lb, lk, kap, kmax[d], V[d]);
+        /* D = kap.P - c0 evaluated with the same origin shift as dim_sums */
+        double dd = 0.0;
+        for (int i = 0; i < n; ++i) dd += kap[i] * (s.P[d][i] - s.P[d][0]);
+        double sk = 0.0;
+        for (int i = 0; i < n; ++i) sk += kap[i];
+        dd += sk * s.P[d][0];
+        if (g->op == OP_NUC) dd -= g->C[d];
+        D[d] = dd;
+    }
+    int L = kmax[0] + kmax[1] + kmax[2];
+    int L1 = L + 1;
+    double *R = (double *)malloc(sizeof(double) * L1 * L1 * L1);
+    rtuv_table(L, rho, D, R);
+    double sum = 0.0;
+    for (int kx = 0; kx <= kmax[0]; ++kx)
+        for (int ky = 0; ky <= kmax[1]; ++ky)
+            for (int kz = 0; kz <= kmax[2]; ++kz) sum += V[0][kx] * V[1][ky] * V[2][kz] * R[(kx * L1 + ky) * L1 + kz];
+    free(R);
+    double pref = pow(g->pi, 1.5 * n) / (s.detM * sqrt(s.detM)) * 2.0 * sqrt(rho / g->pi);
+    return pref * sum;
+}
+
+/* ------------------------------------------------------------------ */
+/* Reference-shaped wrappers of the elementary integrals               */
+/* ------------------------------------------------------------------ */
+static void gi_init(gint_t *g, int n)
+{
+    memset(g, 0, sizeof(*g));
+    g->n = n;
+    g->mass = 1.0;
+    g->pi = PI_CS;
+}
+static void gi_gem(gint_t *g, int i, int j, double gA, double gB)
+{
+    g->gam[i][j] += gA + gB;
+    g->gam[j][i] += gA + gB;
+    g->gamk[i][j] += gB;
+    g->gamk[j][i] += gB;
+}
+
+/* G3_MD_xggs / G3_MD_KE / G3_MD_xgVxCg / G3_MD_xgVeeg (XC1_G3_MD.f:2, 2546,
+ * 711, 1376): particles in argument order 1,2,3. */
+static double g3(int op, const prim_t *b1, const prim_t *b2, const prim_t *b3, const prim_t *k1, const prim_t *k2,
+                 const prim_t *k3, double gA12, double gA13, double gA23, double gB12, double gB13, double gB23,
+                 const double *C, double xmass)
+{
+    gint_t g;
+    gi_init(&g, 3);
+    g.bra[0] = *b1; g.bra[1] = *b2; g.bra[2] = *b3;
+    g.ket[0] = *k1; g.ket[1] = *k2; g.ket[2] = *k3;
+    gi_gem(&g, 0, 1, gA12, gB12);
+    gi_gem(&g, 0, 2, gA13, gB13);
+    gi_gem(&g, 1, 2, gA23, gB23);
+    g.op = op;
+    g.a = 0;
+    g.b = 1;
+    g.mass = xmass;
+    if (C) memcpy(g.C, C, sizeof(g.C));
+    return gint_eval(&g);
+}
+
+/* G2_MD_xggs / G2_MD_xggvec / G2_MD_xggvee / pgike1 (XC1_G2_MD.f:3,190,630;
+ * AC_Integrals/g2.f:545). */
+static double g2(int op, const prim_t *b1, const prim_t *b2, const prim_t *k1, const prim_t *k2, double gA, double gB,
+                 const double *C, double xmass)
+{
+    gint_t g;
+    gi_init(&g, 2);
+    g.bra[0] = *b1; g.bra[1] = *b2;
+    g.ket[0] = *k1; g.ket[1] = *k2;
+    gi_gem(&g, 0, 1, gA, gB);
+    g.op = op;
+    g.a = 0;
+    g.b = 1;
+    g.mass = xmass;
+    if (C) memcpy(g.C, C, sizeof(g.C));
+    return gint_eval(&g);
+}
+
+/* pgike1 (AC_Integrals/g2.f:545-653): kinetic energy of particle 1 with the
+ * ket geminal; its overlaps (pgi_hint0_1D, g2.f:200-236) use the constant
+ * PI = 3.141592650d0 (g2.f:216), which is kept here for parity. */
+static const double PI_AC_G2 = 3.141592650;
+static double pgike1(const prim_t *b1, const prim_t *b2, const prim_t *k1, const prim_t *k2, double gA, double gB,
+                     double xmass)
+{
+    gint_t g;
+    gi_init(&g, 2);
+    g.bra[0] = *b1; g.bra[1] = *b2;
+    g.ket[0] = *k1; g.ket[1] = *k2;
+    gi_gem(&g, 0, 1, gA, gB);
+    g.op = OP_KIN;
+    g.a = 0;
+    g.mass = xmass;
+    g.pi = PI_AC_G2;
+    return gint_eval(&g);
+}
+
+/* gfvee (vee.f:3), gfke (ke.f:2), gfvec (vec.f:2) */
+static double gfvee(const prim_t *b1, const prim_t *b2, const prim_t *k1, const prim_t *k2)
+{
+    return g2(OP_COUL, b1, b2, k1, k2, 0.0, 0.0, NULL, 1.0);
+}
+static double g1(int op, const prim_t *b, const prim_t *k, const double *C, double xmass)
+{
+    gint_t g;
+    gi_init(&g, 1);
+    g.bra[0] = *b;
+    g.ket[0] = *k;
+    g.op = op;
+    g.mass = xmass;
+    if (C) memcpy(g.C, C, sizeof(g.C));
+    return gint_eval(&g);
+}
+
+/* G4_MD_xgVeeg (XC1_G4_MD.f:796): 1/r12 with geminals g14, g34.
+ * G4_MD_xgVepg (XC1_G4_MD.f:1559): 1/r34 with geminals g14, g24. */
+static double g4(int which, const prim_t *b, const prim_t *k, double gx, double gy)
+{
+    gint_t g;
+    gi_init(&g, 4);
+    for (int i = 0; i < 4; ++i) {
+        g.bra[i] = b[i];
+        g.ket[i] = k[i];
+    }
+    g.op = OP_COUL;
+    if (which == 0) { /* Veeg */
+        gi_gem(&g, 0, 3, gx, 0.0);
+        gi_gem(&g, 2, 3, gy, 0.0);
+        g.a = 0;
+        g.b = 1;
+    } else { /* Vepg */
+        gi_gem(&g, 0, 3, gx, 0.0);
+        gi_gem(&g, 1, 3, gy, 0.0);
+        g.a = 2;
+        g.b = 3;
+    }
+    return gint_eval(&g);
+}
+
+/* ------------------------------------------------------------------ */
+/* Primitive tuple term assembly                                      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    i64 nat, ngtg;
+    double pmass;
+    const double *cat, *zan, *bcoef, *gamma;
+} sys_t;
+
+/* xcalc_GAM1_MD (gam_1_OMP.f:398-839): electron e1 = particle 1, proton = 2 */
+static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const prim_t *e1k, const prim_t *pk,
+                      double *ansE, double *ansS)
+{
+    double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
+    double vcegx = 0, vcegg = 0, vcpgx = 0, vcpgg = 0, vmgx = 0, vmgg = 0;
+    for (i64 k = 0; k < S->ngtg; ++k) {
+        double gv = S->gamma[k], bv = S->bcoef[k];
+        tegx += bv * pgike1(e1b, pb, e1k, pk, gv, 0.0, 1.0);
+        tpgx += bv * pgike1(pb, e1b, pk, e1k, gv, 0.0, S->pmass);
+        double vce = 0, vcp = 0;
+        for (i64 c = 0; c < S->nat; ++c) {
+            const double *C = S->cat + 3 * c;
+            vce += g2(OP_NUC, e1b, pb, e1k, pk, gv, 0.0, C, 1.0) * S->zan[c];
+            vcp += g2(OP_NUC, pb, e1b, pk, e1k, gv, 0.0, C, 1.0) * S->zan[c];
+        }
+        double vm = g2(OP_COUL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
+        vcegx += bv * vce;
+        vcpgx += bv * vcp;
+        vmgx += bv * vm;
+        texg += bv * pgike1(e1b, pb, e1k, pk, 0.0, gv, 1.0);
+        tpxg += bv * pgike1(pb, e1b, pk, e1k, 0.0, gv, S->pmass);
+        for (i64 l = 0; l < S->ngtg; ++l) {
+            double gw = S->gamma[l], bw = S->bcoef[l];
+            tegg += bv * bw * pgike1(e1b, pb, e1k, pk, gv, gw, 1.0);
+            tpgg += bv * bw * pgike1(pb, e1b, pk, e1k, gv, gw, S->pmass);
+            double vce2 = 0, vcp2 = 0;
+            for (i64 c = 0; c < S->nat; ++c) {
+                const double *C = S->cat + 3 * c;
+                vce2 += g2(OP_NUC, e1b, pb, e1k, pk, gv, gw, C, 1.0) * S->zan[c];
+                vcp2 += g2(OP_NUC, pb, e1b, pk, e1k, gv, gw, C, 1.0) * S->zan[c];
+            }
+            double vm2 = g2(OP_COUL, e1b, pb, e1k, pk, gv, gw, NULL, 1.0);
+            vcegg += bv * bw * vce2;
+            vcpgg += bv * bw * vcp2;
+            vmgg += bv * bw * vm2;
+        }
+    }
+    double te = tegx + texg + tegg, tp = tpgx + tpxg + tpgg;
+    double vce = 2.0 * vcegx + vcegg, vcp = 2.0 * vcpgx + vcpgg, vmx = 2.0 * vmgx + vmgg;
+    *ansE = te + tp - vce + vcp - vmx;
+    double ogx = 0, ogg = 0;
+    for (i64 k = 0; k < S->ngtg; ++k) {
+        double gv = S->gamma[k], bv = S->bcoef[k];
+        ogx += bv * g2(OP_OVL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
+        for (i64 l = 0; l < S->ngtg; ++l)
+            ogg += bv * S->bcoef[l] * g2(OP_OVL, e1b, pb, e1k, pk, gv, S->gamma[l], NULL, 1.0);
+    }
+    *ansS = 0.0 + 2.0 * ogx + ogg;
+}
+
+/* xcalc_GAM2_MD (gam_2_OMP.f:546-1077): particles e1, e2, p */
+static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE, double *ansS)
+{
+    const prim_t *b1 = &b[0], *b2 = &b[1], *b3 = &b[2], *k1 = &k[0], *k2 = &k[1], *k3 = &k[2];
+    double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik) {
+        double gk = S->gamma[ik], bk = S->bcoef[ik];
+        double xgTE = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, 0, NULL, 1.0);
+        double xgVEC = 0.0;
+        for (i64 c = 0; c < S->nat; ++c) {
+            double v = g3(OP_NUC, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, 0, S->cat + 3 * c, 1.0);
+            underflow(&v);
+            xgVEC -= S->zan[c] * v;
+        }
+        double xgVEP = g3(OP_COUL, b1, b3, b2, k1, k3, k2, 0, 0, gk, 0, 0, 0, NULL, 1.0);
+        underflow(&xgVEP);
+        xgVEP = -xgVEP;
+        double xgVEE = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, 0, NULL, 1.0);
+        underflow(&xgVEE);
+        gTE += bk * xgTE;
+        gVEC += bk * xgVEC;
+        gVEP += bk * xgVEP;
+        gVEE += bk * xgVEE;
+    }
+    double gTPg = 0, gTEg1 = 0, gTEg2 = 0, gTEg3 = 0, gVPCg = 0, gVECg1 = 0, gVECg2 = 0;
+    double gVEPg1 = 0, gVEPg2 = 0, gVEEg1 = 0, gVEEg2 = 0, gsg = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            double xgTPg = g3(OP_KIN, b3, b1, b2, k3, k1, k2, gk, 0, 0, 0, gl, 0, NULL, S->pmass);
+            double xgTEg1 = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            double xgTEg2 = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, gl, 0, NULL, 1.0);
+            double xgTEg3 = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, gl, NULL, 1.0);
+            double xgVPCg = 0, xgVECg1 = 0, xgVECg2 = 0;
+            for (i64 c = 0; c < S->nat; ++c) {
+                const double *C = S->cat + 3 * c;
+                double z = S->zan[c], v;
+                v = g3(OP_NUC, b3, b1, b2, k3, k1, k2, gk, 0, 0, 0, gl, 0, C, 1.0);
+                underflow(&v);
+                xgVPCg += z * v;
+                v = g3(OP_NUC, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, C, 1.0);
+                underflow(&v);
+                xgVECg1 -= z * v;
+                v = g3(OP_NUC, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, gl, C, 1.0);
+                underflow(&v);
+                xgVECg2 -= z * v;
+            }
+            double xgVEPg1 = g3(OP_COUL, b1, b3, b2, k1, k3, k2, gk, 0, 0, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEPg1);
+            xgVEPg1 = -xgVEPg1;
+            double xgVEPg2 = g3(OP_COUL, b1, b3, b2, k1, k3, k2, 0, 0, gk, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEPg2);
+            xgVEPg2 = -xgVEPg2;
+            double xgVEEg1 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, gl, 0, NULL, 1.0);
+            underflow(&xgVEEg1);
+            double xgVEEg2 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEEg2);
+            double xgsg = g3(OP_OVL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            gTPg += bb * xgTPg;
+            gTEg1 += bb * xgTEg1;
+            gTEg2 += bb * xgTEg2;
+            gTEg3 += bb * xgTEg3;
+            gVPCg += bb * xgVPCg;
+            gVECg1 += bb * xgVECg1;
+            gVECg2 += bb * xgVECg2;
+            gVEPg1 += bb * xgVEPg1;
+            gVEPg2 += bb * xgVEPg2;
+            gVEEg1 += bb * xgVEEg1;
+            gVEEg2 += bb * xgVEEg2;
+            gsg += bb * xgsg;
+        }
+    *ansE = 2.0 * (gTE + gVEC + gVEP) + 4.0 * gVEE * 0.5 + gTPg + gVPCg + gTEg1 + gTEg3 + 2.0 * (gVEPg1 + gVECg1) +
+            gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
+    *ansS = gsg;
+}
+
+/* xcalc_GAM3_MD (gam_3_OMP.f:617-983): particles e1, e2, e3, p */
+static void gam3_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
+{
+    double gVEE = 0.0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik) {
+        double xx = g2(OP_OVL, &b[0], &b[3], &k[0], &k[3], S->gamma[ik], 0.0, NULL, 1.0);
+        double yy = gfvee(&b[1], &b[2], &k[1], &k[2]);
+        underflow(&xx);
+        underflow(&yy);
+        gVEE += S->bcoef[ik] * (xx * yy);
+    }
+    double gHEg = 0, gVEEg1 = 0, gVEEg2 = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            double xke = g1(OP_KIN, &b[2], &k[2], NULL, 1.0);
+            double vc = 0.0;
+            for (i64 c = 0; c < S->nat; ++c) vc += S->zan[c] * g1(OP_NUC, &b[2], &k[2], S->cat + 3 * c, 1.0);
+            double xx = xke + vc * (-1.0);
+            double yy = g3(OP_OVL, &b[0], &b[1], &b[3], &k[0], &k[1], &k[3], 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            double zz = g4(1, b, k, gk, gl);
+            underflow(&xx);
+            underflow(&yy);
+            underflow(&zz);
+            double xgHEg = (xx * yy) + (zz * (-1.0));
+            xx = g2(OP_OVL, &b[0], &b[3], &k[0], &k[3], gk, gl, NULL, 1.0);
+            yy = gfvee(&b[1], &b[2], &k[1], &k[2]);
+            underflow(&xx);
+            underflow(&yy);
+            double xgVEEg1 = xx * yy;
+            double xgVEEg2 = g4(0, b, k, gk, gl);
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            gHEg += bb * xgHEg;
+            gVEEg1 += bb * xgVEEg1;
+            gVEEg2 += bb * xgVEEg2;
+        }
+    *ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
+}
+
+/* ------------------------------------------------------------------ */
+/* Basis access (Fortran arrays of the reference)                      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    i64 nebf, npebf, npbf;
+    const i64 *kpestr, *kpeend, *ampeb2c; /* 1-based */
+    const double *agebfcc, *agnbfcc;
+    const double *elcex, *nucex;
+    const i64 *elcam, *nucam;
+    const double *elcbfc, *nucbfc;
+} basis_t;
+
+static prim_t eprim(const basis_t *B, i64 i)
+{ /* i: 0-based primitive index */
+    prim_t p;
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)B->elcam[d * B->npebf + i];
+        p.A[d] = B->elcbfc[d * B->npebf + i];
+    }
+    p.a = B->elcex[i];
+    return p;
+}
+static prim_t pprim(const basis_t *B, i64 i)
+{
+    prim_t p;
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)B->nucam[d * B->npbf + i];
+        p.A[d] = B->nucbfc[d * B->npbf + i];
+    }
+    p.a = B->nucex[i];
+    return p;
+}
+
+/* ------------------------------------------------------------------ */
+/* Exported entry points (ctypes)                                      */
+/* ------------------------------------------------------------------ */
+#define SYS_ARGS i64 nat, i64 ngtg, double pmass, const double *cat, const double *zan, const double *bcoef, const double *gamma
+#define SYS_INIT sys_t S = {nat, ngtg, pmass, cat, zan, bcoef, gamma}
+#define BAS_ARGS                                                                                                   \
+    i64 nebf, i64 npebf, i64 npbf, const i64 *kpestr, const i64 *kpeend, const i64 *ampeb2c, const double *agebfcc, \
+        const double *agnbfcc, const double *elcex, const double *nucex, const i64 *elcam, const i64 *nucam,       \
+        const double *elcbfc, const double *nucbfc
+#define BAS_INIT                                                                                                   \
+    basis_t B = {nebf, npebf, npbf, kpestr, kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc}
+
+/* Evaluate a list of primitive tuples.  idx holds, per tuple, 0-based
+ * primitive indices: cls 1 -> (ie1, ip, je1, jp); cls 2 -> (ie1,ie2,ip,je1,je2,jp);
+ * cls 3 -> (ie1,ie2,ie3,ip,je1,je2,je3,jp).  Outputs ansE (and ansS for
+ * cls 1,2). */
+void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
+{
+    BAS_INIT;
+    SYS_INIT;
+    int w = (int)(2 * (cls + 1));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (i64 t = 0; t < ntup; ++t) {
+        const i64 *x = idx + t * w;
+        prim_t b[4], k[4];
+        int ne = (int)cls;
+        for (int i = 0; i < ne; ++i) {
+            b[i] = eprim(&B, x[i]);
+            k[i] = eprim(&B, x[ne + 1 + i]);
+        }
+        b[ne] = pprim(&B, x[ne]);
+        k[ne] = pprim(&B, x[2 * ne + 1]);
+        double e = 0, s = 0;
+        if (cls == 1)
+            gam1_prim(&S, &b[0], &b[1], &k[0], &k[1], &e, &s);
+        else if (cls == 2)
+            gam2_prim(&S, b, k, &e, &s);
+        else
+            gam3_prim(&S, b, k, &e);
+        outE[t] = e;
+        if (outS) outS[t] = s;
+    }
+}
+
+/* Elementary integral probe: op (0 ovl,1 nuc,2 coul,3 kin), n particles,
+ * powers[2n][3], exps[2n], centres[2n][3] (bra then ket), gem[n][n] bra and
+ * gemk[n][n] ket geminal exponents, a,b,C,mass. */
+double orc_gint(i64 op, i64 n, const i64 *pw, const double *ex, const double *cen, const double *gemA,
+                const double *gemB, i64 a, i64 b, const double *C, double mass, double pi)
+{
+    gint_t g;
+    gi_init(&g, (int)n);
+    if (pi > 0.0) g.pi = pi;
+    for (int i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            g.bra[i].l[d] = (int)pw[3 * i + d];
+            g.ket[i].l[d] = (int)pw[3 * (n + i) + d];
+            g.bra[i].A[d] = cen[3 * i + d];
+            g.ket[i].A[d] = cen[3 * (n + i) + d];
+        }
+        g.bra[i].a = ex[i];
+        g.ket[i].a = ex[n + i];
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) gi_gem(&g, i, j, gemA[i * n + j], gemB[i * n + j]);
+    g.op = (int)op;
+    g.a = (int)a;
+    g.b = (int)b;
+    g.mass = mass;
+    if (C) memcpy(g.C, C, sizeof(g.C));
+    return gint_eval(&g);
+}
+
+/* index_GAM_2PK / index_GAM_3PK (pack_GAM.f:72-147), 0-based */
+static inline i64 ix2(i64 ne, i64 np, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i64 j2)
+{
+    return ((((ip * np + jp) * ne + i1) * ne + j1) * ne + i2) * ne + j2;
+}
+static inline i64 ix3(i64 ne, i64 np, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i64 j2, i64 i3, i64 j3)
+{
+    return ix2(ne, np, ip, jp, i1, j1, i2, j2) * ne * ne + i3 * ne + j3;
+}
+
+/* GAM1_OMP_MD (gam_1_OMP.f:2-397): contracted e-p integrals, layout
+ * pack_4D(nebf,nebf,npbf, jec1,iec1,jp,ip) */
+void orc_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ng1 = nebf * nebf * npbf * npbf;
+    memset(GAM1, 0, sizeof(double) * ng1);
+    memset(GAM1s, 0, sizeof(double) * ng1);
+    for (i64 ip = 0; ip < npbf; ++ip)
+        for (i64 jp = 0; jp < npbf; ++jp)
+            for (i64 ie = 0; ie < npebf; ++ie)
+                for (i64 je = 0; je < npebf; ++je) {
+                    prim_t eb = eprim(&B, ie), ek = eprim(&B, je), pb = pprim(&B, ip), pk = pprim(&B, jp);
+                    double e, s;
+                    gam1_prim(&S, &eb, &pb, &ek, &pk, &e, &s);
+                    i64 ic = ampeb2c[ie] - 1, jc = ampeb2c[je] - 1;
+                    double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[ie] * agebfcc[je];
+                    i64 ia = ((ip * npbf + jp) * nebf + ic) * nebf + jc;
+                    GAM1[ia] += e * cf;
+                    GAM1s[ia] += s * cf;
+                }
+}
+
+/* GAM2_IC1 (gam2_IC1_OMP.f:2-212) with contract_omega2_conv
+ * (gam2_CONV_OMP.f:388-579) and the (x12+x21)/2 symmetrisation. */
+void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng2 = ne * ne * ne * ne * np * np;
+    double *X = (double *)malloc(sizeof(double) * ng2), *Xs = (double *)malloc(sizeof(double) * ng2);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            double om = 0.0, oms = 0.0;
+                            for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                        for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2) {
+                                            prim_t b[3] = {eprim(&B, a1), eprim(&B, a2), pprim(&B, ip)};
+                                            prim_t k[3] = {eprim(&B, c1), eprim(&B, c2), pprim(&B, jp)};
+                                            double e, s;
+                                            gam2_prim(&S, b, k, &e, &s);
+                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                        agebfcc[a2] * agebfcc[c2];
+                                            om += e * cf;
+                                            oms += s * cf;
+                                        }
+                            i64 ia = ix2(ne, np, ip, jp, i1, j1, i2, j2);
+                            X[ia] = om;
+                            Xs[ia] = oms;
+                        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i2, j2, i1, j1);
+                            GM2[a12] = (X[a12] + X[a21]) / 2.0;
+                            GM2s[a12] = (Xs[a12] + Xs[a21]) / 2.0;
+                        }
+    free(X);
+    free(Xs);
+}
+
+/* GAM3_IC1 (gam3_IC1_OMP.f:2-263) with contract_omega3_conv
+ * (gam3_CONV_OMP.f:479-705) and the six-permutation average. */
+void orc_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    double *X = (double *)malloc(sizeof(double) * ng3);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    double om = 0.0;
+                                    for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                        for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                            for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                                for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2)
+                                                    for (i64 a3 = kpestr[i3] - 1; a3 < kpeend[i3]; ++a3)
+                                                        for (i64 c3 = kpestr[j3] - 1; c3 < kpeend[j3]; ++c3) {
+                                                            prim_t b[4] = {eprim(&B, a1), eprim(&B, a2), eprim(&B, a3),
+                                                                           pprim(&B, ip)};
+                                                            prim_t k[4] = {eprim(&B, c1), eprim(&B, c2), eprim(&B, c3),
+                                                                           pprim(&B, jp)};
+                                                            double e;
+                                                            gam3_prim(&S, b, k, &e);
+                                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] *
+                                                                        agebfcc[c1] * agebfcc[a2] * agebfcc[c2] *
+                                                                        agebfcc[a3] * agebfcc[c3];
+                                                            om += e * cf;
+                                                        }
+                                    X[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)] = om;
+                                }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    double x = X[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)] +
+                                               X[ix3(ne, np, ip, jp, i1, j1, i3, j3, i2, j2)] +
+                                               X[ix3(ne, np, ip, jp, i2, j2, i1, j1, i3, j3)] +
+                                               X[ix3(ne, np, ip, jp, i2, j2, i3, j3, i1, j1)] +
+                                               X[ix3(ne, np, ip, jp, i3, j3, i1, j1, i2, j2)] +
+                                               X[ix3(ne, np, ip, jp, i3, j3, i2, j2, i1, j1)];
+                                    GM3[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)] = x / 6.0;
+                                }
+    free(X);
+}
+
+/* calc_GAM_ee (gam_ee.f:2-168): contracted ERIs, layout
+ * pack_4D(nebf,nebf,nebf, jec2,iec2,jec1,iec1) */
+void orc_gam_ee(i64 nebf, i64 npebf, const i64 *ampeb2c, const double *agebfcc, const double *elcex,
+                const i64 *elcam, const double *elcbfc, double *GEE)
+{
+    basis_t B;
+    memset(&B, 0, sizeof(B));
+    B.npebf = npebf;
+    B.elcex = elcex;
+    B.elcam = elcam;
+    B.elcbfc = elcbfc;
+    i64 ne = nebf;
+    memset(GEE, 0, sizeof(double) * ne * ne * ne * ne);
+    for (i64 ie1 = 0; ie1 < npebf; ++ie1)
+        for (i64 je1 = 0; je1 < npebf; ++je1)
+            for (i64 ie2 = 0; ie2 < npebf; ++ie2)
+                for (i64 je2 = 0; je2 < npebf; ++je2) {
+                    prim_t a1 = eprim(&B, ie1), a2 = eprim(&B, ie2), c1 = eprim(&B, je1), c2 = eprim(&B, je2);
+                    double v = gfvee(&a1, &a2, &c1, &c2);
+                    i64 ia = (((ampeb2c[ie1] - 1) * ne + ampeb2c[je1] - 1) * ne + ampeb2c[ie2] - 1) * ne +
+                             ampeb2c[je2] - 1;
+                    GEE[ia] += v * agebfcc[ie1] * agebfcc[je1] * agebfcc[ie2] * agebfcc[je2];
+                }
+}
+
+/* calc_GAM_4_integral (gam_4_OMP.f:964-1192) */
+static double gam4_int(i64 ne, i64 np, const double *G2s, const double *Gee, i64 ip, i64 jp, i64 ie1, i64 je1, i64 ie2,
+                       i64 je2, i64 ie3, i64 je3, i64 ie4, i64 je4)
+{
+#define EE(a, b, c, d) Gee[(((a) * ne + (b)) * ne + (c)) * ne + (d)]
+    double ee34 = EE(ie3, je3, ie4, je4), ee24 = EE(ie2, je2, ie4, je4), ee23 = EE(ie2, je2, ie3, je3);
+    double ee14 = EE(ie1, je1, ie4, je4), ee13 = EE(ie1, je1, ie3, je3), ee12 = EE(ie1, je1, ie2, je2);
+#undef EE
+    underflow(&ee34); underflow(&ee24); underflow(&ee23);
+    underflow(&ee14); underflow(&ee13); underflow(&ee12);
+    double gs12 = G2s[ix2(ne, np, ip, jp, ie1, je1, ie2, je2)], gs13 = G2s[ix2(ne, np, ip, jp, ie1, je1, ie3, je3)];
+    double gs14 = G2s[ix2(ne, np, ip, jp, ie1, je1, ie4, je4)], gs23 = G2s[ix2(ne, np, ip, jp, ie2, je2, ie3, je3)];
+    double gs24 = G2s[ix2(ne, np, ip, jp, ie2, je2, ie4, je4)], gs34 = G2s[ix2(ne, np, ip, jp, ie3, je3, ie4, je4)];
+    underflow(&gs12); underflow(&gs13); underflow(&gs14);
+    underflow(&gs23); underflow(&gs24); underflow(&gs34);
+    double x1 = 0.5 * ee34 * gs12, x2 = 0.5 * ee24 * gs13, x3 = 0.5 * ee23 * gs14;
+    double x4 = 0.5 * ee14 * gs23, x5 = 0.5 * ee13 * gs24, x6 = 0.5 * ee12 * gs34;
+    return (x1 + x2 + x3 + x4 + x5 + x6) / 6.0;
+}
+
+/* symm_gam4 (gam_4_OMP.f:712-918): 24 ket permutations with fixed weights.
+ * Arguments are (ii,i),(jj,j),(kk,k),(ll,l) = (bra,ket) of electrons 1..4. */
+static double symm_gam4(i64 ne, i64 np, const double *G2s, const double *Gee, i64 ip, i64 jp, i64 ii, i64 i, i64 jj,
+                        i64 j, i64 kk, i64 k, i64 ll, i64 l)
+{
+    /* ket order for x1..x24 and their weights (gam_4_OMP.f:788-856) */
+    static const int perm[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0, 2, 1, 3}, {1, 2, 0, 3}, {2, 1, 0, 3},
+                                    {3, 1, 0, 2}, {1, 3, 0, 2}, {0, 3, 1, 2}, {3, 0, 1, 2}, {1, 0, 3, 2}, {0, 1, 3, 2},
+                                    {0, 2, 3, 1}, {2, 0, 3, 1}, {3, 0, 2, 1}, {0, 3, 2, 1}, {2, 3, 0, 1}, {3, 2, 0, 1},
+                                    {3, 2, 1, 0}, {2, 3, 1, 0}, {1, 3, 2, 0}, {3, 1, 2, 0}, {2, 1, 3, 0}, {1, 2, 3, 0}};
+    static const double wt[24] = {16, -8, 4, -8, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2};
+    i64 kets[4] = {i, j, k, l};
+    double ans = 0.0;
+    for (int p = 0; p < 24; ++p) {
+        double x = gam4_int(ne, np, G2s, Gee, ip, jp, ii, kets[perm[p][0]], jj, kets[perm[p][1]], kk, kets[perm[p][2]],
+                            ll, kets[perm[p][3]]);
+        ans += wt[p] * x;
+    }
+    return ans;
+}
+
+/* GAM4_ICR (gam4_ICR_OMP.f:2-270): in-core 5-particle tensor, int64 index
+ * index_GAM_4PK2 (pack_GAM.f:196-240), value symm_gam4/16. */
+void orc_gam4_icr(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee, double *GM4)
+{
+    i64 ne = nebf, np = npbf;
+#pragma omp parallel for collapse(2)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3)
+                                    for (i64 i4 = 0; i4 < ne; ++i4)
+                                        for (i64 j4 = 0; j4 < ne; ++j4) {
+                                            i64 ia = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3) * ne * ne + i4 * ne + j4;
+                                            GM4[ia] = symm_gam4(ne, np, GAM2s, GAMee, ip, jp, i1, j1, i2, j2, i3, j3,
+                                                                i4, j4) /
+                                                      16.0;
+                                        }
+}
+
+/* EG2IC1 / thread_EG2ICR (gam2_IC1_OMP.f:339-570).  FE, FP accumulate
+ * (add2fock); SE, SP are zeroed then filled; E, S overwritten. */
+void orc_eg2ic1(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM2, const double *GM2s,
+                double *FE, double *SE, double *FP, double *SP, double *E, double *S)
+{
+    i64 ne = nebf, np = npbf;
+    double *xfe = (double *)calloc(ne * ne, sizeof(double)), *xfp = (double *)calloc(np * np, sizeof(double));
+    memset(SE, 0, sizeof(double) * ne * ne);
+    memset(SP, 0, sizeof(double) * np * np);
+    double e = 0.0, s = 0.0;
+    /* Fortran DE(i,j) = DE[j*ne + i] (column-major) */
+#define DEm(i, j) DE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i1, j2, i2, j1);
+                            double v = GM2[a12] - 0.5 * GM2[a21];
+                            double vs = GM2s[a12] - 0.5 * GM2s[a21];
+                            xfe[j1 * ne + i1] += DPm(ip, jp) * DEm(i2, j2) * 2.0 * v;
+                            SE[j1 * ne + i1] += DPm(ip, jp) * DEm(i2, j2) * vs;
+                            xfp[jp * np + ip] += DEm(i1, j1) * DEm(i2, j2) * v;
+                            SP[jp * np + ip] += DEm(i1, j1) * DEm(i2, j2) * vs;
+                            e += DEm(i1, j1) * DEm(i2, j2) * DPm(ip, jp) * v;
+                            s += DEm(i1, j1) * DEm(i2, j2) * DPm(ip, jp) * vs;
+                        }
+    for (i64 i = 0; i < ne * ne; ++i) FE[i] += xfe[i];
+    for (i64 i = 0; i < np * np; ++i) FP[i] += xfp[i];
+    *E = e;
+    *S = s;
+    free(xfe);
+    free(xfp);
+}
+
+/* EG3IC1 / thread_EG3ICR (gam3_IC1_OMP.f:433-725) */
+void orc_eg3ic1(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM3, double *FE, double *FP,
+                double *E)
+{
+    i64 ne = nebf, np = npbf;
+    double *xfe = (double *)calloc(ne * ne, sizeof(double)), *xfp = (double *)calloc(np * np, sizeof(double));
+    double e = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    double o123 = GM3[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)];
+                                    double o213 = GM3[ix3(ne, np, ip, jp, i1, j2, i2, j1, i3, j3)];
+                                    double o312 = GM3[ix3(ne, np, ip, jp, i1, j3, i2, j1, i3, j2)];
+                                    double o132 = GM3[ix3(ne, np, ip, jp, i1, j1, i2, j3, i3, j2)];
+                                    double o231 = GM3[ix3(ne, np, ip, jp, i1, j2, i2, j3, i3, j1)];
+                                    double o321 = GM3[ix3(ne, np, ip, jp, i1, j3, i2, j2, i3, j1)];
+                                    double g = 8.0 * o123 - 4.0 * o213 + 2.0 * o312 - 4.0 * o132 + 2.0 * o231 -
+                                               4.0 * o321;
+                                    g = g / 8.0;
+                                    xfe[j1 * ne + i1] += DPm(ip, jp) * DEm(i2, j2) * DEm(i3, j3) * 3.0 * g;
+                                    xfp[jp * np + ip] += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * g;
+                                    e += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * DPm(ip, jp) * g;
+                                }
+    for (i64 i = 0; i < ne * ne; ++i) FE[i] += xfe[i];
+    for (i64 i = 0; i < np * np; ++i) FP[i] += xfp[i];
+    *E = e;
+    free(xfe);
+    free(xfp);
+}
+
+/* EG4ICR / thread_EG4ICR (gam4_ICR_OMP.f:273-490) */
+void orc_eg4icr(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM4, double *FE, double *FP,
+                double *E)
+{
+    i64 ne = nebf, np = npbf;
+    double *xfe = (double *)calloc(ne * ne, sizeof(double)), *xfp = (double *)calloc(np * np, sizeof(double));
+    double e = 0.0;
+    i64 ia = 0;
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3)
+                                    for (i64 i4 = 0; i4 < ne; ++i4)
+                                        for (i64 j4 = 0; j4 < ne; ++j4, ++ia) {
+                                            double g = GM4[ia];
+                                            xfe[j1 * ne + i1] +=
+                                                DPm(ip, jp) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * 4.0 * g;
+                                            xfp[jp * np + ip] += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * g;
+                                            e += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * DPm(ip, jp) * g;
+                                        }
+    for (i64 i = 0; i < ne * ne; ++i) FE[i] += xfe[i];
+    for (i64 i = 0; i < np * np; ++i) FP[i] += xfp[i];
+    *E = e;
+    free(xfe);
+    free(xfp);
+#undef DEm
+#undef DPm
+}
