@@ -1,0 +1,117 @@
+/*
+ * neorx.h -- C ABI of libneorx_hip.so, the MI355X (gfx950) engine for the
+ * geminal-integral and Fock-build path of NEO-RXCHF.
+ *
+ * Conventions (identical to the reference's Fortran interface, built with
+ * -integer-size 64, source/comp:7):
+ *   - every integer is int64_t, every real is double;
+ *   - matrices are column-major: ELCAM(npebf,3) is elcam[k*npebf+i],
+ *     cat(3,nat) is cat[3*iat+k], DE(nebf,nebf) is de[j*nebf+i];
+ *   - index arrays KPESTR/KPEEND/AMPEB2C hold 1-based primitive/contracted
+ *     indices exactly as the reference builds them (make_KPE,
+ *     source/xcneo_HYBRID.f:1269-1307);
+ *   - contracted tensors use the reference packings
+ *     (source/CS_Integrals/Spot_Check/pack_GAM.f:11-240).
+ * Host-pointer entry points copy inputs to HBM, run, and copy results back;
+ * "_dev" entry points take device pointers and a hipStream_t (as void*), and
+ * return without synchronising.
+ *
+ * Every function returns 0 on success and a negative NEORX_E_* code on
+ * failure; neorx_last_error() returns the message.  Nothing falls back to the
+ * CPU: without a usable gfx950 device every entry point fails.
+ */
+#ifndef NEORX_H
+#define NEORX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEORX_OK 0
+#define NEORX_E_NODEVICE (-1)
+#define NEORX_E_ARG (-2)
+#define NEORX_E_HIP (-3)
+#define NEORX_E_NOMEM (-4)
+#define NEORX_E_UNSUPPORTED (-5)
+
+/* ---- library --------------------------------------------------------- */
+int neorx_init(int64_t device);
+int neorx_finalize(void);
+const char *neorx_last_error(void);
+/* name of the device the library runs on (e.g. "gfx950") */
+const char *neorx_device_arch(void);
+
+/* ---- system handle: basis, nuclei and geminal resident in HBM ---------- */
+/* Arguments are the basis and system arrays that the reference passes to
+ * GAM2_IC1 (source/gam2_IC1_OMP.f:2-37); the handle keeps device copies. */
+int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, int64_t nat, int64_t ngtg, double pmass,
+                        const double *cat, const double *zan, const double *bcoef, const double *gamma,
+                        const int64_t *kpestr, const int64_t *kpeend, const int64_t *ampeb2c, const double *agebfcc,
+                        const double *agnbfcc, const double *elcex, const double *nucex, const int64_t *elcam,
+                        const int64_t *nucam, const double *elcbfc, const double *nucbfc, int64_t *handle);
+int neorx_system_destroy(int64_t handle);
+
+/* ---- primitive tuples -------------------------------------------------- */
+/* One fully geminal-summed primitive tuple value per entry:
+ *   cls 1 (e-p)     : xcalc_GAM1_MD, source/gam_1_OMP.f:398   -> ansE, ansS
+ *   cls 2 (e-e-p)   : xcalc_GAM2_MD, source/gam_2_OMP.f:546   -> ansE, ansS
+ *   cls 3 (e-e-e-p) : xcalc_GAM3_MD, source/gam_3_OMP.f:617   -> ansE
+ * d_idx holds 0-based primitive indices per tuple, 2*(cls+1) int64 each, in
+ * the order (ie1..ie_cls, ip, je1..je_cls, jp). */
+int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
+                        double *d_ansS, void *stream);
+/* Same values for the contiguous slice [start, start+count) of the full
+ * primitive enumeration, ordered C-row-major as (ip,jp,ie1,je1,ie2,je2,...)
+ * over primitives re-ordered by angular-momentum class and exponent; the
+ * re-ordering is returned by neorx_system_order(). */
+int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, int64_t count, double *d_ansE, double *d_ansS,
+                         void *stream);
+/* eorder[npebf], porder[npbf]: position k of the enumeration holds original
+ * primitive eorder[k] (0-based). */
+int neorx_system_order(int64_t handle, int64_t *eorder, int64_t *porder);
+
+/* ---- contracted builders (host arrays, reference semantics) ------------ */
+/* GAM1_OMP_MD, source/gam_1_OMP.f:2: GAM_1, GAM_1s in pack_4D(nebf,nebf,npbf)
+ * order (returned here instead of being written to GAM_1.ufm/GAM_1s.ufm). */
+int neorx_gam1(int64_t handle, double *gam1, double *gam1s);
+/* GAM2_IC1, source/gam2_IC1_OMP.f:2: GM2ICR, GM2sICR (ng2 = nebf^4 npbf^2) */
+int neorx_gam2_ic1(int64_t handle, double *gm2icr, double *gm2sicr);
+/* GAM3_IC1, source/gam3_IC1_OMP.f:2: GM3ICR (ng3 = nebf^6 npbf^2) */
+int neorx_gam3_ic1(int64_t handle, double *gm3icr);
+/* calc_GAM_ee, source/gam_ee.f:2: GAM_ee (ngee = nebf^4), returned instead of
+ * GAM_ee.ufm */
+int neorx_gam_ee(int64_t handle, double *gam_ee);
+/* GAM4_ICR, source/gam4_ICR_OMP.f:2: GM4ICR (ng4 = nebf^8 npbf^2) from
+ * GAM_2s (= GM2sICR) and GAM_ee */
+int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, double *gm4icr);
+
+/* Device-resident builders: outputs are device arrays of the same sizes. */
+int neorx_gam2_ic1_dev(int64_t handle, double *d_gm2icr, double *d_gm2sicr, void *stream);
+int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3icr, void *stream);
+
+/* ---- Fock contractions (reference semantics) ---------------------------- */
+/* EG2IC1, source/gam2_IC1_OMP.f:339: FE, FP accumulate; SE, SP overwritten;
+ * E_gam2, S_gam2 overwritten. */
+int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm2icr,
+                 const double *gm2sicr, double *fe, double *se, double *fp, double *sp, double *e_gam2,
+                 double *s_gam2);
+/* EG3IC1, source/gam3_IC1_OMP.f:433: focke, fockp accumulate; E overwritten */
+int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm3icr, double *focke,
+                 double *fockp, double *e_gam3);
+/* EG4ICR, source/gam4_ICR_OMP.f:273 */
+int neorx_eg4icr(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm4icr, double *focke,
+                 double *fockp, double *e_gam4);
+
+/* Device-resident contractions (all pointers device memory; d_e is one
+ * double, d_se/d_sp/d_s may be NULL when gm2s is NULL). */
+int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm2,
+                     const double *d_gm2s, double *d_fe, double *d_se, double *d_fp, double *d_sp, double *d_e,
+                     double *d_s, void *stream);
+int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm3,
+                     double *d_fe, double *d_fp, double *d_e, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

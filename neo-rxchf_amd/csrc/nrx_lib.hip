@@ -1,0 +1,1076 @@
+// nrx_lib.hip -- libneorx_hip.so: HIP kernels for gfx950 and the C ABI of
+// include/neorx.h.
+//
+// Layout in HBM:
+//   * the basis and nuclei are copied once per system handle, in the
+//     reference's own Fortran layouts (column-major, int64 indices);
+//   * primitive tuples are enumerated over primitives re-ordered by
+//     Cartesian power pattern and exponent, so that neighbouring lanes of a
+//     wavefront share loop trip counts;
+//   * contracted tensors use the reference packings (C row-major
+//     [ip][jp][ie1][je1]...).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/neorx.h"
+#include "nrx_engine.h"
+
+typedef int64_t i64;
+using namespace nrx;
+
+// ------------------------------------------------------------------
+// error handling
+// ------------------------------------------------------------------
+static thread_local std::string g_err;
+static std::mutex g_mu;
+static int g_device = -1;
+static char g_arch[64] = "";
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                                  \
+    do {                                                                                                           \
+        hipError_t _e = (x);                                                                                       \
+        if (_e != hipSuccess) return fail(NEORX_E_HIP, "%s failed: %s", #x, hipGetErrorString(_e));              \
+    } while (0)
+
+static int ensure_init()
+{
+    if (g_device >= 0) return NEORX_OK;
+    return neorx_init(0);
+}
+
+// ------------------------------------------------------------------
+// system handle
+// ------------------------------------------------------------------
+struct DevSys {
+    i64 nebf, npebf, npbf, nat, ngtg;
+    double pmass;
+    const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc, *elcex, *nucex, *elcbfc, *nucbfc;
+    const i64 *kpestr, *kpeend, *ampeb2c, *elcam, *nucam;
+    const i64 *eord, *pord; // enumeration order -> original primitive index
+};
+
+struct System {
+    DevSys d;
+    std::vector<void *> allocs;
+    std::vector<i64> eord, pord;
+    int maxl;
+};
+
+static std::map<i64, System *> g_sys;
+static i64 g_next = 1;
+
+template <class T> static int upload(System *s, const T *h, size_t n, const T **out)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    s->allocs.push_back(p);
+    if (h) HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T *)p;
+    return NEORX_OK;
+}
+
+static System *get_sys(i64 h)
+{
+    auto it = g_sys.find(h);
+    return it == g_sys.end() ? nullptr : it->second;
+}
+
+extern "C" int neorx_init(int64_t device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(NEORX_E_NODEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(NEORX_E_ARG, "device %lld out of range (%d devices)", (long long)device, n);
+    HIPCHK(hipSetDevice((int)device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, (int)device));
+    snprintf(g_arch, sizeof(g_arch), "%s", prop.gcnArchName);
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(NEORX_E_NODEVICE, "device %lld is %s; this library is built for gfx950 only", (long long)device,
+                    prop.gcnArchName);
+    g_device = (int)device;
+    return NEORX_OK;
+}
+
+extern "C" int neorx_finalize(void)
+{
+    for (auto &kv : g_sys) {
+        for (void *p : kv.second->allocs) hipFree(p);
+        delete kv.second;
+    }
+    g_sys.clear();
+    g_device = -1;
+    return NEORX_OK;
+}
+
+extern "C" const char *neorx_last_error(void) { return g_err.c_str(); }
+extern "C" const char *neorx_device_arch(void) { return g_arch; }
+
+extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, int64_t nat, int64_t ngtg, double pmass,
+                                   const double *cat, const double *zan, const double *bcoef, const double *gamma,
+                                   const int64_t *kpestr, const int64_t *kpeend, const int64_t *ampeb2c,
+                                   const double *agebfcc, const double *agnbfcc, const double *elcex,
+                                   const double *nucex, const int64_t *elcam, const int64_t *nucam,
+                                   const double *elcbfc, const double *nucbfc, int64_t *handle)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || npebf <= 0 || npbf <= 0 || nat < 0 || ngtg <= 0 || !handle)
+        return fail(NEORX_E_ARG, "bad system sizes nebf=%lld npebf=%lld npbf=%lld nat=%lld ngtg=%lld",
+                    (long long)nebf, (long long)npebf, (long long)npbf, (long long)nat, (long long)ngtg);
+    int maxl = 0;
+    for (i64 i = 0; i < npebf; ++i) {
+        int l = (int)(elcam[i] + elcam[npebf + i] + elcam[2 * npebf + i]);
+        for (int d = 0; d < 3; ++d)
+            if (elcam[d * npebf + i] < 0) return fail(NEORX_E_ARG, "negative angular momentum");
+        maxl = std::max(maxl, l);
+    }
+    for (i64 i = 0; i < npbf; ++i) maxl = std::max(maxl, (int)(nucam[i] + nucam[npbf + i] + nucam[2 * npbf + i]));
+    if (maxl > 2)
+        return fail(NEORX_E_UNSUPPORTED, "angular momentum l=%d > 2 is not supported by this build", maxl);
+    for (i64 c = 0; c < nebf; ++c)
+        if (kpestr[c] < 1 || kpeend[c] < kpestr[c] || kpeend[c] > npebf)
+            return fail(NEORX_E_ARG, "KPESTR/KPEEND out of range for contracted function %lld", (long long)c + 1);
+    for (i64 i = 0; i < npebf; ++i)
+        if (ampeb2c[i] < 1 || ampeb2c[i] > nebf) return fail(NEORX_E_ARG, "AMPEB2C out of range");
+
+    System *s = new System();
+    s->maxl = maxl;
+    DevSys &d = s->d;
+    d.nebf = nebf;
+    d.npebf = npebf;
+    d.npbf = npbf;
+    d.nat = nat;
+    d.ngtg = ngtg;
+    d.pmass = pmass;
+    // enumeration order: by Cartesian power pattern, then exponent
+    auto order = [](i64 n, const i64 *am, const double *ex, std::vector<i64> &ord) {
+        ord.resize(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](i64 a, i64 b) {
+            i64 la = am[a] + am[n + a] + am[2 * n + a], lb = am[b] + am[n + b] + am[2 * n + b];
+            if (la != lb) return la < lb;
+            for (int k = 0; k < 3; ++k)
+                if (am[k * n + a] != am[k * n + b]) return am[k * n + a] > am[k * n + b];
+            return ex[a] < ex[b];
+        });
+    };
+    order(npebf, elcam, elcex, s->eord);
+    order(npbf, nucam, nucex, s->pord);
+    int r = 0;
+    r |= upload(s, cat, 3 * nat, &d.cat);
+    r |= upload(s, zan, nat, &d.zan);
+    r |= upload(s, bcoef, ngtg, &d.bcoef);
+    r |= upload(s, gamma, ngtg, &d.gamma);
+    r |= upload(s, kpestr, nebf, &d.kpestr);
+    r |= upload(s, kpeend, nebf, &d.kpeend);
+    r |= upload(s, ampeb2c, npebf, &d.ampeb2c);
+    r |= upload(s, agebfcc, npebf, &d.agebfcc);
+    r |= upload(s, agnbfcc, npbf, &d.agnbfcc);
+    r |= upload(s, elcex, npebf, &d.elcex);
+    r |= upload(s, nucex, npbf, &d.nucex);
+    r |= upload(s, elcam, 3 * npebf, &d.elcam);
+    r |= upload(s, nucam, 3 * npbf, &d.nucam);
+    r |= upload(s, elcbfc, 3 * npebf, &d.elcbfc);
+    r |= upload(s, nucbfc, 3 * npbf, &d.nucbfc);
+    r |= upload(s, s->eord.data(), npebf, &d.eord);
+    r |= upload(s, s->pord.data(), npbf, &d.pord);
+    if (r) {
+        for (void *p : s->allocs) hipFree(p);
+        delete s;
+        return NEORX_E_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    i64 h = g_next++;
+    g_sys[h] = s;
+    *handle = h;
+    return NEORX_OK;
+}
+
+extern "C" int neorx_system_destroy(int64_t handle)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sys.find(handle);
+    if (it == g_sys.end()) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    for (void *p : it->second->allocs) hipFree(p);
+    delete it->second;
+    g_sys.erase(it);
+    return NEORX_OK;
+}
+
+extern "C" int neorx_system_order(int64_t handle, int64_t *eorder, int64_t *porder)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (eorder) std::copy(s->eord.begin(), s->eord.end(), eorder);
+    if (porder) std::copy(s->pord.begin(), s->pord.end(), porder);
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------
+__device__ inline Prim d_eprim(const DevSys &S, i64 i)
+{
+    Prim p;
+    p.a = S.elcex[i];
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)S.elcam[d * S.npebf + i];
+        p.A[d] = S.elcbfc[d * S.npebf + i];
+    }
+    return p;
+}
+__device__ inline Prim d_pprim(const DevSys &S, i64 i)
+{
+    Prim p;
+    p.a = S.nucex[i];
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)S.nucam[d * S.npbf + i];
+        p.A[d] = S.nucbfc[d * S.npbf + i];
+    }
+    return p;
+}
+__device__ inline SysParams d_params(const DevSys &S)
+{
+    SysParams P;
+    P.nat = (int)S.nat;
+    P.ngtg = (int)S.ngtg;
+    P.pmass = S.pmass;
+    P.cat = S.cat;
+    P.zan = S.zan;
+    P.bcoef = S.bcoef;
+    P.gamma = S.gamma;
+    return P;
+}
+
+// evaluate one tuple given original primitive indices (bra e..., ip, ket e..., jp)
+template <int CLS> __device__ inline void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
+{
+    Pair q[CLS + 1];
+    for (int i = 0; i < CLS; ++i) make_pair(d_eprim(S, x[i]), d_eprim(S, x[CLS + 1 + i]), q[i]);
+    make_pair(d_pprim(S, x[CLS]), d_pprim(S, x[2 * CLS + 1]), q[CLS]);
+    SysParams P = d_params(S);
+    s = 0.0;
+    if constexpr (CLS == 1)
+        gam1_tuple(P, q, e, s);
+    else if constexpr (CLS == 2)
+        gam2_tuple(P, q, e, s);
+    else
+        gam3_tuple(P, q, e);
+}
+
+template <int CLS>
+__global__ void __launch_bounds__(256) k_prim_list(DevSys S, i64 ntup, const i64 *__restrict__ idx,
+                                                   double *__restrict__ outE, double *__restrict__ outS)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntup) return;
+    i64 x[2 * CLS + 2];
+    for (int k = 0; k < 2 * CLS + 2; ++k) x[k] = idx[t * (2 * CLS + 2) + k];
+    double e, s;
+    eval_tuple<CLS>(S, x, e, s);
+    outE[t] = e;
+    if (outS) outS[t] = s;
+}
+
+// enumeration (ip,jp,ie1,je1,...) in re-ordered primitive space
+template <int CLS>
+__global__ void __launch_bounds__(256) k_prim_range(DevSys S, i64 start, i64 count, double *__restrict__ outE,
+                                                    double *__restrict__ outS)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    i64 r = start + t;
+    i64 ne = S.npebf, np = S.npbf;
+    i64 x[2 * CLS + 2];
+    for (int i = CLS - 1; i >= 0; --i) {
+        x[CLS + 1 + i] = S.eord[r % ne];
+        r /= ne;
+        x[i] = S.eord[r % ne];
+        r /= ne;
+    }
+    x[2 * CLS + 1] = S.pord[r % np];
+    r /= np;
+    x[CLS] = S.pord[r % np];
+    double e, s;
+    eval_tuple<CLS>(S, x, e, s);
+    outE[t] = e;
+    if (outS) outS[t] = s;
+}
+
+static inline unsigned nblocks(i64 n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
+                                   double *d_ansS, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (cls < 1 || cls > 3) return fail(NEORX_E_ARG, "class %lld not in 1..3", (long long)cls);
+    if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
+    if (ntup == 0) return NEORX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int bs = 256;
+    // grid limit: 2^31-1 blocks per launch dimension
+    for (i64 off = 0; off < ntup; off += (i64)bs * 1048576) {
+        i64 n = std::min<i64>(ntup - off, (i64)bs * 1048576);
+        const i64 *ix = d_idx + off * (2 * cls + 2);
+        double *oe = d_ansE + off, *os = d_ansS ? d_ansS + off : nullptr;
+        if (cls == 1) hipLaunchKernelGGL(k_prim_list<1>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
+        if (cls == 2) hipLaunchKernelGGL(k_prim_list<2>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
+        if (cls == 3) hipLaunchKernelGGL(k_prim_list<3>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
+        HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
+extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, int64_t count, double *d_ansE,
+                                    double *d_ansS, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (cls < 1 || cls > 3) return fail(NEORX_E_ARG, "class %lld not in 1..3", (long long)cls);
+    double tot = 1.0;
+    for (int i = 0; i < cls; ++i) tot *= (double)s->d.npebf * (double)s->d.npebf;
+    tot *= (double)s->d.npbf * (double)s->d.npbf;
+    if (start < 0 || count < 0 || (double)start + (double)count > tot)
+        return fail(NEORX_E_ARG, "range [%lld,%lld) outside the %.0f tuples of class %lld", (long long)start,
+                    (long long)(start + count), tot, (long long)cls);
+    if (count == 0) return NEORX_OK;
+    if (!d_ansE) return fail(NEORX_E_ARG, "null output");
+    hipStream_t st = (hipStream_t)stream;
+    const int bs = 256;
+    for (i64 off = 0; off < count; off += (i64)bs * 1048576) {
+        i64 n = std::min<i64>(count - off, (i64)bs * 1048576);
+        double *oe = d_ansE + off, *os = d_ansS ? d_ansS + off : nullptr;
+        if (cls == 1) hipLaunchKernelGGL(k_prim_range<1>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
+        if (cls == 2) hipLaunchKernelGGL(k_prim_range<2>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
+        if (cls == 3) hipLaunchKernelGGL(k_prim_range<3>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
+        HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// contracted builders
+// ------------------------------------------------------------------
+// GAM1_OMP_MD: one lane per contracted (ip,jp,iec,jec), primitive sums in
+// the reference's loop order.
+__global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, double *__restrict__ Gs)
+{
+    i64 ne = S.nebf, np = S.npbf;
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * ne * np * np) return;
+    i64 jc = t % ne, ic = (t / ne) % ne, jp = (t / (ne * ne)) % np, ip = t / (ne * ne * np);
+    double acc = 0.0, accs = 0.0;
+    for (i64 ie = S.kpestr[ic] - 1; ie < S.kpeend[ic]; ++ie)
+        for (i64 je = S.kpestr[jc] - 1; je < S.kpeend[jc]; ++je) {
+            i64 x[4] = {ie, ip, je, jp};
+            double e, s;
+            eval_tuple<1>(S, x, e, s);
+            double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[ie] * S.agebfcc[je];
+            acc += e * cf;
+            accs += s * cf;
+        }
+    G[t] = acc;
+    Gs[t] = accs;
+}
+
+// contract_omega2_conv: lane per contracted tuple (ip,jp,i1,j1,i2,j2)
+__global__ void __launch_bounds__(256) k_gam2_contract(DevSys S, i64 off, i64 n, double *__restrict__ X,
+                                                       double *__restrict__ Xs)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t, ne = S.nebf, np = S.npbf;
+    i64 j2 = r % ne; r /= ne;
+    i64 i2 = r % ne; r /= ne;
+    i64 j1 = r % ne; r /= ne;
+    i64 i1 = r % ne; r /= ne;
+    i64 jp = r % np; r /= np;
+    i64 ip = r;
+    double om = 0.0, oms = 0.0;
+    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
+        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
+            for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
+                for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
+                    i64 x[6] = {a1, a2, ip, c1, c2, jp};
+                    double e, s;
+                    eval_tuple<2>(S, x, e, s);
+                    double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] *
+                                S.agebfcc[c2];
+                    om += e * cf;
+                    oms += s * cf;
+                }
+    X[off + t] = om;
+    Xs[off + t] = oms;
+}
+
+// GM2 = (x12 + x21)/2  (gam2_IC1_OMP.f:141-177)
+__global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 np, const double *__restrict__ X,
+                                                   const double *__restrict__ Xs, double *__restrict__ G,
+                                                   double *__restrict__ Gs)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 n = ne * ne * ne * ne * np * np;
+    if (t >= n) return;
+    i64 ne2 = ne * ne;
+    i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
+    i64 t21 = (b * ne2 + p2) * ne2 + p1;
+    G[t] = (X[t] + X[t21]) / 2.0;
+    Gs[t] = (Xs[t] + Xs[t21]) / 2.0;
+}
+
+// contract_omega3_conv: lane per contracted tuple
+__global__ void __launch_bounds__(256) k_gam3_contract(DevSys S, i64 off, i64 n, double *__restrict__ X)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t, ne = S.nebf, np = S.npbf;
+    i64 j3 = r % ne; r /= ne;
+    i64 i3 = r % ne; r /= ne;
+    i64 j2 = r % ne; r /= ne;
+    i64 i2 = r % ne; r /= ne;
+    i64 j1 = r % ne; r /= ne;
+    i64 i1 = r % ne; r /= ne;
+    i64 jp = r % np; r /= np;
+    i64 ip = r;
+    double om = 0.0;
+    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
+        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
+            for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
+                for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2)
+                    for (i64 a3 = S.kpestr[i3] - 1; a3 < S.kpeend[i3]; ++a3)
+                        for (i64 c3 = S.kpestr[j3] - 1; c3 < S.kpeend[j3]; ++c3) {
+                            i64 x[8] = {a1, a2, a3, ip, c1, c2, c3, jp};
+                            double e, s;
+                            eval_tuple<3>(S, x, e, s);
+                            double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[a1] * S.agebfcc[c1] *
+                                        S.agebfcc[a2] * S.agebfcc[c2] * S.agebfcc[a3] * S.agebfcc[c3];
+                            om += e * cf;
+                        }
+    X[off + t] = om;
+}
+
+// six-permutation average (gam3_IC1_OMP.f:149-263)
+__global__ void __launch_bounds__(256) k_gam3_symm(i64 ne, i64 np, const double *__restrict__ X,
+                                                   double *__restrict__ G)
+{
+    i64 ne2 = ne * ne, blk = ne2 * ne2 * ne2;
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= blk * np * np) return;
+    i64 b = t / blk, r = t % blk;
+    i64 p3 = r % ne2, p2 = (r / ne2) % ne2, p1 = r / (ne2 * ne2);
+    const double *Xb = X + b * blk;
+#define IX(a, c, e) (((a) * ne2 + (c)) * ne2 + (e))
+    double s = Xb[IX(p1, p2, p3)] + Xb[IX(p1, p3, p2)] + Xb[IX(p2, p1, p3)] + Xb[IX(p2, p3, p1)] +
+               Xb[IX(p3, p1, p2)] + Xb[IX(p3, p2, p1)];
+#undef IX
+    G[t] = s / 6.0;
+}
+
+// calc_GAM_ee: lane per contracted quadruple, primitive sums in the
+// reference's loop order (ie1, je1, ie2, je2)
+__global__ void __launch_bounds__(256) k_gam_ee(DevSys S, double *__restrict__ G)
+{
+    i64 ne = S.nebf;
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * ne * ne * ne) return;
+    i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = t / (ne * ne * ne);
+    double acc = 0.0;
+    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
+        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
+            for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
+                for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
+                    Pair q[2];
+                    make_pair(d_eprim(S, a1), d_eprim(S, c1), q[0]);
+                    make_pair(d_eprim(S, a2), d_eprim(S, c2), q[1]);
+                    Sys s = mksys(2, &q[0], &q[1], 0, 0, 0.0, 0, 0);
+                    acc += coul(s, 0, 1) * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
+                }
+    G[t] = acc;
+}
+
+// GAM4_ICR / symm_gam4 / calc_GAM_4_integral (gam_4_OMP.f:712-1192)
+__constant__ int c_perm4[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0, 2, 1, 3}, {1, 2, 0, 3}, {2, 1, 0, 3},
+                                   {3, 1, 0, 2}, {1, 3, 0, 2}, {0, 3, 1, 2}, {3, 0, 1, 2}, {1, 0, 3, 2}, {0, 1, 3, 2},
+                                   {0, 2, 3, 1}, {2, 0, 3, 1}, {3, 0, 2, 1}, {0, 3, 2, 1}, {2, 3, 0, 1}, {3, 2, 0, 1},
+                                   {3, 2, 1, 0}, {2, 3, 1, 0}, {1, 3, 2, 0}, {3, 1, 2, 0}, {2, 1, 3, 0}, {1, 2, 3, 0}};
+__constant__ double c_w4[24] = {16, -8, 4, -8, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2};
+
+__global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, const double *__restrict__ G2s,
+                                              const double *__restrict__ Gee, double *__restrict__ G4)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t;
+    i64 br[4], kt[4];
+    for (int k = 3; k >= 0; --k) {
+        kt[k] = r % ne; r /= ne;
+        br[k] = r % ne; r /= ne;
+    }
+    i64 jp = r % np, ip = r / np;
+    i64 ne2 = ne * ne;
+    i64 gb = (ip * np + jp) * ne2 * ne2;
+    double ans = 0.0;
+    for (int p = 0; p < 24; ++p) {
+        i64 k0 = kt[c_perm4[p][0]], k1 = kt[c_perm4[p][1]], k2 = kt[c_perm4[p][2]], k3 = kt[c_perm4[p][3]];
+        i64 pr[4] = {br[0] * ne + k0, br[1] * ne + k1, br[2] * ne + k2, br[3] * ne + k3};
+        double ee34 = Gee[pr[2] * ne2 + pr[3]], ee24 = Gee[pr[1] * ne2 + pr[3]], ee23 = Gee[pr[1] * ne2 + pr[2]];
+        double ee14 = Gee[pr[0] * ne2 + pr[3]], ee13 = Gee[pr[0] * ne2 + pr[2]], ee12 = Gee[pr[0] * ne2 + pr[1]];
+        double gs12 = G2s[gb + pr[0] * ne2 + pr[1]], gs13 = G2s[gb + pr[0] * ne2 + pr[2]];
+        double gs14 = G2s[gb + pr[0] * ne2 + pr[3]], gs23 = G2s[gb + pr[1] * ne2 + pr[2]];
+        double gs24 = G2s[gb + pr[1] * ne2 + pr[3]], gs34 = G2s[gb + pr[2] * ne2 + pr[3]];
+        underflow(ee34); underflow(ee24); underflow(ee23); underflow(ee14); underflow(ee13); underflow(ee12);
+        underflow(gs12); underflow(gs13); underflow(gs14); underflow(gs23); underflow(gs24); underflow(gs34);
+        double x = (0.5 * ee34 * gs12 + 0.5 * ee24 * gs13 + 0.5 * ee23 * gs14 + 0.5 * ee14 * gs23 +
+                    0.5 * ee13 * gs24 + 0.5 * ee12 * gs34) /
+                   6.0;
+        ans += c_w4[p] * x;
+    }
+    G4[off + t] = ans / 16.0;
+}
+
+// ------------------------------------------------------------------
+// host helpers for builders
+// ------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) hipFree(p);
+    }
+    int alloc(size_t bytes)
+    {
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(NEORX_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        }
+        return NEORX_OK;
+    }
+    double *d() { return (double *)p; }
+};
+
+static const i64 CHUNK = (i64)256 * 65536; // lanes per launch of the contracted builders
+
+extern "C" int neorx_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2s, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, ng2 = ne * ne * ne * ne * np * np;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf X, Xs;
+    int rc = X.alloc(ng2 * 8);
+    if (!rc) rc = Xs.alloc(ng2 * 8);
+    if (rc) return rc;
+    for (i64 off = 0; off < ng2; off += CHUNK) {
+        i64 n = std::min(CHUNK, ng2 - off);
+        hipLaunchKernelGGL(k_gam2_contract, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, off, n, X.d(), Xs.d());
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, np, X.d(), Xs.d(), d_gm2, d_gm2s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st)); // X, Xs are released on return
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf X;
+    int rc = X.alloc(ng3 * 8);
+    if (rc) return rc;
+    for (i64 off = 0; off < ng3; off += CHUNK) {
+        i64 n = std::min(CHUNK, ng3 - off);
+        hipLaunchKernelGGL(k_gam3_contract, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, off, n, X.d());
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, np, X.d(), d_gm3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+static int copy_out(double *h, const double *d, i64 n)
+{
+    HIPCHK(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam1(int64_t handle, double *gam1, double *gam1s)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * np * np;
+    DevBuf G, Gs;
+    int rc = G.alloc(n * 8);
+    if (!rc) rc = Gs.alloc(n * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gam1, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d(), Gs.d());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    rc = copy_out(gam1, G.d(), n);
+    if (!rc) rc = copy_out(gam1s, Gs.d(), n);
+    return rc;
+}
+
+extern "C" int neorx_gam2_ic1(int64_t handle, double *gm2icr, double *gm2sicr)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * ne * ne * np * np;
+    DevBuf G, Gs;
+    int rc = G.alloc(n * 8);
+    if (!rc) rc = Gs.alloc(n * 8);
+    if (!rc) rc = neorx_gam2_ic1_dev(handle, G.d(), Gs.d(), 0);
+    if (!rc) rc = copy_out(gm2icr, G.d(), n);
+    if (!rc) rc = copy_out(gm2sicr, Gs.d(), n);
+    return rc;
+}
+
+extern "C" int neorx_gam3_ic1(int64_t handle, double *gm3icr)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * ne * ne * ne * ne * np * np;
+    DevBuf G;
+    int rc = G.alloc(n * 8);
+    if (!rc) rc = neorx_gam3_ic1_dev(handle, G.d(), 0);
+    if (!rc) rc = copy_out(gm3icr, G.d(), n);
+    return rc;
+}
+
+extern "C" int neorx_gam_ee(int64_t handle, double *gam_ee)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, n = ne * ne * ne * ne;
+    DevBuf G;
+    int rc = G.alloc(n * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gam_ee, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return copy_out(gam_ee, G.d(), n);
+}
+
+extern "C" int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, double *gm4icr)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng2 = ne * ne * ne * ne * np * np, ngee = ne * ne * ne * ne, ng4 = ngee * ngee * np * np;
+    DevBuf A, B, C;
+    rc = A.alloc(ng2 * 8);
+    if (!rc) rc = B.alloc(ngee * 8);
+    if (!rc) rc = C.alloc(ng4 * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(A.p, gam_2s, ng2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B.p, gam_ee, ngee * 8, hipMemcpyHostToDevice));
+    for (i64 off = 0; off < ng4; off += CHUNK) {
+        i64 n = std::min(CHUNK, ng4 - off);
+        hipLaunchKernelGGL(k_gam4, dim3(nblocks(n, 256)), dim3(256), 0, 0, ne, np, off, n, A.d(), B.d(), C.d());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipDeviceSynchronize());
+    return copy_out(gm4icr, C.d(), ng4);
+}
+
+// ------------------------------------------------------------------
+// Fock contractions
+// ------------------------------------------------------------------
+// EG2IC1.  Block (ip,jp,a) reads the ne^3 slab G[ip][jp][a][b][c][d] once and
+// forms dir[b] = sum_cd DE(c,d) G[b][c][d] and ex[d] = sum_bc DE(c,b) G[b][c][d]
+// (the exchange read G(a,j2,i2,j1) of thread_EG2ICR re-indexed onto the same
+// slab), for G and Gs.  Partial vectors go to a scratch array that a second
+// kernel reduces in a fixed order (deterministic).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, i64 np, const double *__restrict__ DE,
+                                                 const double *__restrict__ G, const double *__restrict__ Gs,
+                                                 double *__restrict__ part)
+{
+    extern __shared__ double sm[]; // [4][ne]: dir, ex, dirS, exS
+    i64 blk = blockIdx.x; // (ip*np + jp)*ne + a
+    i64 ne3 = ne * ne * ne;
+    const double *g = G + blk * ne3;
+    const double *gs = Gs ? Gs + blk * ne3 : nullptr;
+    for (int k = threadIdx.x; k < 4 * ne; k += NT) sm[k] = 0.0;
+    __syncthreads();
+    // each lane owns one b (row) at a time and strides over (c,d)
+    for (i64 b = 0; b < ne; ++b) {
+        double dsum = 0.0, dsums = 0.0;
+        const double *row = g + b * ne * ne;
+        const double *rows = gs ? gs + b * ne * ne : nullptr;
+        for (i64 cd = threadIdx.x; cd < ne * ne; cd += NT) {
+            i64 c = cd / ne, d = cd % ne;
+            double gv = row[cd];
+            double de_cd = DE[d * ne + c]; // DE(c,d), column-major
+            double de_cb = DE[b * ne + c]; // DE(c,b)
+            dsum += de_cd * gv;
+            atomicAdd(&sm[ne + d], de_cb * gv);
+            if (rows) {
+                double gsv = rows[cd];
+                dsums += de_cd * gsv;
+                atomicAdd(&sm[3 * ne + d], de_cb * gsv);
+            }
+        }
+        // block reduce dsum -> sm[b]
+        for (int o = 32; o > 0; o >>= 1) {
+            dsum += __shfl_down(dsum, o);
+            dsums += __shfl_down(dsums, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&sm[b], dsum);
+            atomicAdd(&sm[2 * ne + b], dsums);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 4 * ne; k += NT) part[blk * 4 * ne + k] = sm[k];
+}
+
+// reduce partials: FE(a,b) += 2 sum_{ip,jp} DP(ip,jp) [dir_a[b] - ex_a[b]/2] etc.
+__global__ void k_eg2_reduce(i64 ne, i64 np, const double *__restrict__ DE, const double *__restrict__ DP,
+                             const double *__restrict__ part, int withS, double *__restrict__ FE,
+                             double *__restrict__ SE, double *__restrict__ FP, double *__restrict__ SP,
+                             double *__restrict__ E, double *__restrict__ S)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 ne2 = ne * ne, np2 = np * np;
+    if (t < ne2) {
+        // FE(a,b): Fortran element (a,b) at b*ne + a
+        i64 a = t % ne, b = t / ne;
+        double f = 0.0, fs = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) {
+            i64 ip = pp / np, jp = pp % np;
+            double dp = DP[jp * np + ip];
+            const double *pr = part + (pp * ne + a) * 4 * ne;
+            f += dp * 2.0 * (pr[b] - 0.5 * pr[ne + b]);
+            fs += dp * (pr[2 * ne + b] - 0.5 * pr[3 * ne + b]);
+        }
+        FE[b * ne + a] += f;
+        if (withS) SE[b * ne + a] = fs;
+    } else if (t < ne2 + np2) {
+        i64 pp = t - ne2, ip = pp / np, jp = pp % np;
+        double f = 0.0, fs = 0.0;
+        for (i64 a = 0; a < ne; ++a) {
+            const double *pr = part + (pp * ne + a) * 4 * ne;
+            for (i64 b = 0; b < ne; ++b) {
+                double dab = DE[b * ne + a];
+                f += dab * (pr[b] - 0.5 * pr[ne + b]);
+                fs += dab * (pr[2 * ne + b] - 0.5 * pr[3 * ne + b]);
+            }
+        }
+        FP[jp * np + ip] += f;
+        if (withS) SP[jp * np + ip] = fs;
+    }
+}
+
+// y += x
+__global__ void k_axpy(i64 n, const double *__restrict__ x, double *__restrict__ y)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] += x[t];
+}
+
+__global__ void k_energy(i64 np, const double *__restrict__ DP, const double *__restrict__ FPd,
+                         const double *__restrict__ FPsd, double *__restrict__ E, double *__restrict__ S)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double e = 0.0, s = 0.0;
+        for (i64 k = 0; k < np * np; ++k) {
+            e += DP[k] * FPd[k];
+            if (FPsd) s += DP[k] * FPsd[k];
+        }
+        *E = e;
+        if (S) *S = s;
+    }
+}
+
+extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                const double *d_gm2, const double *d_gm2s, double *d_fe, double *d_se, double *d_fp,
+                                double *d_sp, double *d_e, double *d_s, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    hipStream_t st = (hipStream_t)stream;
+    i64 nblk = np * np * ne;
+    DevBuf part, fpd, fpsd;
+    rc = part.alloc(nblk * 4 * ne * 8);
+    if (!rc) rc = fpd.alloc(np * np * 8);
+    if (!rc) rc = fpsd.alloc(np * np * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(fpd.p, 0, np * np * 8, st));
+    HIPCHK(hipMemsetAsync(fpsd.p, 0, np * np * 8, st));
+    hipLaunchKernelGGL(k_eg2_slab<256>, dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, np, d_de, d_gm2, d_gm2s,
+                       part.d());
+    HIPCHK(hipGetLastError());
+    int withS = d_gm2s != nullptr;
+    // FE/SE and the per-(ip,jp) proton terms into fpd/fpsd; then add fpd to FP
+    hipLaunchKernelGGL(k_eg2_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_de, d_dp,
+                       part.d(), withS, d_fe, d_se, fpd.d(), fpsd.d(), d_e, d_s);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), withS ? fpsd.d() : nullptr, d_e,
+                       withS ? d_s : nullptr);
+    HIPCHK(hipGetLastError());
+    // FP accumulates (add2fock), SP is overwritten
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(np * np, 256)), dim3(256), 0, st, np * np, fpd.d(), d_fp);
+    HIPCHK(hipGetLastError());
+    if (withS && d_sp) HIPCHK(hipMemcpyAsync(d_sp, fpsd.p, np * np * 8, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st)); // scratch buffers are released on return
+    return NEORX_OK;
+}
+
+extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm2icr,
+                            const double *gm2sicr, double *fe, double *se, double *fp, double *sp, double *e_gam2,
+                            double *s_gam2)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf, ng2 = ne * ne * ne * ne * np * np;
+    DevBuf dDE, dDP, dG, dGs, dFE, dSE, dFP, dSP, dE;
+    rc = dDE.alloc(ne * ne * 8);
+    if (!rc) rc = dDP.alloc(np * np * 8);
+    if (!rc) rc = dG.alloc(ng2 * 8);
+    if (!rc) rc = dGs.alloc(ng2 * 8);
+    if (!rc) rc = dFE.alloc(ne * ne * 8);
+    if (!rc) rc = dSE.alloc(ne * ne * 8);
+    if (!rc) rc = dFP.alloc(np * np * 8);
+    if (!rc) rc = dSP.alloc(np * np * 8);
+    if (!rc) rc = dE.alloc(2 * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dG.p, gm2icr, ng2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dGs.p, gm2sicr, ng2 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFE.p, fe, ne * ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFP.p, fp, np * np * 8, hipMemcpyHostToDevice));
+    rc = neorx_eg2ic1_dev(ne, np, dDE.d(), dDP.d(), dG.d(), dGs.d(), dFE.d(), dSE.d(), dFP.d(), dSP.d(), dE.d(),
+                          dE.d() + 1, 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    double es[2];
+    HIPCHK(hipMemcpy(fe, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(se, dSE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sp, dSP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e_gam2 = es[0];
+    *s_gam2 = es[1];
+    return NEORX_OK;
+}
+
+// EG3IC1.  Lane per entry of GM3 grouped by slabs (ip,jp,i1): each lane reads
+// one value and scatters its six ket-permuted contributions into the slab's
+// FE row (LDS) and the proton/energy partial.
+__constant__ int c_perm3[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}, {0, 2, 1}, {1, 2, 0}, {2, 1, 0}};
+__constant__ double c_w3[6] = {8.0, -4.0, 2.0, -4.0, 2.0, -4.0};
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_eg3_slab(i64 ne, i64 np, const double *__restrict__ DE,
+                                                 const double *__restrict__ G, double *__restrict__ part)
+{
+    // part[blk][ne + 1]: FE row contributions for i1 = a (without DP), and the
+    // proton scalar sum_{...} DE DE DE g
+    extern __shared__ double sm[];
+    i64 blk = blockIdx.x;
+    i64 a = blk % ne;
+    i64 ne5 = ne * ne * ne * ne * ne;
+    const double *g = G + blk * ne5;
+    for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
+    __syncthreads();
+    double pacc = 0.0;
+    for (i64 r = threadIdx.x; r < ne5; r += NT) {
+        double gv = g[r];
+        // r -> (b, c, d, e, f) = (k1, i2, k2, i3, k3)
+        i64 f = r % ne, e = (r / ne) % ne, d = (r / (ne * ne)) % ne, c = (r / (ne * ne * ne)) % ne,
+            b = r / (ne * ne * ne * ne);
+        i64 kk[3] = {b, d, f};
+        // thread_EG3ICR reads OMG3_sigma = G(i1, j_s1, i2, j_s2, i3, j_s3) for
+        // sigma in {123, 213, 312, 132, 231, 321}; this entry is OMG3_sigma of the
+        // tuple with j_{s_m} = kk[m]
+        for (int p = 0; p < 6; ++p) {
+            i64 j[3];
+            // OMG3 orders: 123:(j1,j2,j3) 213:(j2,j1,j3) 312:(j3,j1,j2) 132:(j1,j3,j2) 231:(j2,j3,j1) 321:(j3,j2,j1)
+            j[c_perm3[p][0]] = kk[0];
+            j[c_perm3[p][1]] = kk[1];
+            j[c_perm3[p][2]] = kk[2];
+            double w = c_w3[p] * gv / 8.0;
+            double d2 = DE[j[1] * ne + c], d3 = DE[j[2] * ne + e];
+            atomicAdd(&sm[j[0]], 3.0 * d2 * d3 * w);
+            pacc += DE[j[0] * ne + a] * d2 * d3 * w;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) pacc += __shfl_down(pacc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&sm[ne], pacc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < ne + 1; k += NT) part[blk * (ne + 1) + k] = sm[k];
+}
+
+__global__ void k_eg3_reduce(i64 ne, i64 np, const double *__restrict__ DP, const double *__restrict__ part,
+                             double *__restrict__ FE, double *__restrict__ FPd)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 ne2 = ne * ne, np2 = np * np;
+    if (t < ne2) {
+        i64 a = t % ne, b = t / ne;
+        double f = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) {
+            i64 ip = pp / np, jp = pp % np;
+            f += DP[jp * np + ip] * part[(pp * ne + a) * (ne + 1) + b];
+        }
+        FE[b * ne + a] += f;
+    } else if (t < ne2 + np2) {
+        i64 pp = t - ne2, ip = pp / np, jp = pp % np;
+        double f = 0.0;
+        for (i64 a = 0; a < ne; ++a) f += part[(pp * ne + a) * (ne + 1) + ne];
+        FPd[jp * np + ip] = f;
+    }
+}
+
+extern "C" int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    hipStream_t st = (hipStream_t)stream;
+    i64 nblk = np * np * ne;
+    DevBuf part, fpd;
+    rc = part.alloc(nblk * (ne + 1) * 8);
+    if (!rc) rc = fpd.alloc(np * np * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eg3_slab<256>, dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, st, ne, np, d_de, d_gm3,
+                       part.d());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(),
+                       d_fe, fpd.d());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), nullptr, d_e, nullptr);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(np * np, 256)), dim3(256), 0, st, np * np, fpd.d(), d_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm3icr,
+                            double *focke, double *fockp, double *e_gam3)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    DevBuf dDE, dDP, dG, dFE, dFP, dE;
+    rc = dDE.alloc(ne * ne * 8);
+    if (!rc) rc = dDP.alloc(np * np * 8);
+    if (!rc) rc = dG.alloc(ng3 * 8);
+    if (!rc) rc = dFE.alloc(ne * ne * 8);
+    if (!rc) rc = dFP.alloc(np * np * 8);
+    if (!rc) rc = dE.alloc(8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dG.p, gm3icr, ng3 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFE.p, focke, ne * ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFP.p, fockp, np * np * 8, hipMemcpyHostToDevice));
+    rc = neorx_eg3ic1_dev(ne, np, dDE.d(), dDP.d(), dG.d(), dFE.d(), dFP.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e_gam3, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+// EG4ICR: lane per entry, direct port of the reduction structure
+// (FE row per slab (ip,jp,i1) in LDS).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *__restrict__ DE,
+                                                 const double *__restrict__ G, double *__restrict__ part)
+{
+    extern __shared__ double sm[];
+    i64 blk = blockIdx.x, a = blk % ne;
+    i64 ne7 = ne * ne * ne * ne * ne * ne * ne;
+    const double *g = G + blk * ne7;
+    for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
+    __syncthreads();
+    double pacc = 0.0;
+    for (i64 r = threadIdx.x; r < ne7; r += NT) {
+        double gv = g[r];
+        i64 j4 = r % ne, i4 = (r / ne) % ne, j3 = (r / (ne * ne)) % ne, i3 = (r / (ne * ne * ne)) % ne;
+        i64 j2 = (r / (ne * ne * ne * ne)) % ne, i2 = (r / (ne * ne * ne * ne * ne)) % ne,
+            j1 = r / (ne * ne * ne * ne * ne * ne);
+        double ddd = DE[j2 * ne + i2] * DE[j3 * ne + i3] * DE[j4 * ne + i4];
+        atomicAdd(&sm[j1], 4.0 * ddd * gv);
+        pacc += DE[j1 * ne + a] * ddd * gv;
+    }
+    for (int o = 32; o > 0; o >>= 1) pacc += __shfl_down(pacc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&sm[ne], pacc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < ne + 1; k += NT) part[blk * (ne + 1) + k] = sm[k];
+}
+
+extern "C" int neorx_eg4icr(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm4icr,
+                            double *focke, double *fockp, double *e_gam4)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf, ng4 = ne * ne * ne * ne * ne * ne * ne * ne * np * np;
+    DevBuf dDE, dDP, dG, dFE, dFP, dE, part, fpd;
+    i64 nblk = np * np * ne;
+    rc = dDE.alloc(ne * ne * 8);
+    if (!rc) rc = dDP.alloc(np * np * 8);
+    if (!rc) rc = dG.alloc(ng4 * 8);
+    if (!rc) rc = dFE.alloc(ne * ne * 8);
+    if (!rc) rc = dFP.alloc(np * np * 8);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc) rc = part.alloc(nblk * (ne + 1) * 8);
+    if (!rc) rc = fpd.alloc(np * np * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dG.p, gm4icr, ng4 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dFE.p, focke, ne * ne * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_eg4_slab<256>, dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, 0, ne, np, dDE.d(), dG.d(),
+                       part.d());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, 0, ne, np, dDP.d(), part.d(),
+                       dFE.d(), fpd.d());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, 0, np, dDP.d(), fpd.d(), nullptr, dE.d(), nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<double> h(np * np);
+    HIPCHK(hipMemcpy(h.data(), fpd.p, np * np * 8, hipMemcpyDeviceToHost));
+    for (i64 k = 0; k < np * np; ++k) fockp[k] += h[k];
+    HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e_gam4, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}

@@ -1,0 +1,197 @@
+"""Host-side binding of libneorx_hip.so (the MI355X geminal-integral engine).
+
+The names mirror the reference's Fortran entry points for this path
+(GAM2_IC1, EG2IC1, ... in source/gam2_IC1_OMP.f, source/gam3_IC1_OMP.f,
+source/gam4_ICR_OMP.f, source/gam_1_OMP.f, source/gam_ee.f) and take the same
+arrays: Fortran-layout numpy arrays on the host, or raw device pointers for
+the ``*_dev`` variants.  There is no CPU path: if the HIP library or a gfx950
+device is missing, every call raises ``NeorxError``.
+"""
+import ctypes as _C
+import os as _os
+
+import numpy as _np
+
+from . import synth  # noqa: F401  (synthetic systems of the benchmark configurations)
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIBDIR = _os.path.join(_os.path.dirname(_HERE), "lib")
+LIBPATH = _os.path.join(LIBDIR, "libneorx_hip.so")
+
+_lib = None
+
+
+class NeorxError(RuntimeError):
+    pass
+
+
+_I64 = _C.c_int64
+_D = _C.c_double
+_P = _C.c_void_p
+
+
+def _sig(name, args):
+    f = getattr(_lib, name)
+    f.restype = _C.c_int
+    f.argtypes = args
+    return f
+
+
+def load():
+    """Load libneorx_hip.so (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _os.path.exists(LIBPATH):
+        raise NeorxError(f"{LIBPATH} not built: run __graft_entry__.build()")
+    _lib = _C.CDLL(LIBPATH)
+    _lib.neorx_last_error.restype = _C.c_char_p
+    _lib.neorx_device_arch.restype = _C.c_char_p
+    _sig("neorx_init", [_I64])
+    _sig("neorx_finalize", [])
+    _sig("neorx_system_create", [_I64, _I64, _I64, _I64, _I64, _D] + [_P] * 15 + [_P])
+    _sig("neorx_system_destroy", [_I64])
+    _sig("neorx_system_order", [_I64, _P, _P])
+    _sig("neorx_prim_list_dev", [_I64, _I64, _I64, _P, _P, _P, _P])
+    _sig("neorx_prim_range_dev", [_I64, _I64, _I64, _I64, _P, _P, _P])
+    _sig("neorx_gam1", [_I64, _P, _P])
+    _sig("neorx_gam2_ic1", [_I64, _P, _P])
+    _sig("neorx_gam3_ic1", [_I64, _P])
+    _sig("neorx_gam_ee", [_I64, _P])
+    _sig("neorx_gam4_icr", [_I64, _I64, _P, _P, _P])
+    _sig("neorx_gam2_ic1_dev", [_I64, _P, _P, _P])
+    _sig("neorx_gam3_ic1_dev", [_I64, _P, _P])
+    _sig("neorx_eg2ic1", [_I64, _I64] + [_P] * 10)
+    _sig("neorx_eg3ic1", [_I64, _I64] + [_P] * 6)
+    _sig("neorx_eg4icr", [_I64, _I64] + [_P] * 6)
+    _sig("neorx_eg2ic1_dev", [_I64, _I64] + [_P] * 10 + [_P])
+    _sig("neorx_eg3ic1_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise NeorxError(f"neorx error {rc}: {_lib.neorx_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def init(device=0):
+    load()
+    _check(_lib.neorx_init(device))
+    return _lib.neorx_device_arch().decode()
+
+
+class System:
+    """Basis, nuclei and geminal resident in HBM (the arguments of GAM2_IC1,
+    source/gam2_IC1_OMP.f:2-37).  ``sysd`` is a dict produced by
+    :mod:`neorx.synth` (Fortran-layout numpy arrays)."""
+
+    def __init__(self, sysd):
+        load()
+        s = sysd
+        self.s = s
+        self._keep = [
+            _np.ascontiguousarray(s[k])
+            for k in ("cat", "zan", "bcoef", "gamma", "kpestr", "kpeend", "ampeb2c", "agebfcc", "agnbfcc",
+                      "elcex", "nucex")
+        ]
+        self._keep += [_np.asfortranarray(s[k]) for k in ("elcam", "nucam", "elcbfc", "nucbfc")]
+        h = _C.c_int64()
+        _check(_lib.neorx_system_create(s["nebf"], s["npebf"], s["npbf"], s["nat"], s["ngtg"], s["pmass"],
+                                        *[_p(a) for a in self._keep], _C.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            _check(_lib.neorx_system_destroy(self.handle))
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def order(self):
+        e = _np.zeros(self.s["npebf"], dtype=_np.int64)
+        p = _np.zeros(self.s["npbf"], dtype=_np.int64)
+        _check(_lib.neorx_system_order(self.handle, _p(e), _p(p)))
+        return e, p
+
+    # --- primitive tuples (device pointers) ---
+    def prim_list_dev(self, cls, ntup, d_idx, d_e, d_s, stream=0):
+        _check(_lib.neorx_prim_list_dev(self.handle, cls, ntup, d_idx, d_e, d_s, stream))
+
+    def prim_range_dev(self, cls, start, count, d_e, d_s, stream=0):
+        _check(_lib.neorx_prim_range_dev(self.handle, cls, start, count, d_e, d_s, stream))
+
+    # --- contracted builders (host arrays) ---
+    def GAM1(self):
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne * ne * npb * npb)
+        gs = _np.zeros_like(g)
+        _check(_lib.neorx_gam1(self.handle, _p(g), _p(gs)))
+        return g, gs
+
+    def GAM2_IC1(self):
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne ** 4 * npb * npb)
+        gs = _np.zeros_like(g)
+        _check(_lib.neorx_gam2_ic1(self.handle, _p(g), _p(gs)))
+        return g, gs
+
+    def GAM3_IC1(self):
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne ** 6 * npb * npb)
+        _check(_lib.neorx_gam3_ic1(self.handle, _p(g)))
+        return g
+
+    def GAM_ee(self):
+        ne = self.s["nebf"]
+        g = _np.zeros(ne ** 4)
+        _check(_lib.neorx_gam_ee(self.handle, _p(g)))
+        return g
+
+
+def GAM4_ICR(nebf, npbf, gam_2s, gam_ee):
+    load()
+    g = _np.zeros(nebf ** 8 * npbf * npbf)
+    _check(_lib.neorx_gam4_icr(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
+                               _p(g)))
+    return g
+
+
+def EG2IC1(nebf, npbf, DE, DP, GM2ICR, GM2SICR, FE, FP):
+    """EG2IC1 (source/gam2_IC1_OMP.f:339): FE, FP accumulate in place; returns
+    (SE, SP, E_gam2, S_gam2)."""
+    load()
+    DE = _np.asfortranarray(DE, dtype=_np.float64)
+    DP = _np.asfortranarray(DP, dtype=_np.float64)
+    SE = _np.zeros((nebf, nebf), order="F")
+    SP = _np.zeros((npbf, npbf), order="F")
+    e = _np.zeros(1)
+    s = _np.zeros(1)
+    _check(_lib.neorx_eg2ic1(nebf, npbf, _p(DE), _p(DP), _p(_np.ascontiguousarray(GM2ICR)),
+                             _p(_np.ascontiguousarray(GM2SICR)), _p(FE), _p(SE), _p(FP), _p(SP), _p(e), _p(s)))
+    return SE, SP, float(e[0]), float(s[0])
+
+
+def EG3IC1(nebf, npbf, DE, DP, GM3ICR, FE, FP):
+    load()
+    DE = _np.asfortranarray(DE, dtype=_np.float64)
+    DP = _np.asfortranarray(DP, dtype=_np.float64)
+    e = _np.zeros(1)
+    _check(_lib.neorx_eg3ic1(nebf, npbf, _p(DE), _p(DP), _p(_np.ascontiguousarray(GM3ICR)), _p(FE), _p(FP), _p(e)))
+    return float(e[0])
+
+
+def EG4ICR(nebf, npbf, DE, DP, GM4ICR, FE, FP):
+    load()
+    DE = _np.asfortranarray(DE, dtype=_np.float64)
+    DP = _np.asfortranarray(DP, dtype=_np.float64)
+    e = _np.zeros(1)
+    _check(_lib.neorx_eg4icr(nebf, npbf, _p(DE), _p(DP), _p(_np.ascontiguousarray(GM4ICR)), _p(FE), _p(FP), _p(e)))
+    return float(e[0])

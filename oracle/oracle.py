@@ -1,0 +1,261 @@
+"""ctypes bindings of the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+* ``liboracle.so``  -- this repository's C restatement (neorx_oracle.c);
+* ``_ref/libref.so`` -- the reference's own Fortran hot-path sources compiled
+  by ``make -C oracle ref`` (present only where /root/reference was mounted at
+  build time).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product library never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_P = C.c_void_p
+_o = None
+_r = None
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def lib():
+    global _o
+    if _o is None:
+        _o = C.CDLL(ORACLE_SO)
+        _o.orc_boys.restype = C.c_double
+        _o.orc_gint.restype = C.c_double
+    return _o
+
+
+def _bas(s):
+    keep = [np.ascontiguousarray(s[k]) for k in ("kpestr", "kpeend", "ampeb2c", "agebfcc", "agnbfcc", "elcex",
+                                                 "nucex")]
+    keep += [np.asfortranarray(s[k]) for k in ("elcam", "nucam", "elcbfc", "nucbfc")]
+    args = [C.c_int64(s["nebf"]), C.c_int64(s["npebf"]), C.c_int64(s["npbf"])] + [_p(a) for a in keep]
+    return args, keep
+
+
+def _sysargs(s):
+    keep = [np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma")]
+    args = [C.c_int64(s["nat"]), C.c_int64(s["ngtg"]), C.c_double(s["pmass"])] + [_p(a) for a in keep]
+    return args, keep
+
+
+def boys(m, T):
+    return lib().orc_boys(C.c_int64(m), C.c_double(T))
+
+
+def prim_list(s, cls, idx):
+    """Primitive tuple values (xcalc_GAM{1,2,3}_MD semantics) for an
+    (ntup, 2*(cls+1)) int64 array of 0-based primitive indices."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    n = idx.shape[0]
+    e = np.zeros(n)
+    sv = np.zeros(n)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_prim_list(*ba, *sa, C.c_int64(cls), C.c_int64(n), _p(idx), _p(e), _p(sv))
+    return e, sv
+
+
+def gam1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne * ne * npb * npb)
+    gs = np.zeros_like(g)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_gam1(*ba, *sa, _p(g), _p(gs))
+    return g, gs
+
+
+def gam2_ic1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne ** 4 * npb * npb)
+    gs = np.zeros_like(g)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_gam2_ic1(*ba, *sa, _p(g), _p(gs))
+    return g, gs
+
+
+def gam3_ic1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne ** 6 * npb * npb)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_gam3_ic1(*ba, *sa, _p(g))
+    return g
+
+
+def gam_ee(s):
+    ne = s["nebf"]
+    g = np.zeros(ne ** 4)
+    keep = [np.ascontiguousarray(s["ampeb2c"]), np.ascontiguousarray(s["agebfcc"]), np.ascontiguousarray(s["elcex"]),
+            np.asfortranarray(s["elcam"]), np.asfortranarray(s["elcbfc"])]
+    lib().orc_gam_ee(C.c_int64(ne), C.c_int64(s["npebf"]), *[_p(a) for a in keep], _p(g))
+    return g
+
+
+def gam4_icr(nebf, npbf, g2s, gee):
+    g = np.zeros(nebf ** 8 * npbf * npbf)
+    lib().orc_gam4_icr(C.c_int64(nebf), C.c_int64(npbf), _p(np.ascontiguousarray(g2s)),
+                       _p(np.ascontiguousarray(gee)), _p(g))
+    return g
+
+
+def eg2ic1(nebf, npbf, DE, DP, G, Gs, FE, FP):
+    DE = np.asfortranarray(DE)
+    DP = np.asfortranarray(DP)
+    SE = np.zeros((nebf, nebf), order="F")
+    SP = np.zeros((npbf, npbf), order="F")
+    e = np.zeros(1)
+    s = np.zeros(1)
+    lib().orc_eg2ic1(C.c_int64(nebf), C.c_int64(npbf), _p(DE), _p(DP), _p(np.ascontiguousarray(G)),
+                     _p(np.ascontiguousarray(Gs)), _p(FE), _p(SE), _p(FP), _p(SP), _p(e), _p(s))
+    return SE, SP, float(e[0]), float(s[0])
+
+
+def eg3ic1(nebf, npbf, DE, DP, G, FE, FP):
+    e = np.zeros(1)
+    lib().orc_eg3ic1(C.c_int64(nebf), C.c_int64(npbf), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)),
+                     _p(np.ascontiguousarray(G)), _p(FE), _p(FP), _p(e))
+    return float(e[0])
+
+
+def eg4icr(nebf, npbf, DE, DP, G, FE, FP):
+    e = np.zeros(1)
+    lib().orc_eg4icr(C.c_int64(nebf), C.c_int64(npbf), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)),
+                     _p(np.ascontiguousarray(G)), _p(FE), _p(FP), _p(e))
+    return float(e[0])
+
+
+# ---------------------------------------------------------------------------
+# the reference itself (Fortran ABI: lower case + '_', every argument by
+# reference, INTEGER*8, column-major arrays)
+# ---------------------------------------------------------------------------
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _r
+    if _r is None:
+        _r = C.CDLL(REF_SO)
+    return _r
+
+
+def _i8(v):
+    return C.byref(C.c_int64(int(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def ref_prim_list(s, cls, idx):
+    """xcalc_GAM{1,2,3}_MD of the reference for each tuple."""
+    R = ref()
+    fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_}[cls]
+    am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
+    c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
+    cat = np.ascontiguousarray(s["cat"])
+    zan, b, g = (np.ascontiguousarray(s[k]) for k in ("zan", "bcoef", "gamma"))
+    out_e = np.zeros(len(idx))
+    out_s = np.zeros(len(idx))
+    for t, x in enumerate(np.asarray(idx)):
+        keep, args = [], []
+
+        def prim(am, ex, cen, i):
+            c = np.ascontiguousarray(cen[i], dtype=np.float64)
+            keep.append(c)
+            return [_i8(am[i, 0]), _i8(am[i, 1]), _i8(am[i, 2]), _d(ex[i]), _p(c)]
+
+        for i in range(cls):
+            args += prim(am_e, s["elcex"], c_e, x[i])
+        args += prim(am_p, s["nucex"], c_p, x[cls])
+        for i in range(cls):
+            args += prim(am_e, s["elcex"], c_e, x[cls + 1 + i])
+        args += prim(am_p, s["nucex"], c_p, x[2 * cls + 1])
+        e = C.c_double()
+        sv = C.c_double()
+        tail = [_i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g), C.byref(e)]
+        if cls < 3:
+            tail.append(C.byref(sv))
+        fn(*args, *tail)
+        out_e[t] = e.value
+        out_s[t] = sv.value
+    return out_e, out_s
+
+
+def _ref_basis_args(s):
+    keep = [np.ascontiguousarray(s["kpestr"]), np.ascontiguousarray(s["kpeend"]), np.ascontiguousarray(s["ampeb2c"]),
+            np.ascontiguousarray(s["agebfcc"]), np.ascontiguousarray(s["agnbfcc"]), np.ascontiguousarray(s["elcex"]),
+            np.ascontiguousarray(s["nucex"]), np.asfortranarray(s["elcam"]), np.asfortranarray(s["nucam"]),
+            np.asfortranarray(s["elcbfc"]), np.asfortranarray(s["nucbfc"])]
+    return keep
+
+
+def ref_gam2_ic1(s):
+    """GAM2_IC1 (source/gam2_IC1_OMP.f:2) of the reference."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng2 = ne ** 4 * npb * npb
+    g = np.zeros(ng2)
+    gs = np.zeros(ng2)
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().gam2_ic1_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng2), _i8(0), _i8(s["nat"]), _i8(s["ngtg"]),
+                    _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep], _p(g), _p(gs))
+    return g, gs
+
+
+def ref_gam3_ic1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    ng3 = ne ** 6 * npb * npb
+    g = np.zeros(ng3)
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().gam3_ic1_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng3), _i8(0), _i8(s["nat"]), _i8(s["ngtg"]),
+                    _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep], _p(g))
+    return g
+
+
+def ref_eg2ic1(nebf, npbf, DE, DP, G, Gs, FE, FP):
+    SE = np.zeros((nebf, nebf), order="F")
+    SP = np.zeros((npbf, npbf), order="F")
+    e = C.c_double()
+    sv = C.c_double()
+    ng2 = nebf ** 4 * npbf * npbf
+    ref().eg2ic1_(_i8(1), _i8(nebf), _i8(npbf), _i8(ng2), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)),
+                  _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gs)), _p(FE), _p(SE), _p(FP), _p(SP),
+                  C.byref(e), C.byref(sv))
+    return SE, SP, e.value, sv.value
+
+
+def ref_eg3ic1(nebf, npbf, DE, DP, G, FE, FP):
+    e = C.c_double()
+    ng3 = nebf ** 6 * npbf * npbf
+    ref().eg3ic1_(_i8(1), _i8(nebf), _i8(npbf), _i8(ng3), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)),
+                  _p(np.ascontiguousarray(G)), _p(FE), _p(FP), C.byref(e))
+    return e.value
+
+
+def ref_eg4icr(nebf, npbf, DE, DP, G, FE, FP):
+    e = C.c_double()
+    ng4 = nebf ** 8 * npbf * npbf
+    ref().eg4icr_(_i8(1), _i8(nebf), _i8(npbf), _i8(ng4), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)),
+                  _p(np.ascontiguousarray(G)), _p(FE), _p(FP), C.byref(e))
+    return e.value
+
+
+def ref_boys(m, T):
+    F = np.zeros(m + 1)
+    ref().xgammaf_(_p(F), _d(T), _i8(m))
+    return F[m]

@@ -1,0 +1,68 @@
+"""Shared helpers of the test suite."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# Parity bar for FP64 integrals: |x - ref| <= RTOL |ref| + ATOL_REL * max|ref|.
+# RTOL is the per-value relative bar; ATOL_REL bounds values that are
+# themselves the residue of cancellation (below 1e-12 of the batch scale both
+# implementations carry only rounding noise there).
+RTOL = 1e-11
+ATOL_REL = 1e-12
+
+
+def check_close(got, ref, rtol=RTOL, atol_rel=ATOL_REL, what=""):
+    got = np.asarray(got, dtype=np.float64).ravel()
+    ref = np.asarray(ref, dtype=np.float64).ravel()
+    assert got.shape == ref.shape, (got.shape, ref.shape)
+    scale = np.max(np.abs(ref)) if ref.size else 0.0
+    err = np.abs(got - ref)
+    bad = err > rtol * np.abs(ref) + atol_rel * scale
+    if np.any(bad):
+        i = int(np.argmax(err - rtol * np.abs(ref)))
+        raise AssertionError(f"{what}: {int(bad.sum())}/{ref.size} values outside bar; worst idx {i}: "
+                             f"got {got[i]!r} ref {ref[i]!r} scale {scale:.3e}")
+    rel = err / np.maximum(np.abs(ref), 1e-300)
+    return dict(max_abs_over_scale=float(err.max() / scale) if scale else 0.0,
+                frac_rel_1e12=float(np.mean(rel <= 1e-12)))
+
+
+def random_tuples(sysd, cls, n, seed):
+    rng = np.random.default_rng(seed)
+    w = 2 * (cls + 1)
+    idx = np.zeros((n, w), dtype=np.int64)
+    for i in list(range(cls)) + list(range(cls + 1, 2 * cls + 1)):
+        idx[:, i] = rng.integers(0, sysd["npebf"], n)
+    idx[:, cls] = rng.integers(0, sysd["npbf"], n)
+    idx[:, 2 * cls + 1] = rng.integers(0, sysd["npbf"], n)
+    return idx
+
+
+_eng = None
+
+
+def engine_host():
+    """Host build of the device integral engine (tests/_build)."""
+    global _eng
+    if _eng is None:
+        _eng = C.CDLL(os.path.join(ROOT, "tests", "_build", "libengine_host.so"))
+        _eng.eng_gint.restype = C.c_double
+    return _eng
+
+
+def engine_prim_list(s, cls, idx):
+    E = engine_host()
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    n = len(idx)
+    e, sv = np.zeros(n), np.zeros(n)
+    keep = [np.ascontiguousarray(s["elcex"]), np.ascontiguousarray(s["nucex"]), np.asfortranarray(s["elcam"]),
+            np.asfortranarray(s["nucam"]), np.asfortranarray(s["elcbfc"]), np.asfortranarray(s["nucbfc"])]
+    sk = [np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma")]
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    E.eng_prim_list(C.c_int64(s["npebf"]), C.c_int64(s["npbf"]), *[P(a) for a in keep], C.c_int64(s["nat"]),
+                    C.c_int64(s["ngtg"]), C.c_double(s["pmass"]), *[P(a) for a in sk], C.c_int64(cls), C.c_int64(n),
+                    P(idx), P(e), P(sv))
+    return e, sv

@@ -1,0 +1,68 @@
+// Host build of the device integral engine (nrx_engine.h), used only by the
+// CPU unit tests to check the engine's algebra against the oracle before any
+// GPU run.  Not part of the product library.
+#include <cstdint>
+#include "../neo-rxchf_amd/csrc/nrx_engine.h"
+
+using namespace nrx;
+typedef int64_t i64;
+
+static Prim eprim(i64 npebf, const double *ex, const i64 *am, const double *cen, i64 i)
+{
+    Prim p;
+    p.a = ex[i];
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)am[d * npebf + i];
+        p.A[d] = cen[d * npebf + i];
+    }
+    return p;
+}
+
+extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam,
+                              const i64 *nucam, const double *elcbfc, const double *nucbfc, i64 nat, i64 ngtg,
+                              double pmass, const double *cat, const double *zan, const double *bcoef,
+                              const double *gamma, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
+{
+    SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma};
+    int ne = (int)cls, w = 2 * (ne + 1);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (i64 t = 0; t < ntup; ++t) {
+        const i64 *x = idx + t * w;
+        Pair q[4];
+        for (int i = 0; i < ne; ++i)
+            make_pair(eprim(npebf, elcex, elcam, elcbfc, x[i]), eprim(npebf, elcex, elcam, elcbfc, x[ne + 1 + i]), q[i]);
+        make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
+        double e = 0, s = 0;
+        if (cls == 1) gam1_tuple(S, q, e, s);
+        else if (cls == 2) gam2_tuple(S, q, e, s);
+        else gam3_tuple(S, q, e);
+        outE[t] = e;
+        if (outS) outS[t] = s;
+    }
+}
+
+// Elementary integral in star topology (hub = last particle); op: 0 ovl,
+// 1 nuc, 2 coul, 3 kin; gam[i]: geminal electron i - hub (bra+ket); for kin,
+// gk is the ket geminal between a and partner c (c < 0: none).
+extern "C" double eng_gint(i64 op, i64 n, const i64 *pw, const double *ex, const double *cen, const double *gam,
+                           i64 a, i64 b, const double *C, double mass, i64 c, double gk, double pic)
+{
+    Pair q[4];
+    for (int i = 0; i < n; ++i) {
+        Prim pb, pk;
+        pb.a = ex[i];
+        pk.a = ex[n + i];
+        for (int d = 0; d < 3; ++d) {
+            pb.l[d] = (int)pw[3 * i + d];
+            pk.l[d] = (int)pw[3 * (n + i) + d];
+            pb.A[d] = cen[3 * i + d];
+            pk.A[d] = cen[3 * (n + i) + d];
+        }
+        make_pair(pb, pk, q[i]);
+    }
+    Sys s = mksys((int)n, &q[0], &q[1], &q[2], &q[3], gam[0], gam[1], gam[2]);
+    if (op == 0) return ovl(s, pic);
+    if (op == 1) return nuc(s, (int)a, C, pic);
+    if (op == 2) return coul(s, (int)a, (int)b, pic);
+    return kin(s, (int)a, (int)c, gk, mass, pic);
+}

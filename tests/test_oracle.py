@@ -1,0 +1,184 @@
+"""Pins of the oracle (CPU): closed forms, the reference's recorded spot
+checks, and -- where oracle/_ref was built -- the reference itself."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from neorx import synth
+from _util import check_close, random_tuples
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference checker oracle/_ref not built")
+
+
+def test_boys_closed_forms():
+    # F_0(T) = sqrt(pi/T) erf(sqrt T)/2 (series branch accurate to ~1e-13)
+    for T in [1e-6, 0.01, 0.5, 1.0, 3.7, 9.9, 15.0, 19.99]:
+        exact = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        assert abs(oracle.boys(0, T) - exact) <= 2e-13 * exact
+    for m in range(12):
+        assert abs(oracle.boys(m, 0.0) - 1.0 / (2 * m + 1)) < 1e-15
+    # T > 20: the reference's asymptotic branch, F_m = (2m-1)!!/(2T)^m sqrt(pi/T)/2 exactly
+    for T in [20.0001, 25.0, 60.0]:
+        for m in range(8):
+            asym = 0.88622692545275801365 / math.sqrt(T)
+            for l in range(1, m + 1):
+                asym *= (l - 0.5) / T
+            assert oracle.boys(m, T) == pytest.approx(asym, rel=1e-15)
+    # downward identity F_m = (2T F_{m+1} + e^-T)/(2m+1) on the series branch
+    for T in [0.3, 4.0, 12.0]:
+        for m in range(6):
+            lhs = oracle.boys(m, T)
+            rhs = (2 * T * oracle.boys(m + 1, T) + math.exp(-T)) / (2 * m + 1)
+            assert abs(lhs - rhs) <= 1e-12 * lhs
+
+
+def _gint(op, blk, GA, GB, a=0, b=1, mass=1.0, order=None, pi=0.0):
+    import ctypes as C
+    n = len(blk["bra"])
+    order = order or list(range(n))
+    br = [blk["bra"][i] for i in order]
+    kt = [blk["ket"][i] for i in order]
+    pw = np.array([p[:3] for p in br + kt], dtype=np.int64)
+    ex = np.array([p[3] for p in br + kt])
+    cen = np.array([p[4:] for p in br + kt])
+    return oracle.lib().orc_gint(C.c_int64(op), C.c_int64(n), oracle._p(pw), oracle._p(ex), oracle._p(cen),
+                                 oracle._p(np.ascontiguousarray(GA, dtype=float)),
+                                 oracle._p(np.ascontiguousarray(GB, dtype=float)), C.c_int64(a), C.c_int64(b),
+                                 oracle._p(np.array(blk["C"], float)), C.c_double(mass), C.c_double(pi))
+
+
+def test_spot_check_golden_values():
+    G = json.load(open(os.path.join(HERE, "golden", "spot_check.json")))
+    b = G["G2"]
+    g1, g2 = b["gamma1"], b["gamma2"]
+    GA, GB = np.array([[0, g1], [0, 0]]), np.array([[0, g2], [0, 0]])
+    assert _gint(0, b, GA, GB) == pytest.approx(b["MD"]["XGGS"], rel=1e-12)
+    assert -b["Z"] * _gint(1, b, GA, GB) == pytest.approx(b["MD"]["XGGVEC"], rel=1e-12)
+    assert -_gint(2, b, GA, GB) == pytest.approx(b["MD"]["XGGVEP"], rel=1e-12)
+    b = G["G3"]
+    g1, g2 = b["gamma1"], b["gamma2"]
+
+    def m3(A12=0, A13=0, A23=0):
+        M = np.zeros((3, 3))
+        M[0, 1], M[0, 2], M[1, 2] = A12, A13, A23
+        return M
+
+    md = b["MD"]
+    assert _gint(0, b, m3(A13=g1), m3(A23=g2)) == pytest.approx(md["xgSg"], rel=1e-12)
+    assert _gint(3, b, m3(A23=g1), m3()) == pytest.approx(md["xgTE"], rel=1e-12)
+    assert _gint(3, b, m3(A13=g1), m3(A23=g2)) == pytest.approx(md["xgTEg1"], rel=1e-12)
+    assert _gint(3, b, m3(A23=g1), m3(A23=g2)) == pytest.approx(md["xgTEg3"], rel=1e-12)
+    assert _gint(3, b, m3(A23=g1), m3(A13=g2)) == pytest.approx(md["xgTEg2"], rel=1e-12)
+    assert _gint(3, b, m3(A12=g1), m3(A13=g2), mass=1836.0, order=[2, 0, 1]) == pytest.approx(md["xgTPg"], rel=1e-12)
+    assert -b["Z"] * _gint(1, b, m3(A23=g1), m3()) == pytest.approx(md["xgVEC"], rel=1e-12)
+    b = G["G4"]
+    g1, g2 = b["gamma1"], b["gamma2"]
+    M = np.zeros((4, 4))
+    M[0, 3], M[1, 3] = g1, g2
+    assert -_gint(2, b, M, np.zeros((4, 4)), a=2, b=3) == pytest.approx(b["MD"]["xgVEPg"], rel=1e-12)
+    M = np.zeros((4, 4))
+    M[0, 3], M[2, 3] = g1, g2
+    assert _gint(2, b, M, np.zeros((4, 4)), a=0, b=1) == pytest.approx(b["MD"]["xgVEEg2"], rel=1e-12)
+
+
+def test_s_type_closed_forms():
+    """Geminal-coupled s-type overlap = pi^{3n/2} det(M)^{-3/2} exp(-P'QP);
+    the gamma -> 0 limit of the 3-particle Coulomb is the product of an ERI
+    and an overlap."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        ex = rng.uniform(0.3, 3.0, 6)
+        cen = rng.uniform(-1, 1, (6, 3))
+        g13, g23 = rng.uniform(0.5, 4.0, 2)
+        blk = {"bra": [[0, 0, 0, ex[i], *cen[i]] for i in range(3)],
+               "ket": [[0, 0, 0, ex[3 + i], *cen[3 + i]] for i in range(3)], "C": [0, 0, 0]}
+        GA = np.zeros((3, 3))
+        GA[0, 2], GA[1, 2] = g13, g23
+        val = _gint(0, blk, GA, np.zeros((3, 3)))
+        p = ex[:3] + ex[3:]
+        P = (ex[:3, None] * cen[:3] + ex[3:, None] * cen[3:]) / p[:, None]
+        K = np.exp(-(ex[:3] * ex[3:] / p) * np.sum((cen[:3] - cen[3:]) ** 2, axis=1)).prod()
+        M = np.diag(p) + np.array([[g13, 0, -g13], [0, g23, -g23], [-g13, -g23, g13 + g23]])
+        Q = np.diag(p) - np.diag(p) @ np.linalg.inv(M) @ np.diag(p)
+        expo = sum(P[:, d] @ Q @ P[:, d] for d in range(3))
+        closed = math.pi ** 4.5 / np.linalg.det(M) ** 1.5 * math.exp(-expo) * K
+        assert val == pytest.approx(closed, rel=1e-12)
+        # gamma -> 0: <1/r12> factorises
+        z = np.zeros((3, 3))
+        v3 = _gint(2, blk, z, z, a=0, b=1)
+        b12 = {"bra": blk["bra"][:2], "ket": blk["ket"][:2], "C": [0, 0, 0]}
+        b3 = {"bra": blk["bra"][2:], "ket": blk["ket"][2:], "C": [0, 0, 0]}
+        eri = _gint(2, b12, np.zeros((2, 2)), np.zeros((2, 2)), a=0, b=1)
+        s3 = _gint(0, b3, np.zeros((1, 1)), np.zeros((1, 1)))
+        assert v3 == pytest.approx(eri * s3, rel=1e-12)
+
+
+@needs_ref
+def test_boys_bitwise_vs_reference():
+    for T in [0.0, 1e-3, 0.7, 5.5, 19.5, 20.0, 20.5, 44.0]:
+        for m in range(0, 17, 3):
+            assert oracle.boys(m, T) == oracle.ref_boys(m, T)
+
+
+@needs_ref
+@pytest.mark.parametrize("cls,n", [(1, 120), (2, 12), (3, 2)])
+def test_oracle_tuples_vs_reference(cls, n):
+    s = synth.small(seed=21 + cls, e_shells=(0, 1, 2) if cls < 3 else (0, 1), p_shells=(0, 1, 2) if cls < 3 else (0, 1))
+    idx = random_tuples(s, cls, n, seed=cls)
+    re, rs = oracle.ref_prim_list(s, cls, idx)
+    oe, os_ = oracle.prim_list(s, cls, idx)
+    check_close(oe, re, what=f"oracle vs reference cls{cls} E")
+    if cls < 3:
+        check_close(os_, rs, what=f"oracle vs reference cls{cls} S")
+
+
+@needs_ref
+def test_oracle_gam2_ic1_and_eg2_vs_reference():
+    s = synth.small(seed=5, n_e_centres=1, e_shells=(0,), contract=(2,), p_shells=(0,), ngtg=2)
+    g, gs = oracle.gam2_ic1(s)
+    rg, rgs = oracle.ref_gam2_ic1(s)
+    check_close(g, rg, what="GAM2_IC1 GM2ICR")
+    check_close(gs, rgs, what="GAM2_IC1 GM2sICR")
+    ne, npb = s["nebf"], s["npbf"]
+    rng = np.random.default_rng(1)
+    DE, DP = synth.density(ne, 2.0, 1), synth.density(npb, 1.0, 2)
+    G = rng.standard_normal(ne ** 4 * npb * npb)
+    Gs = rng.standard_normal(ne ** 4 * npb * npb)
+    FE0, FP0 = np.asfortranarray(rng.standard_normal((ne, ne))), np.asfortranarray(rng.standard_normal((npb, npb)))
+    FE1, FP1, FE2, FP2 = FE0.copy(order="F"), FP0.copy(order="F"), FE0.copy(order="F"), FP0.copy(order="F")
+    a = oracle.eg2ic1(ne, npb, DE, DP, G, Gs, FE1, FP1)
+    b = oracle.ref_eg2ic1(ne, npb, DE, DP, G, Gs, FE2, FP2)
+    check_close(FE1, FE2, what="EG2 FE")
+    check_close(FP1, FP2, what="EG2 FP")
+    check_close(a[0], b[0], what="EG2 SE")
+    check_close(a[1], b[1], what="EG2 SP")
+    assert a[2] == pytest.approx(b[2], rel=1e-12) and a[3] == pytest.approx(b[3], rel=1e-12)
+
+
+@needs_ref
+def test_oracle_eg3_eg4_vs_reference():
+    rng = np.random.default_rng(4)
+    ne, npb = 3, 2
+    DE, DP = synth.density(ne, 2.0, 3), synth.density(npb, 1.0, 4)
+    G3 = rng.standard_normal(ne ** 6 * npb * npb)
+    FE0, FP0 = np.asfortranarray(rng.standard_normal((ne, ne))), np.asfortranarray(rng.standard_normal((npb, npb)))
+    F = [FE0.copy(order="F"), FP0.copy(order="F"), FE0.copy(order="F"), FP0.copy(order="F")]
+    e1 = oracle.eg3ic1(ne, npb, DE, DP, G3, F[0], F[1])
+    e2 = oracle.ref_eg3ic1(ne, npb, DE, DP, G3, F[2], F[3])
+    check_close(F[0], F[2], what="EG3 FE")
+    check_close(F[1], F[3], what="EG3 FP")
+    assert e1 == pytest.approx(e2, rel=1e-12)
+    ne = 2
+    DE = synth.density(ne, 2.0, 5)
+    G4 = rng.standard_normal(ne ** 8 * npb * npb)
+    F = [FE0[:ne, :ne].copy(order="F"), FP0.copy(order="F"), FE0[:ne, :ne].copy(order="F"), FP0.copy(order="F")]
+    e1 = oracle.eg4icr(ne, npb, DE, DP, G4, F[0], F[1])
+    e2 = oracle.ref_eg4icr(ne, npb, DE, DP, G4, F[2], F[3])
+    check_close(F[0], F[2], what="EG4 FE")
+    check_close(F[1], F[3], what="EG4 FP")
+    assert e1 == pytest.approx(e2, rel=1e-12)
