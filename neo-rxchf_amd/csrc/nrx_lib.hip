@@ -287,7 +287,17 @@ __global__ void __launch_bounds__(256) k_prim_list(DevSys S, i64 ntup, const i64
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntup) return;
     i64 x[2 * CLS + 2];
-    for (int k = 0; k < 2 * CLS + 2; ++k) x[k] = idx[t * (2 * CLS + 2) + k];
+    bool ok = true;
+    for (int k = 0; k < 2 * CLS + 2; ++k) {
+        x[k] = idx[t * (2 * CLS + 2) + k];
+        i64 lim = (k == CLS || k == 2 * CLS + 1) ? S.npbf : S.npebf;
+        ok = ok && x[k] >= 0 && x[k] < lim;
+    }
+    if (!ok) { // out-of-range primitive index: flag the entry instead of reading outside the basis
+        outE[t] = __builtin_nan("");
+        if (outS) outS[t] = __builtin_nan("");
+        return;
+    }
     double e, s;
     eval_tuple<CLS>(S, x, e, s);
     outE[t] = e;
