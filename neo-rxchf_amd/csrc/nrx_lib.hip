@@ -34,6 +34,7 @@ static thread_local std::string g_err;
 static std::mutex g_mu;
 static int g_device = -1;
 static char g_arch[64] = "";
+static double *g_boys_dev = nullptr; // Boys table in HBM, one per process
 
 static int fail(int code, const char *fmt, ...)
 {
@@ -67,6 +68,7 @@ struct DevSys {
     const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc, *elcex, *nucex, *elcbfc, *nucbfc;
     const i64 *kpestr, *kpeend, *ampeb2c, *elcam, *nucam;
     const i64 *eord, *pord; // enumeration order -> original primitive index
+    const double *bt;       // Boys table (BT_N x BT_M)
 };
 
 struct System {
@@ -109,6 +111,12 @@ extern "C" int neorx_init(int64_t device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(NEORX_E_NODEVICE, "device %lld is %s; this library is built for gfx950 only", (long long)device,
                     prop.gcnArchName);
+    if (!g_boys_dev) {
+        std::vector<double> tab(BT_N * BT_M);
+        boys_table_fill(tab.data());
+        HIPCHK(hipMalloc(&g_boys_dev, tab.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(g_boys_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     g_device = (int)device;
     return NEORX_OK;
 }
@@ -120,6 +128,8 @@ extern "C" int neorx_finalize(void)
         delete kv.second;
     }
     g_sys.clear();
+    if (g_boys_dev) hipFree(g_boys_dev);
+    g_boys_dev = nullptr;
     g_device = -1;
     return NEORX_OK;
 }
@@ -164,6 +174,7 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
     d.nat = nat;
     d.ngtg = ngtg;
     d.pmass = pmass;
+    d.bt = g_boys_dev;
     // enumeration order: by Cartesian power pattern, then exponent
     auto order = [](i64 n, const i64 *am, const double *ex, std::vector<i64> &ord) {
         ord.resize(n);
@@ -261,6 +272,7 @@ __device__ inline SysParams d_params(const DevSys &S)
     P.zan = S.zan;
     P.bcoef = S.bcoef;
     P.gamma = S.gamma;
+    P.bt = S.bt;
     return P;
 }
 
@@ -515,7 +527,7 @@ __global__ void __launch_bounds__(256) k_gam_ee(DevSys S, double *__restrict__ G
                     Pair q[2];
                     make_pair(d_eprim(S, a1), d_eprim(S, c1), q[0]);
                     make_pair(d_eprim(S, a2), d_eprim(S, c2), q[1]);
-                    Sys s = mksys(2, &q[0], &q[1], 0, 0, 0.0, 0, 0);
+                    Sys s = mksys(d_params(S), 2, &q[0], &q[1], 0, 0, 0.0, 0, 0);
                     acc += coul(s, 0, 1) * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
                 }
     G[t] = acc;

@@ -2,10 +2,23 @@
 // CPU unit tests to check the engine's algebra against the oracle before any
 // GPU run.  Not part of the product library.
 #include <cstdint>
+#ifdef NRX_COUNT_FLOPS
+thread_local double nrx_flops = 0.0;
+#endif
 #include "../neo-rxchf_amd/csrc/nrx_engine.h"
 
 using namespace nrx;
 typedef int64_t i64;
+
+static const double *host_boys_table()
+{
+    static double *tab = nullptr;
+    if (!tab) {
+        tab = new double[BT_N * BT_M];
+        boys_table_fill(tab);
+    }
+    return tab;
+}
 
 static Prim eprim(i64 npebf, const double *ex, const i64 *am, const double *cen, i64 i)
 {
@@ -23,7 +36,7 @@ extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const do
                               double pmass, const double *cat, const double *zan, const double *bcoef,
                               const double *gamma, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
 {
-    SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma};
+    SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
     int ne = (int)cls, w = 2 * (ne + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
@@ -60,9 +73,37 @@ extern "C" double eng_gint(i64 op, i64 n, const i64 *pw, const double *ex, const
         }
         make_pair(pb, pk, q[i]);
     }
-    Sys s = mksys((int)n, &q[0], &q[1], &q[2], &q[3], gam[0], gam[1], gam[2]);
+    SysParams SP{0, 0, 1.0, nullptr, nullptr, nullptr, nullptr, host_boys_table()};
+    Sys s = mksys(SP, (int)n, &q[0], &q[1], &q[2], &q[3], gam[0], gam[1], gam[2]);
     if (op == 0) return ovl(s, pic);
     if (op == 1) return nuc(s, (int)a, C, pic);
     if (op == 2) return coul(s, (int)a, (int)b, pic);
     return kin(s, (int)a, (int)c, gk, mass, pic);
 }
+
+#ifdef NRX_COUNT_FLOPS
+// average floating-point operations per tuple of the engine's algorithm
+extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam,
+                                      const i64 *nucam, const double *elcbfc, const double *nucbfc, i64 nat, i64 ngtg,
+                                      double pmass, const double *cat, const double *zan, const double *bcoef,
+                                      const double *gamma, i64 cls, i64 ntup, const i64 *idx)
+{
+    SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
+    int ne = (int)cls, w = 2 * (ne + 1);
+    nrx_flops = 0.0;
+    for (i64 t = 0; t < ntup; ++t) {
+        const i64 *x = idx + t * w;
+        Pair q[4];
+        for (int i = 0; i < ne; ++i)
+            make_pair(eprim(npebf, elcex, elcam, elcbfc, x[i]), eprim(npebf, elcex, elcam, elcbfc, x[ne + 1 + i]), q[i]);
+        make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
+        double e = 0, s = 0;
+        if (cls == 1) gam1_tuple(S, q, e, s);
+        else if (cls == 2) gam2_tuple(S, q, e, s);
+        else gam3_tuple(S, q, e);
+    }
+    return nrx_flops / (double)ntup;
+}
+#endif
+
+extern "C" void eng_boys(i64 M, double T, double *F) { boys_ref((int)M, T, F, host_boys_table()); }

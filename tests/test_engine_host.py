@@ -38,3 +38,43 @@ def test_engine_large_T_branch():
     ee, _ = engine_prim_list(s, 2, idx)
     oe, _ = oracle.prim_list(s, 2, idx)
     check_close(ee, oe, what="large-T")
+
+
+def test_engine_boys_vs_reference_semantics():
+    """Device Boys function: identical to the reference's asymptotic branch
+    for T > 20; for T <= 20 within 1e-15 of the exact function (a 40-digit
+    series) and within 1e-11 of the reference's series, whose absolute 1e-12
+    cut leaves up to ~3e-13 relative error at high order (F_12(0.256))."""
+    import ctypes as C
+    import math
+
+    from _util import engine_host
+
+    E = engine_host()
+    F = (C.c_double * 40)()
+    for T in np.concatenate([np.linspace(0.0, 20.0, 157), [20.0, 20.0000001, 23.7, 41.0, 133.0]]):
+        M = 17
+        E.eng_boys(C.c_int64(M), C.c_double(T), F)
+        for m in range(M + 1):
+            ref = oracle.boys(m, T)  # the oracle's per-order restatement of iboys2
+            if T > 20:
+                asym = 0.88622692545275801365 / math.sqrt(T)
+                for l in range(1, m + 1):
+                    asym = asym * (l - 0.5) / T
+                assert F[m] == asym
+            else:
+                assert abs(F[m] - ref) <= 1e-11 * ref, (m, T, F[m], ref)
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 40
+    for T in [0.0, 0.2564102564102564, 3.3, 11.9, 19.97]:
+        E.eng_boys(C.c_int64(16), C.c_double(T), F)
+        for m in (0, 5, 12, 16):
+            Td = Decimal(T)
+            term = Decimal(1) / Decimal(2 * m + 1)
+            acc = term
+            for k in range(1, 400):
+                term = term * 2 * Td / Decimal(2 * m + 2 * k + 1)
+                acc += term
+            exact = float(acc * (-Td).exp())
+            assert abs(F[m] - exact) <= 1e-15 * exact, (m, T, F[m], exact)
