@@ -42,7 +42,7 @@ def test_engine_large_T_branch():
 
 def test_engine_boys_vs_reference_semantics():
     """Device Boys function: identical to the reference's asymptotic branch
-    for T > 20; for T <= 20 within 1e-15 of the exact function (a 40-digit
+    for T > 20; for T <= 20 within 4e-15 of the exact function (a 40-digit
     series) and within 1e-11 of the reference's series, whose absolute 1e-12
     cut leaves up to ~3e-13 relative error at high order (F_12(0.256))."""
     import ctypes as C
@@ -77,4 +77,4 @@ def test_engine_boys_vs_reference_semantics():
                 term = term * 2 * Td / Decimal(2 * m + 2 * k + 1)
                 acc += term
             exact = float(acc * (-Td).exp())
-            assert abs(F[m] - exact) <= 1e-15 * exact, (m, T, F[m], exact)
+            assert abs(F[m] - exact) <= 4e-15 * exact, (m, T, F[m], exact)
