@@ -46,9 +46,24 @@ def build_shim():
     return out
 
 
+def build_shim_driver():
+    """Fortran test program calling the drop-in subroutines (tests/fortran)."""
+    src = os.path.join(ROOT, "tests", "fortran", "drive_shim.f90")
+    if not os.path.exists(src):
+        return None
+    lib = os.path.join(HERE, "lib")
+    outdir = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "drive_shim")
+    run([FLANG, "-O2", "-fdefault-integer-8", "-J", outdir, "-o", out, src, "-L" + lib, "-lneorx_fshim",
+         "-lneorx_hip", "-Wl,-rpath,$ORIGIN/../../neo-rxchf_amd/lib"])
+    return out
+
+
 def build_all():
     build_hip()
     build_shim()
+    build_shim_driver()
 
 
 if __name__ == "__main__":

@@ -292,8 +292,13 @@ template <int CLS> __device__ inline void eval_tuple(const DevSys &S, const i64 
         gam3_tuple(P, q, e);
 }
 
+// minimum waves per SIMD requested from the register allocator
+#ifndef NRX_PRIM_WAVES
+#define NRX_PRIM_WAVES 1
+#endif
+
 template <int CLS>
-__global__ void __launch_bounds__(256) k_prim_list(DevSys S, i64 ntup, const i64 *__restrict__ idx,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NRX_PRIM_WAVES))) k_prim_list(DevSys S, i64 ntup, const i64 *__restrict__ idx,
                                                    double *__restrict__ outE, double *__restrict__ outS)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,7 +323,7 @@ __global__ void __launch_bounds__(256) k_prim_list(DevSys S, i64 ntup, const i64
 
 // enumeration (ip,jp,ie1,je1,...) in re-ordered primitive space
 template <int CLS>
-__global__ void __launch_bounds__(256) k_prim_range(DevSys S, i64 start, i64 count, double *__restrict__ outE,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NRX_PRIM_WAVES))) k_prim_range(DevSys S, i64 start, i64 count, double *__restrict__ outE,
                                                     double *__restrict__ outS)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;

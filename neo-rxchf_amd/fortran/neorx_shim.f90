@@ -1,0 +1,263 @@
+! neorx_shim.f90 -- drop-in replacements for the reference's geminal
+! integral and Fock-contraction subroutines, forwarding to libneorx_hip.so
+! through iso_c_binding.
+!
+! Every subroutine keeps the name, argument order and meaning of the
+! reference routine it replaces (file:line given above each one), so the
+! reference's SCF drivers link against this object instead of
+! gam2_IC1_OMP.o / gam3_IC1_OMP.o / gam4_ICR_OMP.o / gam_1_OMP.o / gam_ee.o.
+! Integers are 64-bit as in the reference build (-integer-size 64,
+! source/comp:7); the shim declares them as integer(c_int64_t) explicitly.
+! Failures print the library's message on unit 0 and stop the program: the
+! reference's callers have no status argument to check.
+module neorx_capi
+  use iso_c_binding
+  implicit none
+  interface
+    function neorx_init(dev) bind(C, name="neorx_init")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: dev
+      integer(c_int) :: neorx_init
+    end function
+    function neorx_last_error() bind(C, name="neorx_last_error")
+      import :: c_ptr
+      type(c_ptr) :: neorx_last_error
+    end function
+    function neorx_system_create(nebf, npebf, npbf, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
+                                 ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, handle) &
+                                 bind(C, name="neorx_system_create")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nebf, npebf, npbf, nat, ngtg
+      real(c_double), value :: pmass
+      real(c_double) :: cat(*), zan(*), bcoef(*), gamma(*), agebfcc(*), agnbfcc(*), elcex(*), nucex(*)
+      real(c_double) :: elcbfc(*), nucbfc(*)
+      integer(c_int64_t) :: kpestr(*), kpeend(*), ampeb2c(*), elcam(*), nucam(*)
+      integer(c_int64_t) :: handle
+      integer(c_int) :: neorx_system_create
+    end function
+    function neorx_system_destroy(handle) bind(C, name="neorx_system_destroy")
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: handle
+      integer(c_int) :: neorx_system_destroy
+    end function
+    function neorx_gam1(handle, g, gs) bind(C, name="neorx_gam1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: g(*), gs(*)
+      integer(c_int) :: neorx_gam1
+    end function
+    function neorx_gam2_ic1(handle, g, gs) bind(C, name="neorx_gam2_ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: g(*), gs(*)
+      integer(c_int) :: neorx_gam2_ic1
+    end function
+    function neorx_gam3_ic1(handle, g) bind(C, name="neorx_gam3_ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: g(*)
+      integer(c_int) :: neorx_gam3_ic1
+    end function
+    function neorx_gam_ee(handle, g) bind(C, name="neorx_gam_ee")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: g(*)
+      integer(c_int) :: neorx_gam_ee
+    end function
+    function neorx_gam4_icr(ne, np, g2s, gee, g4) bind(C, name="neorx_gam4_icr")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: g2s(*), gee(*), g4(*)
+      integer(c_int) :: neorx_gam4_icr
+    end function
+    function neorx_eg2ic1(ne, np, de, dp, g, gs, fe, se, fp, sp, e, s) bind(C, name="neorx_eg2ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: de(*), dp(*), g(*), gs(*), fe(*), se(*), fp(*), sp(*), e, s
+      integer(c_int) :: neorx_eg2ic1
+    end function
+    function neorx_eg3ic1(ne, np, de, dp, g, fe, fp, e) bind(C, name="neorx_eg3ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: de(*), dp(*), g(*), fe(*), fp(*), e
+      integer(c_int) :: neorx_eg3ic1
+    end function
+    function neorx_eg4icr(ne, np, de, dp, g, fe, fp, e) bind(C, name="neorx_eg4icr")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: de(*), dp(*), g(*), fe(*), fp(*), e
+      integer(c_int) :: neorx_eg4icr
+    end function
+  end interface
+contains
+  subroutine neorx_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    p = neorx_last_error()
+    n = 0
+    call c_f_pointer(p, msg, [512])
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write(0, '(a,a,i0,a,512a)') where, ': neorx error ', rc, ': ', msg(1:n)
+    error stop 1
+  end subroutine
+
+  ! contracted-function ranges from the primitive -> contracted map
+  ! (the primitives of one contracted function are contiguous, make_KPE
+  ! source/xcneo_HYBRID.f:1269-1307)
+  subroutine neorx_kpe(nebf, npebf, ampeb2c, kpestr, kpeend)
+    integer(c_int64_t), intent(in) :: nebf, npebf, ampeb2c(npebf)
+    integer(c_int64_t), intent(out) :: kpestr(nebf), kpeend(nebf)
+    integer(c_int64_t) :: i, c
+    kpestr = 0
+    kpeend = -1
+    do i = 1, npebf
+      c = ampeb2c(i)
+      if (kpestr(c) == 0) kpestr(c) = i
+      kpeend(c) = i
+    end do
+  end subroutine
+end module neorx_capi
+
+! GAM1_OMP_MD, source/gam_1_OMP.f:2-397: e-p integrals written to
+! GAM_1.ufm / GAM_1s.ufm (direct access, one double per record) in the
+! pack_4D(nebf,nebf,npbf,jec1,iec1,jp,ip) order.
+subroutine GAM1_OMP_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                       AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf)
+  real(c_double), allocatable :: g(:), gs(:)
+  call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
+  allocate(g(ng1), gs(ng1))
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, kpestr, &
+                   kpeend, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM1_OMP_MD')
+  call neorx_check(neorx_gam1(h, g, gs), 'GAM1_OMP_MD')
+  call neorx_check(neorx_system_destroy(h), 'GAM1_OMP_MD')
+  open(801, file='GAM_1.ufm', form='unformatted', status='unknown', access='direct', recl=8)
+  open(802, file='GAM_1s.ufm', form='unformatted', status='unknown', access='direct', recl=8)
+  do ia = 1, ng1
+    write(801, rec=ia) g(ia)
+    write(802, rec=ia) gs(ia)
+  end do
+  close(801)
+  close(802)
+  deallocate(g, gs)
+end subroutine
+
+! calc_GAM_ee, source/gam_ee.f:2-168: contracted ERIs written to GAM_ee.ufm
+subroutine calc_GAM_ee(nebf, npebf, ngee, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npebf, ngee
+  integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3)
+  real(c_double) :: AGEBFCC(npebf), ELCEX(npebf), ELCBFC(npebf, 3)
+  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf), nucam(3)
+  real(c_double) :: one(1), zero3(3), g(ngee)
+  call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
+  one = 1.0d0
+  zero3 = 0.0d0
+  nucam = 0
+  call neorx_check(neorx_system_create(nebf, npebf, 1_c_int64_t, 0_c_int64_t, 1_c_int64_t, 1.0d0, zero3, one, one, &
+                   one, kpestr, kpeend, AMPEB2C, AGEBFCC, one, ELCEX, one, ELCAM, nucam, ELCBFC, zero3, h), &
+                   'calc_GAM_ee')
+  call neorx_check(neorx_gam_ee(h, g), 'calc_GAM_ee')
+  call neorx_check(neorx_system_destroy(h), 'calc_GAM_ee')
+  open(810, file='GAM_ee.ufm', form='unformatted', status='unknown', access='direct', recl=8)
+  do ia = 1, ngee
+    write(810, rec=ia) g(ia)
+  end do
+  close(810)
+end subroutine
+
+! GAM2_IC1, source/gam2_IC1_OMP.f:2-212
+subroutine GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                    KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                    GM2ICR, GM2sICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM2ICR(ng2), GM2sICR(ng2)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM2_IC1')
+  call neorx_check(neorx_gam2_ic1(h, GM2ICR, GM2sICR), 'GAM2_IC1')
+  call neorx_check(neorx_system_destroy(h), 'GAM2_IC1')
+end subroutine
+
+! GAM3_IC1, source/gam3_IC1_OMP.f:2-298
+subroutine GAM3_IC1(Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                    KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, GM3ICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM3ICR(ng3)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM3_IC1')
+  call neorx_check(neorx_gam3_ic1(h, GM3ICR), 'GAM3_IC1')
+  call neorx_check(neorx_system_destroy(h), 'GAM3_IC1')
+end subroutine
+
+! GAM4_ICR, source/gam4_ICR_OMP.f:2-138: reads GAM_ee.ufm from the working
+! directory like read_GAM_ee (source/xchf1_independent/GAM_IO.f:28)
+subroutine GAM4_ICR(Nchunks, ne, np, ngee, ng2, ng4, GAM_2s, GAM_4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, ne, np, ngee, ng2, ng4
+  real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
+  real(c_double), allocatable :: gee(:)
+  integer(c_int64_t) :: ia
+  allocate(gee(ngee))
+  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do ia = 1, ngee
+    read(810, rec=ia) gee(ia)
+  end do
+  close(810)
+  call neorx_check(neorx_gam4_icr(ne, np, GAM_2s, gee, GAM_4), 'GAM4_ICR')
+  deallocate(gee)
+end subroutine
+
+! EG2IC1, source/gam2_IC1_OMP.f:339-443
+subroutine EG2IC1(Nchunks, nebf, npbf, ng2, DE, DP, GM2ICR, GM2SICR, FE, SE, FP, SP, E_gam2, S_gam2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2SICR(ng2)
+  real(c_double) :: FE(nebf, nebf), SE(nebf, nebf), FP(npbf, npbf), SP(npbf, npbf), E_gam2, S_gam2
+  call neorx_check(neorx_eg2ic1(nebf, npbf, DE, DP, GM2ICR, GM2SICR, FE, SE, FP, SP, E_gam2, S_gam2), 'EG2IC1')
+end subroutine
+
+! EG3IC1, source/gam3_IC1_OMP.f:433-534
+subroutine EG3IC1(Nchunks, nebf, npbf, ng3, DE, DP, GM3ICR, focke, fockp, E_gam3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), focke(nebf, nebf), fockp(npbf, npbf), E_gam3
+  call neorx_check(neorx_eg3ic1(nebf, npbf, DE, DP, GM3ICR, focke, fockp, E_gam3), 'EG3IC1')
+end subroutine
+
+! EG4ICR, source/gam4_ICR_OMP.f:273-376
+subroutine EG4ICR(Nchunks, nebf, npbf, ng4, DE, DP, GM4ICR, focke, fockp, E_gam4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng4
+  real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), GM4ICR(ng4), focke(nebf, nebf), fockp(npbf, npbf), E_gam4
+  call neorx_check(neorx_eg4icr(nebf, npbf, DE, DP, GM4ICR, focke, fockp, E_gam4), 'EG4ICR')
+end subroutine

@@ -1,0 +1,69 @@
+! drive_shim.f90 -- test driver that calls the drop-in subroutines exactly as
+! the reference's integral phase and Fock build do (xcneo_HYBRID.f:761-1020,
+! xchf1_fock.f:246-299), linked against libneorx_fshim.so.
+! Input and output are raw stream files written/read by the GPU tests.
+program drive_shim
+  use iso_c_binding
+  implicit none
+  interface
+    subroutine GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                        KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                        GM2ICR, GM2sICR)
+      import :: c_int64_t, c_double
+      integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+      real(c_double) :: pmass, cat(*), zan(*), bcoef1(*), gamma1(*)
+      integer(c_int64_t) :: KPESTR(*), KPEEND(*), AMPEB2C(*), ELCAM(*), NUCAM(*)
+      real(c_double) :: AGEBFCC(*), AGNBFCC(*), ELCEX(*), NUCEX(*), ELCBFC(*), NUCBFC(*), GM2ICR(*), GM2sICR(*)
+    end subroutine
+  end interface
+  integer(c_int64_t) :: nebf, npebf, npbf, nat, ngtg, ng2, ng3, ng4, ngee, ng1, one, zero
+  real(c_double) :: pmass, e2, s2, e3, e4
+  real(c_double), allocatable :: cat(:), zan(:), bcoef(:), gamma(:), agebfcc(:), agnbfcc(:), elcex(:), nucex(:)
+  real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:)
+  integer(c_int64_t), allocatable :: kpestr(:), kpeend(:), ampeb2c(:), elcam(:), nucam(:)
+  real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:)
+  character(len=512) :: fin, fout
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(10) nebf, npebf, npbf, nat, ngtg, pmass
+  allocate(cat(3*nat), zan(nat), bcoef(ngtg), gamma(ngtg), kpestr(nebf), kpeend(nebf), ampeb2c(npebf))
+  allocate(agebfcc(npebf), agnbfcc(npbf), elcex(npebf), nucex(npbf), elcam(3*npebf), nucam(3*npbf))
+  allocate(elcbfc(3*npebf), nucbfc(3*npbf), de(nebf, nebf), dp(npbf, npbf))
+  read(10) cat, zan, bcoef, gamma, kpestr, kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, &
+           elcbfc, nucbfc, de, dp
+  close(10)
+  one = 1
+  zero = 0
+  ng1 = nebf**2 * npbf**2
+  ng2 = nebf**4 * npbf**2
+  ng3 = nebf**6 * npbf**2
+  ng4 = nebf**8 * npbf**2
+  ngee = nebf**4
+  allocate(gm2(ng2), gm2s(ng2), gm3(ng3), gm4(ng4), fe(nebf, nebf), se(nebf, nebf), fp(npbf, npbf), sp(npbf, npbf))
+
+  call GAM1_OMP_MD(nebf, npebf, npbf, ng1, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                   ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc)
+  call calc_GAM_ee(nebf, npebf, ngee, ampeb2c, agebfcc, elcex, elcam, elcbfc)
+  call GAM2_IC1(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
+                ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm2, gm2s)
+  call GAM3_IC1(one, nebf, npebf, npbf, ng3, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
+                ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm3)
+  call GAM4_ICR(one, nebf, npbf, ngee, ng2, ng4, gm2s, gm4)
+
+  open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  fe = 0.0d0
+  fp = 0.0d0
+  call EG2IC1(one, nebf, npbf, ng2, de, dp, gm2, gm2s, fe, se, fp, sp, e2, s2)
+  write(11) gm2, gm2s, fe, se, fp, sp, e2, s2
+  fe = 0.0d0
+  fp = 0.0d0
+  call EG3IC1(one, nebf, npbf, ng3, de, dp, gm3, fe, fp, e3)
+  write(11) gm3, fe, fp, e3
+  fe = 0.0d0
+  fp = 0.0d0
+  call EG4ICR(one, nebf, npbf, ng4, de, dp, gm4, fe, fp, e4)
+  write(11) gm4, fe, fp, e4
+  close(11)
+end program

@@ -1,0 +1,100 @@
+"""The Fortran drop-in subroutines (neo-rxchf_amd/fortran/neorx_shim.f90)
+called from a Fortran program with the reference's argument lists, checked
+against the oracle.  The GPU test runs the program; the CPU test checks that
+it links and resolves every library."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from neorx import synth
+from _util import check_close
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "_build", "drive_shim")
+
+
+def test_driver_links():
+    assert os.path.exists(DRIVER), "run __graft_entry__.build()"
+    out = subprocess.run(["ldd", DRIVER], capture_output=True, text=True).stdout
+    assert "libneorx_fshim.so" in out and "libneorx_hip.so" in out
+    assert "not found" not in out
+
+
+def _write_input(path, s, DE, DP):
+    with open(path, "wb") as f:
+        np.array([s["nebf"], s["npebf"], s["npbf"], s["nat"], s["ngtg"]], dtype=np.int64).tofile(f)
+        np.array([s["pmass"]], dtype=np.float64).tofile(f)
+        for k in ("cat", "zan", "bcoef", "gamma"):
+            np.ascontiguousarray(s[k], dtype=np.float64).ravel().tofile(f)
+        for k in ("kpestr", "kpeend", "ampeb2c"):
+            np.ascontiguousarray(s[k], dtype=np.int64).tofile(f)
+        for k in ("agebfcc", "agnbfcc", "elcex", "nucex"):
+            np.ascontiguousarray(s[k], dtype=np.float64).tofile(f)
+        for k in ("elcam", "nucam"):
+            np.asarray(s[k], dtype=np.int64).ravel(order="F").tofile(f)
+        for k in ("elcbfc", "nucbfc"):
+            np.asarray(s[k], dtype=np.float64).ravel(order="F").tofile(f)
+        np.asarray(DE).ravel(order="F").tofile(f)
+        np.asarray(DP).ravel(order="F").tofile(f)
+
+
+@pytest.mark.gpu
+def test_fortran_dropin_matches_oracle():
+    s = synth.small(seed=81, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,), ngtg=2)
+    ne, npb = s["nebf"], s["npbf"]
+    sub = dict(s)
+    # keep the nebf^8 in-core GAM_4 small: first 3 contracted functions
+    keep = 3
+    last = s["kpeend"][keep - 1]
+    for k in ("elcex", "agebfcc"):
+        sub[k] = s[k][:last].copy()
+    for k in ("elcam", "elcbfc"):
+        sub[k] = np.asfortranarray(s[k][:last])
+    sub["ampeb2c"] = s["ampeb2c"][:last].copy()
+    sub["kpestr"], sub["kpeend"] = s["kpestr"][:keep].copy(), s["kpeend"][:keep].copy()
+    sub["nebf"], sub["npebf"] = keep, int(last)
+    s = sub
+    ne, npb = s["nebf"], s["npbf"]
+    DE, DP = synth.density(ne, 2.0, 1), synth.density(npb, 1.0, 2)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        _write_input(fin, s, DE, DP)
+        print("case fortran drop-in driver", flush=True)
+        r = subprocess.run([DRIVER, fin, fout], cwd=td, capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, r.stderr
+        out = np.fromfile(fout, dtype=np.float64)
+        gam1 = np.fromfile(os.path.join(td, "GAM_1.ufm"), dtype=np.float64)
+        gee = np.fromfile(os.path.join(td, "GAM_ee.ufm"), dtype=np.float64)
+    ng2, ng3, ng4 = ne ** 4 * npb ** 2, ne ** 6 * npb ** 2, ne ** 8 * npb ** 2
+    sizes = [ng2, ng2, ne * ne, ne * ne, npb * npb, npb * npb, 1, 1, ng3, ne * ne, npb * npb, 1, ng4, ne * ne,
+             npb * npb, 1]
+    parts = np.split(out, np.cumsum(sizes)[:-1])
+    og2, og2s = oracle.gam2_ic1(s)
+    check_close(parts[0], og2, what="GAM2_IC1")
+    check_close(parts[1], og2s, what="GAM2_IC1 S")
+    FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+    SE, SP, E, S = oracle.eg2ic1(ne, npb, DE, DP, og2, og2s, FE, FP)
+    check_close(parts[2], FE.ravel(order="F"), what="EG2IC1 FE")
+    check_close(parts[3], SE.ravel(order="F"), what="EG2IC1 SE")
+    check_close(parts[4], FP.ravel(order="F"), what="EG2IC1 FP")
+    assert parts[6][0] == pytest.approx(E, rel=1e-11) and parts[7][0] == pytest.approx(S, rel=1e-11)
+    og3 = oracle.gam3_ic1(s)
+    check_close(parts[8], og3, what="GAM3_IC1")
+    FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+    E3 = oracle.eg3ic1(ne, npb, DE, DP, og3, FE, FP)
+    check_close(parts[9], FE.ravel(order="F"), what="EG3IC1 FE")
+    assert parts[11][0] == pytest.approx(E3, rel=1e-11)
+    oee = oracle.gam_ee(s)
+    check_close(gee, oee, what="calc_GAM_ee file")
+    og4 = oracle.gam4_icr(ne, npb, og2s, oee)
+    check_close(parts[12], og4, what="GAM4_ICR")
+    FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+    E4 = oracle.eg4icr(ne, npb, DE, DP, og4, FE, FP)
+    check_close(parts[13], FE.ravel(order="F"), what="EG4ICR FE")
+    assert parts[15][0] == pytest.approx(E4, rel=1e-11)
+    og1, og1s = oracle.gam1(s)
+    check_close(gam1, og1, what="GAM_1 file")
