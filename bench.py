@@ -79,7 +79,18 @@ def main():
     ap.add_argument("--nsample", type=int, default=1 << 20, help="e-e-p and e-e-e-p tuples per step")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
               "(the tuple space shards with no communication, see DESIGN.md)", file=sys.stderr)
@@ -183,11 +194,11 @@ def main():
     print(json.dumps(out))
 
 
-# Algorithmic FP64 flops per primitive tuple of config2, counted by the host
-# build of the engine over the same seeded samples (tests/engine_host.cpp,
-# flop counter); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE.  Filled from the committed profiles.
-FLOPS_PER_TUPLE = {1: 182710.0, 2: 332532.0, 3: 47895.0}
+# Algorithmic FP64 operations per primitive tuple of this workload, counted by
+# the host build of the engine (tests/count_flops.py); traffic = HBM bytes
+# per launch of the dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# (profiles/), scaled by the copy-kernel calibration.
+FLOPS_PER_TUPLE = {1: 93477.0, 2: 198357.0, 3: 42087.0}
 TRAFFIC = {}
 
 if __name__ == "__main__":
