@@ -24,7 +24,7 @@
 //   * F_n follows the reference's two-branch Boys semantics: for T > 20 the
 //     asymptotic form sqrt(pi/T)/2 * (2m-1)!!/(2T)^m exactly as
 //     AC_Integrals/common.f:317-341 of the reference; for T <= 20 (where the
-//     reference's truncated series is within ~1e-13 of the exact function)
+//     reference's truncated series is within ~5e-12 of the exact function)
 //     a 7-term Taylor expansion from a 0.05-spaced table of exact values,
 //     then downward recursion.
 //
@@ -37,18 +37,40 @@
 // Hermite expansion of the core is reused for every classical nucleus, the
 // overlaps of the kinetic operator are shared by its variants, and the terms
 // that depend only on gamma_k + gamma_l are evaluated for k <= l.
+//
+// Register residency: every polynomial has a compile-time capacity and every
+// loop over coefficients has a compile-time trip count (coefficients past the
+// degree are kept at zero), so after inlining all coefficient arrays are
+// indexed by constants and live in VGPRs.  The capacity NL of the absorbed
+// polynomials is a template parameter: kernels are instantiated for a small
+// capacity that covers almost every tuple and for a general one.
 #pragma once
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define NRX_HD __host__ __device__ inline
+#define NRX_HD __host__ __device__ inline __attribute__((always_inline))
 #else
 #include <cmath>
 #define NRX_HD inline
 #endif
 
+#define NRX_UNROLL _Pragma("unroll")
+// loops over the three Cartesian directions are kept rolled so that only one
+// direction's polynomials are live at a time
+#define NRX_DIRS _Pragma("nounroll")
+
+// Compiler fence between elementary integrals of one tuple: the pair records
+// are re-read after it, so values of one integral are not kept live into the
+// next one (they are cheap to reload and expensive to hold in registers).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NRX_FENCE() asm volatile("" ::: "memory")
+#else
+#define NRX_FENCE() ((void)0)
+#endif
+
 // Host builds used for accounting may define NRX_COUNT_FLOPS to tally the
-// floating-point operations the algorithm performs (FMA = 2).
+// floating-point operations the algorithm performs (FMA = 2).  The counts are
+// those of the fixed-capacity loops actually executed.
 #if defined(NRX_COUNT_FLOPS) && !defined(__HIP_DEVICE_COMPILE__)
 extern thread_local double nrx_flops;
 #define NRX_CNT(x) (nrx_flops += (double)(x))
@@ -58,145 +80,151 @@ extern thread_local double nrx_flops;
 
 namespace nrx {
 
-// polynomial capacities: per-particle polynomials have degree <= 4 (d on both
-// sides) or <= 6 inside the kinetic operator; absorbed/core polynomials have
-// degree <= 16 (four particles with d functions).
-constexpr int NS = 7;
-constexpr int NP = 18;
+constexpr int NS = 5;  // per-particle polynomial capacity (degree <= 4, l <= 2 on both sides)
+constexpr int NK = 7;  // per-particle polynomial inside the kinetic operator (degree <= 6)
+constexpr int NB = 17; // Boys-order polynomial (total angular momentum <= 16)
 constexpr int NPART = 4;
 constexpr double PI = 3.141592653589793238462643383279502884;
 // pgike1 (AC_Integrals/g2.f:216) normalises its overlaps with this value
 constexpr double PI_G2 = 3.141592650;
 
 template <int N> struct PolyT {
-    double c[N];
-    int n; // number of coefficients (degree + 1)
+    double c[N]; // c[k] = 0 for k >= n
+    int n;       // number of coefficients (degree + 1)
 };
-using Poly = PolyT<NP>;
-using PolyS = PolyT<NS>;
+
+NRX_HD constexpr double binomc(int n, int k)
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+NRX_HD constexpr double dfact(int n) // (n)!! for odd n, 1 for n <= 0
+{
+    double r = 1.0;
+    for (int i = n; i > 1; i -= 2) r *= i;
+    return r;
+}
+NRX_HD int imin(int a, int b) { return a < b ? a : b; }
 
 template <int N> NRX_HD void p_const(PolyT<N> &p, double v)
 {
-    p.n = 1;
+    NRX_UNROLL for (int k = 0; k < N; ++k) p.c[k] = 0.0;
     p.c[0] = v;
+    p.n = 1;
 }
 
-template <int NA, int NB> NRX_HD void p_copy(const PolyT<NA> &a, PolyT<NB> &b)
+template <int NA, int NR> NRX_HD void p_copy(const PolyT<NA> &a, PolyT<NR> &r)
 {
-    int n = a.n < NB ? a.n : NB;
-    for (int k = 0; k < n; ++k) b.c[k] = a.c[k];
-    b.n = n;
+    NRX_UNROLL for (int k = 0; k < NR; ++k) r.c[k] = (k < NA) ? a.c[k] : 0.0;
+    r.n = imin(a.n, NR);
 }
 
-// r = a * b (truncated to NR coefficients; degrees are sized to fit)
-template <int NA, int NB, int NR> NRX_HD void p_mul(const PolyT<NA> &a, const PolyT<NB> &b, PolyT<NR> &r)
+// r = a * b (capacity NR; degrees are sized to fit)
+template <int NA, int NBB, int NR> NRX_HD void p_mul(const PolyT<NA> &a, const PolyT<NBB> &b, PolyT<NR> &r)
 {
-    int n = a.n + b.n - 1;
-    if (n > NR) n = NR;
-    for (int k = 0; k < n; ++k) r.c[k] = 0.0;
-    for (int i = 0; i < a.n; ++i) {
-        double ai = a.c[i];
-        int jm = b.n;
-        if (i + jm > n) jm = n - i;
-        for (int j = 0; j < jm; ++j) r.c[i + j] += ai * b.c[j];
-        NRX_CNT(2 * jm);
+    double t[NR];
+    NRX_UNROLL for (int k = 0; k < NR; ++k) t[k] = 0.0;
+    NRX_UNROLL for (int i = 0; i < NA; ++i) {
+        NRX_UNROLL for (int j = 0; j < NBB; ++j) {
+            if (i + j < NR) t[i + j] += a.c[i] * b.c[j];
+        }
     }
+    int n = imin(a.n + b.n - 1, NR);
+    NRX_UNROLL for (int k = 0; k < NR; ++k) r.c[k] = t[k];
     r.n = n;
+    NRX_CNT(2 * NA * NBB);
 }
 
-// a *= (y + s)
-template <int N> NRX_HD void p_mul_lin(PolyT<N> &a, double s)
+// a *= (y + s) when on, unchanged otherwise (branch-free)
+template <int N> NRX_HD void p_mul_lin(PolyT<N> &a, double s, bool on = true)
 {
-    int n = a.n;
-    if (n < N) {
-        a.c[n] = a.c[n - 1];
-        for (int k = n - 1; k > 0; --k) a.c[k] = a.c[k - 1] + s * a.c[k];
-        a.c[0] = s * a.c[0];
-        a.n = n + 1;
-        NRX_CNT(2 * n - 1);
-    }
+    double g = on ? 1.0 : 0.0, sl = on ? s : 1.0;
+    double t[N];
+    NRX_UNROLL for (int k = 0; k < N; ++k) t[k] = ((k > 0) ? g * a.c[k - 1] : 0.0) + sl * a.c[k];
+    NRX_UNROLL for (int k = 0; k < N; ++k) a.c[k] = t[k];
+    a.n = imin(a.n + (on ? 1 : 0), N);
+    NRX_CNT(3 * N);
 }
 
-// (y + sa)^la (y + sb)^lb
+// (y + sa)^la (y + sb)^lb with la, lb <= 2 (l <= 2 per Cartesian component
+// is checked when a system is created)
 template <int N> NRX_HD void p_pair(int la, double sa, int lb, double sb, PolyT<N> &r)
 {
     p_const(r, 1.0);
-    for (int i = 0; i < la; ++i) p_mul_lin(r, sa);
-    for (int i = 0; i < lb; ++i) p_mul_lin(r, sb);
+    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(r, sa, i < la);
+    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(r, sb, i < lb);
 }
 
-// h(s) = E[f(s + e)], e ~ N(0, var)
+// h(s) = E[f(s + e)], e ~ N(0, var):  h_j = sum_i f_{j+2i} C(j+2i, j) (2i-1)!! var^i
 template <int N> NRX_HD void p_smooth(const PolyT<N> &f, double var, PolyT<N> &h)
 {
-    double mom[N];
-    mom[0] = 1.0;
-    if (N > 1) mom[1] = 0.0;
-    for (int k = 2; k < f.n; ++k) mom[k] = (k & 1) ? 0.0 : mom[k - 2] * (k - 1) * var;
-    for (int j = 0; j < f.n; ++j) {
-        double acc = f.c[j], bin = 1.0; // C(k, j) for k = j + 2i
-        for (int k = j + 2; k < f.n; k += 2) {
-            bin = bin * (double)(k * (k - 1)) / (double)((k - j) * (k - j - 1));
-            acc += f.c[k] * bin * mom[k - j];
-        }
-        h.c[j] = acc;
-        NRX_CNT(6 * ((f.n - j) / 2));
+    double vp[N / 2 + 1];
+    vp[0] = 1.0;
+    NRX_UNROLL for (int i = 1; i <= N / 2; ++i) vp[i] = vp[i - 1] * var;
+    double t[N];
+    NRX_UNROLL for (int j = 0; j < N; ++j) {
+        double acc = f.c[j];
+        NRX_UNROLL for (int i = 1; j + 2 * i < N; ++i) acc += f.c[j + 2 * i] * (binomc(j + 2 * i, j) * dfact(2 * i - 1)) * vp[i];
+        t[j] = acc;
     }
-    NRX_CNT(2 * f.n);
+    NRX_UNROLL for (int j = 0; j < N; ++j) h.c[j] = t[j];
     h.n = f.n;
+    NRX_CNT(N * N / 2 + N);
 }
 
 template <int N> NRX_HD double p_eval(const PolyT<N> &h, double s)
 {
     double v = 0.0;
-    for (int k = h.n - 1; k >= 0; --k) v = v * s + h.c[k];
-    NRX_CNT(2 * h.n);
+    NRX_UNROLL for (int k = N - 1; k >= 0; --k) v = v * s + h.c[k];
+    NRX_CNT(2 * N);
     return v;
 }
 
-// g(y) = h(m0 + beta y)  (g may alias h)
+// g(y) = h(m0 + beta y):  g_j = beta^j sum_{k>=j} C(k,j) m0^{k-j} h_k   (g may alias h)
 template <int N> NRX_HD void p_compose(const PolyT<N> &h, double m0, double beta, PolyT<N> &g)
 {
-    PolyT<N> t = h;
-    for (int i = 0; i < t.n; ++i)
-        for (int k = t.n - 2; k >= i; --k) t.c[k] += m0 * t.c[k + 1];
+    double mp[N], t[N];
+    mp[0] = 1.0;
+    NRX_UNROLL for (int i = 1; i < N; ++i) mp[i] = mp[i - 1] * m0;
     double bp = 1.0;
-    for (int k = 0; k < t.n; ++k) {
-        g.c[k] = t.c[k] * bp;
+    NRX_UNROLL for (int j = 0; j < N; ++j) {
+        double acc = h.c[j];
+        NRX_UNROLL for (int k = j + 1; k < N; ++k) acc += binomc(k, j) * mp[k - j] * h.c[k];
+        t[j] = acc * bp;
         bp *= beta;
     }
-    g.n = t.n;
-    NRX_CNT(t.n * (t.n - 1) + 2 * t.n);
+    NRX_UNROLL for (int j = 0; j < N; ++j) g.c[j] = t[j];
+    g.n = h.n;
+    NRX_CNT(N * N + 3 * N);
 }
 
 // Hermite-Gaussian expansion: f(s) exp(-w s^2) = sum_t e_t Lambda_t(s),
 // Lambda_t = d^t/dc^t exp(-w (s - c)^2) at c = 0;
-// s Lambda_t = Lambda_{t+1}/(2w) + t Lambda_{t-1}
+// s Lambda_t = Lambda_{t+1}/(2w) + t Lambda_{t-1}  (Horner in s)
 template <int N> NRX_HD void p_hermite(const PolyT<N> &f, double w, PolyT<N> &e)
 {
     double i2w = 0.5 / w;
-    int n = f.n;
     double cur[N];
-    cur[0] = f.c[n - 1];
-    int m = 1;
-    for (int k = n - 2; k >= 0; --k) {
-        double prev = 0.0; // cur[t-1] before update
-        for (int t = 0; t <= m; ++t) {
-            double old = (t < m) ? cur[t] : 0.0;
-            double v = prev * i2w;
-            if (t + 1 < m) v += (t + 1) * cur[t + 1];
-            prev = old;
-            cur[t] = v;
+    NRX_UNROLL for (int t = 0; t < N; ++t) cur[t] = 0.0;
+    cur[0] = f.c[N - 1];
+    NRX_UNROLL for (int k = N - 2; k >= 0; --k) {
+        double nw[N];
+        NRX_UNROLL for (int t = 0; t < N; ++t) {
+            double v = (t >= 1) ? cur[t - 1] * i2w : 0.0;
+            if (t + 1 < N) v += (t + 1) * cur[t + 1];
+            nw[t] = v;
         }
-        ++m;
-        cur[0] += f.c[k];
-        NRX_CNT(4 * m);
+        nw[0] += f.c[k];
+        NRX_UNROLL for (int t = 0; t < N; ++t) cur[t] = nw[t];
     }
-    for (int t = 0; t < m; ++t) e.c[t] = cur[t];
-    e.n = m;
+    NRX_UNROLL for (int t = 0; t < N; ++t) e.c[t] = cur[t];
+    e.n = f.n;
+    NRX_CNT(4 * N * N);
 }
 
 // ------------------------------------------------------------------
-// Boys function, reference semantics, F[0..M]
+// Boys function, reference semantics
 // ------------------------------------------------------------------
 constexpr int BT_N = 401;      // grid T = 0, 0.05, ..., 20
 constexpr double BT_H = 0.05;
@@ -204,25 +232,54 @@ constexpr int BT_M = 32;       // orders stored per grid point
 constexpr int BT_K = 7;        // Taylor terms
 constexpr int BT_MAXM = BT_M - BT_K; // largest order evaluated from the table
 
+// F_M(T) for T <= 20 from the table: sum_k F_{M+k}(T_i) (-dt)^k / k!
+NRX_HD double boys_top(int M, double T, const double *tab)
+{
+    int i = (int)(T * (1.0 / BT_H) + 0.5);
+    double dt = T - i * BT_H;
+    const double *row = tab + i * BT_M + M;
+    double v = row[BT_K - 1];
+    NRX_UNROLL for (int k = BT_K - 1; k >= 1; --k) v = row[k - 1] - dt * v * (1.0 / k);
+    return v;
+}
+
+// F[0..M] (host-side helper and tests)
 NRX_HD void boys_ref(int M, double T, double *F, const double *tab)
 {
     if (T > 20.0) {
         // common.f:326-328
         F[0] = 0.88622692545275801365 / sqrt(T);
         for (int l = 1; l <= M; ++l) F[l] = F[l - 1] * (1.0 * l - 0.5) / T;
-        NRX_CNT(3 * M + 2);
         return;
     }
-    int i = (int)(T * (1.0 / BT_H) + 0.5);
-    double dt = T - i * BT_H;
-    const double *row = tab + i * BT_M + M;
-    // F_M(T) = sum_k F_{M+k}(T_i) (-dt)^k / k!
-    double v = row[BT_K - 1];
-    for (int k = BT_K - 1; k >= 1; --k) v = row[k - 1] - dt * v * (1.0 / k);
-    F[M] = v;
+    F[M] = boys_top(M, T, tab);
     double ex = exp(-T), t2 = 2.0 * T;
     for (int l = M - 1; l >= 0; --l) F[l] = (t2 * F[l + 1] + ex) * (1.0 / (2 * l + 1));
-    NRX_CNT(4 * BT_K + 4 * M + 10);
+}
+
+// sum_k c_k F_k(T) for a polynomial c of degree M = c.n - 1
+NRX_HD double boys_dot(const PolyT<NB> &c, double T, const double *tab)
+{
+    int M = c.n - 1;
+    double s = 0.0;
+    if (T > 20.0) {
+        double f = 0.88622692545275801365 / sqrt(T), it = 1.0 / T;
+        s = c.c[0] * f;
+        NRX_UNROLL for (int l = 1; l < NB; ++l) {
+            f = f * (l - 0.5) * it;
+            s += c.c[l] * f;
+        }
+    } else {
+        double f = boys_top(M, T, tab);
+        double ex = exp(-T), t2 = 2.0 * T;
+        NRX_UNROLL for (int k = NB - 1; k >= 0; --k) {
+            double fn = (t2 * f + ex) * (1.0 / (2 * k + 1));
+            f = (k < M) ? fn : f;
+            s += c.c[k] * f; // c.c[k] = 0 for k > M
+        }
+    }
+    NRX_CNT(4 * NB + 4 * BT_K + 10);
+    return s;
 }
 
 // host-side construction of the table (exact values, extended precision)
@@ -244,7 +301,7 @@ inline void boys_table_fill(double *tab)
 }
 
 // ------------------------------------------------------------------
-// per-particle bra*ket product
+// bra*ket product of one particle (precomputed per primitive pair)
 // ------------------------------------------------------------------
 struct Prim {
     double a;
@@ -252,14 +309,14 @@ struct Prim {
     int l[3];
 };
 
-struct Pair {
-    double p, P[3], K; // exponent, centre, prod_d exp(-ab/p (A-B)^2)
-    PolyS f[3];        // per-direction polynomial in y = x - P
-    double beta, B[3], sA[3], sB[3];
-    int la[3], lb[3];
+struct PairRec {
+    double p, K, beta;       // exponent a+b, prod_d exp(-ab/p (A-B)^2), ket exponent
+    double P[3], B[3];       // product centre, ket centre
+    double sA[3], sB[3];     // P - A, P - B
+    int la[3], lb[3];        // bra / ket powers
 };
 
-NRX_HD void make_pair(const Prim &bra, const Prim &ket, Pair &q)
+NRX_HD void make_pair(const Prim &bra, const Prim &ket, PairRec &q)
 {
     double a = bra.a, b = ket.a, p = a + b;
     q.p = p;
@@ -275,43 +332,41 @@ NRX_HD void make_pair(const Prim &bra, const Prim &ket, Pair &q)
         q.sB[d] = P - ket.A[d];
         q.la[d] = bra.l[d];
         q.lb[d] = ket.l[d];
-        p_pair(bra.l[d], q.sA[d], ket.l[d], q.sB[d], q.f[d]);
     }
     q.K = exp(-a * b / p * r2);
-    NRX_CNT(3 * 12 + 6);
+}
+
+// per-direction polynomial of a particle in y = x - P
+template <int N> NRX_HD void fpoly(const PairRec &q, int d, PolyT<N> &f)
+{
+    p_pair(q.la[d], q.sA[d], q.lb[d], q.sB[d], f);
 }
 
 // ------------------------------------------------------------------
 // Star elimination per direction
 // ------------------------------------------------------------------
-// variable being reduced: polynomial F in the particle coordinate y and
-// Gaussian exp(-w (y - c)^2)
 template <int N> struct VarT {
-    PolyT<N> F;
-    double w, c;
+    PolyT<N> F; // polynomial in the particle coordinate y
+    double w, c; // Gaussian exp(-w (y - c)^2)
 };
-using Var = VarT<NP>;
 
 // Integrate leaf L (edge g, d = P_leaf - P_parent) into parent Q; returns the
 // scalar factor sqrt(pi/(wL+g)) * exp(...).
-template <int NL, int NQ>
-NRX_HD double absorb_leaf(const PolyT<NL> &LF, double Lw, double Lc, double g, double d, VarT<NQ> &Q, double pic)
+template <int NLF, int NQ>
+NRX_HD double absorb_leaf(const PolyT<NLF> &LF, double Lw, double Lc, double g, double d, VarT<NQ> &Q, double pic)
 {
     double W = Lw + g;
-    PolyT<NL> h;
+    PolyT<NLF> h;
     p_smooth(LF, 0.5 / W, h);
     double fac = sqrt(pic / W);
     if (g == 0.0) {
         double v = p_eval(h, Lc);
-        for (int k = 0; k < Q.F.n; ++k) Q.F.c[k] *= v;
-        NRX_CNT(Q.F.n + 4);
+        NRX_UNROLL for (int k = 0; k < NQ; ++k) Q.F.c[k] *= v;
         return fac;
     }
     double beta = g / W, m0 = (Lw * Lc - g * d) / W;
     p_compose(h, m0, beta, h);
-    PolyT<NQ> r;
-    p_mul(Q.F, h, r);
-    Q.F = r;
+    p_mul(Q.F, h, Q.F);
     double kap = Lw * g / W;
     double cl = d + Lc;
     double Wq = Q.w + kap;
@@ -334,7 +389,7 @@ template <int N> NRX_HD double close_var(const VarT<N> &V, double pic)
 // hub by gam[i] (bra+ket).
 struct Sys {
     int n;
-    const Pair *q[NPART];
+    const PairRec *q[NPART];
     double gam[NPART];
     const double *bt; // Boys table
 };
@@ -342,168 +397,162 @@ struct Sys {
 NRX_HD double Kprod(const Sys &s)
 {
     double k = 1.0;
-    for (int i = 0; i < s.n; ++i) k *= s.q[i]->K;
+    NRX_UNROLL for (int i = 0; i < NPART; ++i)
+        if (i < s.n) k *= s.q[i]->K;
     return k;
 }
 
-// hub state along d after absorbing every electron whose skip flag is unset
-NRX_HD double hub_absorb(const Sys &s, int d, unsigned skip, Var &H, double pic)
+// 1-D overlap along d; particles ia and ic (when >= 0) use the given
+// polynomials instead of their own
+template <int NL>
+NRX_HD double ovl_dim(const Sys &s, int d, int ia, const PolyT<NK> &Fa, int ic, const PolyT<NK> &Fc, double pic)
 {
     int h = s.n - 1;
-    p_copy(s.q[h]->f[d], H.F);
+    VarT<NL> H;
+    if (h == ia)
+        p_copy(Fa, H.F);
+    else if (h == ic)
+        p_copy(Fc, H.F);
+    else
+        fpoly(*s.q[h], d, H.F);
     H.w = s.q[h]->p;
     H.c = 0.0;
     double fac = 1.0;
-    for (int i = 0; i < h; ++i)
-        if (!((skip >> i) & 1u))
-            fac *= absorb_leaf(s.q[i]->f[d], s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
-    return fac;
-}
-
-// 1-D overlap along d with the given per-particle polynomials
-NRX_HD double ovl_dim(const Sys &s, int d, const PolyS *const *F, double pic)
-{
-    int h = s.n - 1;
-    Var H;
-    p_copy(*F[h], H.F);
-    H.w = s.q[h]->p;
-    H.c = 0.0;
-    double fac = 1.0;
-    for (int i = 0; i < h; ++i) fac *= absorb_leaf(*F[i], s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
+    NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
+        if (i >= h) continue;
+        PolyT<NK> f;
+        if (i == ia)
+            f = Fa;
+        else if (i == ic)
+            f = Fc;
+        else
+            fpoly(*s.q[i], d, f);
+        fac *= absorb_leaf(f, s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
+    }
     return fac * close_var(H, pic);
 }
 
-NRX_HD double ovl(const Sys &s, double pic = PI)
+template <int NL> NRX_HD double ovl(const Sys &s, double pic = PI)
 {
+    NRX_FENCE();
     double r = Kprod(s);
-    for (int d = 0; d < 3; ++d) {
-        const PolyS *F[NPART];
-        for (int i = 0; i < s.n; ++i) F[i] = &s.q[i]->f[d];
-        r *= ovl_dim(s, d, F, pic);
-    }
+    PolyT<NK> dummy;
+    p_const(dummy, 1.0);
+    NRX_DIRS for (int d = 0; d < 3; ++d) r *= ovl_dim<NL>(s, d, -1, dummy, -1, dummy, pic);
     return r;
 }
 
 // Kinetic energy of particle a (-1/(2m) <bra| d^2/dx_a^2 |ket>) for nv <= 2
 // variants sharing one geminal configuration: variant v has a ket geminal of
 // exponent gk[v] between a and partner c (gk[v] == 0 or c < 0: none).
+template <int NL>
 NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, double mass, double pic, double *out)
 {
-    double S[3], T[3][2];
-    const Pair &A = *s.q[a];
-    for (int d = 0; d < 3; ++d) {
-        const PolyS *F[NPART];
-        for (int i = 0; i < s.n; ++i) F[i] = &s.q[i]->f[d];
-        S[d] = ovl_dim(s, d, F, pic);
+    NRX_FENCE();
+    // product rule over directions, accumulated direction by direction:
+    // R_v = sum_d T_d prod_{d' != d} S_d',  Pr = prod_d S_d
+    double Pr = 1.0, R[2] = {0.0, 0.0};
+    const PairRec &A = *s.q[a];
+    PolyT<NK> dummy;
+    p_const(dummy, 1.0);
+    NRX_DIRS for (int d = 0; d < 3; ++d) {
+        double Sd = ovl_dim<NL>(s, d, -1, dummy, -1, dummy, pic);
         int L = A.lb[d];
-        PolyS xm[5]; // u_A(y) X^{L-2+j}, j = 0..4
-        for (int j = 0; j < 5; ++j)
-            if (L - 2 + j >= 0) p_pair(A.la[d], A.sA[d], L - 2 + j, A.sB[d], xm[j]);
-        for (int v = 0; v < nv; ++v) {
+        double sA = A.sA[d], sB = A.sB[d];
+        int la = A.la[d];
+        NRX_UNROLL for (int v = 0; v < 2; ++v) {
+            if (v >= nv) continue;
             double g = (c >= 0) ? gk[v] : 0.0;
             double al = A.beta + g;
             double dl = (g != 0.0) ? g * (A.B[d] - s.q[c]->B[d]) : 0.0;
+            // d^2/dx^2 of the ket X^L exp(-beta X^2) exp(-g (x - x_c)^2), X = y + sB:
             // F0 = L(L-1)X^{L-2} - 4L dl X^{L-1} + (-4L al + 4dl^2 - 2al) X^L + 8 al dl X^{L+1} + 4al^2 X^{L+2}
             // F1 = 4 L g X^{L-1} - 8 dl g X^L - 8 al g X^{L+1}      (times X_c)
             // F2 = 4 g^2 X^L                                        (times X_c^2)
+            // each times the bra factor u_A = (y + sA)^la
             double cf0[5] = {(double)(L * (L - 1)), -4.0 * L * dl, -4.0 * L * al + 4.0 * dl * dl - 2.0 * al,
                              8.0 * al * dl, 4.0 * al * al};
-            PolyS F0;
-            F0.n = 1;
-            F0.c[0] = 0.0;
-            for (int j = 0; j < 5; ++j) {
-                if (L - 2 + j < 0 || cf0[j] == 0.0) continue;
-                for (int k = F0.n; k < xm[j].n; ++k) F0.c[k] = 0.0;
-                if (xm[j].n > F0.n) F0.n = xm[j].n;
-                for (int k = 0; k < xm[j].n; ++k) F0.c[k] += cf0[j] * xm[j].c[k];
-                NRX_CNT(2 * xm[j].n);
+            double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
+            // sum_j cf[j] X^{L-2+j} over L-2+j >= 0 by Horner in X, then X^{max(L-2,0)} u_A
+            int j0 = (L >= 2) ? 0 : 2 - L;
+            PolyT<NK> F0, F1;
+            p_const(F0, cf0[4]);
+            p_const(F1, cf1[4]);
+            NRX_UNROLL for (int j = 3; j >= 0; --j) {
+                bool on = j >= j0;
+                p_mul_lin(F0, sB, on);
+                p_mul_lin(F1, sB, on);
+                F0.c[0] += on ? cf0[j] : 0.0;
+                F1.c[0] += on ? cf1[j] : 0.0;
             }
-            const PolyS *F[NPART];
-            for (int i = 0; i < s.n; ++i) F[i] = &s.q[i]->f[d];
-            F[a] = &F0;
-            double t = ovl_dim(s, d, F, pic);
+            NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                p_mul_lin(F0, sB, i < L - 2);
+                p_mul_lin(F1, sB, i < L - 2);
+            }
+            NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                p_mul_lin(F0, sA, i < la);
+                p_mul_lin(F1, sA, i < la);
+            }
+            double t = ovl_dim<NL>(s, d, a, F0, -1, dummy, pic);
             if (g != 0.0) {
-                double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
-                PolyS F1, F2;
-                F1.n = 1;
-                F1.c[0] = 0.0;
-                for (int j = 1; j < 4; ++j) {
-                    if (L - 2 + j < 0 || cf1[j] == 0.0) continue;
-                    for (int k = F1.n; k < xm[j].n; ++k) F1.c[k] = 0.0;
-                    if (xm[j].n > F1.n) F1.n = xm[j].n;
-                    for (int k = 0; k < xm[j].n; ++k) F1.c[k] += cf1[j] * xm[j].c[k];
-                }
-                F2 = xm[2];
-                for (int k = 0; k < F2.n; ++k) F2.c[k] *= 4.0 * g * g;
-                PolyS fc1 = s.q[c]->f[d];
-                double xc = s.q[c]->P[d] - s.q[c]->B[d];
-                p_mul_lin(fc1, xc);
-                PolyS fc2 = fc1;
-                p_mul_lin(fc2, xc);
-                F[a] = &F1;
-                F[c] = &fc1;
-                t += ovl_dim(s, d, F, pic);
-                F[a] = &F2;
-                F[c] = &fc2;
-                t += ovl_dim(s, d, F, pic);
+                PolyT<NK> F2;
+                p_const(F2, 4.0 * g * g);
+                NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
+                NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
+                PolyT<NK> fc;
+                fpoly(*s.q[c], d, fc);
+                double xc = s.q[c]->sB[d];
+                p_mul_lin(fc, xc);
+                t += ovl_dim<NL>(s, d, a, F1, c, fc, pic);
+                p_mul_lin(fc, xc);
+                t += ovl_dim<NL>(s, d, a, F2, c, fc, pic);
             }
-            T[d][v] = t;
+            R[v] = R[v] * Sd + Pr * t;
         }
+        Pr *= Sd;
     }
     double kp = Kprod(s);
-    for (int v = 0; v < nv; ++v) {
-        double r = T[0][v] * S[1] * S[2] + S[0] * T[1][v] * S[2] + S[0] * S[1] * T[2][v];
-        out[v] = -0.5 * r / mass * kp;
+    NRX_UNROLL for (int v = 0; v < 2; ++v) {
+        if (v >= nv) continue;
+        out[v] = -0.5 * R[v] / mass * kp;
     }
 }
 
-NRX_HD double kin(const Sys &s, int a, int c, double gk, double mass, double pic = PI)
+template <int NL> NRX_HD double kin(const Sys &s, int a, int c, double gk, double mass, double pic = PI)
 {
-    double o;
-    kin_multi(s, a, c, &gk, 1, mass, pic, &o);
-    return o;
+    double o[2];
+    double g2[2] = {gk, gk};
+    kin_multi<NL>(s, a, c, g2, 1, mass, pic, o);
+    return o[0];
 }
 
 // P(u^2) = sum_k hk R_k(u^2), R_0 = 1, R_1 = -2 lam D,
 // R_{k+1} = -2 lam (D R_k + k R_{k-1}), lam = l0 + l1 u^2
-NRX_HD void hermite_coulomb(const Poly &hk, double D, double l0, double l1, Poly &P)
+template <int N> NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, PolyT<N> &P)
 {
-    double Rm[NP], R[NP];
-    int rn = 1;
-    R[0] = 1.0;
-    Rm[0] = 0.0;
-    P.n = hk.n;
-    for (int j = 0; j < hk.n; ++j) P.c[j] = 0.0;
-    P.c[0] = hk.c[0];
-    for (int k = 0; k + 1 < hk.n; ++k) {
-        // Rn = -2 (l0 + l1 u) (D R + k Rm); Rm has rn-1 coefficients
-        double prevt = 0.0;
-        for (int j = 0; j <= rn; ++j) {
-            double tj = (j < rn) ? D * R[j] + ((j < rn - 1) ? k * Rm[j] : 0.0) : 0.0;
-            double v = l0 * tj + l1 * prevt;
-            prevt = tj;
-            if (j < rn) Rm[j] = R[j];
-            R[j] = -2.0 * v;
-        }
-        ++rn;
-        double h = hk.c[k + 1];
-        for (int j = 0; j < rn; ++j) P.c[j] += h * R[j];
-        NRX_CNT(9 * rn);
+    double Rm[N], R[N];
+    NRX_UNROLL for (int j = 0; j < N; ++j) {
+        R[j] = (j == 0) ? 1.0 : 0.0;
+        Rm[j] = 0.0;
+        P.c[j] = (j == 0) ? hk.c[0] : 0.0;
     }
-}
-
-NRX_HD double boys_combine(const Poly *P3, double T, const double *bt)
-{
-    Poly xy, xyz;
-    p_mul(P3[0], P3[1], xy);
-    p_mul(xy, P3[2], xyz);
-    double F[NP];
-    int M = xyz.n - 1;
-    boys_ref(M, T, F, bt);
-    double s = 0.0;
-    for (int k = 0; k <= M; ++k) s += xyz.c[k] * F[k];
-    NRX_CNT(2 * (M + 1));
-    return s;
+    NRX_UNROLL for (int k = 0; k + 1 < N; ++k) {
+        double t[N];
+        NRX_UNROLL for (int j = 0; j <= k; ++j) t[j] = D * R[j] + k * Rm[j];
+        double Rn[N];
+        NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
+            double v = (j <= k) ? l0 * t[j] : 0.0;
+            if (j >= 1) v += l1 * t[j - 1];
+            Rn[j] = -2.0 * v;
+        }
+        NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
+            Rm[j] = (j <= k) ? R[j] : 0.0;
+            R[j] = Rn[j];
+            P.c[j] += hk.c[k + 1] * Rn[j];
+        }
+    }
+    P.n = hk.n;
+    NRX_CNT(5 * N * N);
 }
 
 NRX_HD void underflow(double &x)
@@ -513,15 +562,24 @@ NRX_HD void underflow(double &x)
 
 // Hermite coefficients and absolute centre of the core variable of a
 // nuclear-attraction integral on particle a (everything else absorbed)
-NRX_HD double nuc_core(const Sys &s, int a, int d, Poly &e, double &X, double &w, double pic)
+template <int NL> NRX_HD double nuc_core(const Sys &s, int a, int d, PolyT<NL> &e, double &X, double &w, double pic)
 {
     int h = s.n - 1;
-    Var H;
-    double fac = hub_absorb(s, d, (a == h) ? 0u : (1u << a), H, pic);
-    Var A;
+    VarT<NL> H;
+    fpoly(*s.q[h], d, H.F);
+    H.w = s.q[h]->p;
+    H.c = 0.0;
+    double fac = 1.0;
+    NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
+        if (i >= h || i == a) continue;
+        PolyT<NS> f;
+        fpoly(*s.q[i], d, f);
+        fac *= absorb_leaf(f, s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
+    }
     int core = h;
+    VarT<NL> A;
     if (a != h) {
-        p_copy(s.q[a]->f[d], A.F);
+        fpoly(*s.q[a], d, A.F);
         A.w = s.q[a]->p;
         A.c = 0.0;
         fac *= absorb_leaf(H.F, H.w, H.c, s.gam[a], s.q[h]->P[d] - s.q[a]->P[d], A, pic);
@@ -529,9 +587,8 @@ NRX_HD double nuc_core(const Sys &s, int a, int d, Poly &e, double &X, double &w
     } else {
         A = H;
     }
-    Poly sh;
-    p_compose(A.F, A.c, 1.0, sh);
-    p_hermite(sh, A.w, e);
+    p_compose(A.F, A.c, 1.0, A.F);
+    p_hermite(A.F, A.w, e);
     X = s.q[core]->P[d] + A.c;
     w = A.w;
     return fac;
@@ -539,68 +596,77 @@ NRX_HD double nuc_core(const Sys &s, int a, int d, Poly &e, double &X, double &w
 
 // sum_c zan[c] <1/|r_a - C_c|>, each centre's value flushed below 1e-100 when
 // flush is set (the reference applies underflow() per centre)
+template <int NL>
 NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const double *zan, bool flush, double pic = PI)
 {
-    Poly e[3];
+    NRX_FENCE();
+    PolyT<NL> e[3];
     double X[3], w = 1.0, fac = Kprod(s);
-    for (int d = 0; d < 3; ++d) fac *= nuc_core(s, a, d, e[d], X[d], w, pic);
+    NRX_UNROLL for (int d = 0; d < 3; ++d) fac *= nuc_core<NL>(s, a, d, e[d], X[d], w, pic);
     double pre = fac * 2.0 * pic / w;
     double tot = 0.0;
     for (int c = 0; c < nat; ++c) {
-        Poly P3[3];
-        double D[3];
-        for (int d = 0; d < 3; ++d) {
-            D[d] = X[d] - cat[3 * c + d];
-            hermite_coulomb(e[d], D[d], 0.0, w, P3[d]);
+        PolyT<NB> cc;
+        p_const(cc, 1.0);
+        double D2 = 0.0;
+        NRX_UNROLL for (int d = 0; d < 3; ++d) {
+            double D = X[d] - cat[3 * c + d];
+            PolyT<NL> P;
+            hermite_coulomb(e[d], D, 0.0, w, P);
+            p_mul(cc, P, cc);
+            D2 += D * D;
         }
-        double T = w * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
-        double v = pre * boys_combine(P3, T, s.bt);
+        double T = w * D2;
+        double v = pre * boys_dot(cc, T, s.bt);
         if (flush) underflow(v);
         tot += zan[c] * v;
     }
     return tot;
 }
 
-NRX_HD double nuc(const Sys &s, int a, const double *C, double pic = PI)
+// hk[t+u] += ea[t] eb[u] (-1)^u
+template <int N> NRX_HD void conv_add(const PolyT<N> &ea, const PolyT<N> &eb, PolyT<N> &hk)
 {
-    double z = 1.0;
-    return nuc_sum(s, a, 1, C, &z, false, pic);
-}
-
-// Hermite convolution of the two Coulomb sides: hk[t+u] += ea[t] eb[u] (-1)^u
-NRX_HD void conv_add(const Poly &ea, const Poly &eb, Poly &hk)
-{
-    int n = ea.n + eb.n - 1;
-    for (int k = hk.n; k < n; ++k) hk.c[k] = 0.0;
-    if (n > hk.n) hk.n = n;
-    for (int t = 0; t < ea.n; ++t)
-        for (int u = 0; u < eb.n; ++u) hk.c[t + u] += ea.c[t] * ((u & 1) ? -eb.c[u] : eb.c[u]);
-    NRX_CNT(2 * ea.n * eb.n);
+    NRX_UNROLL for (int t = 0; t < N; ++t) {
+        NRX_UNROLL for (int u = 0; t + u < N; ++u) hk.c[t + u] += ea.c[t] * ((u & 1) ? -eb.c[u] : eb.c[u]);
+    }
+    int n = imin(ea.n + eb.n - 1, N);
+    hk.n = hk.n > n ? hk.n : n;
+    NRX_CNT(N * N);
 }
 
 // Coulomb <1/|r_a - r_b|>, a < b
-NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
+template <int NL> NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
 {
+    NRX_FENCE();
     int h = s.n - 1;
-    Poly P3[3];
-    double Dv[3], fac = Kprod(s);
+    PolyT<NB> cc; // product over directions of the polynomials in u^2
+    p_const(cc, 1.0);
+    double D2 = 0.0, fac = Kprod(s);
     double wa = 1.0, wb = 1.0, gab = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        unsigned skip = (1u << a) | ((b < h) ? (1u << b) : 0u);
-        Var H;
-        fac *= hub_absorb(s, d, skip, H, pic);
-        Var A, B;
-        p_copy(s.q[a]->f[d], A.F);
+    NRX_DIRS for (int d = 0; d < 3; ++d) {
+        VarT<NL> H;
+        fpoly(*s.q[h], d, H.F);
+        H.w = s.q[h]->p;
+        H.c = 0.0;
+        NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
+            if (i >= h || i == a || i == b) continue;
+            PolyT<NS> f;
+            fpoly(*s.q[i], d, f);
+            fac *= absorb_leaf(f, s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
+        }
+        VarT<NL> A, B;
+        fpoly(*s.q[a], d, A.F);
         A.w = s.q[a]->p;
         A.c = 0.0;
-        Poly hk;
-        hk.n = 1;
-        hk.c[0] = 0.0;
+        PolyT<NL> hk;
+        p_const(hk, 0.0);
+        bool cycle = false;
         if (b == h) {
             gab = s.gam[a];
             B = H;
         } else {
-            p_copy(s.q[b]->f[d], B.F);
+            fpoly(*s.q[b], d, B.F);
             B.w = s.q[b]->p;
             B.c = 0.0;
             double ga = s.gam[a], gb = s.gam[b];
@@ -612,6 +678,7 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
                     fac *= absorb_leaf(H.F, H.w, H.c, gb, s.q[h]->P[d] - s.q[b]->P[d], B, pic);
             } else {
                 // hub linked to both: integrate the hub conditionally on (y_a, y_b)
+                cycle = true;
                 double W = H.w + ga + gb;
                 double dpa = s.q[h]->P[d] - s.q[a]->P[d], dpb = s.q[h]->P[d] - s.q[b]->P[d];
                 fac *= sqrt(pic / W);
@@ -630,57 +697,58 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
                     B.w = Wn;
                 }
                 gab = ga * gb / W;
-                // h(m0 + ba ya + bb yb) = sum_j (bb yb)^j / j! h^{(j)}(m0 + ba ya)
-                Poly hs;
-                p_smooth(H.F, 0.5 / W, hs);
+                // h(m0 + ba ya + bb yb) = sum_j (bb yb)^j / j! h^{(j)}(m0 + ba ya);
+                // the hub polynomial has degree <= NS - 1 here (its own
+                // polynomial only: every other electron is a or b)
+                PolyT<NS> hs;
+                p_copy(H.F, hs);
+                p_smooth(hs, 0.5 / W, hs);
                 double m0 = (H.w * H.c - ga * dpa - gb * dpb) / W, ba = ga / W, bb = gb / W;
-                double bbj = 1.0;
-                int hn = hs.n;
-                for (int j = 0; j < hn; ++j) {
-                    if (j > 0) {
-                        for (int k = 0; k + 1 < hn - (j - 1); ++k) hs.c[k] = hs.c[k + 1] * (k + 1) / (double)j;
-                        bbj *= bb;
-                    }
-                    Poly der;
-                    der.n = hn - j;
-                    for (int k = 0; k < der.n; ++k) der.c[k] = hs.c[k];
-                    Poly ga_, Fa, Fb, ea, eb;
-                    p_compose(der, m0, ba, ga_);
-                    p_mul(A.F, ga_, Fa);
-                    Fb.n = B.F.n + j;
-                    for (int k = 0; k < j; ++k) Fb.c[k] = 0.0;
-                    for (int k = 0; k < B.F.n; ++k) Fb.c[k + j] = B.F.c[k] * bbj;
+                // D = h^{(j)} / j!  (D_{j+1,k} = (k+1)/(j+1) D_{j,k+1}),
+                // Yb = B.F (bb yb)^j; the loop stays rolled, one term live
+                PolyT<NS> D = hs;
+                PolyT<NL> Yb;
+                p_copy(B.F, Yb);
+                _Pragma("nounroll") for (int j = 0; j < NS; ++j) {
+                    PolyT<NS> der;
+                    p_compose(D, m0, ba, der);
+                    PolyT<NL> Fa, Fb, ea, eb;
+                    p_mul(A.F, der, Fa);
                     p_compose(Fa, A.c, 1.0, Fa);
-                    p_compose(Fb, B.c, 1.0, Fb);
+                    p_compose(Yb, B.c, 1.0, Fb);
                     p_hermite(Fa, A.w, ea);
                     p_hermite(Fb, B.w, eb);
                     conv_add(ea, eb, hk);
+                    double ij = 1.0 / (j + 1);
+                    NRX_UNROLL for (int k = 0; k < NS; ++k) D.c[k] = (k + 1 < NS) ? D.c[k + 1] * ((k + 1) * ij) : 0.0;
+                    NRX_UNROLL for (int k = NL - 1; k >= 0; --k) Yb.c[k] = (k >= 1) ? Yb.c[k - 1] * bb : 0.0;
+                    Yb.n = imin(Yb.n + 1, NL);
                 }
-                goto have_hk;
             }
         }
-        {
-            Poly sa, sb, ea, eb;
-            p_compose(A.F, A.c, 1.0, sa);
-            p_compose(B.F, B.c, 1.0, sb);
-            p_hermite(sa, A.w, ea);
-            p_hermite(sb, B.w, eb);
+        if (!cycle) {
+            PolyT<NL> ea, eb;
+            p_compose(A.F, A.c, 1.0, A.F);
+            p_compose(B.F, B.c, 1.0, B.F);
+            p_hermite(A.F, A.w, ea);
+            p_hermite(B.F, B.w, eb);
             conv_add(ea, eb, hk);
         }
-    have_hk:
         wa = A.w;
         wb = B.w;
         double rho = wa * wb / (wa + wb);
         double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
-        Dv[d] = (s.q[a]->P[d] + A.c) - (s.q[b]->P[d] + B.c);
-        hermite_coulomb(hk, Dv[d], l0, l1, P3[d]);
+        double Dd = (s.q[a]->P[d] + A.c) - (s.q[b]->P[d] + B.c);
+        PolyT<NL> P;
+        hermite_coulomb(hk, Dd, l0, l1, P);
+        p_mul(cc, P, cc);
+        D2 += Dd * Dd;
     }
     double rho = wa * wb / (wa + wb);
     double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
-    double D2 = Dv[0] * Dv[0] + Dv[1] * Dv[1] + Dv[2] * Dv[2];
     double pre = pic / (wa + wb);
     pre = pre * sqrt(pre) * 2.0 * pic / (rho + gab);
-    return fac * pre * exp(-l0 * D2) * boys_combine(P3, l1 * D2, s.bt);
+    return fac * pre * exp(-l0 * D2) * boys_dot(cc, l1 * D2, s.bt);
 }
 
 // ------------------------------------------------------------------
@@ -693,15 +761,15 @@ struct SysParams {
     const double *bt; // Boys table
 };
 
-NRX_HD Sys mksys(const SysParams &S, int n, const Pair *p0, const Pair *p1, const Pair *p2, const Pair *p3, double g0,
-                 double g1, double g2)
+NRX_HD Sys mksys(const SysParams &S, int n, const PairRec *p0, const PairRec *p1, const PairRec *p2,
+                 const PairRec *p3, double g0, double g1, double g2)
 {
     Sys s;
     s.n = n;
     s.q[0] = p0;
-    s.q[1] = p1;
-    s.q[2] = p2;
-    s.q[3] = p3;
+    s.q[1] = p1 ? p1 : p0;
+    s.q[2] = p2 ? p2 : p0;
+    s.q[3] = p3 ? p3 : p0;
     s.gam[0] = g0;
     s.gam[1] = g1;
     s.gam[2] = g2;
@@ -710,11 +778,25 @@ NRX_HD Sys mksys(const SysParams &S, int n, const Pair *p0, const Pair *p1, cons
     return s;
 }
 
-// xcalc_GAM1_MD (gam_1_OMP.f:398-839).  q[0] = electron, q[1] = proton.
-// Terms that depend on gamma_k + gamma_l only are summed over k <= l.
-NRX_HD void gam1_tuple(const SysParams &S, const Pair *q, double &ansE, double &ansS)
+// polynomial capacity needed by a tuple: the largest per-direction sum of
+// bra+ket powers over its particles, plus 2 where a multi-particle kinetic
+// operator raises one polynomial (e-p and e-e-p classes)
+NRX_HD int tuple_degree(const PairRec *const *q, int n, bool kinetic)
 {
-    const Pair *e = &q[0], *p = &q[1];
+    int m = 0;
+    for (int d = 0; d < 3; ++d) {
+        int s = 0;
+        for (int i = 0; i < n; ++i) s += q[i]->la[d] + q[i]->lb[d];
+        m = s > m ? s : m;
+    }
+    return m + (kinetic ? 2 : 0);
+}
+
+// xcalc_GAM1_MD (gam_1_OMP.f:398-839).  e = electron, p = proton.
+// Terms that depend on gamma_k + gamma_l only are summed over k <= l.
+template <int NL>
+NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, double &ansE, double &ansS)
+{
     double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
     double vcegx = 0, vcegg = 0, vcpgx = 0, vcpgg = 0, vmgx = 0, vmgg = 0;
     double ogx = 0, ogg = 0;
@@ -723,28 +805,28 @@ NRX_HD void gam1_tuple(const SysParams &S, const Pair *q, double &ansE, double &
         Sys s1 = mksys(S, 2, e, p, 0, 0, gv, 0, 0);  // (e, p): hub = p
         Sys s1r = mksys(S, 2, p, e, 0, 0, gv, 0, 0); // (p, e): hub = e
         double gks[2] = {0.0, gv}, kk[2];
-        kin_multi(s1, 0, 1, gks, 2, 1.0, PI_G2, kk);
+        kin_multi<NL>(s1, 0, 1, gks, 2, 1.0, PI_G2, kk);
         tegx += bv * kk[0];
         texg += bv * kk[1];
-        kin_multi(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
+        kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
         tpgx += bv * kk[0];
         tpxg += bv * kk[1];
-        vcegx += bv * nuc_sum(s1, 0, S.nat, S.cat, S.zan, false);
-        vcpgx += bv * nuc_sum(s1, 1, S.nat, S.cat, S.zan, false);
-        vmgx += bv * coul(s1, 0, 1);
-        ogx += bv * ovl(s1);
+        vcegx += bv * nuc_sum<NL>(s1, 0, S.nat, S.cat, S.zan, false);
+        vcpgx += bv * nuc_sum<NL>(s1, 1, S.nat, S.cat, S.zan, false);
+        vmgx += bv * coul<NL>(s1, 0, 1);
+        ogx += bv * ovl<NL>(s1);
         for (int l = 0; l < S.ngtg; ++l) {
             double gw = S.gamma[l], bb = bv * S.bcoef[l];
             Sys s2 = mksys(S, 2, e, p, 0, 0, gv + gw, 0, 0);
             Sys s2r = mksys(S, 2, p, e, 0, 0, gv + gw, 0, 0);
-            tegg += bb * kin(s2, 0, 1, gw, 1.0, PI_G2);
-            tpgg += bb * kin(s2r, 0, 1, gw, S.pmass, PI_G2);
+            tegg += bb * kin<NL>(s2, 0, 1, gw, 1.0, PI_G2);
+            tpgg += bb * kin<NL>(s2r, 0, 1, gw, S.pmass, PI_G2);
             if (l < k) continue;
             double w = (l == k) ? bb : 2.0 * bb;
-            vcegg += w * nuc_sum(s2, 0, S.nat, S.cat, S.zan, false);
-            vcpgg += w * nuc_sum(s2, 1, S.nat, S.cat, S.zan, false);
-            vmgg += w * coul(s2, 0, 1);
-            ogg += w * ovl(s2);
+            vcegg += w * nuc_sum<NL>(s2, 0, S.nat, S.cat, S.zan, false);
+            vcpgg += w * nuc_sum<NL>(s2, 1, S.nat, S.cat, S.zan, false);
+            vmgg += w * coul<NL>(s2, 0, 1);
+            ogg += w * ovl<NL>(s2);
         }
     }
     double te = tegx + texg + tegg, tp = tpgx + tpxg + tpgg;
@@ -753,20 +835,21 @@ NRX_HD void gam1_tuple(const SysParams &S, const Pair *q, double &ansE, double &
     ansS = 2.0 * ogx + ogg;
 }
 
-// xcalc_GAM2_MD (gam_2_OMP.f:546-1077).  q = (e1, e2, p).
-NRX_HD void gam2_tuple(const SysParams &S, const Pair *q, double &ansE, double &ansS)
+// xcalc_GAM2_MD (gam_2_OMP.f:546-1077).
+template <int NL>
+NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
+                       double &ansS)
 {
-    const Pair *e1 = &q[0], *e2 = &q[1], *p = &q[2];
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
     for (int ik = 0; ik < S.ngtg; ++ik) {
         double gk = S.gamma[ik], bk = S.bcoef[ik];
         Sys s = mksys(S, 3, e1, e2, p, 0, 0.0, gk, 0); // g(e2,p)
-        double xgTE = kin(s, 0, -1, 0.0, 1.0);
-        double xgVEC = -nuc_sum(s, 0, S.nat, S.cat, S.zan, true);
-        double xgVEP = coul(s, 0, 2);
+        double xgTE = kin<NL>(s, 0, -1, 0.0, 1.0);
+        double xgVEC = -nuc_sum<NL>(s, 0, S.nat, S.cat, S.zan, true);
+        double xgVEP = coul<NL>(s, 0, 2);
         underflow(xgVEP);
         xgVEP = -xgVEP;
-        double xgVEE = coul(s, 0, 1);
+        double xgVEE = coul<NL>(s, 0, 1);
         underflow(xgVEE);
         gTE += bk * xgTE;
         gVEC += bk * xgVEC;
@@ -784,29 +867,29 @@ NRX_HD void gam2_tuple(const SysParams &S, const Pair *q, double &ansE, double &
             // on this configuration with ket partner exponent gk.
             Sys sb = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
             double gks[2] = {0.0, gk}, kk[2];
-            kin_multi(sb, 0, 2, gks, 2, 1.0, PI, kk);
+            kin_multi<NL>(sb, 0, 2, gks, 2, 1.0, PI, kk);
             gTEg1 += bb * kk[0];
             gTEg2 += bb * kk[1];
-            gTPg += bb * kin(sb, 2, 1, gl, S.pmass);
-            gVPCg += bb * nuc_sum(sb, 2, S.nat, S.cat, S.zan, true);
-            gVECg1 -= bb * nuc_sum(sb, 0, S.nat, S.cat, S.zan, true);
-            double v = coul(sb, 0, 2);
+            gTPg += bb * kin<NL>(sb, 2, 1, gl, S.pmass);
+            gVPCg += bb * nuc_sum<NL>(sb, 2, S.nat, S.cat, S.zan, true);
+            gVECg1 -= bb * nuc_sum<NL>(sb, 0, S.nat, S.cat, S.zan, true);
+            double v = coul<NL>(sb, 0, 2);
             underflow(v);
             gVEPg1 -= bb * v;
-            v = coul(sb, 0, 1);
+            v = coul<NL>(sb, 0, 1);
             underflow(v);
             gVEEg2 += bb * v;
-            gsg += bb * ovl(sb);
+            gsg += bb * ovl<NL>(sb);
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 3, e1, e2, p, 0, 0.0, gk + gl, 0); // g(e2,p)=gk+gl
             Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0); // g(e1,p)=gk+gl
-            gTEg3 += w * kin(s2, 0, -1, 0.0, 1.0);
-            gVECg2 -= w * nuc_sum(s2, 0, S.nat, S.cat, S.zan, true);
-            v = coul(s2, 0, 2);
+            gTEg3 += w * kin<NL>(s2, 0, -1, 0.0, 1.0);
+            gVECg2 -= w * nuc_sum<NL>(s2, 0, S.nat, S.cat, S.zan, true);
+            v = coul<NL>(s2, 0, 2);
             underflow(v);
             gVEPg2 -= w * v;
-            v = coul(s1, 0, 1);
+            v = coul<NL>(s1, 0, 1);
             underflow(v);
             gVEEg1 += w * v;
         }
@@ -815,21 +898,22 @@ NRX_HD void gam2_tuple(const SysParams &S, const Pair *q, double &ansE, double &
     ansS = gsg;
 }
 
-// xcalc_GAM3_MD (gam_3_OMP.f:617-983).  q = (e1, e2, e3, p).
-NRX_HD void gam3_tuple(const SysParams &S, const Pair *q, double &ans)
+// xcalc_GAM3_MD (gam_3_OMP.f:617-983).
+template <int NL>
+NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3, const PairRec *p,
+                       double &ans)
 {
-    const Pair *e1 = &q[0], *e2 = &q[1], *e3 = &q[2], *p = &q[3];
     // geminal-independent factors
     Sys s23 = mksys(S, 2, e2, e3, 0, 0, 0.0, 0, 0);
-    double vee23 = coul(s23, 0, 1);
+    double vee23 = coul<NL>(s23, 0, 1);
     underflow(vee23);
     Sys s3 = mksys(S, 1, e3, 0, 0, 0, 0, 0, 0);
-    double hcore = kin(s3, 0, -1, 0.0, 1.0) - nuc_sum(s3, 0, S.nat, S.cat, S.zan, false);
+    double hcore = kin<NL>(s3, 0, -1, 0.0, 1.0) - nuc_sum<NL>(s3, 0, S.nat, S.cat, S.zan, false);
     underflow(hcore);
     double gVEE = 0.0;
     for (int ik = 0; ik < S.ngtg; ++ik) {
         Sys s = mksys(S, 2, e1, p, 0, 0, S.gamma[ik], 0, 0);
-        double xx = ovl(s);
+        double xx = ovl<NL>(s);
         underflow(xx);
         gVEE += S.bcoef[ik] * (xx * vee23);
     }
@@ -839,22 +923,59 @@ NRX_HD void gam3_tuple(const SysParams &S, const Pair *q, double &ans)
             double gk = S.gamma[ik], gl = S.gamma[il];
             double bb = S.bcoef[ik] * S.bcoef[il];
             Sys s3p = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
-            double yy = ovl(s3p);
+            double yy = ovl<NL>(s3p);
             Sys s4a = mksys(S, 4, e1, e2, e3, p, gk, gl, 0.0); // 1/r34, g14 g24
-            double zz = coul(s4a, 2, 3);
+            double zz = coul<NL>(s4a, 2, 3);
             underflow(yy);
             underflow(zz);
             gHEg += bb * ((hcore * yy) + (zz * (-1.0)));
             Sys s4b = mksys(S, 4, e1, e2, e3, p, gk, 0.0, gl); // 1/r12, g14 g34
-            gVEEg2 += bb * coul(s4b, 0, 1);
+            gVEEg2 += bb * coul<NL>(s4b, 0, 1);
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 2, e1, p, 0, 0, gk + gl, 0, 0);
-            double xx = ovl(s2);
+            double xx = ovl<NL>(s2);
             underflow(xx);
             gVEEg1 += w * (xx * vee23);
         }
     ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
+}
+
+// Polynomial capacities per tuple class.  GEN covers every tuple allowed by
+// l <= 2 (per particle and direction bra+ket powers <= 4): e-p 4+4+2,
+// e-e-p 3*4+2, e-e-e-p 4*4.  SMALL covers tuples with tuple_degree <= SMALL-1,
+// which is almost all of them; the rest are evaluated by a second launch.
+template <int CLS> struct Tier;
+template <> struct Tier<1> {
+    static constexpr int SMALL = 11, GEN = 11;
+    static constexpr bool KIN = true;
+};
+template <> struct Tier<2> {
+    static constexpr int SMALL = 9, GEN = 15;
+    static constexpr bool KIN = true;
+};
+template <> struct Tier<3> {
+    static constexpr int SMALL = 9, GEN = 17;
+    static constexpr bool KIN = false;
+};
+constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
+
+// q[0..CLS-1] electron pairs, q[CLS] proton pair
+template <int CLS, int NL>
+NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s)
+{
+    s = 0.0;
+    if constexpr (CLS == 1)
+        gam1_tuple<NL>(S, q[0], q[1], e, s);
+    else if constexpr (CLS == 2)
+        gam2_tuple<NL>(S, q[0], q[1], q[2], e, s);
+    else
+        gam3_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
+}
+
+template <int CLS> NRX_HD bool tuple_small(const PairRec *const *q)
+{
+    return tuple_degree(q, CLS + 1, Tier<CLS>::KIN) <= Tier<CLS>::SMALL - 1;
 }
 
 } // namespace nrx

@@ -25,6 +25,7 @@
 #include "nrx_engine.h"
 
 typedef int64_t i64;
+#define NRX_DI __device__ inline __attribute__((always_inline))
 using namespace nrx;
 
 // ------------------------------------------------------------------
@@ -65,16 +66,18 @@ static int ensure_init()
 struct DevSys {
     i64 nebf, npebf, npbf, nat, ngtg;
     double pmass;
-    const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc, *elcex, *nucex, *elcbfc, *nucbfc;
-    const i64 *kpestr, *kpeend, *ampeb2c, *elcam, *nucam;
-    const i64 *eord, *pord; // enumeration order -> original primitive index
-    const double *bt;       // Boys table (BT_N x BT_M)
+    const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc;
+    const i64 *kpestr, *kpeend, *ampeb2c;
+    const i64 *eord, *pord;   // enumeration order -> original primitive index
+    const PairRec *ep, *pp;   // bra*ket tables [i][j] over primitives (npebf^2, npbf^2)
+    const double *bt;         // Boys table (BT_N x BT_M)
 };
 
 struct System {
     DevSys d;
     std::vector<void *> allocs;
     std::vector<i64> eord, pord;
+    std::vector<i64> kpestr; // host copy (1-based)
     int maxl;
 };
 
@@ -94,6 +97,7 @@ template <class T> static int upload(System *s, const T *h, size_t n, const T **
 
 static System *get_sys(i64 h)
 {
+    std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sys.find(h);
     return it == g_sys.end() ? nullptr : it->second;
 }
@@ -123,12 +127,13 @@ extern "C" int neorx_init(int64_t device)
 
 extern "C" int neorx_finalize(void)
 {
+    std::lock_guard<std::mutex> lk(g_mu);
     for (auto &kv : g_sys) {
-        for (void *p : kv.second->allocs) hipFree(p);
+        for (void *p : kv.second->allocs) (void)hipFree(p);
         delete kv.second;
     }
     g_sys.clear();
-    if (g_boys_dev) hipFree(g_boys_dev);
+    if (g_boys_dev) (void)hipFree(g_boys_dev);
     g_boys_dev = nullptr;
     g_device = -1;
     return NEORX_OK;
@@ -136,6 +141,24 @@ extern "C" int neorx_finalize(void)
 
 extern "C" const char *neorx_last_error(void) { return g_err.c_str(); }
 extern "C" const char *neorx_device_arch(void) { return g_arch; }
+
+static Prim host_prim(i64 n, const double *ex, const i64 *am, const double *cen, i64 i)
+{
+    Prim p;
+    p.a = ex[i];
+    for (int d = 0; d < 3; ++d) {
+        p.l[d] = (int)am[d * n + i];
+        p.A[d] = cen[d * n + i];
+    }
+    return p;
+}
+
+static void pair_table(i64 n, const double *ex, const i64 *am, const double *cen, std::vector<PairRec> &tab)
+{
+    tab.resize(n * n);
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) make_pair(host_prim(n, ex, am, cen, i), host_prim(n, ex, am, cen, j), tab[i * n + j]);
+}
 
 extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, int64_t nat, int64_t ngtg, double pmass,
                                    const double *cat, const double *zan, const double *bcoef, const double *gamma,
@@ -156,7 +179,11 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
             if (elcam[d * npebf + i] < 0) return fail(NEORX_E_ARG, "negative angular momentum");
         maxl = std::max(maxl, l);
     }
-    for (i64 i = 0; i < npbf; ++i) maxl = std::max(maxl, (int)(nucam[i] + nucam[npbf + i] + nucam[2 * npbf + i]));
+    for (i64 i = 0; i < npbf; ++i) {
+        for (int d = 0; d < 3; ++d)
+            if (nucam[d * npbf + i] < 0) return fail(NEORX_E_ARG, "negative angular momentum");
+        maxl = std::max(maxl, (int)(nucam[i] + nucam[npbf + i] + nucam[2 * npbf + i]));
+    }
     if (maxl > 2)
         return fail(NEORX_E_UNSUPPORTED, "angular momentum l=%d > 2 is not supported by this build", maxl);
     for (i64 c = 0; c < nebf; ++c)
@@ -164,9 +191,21 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
             return fail(NEORX_E_ARG, "KPESTR/KPEEND out of range for contracted function %lld", (long long)c + 1);
     for (i64 i = 0; i < npebf; ++i)
         if (ampeb2c[i] < 1 || ampeb2c[i] > nebf) return fail(NEORX_E_ARG, "AMPEB2C out of range");
+    // each contracted function owns a contiguous run of primitives, in order
+    // (the layout make_KPE derives from AMPEB2C)
+    for (i64 c = 0; c < nebf; ++c) {
+        i64 expect = c == 0 ? 1 : kpeend[c - 1] + 1;
+        if (kpestr[c] != expect) return fail(NEORX_E_ARG, "KPESTR(%lld) = %lld, expected %lld (primitives of contracted "
+                                             "functions must be contiguous and in order)", (long long)c + 1,
+                                             (long long)kpestr[c], (long long)expect);
+        for (i64 i = kpestr[c] - 1; i < kpeend[c]; ++i)
+            if (ampeb2c[i] != c + 1) return fail(NEORX_E_ARG, "AMPEB2C(%lld) disagrees with KPESTR/KPEEND", (long long)i + 1);
+    }
+    if (kpeend[nebf - 1] != npebf) return fail(NEORX_E_ARG, "KPEEND(NEBF) != NPEBF");
 
     System *s = new System();
     s->maxl = maxl;
+    s->kpestr.assign(kpestr, kpestr + nebf);
     DevSys &d = s->d;
     d.nebf = nebf;
     d.npebf = npebf;
@@ -189,6 +228,9 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
     };
     order(npebf, elcam, elcex, s->eord);
     order(npbf, nucam, nucex, s->pord);
+    std::vector<PairRec> ept, ppt;
+    pair_table(npebf, elcex, elcam, elcbfc, ept);
+    pair_table(npbf, nucex, nucam, nucbfc, ppt);
     int r = 0;
     r |= upload(s, cat, 3 * nat, &d.cat);
     r |= upload(s, zan, nat, &d.zan);
@@ -199,16 +241,12 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
     r |= upload(s, ampeb2c, npebf, &d.ampeb2c);
     r |= upload(s, agebfcc, npebf, &d.agebfcc);
     r |= upload(s, agnbfcc, npbf, &d.agnbfcc);
-    r |= upload(s, elcex, npebf, &d.elcex);
-    r |= upload(s, nucex, npbf, &d.nucex);
-    r |= upload(s, elcam, 3 * npebf, &d.elcam);
-    r |= upload(s, nucam, 3 * npbf, &d.nucam);
-    r |= upload(s, elcbfc, 3 * npebf, &d.elcbfc);
-    r |= upload(s, nucbfc, 3 * npbf, &d.nucbfc);
     r |= upload(s, s->eord.data(), npebf, &d.eord);
     r |= upload(s, s->pord.data(), npbf, &d.pord);
+    r |= upload(s, ept.data(), ept.size(), &d.ep);
+    r |= upload(s, ppt.data(), ppt.size(), &d.pp);
     if (r) {
-        for (void *p : s->allocs) hipFree(p);
+        for (void *p : s->allocs) (void)hipFree(p);
         delete s;
         return NEORX_E_HIP;
     }
@@ -224,7 +262,7 @@ extern "C" int neorx_system_destroy(int64_t handle)
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sys.find(handle);
     if (it == g_sys.end()) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    for (void *p : it->second->allocs) hipFree(p);
+    for (void *p : it->second->allocs) (void)hipFree(p);
     delete it->second;
     g_sys.erase(it);
     return NEORX_OK;
@@ -242,27 +280,7 @@ extern "C" int neorx_system_order(int64_t handle, int64_t *eorder, int64_t *pord
 // ------------------------------------------------------------------
 // device helpers
 // ------------------------------------------------------------------
-__device__ inline Prim d_eprim(const DevSys &S, i64 i)
-{
-    Prim p;
-    p.a = S.elcex[i];
-    for (int d = 0; d < 3; ++d) {
-        p.l[d] = (int)S.elcam[d * S.npebf + i];
-        p.A[d] = S.elcbfc[d * S.npebf + i];
-    }
-    return p;
-}
-__device__ inline Prim d_pprim(const DevSys &S, i64 i)
-{
-    Prim p;
-    p.a = S.nucex[i];
-    for (int d = 0; d < 3; ++d) {
-        p.l[d] = (int)S.nucam[d * S.npbf + i];
-        p.A[d] = S.nucbfc[d * S.npbf + i];
-    }
-    return p;
-}
-__device__ inline SysParams d_params(const DevSys &S)
+NRX_DI SysParams d_params(const DevSys &S)
 {
     SysParams P;
     P.nat = (int)S.nat;
@@ -276,77 +294,187 @@ __device__ inline SysParams d_params(const DevSys &S)
     return P;
 }
 
-// evaluate one tuple given original primitive indices (bra e..., ip, ket e..., jp)
-template <int CLS> __device__ inline void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
+// pair records of one tuple given original primitive indices
+// (bra e..., ip, ket e..., jp)
+template <int CLS> NRX_DI void tuple_pairs(const DevSys &S, const i64 *x, const PairRec **q)
 {
-    Pair q[CLS + 1];
-    for (int i = 0; i < CLS; ++i) make_pair(d_eprim(S, x[i]), d_eprim(S, x[CLS + 1 + i]), q[i]);
-    make_pair(d_pprim(S, x[CLS]), d_pprim(S, x[2 * CLS + 1]), q[CLS]);
-    SysParams P = d_params(S);
-    s = 0.0;
-    if constexpr (CLS == 1)
-        gam1_tuple(P, q, e, s);
-    else if constexpr (CLS == 2)
-        gam2_tuple(P, q, e, s);
-    else
-        gam3_tuple(P, q, e);
+    for (int i = 0; i < CLS; ++i) q[i] = S.ep + x[i] * S.npebf + x[CLS + 1 + i];
+    q[CLS] = S.pp + x[CLS] * S.npbf + x[2 * CLS + 1];
 }
 
-// minimum waves per SIMD requested from the register allocator
-#ifndef NRX_PRIM_WAVES
-#define NRX_PRIM_WAVES 1
-#endif
-
-template <int CLS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NRX_PRIM_WAVES))) k_prim_list(DevSys S, i64 ntup, const i64 *__restrict__ idx,
-                                                   double *__restrict__ outE, double *__restrict__ outS)
+template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntup) return;
-    i64 x[2 * CLS + 2];
+    const PairRec *q[CLS + 1];
+    tuple_pairs<CLS>(S, x, q);
+    tuple_eval<CLS, NL>(d_params(S), q, e, s);
+}
+
+template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
+{
+    const PairRec *q[CLS + 1];
+    tuple_pairs<CLS>(S, x, q);
+    return tuple_small<CLS>(q);
+}
+
+// index tuple t of a list, validated against the basis sizes
+template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t, i64 *x)
+{
     bool ok = true;
     for (int k = 0; k < 2 * CLS + 2; ++k) {
         x[k] = idx[t * (2 * CLS + 2) + k];
         i64 lim = (k == CLS || k == 2 * CLS + 1) ? S.npbf : S.npebf;
         ok = ok && x[k] >= 0 && x[k] < lim;
     }
-    if (!ok) { // out-of-range primitive index: flag the entry instead of reading outside the basis
-        outE[t] = __builtin_nan("");
-        if (outS) outS[t] = __builtin_nan("");
-        return;
-    }
-    double e, s;
-    eval_tuple<CLS>(S, x, e, s);
-    outE[t] = e;
-    if (outS) outS[t] = s;
+    return ok;
 }
 
-// enumeration (ip,jp,ie1,je1,...) in re-ordered primitive space
-template <int CLS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NRX_PRIM_WAVES))) k_prim_range(DevSys S, i64 start, i64 count, double *__restrict__ outE,
-                                                    double *__restrict__ outS)
+// tuple r of the enumeration (ip,jp,ie1,je1,...,ieN,jeN), last index
+// fastest, in the system's sorted primitive order (SORTED) or in the
+// caller's original order
+template <int CLS, bool SORTED> NRX_DI void range_tuple(const DevSys &S, i64 r, i64 *x)
+{
+    i64 ne = S.npebf, np = S.npbf;
+    for (int i = CLS - 1; i >= 0; --i) {
+        i64 k = r % ne;
+        x[CLS + 1 + i] = SORTED ? S.eord[k] : k;
+        r /= ne;
+        k = r % ne;
+        x[i] = SORTED ? S.eord[k] : k;
+        r /= ne;
+    }
+    i64 k = r % np;
+    x[2 * CLS + 1] = SORTED ? S.pord[k] : k;
+    r /= np;
+    k = r % np;
+    x[CLS] = SORTED ? S.pord[k] : k;
+}
+
+// Tuple sources: a caller's index list, or a range of the enumeration in
+// sorted or in original primitive order.
+enum { SRC_LIST = 0, SRC_SORTED = 1, SRC_ORIG = 2 };
+
+template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *idx, i64 start, i64 t, i64 *x)
+{
+    if constexpr (SRC == SRC_LIST)
+        load_tuple<CLS>(S, idx, t, x);
+    else
+        range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
+}
+
+// First pass over a list / range: tuples that fit the small capacity are
+// evaluated; the others are appended to ovf (their position t) for the
+// general-capacity pass.  With SMALL == GEN every tuple is evaluated here.
+template <int CLS, int SRC>
+__global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
+                                                    double *__restrict__ outE, double *__restrict__ outS,
+                                                    i64 *__restrict__ ovf, unsigned *__restrict__ novf)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    i64 r = start + t;
-    i64 ne = S.npebf, np = S.npbf;
+    if (t >= ntup) return;
     i64 x[2 * CLS + 2];
-    for (int i = CLS - 1; i >= 0; --i) {
-        x[CLS + 1 + i] = S.eord[r % ne];
-        r /= ne;
-        x[i] = S.eord[r % ne];
-        r /= ne;
+    if constexpr (SRC == SRC_LIST) {
+        if (!load_tuple<CLS>(S, idx, t, x)) { // out-of-range primitive index: flag the entry
+            outE[t] = __builtin_nan("");
+            if (outS) outS[t] = __builtin_nan("");
+            return;
+        }
+    } else {
+        range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
     }
-    x[2 * CLS + 1] = S.pord[r % np];
-    r /= np;
-    x[CLS] = S.pord[r % np];
+    constexpr int NS_ = Tier<CLS>::SMALL, NG = Tier<CLS>::GEN;
+    if constexpr (NS_ != NG) {
+        if (!small_tuple<CLS>(S, x)) {
+            unsigned k = atomicAdd(novf, 1u);
+            ovf[k] = t;
+            return;
+        }
+    }
     double e, s;
-    eval_tuple<CLS>(S, x, e, s);
+    eval_tuple<CLS, NS_>(S, x, e, s);
     outE[t] = e;
     if (outS) outS[t] = s;
 }
 
+// Second pass: the tuples listed by the first, general capacity, grid-stride
+// over the device-side count (every lane exits when the list is done).
+template <int CLS, int SRC>
+__global__ void __launch_bounds__(256) k_prim_gen(DevSys S, i64 start, const i64 *__restrict__ idx,
+                                                  double *__restrict__ outE, double *__restrict__ outS,
+                                                  const i64 *__restrict__ ovf, const unsigned *__restrict__ novf)
+{
+    i64 n = *novf;
+    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
+        i64 t = ovf[k];
+        i64 x[2 * CLS + 2];
+        src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by the first pass
+        double e, s;
+        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s);
+        outE[t] = e;
+        if (outS) outS[t] = s;
+    }
+}
+
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+
 static inline unsigned nblocks(i64 n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// tuples per launch of the primitive kernels (bounds the overflow list)
+static const i64 PRIM_CHUNK = (i64)1 << 24;
+static const unsigned GEN_BLOCKS = 2048;
+
+// stream-ordered scratch: the overflow list and its counter
+struct StreamBuf {
+    void *p = nullptr;
+    hipStream_t st = 0;
+    ~StreamBuf()
+    {
+        if (p) (void)hipFreeAsync(p, st);
+    }
+    int alloc(size_t bytes, hipStream_t s)
+    {
+        st = s;
+        hipError_t e = hipMallocAsync(&p, bytes, s);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(NEORX_E_NOMEM, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+        }
+        return NEORX_OK;
+    }
+};
+
+template <int CLS, int SRC>
+static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st)
+{
+    constexpr bool tiered = Tier<CLS>::SMALL != Tier<CLS>::GEN;
+    StreamBuf ws;
+    i64 *ovf = nullptr;
+    unsigned *cnt = nullptr;
+    if (tiered) {
+        i64 cap = std::min(ntup, PRIM_CHUNK);
+        int rc = ws.alloc(cap * sizeof(i64) + 256, st);
+        if (rc) return rc;
+        cnt = (unsigned *)ws.p;
+        ovf = (i64 *)((char *)ws.p + 256);
+    }
+    for (i64 off = 0; off < ntup; off += PRIM_CHUNK) {
+        i64 n = std::min(ntup - off, PRIM_CHUNK);
+        const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * CLS + 2) : nullptr;
+        double *e = oe + off, *s = os ? os + off : nullptr;
+        if (tiered) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, cnt);
+            HIPCHK(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_prim_small<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e,
+                           s, ovf, cnt);
+        HIPCHK(hipGetLastError());
+        if (tiered) {
+            hipLaunchKernelGGL((k_prim_gen<CLS, SRC>), dim3(GEN_BLOCKS), dim3(256), 0, st, d, start + off, ix, e, s,
+                               ovf, cnt);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return NEORX_OK;
+}
 
 extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
                                    double *d_ansS, void *stream)
@@ -357,18 +485,9 @@ extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, co
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
-    const int bs = 256;
-    // grid limit: 2^31-1 blocks per launch dimension
-    for (i64 off = 0; off < ntup; off += (i64)bs * 1048576) {
-        i64 n = std::min<i64>(ntup - off, (i64)bs * 1048576);
-        const i64 *ix = d_idx + off * (2 * cls + 2);
-        double *oe = d_ansE + off, *os = d_ansS ? d_ansS + off : nullptr;
-        if (cls == 1) hipLaunchKernelGGL(k_prim_list<1>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
-        if (cls == 2) hipLaunchKernelGGL(k_prim_list<2>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
-        if (cls == 3) hipLaunchKernelGGL(k_prim_list<3>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, n, ix, oe, os);
-        HIPCHK(hipGetLastError());
-    }
-    return NEORX_OK;
+    if (cls == 1) return launch_prims<1, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
+    if (cls == 2) return launch_prims<2, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
+    return launch_prims<3, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
 }
 
 extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, int64_t count, double *d_ansE,
@@ -386,23 +505,16 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
     if (count == 0) return NEORX_OK;
     if (!d_ansE) return fail(NEORX_E_ARG, "null output");
     hipStream_t st = (hipStream_t)stream;
-    const int bs = 256;
-    for (i64 off = 0; off < count; off += (i64)bs * 1048576) {
-        i64 n = std::min<i64>(count - off, (i64)bs * 1048576);
-        double *oe = d_ansE + off, *os = d_ansS ? d_ansS + off : nullptr;
-        if (cls == 1) hipLaunchKernelGGL(k_prim_range<1>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
-        if (cls == 2) hipLaunchKernelGGL(k_prim_range<2>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
-        if (cls == 3) hipLaunchKernelGGL(k_prim_range<3>, dim3(nblocks(n, bs)), dim3(bs), 0, st, s->d, start + off, n, oe, os);
-        HIPCHK(hipGetLastError());
-    }
-    return NEORX_OK;
+    if (cls == 1) return launch_prims<1, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
+    if (cls == 2) return launch_prims<2, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
+    return launch_prims<3, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
 }
 
 // ------------------------------------------------------------------
 // contracted builders
 // ------------------------------------------------------------------
 // GAM1_OMP_MD: one lane per contracted (ip,jp,iec,jec), primitive sums in
-// the reference's loop order.
+// the reference's loop order.  Every e-p tuple fits Tier<1>::GEN.
 __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, double *__restrict__ Gs)
 {
     i64 ne = S.nebf, np = S.npbf;
@@ -414,7 +526,7 @@ __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, 
         for (i64 je = S.kpestr[jc] - 1; je < S.kpeend[jc]; ++je) {
             i64 x[4] = {ie, ip, je, jp};
             double e, s;
-            eval_tuple<1>(S, x, e, s);
+            eval_tuple<1, Tier<1>::GEN>(S, x, e, s);
             double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[ie] * S.agebfcc[je];
             acc += e * cf;
             accs += s * cf;
@@ -423,34 +535,39 @@ __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, 
     Gs[t] = accs;
 }
 
-// contract_omega2_conv: lane per contracted tuple (ip,jp,i1,j1,i2,j2)
-__global__ void __launch_bounds__(256) k_gam2_contract(DevSys S, i64 off, i64 n, double *__restrict__ X,
-                                                       double *__restrict__ Xs)
+// Contracted tuples of GAM2_IC1 / GAM3_IC1 are built in two steps: the
+// primitive kernels above fill a slab of primitive values in original order,
+// then a contraction kernel sums each contracted tuple over its primitives
+// (contract_omega2_conv / contract_omega3_conv loop order).  A slab is a run
+// of consecutive leading indices L -- (ip,jp) for e-e-p, (ip,jp,i1) for
+// e-e-e-p -- whose primitive tuples are contiguous in the enumeration
+// because each contracted function owns a contiguous run of primitives.
+//
+// e-e-p: lane per contracted (L, i1, j1, i2, j2), L = ip*npbf + jp
+__global__ void __launch_bounds__(256) k_contract2(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
+                                                   const double *__restrict__ Ys, double *__restrict__ X,
+                                                   double *__restrict__ Xs)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    i64 r = off + t, ne = S.nebf, np = S.npbf;
-    i64 j2 = r % ne; r /= ne;
-    i64 i2 = r % ne; r /= ne;
-    i64 j1 = r % ne; r /= ne;
-    i64 i1 = r % ne; r /= ne;
-    i64 jp = r % np; r /= np;
-    i64 ip = r;
+    i64 ne = S.nebf, np = S.npbf, pe = S.npebf;
+    i64 ne4 = ne * ne * ne * ne, pe4 = pe * pe * pe * pe;
+    i64 L = L0 + t / ne4, rem = t % ne4;
+    i64 j2 = rem % ne, i2 = (rem / ne) % ne, j1 = (rem / (ne * ne)) % ne, i1 = rem / (ne * ne * ne);
+    i64 ip = L / np, jp = L % np;
+    double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
     double om = 0.0, oms = 0.0;
     for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
         for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
             for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
                 for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
-                    i64 x[6] = {a1, a2, ip, c1, c2, jp};
-                    double e, s;
-                    eval_tuple<2>(S, x, e, s);
-                    double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] *
-                                S.agebfcc[c2];
-                    om += e * cf;
-                    oms += s * cf;
+                    i64 r = L * pe4 + ((a1 * pe + c1) * pe + a2) * pe + c2 - R0;
+                    double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
+                    om += Y[r] * cf;
+                    oms += Ys[r] * cf;
                 }
-    X[off + t] = om;
-    Xs[off + t] = oms;
+    X[L * ne4 + rem] = om;
+    Xs[L * ne4 + rem] = oms;
 }
 
 // GM2 = (x12 + x21)/2  (gam2_IC1_OMP.f:141-177)
@@ -468,20 +585,19 @@ __global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 np, const double 
     Gs[t] = (Xs[t] + Xs[t21]) / 2.0;
 }
 
-// contract_omega3_conv: lane per contracted tuple
-__global__ void __launch_bounds__(256) k_gam3_contract(DevSys S, i64 off, i64 n, double *__restrict__ X)
+// e-e-e-p: lane per contracted (L, j1, i2, j2, i3, j3), L = (ip*npbf + jp)*nebf + i1
+__global__ void __launch_bounds__(256) k_contract3(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
+                                                   double *__restrict__ X)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    i64 r = off + t, ne = S.nebf, np = S.npbf;
-    i64 j3 = r % ne; r /= ne;
-    i64 i3 = r % ne; r /= ne;
-    i64 j2 = r % ne; r /= ne;
-    i64 i2 = r % ne; r /= ne;
-    i64 j1 = r % ne; r /= ne;
-    i64 i1 = r % ne; r /= ne;
-    i64 jp = r % np; r /= np;
-    i64 ip = r;
+    i64 ne = S.nebf, np = S.npbf, pe = S.npebf;
+    i64 ne5 = ne * ne * ne * ne * ne, pe5 = pe * pe * pe * pe * pe;
+    i64 L = L0 + t / ne5, rem = t % ne5;
+    i64 j3 = rem % ne, i3 = (rem / ne) % ne, j2 = (rem / (ne * ne)) % ne, i2 = (rem / (ne * ne * ne)) % ne,
+        j1 = rem / (ne * ne * ne * ne);
+    i64 i1 = L % ne, pp = L / ne, ip = pp / np, jp = pp % np;
+    double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
     double om = 0.0;
     for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
         for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
@@ -489,14 +605,12 @@ __global__ void __launch_bounds__(256) k_gam3_contract(DevSys S, i64 off, i64 n,
                 for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2)
                     for (i64 a3 = S.kpestr[i3] - 1; a3 < S.kpeend[i3]; ++a3)
                         for (i64 c3 = S.kpestr[j3] - 1; c3 < S.kpeend[j3]; ++c3) {
-                            i64 x[8] = {a1, a2, a3, ip, c1, c2, c3, jp};
-                            double e, s;
-                            eval_tuple<3>(S, x, e, s);
-                            double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[a1] * S.agebfcc[c1] *
-                                        S.agebfcc[a2] * S.agebfcc[c2] * S.agebfcc[a3] * S.agebfcc[c3];
-                            om += e * cf;
+                            i64 r = (pp * pe + a1) * pe5 + (((c1 * pe + a2) * pe + c2) * pe + a3) * pe + c3 - R0;
+                            double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2] *
+                                        S.agebfcc[a3] * S.agebfcc[c3];
+                            om += Y[r] * cf;
                         }
-    X[off + t] = om;
+    X[L * ne5 + rem] = om;
 }
 
 // six-permutation average (gam3_IC1_OMP.f:149-263)
@@ -524,16 +638,15 @@ __global__ void __launch_bounds__(256) k_gam_ee(DevSys S, double *__restrict__ G
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ne * ne * ne * ne) return;
     i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = t / (ne * ne * ne);
+    SysParams P = d_params(S);
     double acc = 0.0;
     for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
         for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
             for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
                 for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
-                    Pair q[2];
-                    make_pair(d_eprim(S, a1), d_eprim(S, c1), q[0]);
-                    make_pair(d_eprim(S, a2), d_eprim(S, c2), q[1]);
-                    Sys s = mksys(d_params(S), 2, &q[0], &q[1], 0, 0, 0.0, 0, 0);
-                    acc += coul(s, 0, 1) * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
+                    const PairRec *q0 = S.ep + a1 * S.npebf + c1, *q1 = S.ep + a2 * S.npebf + c2;
+                    Sys s = mksys(P, 2, q0, q1, 0, 0, 0.0, 0, 0);
+                    acc += coul<NL_EE>(s, 0, 1) * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
                 }
     G[t] = acc;
 }
@@ -585,7 +698,7 @@ struct DevBuf {
     void *p = nullptr;
     ~DevBuf()
     {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
     int alloc(size_t bytes)
     {
@@ -601,24 +714,62 @@ struct DevBuf {
 
 static const i64 CHUNK = (i64)256 * 65536; // lanes per launch of the contracted builders
 
+// primitive values held per slab (each of E and S); a slab holds at least
+// one leading index, whatever its size
+static const i64 SLAB_CAP = (i64)1 << 26;
+
+// first primitive tuple (original order) of leading index L of class cls;
+// L == number of leading indices gives the end of the enumeration
+static i64 slab_start(const System *s, int cls, i64 L)
+{
+    i64 pe = s->d.npebf, pe4 = pe * pe * pe * pe;
+    if (cls == 2) return L * pe4;
+    i64 ne = s->d.nebf, pp = L / ne, i1 = L % ne;
+    return (pp * pe + (s->kpestr[i1] - 1)) * pe4 * pe; // i1 == 0 at the end: pp * pe * pe^5
+}
+
+// consecutive slabs [L0, L1) covering nL leading indices
+static std::vector<std::pair<i64, i64>> make_slabs(const System *s, int cls, i64 nL, i64 &cap)
+{
+    std::vector<std::pair<i64, i64>> out;
+    cap = 0;
+    for (i64 L0 = 0; L0 < nL;) {
+        i64 L1 = L0 + 1;
+        while (L1 < nL && slab_start(s, cls, L1 + 1) - slab_start(s, cls, L0) <= SLAB_CAP) ++L1;
+        cap = std::max(cap, slab_start(s, cls, L1) - slab_start(s, cls, L0));
+        out.push_back({L0, L1});
+        L0 = L1;
+    }
+    return out;
+}
+
 extern "C" int neorx_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2s, void *stream)
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    i64 ne = s->d.nebf, np = s->d.npbf, ng2 = ne * ne * ne * ne * np * np;
+    if (!d_gm2 || !d_gm2s) return fail(NEORX_E_ARG, "null output");
+    i64 ne = s->d.nebf, np = s->d.npbf, ne4 = ne * ne * ne * ne, ng2 = ne4 * np * np;
     hipStream_t st = (hipStream_t)stream;
-    DevBuf X, Xs;
+    i64 cap = 0;
+    auto slabs = make_slabs(s, 2, np * np, cap);
+    DevBuf X, Xs, Y, Ys;
     int rc = X.alloc(ng2 * 8);
     if (!rc) rc = Xs.alloc(ng2 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
     if (rc) return rc;
-    for (i64 off = 0; off < ng2; off += CHUNK) {
-        i64 n = std::min(CHUNK, ng2 - off);
-        hipLaunchKernelGGL(k_gam2_contract, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, off, n, X.d(), Xs.d());
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 2, sl.first), R1 = slab_start(s, 2, sl.second);
+        rc = launch_prims<2, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * ne4;
+        hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(), Ys.d(),
+                           X.d(), Xs.d());
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, np, X.d(), Xs.d(), d_gm2, d_gm2s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st)); // X, Xs are released on return
+    HIPCHK(hipStreamSynchronize(st)); // scratch buffers are released on return
     return NEORX_OK;
 }
 
@@ -626,14 +777,21 @@ extern "C" int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3, void *stream)
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    if (!d_gm3) return fail(NEORX_E_ARG, "null output");
+    i64 ne = s->d.nebf, np = s->d.npbf, ne5 = ne * ne * ne * ne * ne, ng3 = ne5 * ne * np * np;
     hipStream_t st = (hipStream_t)stream;
-    DevBuf X;
+    i64 cap = 0;
+    auto slabs = make_slabs(s, 3, np * np * ne, cap);
+    DevBuf X, Y;
     int rc = X.alloc(ng3 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
     if (rc) return rc;
-    for (i64 off = 0; off < ng3; off += CHUNK) {
-        i64 n = std::min(CHUNK, ng3 - off);
-        hipLaunchKernelGGL(k_gam3_contract, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, off, n, X.d());
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 3, sl.first), R1 = slab_start(s, 3, sl.second);
+        rc = launch_prims<3, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), nullptr, st);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * ne5;
+        hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(), X.d());
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, np, X.d(), d_gm3);

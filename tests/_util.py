@@ -53,7 +53,9 @@ def engine_host():
     return _eng
 
 
-def engine_prim_list(s, cls, idx):
+def engine_prim_list(s, cls, idx, tier=0):
+    """Host engine values; tier 0 picks the capacity as the device does,
+    tier 1 forces the general capacity."""
     E = engine_host()
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     n = len(idx)
@@ -64,5 +66,5 @@ def engine_prim_list(s, cls, idx):
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
     E.eng_prim_list(C.c_int64(s["npebf"]), C.c_int64(s["npbf"]), *[P(a) for a in keep], C.c_int64(s["nat"]),
                     C.c_int64(s["ngtg"]), C.c_double(s["pmass"]), *[P(a) for a in sk], C.c_int64(cls), C.c_int64(n),
-                    P(idx), P(e), P(sv))
+                    P(idx), P(e), P(sv), C.c_int64(tier))
     return e, sv

@@ -31,36 +31,60 @@ static Prim eprim(i64 npebf, const double *ex, const i64 *am, const double *cen,
     return p;
 }
 
+// tier: 0 = choose as the device does (small capacity when the tuple fits),
+// 1 = always the general capacity
+template <int CLS> static void eval_one(const SysParams &S, const PairRec *const *q, int tier, double &e, double &s)
+{
+    if (tier == 0 && tuple_small<CLS>(q))
+        tuple_eval<CLS, Tier<CLS>::SMALL>(S, q, e, s);
+    else
+        tuple_eval<CLS, Tier<CLS>::GEN>(S, q, e, s);
+}
+
+static void pairs_of(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam, const i64 *nucam,
+                     const double *elcbfc, const double *nucbfc, int ne, const i64 *x, PairRec *q)
+{
+    for (int i = 0; i < ne; ++i)
+        make_pair(eprim(npebf, elcex, elcam, elcbfc, x[i]), eprim(npebf, elcex, elcam, elcbfc, x[ne + 1 + i]), q[i]);
+    make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
+}
+
+static void eval_tuple(const SysParams &S, int cls, const PairRec *q, int tier, double &e, double &s)
+{
+    const PairRec *qp[4] = {&q[0], &q[1], &q[2], &q[3]};
+    e = s = 0.0;
+    if (cls == 1) eval_one<1>(S, qp, tier, e, s);
+    else if (cls == 2) eval_one<2>(S, qp, tier, e, s);
+    else eval_one<3>(S, qp, tier, e, s);
+}
+
 extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam,
                               const i64 *nucam, const double *elcbfc, const double *nucbfc, i64 nat, i64 ngtg,
                               double pmass, const double *cat, const double *zan, const double *bcoef,
-                              const double *gamma, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
+                              const double *gamma, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS,
+                              i64 tier)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
     int ne = (int)cls, w = 2 * (ne + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
-        const i64 *x = idx + t * w;
-        Pair q[4];
-        for (int i = 0; i < ne; ++i)
-            make_pair(eprim(npebf, elcex, elcam, elcbfc, x[i]), eprim(npebf, elcex, elcam, elcbfc, x[ne + 1 + i]), q[i]);
-        make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
-        double e = 0, s = 0;
-        if (cls == 1) gam1_tuple(S, q, e, s);
-        else if (cls == 2) gam2_tuple(S, q, e, s);
-        else gam3_tuple(S, q, e);
+        PairRec q[4];
+        pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
+        double e, s;
+        eval_tuple(S, ne, q, (int)tier, e, s);
         outE[t] = e;
         if (outS) outS[t] = s;
     }
 }
 
 // Elementary integral in star topology (hub = last particle); op: 0 ovl,
-// 1 nuc, 2 coul, 3 kin; gam[i]: geminal electron i - hub (bra+ket); for kin,
-// gk is the ket geminal between a and partner c (c < 0: none).
+// 1 nuc (one centre C of unit charge), 2 coul, 3 kin; gam[i]: geminal
+// electron i - hub (bra+ket); for kin, gk is the ket geminal between a and
+// partner c (c < 0: none).
 extern "C" double eng_gint(i64 op, i64 n, const i64 *pw, const double *ex, const double *cen, const double *gam,
                            i64 a, i64 b, const double *C, double mass, i64 c, double gk, double pic)
 {
-    Pair q[4];
+    PairRec q[4];
     for (int i = 0; i < n; ++i) {
         Prim pb, pk;
         pb.a = ex[i];
@@ -75,14 +99,19 @@ extern "C" double eng_gint(i64 op, i64 n, const i64 *pw, const double *ex, const
     }
     SysParams SP{0, 0, 1.0, nullptr, nullptr, nullptr, nullptr, host_boys_table()};
     Sys s = mksys(SP, (int)n, &q[0], &q[1], &q[2], &q[3], gam[0], gam[1], gam[2]);
-    if (op == 0) return ovl(s, pic);
-    if (op == 1) return nuc(s, (int)a, C, pic);
-    if (op == 2) return coul(s, (int)a, (int)b, pic);
-    return kin(s, (int)a, (int)c, gk, mass, pic);
+    constexpr int NL = 17;
+    if (op == 0) return ovl<NL>(s, pic);
+    if (op == 1) {
+        double one = 1.0;
+        return nuc_sum<NL>(s, (int)a, 1, C, &one, false, pic);
+    }
+    if (op == 2) return coul<NL>(s, (int)a, (int)b, pic);
+    return kin<NL>(s, (int)a, (int)c, gk, mass, pic);
 }
 
 #ifdef NRX_COUNT_FLOPS
-// average floating-point operations per tuple of the engine's algorithm
+// average floating-point operations per tuple of the engine's algorithm, in
+// the capacity tier the device uses for each tuple
 extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam,
                                       const i64 *nucam, const double *elcbfc, const double *nucbfc, i64 nat, i64 ngtg,
                                       double pmass, const double *cat, const double *zan, const double *bcoef,
@@ -92,15 +121,10 @@ extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, 
     int ne = (int)cls, w = 2 * (ne + 1);
     nrx_flops = 0.0;
     for (i64 t = 0; t < ntup; ++t) {
-        const i64 *x = idx + t * w;
-        Pair q[4];
-        for (int i = 0; i < ne; ++i)
-            make_pair(eprim(npebf, elcex, elcam, elcbfc, x[i]), eprim(npebf, elcex, elcam, elcbfc, x[ne + 1 + i]), q[i]);
-        make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
-        double e = 0, s = 0;
-        if (cls == 1) gam1_tuple(S, q, e, s);
-        else if (cls == 2) gam2_tuple(S, q, e, s);
-        else gam3_tuple(S, q, e);
+        PairRec q[4];
+        pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
+        double e, s;
+        eval_tuple(S, ne, q, 0, e, s);
     }
     return nrx_flops / (double)ntup;
 }
