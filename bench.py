@@ -166,8 +166,10 @@ def main():
         ntup = n1 if dom == 1 else ns
         ach = flops_per * ntup / (np.mean(per_class_ms[dom]) / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / FP64_PEAK_TFLOPS, 5), "traffic": TRAFFIC.get(dom),
-                "kernel": {1: "k_prim_range<1>", 2: "k_prim_list<2>", 3: "k_prim_list<3>"}[dom],
+                "frac": round(ach / FP64_PEAK_TFLOPS, 5),
+                "traffic": TRAFFIC.get(dom) if args.nsample == (1 << 20) else None,
+                "kernel": {1: "k_prim_small<1,SRC_SORTED>", 2: "k_prim_small<2,SRC_LIST> + k_prim_gen<2,SRC_LIST>",
+                           3: "k_prim_small<3,SRC_LIST> + k_prim_gen<3,SRC_LIST>"}[dom],
                 "flops_per_tuple": flops_per}
 
     cpu = None
@@ -195,11 +197,12 @@ def main():
 
 
 # Algorithmic FP64 operations per primitive tuple of this workload, counted by
-# the host build of the engine (tests/count_flops.py); traffic = HBM bytes
-# per launch of the dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# (profiles/), scaled by the copy-kernel calibration.
-FLOPS_PER_TUPLE = {1: 93477.0, 2: 198357.0, 3: 42087.0}
-TRAFFIC = {}
+# the host build of the engine over the coefficients the operands' degrees
+# make nonzero (tests/count_flops.py); traffic = HBM bytes per launch of the
+# dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), scaled
+# by the copy-kernel calibration.
+FLOPS_PER_TUPLE = {1: 99762.1, 2: 251422.5, 3: 46648.4}
+TRAFFIC = {2: 131.9e6}  # e-e-p launch group at the default 1,048,576 sampled tuples
 
 if __name__ == "__main__":
     main()

@@ -69,8 +69,9 @@
 #endif
 
 // Host builds used for accounting may define NRX_COUNT_FLOPS to tally the
-// floating-point operations the algorithm performs (FMA = 2).  The counts are
-// those of the fixed-capacity loops actually executed.
+// floating-point operations the algorithm needs (FMA = 2): each polynomial
+// operation is counted over the coefficients its operands' degrees make
+// nonzero, not over the zero-padded capacity the kernels execute.
 #if defined(NRX_COUNT_FLOPS) && !defined(__HIP_DEVICE_COMPILE__)
 extern thread_local double nrx_flops;
 #define NRX_CNT(x) (nrx_flops += (double)(x))
@@ -133,7 +134,7 @@ template <int NA, int NBB, int NR> NRX_HD void p_mul(const PolyT<NA> &a, const P
     int n = imin(a.n + b.n - 1, NR);
     NRX_UNROLL for (int k = 0; k < NR; ++k) r.c[k] = t[k];
     r.n = n;
-    NRX_CNT(2 * NA * NBB);
+    NRX_CNT(2 * a.n * b.n);
 }
 
 // a *= (y + s) when on, unchanged otherwise (branch-free)
@@ -143,8 +144,8 @@ template <int N> NRX_HD void p_mul_lin(PolyT<N> &a, double s, bool on = true)
     double t[N];
     NRX_UNROLL for (int k = 0; k < N; ++k) t[k] = ((k > 0) ? g * a.c[k - 1] : 0.0) + sl * a.c[k];
     NRX_UNROLL for (int k = 0; k < N; ++k) a.c[k] = t[k];
+    NRX_CNT(on ? 2 * (a.n + 1) : 0);
     a.n = imin(a.n + (on ? 1 : 0), N);
-    NRX_CNT(3 * N);
 }
 
 // (y + sa)^la (y + sb)^lb with la, lb <= 2 (l <= 2 per Cartesian component
@@ -170,14 +171,14 @@ template <int N> NRX_HD void p_smooth(const PolyT<N> &f, double var, PolyT<N> &h
     }
     NRX_UNROLL for (int j = 0; j < N; ++j) h.c[j] = t[j];
     h.n = f.n;
-    NRX_CNT(N * N / 2 + N);
+    NRX_CNT(f.n * f.n / 2 + f.n);
 }
 
 template <int N> NRX_HD double p_eval(const PolyT<N> &h, double s)
 {
     double v = 0.0;
     NRX_UNROLL for (int k = N - 1; k >= 0; --k) v = v * s + h.c[k];
-    NRX_CNT(2 * N);
+    NRX_CNT(2 * h.n);
     return v;
 }
 
@@ -196,7 +197,7 @@ template <int N> NRX_HD void p_compose(const PolyT<N> &h, double m0, double beta
     }
     NRX_UNROLL for (int j = 0; j < N; ++j) g.c[j] = t[j];
     g.n = h.n;
-    NRX_CNT(N * N + 3 * N);
+    NRX_CNT(h.n * h.n + 3 * h.n);
 }
 
 // Hermite-Gaussian expansion: f(s) exp(-w s^2) = sum_t e_t Lambda_t(s),
@@ -220,7 +221,7 @@ template <int N> NRX_HD void p_hermite(const PolyT<N> &f, double w, PolyT<N> &e)
     }
     NRX_UNROLL for (int t = 0; t < N; ++t) e.c[t] = cur[t];
     e.n = f.n;
-    NRX_CNT(4 * N * N);
+    NRX_CNT(2 * f.n * f.n);
 }
 
 // ------------------------------------------------------------------
@@ -278,7 +279,7 @@ NRX_HD double boys_dot(const PolyT<NB> &c, double T, const double *tab)
             s += c.c[k] * f; // c.c[k] = 0 for k > M
         }
     }
-    NRX_CNT(4 * NB + 4 * BT_K + 10);
+    NRX_CNT(5 * c.n + 2 * BT_K + 10);
     return s;
 }
 
@@ -552,7 +553,7 @@ template <int N> NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, doubl
         }
     }
     P.n = hk.n;
-    NRX_CNT(5 * N * N);
+    NRX_CNT(4 * hk.n * hk.n);
 }
 
 NRX_HD void underflow(double &x)
@@ -632,7 +633,7 @@ template <int N> NRX_HD void conv_add(const PolyT<N> &ea, const PolyT<N> &eb, Po
     }
     int n = imin(ea.n + eb.n - 1, N);
     hk.n = hk.n > n ? hk.n : n;
-    NRX_CNT(N * N);
+    NRX_CNT(2 * ea.n * eb.n);
 }
 
 // Coulomb <1/|r_a - r_b|>, a < b
