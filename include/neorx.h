@@ -111,6 +111,32 @@ int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const doubl
 int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm3,
                      double *d_fe, double *d_fp, double *d_e, void *stream);
 
+/* ---- direct-SCF routines (reference semantics) -------------------------- */
+/* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per
+ * call (every primitive value through underflow(), as contract_omega2 does),
+ * symmetrised and contracted like EG2IC1; focke, fockp, se2, sp2 accumulate
+ * (add2fock), E_gam2, S_gam2 overwritten.  The handle supplies the basis and
+ * system arrays the reference routine receives. */
+int neorx_gam2_dscf(int64_t handle, const double *de, const double *dp, double *focke, double *fockp, double *se2,
+                    double *sp2, double *e_gam2, double *s_gam2);
+/* GAM3_DSCF, source/gam3_DSCF_OMP.f:2: focke, fockp accumulate; E overwritten */
+int neorx_gam3_dscf(int64_t handle, const double *de, const double *dp, double *focke, double *fockp,
+                    double *e_gam3);
+/* GAM4_DSCF, source/gam4_DSCF_OMP.f:2: the 5-particle values are formed per
+ * entry from GAM_2s and GAM_ee (passed in; the reference reads GAM_2s.ufm and
+ * GAM_ee.ufm) and contracted as EG4ICR does; no ng4 tensor is stored. */
+int neorx_gam4_dscf(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, const double *de,
+                    const double *dp, double *focke, double *fockp, double *e_gam4);
+
+/* Device-resident direct-SCF routines (all arrays device memory; d_e, d_s one
+ * double each; synchronise the stream before returning). */
+int neorx_gam2_dscf_dev(int64_t handle, const double *d_de, const double *d_dp, double *d_fe, double *d_fp,
+                        double *d_se2, double *d_sp2, double *d_e, double *d_s, void *stream);
+int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const double *d_dp, double *d_fe, double *d_fp,
+                        double *d_e, void *stream);
+int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_gam2s, const double *d_gamee, const double *d_de,
+                        const double *d_dp, double *d_fe, double *d_fp, double *d_e, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

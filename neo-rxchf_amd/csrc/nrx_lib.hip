@@ -544,9 +544,11 @@ __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, 
 // because each contracted function owns a contiguous run of primitives.
 //
 // e-e-p: lane per contracted (L, i1, j1, i2, j2), L = ip*npbf + jp
+// X holds leading indices from Xb on; with flush set every primitive value
+// goes through underflow() first (contract_omega2, gam2_DSCF_OMP.f:514-515)
 __global__ void __launch_bounds__(256) k_contract2(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
                                                    const double *__restrict__ Ys, double *__restrict__ X,
-                                                   double *__restrict__ Xs)
+                                                   double *__restrict__ Xs, i64 Xb, int flush)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -563,20 +565,25 @@ __global__ void __launch_bounds__(256) k_contract2(DevSys S, i64 L0, i64 n, i64 
                 for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
                     i64 r = L * pe4 + ((a1 * pe + c1) * pe + a2) * pe + c2 - R0;
                     double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
-                    om += Y[r] * cf;
-                    oms += Ys[r] * cf;
+                    double y = Y[r], ys = Ys[r];
+                    if (flush) {
+                        underflow(y);
+                        underflow(ys);
+                    }
+                    om += y * cf;
+                    oms += ys * cf;
                 }
-    X[L * ne4 + rem] = om;
-    Xs[L * ne4 + rem] = oms;
+    X[(L - Xb) * ne4 + rem] = om;
+    Xs[(L - Xb) * ne4 + rem] = oms;
 }
 
 // GM2 = (x12 + x21)/2  (gam2_IC1_OMP.f:141-177)
-__global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 np, const double *__restrict__ X,
+// over n entries of whole (ip,jp) blocks
+__global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 n, const double *__restrict__ X,
                                                    const double *__restrict__ Xs, double *__restrict__ G,
                                                    double *__restrict__ Gs)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    i64 n = ne * ne * ne * ne * np * np;
     if (t >= n) return;
     i64 ne2 = ne * ne;
     i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
@@ -587,7 +594,7 @@ __global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 np, const double 
 
 // e-e-e-p: lane per contracted (L, j1, i2, j2, i3, j3), L = (ip*npbf + jp)*nebf + i1
 __global__ void __launch_bounds__(256) k_contract3(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
-                                                   double *__restrict__ X)
+                                                   double *__restrict__ X, i64 Xb)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -610,16 +617,17 @@ __global__ void __launch_bounds__(256) k_contract3(DevSys S, i64 L0, i64 n, i64 
                                         S.agebfcc[a3] * S.agebfcc[c3];
                             om += Y[r] * cf;
                         }
-    X[L * ne5 + rem] = om;
+    X[(L - Xb) * ne5 + rem] = om;
 }
 
 // six-permutation average (gam3_IC1_OMP.f:149-263)
-__global__ void __launch_bounds__(256) k_gam3_symm(i64 ne, i64 np, const double *__restrict__ X,
+// over n entries of whole (ip,jp) blocks
+__global__ void __launch_bounds__(256) k_gam3_symm(i64 ne, i64 n, const double *__restrict__ X,
                                                    double *__restrict__ G)
 {
     i64 ne2 = ne * ne, blk = ne2 * ne2 * ne2;
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= blk * np * np) return;
+    if (t >= n) return;
     i64 b = t / blk, r = t % blk;
     i64 p3 = r % ne2, p2 = (r / ne2) % ne2, p1 = r / (ne2 * ne2);
     const double *Xb = X + b * blk;
@@ -658,18 +666,10 @@ __constant__ int c_perm4[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0,
                                    {3, 2, 1, 0}, {2, 3, 1, 0}, {1, 3, 2, 0}, {3, 1, 2, 0}, {2, 1, 3, 0}, {1, 2, 3, 0}};
 __constant__ double c_w4[24] = {16, -8, 4, -8, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2};
 
-__global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, const double *__restrict__ G2s,
-                                              const double *__restrict__ Gee, double *__restrict__ G4)
+// symm_gam4/16 of entry (ip,jp, br[k] bra / kt[k] ket of electron k)
+NRX_DI double gam4_value(i64 ne, i64 np, const double *__restrict__ G2s, const double *__restrict__ Gee, i64 ip,
+                         i64 jp, const i64 *br, const i64 *kt)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 r = off + t;
-    i64 br[4], kt[4];
-    for (int k = 3; k >= 0; --k) {
-        kt[k] = r % ne; r /= ne;
-        br[k] = r % ne; r /= ne;
-    }
-    i64 jp = r % np, ip = r / np;
     i64 ne2 = ne * ne;
     i64 gb = (ip * np + jp) * ne2 * ne2;
     double ans = 0.0;
@@ -688,7 +688,22 @@ __global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, co
                    6.0;
         ans += c_w4[p] * x;
     }
-    G4[off + t] = ans / 16.0;
+    return ans / 16.0;
+}
+
+__global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, const double *__restrict__ G2s,
+                                              const double *__restrict__ Gee, double *__restrict__ G4)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t;
+    i64 br[4], kt[4];
+    for (int k = 3; k >= 0; --k) {
+        kt[k] = r % ne; r /= ne;
+        br[k] = r % ne; r /= ne;
+    }
+    i64 jp = r % np, ip = r / np;
+    G4[off + t] = gam4_value(ne, np, G2s, Gee, ip, jp, br, kt);
 }
 
 // ------------------------------------------------------------------
@@ -717,25 +732,28 @@ static const i64 CHUNK = (i64)256 * 65536; // lanes per launch of the contracted
 // primitive values held per slab (each of E and S); a slab holds at least
 // one leading index, whatever its size
 static const i64 SLAB_CAP = (i64)1 << 26;
+// contracted values held per block of (ip,jp) pairs by the direct-SCF routines
+static const i64 BLOCK_CAP = (i64)1 << 27;
 
-// first primitive tuple (original order) of leading index L of class cls;
-// L == number of leading indices gives the end of the enumeration
+// first primitive tuple (original order) of leading index L of class cls
+// (e-e-p: L = ip*npbf + jp; e-e-e-p: L = (ip*npbf + jp)*nebf + i1); the
+// number of leading indices gives the end of the enumeration
 static i64 slab_start(const System *s, int cls, i64 L)
 {
     i64 pe = s->d.npebf, pe4 = pe * pe * pe * pe;
     if (cls == 2) return L * pe4;
     i64 ne = s->d.nebf, pp = L / ne, i1 = L % ne;
-    return (pp * pe + (s->kpestr[i1] - 1)) * pe4 * pe; // i1 == 0 at the end: pp * pe * pe^5
+    return (pp * pe + (s->kpestr[i1] - 1)) * pe4 * pe; // i1 == 0 past the end: pp * pe^6
 }
 
-// consecutive slabs [L0, L1) covering nL leading indices
-static std::vector<std::pair<i64, i64>> make_slabs(const System *s, int cls, i64 nL, i64 &cap)
+// consecutive slabs [L0, L1) covering [La, Lb); cap = largest primitive span
+static std::vector<std::pair<i64, i64>> make_slabs(const System *s, int cls, i64 La, i64 Lb, i64 &cap)
 {
     std::vector<std::pair<i64, i64>> out;
     cap = 0;
-    for (i64 L0 = 0; L0 < nL;) {
+    for (i64 L0 = La; L0 < Lb;) {
         i64 L1 = L0 + 1;
-        while (L1 < nL && slab_start(s, cls, L1 + 1) - slab_start(s, cls, L0) <= SLAB_CAP) ++L1;
+        while (L1 < Lb && slab_start(s, cls, L1 + 1) - slab_start(s, cls, L0) <= SLAB_CAP) ++L1;
         cap = std::max(cap, slab_start(s, cls, L1) - slab_start(s, cls, L0));
         out.push_back({L0, L1});
         L0 = L1;
@@ -743,31 +761,65 @@ static std::vector<std::pair<i64, i64>> make_slabs(const System *s, int cls, i64
     return out;
 }
 
+// largest primitive span of any slab over all leading indices of cls
+static i64 slab_cap(const System *s, int cls)
+{
+    i64 cap = 0;
+    i64 nL = s->d.npbf * s->d.npbf * (cls == 3 ? s->d.nebf : 1);
+    make_slabs(s, cls, 0, nL, cap);
+    return cap;
+}
+
+// Unsymmetrised contracted e-e-p values of (ip,jp) pairs [p0, p1) into X, Xs
+// (block-local, ne^4 per pair), through primitive slabs in Y, Ys.
+static int build2_block(System *s, i64 p0, i64 p1, double *X, double *Xs, double *Y, double *Ys, int flush,
+                        hipStream_t st)
+{
+    i64 ne = s->d.nebf, ne4 = ne * ne * ne * ne, cap;
+    for (auto &sl : make_slabs(s, 2, p0, p1, cap)) {
+        i64 R0 = slab_start(s, 2, sl.first), R1 = slab_start(s, 2, sl.second);
+        int rc = launch_prims<2, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, Ys, st);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * ne4;
+        hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y, Ys, X, Xs,
+                           p0, flush);
+        HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
+// Unsymmetrised contracted e-e-e-p values of (ip,jp) pairs [p0, p1) into X
+// (block-local, ne^6 per pair).
+static int build3_block(System *s, i64 p0, i64 p1, double *X, double *Y, hipStream_t st)
+{
+    i64 ne = s->d.nebf, ne5 = ne * ne * ne * ne * ne, cap;
+    for (auto &sl : make_slabs(s, 3, p0 * ne, p1 * ne, cap)) {
+        i64 R0 = slab_start(s, 3, sl.first), R1 = slab_start(s, 3, sl.second);
+        int rc = launch_prims<3, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, nullptr, st);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * ne5;
+        hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y, X, p0 * ne);
+        HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
 extern "C" int neorx_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2s, void *stream)
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
     if (!d_gm2 || !d_gm2s) return fail(NEORX_E_ARG, "null output");
-    i64 ne = s->d.nebf, np = s->d.npbf, ne4 = ne * ne * ne * ne, ng2 = ne4 * np * np;
+    i64 ne = s->d.nebf, np = s->d.npbf, ng2 = ne * ne * ne * ne * np * np;
     hipStream_t st = (hipStream_t)stream;
-    i64 cap = 0;
-    auto slabs = make_slabs(s, 2, np * np, cap);
+    i64 cap = slab_cap(s, 2);
     DevBuf X, Xs, Y, Ys;
     int rc = X.alloc(ng2 * 8);
     if (!rc) rc = Xs.alloc(ng2 * 8);
     if (!rc) rc = Y.alloc(cap * 8);
     if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = build2_block(s, 0, np * np, X.d(), Xs.d(), Y.d(), Ys.d(), 0, st);
     if (rc) return rc;
-    for (auto &sl : slabs) {
-        i64 R0 = slab_start(s, 2, sl.first), R1 = slab_start(s, 2, sl.second);
-        rc = launch_prims<2, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st);
-        if (rc) return rc;
-        i64 n = (sl.second - sl.first) * ne4;
-        hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(), Ys.d(),
-                           X.d(), Xs.d());
-        HIPCHK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, np, X.d(), Xs.d(), d_gm2, d_gm2s);
+    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st)); // scratch buffers are released on return
     return NEORX_OK;
@@ -778,23 +830,15 @@ extern "C" int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3, void *stream)
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
     if (!d_gm3) return fail(NEORX_E_ARG, "null output");
-    i64 ne = s->d.nebf, np = s->d.npbf, ne5 = ne * ne * ne * ne * ne, ng3 = ne5 * ne * np * np;
+    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
     hipStream_t st = (hipStream_t)stream;
-    i64 cap = 0;
-    auto slabs = make_slabs(s, 3, np * np * ne, cap);
+    i64 cap = slab_cap(s, 3);
     DevBuf X, Y;
     int rc = X.alloc(ng3 * 8);
     if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = build3_block(s, 0, np * np, X.d(), Y.d(), st);
     if (rc) return rc;
-    for (auto &sl : slabs) {
-        i64 R0 = slab_start(s, 3, sl.first), R1 = slab_start(s, 3, sl.second);
-        rc = launch_prims<3, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), nullptr, st);
-        if (rc) return rc;
-        i64 n = (sl.second - sl.first) * ne5;
-        hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(), X.d());
-        HIPCHK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, np, X.d(), d_gm3);
+    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, ng3, X.d(), d_gm3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
@@ -940,10 +984,12 @@ __global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, i64 np, const double *_
 
 // reduce partials: FE(a,b) += 2 sum_{ip,jp} DP(ip,jp) [dir_a[b] - ex_a[b]/2] etc.
 __global__ void k_eg2_reduce(i64 ne, i64 np, const double *__restrict__ DE, const double *__restrict__ DP,
-                             const double *__restrict__ part, int withS, double *__restrict__ FE,
+                             const double *__restrict__ part, int withS, int sacc, double *__restrict__ FE,
                              double *__restrict__ SE, double *__restrict__ FP, double *__restrict__ SP,
                              double *__restrict__ E, double *__restrict__ S)
 {
+    // SE is overwritten (EG2IC1 zeroes it first) or, with sacc, accumulated
+    // (GAM2_DSCF's add2fock)
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     i64 ne2 = ne * ne, np2 = np * np;
     if (t < ne2) {
@@ -958,7 +1004,7 @@ __global__ void k_eg2_reduce(i64 ne, i64 np, const double *__restrict__ DE, cons
             fs += dp * (pr[2 * ne + b] - 0.5 * pr[3 * ne + b]);
         }
         FE[b * ne + a] += f;
-        if (withS) SE[b * ne + a] = fs;
+        if (withS) SE[b * ne + a] = sacc ? SE[b * ne + a] + fs : fs;
     } else if (t < ne2 + np2) {
         i64 pp = t - ne2, ip = pp / np, jp = pp % np;
         double f = 0.0, fs = 0.0;
@@ -1019,7 +1065,7 @@ extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, 
     int withS = d_gm2s != nullptr;
     // FE/SE and the per-(ip,jp) proton terms into fpd/fpsd; then add fpd to FP
     hipLaunchKernelGGL(k_eg2_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_de, d_dp,
-                       part.d(), withS, d_fe, d_se, fpd.d(), fpsd.d(), d_e, d_s);
+                       part.d(), withS, 0, d_fe, d_se, fpd.d(), fpsd.d(), d_e, d_s);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), withS ? fpsd.d() : nullptr, d_e,
                        withS ? d_s : nullptr);
@@ -1196,22 +1242,32 @@ extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const 
 
 // EG4ICR: lane per entry, direct port of the reduction structure
 // (FE row per slab (ip,jp,i1) in LDS).
-template <int NT>
+template <int NT, bool ONFLY>
 __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *__restrict__ DE,
-                                                 const double *__restrict__ G, double *__restrict__ part)
+                                                 const double *__restrict__ G, const double *__restrict__ G2s,
+                                                 const double *__restrict__ Gee, double *__restrict__ part)
 {
+    // block (ip, jp, i1): ONFLY evaluates symm_gam4/16 per entry from GAM_2s
+    // and GAM_ee (thread_gam4_DSCF, gam4_DSCF_OMP.f:322-420) instead of
+    // reading the in-core tensor
     extern __shared__ double sm[];
-    i64 blk = blockIdx.x, a = blk % ne;
+    i64 blk = blockIdx.x, a = blk % ne, pp = blk / ne, ip = pp / np, jp = pp % np;
     i64 ne7 = ne * ne * ne * ne * ne * ne * ne;
-    const double *g = G + blk * ne7;
+    const double *g = ONFLY ? nullptr : G + blk * ne7;
     for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
     __syncthreads();
     double pacc = 0.0;
     for (i64 r = threadIdx.x; r < ne7; r += NT) {
-        double gv = g[r];
         i64 j4 = r % ne, i4 = (r / ne) % ne, j3 = (r / (ne * ne)) % ne, i3 = (r / (ne * ne * ne)) % ne;
         i64 j2 = (r / (ne * ne * ne * ne)) % ne, i2 = (r / (ne * ne * ne * ne * ne)) % ne,
             j1 = r / (ne * ne * ne * ne * ne * ne);
+        double gv;
+        if constexpr (ONFLY) {
+            i64 br[4] = {a, i2, i3, i4}, kt[4] = {j1, j2, j3, j4};
+            gv = gam4_value(ne, np, G2s, Gee, ip, jp, br, kt);
+        } else {
+            gv = g[r];
+        }
         double ddd = DE[j2 * ne + i2] * DE[j3 * ne + i3] * DE[j4 * ne + i4];
         atomicAdd(&sm[j1], 4.0 * ddd * gv);
         pacc += DE[j1 * ne + a] * ddd * gv;
@@ -1243,8 +1299,8 @@ extern "C" int neorx_eg4icr(int64_t nebf, int64_t npbf, const double *de, const 
     HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dG.p, gm4icr, ng4 * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dFE.p, focke, ne * ne * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_eg4_slab<256>, dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, 0, ne, np, dDE.d(), dG.d(),
-                       part.d());
+    hipLaunchKernelGGL((k_eg4_slab<256, false>), dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, 0, ne, np, dDE.d(),
+                       dG.d(), nullptr, nullptr, part.d());
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, 0, ne, np, dDP.d(), part.d(),
                        dFE.d(), fpd.d());
@@ -1256,6 +1312,193 @@ extern "C" int neorx_eg4icr(int64_t nebf, int64_t npbf, const double *de, const 
     HIPCHK(hipMemcpy(h.data(), fpd.p, np * np * 8, hipMemcpyDeviceToHost));
     for (i64 k = 0; k < np * np; ++k) fockp[k] += h[k];
     HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e_gam4, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// direct-SCF routines: contracted values are rebuilt per call, one block of
+// (ip,jp) pairs at a time, symmetrised and contracted into the Fock partials
+// of that block; no ng2/ng3/ng4 tensor is stored
+// ------------------------------------------------------------------
+extern "C" int neorx_gam2_dscf_dev(int64_t handle, const double *d_de, const double *d_dp, double *d_fe,
+                                   double *d_fp, double *d_se2, double *d_sp2, double *d_e, double *d_s, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (!d_de || !d_dp || !d_fe || !d_fp || !d_se2 || !d_sp2 || !d_e || !d_s)
+        return fail(NEORX_E_ARG, "null argument");
+    i64 ne = s->d.nebf, np = s->d.npbf, ne4 = ne * ne * ne * ne, npp = np * np;
+    hipStream_t st = (hipStream_t)stream;
+    i64 nb = std::max<i64>(1, std::min<i64>(npp, BLOCK_CAP / ne4));
+    DevBuf X, Xs, G, Gs, Y, Ys, part, fpd, fpsd;
+    i64 cap = slab_cap(s, 2);
+    int rc = X.alloc(nb * ne4 * 8);
+    if (!rc) rc = Xs.alloc(nb * ne4 * 8);
+    if (!rc) rc = G.alloc(nb * ne4 * 8);
+    if (!rc) rc = Gs.alloc(nb * ne4 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = part.alloc(npp * ne * 4 * ne * 8);
+    if (!rc) rc = fpd.alloc(npp * 8);
+    if (!rc) rc = fpsd.alloc(npp * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(fpd.p, 0, npp * 8, st));
+    HIPCHK(hipMemsetAsync(fpsd.p, 0, npp * 8, st));
+    for (i64 p0 = 0; p0 < npp; p0 += nb) {
+        i64 p1 = std::min(npp, p0 + nb), n = (p1 - p0) * ne4;
+        rc = build2_block(s, p0, p1, X.d(), Xs.d(), Y.d(), Ys.d(), 1, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), Xs.d(), G.d(), Gs.d());
+        hipLaunchKernelGGL(k_eg2_slab<256>, dim3((unsigned)((p1 - p0) * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
+                           G.d(), Gs.d(), part.d() + p0 * ne * 4 * ne);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_eg2_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_de, d_dp, part.d(),
+                       1, 1, d_fe, d_se2, fpd.d(), fpsd.d(), d_e, d_s);
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), fpsd.d(), d_e, d_s);
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(npp, 256)), dim3(256), 0, st, npp, fpd.d(), d_fp);
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(npp, 256)), dim3(256), 0, st, npp, fpsd.d(), d_sp2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const double *d_dp, double *d_fe,
+                                   double *d_fp, double *d_e, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (!d_de || !d_dp || !d_fe || !d_fp || !d_e) return fail(NEORX_E_ARG, "null argument");
+    i64 ne = s->d.nebf, np = s->d.npbf, ne6 = ne * ne * ne * ne * ne * ne, npp = np * np;
+    hipStream_t st = (hipStream_t)stream;
+    i64 nb = std::max<i64>(1, std::min<i64>(npp, BLOCK_CAP / ne6));
+    DevBuf X, G, Y, part, fpd;
+    int rc = X.alloc(nb * ne6 * 8);
+    if (!rc) rc = G.alloc(nb * ne6 * 8);
+    if (!rc) rc = Y.alloc(slab_cap(s, 3) * 8);
+    if (!rc) rc = part.alloc(npp * ne * (ne + 1) * 8);
+    if (!rc) rc = fpd.alloc(npp * 8);
+    if (rc) return rc;
+    for (i64 p0 = 0; p0 < npp; p0 += nb) {
+        i64 p1 = std::min(npp, p0 + nb), n = (p1 - p0) * ne6;
+        rc = build3_block(s, p0, p1, X.d(), Y.d(), st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), G.d());
+        hipLaunchKernelGGL(k_eg3_slab<256>, dim3((unsigned)((p1 - p0) * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+                           G.d(), part.d() + p0 * ne * (ne + 1));
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
+                       fpd.d());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), nullptr, d_e, nullptr);
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(npp, 256)), dim3(256), 0, st, npp, fpd.d(), d_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_gam2s, const double *d_gamee,
+                                   const double *d_de, const double *d_dp, double *d_fe, double *d_fp, double *d_e,
+                                   void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf, npp = np * np;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (!d_gam2s || !d_gamee || !d_de || !d_dp || !d_fe || !d_fp || !d_e) return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf part, fpd;
+    rc = part.alloc(npp * ne * (ne + 1) * 8);
+    if (!rc) rc = fpd.alloc(npp * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_eg4_slab<256, true>), dim3((unsigned)(npp * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+                       nullptr, d_gam2s, d_gamee, part.d());
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
+                       fpd.d());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), nullptr, d_e, nullptr);
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(npp, 256)), dim3(256), 0, st, npp, fpd.d(), d_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+// host-array wrappers: inputs copied to HBM, outputs copied back
+static int h2d(DevBuf &b, const double *h, i64 n)
+{
+    int rc = b.alloc(n * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(b.p, h, n * 8, hipMemcpyHostToDevice));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam2_dscf(int64_t handle, const double *de, const double *dp, double *focke, double *fockp,
+                               double *se2, double *sp2, double *e_gam2, double *s_gam2)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf;
+    DevBuf dDE, dDP, dFE, dFP, dSE, dSP, dE;
+    int rc = h2d(dDE, de, ne * ne);
+    if (!rc) rc = h2d(dDP, dp, np * np);
+    if (!rc) rc = h2d(dFE, focke, ne * ne);
+    if (!rc) rc = h2d(dFP, fockp, np * np);
+    if (!rc) rc = h2d(dSE, se2, ne * ne);
+    if (!rc) rc = h2d(dSP, sp2, np * np);
+    if (!rc) rc = dE.alloc(16);
+    if (!rc) rc = neorx_gam2_dscf_dev(handle, dDE.d(), dDP.d(), dFE.d(), dFP.d(), dSE.d(), dSP.d(), dE.d(), dE.d() + 1, 0);
+    if (rc) return rc;
+    double es[2];
+    HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(se2, dSE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sp2, dSP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e_gam2 = es[0];
+    *s_gam2 = es[1];
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam3_dscf(int64_t handle, const double *de, const double *dp, double *focke, double *fockp,
+                               double *e_gam3)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf;
+    DevBuf dDE, dDP, dFE, dFP, dE;
+    int rc = h2d(dDE, de, ne * ne);
+    if (!rc) rc = h2d(dDP, dp, np * np);
+    if (!rc) rc = h2d(dFE, focke, ne * ne);
+    if (!rc) rc = h2d(dFP, fockp, np * np);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc) rc = neorx_gam3_dscf_dev(handle, dDE.d(), dDP.d(), dFE.d(), dFP.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e_gam3, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_gam4_dscf(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee,
+                               const double *de, const double *dp, double *focke, double *fockp, double *e_gam4)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng2 = ne * ne * ne * ne * np * np, ngee = ne * ne * ne * ne;
+    DevBuf dG2, dGee, dDE, dDP, dFE, dFP, dE;
+    rc = h2d(dG2, gam_2s, ng2);
+    if (!rc) rc = h2d(dGee, gam_ee, ngee);
+    if (!rc) rc = h2d(dDE, de, ne * ne);
+    if (!rc) rc = h2d(dDP, dp, np * np);
+    if (!rc) rc = h2d(dFE, focke, ne * ne);
+    if (!rc) rc = h2d(dFP, fockp, np * np);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc) rc = neorx_gam4_dscf_dev(ne, np, dG2.d(), dGee.d(), dDE.d(), dDP.d(), dFE.d(), dFP.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(e_gam4, dE.p, 8, hipMemcpyDeviceToHost));
     return NEORX_OK;
 }

@@ -88,6 +88,24 @@ module neorx_capi
       real(c_double) :: de(*), dp(*), g(*), fe(*), fp(*), e
       integer(c_int) :: neorx_eg4icr
     end function
+    function neorx_gam2_dscf(handle, de, dp, fe, fp, se, sp, e, s) bind(C, name="neorx_gam2_dscf")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: de(*), dp(*), fe(*), fp(*), se(*), sp(*), e, s
+      integer(c_int) :: neorx_gam2_dscf
+    end function
+    function neorx_gam3_dscf(handle, de, dp, fe, fp, e) bind(C, name="neorx_gam3_dscf")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: handle
+      real(c_double) :: de(*), dp(*), fe(*), fp(*), e
+      integer(c_int) :: neorx_gam3_dscf
+    end function
+    function neorx_gam4_dscf(ne, np, g2s, gee, de, dp, fe, fp, e) bind(C, name="neorx_gam4_dscf")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: g2s(*), gee(*), de(*), dp(*), fe(*), fp(*), e
+      integer(c_int) :: neorx_gam4_dscf
+    end function
   end interface
 contains
   subroutine neorx_check(rc, where)
@@ -260,4 +278,66 @@ subroutine EG4ICR(Nchunks, nebf, npbf, ng4, DE, DP, GM4ICR, focke, fockp, E_gam4
   integer(c_int64_t) :: Nchunks, nebf, npbf, ng4
   real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), GM4ICR(ng4), focke(nebf, nebf), fockp(npbf, npbf), E_gam4
   call neorx_check(neorx_eg4icr(nebf, npbf, DE, DP, GM4ICR, focke, fockp, E_gam4), 'EG4ICR')
+end subroutine
+
+! GAM2_DSCF, source/gam2_DSCF_OMP.f:2-139
+subroutine GAM2_DSCF(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                     KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                     DE, DP, focke, fockp, se2, sp2, E_gam2, S_gam2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), focke(nebf, nebf), fockp(npbf, npbf)
+  real(c_double) :: se2(nebf, nebf), sp2(npbf, npbf), E_gam2, S_gam2
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM2_DSCF')
+  call neorx_check(neorx_gam2_dscf(h, DE, DP, focke, fockp, se2, sp2, E_gam2, S_gam2), 'GAM2_DSCF')
+  call neorx_check(neorx_system_destroy(h), 'GAM2_DSCF')
+end subroutine
+
+! GAM3_DSCF, source/gam3_DSCF_OMP.f:2-138
+subroutine GAM3_DSCF(Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                     KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                     DE, DP, focke, fockp, E_gam3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: DE(nebf, nebf), DP(npbf, npbf), focke(nebf, nebf), fockp(npbf, npbf), E_gam3
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM3_DSCF')
+  call neorx_check(neorx_gam3_dscf(h, DE, DP, focke, fockp, E_gam3), 'GAM3_DSCF')
+  call neorx_check(neorx_system_destroy(h), 'GAM3_DSCF')
+end subroutine
+
+! GAM4_DSCF, source/gam4_DSCF_OMP.f:2-104: reads GAM_ee.ufm and GAM_2s.ufm
+! from the working directory like read_GAM_ee / read_GAM_2s
+! (source/xchf1_independent/GAM_IO.f:28, source/gam_4_OMP.f:1195)
+subroutine GAM4_DSCF(NG4CHK, ne, np, ngee, ng2, ng4, DE, DP, focke, fockp, E_gam4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: NG4CHK, ne, np, ngee, ng2, ng4
+  real(c_double) :: DE(ne, ne), DP(np, np), focke(ne, ne), fockp(np, np), E_gam4
+  real(c_double), allocatable :: gee(:), g2s(:)
+  integer(c_int64_t) :: ia
+  allocate(gee(ngee), g2s(ng2))
+  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do ia = 1, ngee
+    read(810, rec=ia) gee(ia)
+  end do
+  close(810)
+  open(804, file='GAM_2s.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do ia = 1, ng2
+    read(804, rec=ia) g2s(ia)
+  end do
+  close(804)
+  call neorx_check(neorx_gam4_dscf(ne, np, g2s, gee, DE, DP, focke, fockp, E_gam4), 'GAM4_DSCF')
+  deallocate(gee, g2s)
 end subroutine

@@ -66,6 +66,12 @@ def load():
     _sig("neorx_eg4icr", [_I64, _I64] + [_P] * 6)
     _sig("neorx_eg2ic1_dev", [_I64, _I64] + [_P] * 10 + [_P])
     _sig("neorx_eg3ic1_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_gam2_dscf", [_I64] + [_P] * 8)
+    _sig("neorx_gam3_dscf", [_I64] + [_P] * 5)
+    _sig("neorx_gam4_dscf", [_I64, _I64] + [_P] * 7)
+    _sig("neorx_gam2_dscf_dev", [_I64] + [_P] * 8 + [_P])
+    _sig("neorx_gam3_dscf_dev", [_I64] + [_P] * 5 + [_P])
+    _sig("neorx_gam4_dscf_dev", [_I64, _I64] + [_P] * 7 + [_P])
     return _lib
 
 
@@ -155,6 +161,25 @@ class System:
         _check(_lib.neorx_gam_ee(self.handle, _p(g)))
         return g
 
+    # --- direct-SCF routines (host arrays) ---
+    def GAM2_DSCF(self, DE, DP, focke, fockp, se2, sp2):
+        """GAM2_DSCF (source/gam2_DSCF_OMP.f:2): focke, fockp, se2, sp2
+        (Fortran-ordered float64) accumulate in place; returns (E_gam2, S_gam2)."""
+        DE = _np.asfortranarray(DE, dtype=_np.float64)
+        DP = _np.asfortranarray(DP, dtype=_np.float64)
+        e = _np.zeros(2)
+        _check(_lib.neorx_gam2_dscf(self.handle, _p(DE), _p(DP), _p(focke), _p(fockp), _p(se2), _p(sp2), _p(e),
+                                    _p(e[1:])))
+        return float(e[0]), float(e[1])
+
+    def GAM3_DSCF(self, DE, DP, focke, fockp):
+        """GAM3_DSCF (source/gam3_DSCF_OMP.f:2): returns E_gam3."""
+        DE = _np.asfortranarray(DE, dtype=_np.float64)
+        DP = _np.asfortranarray(DP, dtype=_np.float64)
+        e = _np.zeros(1)
+        _check(_lib.neorx_gam3_dscf(self.handle, _p(DE), _p(DP), _p(focke), _p(fockp), _p(e)))
+        return float(e[0])
+
 
 def GAM4_ICR(nebf, npbf, gam_2s, gam_ee):
     load()
@@ -162,6 +187,18 @@ def GAM4_ICR(nebf, npbf, gam_2s, gam_ee):
     _check(_lib.neorx_gam4_icr(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
                                _p(g)))
     return g
+
+
+def GAM4_DSCF(nebf, npbf, gam_2s, gam_ee, DE, DP, focke, fockp):
+    """GAM4_DSCF (source/gam4_DSCF_OMP.f:2) with GAM_2s and GAM_ee passed in
+    (the reference reads them from GAM_2s.ufm / GAM_ee.ufm); returns E_gam4."""
+    load()
+    DE = _np.asfortranarray(DE, dtype=_np.float64)
+    DP = _np.asfortranarray(DP, dtype=_np.float64)
+    e = _np.zeros(1)
+    _check(_lib.neorx_gam4_dscf(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
+                                _p(DE), _p(DP), _p(focke), _p(fockp), _p(e)))
+    return float(e[0])
 
 
 def EG2IC1(nebf, npbf, DE, DP, GM2ICR, GM2SICR, FE, FP):

@@ -921,11 +921,16 @@ void orc_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
 }
 
 /* GAM2_IC1 (gam2_IC1_OMP.f:2-212) with contract_omega2_conv
- * (gam2_CONV_OMP.f:388-579) and the (x12+x21)/2 symmetrisation. */
-void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
+ * (gam2_CONV_OMP.f:388-579) and the (x12+x21)/2 symmetrisation.  With flush
+ * set every primitive value goes through underflow() first, as
+ * contract_omega2 of the direct-SCF path does (gam2_DSCF_OMP.f:514-515). */
+static void gam2_ic1_impl(const basis_t *Bp, const sys_t *Sp, int flush, double *GM2, double *GM2s)
 {
-    BAS_INIT;
-    SYS_INIT;
+    basis_t B = *Bp;
+    sys_t S = *Sp;
+    i64 nebf = B.nebf, npbf = B.npbf;
+    const i64 *kpestr = B.kpestr, *kpeend = B.kpeend;
+    const double *agebfcc = B.agebfcc, *agnbfcc = B.agnbfcc;
     i64 ne = nebf, np = npbf, ng2 = ne * ne * ne * ne * np * np;
     double *X = (double *)malloc(sizeof(double) * ng2), *Xs = (double *)malloc(sizeof(double) * ng2);
 #pragma omp parallel for collapse(2) schedule(dynamic, 1)
@@ -944,6 +949,10 @@ void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
                                             prim_t k[3] = {eprim(&B, c1), eprim(&B, c2), pprim(&B, jp)};
                                             double e, s;
                                             gam2_prim(&S, b, k, &e, &s);
+                                            if (flush) {
+                                                underflow(&e);
+                                                underflow(&s);
+                                            }
                                             double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
                                                         agebfcc[a2] * agebfcc[c2];
                                             om += e * cf;
@@ -965,6 +974,13 @@ void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
                         }
     free(X);
     free(Xs);
+}
+
+void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    gam2_ic1_impl(&B, &S, 0, GM2, GM2s);
 }
 
 /* GAM3_IC1 (gam3_IC1_OMP.f:2-263) with contract_omega3_conv
@@ -1114,13 +1130,15 @@ void orc_gam4_icr(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee, 
 
 /* EG2IC1 / thread_EG2ICR (gam2_IC1_OMP.f:339-570).  FE, FP accumulate
  * (add2fock); SE, SP are zeroed then filled; E, S overwritten. */
-void orc_eg2ic1(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM2, const double *GM2s,
-                double *FE, double *SE, double *FP, double *SP, double *E, double *S)
+static void eg2_impl(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM2, const double *GM2s,
+                     double *FE, double *SE, double *FP, double *SP, double *E, double *S, int zero_s)
 {
     i64 ne = nebf, np = npbf;
     double *xfe = (double *)calloc(ne * ne, sizeof(double)), *xfp = (double *)calloc(np * np, sizeof(double));
-    memset(SE, 0, sizeof(double) * ne * ne);
-    memset(SP, 0, sizeof(double) * np * np);
+    if (zero_s) {
+        memset(SE, 0, sizeof(double) * ne * ne);
+        memset(SP, 0, sizeof(double) * np * np);
+    }
     double e = 0.0, s = 0.0;
     /* Fortran DE(i,j) = DE[j*ne + i] (column-major) */
 #define DEm(i, j) DE[(j) * ne + (i)]
@@ -1147,6 +1165,12 @@ void orc_eg2ic1(i64 nebf, i64 npbf, const double *DE, const double *DP, const do
     *S = s;
     free(xfe);
     free(xfp);
+}
+
+void orc_eg2ic1(i64 nebf, i64 npbf, const double *DE, const double *DP, const double *GM2, const double *GM2s,
+                double *FE, double *SE, double *FP, double *SP, double *E, double *S)
+{
+    eg2_impl(nebf, npbf, DE, DP, GM2, GM2s, FE, SE, FP, SP, E, S, 1);
 }
 
 /* EG3IC1 / thread_EG3ICR (gam3_IC1_OMP.f:433-725) */
@@ -1215,4 +1239,76 @@ void orc_eg4icr(i64 nebf, i64 npbf, const double *DE, const double *DP, const do
     free(xfp);
 #undef DEm
 #undef DPm
+}
+
+/* ------------------------------------------------------------------------
+ * Direct-SCF routines: the same contracted values as the in-core path,
+ * recomputed and contracted in one call.
+ * ------------------------------------------------------------------------ */
+
+/* GAM2_DSCF (gam2_DSCF_OMP.f:2-139) with thread_gam2 (:141-343) and
+ * contract_omega2 (:345-559): GAM2 = OMG2(1,2) - OMG2(exchange)/2 from
+ * symmetrised contracted values whose primitives went through underflow();
+ * focke, fockp, se2, sp2 accumulate (add2fock, :130-133), E, S overwritten. */
+void orc_gam2_dscf(BAS_ARGS, SYS_ARGS, const double *DE, const double *DP, double *FE, double *FP, double *SE2,
+                   double *SP2, double *E, double *S2)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ng2 = nebf * nebf * nebf * nebf * npbf * npbf;
+    double *G = (double *)malloc(sizeof(double) * ng2), *Gs = (double *)malloc(sizeof(double) * ng2);
+    gam2_ic1_impl(&B, &S, 1, G, Gs);
+    eg2_impl(nebf, npbf, DE, DP, G, Gs, FE, SE2, FP, SP2, E, S2, 0);
+    free(G);
+    free(Gs);
+}
+
+/* GAM3_DSCF (gam3_DSCF_OMP.f:2-365): contract_omega3 (:367-717) is the
+ * six-permutation average of contract_omega3_conv, combined and contracted
+ * exactly as EG3IC1 does. */
+void orc_gam3_dscf(BAS_ARGS, SYS_ARGS, const double *DE, const double *DP, double *FE, double *FP, double *E)
+{
+    i64 ng3 = nebf * nebf * nebf * nebf * nebf * nebf * npbf * npbf;
+    double *G = (double *)malloc(sizeof(double) * ng3);
+    orc_gam3_ic1(nebf, npebf, npbf, kpestr, kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc,
+                 nucbfc, nat, ngtg, pmass, cat, zan, bcoef, gamma, G);
+    orc_eg3ic1(nebf, npbf, DE, DP, G, FE, FP, E);
+    free(G);
+}
+
+/* GAM4_DSCF (gam4_DSCF_OMP.f:2-320): symm_gam4/16 evaluated per entry from
+ * GAM_2s and GAM_ee and contracted as EG4ICR does, without the ng4 tensor. */
+void orc_gam4_dscf(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee, const double *DE, const double *DP,
+                   double *FE, double *FP, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    double *xfe = (double *)calloc(ne * ne, sizeof(double)), *xfp = (double *)calloc(np * np, sizeof(double));
+    double e = 0.0;
+#define DEm(i, j) DE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3)
+                                    for (i64 i4 = 0; i4 < ne; ++i4)
+                                        for (i64 j4 = 0; j4 < ne; ++j4) {
+                                            double g = symm_gam4(ne, np, GAM2s, GAMee, ip, jp, i1, j1, i2, j2, i3, j3,
+                                                                 i4, j4) /
+                                                       16.0;
+                                            xfe[j1 * ne + i1] +=
+                                                DPm(ip, jp) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * 4.0 * g;
+                                            xfp[jp * np + ip] += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * g;
+                                            e += DEm(i1, j1) * DEm(i2, j2) * DEm(i3, j3) * DEm(i4, j4) * DPm(ip, jp) * g;
+                                        }
+#undef DEm
+#undef DPm
+    for (i64 i = 0; i < ne * ne; ++i) FE[i] += xfe[i];
+    for (i64 i = 0; i < np * np; ++i) FP[i] += xfp[i];
+    *E = e;
+    free(xfe);
+    free(xfp);
 }

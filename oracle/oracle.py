@@ -137,6 +137,33 @@ def eg4icr(nebf, npbf, DE, DP, G, FE, FP):
     return float(e[0])
 
 
+def gam2_dscf(s, DE, DP, FE, FP, SE2, SP2):
+    """GAM2_DSCF semantics: FE, FP, SE2, SP2 (Fortran-ordered arrays) are
+    accumulated in place; returns (E_gam2, S_gam2)."""
+    e, sv = np.zeros(1), np.zeros(1)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_gam2_dscf(*ba, *sa, _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE), _p(FP), _p(SE2),
+                        _p(SP2), _p(e), _p(sv))
+    return float(e[0]), float(sv[0])
+
+
+def gam3_dscf(s, DE, DP, FE, FP):
+    e = np.zeros(1)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_gam3_dscf(*ba, *sa, _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE), _p(FP), _p(e))
+    return float(e[0])
+
+
+def gam4_dscf(nebf, npbf, g2s, gee, DE, DP, FE, FP):
+    e = np.zeros(1)
+    lib().orc_gam4_dscf(C.c_int64(nebf), C.c_int64(npbf), _p(np.ascontiguousarray(g2s)),
+                        _p(np.ascontiguousarray(gee)), _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE),
+                        _p(FP), _p(e))
+    return float(e[0])
+
+
 # ---------------------------------------------------------------------------
 # the reference itself (Fortran ABI: lower case + '_', every argument by
 # reference, INTEGER*8, column-major arrays)
@@ -259,3 +286,46 @@ def ref_boys(m, T):
     F = np.zeros(m + 1)
     ref().xgammaf_(_p(F), _d(T), _i8(m))
     return F[m]
+
+
+def ref_gam2_dscf(s, DE, DP, FE, FP, SE2, SP2):
+    """GAM2_DSCF (source/gam2_DSCF_OMP.f:2) of the reference; arrays as in
+    gam2_dscf."""
+    ne, npb = s["nebf"], s["npbf"]
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    e, sv = C.c_double(), C.c_double()
+    ref().gam2_dscf_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ne ** 4 * npb * npb), _i8(0), _i8(s["nat"]),
+                     _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep],
+                     _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE), _p(FP), _p(SE2), _p(SP2),
+                     C.byref(e), C.byref(sv))
+    return e.value, sv.value
+
+
+def ref_gam3_dscf(s, DE, DP, FE, FP):
+    ne, npb = s["nebf"], s["npbf"]
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    e = C.c_double()
+    ref().gam3_dscf_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ne ** 6 * npb * npb), _i8(0), _i8(s["nat"]),
+                     _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep],
+                     _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE), _p(FP), C.byref(e))
+    return e.value
+
+
+def ref_gam4_dscf(nebf, npbf, g2s, gee, DE, DP, FE, FP, workdir):
+    """GAM4_DSCF (source/gam4_DSCF_OMP.f:2) reads GAM_ee.ufm and GAM_2s.ufm
+    (direct access, 8-byte records) from the working directory: they are
+    written into workdir, which is made current for the call."""
+    np.ascontiguousarray(gee, dtype=np.float64).tofile(os.path.join(workdir, "GAM_ee.ufm"))
+    np.ascontiguousarray(g2s, dtype=np.float64).tofile(os.path.join(workdir, "GAM_2s.ufm"))
+    ngee, ng2 = nebf ** 4, nebf ** 4 * npbf ** 2
+    e = C.c_double()
+    cwd = os.getcwd()
+    os.chdir(workdir)
+    try:
+        ref().gam4_dscf_(_i8(1), _i8(nebf), _i8(npbf), _i8(ngee), _i8(ng2), _i8(ngee * ngee * npbf * npbf),
+                         _p(np.asfortranarray(DE)), _p(np.asfortranarray(DP)), _p(FE), _p(FP), C.byref(e))
+    finally:
+        os.chdir(cwd)
+    return e.value

@@ -16,7 +16,7 @@ program drive_shim
       real(c_double) :: AGEBFCC(*), AGNBFCC(*), ELCEX(*), NUCEX(*), ELCBFC(*), NUCBFC(*), GM2ICR(*), GM2sICR(*)
     end subroutine
   end interface
-  integer(c_int64_t) :: nebf, npebf, npbf, nat, ngtg, ng2, ng3, ng4, ngee, ng1, one, zero
+  integer(c_int64_t) :: nebf, npebf, npbf, nat, ngtg, ng2, ng3, ng4, ngee, ng1, one, zero, ia
   real(c_double) :: pmass, e2, s2, e3, e4
   real(c_double), allocatable :: cat(:), zan(:), bcoef(:), gamma(:), agebfcc(:), agnbfcc(:), elcex(:), nucex(:)
   real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:)
@@ -65,5 +65,28 @@ program drive_shim
   fp = 0.0d0
   call EG4ICR(one, nebf, npbf, ng4, de, dp, gm4, fe, fp, e4)
   write(11) gm4, fe, fp, e4
+  ! direct-SCF routines (xchf1_fock.f:247, :273, :290); GAM4_DSCF reads
+  ! GAM_2s.ufm, which the reference's integral phase writes
+  open(804, file='GAM_2s.ufm', form='unformatted', status='replace', access='direct', recl=8)
+  do ia = 1, ng2
+    write(804, rec=ia) gm2s(ia)
+  end do
+  close(804)
+  fe = 0.0d0
+  fp = 0.0d0
+  se = 0.0d0
+  sp = 0.0d0
+  call GAM2_DSCF(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
+                 ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, de, dp, fe, fp, se, sp, e2, s2)
+  write(11) fe, fp, se, sp, e2, s2
+  fe = 0.0d0
+  fp = 0.0d0
+  call GAM3_DSCF(one, nebf, npebf, npbf, ng3, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
+                 ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, de, dp, fe, fp, e3)
+  write(11) fe, fp, e3
+  fe = 0.0d0
+  fp = 0.0d0
+  call GAM4_DSCF(one, nebf, npbf, ngee, ng2, ng4, de, dp, fe, fp, e4)
+  write(11) fe, fp, e4
   close(11)
 end program

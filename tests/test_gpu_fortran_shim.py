@@ -71,7 +71,11 @@ def test_fortran_dropin_matches_oracle():
         gee = np.fromfile(os.path.join(td, "GAM_ee.ufm"), dtype=np.float64)
     ng2, ng3, ng4 = ne ** 4 * npb ** 2, ne ** 6 * npb ** 2, ne ** 8 * npb ** 2
     sizes = [ng2, ng2, ne * ne, ne * ne, npb * npb, npb * npb, 1, 1, ng3, ne * ne, npb * npb, 1, ng4, ne * ne,
-             npb * npb, 1]
+             npb * npb, 1,
+             ne * ne, npb * npb, ne * ne, npb * npb, 1, 1,  # GAM2_DSCF: FE, FP, SE2, SP2, E, S
+             ne * ne, npb * npb, 1,  # GAM3_DSCF
+             ne * ne, npb * npb, 1]  # GAM4_DSCF
+    assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
     check_close(parts[0], og2, what="GAM2_IC1")
@@ -98,3 +102,19 @@ def test_fortran_dropin_matches_oracle():
     assert parts[15][0] == pytest.approx(E4, rel=1e-11)
     og1, og1s = oracle.gam1(s)
     check_close(gam1, og1, what="GAM_1 file")
+    # direct-SCF routines against the oracle's restatement of GAM*_DSCF
+    F = [np.zeros((n, n), order="F") for n in (ne, npb, ne, npb)]
+    E2, S2 = oracle.gam2_dscf(s, DE, DP, *F)
+    for k, nm in enumerate(("FE", "FP", "SE2", "SP2")):
+        check_close(parts[16 + k], F[k].ravel(order="F"), what=f"GAM2_DSCF {nm}")
+    assert parts[20][0] == pytest.approx(E2, rel=1e-11) and parts[21][0] == pytest.approx(S2, rel=1e-11)
+    F = [np.zeros((n, n), order="F") for n in (ne, npb)]
+    E3d = oracle.gam3_dscf(s, DE, DP, *F)
+    check_close(parts[22], F[0].ravel(order="F"), what="GAM3_DSCF FE")
+    check_close(parts[23], F[1].ravel(order="F"), what="GAM3_DSCF FP")
+    assert parts[24][0] == pytest.approx(E3d, rel=1e-11)
+    F = [np.zeros((n, n), order="F") for n in (ne, npb)]
+    E4d = oracle.gam4_dscf(ne, npb, og2s, oee, DE, DP, *F)
+    check_close(parts[25], F[0].ravel(order="F"), what="GAM4_DSCF FE")
+    check_close(parts[26], F[1].ravel(order="F"), what="GAM4_DSCF FP")
+    assert parts[27][0] == pytest.approx(E4d, rel=1e-11)

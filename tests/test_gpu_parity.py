@@ -165,3 +165,39 @@ def test_eg3_eg4(torch_dev):
     check_close(F[0], F[2], what="EG4 FE")
     check_close(F[1], F[3], what="EG4 FP")
     assert e1 == pytest.approx(e2, rel=1e-11)
+
+
+def test_direct_scf_routines(torch_dev):
+    """GAM2/3/4_DSCF through the C ABI against the oracle's restatement of the
+    reference routines; Fock matrices start non-zero to check accumulation."""
+    s = synth.small(seed=71, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    ne, npb = s["nebf"], s["npbf"]
+    print(f"case direct SCF ne={ne} np={npb}", flush=True)
+    DE, DP = synth.density(ne, 2.0, 3), synth.density(npb, 1.0, 4)
+    rng = np.random.default_rng(5)
+    F0 = [np.asfortranarray(rng.standard_normal((n, n))) for n in (ne, npb, ne, npb)]
+    S = neorx.System(s)
+    G = [f.copy(order="F") for f in F0]
+    O = [f.copy(order="F") for f in F0]
+    ge, gs = S.GAM2_DSCF(DE, DP, *G)
+    oe, os_ = oracle.gam2_dscf(s, DE, DP, *O)
+    for g, o, nm in zip(G, O, ("FE", "FP", "SE2", "SP2")):
+        check_close(g, o, what=f"GAM2_DSCF {nm}")
+    assert ge == pytest.approx(oe, rel=1e-11) and gs == pytest.approx(os_, rel=1e-11)
+    G = [f.copy(order="F") for f in F0[:2]]
+    O = [f.copy(order="F") for f in F0[:2]]
+    ge3 = S.GAM3_DSCF(DE, DP, *G)
+    oe3 = oracle.gam3_dscf(s, DE, DP, *O)
+    for g, o, nm in zip(G, O, ("FE", "FP")):
+        check_close(g, o, what=f"GAM3_DSCF {nm}")
+    assert ge3 == pytest.approx(oe3, rel=1e-11)
+    og2, og2s = oracle.gam2_ic1(s)
+    oee = oracle.gam_ee(s)
+    G = [f.copy(order="F") for f in F0[:2]]
+    O = [f.copy(order="F") for f in F0[:2]]
+    ge4 = neorx.GAM4_DSCF(ne, npb, og2s, oee, DE, DP, *G)
+    oe4 = oracle.gam4_dscf(ne, npb, og2s, oee, DE, DP, *O)
+    for g, o, nm in zip(G, O, ("FE", "FP")):
+        check_close(g, o, what=f"GAM4_DSCF {nm}")
+    assert ge4 == pytest.approx(oe4, rel=1e-11)
+    S.close()

@@ -182,3 +182,36 @@ def test_oracle_eg3_eg4_vs_reference():
     check_close(F[0], F[2], what="EG4 FE")
     check_close(F[1], F[3], what="EG4 FP")
     assert e1 == pytest.approx(e2, rel=1e-12)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference checker oracle/_ref not built")
+def test_oracle_direct_scf_vs_reference(tmp_path):
+    """The oracle's GAM2/3/4_DSCF against the reference's own routines
+    (gam2_DSCF_OMP.f, gam3_DSCF_OMP.f, gam4_DSCF_OMP.f), Fock matrices starting
+    non-zero so that the accumulation semantics are pinned too."""
+    s = synth.small(seed=5, n_e_centres=2, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    ne, npb = s["nebf"], s["npbf"]
+    DE, DP = synth.density(ne, 2.0, 1), synth.density(npb, 1.0, 2)
+    rng = np.random.default_rng(0)
+    F0 = [np.asfortranarray(rng.standard_normal((n, n))) for n in (ne, npb, ne, npb)]
+    A = [f.copy(order="F") for f in F0]
+    B = [f.copy(order="F") for f in F0]
+    eo = oracle.gam2_dscf(s, DE, DP, *A)
+    er = oracle.ref_gam2_dscf(s, DE, DP, *B)
+    for a, b in zip(A, B):
+        check_close(a, b, what="GAM2_DSCF")
+    assert eo[0] == pytest.approx(er[0], rel=1e-11) and eo[1] == pytest.approx(er[1], rel=1e-11)
+    A = [f.copy(order="F") for f in F0[:2]]
+    B = [f.copy(order="F") for f in F0[:2]]
+    assert oracle.gam3_dscf(s, DE, DP, *A) == pytest.approx(oracle.ref_gam3_dscf(s, DE, DP, *B), rel=1e-11)
+    for a, b in zip(A, B):
+        check_close(a, b, what="GAM3_DSCF")
+    g2, g2s = oracle.gam2_ic1(s)
+    gee = oracle.gam_ee(s)
+    A = [f.copy(order="F") for f in F0[:2]]
+    B = [f.copy(order="F") for f in F0[:2]]
+    eo4 = oracle.gam4_dscf(ne, npb, g2s, gee, DE, DP, *A)
+    er4 = oracle.ref_gam4_dscf(ne, npb, g2s, gee, DE, DP, *B, workdir=str(tmp_path))
+    assert eo4 == pytest.approx(er4, rel=1e-11)
+    for a, b in zip(A, B):
+        check_close(a, b, what="GAM4_DSCF")
