@@ -329,3 +329,49 @@ def ref_gam4_dscf(nebf, npbf, g2s, gee, DE, DP, FE, FP, workdir):
     finally:
         os.chdir(cwd)
     return e.value
+
+
+def _in_dir(workdir, fn):
+    cwd = os.getcwd()
+    os.chdir(workdir)
+    try:
+        return fn()
+    finally:
+        os.chdir(cwd)
+
+
+def ref_gam1(s, workdir):
+    """GAM1_OMP_MD (source/gam_1_OMP.f:2) of the reference: it writes GAM_1.ufm
+    and GAM_1s.ufm (direct access, 8-byte records) into workdir."""
+    ne, npb = s["nebf"], s["npbf"]
+    keep = [np.ascontiguousarray(s["ampeb2c"]), np.ascontiguousarray(s["agebfcc"]),
+            np.ascontiguousarray(s["agnbfcc"]), np.ascontiguousarray(s["elcex"]), np.ascontiguousarray(s["nucex"]),
+            np.asfortranarray(s["elcam"]), np.asfortranarray(s["nucam"]), np.asfortranarray(s["elcbfc"]),
+            np.asfortranarray(s["nucbfc"])]
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    _in_dir(workdir, lambda: ref().gam1_omp_md_(_i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ne * ne * npb * npb),
+                                                 _i8(s["npebf"] ** 2 * npb * npb),
+                                                 _i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan),
+                                                 _p(b), _p(gm), *[_p(a) for a in keep]))
+    g = np.fromfile(os.path.join(workdir, "GAM_1.ufm"), dtype=np.float64)
+    gs = np.fromfile(os.path.join(workdir, "GAM_1s.ufm"), dtype=np.float64)
+    return g, gs
+
+
+def ref_gam_ee(s, workdir):
+    """calc_GAM_ee (source/gam_ee.f:2): writes GAM_ee.ufm into workdir."""
+    ne = s["nebf"]
+    keep = [np.ascontiguousarray(s["ampeb2c"]), np.ascontiguousarray(s["agebfcc"]), np.ascontiguousarray(s["elcex"]),
+            np.asfortranarray(s["elcam"]), np.asfortranarray(s["elcbfc"])]
+    _in_dir(workdir, lambda: ref().calc_gam_ee_(_i8(ne), _i8(s["npebf"]), _i8(ne ** 4), *[_p(a) for a in keep]))
+    return np.fromfile(os.path.join(workdir, "GAM_ee.ufm"), dtype=np.float64)
+
+
+def ref_gam4_icr(nebf, npbf, g2s, gee, workdir):
+    """GAM4_ICR (source/gam4_ICR_OMP.f:2): reads GAM_ee.ufm from workdir."""
+    np.ascontiguousarray(gee, dtype=np.float64).tofile(os.path.join(workdir, "GAM_ee.ufm"))
+    ngee, ng2 = nebf ** 4, nebf ** 4 * npbf ** 2
+    g4 = np.zeros(ngee * ngee * npbf * npbf)
+    _in_dir(workdir, lambda: ref().gam4_icr_(_i8(1), _i8(nebf), _i8(npbf), _i8(ngee), _i8(ng2), _i8(g4.size),
+                                             _p(np.ascontiguousarray(g2s)), _p(g4)))
+    return g4

@@ -215,3 +215,24 @@ def test_oracle_direct_scf_vs_reference(tmp_path):
     assert eo4 == pytest.approx(er4, rel=1e-11)
     for a, b in zip(A, B):
         check_close(a, b, what="GAM4_DSCF")
+
+
+@needs_ref
+def test_oracle_cheap_builders_vs_reference(tmp_path):
+    """GAM1_OMP_MD, calc_GAM_ee, GAM3_IC1 and GAM4_ICR of the oracle against
+    the reference's own routines (files they write/read live in tmp_path)."""
+    s = synth.small(seed=14, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0,), ngtg=2)
+    ne, npb = s["nebf"], s["npbf"]
+    g1, g1s = oracle.gam1(s)
+    r1, r1s = oracle.ref_gam1(s, str(tmp_path))
+    check_close(g1, r1, what="GAM_1")
+    check_close(g1s, r1s, what="GAM_1s")
+    gee = oracle.gam_ee(s)
+    check_close(gee, oracle.ref_gam_ee(s, str(tmp_path)), what="GAM_ee")
+    t = synth.small(seed=15, n_e_centres=1, e_shells=(0, 1), contract=(1,), p_shells=(0,), ngtg=2)
+    check_close(oracle.gam3_ic1(t), oracle.ref_gam3_ic1(t), what="GAM3_IC1")
+    u = synth.small(seed=16, n_e_centres=2, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    g2, g2s = oracle.gam2_ic1(u)
+    ue = oracle.gam_ee(u)
+    check_close(oracle.gam4_icr(u["nebf"], u["npbf"], g2s, ue),
+                oracle.ref_gam4_icr(u["nebf"], u["npbf"], g2s, ue, str(tmp_path)), what="GAM4_ICR")
