@@ -71,6 +71,97 @@ def cpu_baseline(np, sysd, budget_s, cores):
     return kind, rates, detail
 
 
+def fock_bench(np, torch, neorx, synth, ne, npb, reps):
+    """One in-core Fock build's contractions (EG2IC1 + EG3IC1) at nebf=ne,
+    npbf=npb on synthetic tensors of that size, FP64 vs FP32-stored tensors,
+    plus the energy difference of the two paths on a real small system."""
+    f64, f32 = torch.float64, torch.float32
+    ng2, ng3 = ne ** 4 * npb * npb, ne ** 6 * npb * npb
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    g2 = torch.randn(ng2, dtype=f64, device="cuda", generator=gen)
+    g2s = torch.randn(ng2, dtype=f64, device="cuda", generator=gen)
+    g3 = torch.randn(ng3, dtype=f64, device="cuda", generator=gen)
+    h = {k: torch.empty(v.numel(), dtype=f32, device="cuda") for k, v in (("g2", g2), ("g2s", g2s), ("g3", g3))}
+    for k, v in (("g2", g2), ("g2s", g2s), ("g3", g3)):
+        neorx.call_dev("neorx_convert_f32_dev", v.numel(), v.data_ptr(), h[k].data_ptr(), 0)
+    DE = torch.from_numpy(np.ascontiguousarray(synth.density(ne, 14.0, 1).ravel(order="F"))).cuda()
+    DP = torch.from_numpy(np.ascontiguousarray(synth.density(npb, 1.0, 2).ravel(order="F"))).cuda()
+    out = [torch.zeros(n, dtype=f64, device="cuda") for n in (ne * ne, ne * ne, npb * npb, npb * npb, 2, 1)]
+    ptr = [o.data_ptr() for o in out]
+
+    def build(mixed):
+        if mixed:
+            neorx.call_dev("neorx_eg2ic1_f32_dev", ne, npb, DE.data_ptr(), DP.data_ptr(), h["g2"].data_ptr(),
+                           h["g2s"].data_ptr(), ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[4] + 8, 0)
+            ev[0].record()
+            neorx.call_dev("neorx_eg3ic1_f32_dev", ne, npb, DE.data_ptr(), DP.data_ptr(), h["g3"].data_ptr(), ptr[0],
+                           ptr[2], ptr[5], 0)
+            ev[1].record()
+        else:
+            neorx.call_dev("neorx_eg2ic1_dev", ne, npb, DE.data_ptr(), DP.data_ptr(), g2.data_ptr(), g2s.data_ptr(),
+                           ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[4] + 8, 0)
+            ev[0].record()
+            neorx.call_dev("neorx_eg3ic1_dev", ne, npb, DE.data_ptr(), DP.data_ptr(), g3.data_ptr(), ptr[0], ptr[2],
+                           ptr[5], 0)
+            ev[1].record()
+
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    res = {}
+    for mixed in (False, True):
+        build(mixed)
+        torch.cuda.synchronize()
+        t, t3 = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            build(mixed)
+            torch.cuda.synchronize()
+            t.append((time.perf_counter() - t0) * 1e3)
+            t3.append(ev[0].elapsed_time(ev[1]))
+        res[mixed] = (min(t), min(t3))
+    del g2, g2s, g3, h
+    torch.cuda.empty_cache()
+    # energy difference of the two paths on real integrals (small system)
+    sysd = synth.small(seed=3, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=3)
+    S = neorx.System(sysd)
+    n2, n3 = sysd["nebf"] ** 4 * sysd["npbf"] ** 2, sysd["nebf"] ** 6 * sysd["npbf"] ** 2
+    G2, G2s, G3 = (torch.empty(n, dtype=f64, device="cuda") for n in (n2, n2, n3))
+    neorx.call_dev("neorx_gam2_ic1_dev", S.handle, G2.data_ptr(), G2s.data_ptr(), 0)
+    neorx.call_dev("neorx_gam3_ic1_dev", S.handle, G3.data_ptr(), 0)
+    F2, F2s, F3 = (torch.empty(n, dtype=f32, device="cuda") for n in (n2, n2, n3))
+    for a, b in ((G2, F2), (G2s, F2s), (G3, F3)):
+        neorx.call_dev("neorx_convert_f32_dev", a.numel(), a.data_ptr(), b.data_ptr(), 0)
+    me, mp = sysd["nebf"], sysd["npbf"]
+    de = torch.from_numpy(np.ascontiguousarray(synth.density(me, 4.0, 3).ravel(order="F"))).cuda()
+    dp = torch.from_numpy(np.ascontiguousarray(synth.density(mp, 1.0, 4).ravel(order="F"))).cuda()
+    energies = {}
+    for mixed in (False, True):
+        o = [torch.zeros(n, dtype=f64, device="cuda") for n in (me * me, me * me, mp * mp, mp * mp, 2, 1)]
+        q = [x.data_ptr() for x in o]
+        if mixed:
+            neorx.call_dev("neorx_eg2ic1_f32_dev", me, mp, de.data_ptr(), dp.data_ptr(), F2.data_ptr(),
+                           F2s.data_ptr(), q[0], q[1], q[2], q[3], q[4], q[4] + 8, 0)
+            neorx.call_dev("neorx_eg3ic1_f32_dev", me, mp, de.data_ptr(), dp.data_ptr(), F3.data_ptr(), q[0], q[2],
+                           q[5], 0)
+        else:
+            neorx.call_dev("neorx_eg2ic1_dev", me, mp, de.data_ptr(), dp.data_ptr(), G2.data_ptr(), G2s.data_ptr(),
+                           q[0], q[1], q[2], q[3], q[4], q[4] + 8, 0)
+            neorx.call_dev("neorx_eg3ic1_dev", me, mp, de.data_ptr(), dp.data_ptr(), G3.data_ptr(), q[0], q[2], q[5],
+                           0)
+        torch.cuda.synchronize()
+        energies[mixed] = float(o[4].cpu()[0] + o[5].cpu()[0])
+    S.close()
+    bytes3 = {False: 8 * ng3, True: 4 * ng3}
+    return {
+        "tensors": f"nebf={ne} npbf={npb}: GM2 {ng2} x2, GM3 {ng3} entries (synthetic values)",
+        "ms_fp64": res[False][0], "ms_mixed": res[True][0], "speedup": res[False][0] / res[True][0],
+        "eg3_ms_fp64": res[False][1], "eg3_ms_mixed": res[True][1],
+        "eg3_GBps_fp64": bytes3[False] / res[False][1] / 1e6, "eg3_GBps_mixed": bytes3[True] / res[True][1] / 1e6,
+        "dE_mixed_vs_fp64_Eh": abs(energies[True] - energies[False]),
+        "dE_system": f"nebf={me} npbf={mp} real GAM2/GAM3 integrals, E_gam2 + E_gam3 = {energies[False]:.12f} Eh",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +170,8 @@ def main():
     ap.add_argument("--nsample", type=int, default=1 << 20, help="e-e-p and e-e-e-p tuples per step")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fock", type=str, default="24,8",
+                    help="nebf,npbf of the Fock-build contraction timing (FP64 vs mixed); 'off' to skip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -172,6 +265,12 @@ def main():
                            3: "k_prim_small<3,SRC_LIST> + k_prim_gen<3,SRC_LIST>"}[dom],
                 "flops_per_tuple": flops_per}
 
+    fock = None
+    if args.fock != "off":
+        fne, fnp = (int(x) for x in args.fock.split(","))
+        print(f"# Fock-build contractions nebf={fne} npbf={fnp}", file=sys.stderr, flush=True)
+        fock = fock_bench(np, torch, neorx, synth, fne, fnp, 3)
+
     cpu = None
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
@@ -191,7 +290,7 @@ def main():
                                "geminal; all e-p tuples + seeded samples of e-e-p and e-e-e-p tuples",
                    "e_p_tuples": n1, "e_e_p_tuples": ns, "e_e_e_p_tuples": ns, "device": arch},
         "per_class_integrals_per_s": {"e-p": rates[1], "e-e-p": rates[2], "e-e-e-p": rates[3]},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock,
     }
     print(json.dumps(out))
 

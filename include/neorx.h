@@ -111,6 +111,34 @@ int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const doubl
 int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm3,
                      double *d_fe, double *d_fp, double *d_e, void *stream);
 
+/* EG4ICR on device arrays. */
+int neorx_eg4icr_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm4,
+                     double *d_fe, double *d_fp, double *d_e, void *stream);
+
+/* ---- mixed precision ------------------------------------------------------
+ * No counterpart in the reference, whose Fock build is FP64 throughout.  The
+ * in-core tensors are stored in FP32 (half the HBM bytes of every Fock build);
+ * products and sums stay FP64, so the only rounding is that of the stored
+ * values.  Same semantics as the FP64 entry points above. */
+int neorx_convert_f32_dev(int64_t n, const double *d_src, float *d_dst, void *stream);
+int neorx_eg2ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const float *d_gm2,
+                         const float *d_gm2s, double *d_fe, double *d_se, double *d_fp, double *d_sp, double *d_e,
+                         double *d_s, void *stream);
+int neorx_eg3ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const float *d_gm3,
+                         double *d_fe, double *d_fp, double *d_e, void *stream);
+int neorx_eg4icr_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const float *d_gm4,
+                         double *d_fe, double *d_fp, double *d_e, void *stream);
+/* Switch back to FP64 (no counterpart in the reference).  The reference's SCF
+ * exposes no DIIS error to these routines; the signal the library sees on
+ * every Fock build is the density itself, so the switch compares successive
+ * electronic densities: once max|DE - DE_previous| < threshold the caller
+ * should use the FP64 entry points, and *use_fp64 stays 1 from then on.
+ * d_state: device array of nebf*nebf + 2 doubles, zeroed before the first
+ * call; *delta (may be NULL) receives max|DE - DE_previous| (+inf on the
+ * first call).  Synchronises the stream. */
+int neorx_precision_switch_dev(int64_t nebf, const double *d_de, double *d_state, double threshold,
+                               int64_t *use_fp64, double *delta, void *stream);
+
 /* ---- direct-SCF routines (reference semantics) -------------------------- */
 /* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per
  * call (every primitive value through underflow(), as contract_omega2 does),

@@ -727,6 +727,14 @@ struct DevBuf {
     double *d() { return (double *)p; }
 };
 
+static int h2d(DevBuf &b, const double *h, i64 n)
+{
+    int rc = b.alloc(n * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(b.p, h, n * 8, hipMemcpyHostToDevice));
+    return NEORX_OK;
+}
+
 static const i64 CHUNK = (i64)256 * 65536; // lanes per launch of the contracted builders
 
 // primitive values held per slab (each of E and S); a slab holds at least
@@ -938,32 +946,32 @@ extern "C" int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, 
 // (the exchange read G(a,j2,i2,j1) of thread_EG2ICR re-indexed onto the same
 // slab), for G and Gs.  Partial vectors go to a scratch array that a second
 // kernel reduces in a fixed order (deterministic).
-template <int NT>
+template <int NT, class T>
 __global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, i64 np, const double *__restrict__ DE,
-                                                 const double *__restrict__ G, const double *__restrict__ Gs,
+                                                 const T *__restrict__ G, const T *__restrict__ Gs,
                                                  double *__restrict__ part)
 {
     extern __shared__ double sm[]; // [4][ne]: dir, ex, dirS, exS
     i64 blk = blockIdx.x; // (ip*np + jp)*ne + a
     i64 ne3 = ne * ne * ne;
-    const double *g = G + blk * ne3;
-    const double *gs = Gs ? Gs + blk * ne3 : nullptr;
+    const T *g = G + blk * ne3;
+    const T *gs = Gs ? Gs + blk * ne3 : nullptr;
     for (int k = threadIdx.x; k < 4 * ne; k += NT) sm[k] = 0.0;
     __syncthreads();
     // each lane owns one b (row) at a time and strides over (c,d)
     for (i64 b = 0; b < ne; ++b) {
         double dsum = 0.0, dsums = 0.0;
-        const double *row = g + b * ne * ne;
-        const double *rows = gs ? gs + b * ne * ne : nullptr;
+        const T *row = g + b * ne * ne;
+        const T *rows = gs ? gs + b * ne * ne : nullptr;
         for (i64 cd = threadIdx.x; cd < ne * ne; cd += NT) {
             i64 c = cd / ne, d = cd % ne;
-            double gv = row[cd];
+            double gv = (double)row[cd];
             double de_cd = DE[d * ne + c]; // DE(c,d), column-major
             double de_cb = DE[b * ne + c]; // DE(c,b)
             dsum += de_cd * gv;
             atomicAdd(&sm[ne + d], de_cb * gv);
             if (rows) {
-                double gsv = rows[cd];
+                double gsv = (double)rows[cd];
                 dsums += de_cd * gsv;
                 atomicAdd(&sm[3 * ne + d], de_cb * gsv);
             }
@@ -1042,9 +1050,10 @@ __global__ void k_energy(i64 np, const double *__restrict__ DP, const double *__
     }
 }
 
-extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
-                                const double *d_gm2, const double *d_gm2s, double *d_fe, double *d_se, double *d_fp,
-                                double *d_sp, double *d_e, double *d_s, void *stream)
+template <class T>
+static int eg2_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const T *d_gm2,
+                   const T *d_gm2s, double *d_fe, double *d_se, double *d_fp, double *d_sp, double *d_e, double *d_s,
+                   void *stream)
 {
     int rc = ensure_init();
     if (rc) return rc;
@@ -1059,8 +1068,8 @@ extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, 
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(fpd.p, 0, np * np * 8, st));
     HIPCHK(hipMemsetAsync(fpsd.p, 0, np * np * 8, st));
-    hipLaunchKernelGGL(k_eg2_slab<256>, dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, np, d_de, d_gm2, d_gm2s,
-                       part.d());
+    hipLaunchKernelGGL((k_eg2_slab<256, T>), dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, np, d_de, d_gm2,
+                       d_gm2s, part.d());
     HIPCHK(hipGetLastError());
     int withS = d_gm2s != nullptr;
     // FE/SE and the per-(ip,jp) proton terms into fpd/fpsd; then add fpd to FP
@@ -1076,6 +1085,20 @@ extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, 
     if (withS && d_sp) HIPCHK(hipMemcpyAsync(d_sp, fpsd.p, np * np * 8, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipStreamSynchronize(st)); // scratch buffers are released on return
     return NEORX_OK;
+}
+
+extern "C" int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                const double *d_gm2, const double *d_gm2s, double *d_fe, double *d_se, double *d_fp,
+                                double *d_sp, double *d_e, double *d_s, void *stream)
+{
+    return eg2_dev<double>(nebf, npbf, d_de, d_dp, d_gm2, d_gm2s, d_fe, d_se, d_fp, d_sp, d_e, d_s, stream);
+}
+
+extern "C" int neorx_eg2ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                    const float *d_gm2, const float *d_gm2s, double *d_fe, double *d_se,
+                                    double *d_fp, double *d_sp, double *d_e, double *d_s, void *stream)
+{
+    return eg2_dev<float>(nebf, npbf, d_de, d_dp, d_gm2, d_gm2s, d_fe, d_se, d_fp, d_sp, d_e, d_s, stream);
 }
 
 extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm2icr,
@@ -1117,51 +1140,102 @@ extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const 
     return NEORX_OK;
 }
 
-// EG3IC1.  Lane per entry of GM3 grouped by slabs (ip,jp,i1): each lane reads
-// one value and scatters its six ket-permuted contributions into the slab's
-// FE row (LDS) and the proton/energy partial.
-__constant__ int c_perm3[6][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}, {0, 2, 1}, {1, 2, 0}, {2, 1, 0}};
-__constant__ double c_w3[6] = {8.0, -4.0, 2.0, -4.0, 2.0, -4.0};
-
-template <int NT>
-__global__ void __launch_bounds__(NT) k_eg3_slab(i64 ne, i64 np, const double *__restrict__ DE,
-                                                 const double *__restrict__ G, double *__restrict__ part)
+// sum over the 64 lanes of a wavefront (every lane gets the total)
+NRX_DI double wave_sum(double v)
 {
-    // part[blk][ne + 1]: FE row contributions for i1 = a (without DP), and the
-    // proton scalar sum_{...} DE DE DE g
-    extern __shared__ double sm[];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// EG3IC1 (thread_EG3ICR, gam3_IC1_OMP.f:537-725).  Block = slab (ip,jp,i1=a)
+// of GM3, entries g = G[a][b][c][d][e][f] (i1, j-slot 1, i2, j-slot 2, i3,
+// j-slot 3).  thread_EG3ICR's six ket-permuted reads, weighted
+// (8,-4,2,-4,2,-4)/8 and times 3 for FE, become, per stored entry,
+//   F[b] += 3 ( DE(c,d)DE(e,f) - 1/2 DE(c,f)DE(e,d) ) g
+//   F[d] += 3 (-1/2 DE(c,b)DE(e,f) + 1/4 DE(c,f)DE(e,b) ) g
+//   F[f] += 3 ( 1/4 DE(c,b)DE(e,d) - 1/2 DE(c,d)DE(e,b) ) g
+// (F = FE row a before the DP(ip,jp) weight).  Each thread owns one (e,f)
+// column pair, so every read of the slab is a contiguous row of ne^2 values;
+// the sums over c run in registers, F[b] and F[d] are wave-reduced before one
+// LDS atomic per wavefront.  T is the storage type of GM3 (double, or float
+// for the mixed-precision path); all arithmetic is FP64.
+// part[blk][0..ne-1] = F, part[blk][ne] = sum_x DE(a,x) F[x] / 3.
+template <class T>
+__global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__restrict__ DE, const T *__restrict__ G,
+                                                   double *__restrict__ part)
+{
+    // offsets inside one slab are 32-bit (ne^5 < 2^31 is checked at launch)
+    extern __shared__ double F[]; // [ne]
+    const unsigned ne = (unsigned)ne_;
     i64 blk = blockIdx.x;
-    i64 a = blk % ne;
-    i64 ne5 = ne * ne * ne * ne * ne;
-    const double *g = G + blk * ne5;
-    for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
+    unsigned a = (unsigned)(blk % ne);
+    const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne;
+    const T *g = G + blk * (i64)ne4 * ne;
+    for (int k = threadIdx.x; k < ne; k += blockDim.x) F[k] = 0.0;
     __syncthreads();
-    double pacc = 0.0;
-    for (i64 r = threadIdx.x; r < ne5; r += NT) {
-        double gv = g[r];
-        // r -> (b, c, d, e, f) = (k1, i2, k2, i3, k3)
-        i64 f = r % ne, e = (r / ne) % ne, d = (r / (ne * ne)) % ne, c = (r / (ne * ne * ne)) % ne,
-            b = r / (ne * ne * ne * ne);
-        i64 kk[3] = {b, d, f};
-        // thread_EG3ICR reads OMG3_sigma = G(i1, j_s1, i2, j_s2, i3, j_s3) for
-        // sigma in {123, 213, 312, 132, 231, 321}; this entry is OMG3_sigma of the
-        // tuple with j_{s_m} = kk[m]
-        for (int p = 0; p < 6; ++p) {
-            i64 j[3];
-            // OMG3 orders: 123:(j1,j2,j3) 213:(j2,j1,j3) 312:(j3,j1,j2) 132:(j1,j3,j2) 231:(j2,j3,j1) 321:(j3,j2,j1)
-            j[c_perm3[p][0]] = kk[0];
-            j[c_perm3[p][1]] = kk[1];
-            j[c_perm3[p][2]] = kk[2];
-            double w = c_w3[p] * gv / 8.0;
-            double d2 = DE[j[1] * ne + c], d3 = DE[j[2] * ne + e];
-            atomicAdd(&sm[j[0]], 3.0 * d2 * d3 * w);
-            pacc += DE[j[0] * ne + a] * d2 * d3 * w;
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (unsigned ef0 = 0; ef0 < ne2; ef0 += blockDim.x) {
+        unsigned ef = ef0 + threadIdx.x;
+        bool on = ef < ne2;
+        unsigned e = on ? ef / ne : 0, f = on ? ef % ne : 0;
+        double de_ef = on ? DE[f * ne + e] : 0.0; // DE(e,f), column-major
+        double accf = 0.0;
+        for (unsigned b = 0; b < ne; ++b) {
+            double de_eb = on ? DE[b * ne + e] : 0.0;
+            double accb = 0.0;
+            for (unsigned d = 0; d < ne; ++d) {
+                double de_ed = on ? DE[d * ne + e] : 0.0;
+                double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                if (on) {
+                    const unsigned o0 = b * ne4 + d * ne2 + ef;
+                    const double *dd = DE + d * ne, *db = DE + b * ne, *df = DE + f * ne;
+                    unsigned c = 0;
+                    // eight independent loads in flight per lane before the FMAs
+                    for (; c + 8 <= ne; c += 8) {
+                        double gv[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) gv[k] = (double)g[o0 + (c + k) * ne3];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            s1 += dd[c + k] * gv[k]; // DE(c,d)
+                            s2 += db[c + k] * gv[k]; // DE(c,b)
+                            s3 += df[c + k] * gv[k]; // DE(c,f)
+                        }
+                    }
+                    for (; c < ne; ++c) {
+                        double gv = (double)g[o0 + c * ne3];
+                        s1 += dd[c] * gv;
+                        s2 += db[c] * gv;
+                        s3 += df[c] * gv;
+                    }
+                }
+                accb += de_ef * s1 - 0.5 * de_ed * s3;
+                accf += 0.25 * de_ed * s2 - 0.5 * de_eb * s1;
+                double vd = wave_sum(-0.5 * de_ef * s2 + 0.25 * de_eb * s3);
+                if (lead) atomicAdd(&F[d], 3.0 * vd);
+            }
+            accb = wave_sum(accb);
+            if (lead) atomicAdd(&F[b], 3.0 * accb);
         }
+        if (on) atomicAdd(&F[f], 3.0 * accf);
     }
-    for (int o = 32; o > 0; o >>= 1) pacc += __shfl_down(pacc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&sm[ne], pacc);
     __syncthreads();
-    for (int k = threadIdx.x; k < ne + 1; k += NT) part[blk * (ne + 1) + k] = sm[k];
+    if (threadIdx.x < 64) {
+        double p = 0.0;
+        for (unsigned x = threadIdx.x; x < ne; x += 64) p += DE[x * ne + a] * F[x];
+        p = wave_sum(p);
+        for (unsigned x = threadIdx.x; x < ne; x += 64) part[blk * (ne + 1) + x] = F[x];
+        if (threadIdx.x == 0) part[blk * (ne + 1) + ne] = p / 3.0;
+    }
+}
+
+// largest nebf whose e-e-e-p slab (nebf^5 entries) has 32-bit offsets
+static const i64 EG3_MAX_NE = 73;
+
+static inline unsigned eg3_threads(i64 ne)
+{
+    i64 t = ((ne * ne + 63) / 64) * 64;
+    return (unsigned)std::min<i64>(t, 1024);
 }
 
 __global__ void k_eg3_reduce(i64 ne, i64 np, const double *__restrict__ DP, const double *__restrict__ part,
@@ -1185,20 +1259,22 @@ __global__ void k_eg3_reduce(i64 ne, i64 np, const double *__restrict__ DP, cons
     }
 }
 
-extern "C" int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
-                                const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
+template <class T>
+static int eg3_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const T *d_gm3, double *d_fe,
+                   double *d_fp, double *d_e, void *stream)
 {
     int rc = ensure_init();
     if (rc) return rc;
     i64 ne = nebf, np = npbf;
     if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (ne > EG3_MAX_NE) return fail(NEORX_E_UNSUPPORTED, "EG3IC1: nebf=%lld > %lld", (long long)ne, (long long)EG3_MAX_NE);
     hipStream_t st = (hipStream_t)stream;
     i64 nblk = np * np * ne;
     DevBuf part, fpd;
     rc = part.alloc(nblk * (ne + 1) * 8);
     if (!rc) rc = fpd.alloc(np * np * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_eg3_slab<256>, dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, st, ne, np, d_de, d_gm3,
+    hipLaunchKernelGGL(k_eg3_slab<T>, dim3((unsigned)nblk), dim3(eg3_threads(ne)), ne * 8, st, ne, d_de, d_gm3,
                        part.d());
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(),
@@ -1210,6 +1286,18 @@ extern "C" int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, 
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
+}
+
+extern "C" int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    return eg3_dev<double>(nebf, npbf, d_de, d_dp, d_gm3, d_fe, d_fp, d_e, stream);
+}
+
+extern "C" int neorx_eg3ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                    const float *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    return eg3_dev<float>(nebf, npbf, d_de, d_dp, d_gm3, d_fe, d_fp, d_e, stream);
 }
 
 extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm3icr,
@@ -1242,9 +1330,9 @@ extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const 
 
 // EG4ICR: lane per entry, direct port of the reduction structure
 // (FE row per slab (ip,jp,i1) in LDS).
-template <int NT, bool ONFLY>
+template <int NT, bool ONFLY, class T>
 __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *__restrict__ DE,
-                                                 const double *__restrict__ G, const double *__restrict__ G2s,
+                                                 const T *__restrict__ G, const double *__restrict__ G2s,
                                                  const double *__restrict__ Gee, double *__restrict__ part)
 {
     // block (ip, jp, i1): ONFLY evaluates symm_gam4/16 per entry from GAM_2s
@@ -1253,7 +1341,7 @@ __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *_
     extern __shared__ double sm[];
     i64 blk = blockIdx.x, a = blk % ne, pp = blk / ne, ip = pp / np, jp = pp % np;
     i64 ne7 = ne * ne * ne * ne * ne * ne * ne;
-    const double *g = ONFLY ? nullptr : G + blk * ne7;
+    const T *g = ONFLY ? nullptr : G + blk * ne7;
     for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
     __syncthreads();
     double pacc = 0.0;
@@ -1266,7 +1354,7 @@ __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *_
             i64 br[4] = {a, i2, i3, i4}, kt[4] = {j1, j2, j3, j4};
             gv = gam4_value(ne, np, G2s, Gee, ip, jp, br, kt);
         } else {
-            gv = g[r];
+            gv = (double)g[r];
         }
         double ddd = DE[j2 * ne + i2] * DE[j3 * ne + i3] * DE[j4 * ne + i4];
         atomicAdd(&sm[j1], 4.0 * ddd * gv);
@@ -1278,41 +1366,132 @@ __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *_
     for (int k = threadIdx.x; k < ne + 1; k += NT) part[blk * (ne + 1) + k] = sm[k];
 }
 
+template <class T>
+static int eg4_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const T *d_gm4, double *d_fe,
+                   double *d_fp, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    hipStream_t st = (hipStream_t)stream;
+    i64 nblk = np * np * ne;
+    DevBuf part, fpd;
+    rc = part.alloc(nblk * (ne + 1) * 8);
+    if (!rc) rc = fpd.alloc(np * np * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_eg4_slab<256, false, T>), dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+                       d_gm4, nullptr, nullptr, part.d());
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(),
+                       d_fe, fpd.d());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd.d(), nullptr, d_e, nullptr);
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(np * np, 256)), dim3(256), 0, st, np * np, fpd.d(), d_fp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_eg4icr_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                const double *d_gm4, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    return eg4_dev<double>(nebf, npbf, d_de, d_dp, d_gm4, d_fe, d_fp, d_e, stream);
+}
+
+extern "C" int neorx_eg4icr_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                    const float *d_gm4, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    return eg4_dev<float>(nebf, npbf, d_de, d_dp, d_gm4, d_fe, d_fp, d_e, stream);
+}
+
 extern "C" int neorx_eg4icr(int64_t nebf, int64_t npbf, const double *de, const double *dp, const double *gm4icr,
                             double *focke, double *fockp, double *e_gam4)
 {
     int rc = ensure_init();
     if (rc) return rc;
     i64 ne = nebf, np = npbf, ng4 = ne * ne * ne * ne * ne * ne * ne * ne * np * np;
-    DevBuf dDE, dDP, dG, dFE, dFP, dE, part, fpd;
-    i64 nblk = np * np * ne;
-    rc = dDE.alloc(ne * ne * 8);
-    if (!rc) rc = dDP.alloc(np * np * 8);
-    if (!rc) rc = dG.alloc(ng4 * 8);
-    if (!rc) rc = dFE.alloc(ne * ne * 8);
-    if (!rc) rc = dFP.alloc(np * np * 8);
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    DevBuf dDE, dDP, dG, dFE, dFP, dE;
+    rc = h2d(dDE, de, ne * ne);
+    if (!rc) rc = h2d(dDP, dp, np * np);
+    if (!rc) rc = h2d(dG, gm4icr, ng4);
+    if (!rc) rc = h2d(dFE, focke, ne * ne);
+    if (!rc) rc = h2d(dFP, fockp, np * np);
     if (!rc) rc = dE.alloc(8);
-    if (!rc) rc = part.alloc(nblk * (ne + 1) * 8);
-    if (!rc) rc = fpd.alloc(np * np * 8);
+    if (!rc) rc = eg4_dev<double>(ne, np, dDE.d(), dDP.d(), dG.d(), dFE.d(), dFP.d(), dE.d(), 0);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dG.p, gm4icr, ng4 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dFE.p, focke, ne * ne * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL((k_eg4_slab<256, false>), dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, 0, ne, np, dDE.d(),
-                       dG.d(), nullptr, nullptr, part.d());
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, 0, ne, np, dDP.d(), part.d(),
-                       dFE.d(), fpd.d());
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, 0, np, dDP.d(), fpd.d(), nullptr, dE.d(), nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipDeviceSynchronize());
-    std::vector<double> h(np * np);
-    HIPCHK(hipMemcpy(h.data(), fpd.p, np * np * 8, hipMemcpyDeviceToHost));
-    for (i64 k = 0; k < np * np; ++k) fockp[k] += h[k];
     HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(e_gam4, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// mixed precision: FP32 copies of the in-core tensors and the switch back
+// to FP64
+// ------------------------------------------------------------------
+__global__ void k_to_f32(i64 n, const double *__restrict__ x, float *__restrict__ y)
+{
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x)
+        y[t] = (float)x[t];
+}
+
+extern "C" int neorx_convert_f32_dev(int64_t n, const double *d_src, float *d_dst, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_src || !d_dst))) return fail(NEORX_E_ARG, "bad conversion arguments");
+    if (n == 0) return NEORX_OK;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<i64>(nblocks(n, 256), 65536)), dim3(256), 0,
+                       (hipStream_t)stream, n, d_src, d_dst);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+// state[0] = 1 once FP64 is selected (sticky), state[1] = 1 once a previous
+// density is held, state[2..] = previous DE; out = max|DE - previous|
+__global__ void k_switch(i64 n, const double *__restrict__ de, double *__restrict__ state, double threshold,
+                         double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    bool have = state[1] != 0.0;
+    for (i64 k = threadIdx.x; k < n; k += blockDim.x) {
+        double d = have ? fabs(de[k] - state[2 + k]) : INFINITY;
+        m = d > m ? d : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    for (i64 k = threadIdx.x; k < n; k += blockDim.x) state[2 + k] = de[k];
+    if (threadIdx.x == 0) {
+        double delta = red[0];
+        if (have && delta < threshold) state[0] = 1.0;
+        state[1] = 1.0;
+        out[0] = delta;
+        out[1] = state[0];
+    }
+}
+
+extern "C" int neorx_precision_switch_dev(int64_t nebf, const double *d_de, double *d_state, double threshold,
+                                          int64_t *use_fp64, double *delta, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || !d_de || !d_state || !use_fp64) return fail(NEORX_E_ARG, "bad switch arguments");
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf out;
+    rc = out.alloc(16);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_switch, dim3(1), dim3(256), 0, st, nebf * nebf, d_de, d_state, threshold, out.d());
+    HIPCHK(hipGetLastError());
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, out.p, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *use_fp64 = h[1] != 0.0;
+    if (delta) *delta = h[0];
     return NEORX_OK;
 }
 
@@ -1350,7 +1529,7 @@ extern "C" int neorx_gam2_dscf_dev(int64_t handle, const double *d_de, const dou
         rc = build2_block(s, p0, p1, X.d(), Xs.d(), Y.d(), Ys.d(), 1, st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), Xs.d(), G.d(), Gs.d());
-        hipLaunchKernelGGL(k_eg2_slab<256>, dim3((unsigned)((p1 - p0) * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
+        hipLaunchKernelGGL((k_eg2_slab<256, double>), dim3((unsigned)((p1 - p0) * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
                            G.d(), Gs.d(), part.d() + p0 * ne * 4 * ne);
         HIPCHK(hipGetLastError());
     }
@@ -1369,6 +1548,8 @@ extern "C" int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const dou
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (s->d.nebf > EG3_MAX_NE)
+        return fail(NEORX_E_UNSUPPORTED, "GAM3_DSCF: nebf=%lld > %lld", (long long)s->d.nebf, (long long)EG3_MAX_NE);
     if (!d_de || !d_dp || !d_fe || !d_fp || !d_e) return fail(NEORX_E_ARG, "null argument");
     i64 ne = s->d.nebf, np = s->d.npbf, ne6 = ne * ne * ne * ne * ne * ne, npp = np * np;
     hipStream_t st = (hipStream_t)stream;
@@ -1385,8 +1566,8 @@ extern "C" int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const dou
         rc = build3_block(s, p0, p1, X.d(), Y.d(), st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), G.d());
-        hipLaunchKernelGGL(k_eg3_slab<256>, dim3((unsigned)((p1 - p0) * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
-                           G.d(), part.d() + p0 * ne * (ne + 1));
+        hipLaunchKernelGGL(k_eg3_slab<double>, dim3((unsigned)((p1 - p0) * ne)), dim3(eg3_threads(ne)), ne * 8, st, ne,
+                           d_de, G.d(), part.d() + p0 * ne * (ne + 1));
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
@@ -1412,7 +1593,7 @@ extern "C" int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_g
     rc = part.alloc(npp * ne * (ne + 1) * 8);
     if (!rc) rc = fpd.alloc(npp * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_eg4_slab<256, true>), dim3((unsigned)(npp * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+    hipLaunchKernelGGL((k_eg4_slab<256, true, double>), dim3((unsigned)(npp * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
                        nullptr, d_gam2s, d_gamee, part.d());
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
                        fpd.d());
@@ -1424,13 +1605,6 @@ extern "C" int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_g
 }
 
 // host-array wrappers: inputs copied to HBM, outputs copied back
-static int h2d(DevBuf &b, const double *h, i64 n)
-{
-    int rc = b.alloc(n * 8);
-    if (rc) return rc;
-    HIPCHK(hipMemcpy(b.p, h, n * 8, hipMemcpyHostToDevice));
-    return NEORX_OK;
-}
 
 extern "C" int neorx_gam2_dscf(int64_t handle, const double *de, const double *dp, double *focke, double *fockp,
                                double *se2, double *sp2, double *e_gam2, double *s_gam2)

@@ -66,6 +66,12 @@ def load():
     _sig("neorx_eg4icr", [_I64, _I64] + [_P] * 6)
     _sig("neorx_eg2ic1_dev", [_I64, _I64] + [_P] * 10 + [_P])
     _sig("neorx_eg3ic1_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_eg4icr_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_convert_f32_dev", [_I64, _P, _P, _P])
+    _sig("neorx_eg2ic1_f32_dev", [_I64, _I64] + [_P] * 10 + [_P])
+    _sig("neorx_eg3ic1_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_eg4icr_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_precision_switch_dev", [_I64, _P, _P, _D, _P, _P, _P])
     _sig("neorx_gam2_dscf", [_I64] + [_P] * 8)
     _sig("neorx_gam3_dscf", [_I64] + [_P] * 5)
     _sig("neorx_gam4_dscf", [_I64, _I64] + [_P] * 7)
@@ -232,3 +238,19 @@ def EG4ICR(nebf, npbf, DE, DP, GM4ICR, FE, FP):
     e = _np.zeros(1)
     _check(_lib.neorx_eg4icr(nebf, npbf, _p(DE), _p(DP), _p(_np.ascontiguousarray(GM4ICR)), _p(FE), _p(FP), _p(e)))
     return float(e[0])
+
+
+def precision_switch_dev(nebf, d_de, d_state, threshold=1e-5, stream=0):
+    """Mixed-precision switch: returns (use_fp64, max|DE - DE_previous|);
+    d_state is a zeroed device array of nebf*nebf + 2 doubles."""
+    load()
+    u = _C.c_int64()
+    d = _C.c_double()
+    _check(_lib.neorx_precision_switch_dev(nebf, d_de, d_state, threshold, _C.byref(u), _C.byref(d), stream))
+    return bool(u.value), d.value
+
+
+def call_dev(name, *args):
+    """Call a ``neorx_*_dev`` entry point with raw device pointers / ints."""
+    load()
+    _check(getattr(_lib, name)(*args))

@@ -201,3 +201,97 @@ def test_direct_scf_routines(torch_dev):
         check_close(g, o, what=f"GAM4_DSCF {nm}")
     assert ge4 == pytest.approx(oe4, rel=1e-11)
     S.close()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("ne,npb", [(5, 3), (12, 2)])
+def test_mixed_precision_contractions(torch_dev, ne, npb):
+    """FP32-stored tensors with FP64 arithmetic: the GPU result equals the
+    oracle's FP64 contraction of the FP32-rounded tensor (bar of tests/_util),
+    and stays within 1e-6 (relative, per matrix) of the FP64 path."""
+    torch = torch_dev
+    print(f"case mixed precision ne={ne} np={npb}", flush=True)
+    rng = np.random.default_rng(ne)
+    DE, DP = synth.density(ne, 2.0, 7), synth.density(npb, 1.0, 8)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    g3r = g3.astype(np.float32).astype(np.float64)
+    d_de, d_dp = _dev(torch, DE.ravel(order="F")), _dev(torch, DP.ravel(order="F"))
+    d_g3 = _dev(torch, g3)
+    d_g3f = torch.empty(g3.size, dtype=torch.float32, device="cuda")
+    neorx.call_dev("neorx_convert_f32_dev", g3.size, d_g3.data_ptr(), d_g3f.data_ptr(), 0)
+    res = {}
+    for name, gp in (("neorx_eg3ic1_dev", d_g3), ("neorx_eg3ic1_f32_dev", d_g3f)):
+        fe = torch.zeros(ne * ne, dtype=torch.float64, device="cuda")
+        fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+        e = torch.zeros(1, dtype=torch.float64, device="cuda")
+        neorx.call_dev(name, ne, npb, d_de.data_ptr(), d_dp.data_ptr(), gp.data_ptr(), fe.data_ptr(), fp.data_ptr(),
+                       e.data_ptr(), 0)
+        torch.cuda.synchronize()
+        res[name] = (fe.cpu().numpy(), fp.cpu().numpy(), float(e.cpu()[0]))
+    for name, g in (("neorx_eg3ic1_dev", g3), ("neorx_eg3ic1_f32_dev", g3r)):
+        FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+        E = oracle.eg3ic1(ne, npb, DE, DP, g, FE, FP)
+        check_close(res[name][0], FE.ravel(order="F"), what=f"{name} FE")
+        check_close(res[name][1], FP.ravel(order="F"), what=f"{name} FP")
+        assert res[name][2] == pytest.approx(E, rel=1e-11)
+    a, b = res["neorx_eg3ic1_dev"], res["neorx_eg3ic1_f32_dev"]
+    assert np.max(np.abs(a[0] - b[0])) <= 1e-6 * np.max(np.abs(a[0]))
+    if ne <= 5:
+        g2, g2s = rng.standard_normal(ne ** 4 * npb * npb), rng.standard_normal(ne ** 4 * npb * npb)
+        g4 = rng.standard_normal(ne ** 8 * npb * npb)
+        d = {k: _dev(torch, v) for k, v in (("g2", g2), ("g2s", g2s), ("g4", g4))}
+        f = {k: torch.empty(v.numel(), dtype=torch.float32, device="cuda") for k, v in d.items()}
+        for k in d:
+            neorx.call_dev("neorx_convert_f32_dev", d[k].numel(), d[k].data_ptr(), f[k].data_ptr(), 0)
+        rd = lambda x: x.astype(np.float32).astype(np.float64)  # noqa: E731
+        out = [torch.zeros(n, dtype=torch.float64, device="cuda") for n in (ne * ne, ne * ne, npb * npb, npb * npb, 2)]
+        neorx.call_dev("neorx_eg2ic1_f32_dev", ne, npb, d_de.data_ptr(), d_dp.data_ptr(), f["g2"].data_ptr(),
+                       f["g2s"].data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
+                       out[3].data_ptr(), out[4].data_ptr(), out[4].data_ptr() + 8, 0)
+        torch.cuda.synchronize()
+        FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+        SE, SP, E, S = oracle.eg2ic1(ne, npb, DE, DP, rd(g2), rd(g2s), FE, FP)
+        check_close(out[0].cpu().numpy(), FE.ravel(order="F"), what="EG2 f32 FE")
+        check_close(out[1].cpu().numpy(), SE.ravel(order="F"), what="EG2 f32 SE")
+        check_close(out[2].cpu().numpy(), FP.ravel(order="F"), what="EG2 f32 FP")
+        assert out[4].cpu().numpy()[0] == pytest.approx(E, rel=1e-11)
+        fe = torch.zeros(ne * ne, dtype=torch.float64, device="cuda")
+        fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+        e = torch.zeros(1, dtype=torch.float64, device="cuda")
+        neorx.call_dev("neorx_eg4icr_f32_dev", ne, npb, d_de.data_ptr(), d_dp.data_ptr(), f["g4"].data_ptr(),
+                       fe.data_ptr(), fp.data_ptr(), e.data_ptr(), 0)
+        torch.cuda.synchronize()
+        FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+        E4 = oracle.eg4icr(ne, npb, DE, DP, rd(g4), FE, FP)
+        check_close(fe.cpu().numpy(), FE.ravel(order="F"), what="EG4 f32 FE")
+        check_close(fp.cpu().numpy(), FP.ravel(order="F"), what="EG4 f32 FP")
+        assert float(e.cpu()[0]) == pytest.approx(E4, rel=1e-11)
+
+
+def test_precision_switch(torch_dev):
+    """FP64 is selected once successive densities differ by less than the
+    threshold, and stays selected."""
+    torch = torch_dev
+    ne = 7
+    rng = np.random.default_rng(3)
+    D0 = synth.density(ne, 2.0, 9)
+    state = torch.zeros(ne * ne + 2, dtype=torch.float64, device="cuda")
+    seen = []
+    for k in range(1, 10):
+        noise = rng.uniform(-1, 1, (ne, ne))
+        D = D0 + 10.0 ** (-k) * (noise + noise.T)
+        d = _dev(torch, D.ravel(order="F"))
+        use, delta = neorx.precision_switch_dev(ne, d.data_ptr(), state.data_ptr(), 1e-5)
+        if k == 1:
+            assert delta == float("inf") and not use
+        else:
+            ref = np.max(np.abs(D - prev))
+            assert delta == pytest.approx(ref, rel=1e-15)
+        seen.append(use)
+        prev = D
+    first = seen.index(True)
+    assert all(seen[first:]) and not any(seen[:first])
+    assert 4 <= first <= 7
