@@ -139,6 +139,48 @@ int neorx_eg4icr_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const d
 int neorx_precision_switch_dev(int64_t nebf, const double *d_de, double *d_state, double threshold,
                                int64_t *use_fp64, double *delta, void *stream);
 
+/* ---- XCUHF contractions (reference semantics, source/xcuhf/) ------------ *
+ * nae, nbe: alpha / beta electron counts; dae, dbe: alpha / beta densities.
+ * FAE_* add into fae (the alpha Fock matrix; the beta one is the same call
+ * with the spins swapped, xcuhf_fock.f:221, :241); UFP_* add into fp;
+ * sae2 / sp2 and the energies are overwritten. */
+/* FAE_OMG2, source/xcuhf/xcuhf_omg2.f:2 */
+int neorx_fae_omg2(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm2, const double *gm2s, double *fae, double *sae2, double *e,
+                   double *s);
+/* UFP_OMG2, source/xcuhf/xcuhf_omg2.f:302 */
+int neorx_ufp_omg2(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm2, const double *gm2s, double *fp, double *sp2, double *e,
+                   double *s);
+/* FAE_OMG3 / UFP_OMG3, source/xcuhf/xcuhf_omg3.f:2, :284 */
+int neorx_fae_omg3(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm3, double *fae, double *e);
+int neorx_ufp_omg3(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm3, double *fp, double *e);
+/* FAE_OMG4 / UFP_OMG4, source/xcuhf/xcuhf_omg4.f:2, :512 */
+int neorx_fae_omg4(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm4, double *fae, double *e);
+int neorx_ufp_omg4(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                   const double *dp, const double *gm4, double *fp, double *e);
+/* OMG4_ICR, source/xcuhf/xcuhf_omg4.f:1080: the unsymmetrised 5-particle
+ * tensor (calc_GAM_4_integral per entry) from GAM_2s and GAM_ee */
+int neorx_omg4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, double *gm4);
+/* device-array forms (synchronise the stream) */
+int neorx_fae_omg2_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm2, const double *d_gm2s, double *d_fae, double *d_sae2,
+                       double *d_e, double *d_s, void *stream);
+int neorx_ufp_omg2_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm2, const double *d_gm2s, double *d_fp, double *d_sp2,
+                       double *d_e, double *d_s, void *stream);
+int neorx_fae_omg3_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm3, double *d_fae, double *d_e, void *stream);
+int neorx_ufp_omg3_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm3, double *d_fp, double *d_e, void *stream);
+int neorx_fae_omg4_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm4, double *d_fae, double *d_e, void *stream);
+int neorx_ufp_omg4_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                       const double *d_dp, const double *d_gm4, double *d_fp, double *d_e, void *stream);
+
 /* ---- direct-SCF routines (reference semantics) -------------------------- */
 /* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per
  * call (every primitive value through underflow(), as contract_omega2 does),

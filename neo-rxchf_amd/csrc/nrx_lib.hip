@@ -666,27 +666,35 @@ __constant__ int c_perm4[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0,
                                    {3, 2, 1, 0}, {2, 3, 1, 0}, {1, 3, 2, 0}, {3, 1, 2, 0}, {2, 1, 3, 0}, {1, 2, 3, 0}};
 __constant__ double c_w4[24] = {16, -8, 4, -8, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2};
 
+// calc_GAM_4_integral (gam_4_OMP.f:964-1192) of entry (ip,jp, br[k] bra /
+// kt[k] ket of electron k): (1/6) sum over the six electron pairs of
+// 1/2 GAM_ee(pair) GAM_2s(complement), every factor through underflow()
+NRX_DI double gam4_term(i64 ne, i64 np, const double *__restrict__ G2s, const double *__restrict__ Gee, i64 ip,
+                        i64 jp, const i64 *br, const i64 *kt)
+{
+    i64 ne2 = ne * ne;
+    i64 gb = (ip * np + jp) * ne2 * ne2;
+    i64 pr[4] = {br[0] * ne + kt[0], br[1] * ne + kt[1], br[2] * ne + kt[2], br[3] * ne + kt[3]};
+    double ee34 = Gee[pr[2] * ne2 + pr[3]], ee24 = Gee[pr[1] * ne2 + pr[3]], ee23 = Gee[pr[1] * ne2 + pr[2]];
+    double ee14 = Gee[pr[0] * ne2 + pr[3]], ee13 = Gee[pr[0] * ne2 + pr[2]], ee12 = Gee[pr[0] * ne2 + pr[1]];
+    double gs12 = G2s[gb + pr[0] * ne2 + pr[1]], gs13 = G2s[gb + pr[0] * ne2 + pr[2]];
+    double gs14 = G2s[gb + pr[0] * ne2 + pr[3]], gs23 = G2s[gb + pr[1] * ne2 + pr[2]];
+    double gs24 = G2s[gb + pr[1] * ne2 + pr[3]], gs34 = G2s[gb + pr[2] * ne2 + pr[3]];
+    underflow(ee34); underflow(ee24); underflow(ee23); underflow(ee14); underflow(ee13); underflow(ee12);
+    underflow(gs12); underflow(gs13); underflow(gs14); underflow(gs23); underflow(gs24); underflow(gs34);
+    return (0.5 * ee34 * gs12 + 0.5 * ee24 * gs13 + 0.5 * ee23 * gs14 + 0.5 * ee14 * gs23 + 0.5 * ee13 * gs24 +
+            0.5 * ee12 * gs34) /
+           6.0;
+}
+
 // symm_gam4/16 of entry (ip,jp, br[k] bra / kt[k] ket of electron k)
 NRX_DI double gam4_value(i64 ne, i64 np, const double *__restrict__ G2s, const double *__restrict__ Gee, i64 ip,
                          i64 jp, const i64 *br, const i64 *kt)
 {
-    i64 ne2 = ne * ne;
-    i64 gb = (ip * np + jp) * ne2 * ne2;
     double ans = 0.0;
     for (int p = 0; p < 24; ++p) {
-        i64 k0 = kt[c_perm4[p][0]], k1 = kt[c_perm4[p][1]], k2 = kt[c_perm4[p][2]], k3 = kt[c_perm4[p][3]];
-        i64 pr[4] = {br[0] * ne + k0, br[1] * ne + k1, br[2] * ne + k2, br[3] * ne + k3};
-        double ee34 = Gee[pr[2] * ne2 + pr[3]], ee24 = Gee[pr[1] * ne2 + pr[3]], ee23 = Gee[pr[1] * ne2 + pr[2]];
-        double ee14 = Gee[pr[0] * ne2 + pr[3]], ee13 = Gee[pr[0] * ne2 + pr[2]], ee12 = Gee[pr[0] * ne2 + pr[1]];
-        double gs12 = G2s[gb + pr[0] * ne2 + pr[1]], gs13 = G2s[gb + pr[0] * ne2 + pr[2]];
-        double gs14 = G2s[gb + pr[0] * ne2 + pr[3]], gs23 = G2s[gb + pr[1] * ne2 + pr[2]];
-        double gs24 = G2s[gb + pr[1] * ne2 + pr[3]], gs34 = G2s[gb + pr[2] * ne2 + pr[3]];
-        underflow(ee34); underflow(ee24); underflow(ee23); underflow(ee14); underflow(ee13); underflow(ee12);
-        underflow(gs12); underflow(gs13); underflow(gs14); underflow(gs23); underflow(gs24); underflow(gs34);
-        double x = (0.5 * ee34 * gs12 + 0.5 * ee24 * gs13 + 0.5 * ee23 * gs14 + 0.5 * ee14 * gs23 +
-                    0.5 * ee13 * gs24 + 0.5 * ee12 * gs34) /
-                   6.0;
-        ans += c_w4[p] * x;
+        i64 kp[4] = {kt[c_perm4[p][0]], kt[c_perm4[p][1]], kt[c_perm4[p][2]], kt[c_perm4[p][3]]};
+        ans += c_w4[p] * gam4_term(ne, np, G2s, Gee, ip, jp, br, kp);
     }
     return ans / 16.0;
 }
@@ -1674,5 +1682,395 @@ extern "C" int neorx_gam4_dscf(int64_t nebf, int64_t npbf, const double *gam_2s,
     HIPCHK(hipMemcpy(focke, dFE.p, ne * ne * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(fockp, dFP.p, np * np * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(e_gam4, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// XCUHF contractions (source/xcuhf/xcuhf_omg{2,3,4}.f).  One block per slab
+// (ip,jp,i1); each entry reads its ket-permuted partners inside the same slab
+// and forms the spin-resolved value of the reference routine.  FAE variants
+// add EFAC*t to row j1 of the slab's Fock row and DAE(i1,j1)*t to its energy
+// partial; UFP variants add t to the slab's scalar.  part[blk] = [F row (ne),
+// S row (ne), energy, overlap energy]; k_uhf_reduce applies DP(ip,jp).
+// ------------------------------------------------------------------
+// ket permutations of the 24 reads of calc_FAEOMG4 / calc_UFPOMG4, in the
+// reference's order (OMG, OMG_23, OMG_12, OMG_12_23, OMG_13_23, OMG_13,
+// OMG_24, OMG_12_24, OMG_12_14, OMG_14, OMG_34, OMG_12_34, OMG_13_34,
+// OMG_14_34, OMG_13_24, OMG_14_23, OMG_23_34, OMG_24_34, OMG_12_23_34,
+// OMG_12_23_24, OMG_12_13_34, OMG_13_23_24, OMG_14_24_34, OMG_14_23_34)
+__constant__ int c_omg4[24][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}, {1, 0, 2, 3}, {1, 2, 0, 3}, {2, 0, 1, 3}, {2, 1, 0, 3},
+                                  {0, 3, 2, 1}, {1, 3, 2, 0}, {3, 0, 2, 1}, {3, 1, 2, 0}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                  {2, 1, 3, 0}, {3, 1, 0, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}, {0, 2, 3, 1}, {0, 3, 1, 2},
+                                  {1, 2, 3, 0}, {1, 3, 0, 2}, {2, 0, 3, 1}, {2, 3, 1, 0}, {3, 0, 1, 2}, {3, 2, 0, 1}};
+// signs of the antisymmetriser over those reads, and the 6-term and 4-term
+// combinations of the mixed-spin terms (indices into the 24 reads)
+__constant__ double c_anti[24] = {1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1};
+
+enum UhfFlag {
+    U_AA = 1, U_AB = 2, U_BB = 4,              // OMG2 (FAE: AA, AB; UFP: AA, AB, BB)
+    U_AAA = 8, U_BBB = 16, U_AAB = 32, U_BBA = 64, // OMG3/OMG4 (FAE4: AAA, BBB, AAB, BBA; UFP3 likewise)
+    U_4A = 128, U_4B = 256, U_3A1B = 512, U_1A3B = 1024, U_2A2B = 2048, // UFP4
+};
+
+template <int N, bool FAE, class T>
+__global__ void __launch_bounds__(256) k_uhf_slab(i64 ne, i64 np, int flags, const double *__restrict__ DA,
+                                                  const double *__restrict__ DB, const T *__restrict__ G,
+                                                  const T *__restrict__ Gs, double *__restrict__ part)
+{
+    extern __shared__ double sm[]; // [2 ne + 2]
+    i64 blk = blockIdx.x, a = blk % ne, pp = blk / ne;
+    i64 ne2 = ne * ne;
+    i64 nrem = 1, nblock = 1;
+    for (int k = 0; k < 2 * N - 1; ++k) nrem *= ne;
+    nblock = nrem * ne;
+    const T *Gb = G + pp * nblock;
+    const T *Gsb = Gs ? Gs + pp * nblock : nullptr;
+    for (int k = threadIdx.x; k < 2 * ne + 2; k += blockDim.x) sm[k] = 0.0;
+    __syncthreads();
+    double pe = 0.0, ps = 0.0;
+#define DA_(i, j) DA[(j) * ne + (i)]
+#define DB_(i, j) DB[(j) * ne + (i)]
+    for (i64 r0 = threadIdx.x; r0 < nrem; r0 += blockDim.x) {
+        i64 ii[N], jj[N], r = r0;
+        for (int k = N - 1; k >= 1; --k) {
+            jj[k] = r % ne;
+            r /= ne;
+            ii[k] = r % ne;
+            r /= ne;
+        }
+        jj[0] = r;
+        ii[0] = a;
+        // read with ket of electron k = jj[perm[k]]
+        auto rd = [&](const T *B, const int *perm) -> double {
+            i64 off = 0;
+            for (int k = 0; k < N; ++k) off = off * ne2 + ii[k] * ne + jj[perm[k]];
+            return (double)B[off];
+        };
+        double t = 0.0, ts = 0.0;
+        if constexpr (N == 2) {
+            const int p12[2] = {0, 1}, p21[2] = {1, 0};
+            double o12 = rd(Gb, p12), o21 = rd(Gb, p21), so12 = rd(Gsb, p12), so21 = rd(Gsb, p21);
+            if constexpr (FAE) {
+                double c1 = (flags & U_AA) ? DA_(ii[1], jj[1]) : 0.0, c2 = (flags & U_AB) ? DB_(ii[1], jj[1]) : 0.0;
+                t = c1 * (o12 - o21) + c2 * o12;
+                ts = c1 * (so12 - so21) + c2 * so12;
+            } else {
+                double a1 = DA_(ii[0], jj[0]), b1 = DB_(ii[0], jj[0]), a2 = DA_(ii[1], jj[1]), b2 = DB_(ii[1], jj[1]);
+                double c1 = (flags & U_AB) ? a1 * b2 + b1 * a2 : 0.0;
+                double c2 = (flags & U_AA) ? a1 * a2 : 0.0, c3 = (flags & U_BB) ? b1 * b2 : 0.0;
+                t = c1 * o12 + (c2 + c3) * (o12 - o21);
+                ts = c1 * so12 + (c2 + c3) * (so12 - so21);
+            }
+        } else if constexpr (N == 3) {
+            const int p123[3] = {0, 1, 2}, p213[3] = {1, 0, 2}, p312[3] = {2, 0, 1}, p132[3] = {0, 2, 1},
+                      p231[3] = {1, 2, 0}, p321[3] = {2, 1, 0};
+            double g123 = rd(Gb, p123), g213 = rd(Gb, p213), g312 = rd(Gb, p312), g132 = rd(Gb, p132),
+                   g231 = rd(Gb, p231), g321 = rd(Gb, p321);
+            double o6 = g123 - g213 + g312 - g132 + g231 - g321;
+            double o23 = g123 - g132, o12 = g123 - g213, o13 = g123 - g321;
+            double a1 = DA_(ii[0], jj[0]), b1 = DB_(ii[0], jj[0]), a2 = DA_(ii[1], jj[1]), b2 = DB_(ii[1], jj[1]);
+            double a3 = DA_(ii[2], jj[2]), b3 = DB_(ii[2], jj[2]);
+            if constexpr (FAE) {
+                // FAE_OMG3 flags: U_AAA = LAA (NAE>=3), U_BBB = LBB (NBE>=2), U_AAB = LAB
+                double xaa = (flags & U_AAA) ? a2 * a3 * o6 : 0.0;
+                double xbb = (flags & U_BBB) ? b2 * b3 * o23 : 0.0;
+                double xab = (flags & U_AAB) ? a2 * b3 * o12 : 0.0;
+                double xba = (flags & U_AAB) ? b2 * a3 * o13 : 0.0;
+                t = xaa + xbb + xab + xba;
+            } else {
+                double caaa = (flags & U_AAA) ? a1 * a2 * a3 : 0.0, cbbb = (flags & U_BBB) ? b1 * b2 * b3 : 0.0;
+                double caab = 0, caba = 0, cbaa = 0, cbba = 0, cbab = 0, cabb = 0;
+                if (flags & U_AAB) {
+                    caab = a1 * a2 * b3;
+                    caba = a1 * b2 * a3;
+                    cbaa = b1 * a2 * a3;
+                }
+                if (flags & U_BBA) {
+                    cbba = b1 * b2 * a3;
+                    cbab = b1 * a2 * b3;
+                    cabb = a1 * b2 * b3;
+                }
+                t = (caaa + cbbb) * o6 + (caab + cbba) * o12 + (caba + cbab) * o13 + (cabb + cbaa) * o23;
+            }
+        } else {
+            double o[24];
+            for (int p = 0; p < 24; ++p) o[p] = rd(Gb, c_omg4[p]);
+            double anti = 0.0;
+            for (int p = 0; p < 24; ++p) anti += c_anti[p] * o[p];
+            // 0 OMG 1 23 2 12 3 12_23 4 13_23 5 13 6 24 7 12_24 8 12_14 9 14 10 34 11 12_34 12 13_34 13 14_34
+            // 14 13_24 15 14_23 16 23_34 17 24_34
+            double aab = o[0] - o[1] - o[2] + o[3] + o[4] - o[5];
+            double aba = o[0] - o[6] - o[2] + o[7] + o[8] - o[9];
+            double baa = o[0] - o[10] - o[5] + o[12] + o[13] - o[9];
+            double abb = o[0] - o[10] - o[2] + o[11];
+            double bab = o[0] - o[6] - o[5] + o[14];
+            double bba = o[0] - o[1] - o[9] + o[15];
+            double bbb = o[0] - o[10] - o[1] + o[16] + o[17] - o[6];
+            double A[4], B[4];
+            for (int k = 0; k < 4; ++k) {
+                A[k] = DA_(ii[k], jj[k]);
+                B[k] = DB_(ii[k], jj[k]);
+            }
+            if constexpr (FAE) {
+                double x = 0.0;
+                x += (flags & U_AAA) ? A[1] * A[2] * A[3] * anti : 0.0;
+                x += (flags & U_AAB) ? A[1] * A[2] * B[3] * aab : 0.0;
+                x += (flags & U_AAB) ? A[1] * B[2] * A[3] * aba : 0.0;
+                x += (flags & U_AAB) ? B[1] * A[2] * A[3] * baa : 0.0;
+                x += (flags & U_BBA) ? A[1] * B[2] * B[3] * abb : 0.0;
+                x += (flags & U_BBA) ? B[1] * A[2] * B[3] * bab : 0.0;
+                x += (flags & U_BBA) ? B[1] * B[2] * A[3] * bba : 0.0;
+                x += (flags & U_BBB) ? B[1] * B[2] * B[3] * bbb : 0.0;
+                t = x;
+            } else {
+                double x = 0.0;
+                x += (flags & U_4A) ? A[0] * A[1] * A[2] * A[3] * anti : 0.0;
+                x += (flags & U_3A1B) ? A[0] * A[1] * A[2] * B[3] * aab : 0.0;
+                x += (flags & U_3A1B) ? A[0] * A[1] * B[2] * A[3] * aba : 0.0;
+                x += (flags & U_3A1B) ? A[0] * B[1] * A[2] * A[3] * baa : 0.0;
+                x += (flags & U_2A2B) ? A[0] * A[1] * B[2] * B[3] * abb : 0.0;
+                x += (flags & U_2A2B) ? A[0] * B[1] * A[2] * B[3] * bab : 0.0;
+                x += (flags & U_2A2B) ? A[0] * B[1] * B[2] * A[3] * bba : 0.0;
+                x += (flags & U_1A3B) ? A[0] * B[1] * B[2] * B[3] * bbb : 0.0;
+                x += (flags & U_4B) ? B[0] * B[1] * B[2] * B[3] * anti : 0.0;
+                x += (flags & U_1A3B) ? B[0] * B[1] * B[2] * A[3] * aab : 0.0;
+                x += (flags & U_1A3B) ? B[0] * B[1] * A[2] * B[3] * aba : 0.0;
+                x += (flags & U_1A3B) ? B[0] * A[1] * B[2] * B[3] * baa : 0.0;
+                x += (flags & U_2A2B) ? B[0] * B[1] * A[2] * A[3] * abb : 0.0;
+                x += (flags & U_2A2B) ? B[0] * A[1] * B[2] * A[3] * bab : 0.0;
+                x += (flags & U_2A2B) ? B[0] * A[1] * A[2] * B[3] * bba : 0.0;
+                x += (flags & U_3A1B) ? B[0] * A[1] * A[2] * A[3] * bbb : 0.0;
+                t = x;
+            }
+        }
+        if constexpr (FAE) {
+            atomicAdd(&sm[jj[0]], (double)N * t);
+            if (N == 2) atomicAdd(&sm[ne + jj[0]], 2.0 * ts);
+            pe += DA_(a, jj[0]) * t;
+            ps += DA_(a, jj[0]) * ts;
+        } else {
+            pe += t;
+            ps += ts;
+        }
+    }
+#undef DA_
+#undef DB_
+    pe = wave_sum(pe);
+    ps = wave_sum(ps);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&sm[2 * ne], pe);
+        atomicAdd(&sm[2 * ne + 1], ps);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * ne + 2; k += blockDim.x) part[blk * (2 * ne + 2) + k] = sm[k];
+}
+
+// FAE: FE(a,b) += sum_pp DP(pp) F_pp,a[b], SE(a,b) = sum_pp DP(pp) S_pp,a[b];
+// UFP: FP(pp) += sum_a e_pp,a, SP(pp) = sum_a s_pp,a;
+// both: E = sum_pp DP(pp) sum_a e_pp,a, S likewise
+__global__ void k_uhf_reduce(i64 ne, i64 np, const double *__restrict__ DP, const double *__restrict__ part, int fae,
+                             double *__restrict__ FE, double *__restrict__ SE, double *__restrict__ FP,
+                             double *__restrict__ SP, double *__restrict__ E, double *__restrict__ S)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 ne2 = ne * ne, np2 = np * np, w = 2 * ne + 2;
+    if (fae && t < ne2) {
+        i64 a = t % ne, b = t / ne;
+        double f = 0.0, fs = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) {
+            double dp = DP[(pp % np) * np + pp / np];
+            f += dp * part[(pp * ne + a) * w + b];
+            fs += dp * part[(pp * ne + a) * w + ne + b];
+        }
+        FE[b * ne + a] += f;
+        if (SE) SE[b * ne + a] = fs;
+    }
+    if (!fae && t < np2) {
+        i64 ip = t / np, jp = t % np;
+        double f = 0.0, fs = 0.0;
+        for (i64 a = 0; a < ne; ++a) {
+            f += part[(t * ne + a) * w + 2 * ne];
+            fs += part[(t * ne + a) * w + 2 * ne + 1];
+        }
+        FP[jp * np + ip] += f;
+        if (SP) SP[jp * np + ip] = fs;
+    }
+    if (t == 0) {
+        double e = 0.0, s = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) {
+            double dp = DP[(pp % np) * np + pp / np], pe = 0.0, ps = 0.0;
+            for (i64 a = 0; a < ne; ++a) {
+                pe += part[(pp * ne + a) * w + 2 * ne];
+                ps += part[(pp * ne + a) * w + 2 * ne + 1];
+            }
+            e += dp * pe;
+            s += dp * ps;
+        }
+        *E = e;
+        if (S) *S = s;
+    }
+}
+
+template <int N, bool FAE>
+static int uhf_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                   const double *d_dp, const double *d_g, const double *d_gs, double *d_f, double *d_s2, double *d_e,
+                   double *d_s, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0 || nae < 0 || nbe < 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (!d_dae || !d_dbe || !d_dp || !d_g || !d_f || !d_e || (N == 2 && (!d_gs || !d_s2 || !d_s)))
+        return fail(NEORX_E_ARG, "null argument");
+    int fl = 0;
+    if (N == 2 && FAE) fl = (nae >= 2 ? U_AA : 0) | (nae >= 1 && nbe >= 1 ? U_AB : 0);
+    if (N == 2 && !FAE) fl = (nae >= 1 && nbe >= 1 ? U_AB : 0) | (nae >= 2 ? U_AA : 0) | (nbe >= 2 ? U_BB : 0);
+    if (N == 3 && FAE) fl = (nbe >= 1 && nae >= 2 ? U_AAB : 0) | (nbe >= 2 ? U_BBB : 0) | (nae >= 3 ? U_AAA : 0);
+    if (N == 3 && !FAE)
+        fl = (nae >= 3 ? U_AAA : 0) | (nbe >= 3 ? U_BBB : 0) | (nae >= 2 && nbe >= 1 ? U_AAB : 0) |
+             (nbe >= 2 && nae >= 1 ? U_BBA : 0);
+    if (N == 4 && FAE)
+        fl = (nae >= 4 ? U_AAA : 0) | (nbe >= 3 ? U_BBB : 0) | (nae >= 3 && nbe >= 1 ? U_AAB : 0) |
+             (nbe >= 2 && nae >= 2 ? U_BBA : 0);
+    if (N == 4 && !FAE)
+        fl = (nae >= 4 ? U_4A : 0) | (nbe >= 4 ? U_4B : 0) | (nae >= 3 && nbe >= 1 ? U_3A1B : 0) |
+             (nae >= 1 && nbe >= 3 ? U_1A3B : 0) | (nae >= 2 && nbe >= 2 ? U_2A2B : 0);
+    hipStream_t st = (hipStream_t)stream;
+    i64 nblk = np * np * ne, w = 2 * ne + 2;
+    DevBuf part;
+    rc = part.alloc(nblk * w * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_uhf_slab<N, FAE, double>), dim3((unsigned)nblk), dim3(256), w * 8, st, ne, np, fl, d_dae,
+                       d_dbe, d_g, N == 2 ? d_gs : nullptr, part.d());
+    hipLaunchKernelGGL(k_uhf_reduce, dim3(nblocks(std::max(ne * ne, np * np), 256)), dim3(256), 0, st, ne, np, d_dp,
+                       part.d(), FAE ? 1 : 0, FAE ? d_f : nullptr, FAE ? d_s2 : nullptr, FAE ? nullptr : d_f,
+                       FAE ? nullptr : d_s2, d_e, N == 2 ? d_s : nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+#define UHF_DEV(NAME, N, FAE)                                                                                      \
+    extern "C" int NAME(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae,                \
+                        const double *d_dbe, const double *d_dp, const double *d_g, double *d_f, double *d_e,     \
+                        void *stream)                                                                             \
+    {                                                                                                             \
+        return uhf_dev<N, FAE>(nae, nbe, nebf, npbf, d_dae, d_dbe, d_dp, d_g, nullptr, d_f, nullptr, d_e, nullptr, \
+                               stream);                                                                           \
+    }
+UHF_DEV(neorx_fae_omg3_dev, 3, true)
+UHF_DEV(neorx_ufp_omg3_dev, 3, false)
+UHF_DEV(neorx_fae_omg4_dev, 4, true)
+UHF_DEV(neorx_ufp_omg4_dev, 4, false)
+#undef UHF_DEV
+
+extern "C" int neorx_fae_omg2_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae,
+                                  const double *d_dbe, const double *d_dp, const double *d_gm2, const double *d_gm2s,
+                                  double *d_fae, double *d_sae2, double *d_e, double *d_s, void *stream)
+{
+    return uhf_dev<2, true>(nae, nbe, nebf, npbf, d_dae, d_dbe, d_dp, d_gm2, d_gm2s, d_fae, d_sae2, d_e, d_s, stream);
+}
+
+extern "C" int neorx_ufp_omg2_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae,
+                                  const double *d_dbe, const double *d_dp, const double *d_gm2, const double *d_gm2s,
+                                  double *d_fp, double *d_sp2, double *d_e, double *d_s, void *stream)
+{
+    return uhf_dev<2, false>(nae, nbe, nebf, npbf, d_dae, d_dbe, d_dp, d_gm2, d_gm2s, d_fp, d_sp2, d_e, d_s, stream);
+}
+
+// host-array forms, as the Fortran routines take them
+template <int N, bool FAE>
+static int uhf_host(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                    const double *dp, const double *g, const double *gs, double *f, double *s2, double *e, double *s)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf, ng = np * np;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    for (int k = 0; k < 2 * N; ++k) ng *= ne;
+    i64 nf = FAE ? ne * ne : np * np;
+    DevBuf dA, dB, dP, dG, dGs, dF, dS2, dE;
+    rc = h2d(dA, dae, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(dG, g, ng);
+    if (!rc && N == 2) rc = h2d(dGs, gs, ng);
+    if (!rc) rc = h2d(dF, f, nf);
+    if (!rc && N == 2) rc = dS2.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(16);
+    if (!rc)
+        rc = uhf_dev<N, FAE>(nae, nbe, ne, np, dA.d(), dB.d(), dP.d(), dG.d(), N == 2 ? dGs.d() : nullptr, dF.d(),
+                             N == 2 ? dS2.d() : nullptr, dE.d(), N == 2 ? dE.d() + 1 : nullptr, 0);
+    if (rc) return rc;
+    double es[2];
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    if (N == 2) HIPCHK(hipMemcpy(s2, dS2.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e = es[0];
+    if (N == 2) *s = es[1];
+    return NEORX_OK;
+}
+
+extern "C" int neorx_fae_omg2(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae,
+                              const double *dbe, const double *dp, const double *gm2, const double *gm2s, double *fae,
+                              double *sae2, double *e, double *s)
+{
+    return uhf_host<2, true>(nae, nbe, nebf, npbf, dae, dbe, dp, gm2, gm2s, fae, sae2, e, s);
+}
+extern "C" int neorx_ufp_omg2(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae,
+                              const double *dbe, const double *dp, const double *gm2, const double *gm2s, double *fp,
+                              double *sp2, double *e, double *s)
+{
+    return uhf_host<2, false>(nae, nbe, nebf, npbf, dae, dbe, dp, gm2, gm2s, fp, sp2, e, s);
+}
+#define UHF_HOST(NAME, N, FAE)                                                                                     \
+    extern "C" int NAME(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *dae, const double *dbe, \
+                        const double *dp, const double *g, double *f, double *e)                                  \
+    {                                                                                                             \
+        return uhf_host<N, FAE>(nae, nbe, nebf, npbf, dae, dbe, dp, g, nullptr, f, nullptr, e, nullptr);          \
+    }
+UHF_HOST(neorx_fae_omg3, 3, true)
+UHF_HOST(neorx_ufp_omg3, 3, false)
+UHF_HOST(neorx_fae_omg4, 4, true)
+UHF_HOST(neorx_ufp_omg4, 4, false)
+#undef UHF_HOST
+
+// OMG4_ICR (xcuhf_omg4.f:1080-1320): the unsymmetrised 5-particle value
+__global__ void __launch_bounds__(256) k_omg4(i64 ne, i64 np, i64 off, i64 n, const double *__restrict__ G2s,
+                                              const double *__restrict__ Gee, double *__restrict__ G4)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t;
+    i64 br[4], kt[4];
+    for (int k = 3; k >= 0; --k) {
+        kt[k] = r % ne;
+        r /= ne;
+        br[k] = r % ne;
+        r /= ne;
+    }
+    i64 jp = r % np, ip = r / np;
+    G4[off + t] = gam4_term(ne, np, G2s, Gee, ip, jp, br, kt);
+}
+
+extern "C" int neorx_omg4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, double *gm4)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng2 = ne * ne * ne * ne * np * np, ngee = ne * ne * ne * ne, ng4 = ngee * ngee * np * np;
+    DevBuf A, B, C;
+    rc = h2d(A, gam_2s, ng2);
+    if (!rc) rc = h2d(B, gam_ee, ngee);
+    if (!rc) rc = C.alloc(ng4 * 8);
+    if (rc) return rc;
+    for (i64 off = 0; off < ng4; off += CHUNK) {
+        i64 n = std::min(CHUNK, ng4 - off);
+        hipLaunchKernelGGL(k_omg4, dim3(nblocks(n, 256)), dim3(256), 0, 0, ne, np, off, n, A.d(), B.d(), C.d());
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(gm4, C.p, ng4 * 8, hipMemcpyDeviceToHost));
     return NEORX_OK;
 }

@@ -106,6 +106,48 @@ module neorx_capi
       real(c_double) :: g2s(*), gee(*), de(*), dp(*), fe(*), fp(*), e
       integer(c_int) :: neorx_gam4_dscf
     end function
+    function neorx_fae_omg2(nae, nbe, ne, np, da, db, dp, g, gs, f, s2, e, s) bind(C, name="neorx_fae_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gs(*), f(*), s2(*), e, s
+      integer(c_int) :: neorx_fae_omg2
+    end function
+    function neorx_ufp_omg2(nae, nbe, ne, np, da, db, dp, g, gs, f, s2, e, s) bind(C, name="neorx_ufp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gs(*), f(*), s2(*), e, s
+      integer(c_int) :: neorx_ufp_omg2
+    end function
+    function neorx_fae_omg3(nae, nbe, ne, np, da, db, dp, g, f, e) bind(C, name="neorx_fae_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_fae_omg3
+    end function
+    function neorx_ufp_omg3(nae, nbe, ne, np, da, db, dp, g, f, e) bind(C, name="neorx_ufp_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_ufp_omg3
+    end function
+    function neorx_fae_omg4(nae, nbe, ne, np, da, db, dp, g, f, e) bind(C, name="neorx_fae_omg4")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_fae_omg4
+    end function
+    function neorx_ufp_omg4(nae, nbe, ne, np, da, db, dp, g, f, e) bind(C, name="neorx_ufp_omg4")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nae, nbe, ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_ufp_omg4
+    end function
+    function neorx_omg4_icr(ne, np, g2s, gee, g4) bind(C, name="neorx_omg4_icr")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: g2s(*), gee(*), g4(*)
+      integer(c_int) :: neorx_omg4_icr
+    end function
   end interface
 contains
   subroutine neorx_check(rc, where)
@@ -340,4 +382,86 @@ subroutine GAM4_DSCF(NG4CHK, ne, np, ngee, ng2, ng4, DE, DP, focke, fockp, E_gam
   close(804)
   call neorx_check(neorx_gam4_dscf(ne, np, g2s, gee, DE, DP, focke, fockp, E_gam4), 'GAM4_DSCF')
   deallocate(gee, g2s)
+end subroutine
+
+! ---------------------------------------------------------------------
+! XCUHF contractions (source/xcuhf/xcuhf_omg{2,3,4}.f)
+! ---------------------------------------------------------------------
+! FAE_OMG2, source/xcuhf/xcuhf_omg2.f:2-136 (DETOT is not used by it)
+subroutine FAE_OMG2(Nchunks, NAE, NBE, nebf, npbf, ng2, DAE, DBE, DETOT, DP, GM2ICR, GM2SICR, FAE, SAE2, &
+                    E_AE_OMG2, S_AE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, NAE, NBE, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DETOT(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2SICR(ng2)
+  real(c_double) :: FAE(nebf, nebf), SAE2(nebf, nebf), E_AE_OMG2, S_AE_OMG2
+  call neorx_check(neorx_fae_omg2(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM2ICR, GM2SICR, FAE, SAE2, E_AE_OMG2, &
+                   S_AE_OMG2), 'FAE_OMG2')
+end subroutine
+
+! UFP_OMG2, source/xcuhf/xcuhf_omg2.f:302-440
+subroutine UFP_OMG2(Nchunks, nebf, npbf, ng2, NAE, NBE, DAE, DBE, DETOT, DP, GM2ICR, GM2SICR, FP, SP2, &
+                    E_P_OMG2, S_P_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DETOT(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2SICR(ng2)
+  real(c_double) :: FP(npbf, npbf), SP2(npbf, npbf), E_P_OMG2, S_P_OMG2
+  call neorx_check(neorx_ufp_omg2(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM2ICR, GM2SICR, FP, SP2, E_P_OMG2, &
+                   S_P_OMG2), 'UFP_OMG2')
+end subroutine
+
+! FAE_OMG3, source/xcuhf/xcuhf_omg3.f:2-128
+subroutine FAE_OMG3(Nchunks, nebf, npbf, ng3, NAE, NBE, DAE, DBE, DP, GM3ICR, FAE, E_AE_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), FAE(nebf, nebf), E_AE_OMG3
+  call neorx_check(neorx_fae_omg3(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM3ICR, FAE, E_AE_OMG3), 'FAE_OMG3')
+end subroutine
+
+! UFP_OMG3, source/xcuhf/xcuhf_omg3.f:284-407
+subroutine UFP_OMG3(Nchunks, nebf, npbf, ng3, NAE, NBE, DAE, DBE, DP, GM3ICR, FP, E_P_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), FP(npbf, npbf), E_P_OMG3
+  call neorx_check(neorx_ufp_omg3(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM3ICR, FP, E_P_OMG3), 'UFP_OMG3')
+end subroutine
+
+! FAE_OMG4, source/xcuhf/xcuhf_omg4.f:2-124
+subroutine FAE_OMG4(Nchunks, nebf, npbf, ng4, NAE, NBE, DAE, DBE, DP, GM4ICR, FAE, E_AE_OMG4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng4, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM4ICR(ng4), FAE(nebf, nebf), E_AE_OMG4
+  call neorx_check(neorx_fae_omg4(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM4ICR, FAE, E_AE_OMG4), 'FAE_OMG4')
+end subroutine
+
+! UFP_OMG4, source/xcuhf/xcuhf_omg4.f:512-636
+subroutine UFP_OMG4(Nchunks, nebf, npbf, ng4, NAE, NBE, DAE, DBE, DP, GM4ICR, FP, E_P_OMG4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng4, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM4ICR(ng4), FP(npbf, npbf), E_P_OMG4
+  call neorx_check(neorx_ufp_omg4(NAE, NBE, nebf, npbf, DAE, DBE, DP, GM4ICR, FP, E_P_OMG4), 'UFP_OMG4')
+end subroutine
+
+! OMG4_ICR, source/xcuhf/xcuhf_omg4.f:1080-1210: reads GAM_ee.ufm from the
+! working directory like read_GAM_ee
+subroutine OMG4_ICR(Nchunks, ne, np, ngee, ng2, ng4, GAM_2s, GAM_4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, ne, np, ngee, ng2, ng4
+  real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
+  real(c_double), allocatable :: gee(:)
+  integer(c_int64_t) :: ia
+  allocate(gee(ngee))
+  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do ia = 1, ngee
+    read(810, rec=ia) gee(ia)
+  end do
+  close(810)
+  call neorx_check(neorx_omg4_icr(ne, np, GAM_2s, gee, GAM_4), 'OMG4_ICR')
+  deallocate(gee)
 end subroutine

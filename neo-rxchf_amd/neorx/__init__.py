@@ -72,6 +72,11 @@ def load():
     _sig("neorx_eg3ic1_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_eg4icr_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_precision_switch_dev", [_I64, _P, _P, _D, _P, _P, _P])
+    _sig("neorx_fae_omg2", [_I64] * 4 + [_P] * 9)
+    _sig("neorx_ufp_omg2", [_I64] * 4 + [_P] * 9)
+    for n in ("fae_omg3", "ufp_omg3", "fae_omg4", "ufp_omg4"):
+        _sig("neorx_" + n, [_I64] * 4 + [_P] * 6)
+    _sig("neorx_omg4_icr", [_I64, _I64, _P, _P, _P])
     _sig("neorx_gam2_dscf", [_I64] + [_P] * 8)
     _sig("neorx_gam3_dscf", [_I64] + [_P] * 5)
     _sig("neorx_gam4_dscf", [_I64, _I64] + [_P] * 7)
@@ -254,3 +259,68 @@ def call_dev(name, *args):
     """Call a ``neorx_*_dev`` entry point with raw device pointers / ints."""
     load()
     _check(getattr(_lib, name)(*args))
+
+
+def _fort(a):
+    return _np.asfortranarray(a, dtype=_np.float64)
+
+
+def FAE_OMG2(nae, nbe, nebf, npbf, DAE, DBE, DP, GM2ICR, GM2SICR, FAE):
+    """FAE_OMG2 (source/xcuhf/xcuhf_omg2.f:2): FAE accumulates in place;
+    returns (SAE2, E, S)."""
+    load()
+    SAE2 = _np.zeros((nebf, nebf), order="F")
+    e = _np.zeros(2)
+    _check(_lib.neorx_fae_omg2(nae, nbe, nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
+                               _p(_np.ascontiguousarray(GM2ICR)), _p(_np.ascontiguousarray(GM2SICR)), _p(FAE),
+                               _p(SAE2), _p(e), _p(e[1:])))
+    return SAE2, float(e[0]), float(e[1])
+
+
+def UFP_OMG2(nae, nbe, nebf, npbf, DAE, DBE, DP, GM2ICR, GM2SICR, FP):
+    """UFP_OMG2 (source/xcuhf/xcuhf_omg2.f:302): FP accumulates; returns
+    (SP2, E, S)."""
+    load()
+    SP2 = _np.zeros((npbf, npbf), order="F")
+    e = _np.zeros(2)
+    _check(_lib.neorx_ufp_omg2(nae, nbe, nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
+                               _p(_np.ascontiguousarray(GM2ICR)), _p(_np.ascontiguousarray(GM2SICR)), _p(FP),
+                               _p(SP2), _p(e), _p(e[1:])))
+    return SP2, float(e[0]), float(e[1])
+
+
+def _uhf(name, nae, nbe, nebf, npbf, DAE, DBE, DP, G, F):
+    load()
+    e = _np.zeros(1)
+    _check(getattr(_lib, name)(nae, nbe, nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
+                               _p(_np.ascontiguousarray(G)), _p(F), _p(e)))
+    return float(e[0])
+
+
+def FAE_OMG3(*a):
+    """FAE_OMG3 (source/xcuhf/xcuhf_omg3.f:2): returns E."""
+    return _uhf("neorx_fae_omg3", *a)
+
+
+def UFP_OMG3(*a):
+    """UFP_OMG3 (source/xcuhf/xcuhf_omg3.f:284): returns E."""
+    return _uhf("neorx_ufp_omg3", *a)
+
+
+def FAE_OMG4(*a):
+    """FAE_OMG4 (source/xcuhf/xcuhf_omg4.f:2): returns E."""
+    return _uhf("neorx_fae_omg4", *a)
+
+
+def UFP_OMG4(*a):
+    """UFP_OMG4 (source/xcuhf/xcuhf_omg4.f:512): returns E."""
+    return _uhf("neorx_ufp_omg4", *a)
+
+
+def OMG4_ICR(nebf, npbf, gam_2s, gam_ee):
+    """OMG4_ICR (source/xcuhf/xcuhf_omg4.f:1080) with GAM_ee passed in."""
+    load()
+    g = _np.zeros(nebf ** 8 * npbf * npbf)
+    _check(_lib.neorx_omg4_icr(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
+                               _p(g)))
+    return g

@@ -1312,3 +1312,292 @@ void orc_gam4_dscf(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee,
     free(xfe);
     free(xfp);
 }
+
+/* ------------------------------------------------------------------------
+ * XCUHF contractions (source/xcuhf/xcuhf_omg{2,3,4}.f): spin-resolved
+ * contractions of the same in-core tensors.  FAE_* add into FAE (the alpha
+ * Fock matrix; the beta one is the same call with the spins swapped,
+ * xcuhf_fock.f:221, :241); UFP_* add into FP.
+ * ------------------------------------------------------------------------ */
+#define DAm(i, j) DAE[(j) * ne + (i)]
+#define DBm(i, j) DBE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+
+/* FAE_OMG2 / thread_FAEOMG2 (xcuhf_omg2.f:2-300); SAE2 overwritten */
+void orc_fae_omg2(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM2, const double *GM2s, double *FAE, double *SAE2, double *E, double *S)
+{
+    i64 ne = nebf, np = npbf;
+    int LAA = nae >= 2, LAB = nae >= 1 && nbe >= 1;
+    double *xf = (double *)calloc(ne * ne, sizeof(double));
+    memset(SAE2, 0, sizeof(double) * ne * ne);
+    double e = 0.0, s = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i1, j2, i2, j1);
+                            double o12 = GM2[a12], o21 = GM2[a21], so12 = GM2s[a12], so21 = GM2s[a21];
+                            double c1 = LAA ? DAm(i2, j2) * DPm(ip, jp) : 0.0;
+                            double c2 = LAB ? DBm(i2, j2) * DPm(ip, jp) : 0.0;
+                            xf[j1 * ne + i1] += 2.0 * (c1 * (o12 - o21) + c2 * o12);
+                            SAE2[j1 * ne + i1] += 2.0 * (c1 * (so12 - so21) + c2 * so12);
+                            e += DAm(i1, j1) * (c1 * (o12 - o21) + c2 * o12);
+                            s += DAm(i1, j1) * (c1 * (so12 - so21) + c2 * so12);
+                        }
+    for (i64 k = 0; k < ne * ne; ++k) FAE[k] += xf[k];
+    *E = e;
+    *S = s;
+    free(xf);
+}
+
+/* UFP_OMG2 / thread_UFPOMG2 (xcuhf_omg2.f:302-560); SP2 overwritten */
+void orc_ufp_omg2(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM2, const double *GM2s, double *FP, double *SP2, double *E, double *S)
+{
+    i64 ne = nebf, np = npbf;
+    int LAB = nae >= 1 && nbe >= 1, LAA = nae >= 2, LBB = nbe >= 2;
+    double *xf = (double *)calloc(np * np, sizeof(double));
+    memset(SP2, 0, sizeof(double) * np * np);
+    double e = 0.0, s = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i1, j2, i2, j1);
+                            double o12 = GM2[a12], o21 = GM2[a21], so12 = GM2s[a12], so21 = GM2s[a21];
+                            double c1 = LAB ? DAm(i1, j1) * DBm(i2, j2) + DBm(i1, j1) * DAm(i2, j2) : 0.0;
+                            double c2 = LAA ? DAm(i1, j1) * DAm(i2, j2) : 0.0;
+                            double c3 = LBB ? DBm(i1, j1) * DBm(i2, j2) : 0.0;
+                            double v = c1 * o12 + (c2 + c3) * (o12 - o21), vs = c1 * so12 + (c2 + c3) * (so12 - so21);
+                            xf[jp * np + ip] += v;
+                            SP2[jp * np + ip] += vs;
+                            e += DPm(ip, jp) * v;
+                            s += DPm(ip, jp) * vs;
+                        }
+    for (i64 k = 0; k < np * np; ++k) FP[k] += xf[k];
+    *E = e;
+    *S = s;
+    free(xf);
+}
+
+/* the six ket-permuted reads of thread_FAEOMG3 / thread_UFPOMG3 */
+static void omg3_reads(i64 ne, i64 np, const double *G, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i64 j2, i64 i3, i64 j3,
+                       double *o6, double *o23, double *o12, double *o13)
+{
+    double g123 = G[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)], g213 = G[ix3(ne, np, ip, jp, i1, j2, i2, j1, i3, j3)];
+    double g312 = G[ix3(ne, np, ip, jp, i1, j3, i2, j1, i3, j2)], g132 = G[ix3(ne, np, ip, jp, i1, j1, i2, j3, i3, j2)];
+    double g231 = G[ix3(ne, np, ip, jp, i1, j2, i2, j3, i3, j1)], g321 = G[ix3(ne, np, ip, jp, i1, j3, i2, j2, i3, j1)];
+    *o6 = g123 - g213 + g312 - g132 + g231 - g321;
+    *o23 = g123 - g132;
+    *o12 = g123 - g213;
+    *o13 = g123 - g321;
+}
+
+#define LOOP3                                                                                                      \
+    for (i64 ip = 0; ip < np; ++ip)                                                                                \
+        for (i64 jp = 0; jp < np; ++jp)                                                                            \
+            for (i64 i1 = 0; i1 < ne; ++i1)                                                                        \
+                for (i64 j1 = 0; j1 < ne; ++j1)                                                                    \
+                    for (i64 i2 = 0; i2 < ne; ++i2)                                                                \
+                        for (i64 j2 = 0; j2 < ne; ++j2)                                                            \
+                            for (i64 i3 = 0; i3 < ne; ++i3)                                                        \
+                                for (i64 j3 = 0; j3 < ne; ++j3)
+
+/* FAE_OMG3 / thread_FAEOMG3 (xcuhf_omg3.f:2-282) */
+void orc_fae_omg3(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM3, double *FAE, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    int LAB = nbe >= 1 && nae >= 2, LBB = nbe >= 2, LAA = nae >= 3;
+    double *xf = (double *)calloc(ne * ne, sizeof(double));
+    double e = 0.0;
+    LOOP3
+    {
+        double o6, o23, o12, o13;
+        omg3_reads(ne, np, GM3, ip, jp, i1, j1, i2, j2, i3, j3, &o6, &o23, &o12, &o13);
+        double xaa = LAA ? DAm(i2, j2) * DAm(i3, j3) * o6 : 0.0;
+        double xbb = LBB ? DBm(i2, j2) * DBm(i3, j3) * o23 : 0.0;
+        double xab = LAB ? DAm(i2, j2) * DBm(i3, j3) * o12 : 0.0;
+        double xba = LAB ? DBm(i2, j2) * DAm(i3, j3) * o13 : 0.0;
+        xf[j1 * ne + i1] += 3.0 * DPm(ip, jp) * (xaa + xbb + xab + xba);
+        e += DPm(ip, jp) * DAm(i1, j1) * (xaa + xbb + xab + xba);
+    }
+    for (i64 k = 0; k < ne * ne; ++k) FAE[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+
+/* UFP_OMG3 / thread_UFPOMG3 (xcuhf_omg3.f:284-560) */
+void orc_ufp_omg3(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM3, double *FP, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    int LAAA = nae >= 3, LBBB = nbe >= 3, LAAB = nae >= 2 && nbe >= 1, LBBA = nbe >= 2 && nae >= 1;
+    double *xf = (double *)calloc(np * np, sizeof(double));
+    double e = 0.0;
+    LOOP3
+    {
+        double o6, o23, o12, o13;
+        omg3_reads(ne, np, GM3, ip, jp, i1, j1, i2, j2, i3, j3, &o6, &o23, &o12, &o13);
+        double caaa = LAAA ? DAm(i1, j1) * DAm(i2, j2) * DAm(i3, j3) : 0.0;
+        double cbbb = LBBB ? DBm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) : 0.0;
+        double caab = 0, caba = 0, cbaa = 0, cbba = 0, cbab = 0, cabb = 0;
+        if (LAAB) {
+            caab = DAm(i1, j1) * DAm(i2, j2) * DBm(i3, j3);
+            caba = DAm(i1, j1) * DBm(i2, j2) * DAm(i3, j3);
+            cbaa = DBm(i1, j1) * DAm(i2, j2) * DAm(i3, j3);
+        }
+        if (LBBA) {
+            cbba = DBm(i1, j1) * DBm(i2, j2) * DAm(i3, j3);
+            cbab = DBm(i1, j1) * DAm(i2, j2) * DBm(i3, j3);
+            cabb = DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3);
+        }
+        double v = (caaa + cbbb) * o6 + (caab + cbba) * o12 + (caba + cbab) * o13 + (cabb + cbaa) * o23;
+        xf[jp * np + ip] += v;
+        e += DPm(ip, jp) * v;
+    }
+    for (i64 k = 0; k < np * np; ++k) FP[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+
+/* the 24 ket-permuted reads of calc_FAEOMG4 / calc_UFPOMG4
+ * (xcuhf_omg4.f:234-510, :745-1078), in the reference's order:
+ * OMG, OMG_23, OMG_12, OMG_12_23, OMG_13_23, OMG_13, OMG_24, OMG_12_24,
+ * OMG_12_14, OMG_14, OMG_34, OMG_12_34, OMG_13_34, OMG_14_34, OMG_13_24,
+ * OMG_14_23, OMG_23_34, OMG_24_34, OMG_12_23_34, OMG_12_23_24, OMG_12_13_34,
+ * OMG_13_23_24, OMG_14_24_34, OMG_14_23_34 (ket of electron k = j[perm[k]]) */
+static const int omg4_perm[24][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}, {1, 0, 2, 3}, {1, 2, 0, 3}, {2, 0, 1, 3}, {2, 1, 0, 3},
+                                     {0, 3, 2, 1}, {1, 3, 2, 0}, {3, 0, 2, 1}, {3, 1, 2, 0}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                     {2, 1, 3, 0}, {3, 1, 0, 2}, {2, 3, 0, 1}, {3, 2, 1, 0}, {0, 2, 3, 1}, {0, 3, 1, 2},
+                                     {1, 2, 3, 0}, {1, 3, 0, 2}, {2, 0, 3, 1}, {2, 3, 1, 0}, {3, 0, 1, 2}, {3, 2, 0, 1}};
+enum { O_, O23, O12, O12_23, O13_23, O13, O24, O12_24, O12_14, O14, O34, O12_34, O13_34, O14_34, O13_24, O14_23,
+       O23_34, O24_34, O12_23_34, O12_23_24, O12_13_34, O13_23_24, O14_24_34, O14_23_34 };
+
+static void omg4_reads(i64 ne, i64 np, const double *G, i64 ip, i64 jp, const i64 *ii, const i64 *jj, double *o)
+{
+    for (int p = 0; p < 24; ++p) {
+        const int *q = omg4_perm[p];
+        i64 ia = ix3(ne, np, ip, jp, ii[0], jj[q[0]], ii[1], jj[q[1]], ii[2], jj[q[2]]) * ne * ne + ii[3] * ne + jj[q[3]];
+        o[p] = G[ia];
+    }
+}
+
+static double omg4_anti(const double *o)
+{
+    return o[O_] - o[O23] - o[O12] - o[O13] - o[O24] - o[O14] - o[O34] + o[O12_23] + o[O13_23] + o[O12_24] +
+           o[O12_14] + o[O12_34] + o[O13_34] + o[O14_34] + o[O13_24] + o[O14_23] + o[O23_34] + o[O24_34] -
+           o[O12_23_34] - o[O12_23_24] - o[O12_13_34] - o[O13_23_24] - o[O14_24_34] - o[O14_23_34];
+}
+#define OM_AAB(o) (o[O_] - o[O23] - o[O12] + o[O12_23] + o[O13_23] - o[O13])
+#define OM_ABA(o) (o[O_] - o[O24] - o[O12] + o[O12_24] + o[O12_14] - o[O14])
+#define OM_BAA(o) (o[O_] - o[O34] - o[O13] + o[O13_34] + o[O14_34] - o[O14])
+#define OM_ABB(o) (o[O_] - o[O34] - o[O12] + o[O12_34])
+#define OM_BAB(o) (o[O_] - o[O24] - o[O13] + o[O13_24])
+#define OM_BBA(o) (o[O_] - o[O23] - o[O14] + o[O14_23])
+#define OM_BBB(o) (o[O_] - o[O34] - o[O23] + o[O23_34] + o[O24_34] - o[O24])
+
+#define LOOP4                                                                                                      \
+    LOOP3 for (i64 i4 = 0; i4 < ne; ++i4) for (i64 j4 = 0; j4 < ne; ++j4)
+
+/* FAE_OMG4 / thread_FAEOMG4 / calc_FAEOMG4 (xcuhf_omg4.f:2-510) */
+void orc_fae_omg4(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM4, double *FAE, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    int LAAA = nae >= 4, LBBB = nbe >= 3, LAAB = nae >= 3 && nbe >= 1, LBBA = nbe >= 2 && nae >= 2;
+    double *xf = (double *)calloc(ne * ne, sizeof(double));
+    double e = 0.0;
+    LOOP4
+    {
+        i64 ii[4] = {i1, i2, i3, i4}, jj[4] = {j1, j2, j3, j4};
+        double o[24];
+        omg4_reads(ne, np, GM4, ip, jp, ii, jj, o);
+        double xaaa = LAAA ? DAm(i2, j2) * DAm(i3, j3) * DAm(i4, j4) * omg4_anti(o) : 0.0;
+        double xaab = LAAB ? DAm(i2, j2) * DAm(i3, j3) * DBm(i4, j4) * OM_AAB(o) : 0.0;
+        double xaba = LAAB ? DAm(i2, j2) * DBm(i3, j3) * DAm(i4, j4) * OM_ABA(o) : 0.0;
+        double xbaa = LAAB ? DBm(i2, j2) * DAm(i3, j3) * DAm(i4, j4) * OM_BAA(o) : 0.0;
+        double xabb = LBBA ? DAm(i2, j2) * DBm(i3, j3) * DBm(i4, j4) * OM_ABB(o) : 0.0;
+        double xbab = LBBA ? DBm(i2, j2) * DAm(i3, j3) * DBm(i4, j4) * OM_BAB(o) : 0.0;
+        double xbba = LBBA ? DBm(i2, j2) * DBm(i3, j3) * DAm(i4, j4) * OM_BBA(o) : 0.0;
+        double xbbb = LBBB ? DBm(i2, j2) * DBm(i3, j3) * DBm(i4, j4) * OM_BBB(o) : 0.0;
+        double ans = DPm(ip, jp) * (xaaa + xaab + xaba + xbaa + xabb + xbab + xbba + xbbb);
+        xf[j1 * ne + i1] += 4.0 * ans;
+        e += DAm(i1, j1) * ans;
+    }
+    for (i64 k = 0; k < ne * ne; ++k) FAE[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+
+/* UFP_OMG4 / thread_UFPOMG4 / calc_UFPOMG4 (xcuhf_omg4.f:512-1078) */
+void orc_ufp_omg4(i64 nae, i64 nbe, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                  const double *GM4, double *FP, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    int L4A = nae >= 4, L4B = nbe >= 4, L3A1B = nae >= 3 && nbe >= 1, L1A3B = nae >= 1 && nbe >= 3,
+        L2A2B = nae >= 2 && nbe >= 2;
+    double *xf = (double *)calloc(np * np, sizeof(double));
+    double e = 0.0;
+    LOOP4
+    {
+        i64 ii[4] = {i1, i2, i3, i4}, jj[4] = {j1, j2, j3, j4};
+        double o[24];
+        omg4_reads(ne, np, GM4, ip, jp, ii, jj, o);
+        double A[4] = {DAm(i1, j1), DAm(i2, j2), DAm(i3, j3), DAm(i4, j4)};
+        double B[4] = {DBm(i1, j1), DBm(i2, j2), DBm(i3, j3), DBm(i4, j4)};
+        double anti = omg4_anti(o);
+        double x = 0.0;
+        x += L4A ? A[0] * A[1] * A[2] * A[3] * anti : 0.0;
+        x += L3A1B ? A[0] * A[1] * A[2] * B[3] * OM_AAB(o) : 0.0;
+        x += L3A1B ? A[0] * A[1] * B[2] * A[3] * OM_ABA(o) : 0.0;
+        x += L3A1B ? A[0] * B[1] * A[2] * A[3] * OM_BAA(o) : 0.0;
+        x += L2A2B ? A[0] * A[1] * B[2] * B[3] * OM_ABB(o) : 0.0;
+        x += L2A2B ? A[0] * B[1] * A[2] * B[3] * OM_BAB(o) : 0.0;
+        x += L2A2B ? A[0] * B[1] * B[2] * A[3] * OM_BBA(o) : 0.0;
+        x += L1A3B ? A[0] * B[1] * B[2] * B[3] * OM_BBB(o) : 0.0;
+        x += L4B ? B[0] * B[1] * B[2] * B[3] * anti : 0.0;
+        x += L1A3B ? B[0] * B[1] * B[2] * A[3] * OM_AAB(o) : 0.0;
+        x += L1A3B ? B[0] * B[1] * A[2] * B[3] * OM_ABA(o) : 0.0;
+        x += L1A3B ? B[0] * A[1] * B[2] * B[3] * OM_BAA(o) : 0.0;
+        x += L2A2B ? B[0] * B[1] * A[2] * A[3] * OM_ABB(o) : 0.0;
+        x += L2A2B ? B[0] * A[1] * B[2] * A[3] * OM_BAB(o) : 0.0;
+        x += L2A2B ? B[0] * A[1] * A[2] * B[3] * OM_BBA(o) : 0.0;
+        x += L3A1B ? B[0] * A[1] * A[2] * A[3] * OM_BBB(o) : 0.0;
+        xf[jp * np + ip] += x;
+        e += DPm(ip, jp) * x;
+    }
+    for (i64 k = 0; k < np * np; ++k) FP[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+
+/* OMG4_ICR / thread_omg4_IC (xcuhf_omg4.f:1080-1320): the unsymmetrised
+ * calc_GAM_4_integral value per entry */
+void orc_omg4_icr(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee, double *GM4)
+{
+    i64 ne = nebf, np = npbf;
+#pragma omp parallel for collapse(2)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3)
+                                    for (i64 i4 = 0; i4 < ne; ++i4)
+                                        for (i64 j4 = 0; j4 < ne; ++j4) {
+                                            i64 ia = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3) * ne * ne + i4 * ne + j4;
+                                            GM4[ia] = gam4_int(ne, np, GAM2s, GAMee, ip, jp, i1, j1, i2, j2, i3, j3, i4,
+                                                               j4);
+                                        }
+}
+#undef DAm
+#undef DBm
+#undef DPm

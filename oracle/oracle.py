@@ -375,3 +375,106 @@ def ref_gam4_icr(nebf, npbf, g2s, gee, workdir):
     _in_dir(workdir, lambda: ref().gam4_icr_(_i8(1), _i8(nebf), _i8(npbf), _i8(ngee), _i8(ng2), _i8(g4.size),
                                              _p(np.ascontiguousarray(g2s)), _p(g4)))
     return g4
+
+
+# ---------------------------------------------------------------------------
+# XCUHF contractions (oracle and reference); F* arrays accumulate in place
+# ---------------------------------------------------------------------------
+def _f(a):
+    return _p(np.asfortranarray(a))
+
+
+def fae_omg2(nae, nbe, ne, npb, DAE, DBE, DP, G, Gs, FAE, SAE2):
+    e, s = np.zeros(1), np.zeros(1)
+    lib().orc_fae_omg2(C.c_int64(nae), C.c_int64(nbe), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                       _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gs)), _p(FAE), _p(SAE2), _p(e), _p(s))
+    return float(e[0]), float(s[0])
+
+
+def ufp_omg2(nae, nbe, ne, npb, DAE, DBE, DP, G, Gs, FP, SP2):
+    e, s = np.zeros(1), np.zeros(1)
+    lib().orc_ufp_omg2(C.c_int64(nae), C.c_int64(nbe), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                       _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gs)), _p(FP), _p(SP2), _p(e), _p(s))
+    return float(e[0]), float(s[0])
+
+
+def _omg(name, nae, nbe, ne, npb, DAE, DBE, DP, G, F):
+    e = np.zeros(1)
+    getattr(lib(), name)(C.c_int64(nae), C.c_int64(nbe), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                         _p(np.ascontiguousarray(G)), _p(F), _p(e))
+    return float(e[0])
+
+
+def fae_omg3(*a):
+    return _omg("orc_fae_omg3", *a)
+
+
+def ufp_omg3(*a):
+    return _omg("orc_ufp_omg3", *a)
+
+
+def fae_omg4(*a):
+    return _omg("orc_fae_omg4", *a)
+
+
+def ufp_omg4(*a):
+    return _omg("orc_ufp_omg4", *a)
+
+
+def omg4_icr(nebf, npbf, g2s, gee):
+    g = np.zeros(nebf ** 8 * npbf * npbf)
+    lib().orc_omg4_icr(C.c_int64(nebf), C.c_int64(npbf), _p(np.ascontiguousarray(g2s)), _p(np.ascontiguousarray(gee)),
+                       _p(g))
+    return g
+
+
+def ref_fae_omg2(nae, nbe, ne, npb, DAE, DBE, DP, G, Gs, FAE, SAE2):
+    """FAE_OMG2 (source/xcuhf/xcuhf_omg2.f:2) of the reference."""
+    e, s = C.c_double(), C.c_double()
+    DT = np.asfortranarray(DAE + DBE)
+    ref().fae_omg2_(_i8(1), _i8(nae), _i8(nbe), _i8(ne), _i8(npb), _i8(ne ** 4 * npb * npb), _f(DAE), _f(DBE),
+                    _p(DT), _f(DP), _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gs)), _p(FAE), _p(SAE2),
+                    C.byref(e), C.byref(s))
+    return e.value, s.value
+
+
+def ref_ufp_omg2(nae, nbe, ne, npb, DAE, DBE, DP, G, Gs, FP, SP2):
+    e, s = C.c_double(), C.c_double()
+    DT = np.asfortranarray(DAE + DBE)
+    ref().ufp_omg2_(_i8(1), _i8(ne), _i8(npb), _i8(ne ** 4 * npb * npb), _i8(nae), _i8(nbe), _f(DAE), _f(DBE),
+                    _p(DT), _f(DP), _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gs)), _p(FP), _p(SP2),
+                    C.byref(e), C.byref(s))
+    return e.value, s.value
+
+
+def _ref_omg(name, n, nae, nbe, ne, npb, DAE, DBE, DP, G, F):
+    e = C.c_double()
+    getattr(ref(), name)(_i8(1), _i8(ne), _i8(npb), _i8(ne ** (2 * n) * npb * npb), _i8(nae), _i8(nbe), _f(DAE),
+                         _f(DBE), _f(DP), _p(np.ascontiguousarray(G)), _p(F), C.byref(e))
+    return e.value
+
+
+def ref_fae_omg3(*a):
+    return _ref_omg("fae_omg3_", 3, *a)
+
+
+def ref_ufp_omg3(*a):
+    return _ref_omg("ufp_omg3_", 3, *a)
+
+
+def ref_fae_omg4(*a):
+    return _ref_omg("fae_omg4_", 4, *a)
+
+
+def ref_ufp_omg4(*a):
+    return _ref_omg("ufp_omg4_", 4, *a)
+
+
+def ref_omg4_icr(nebf, npbf, g2s, gee, workdir):
+    """OMG4_ICR (source/xcuhf/xcuhf_omg4.f:1080): reads GAM_ee.ufm from workdir."""
+    np.ascontiguousarray(gee, dtype=np.float64).tofile(os.path.join(workdir, "GAM_ee.ufm"))
+    ngee, ng2 = nebf ** 4, nebf ** 4 * npbf ** 2
+    g4 = np.zeros(ngee * ngee * npbf * npbf)
+    _in_dir(workdir, lambda: ref().omg4_icr_(_i8(1), _i8(nebf), _i8(npbf), _i8(ngee), _i8(ng2), _i8(g4.size),
+                                             _p(np.ascontiguousarray(g2s)), _p(g4)))
+    return g4

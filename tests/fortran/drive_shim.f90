@@ -16,10 +16,10 @@ program drive_shim
       real(c_double) :: AGEBFCC(*), AGNBFCC(*), ELCEX(*), NUCEX(*), ELCBFC(*), NUCBFC(*), GM2ICR(*), GM2sICR(*)
     end subroutine
   end interface
-  integer(c_int64_t) :: nebf, npebf, npbf, nat, ngtg, ng2, ng3, ng4, ngee, ng1, one, zero, ia
+  integer(c_int64_t) :: nebf, npebf, npbf, nat, ngtg, ng2, ng3, ng4, ngee, ng1, one, zero, ia, nae, nbe
   real(c_double) :: pmass, e2, s2, e3, e4
   real(c_double), allocatable :: cat(:), zan(:), bcoef(:), gamma(:), agebfcc(:), agnbfcc(:), elcex(:), nucex(:)
-  real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:)
+  real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:), dae(:,:), dbe(:,:)
   integer(c_int64_t), allocatable :: kpestr(:), kpeend(:), ampeb2c(:), elcam(:), nucam(:)
   real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:)
   character(len=512) :: fin, fout
@@ -88,5 +88,32 @@ program drive_shim
   fp = 0.0d0
   call GAM4_DSCF(one, nebf, npbf, ngee, ng2, ng4, de, dp, fe, fp, e4)
   write(11) fe, fp, e4
+  ! XCUHF Fock pieces (xcuhf_fock.f:896-916, :1014-1034) with NAE=2, NBE=1,
+  ! DAE = 2/3 DE, DBE = 1/3 DE; alpha and beta FAE share one call shape
+  nae = 2
+  nbe = 1
+  allocate(dae(nebf, nebf), dbe(nebf, nebf))
+  dae = de * (2.0d0 / 3.0d0)
+  dbe = de * (1.0d0 / 3.0d0)
+  call OMG4_ICR(one, nebf, npbf, ngee, ng2, ng4, gm2s, gm4)
+  write(11) gm4
+  fe = 0.0d0
+  call FAE_OMG2(one, nae, nbe, nebf, npbf, ng2, dae, dbe, de, dp, gm2, gm2s, fe, se, e2, s2)
+  write(11) fe, se, e2, s2
+  fp = 0.0d0
+  call UFP_OMG2(one, nebf, npbf, ng2, nae, nbe, dae, dbe, de, dp, gm2, gm2s, fp, sp, e2, s2)
+  write(11) fp, sp, e2, s2
+  fe = 0.0d0
+  call FAE_OMG3(one, nebf, npbf, ng3, nae, nbe, dae, dbe, dp, gm3, fe, e3)
+  write(11) fe, e3
+  fp = 0.0d0
+  call UFP_OMG3(one, nebf, npbf, ng3, nae, nbe, dae, dbe, dp, gm3, fp, e3)
+  write(11) fp, e3
+  fe = 0.0d0
+  call FAE_OMG4(one, nebf, npbf, ng4, nae, nbe, dae, dbe, dp, gm4, fe, e4)
+  write(11) fe, e4
+  fp = 0.0d0
+  call UFP_OMG4(one, nebf, npbf, ng4, nae, nbe, dae, dbe, dp, gm4, fp, e4)
+  write(11) fp, e4
   close(11)
 end program

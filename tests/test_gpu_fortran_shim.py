@@ -74,7 +74,12 @@ def test_fortran_dropin_matches_oracle():
              npb * npb, 1,
              ne * ne, npb * npb, ne * ne, npb * npb, 1, 1,  # GAM2_DSCF: FE, FP, SE2, SP2, E, S
              ne * ne, npb * npb, 1,  # GAM3_DSCF
-             ne * ne, npb * npb, 1]  # GAM4_DSCF
+             ne * ne, npb * npb, 1,  # GAM4_DSCF
+             ng4,  # OMG4_ICR
+             ne * ne, ne * ne, 1, 1,  # FAE_OMG2: FAE, SAE2, E, S
+             npb * npb, npb * npb, 1, 1,  # UFP_OMG2
+             ne * ne, 1, npb * npb, 1,  # FAE_OMG3, UFP_OMG3
+             ne * ne, 1, npb * npb, 1]  # FAE_OMG4, UFP_OMG4
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -118,3 +123,24 @@ def test_fortran_dropin_matches_oracle():
     check_close(parts[25], F[0].ravel(order="F"), what="GAM4_DSCF FE")
     check_close(parts[26], F[1].ravel(order="F"), what="GAM4_DSCF FP")
     assert parts[27][0] == pytest.approx(E4d, rel=1e-11)
+    # XCUHF pieces, NAE=2, NBE=1, DAE = 2/3 DE, DBE = 1/3 DE
+    og4u = oracle.omg4_icr(ne, npb, og2s, oee)
+    check_close(parts[28], og4u, what="OMG4_ICR")
+    DAE, DBE = DE * (2.0 / 3.0), DE * (1.0 / 3.0)
+    a = (2, 1, ne, npb, DAE, DBE, DP)
+    F, S2 = np.zeros((ne, ne), order="F"), np.zeros((ne, ne), order="F")
+    e, sv = oracle.fae_omg2(*a, og2, og2s, F, S2)
+    check_close(parts[29], F.ravel(order="F"), what="FAE_OMG2")
+    check_close(parts[30], S2.ravel(order="F"), what="FAE_OMG2 SAE2")
+    assert parts[31][0] == pytest.approx(e, rel=1e-11) and parts[32][0] == pytest.approx(sv, rel=1e-11)
+    F, S2 = np.zeros((npb, npb), order="F"), np.zeros((npb, npb), order="F")
+    e, sv = oracle.ufp_omg2(*a, og2, og2s, F, S2)
+    check_close(parts[33], F.ravel(order="F"), what="UFP_OMG2")
+    check_close(parts[34], S2.ravel(order="F"), what="UFP_OMG2 SP2")
+    assert parts[35][0] == pytest.approx(e, rel=1e-11) and parts[36][0] == pytest.approx(sv, rel=1e-11)
+    for k, (fn, G, n) in enumerate(((oracle.fae_omg3, og3, ne), (oracle.ufp_omg3, og3, npb),
+                                    (oracle.fae_omg4, og4u, ne), (oracle.ufp_omg4, og4u, npb))):
+        F = np.zeros((n, n), order="F")
+        e = fn(*a, G, F)
+        check_close(parts[37 + 2 * k], F.ravel(order="F"), what=fn.__name__)
+        assert parts[38 + 2 * k][0] == pytest.approx(e, rel=1e-11, abs=1e-14), fn.__name__

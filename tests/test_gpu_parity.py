@@ -295,3 +295,43 @@ def test_precision_switch(torch_dev):
     first = seen.index(True)
     assert all(seen[first:]) and not any(seen[:first])
     assert 4 <= first <= 7
+
+
+@pytest.mark.parametrize("nae,nbe", [(1, 1), (3, 2), (4, 4), (2, 3)])
+def test_uhf_contractions(torch_dev, nae, nbe):
+    """FAE_OMG2/3/4, UFP_OMG2/3/4 through the C ABI against the oracle's
+    restatement of source/xcuhf/xcuhf_omg{2,3,4}.f, electron counts chosen to
+    switch the spin-pair terms on and off; Fock matrices start non-zero."""
+    print(f"case uhf nae={nae} nbe={nbe}", flush=True)
+    rng = np.random.default_rng(20 + 10 * nae + nbe)
+    ne, npb = 4, 2
+    DAE, DBE = synth.density(ne, float(nae), 5), synth.density(ne, float(nbe), 6)
+    DP = synth.density(npb, 1.0, 7)
+    g2, g2s = rng.standard_normal(ne ** 4 * npb * npb), rng.standard_normal(ne ** 4 * npb * npb)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    g4 = rng.standard_normal(ne ** 8 * npb * npb)
+    a = (nae, nbe, ne, npb, DAE, DBE, DP)
+    for gfn, ofn, n in ((neorx.FAE_OMG2, oracle.fae_omg2, ne), (neorx.UFP_OMG2, oracle.ufp_omg2, npb)):
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        Fg, Fo, So = F0.copy(order="F"), F0.copy(order="F"), np.zeros((n, n), order="F")
+        Sg, eg, sg = gfn(*a, g2, g2s, Fg)
+        eo, so = ofn(*a, g2, g2s, Fo, So)
+        check_close(Fg, Fo, what=gfn.__name__)
+        check_close(Sg, So, what=gfn.__name__ + " S")
+        assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14) and sg == pytest.approx(so, rel=1e-11, abs=1e-14)
+    for gfn, ofn, G, n in ((neorx.FAE_OMG3, oracle.fae_omg3, g3, ne), (neorx.UFP_OMG3, oracle.ufp_omg3, g3, npb),
+                           (neorx.FAE_OMG4, oracle.fae_omg4, g4, ne), (neorx.UFP_OMG4, oracle.ufp_omg4, g4, npb)):
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        Fg, Fo = F0.copy(order="F"), F0.copy(order="F")
+        eg = gfn(*a, G, Fg)
+        eo = ofn(*a, G, Fo)
+        check_close(Fg, Fo, what=gfn.__name__)
+        assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14), gfn.__name__
+
+
+def test_omg4_icr(torch_dev):
+    s = synth.small(seed=16, n_e_centres=2, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    g2, g2s = oracle.gam2_ic1(s)
+    gee = oracle.gam_ee(s)
+    check_close(neorx.OMG4_ICR(s["nebf"], s["npbf"], g2s, gee), oracle.omg4_icr(s["nebf"], s["npbf"], g2s, gee),
+                what="OMG4_ICR")

@@ -236,3 +236,46 @@ def test_oracle_cheap_builders_vs_reference(tmp_path):
     ue = oracle.gam_ee(u)
     check_close(oracle.gam4_icr(u["nebf"], u["npbf"], g2s, ue),
                 oracle.ref_gam4_icr(u["nebf"], u["npbf"], g2s, ue, str(tmp_path)), what="GAM4_ICR")
+
+
+@needs_ref
+@pytest.mark.parametrize("nae,nbe", [(1, 1), (2, 1), (3, 2), (4, 4), (2, 3)])
+def test_oracle_uhf_contractions_vs_reference(nae, nbe):
+    """FAE_OMG2/3/4 and UFP_OMG2/3/4 of the oracle against the reference's
+    own routines (source/xcuhf/xcuhf_omg{2,3,4}.f) on random tensors, for
+    electron counts that switch the spin-pair flags on and off."""
+    rng = np.random.default_rng(10 * nae + nbe)
+    ne, npb = 3, 2
+    DAE, DBE = synth.density(ne, float(nae), 1), synth.density(ne, float(nbe), 2)
+    DP = synth.density(npb, 1.0, 3)
+    g2, g2s = rng.standard_normal(ne ** 4 * npb * npb), rng.standard_normal(ne ** 4 * npb * npb)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    g4 = rng.standard_normal(ne ** 8 * npb * npb)
+    args = (nae, nbe, ne, npb, DAE, DBE, DP)
+    for fn, rfn, G, n in ((oracle.fae_omg2, oracle.ref_fae_omg2, (g2, g2s), ne),
+                          (oracle.ufp_omg2, oracle.ref_ufp_omg2, (g2, g2s), npb)):
+        F0 = [np.asfortranarray(rng.standard_normal((n, n))) for _ in range(2)]
+        A = [f.copy(order="F") for f in F0]
+        B = [f.copy(order="F") for f in F0]
+        ea, sa = fn(*args, *G, *A)
+        eb, sb = rfn(*args, *G, *B)
+        for a, b in zip(A, B):
+            check_close(a, b, what=fn.__name__)
+        assert ea == pytest.approx(eb, rel=1e-12, abs=1e-14) and sa == pytest.approx(sb, rel=1e-12, abs=1e-14)
+    for fn, rfn, G, n in ((oracle.fae_omg3, oracle.ref_fae_omg3, g3, ne), (oracle.ufp_omg3, oracle.ref_ufp_omg3, g3, npb),
+                          (oracle.fae_omg4, oracle.ref_fae_omg4, g4, ne), (oracle.ufp_omg4, oracle.ref_ufp_omg4, g4, npb)):
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        ea = fn(*args, G, A)
+        eb = rfn(*args, G, B)
+        check_close(A, B, what=fn.__name__)
+        assert ea == pytest.approx(eb, rel=1e-12, abs=1e-14), fn.__name__
+
+
+@needs_ref
+def test_oracle_omg4_icr_vs_reference(tmp_path):
+    u = synth.small(seed=16, n_e_centres=2, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    g2, g2s = oracle.gam2_ic1(u)
+    ue = oracle.gam_ee(u)
+    check_close(oracle.omg4_icr(u["nebf"], u["npbf"], g2s, ue),
+                oracle.ref_omg4_icr(u["nebf"], u["npbf"], g2s, ue, str(tmp_path)), what="OMG4_ICR")
