@@ -57,8 +57,9 @@ int neorx_system_destroy(int64_t handle);
  *   cls 1 (e-p)     : xcalc_GAM1_MD, source/gam_1_OMP.f:398   -> ansE, ansS
  *   cls 2 (e-e-p)   : xcalc_GAM2_MD, source/gam_2_OMP.f:546   -> ansE, ansS
  *   cls 3 (e-e-e-p) : xcalc_GAM3_MD, source/gam_3_OMP.f:617   -> ansE
- * d_idx holds 0-based primitive indices per tuple, 2*(cls+1) int64 each, in
- * the order (ie1..ie_cls, ip, je1..je_cls, jp). */
+ *   cls 4 (e-e-p)   : RXCHFne_xcalc_GAM2_MD, rxchfne/rxchfne_gam_2_OMP.f:2 -> ansE1, ansE2
+ * d_idx holds 0-based primitive indices per tuple, 2*(n+1) int64 each for n
+ * electrons (n = cls, 2 for cls 4), in the order (ie1..ie_n, ip, je1..je_n, jp). */
 int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
                         double *d_ansS, void *stream);
 /* Same values for the contiguous slice [start, start+count) of the full
@@ -180,6 +181,35 @@ int neorx_fae_omg4_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, con
                        const double *d_dp, const double *d_gm4, double *d_fae, double *d_e, void *stream);
 int neorx_ufp_omg4_dev(int64_t nae, int64_t nbe, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
                        const double *d_dp, const double *d_gm4, double *d_fp, double *d_e, void *stream);
+
+/* ---- RXCHF-ne (reference semantics, source/rxchfne/) ---------------------
+ * The RXCHF-ne e-e-p tuple RXCHFne_xcalc_GAM2_MD is class 4 of
+ * neorx_prim_list_dev / neorx_prim_range_dev (outputs ansE1, ansE2 in the E
+ * and S arrays; electron 1 is the special electron). */
+/* RXCHFne_GAM2_IC1, source/rxchfne/rxchfne_gam2_IC1_OMP.f:2: GM2ICR
+ * (contracted ansE1, not symmetrised) and GM2exICR ((x12 + x21)/2 of ansE2) */
+int neorx_rxchfne_gam2_ic1(int64_t handle, double *gm2icr, double *gm2exicr);
+int neorx_rxchfne_gam2_ic1_dev(int64_t handle, double *d_gm2icr, double *d_gm2exicr, void *stream);
+/* RXCHFne_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2, source/rxchfne/rxchfne_omg2.f:2,
+ * :212, :409 (dae: regular, dbe: special electron density); F accumulates,
+ * E overwritten */
+int neorx_rxchfne_fae_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                           const double *gm2, double *fae, double *e);
+int neorx_rxchfne_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                          const double *gm2, double *fp, double *e);
+int neorx_rxchfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                           const double *gm2, double *fbe, double *e);
+/* RXCUHFne_FE_OMG2 / _FP_OMG2 / _FBE_OMG2, source/rxchfne/rxcuhfne_omg2.f:2,
+ * :239, :445 (daalp: regular alpha, datot: regular total density) */
+int neorx_rxcuhfne_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,
+                           const double *dp, const double *gm2, const double *gm2ex, double *faalp, double *fabet,
+                           double *e);
+int neorx_rxcuhfne_fp_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,
+                           const double *dp, const double *gm2, const double *gm2ex, double *fp, double *e);
+int neorx_rxcuhfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,
+                            const double *dp, const double *gm2, const double *gm2ex, double *fbe, double *e);
+/* RXCHFne_FAE_GAMee, source/rxchfne/rxchfne_hfterms.f:2 */
+int neorx_rxchfne_fae_gamee(int64_t nebf, const double *gam_ee, const double *dae, double *fae, double *e);
 
 /* ---- direct-SCF routines (reference semantics) -------------------------- */
 /* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per

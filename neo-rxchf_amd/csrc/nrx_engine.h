@@ -942,6 +942,35 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
     ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
 }
 
+// RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): the RXCHF-ne
+// e-e-p tuple, two outputs summed over all (k,l):
+//   ansE1 = -<1/r(e2,p)> with g(e1,p) = gamma_k + gamma_l  +  <1/r12> with the same geminal
+//   ansE2 =  <1/r12> with g(e1,p) = gamma_k, g(e2,p) = gamma_l
+template <int NL>
+NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE1,
+                         double &ansE2)
+{
+    double gVEPg2 = 0, gVEEg1 = 0, gVEEg2 = 0;
+    for (int ik = 0; ik < S.ngtg; ++ik)
+        for (int il = 0; il < S.ngtg; ++il) {
+            double gk = S.gamma[ik], gl = S.gamma[il];
+            double bb = S.bcoef[ik] * S.bcoef[il];
+            Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0);
+            double vep = coul<NL>(s1, 1, 2);
+            underflow(vep);
+            double vee1 = coul<NL>(s1, 0, 1);
+            underflow(vee1);
+            Sys s2 = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
+            double vee2 = coul<NL>(s2, 0, 1);
+            underflow(vee2);
+            gVEPg2 += bb * (-vep);
+            gVEEg1 += bb * vee1;
+            gVEEg2 += bb * vee2;
+        }
+    ansE1 = gVEPg2 + gVEEg1;
+    ansE2 = gVEEg2;
+}
+
 // Polynomial capacities per tuple class.  GEN covers every tuple allowed by
 // l <= 2 (per particle and direction bra+ket powers <= 4): e-p 4+4+2,
 // e-e-p 3*4+2, e-e-e-p 4*4.  SMALL covers tuples with tuple_degree <= SMALL-1,
@@ -959,7 +988,15 @@ template <> struct Tier<3> {
     static constexpr int SMALL = 9, GEN = 17;
     static constexpr bool KIN = false;
 };
+// class 4: the RXCHF-ne e-e-p tuple (Coulomb terms only: 3*4)
+template <> struct Tier<4> {
+    static constexpr int SMALL = 9, GEN = 13;
+    static constexpr bool KIN = false;
+};
 constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
+
+// electrons per tuple of a class
+template <int CLS> constexpr int nel() { return CLS == 4 ? 2 : CLS; }
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
 template <int CLS, int NL>
@@ -970,13 +1007,15 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
         gam1_tuple<NL>(S, q[0], q[1], e, s);
     else if constexpr (CLS == 2)
         gam2_tuple<NL>(S, q[0], q[1], q[2], e, s);
+    else if constexpr (CLS == 4)
+        gam2ne_tuple<NL>(S, q[0], q[1], q[2], e, s);
     else
         gam3_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
 }
 
 template <int CLS> NRX_HD bool tuple_small(const PairRec *const *q)
 {
-    return tuple_degree(q, CLS + 1, Tier<CLS>::KIN) <= Tier<CLS>::SMALL - 1;
+    return tuple_degree(q, nel<CLS>() + 1, Tier<CLS>::KIN) <= Tier<CLS>::SMALL - 1;
 }
 
 } // namespace nrx

@@ -298,20 +298,21 @@ NRX_DI SysParams d_params(const DevSys &S)
 // (bra e..., ip, ket e..., jp)
 template <int CLS> NRX_DI void tuple_pairs(const DevSys &S, const i64 *x, const PairRec **q)
 {
-    for (int i = 0; i < CLS; ++i) q[i] = S.ep + x[i] * S.npebf + x[CLS + 1 + i];
-    q[CLS] = S.pp + x[CLS] * S.npbf + x[2 * CLS + 1];
+    constexpr int E = nel<CLS>();
+    for (int i = 0; i < E; ++i) q[i] = S.ep + x[i] * S.npebf + x[E + 1 + i];
+    q[E] = S.pp + x[E] * S.npbf + x[2 * E + 1];
 }
 
 template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
 {
-    const PairRec *q[CLS + 1];
+    const PairRec *q[nel<CLS>() + 1];
     tuple_pairs<CLS>(S, x, q);
     tuple_eval<CLS, NL>(d_params(S), q, e, s);
 }
 
 template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
 {
-    const PairRec *q[CLS + 1];
+    const PairRec *q[nel<CLS>() + 1];
     tuple_pairs<CLS>(S, x, q);
     return tuple_small<CLS>(q);
 }
@@ -320,9 +321,10 @@ template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
 template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t, i64 *x)
 {
     bool ok = true;
-    for (int k = 0; k < 2 * CLS + 2; ++k) {
-        x[k] = idx[t * (2 * CLS + 2) + k];
-        i64 lim = (k == CLS || k == 2 * CLS + 1) ? S.npbf : S.npebf;
+    constexpr int E = nel<CLS>();
+    for (int k = 0; k < 2 * E + 2; ++k) {
+        x[k] = idx[t * (2 * E + 2) + k];
+        i64 lim = (k == E || k == 2 * E + 1) ? S.npbf : S.npebf;
         ok = ok && x[k] >= 0 && x[k] < lim;
     }
     return ok;
@@ -334,19 +336,20 @@ template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t
 template <int CLS, bool SORTED> NRX_DI void range_tuple(const DevSys &S, i64 r, i64 *x)
 {
     i64 ne = S.npebf, np = S.npbf;
-    for (int i = CLS - 1; i >= 0; --i) {
+    constexpr int CLS_E = nel<CLS>();
+    for (int i = CLS_E - 1; i >= 0; --i) {
         i64 k = r % ne;
-        x[CLS + 1 + i] = SORTED ? S.eord[k] : k;
+        x[CLS_E + 1 + i] = SORTED ? S.eord[k] : k;
         r /= ne;
         k = r % ne;
         x[i] = SORTED ? S.eord[k] : k;
         r /= ne;
     }
     i64 k = r % np;
-    x[2 * CLS + 1] = SORTED ? S.pord[k] : k;
+    x[2 * CLS_E + 1] = SORTED ? S.pord[k] : k;
     r /= np;
     k = r % np;
-    x[CLS] = SORTED ? S.pord[k] : k;
+    x[CLS_E] = SORTED ? S.pord[k] : k;
 }
 
 // Tuple sources: a caller's index list, or a range of the enumeration in
@@ -371,7 +374,7 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntup) return;
-    i64 x[2 * CLS + 2];
+    i64 x[2 * nel<CLS>() + 2];
     if constexpr (SRC == SRC_LIST) {
         if (!load_tuple<CLS>(S, idx, t, x)) { // out-of-range primitive index: flag the entry
             outE[t] = __builtin_nan("");
@@ -405,7 +408,7 @@ __global__ void __launch_bounds__(256) k_prim_gen(DevSys S, i64 start, const i64
     i64 n = *novf;
     for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
         i64 t = ovf[k];
-        i64 x[2 * CLS + 2];
+        i64 x[2 * nel<CLS>() + 2];
         src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by the first pass
         double e, s;
         eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s);
@@ -458,7 +461,7 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
     }
     for (i64 off = 0; off < ntup; off += PRIM_CHUNK) {
         i64 n = std::min(ntup - off, PRIM_CHUNK);
-        const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * CLS + 2) : nullptr;
+        const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * nel<CLS>() + 2) : nullptr;
         double *e = oe + off, *s = os ? os + off : nullptr;
         if (tiered) {
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, cnt);
@@ -481,12 +484,13 @@ extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, co
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 3) return fail(NEORX_E_ARG, "class %lld not in 1..3", (long long)cls);
+    if (cls < 1 || cls > 4) return fail(NEORX_E_ARG, "class %lld not in 1..4", (long long)cls);
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
     if (cls == 1) return launch_prims<1, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
     if (cls == 2) return launch_prims<2, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
+    if (cls == 4) return launch_prims<4, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
     return launch_prims<3, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
 }
 
@@ -495,9 +499,9 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 3) return fail(NEORX_E_ARG, "class %lld not in 1..3", (long long)cls);
+    if (cls < 1 || cls > 4) return fail(NEORX_E_ARG, "class %lld not in 1..4", (long long)cls);
     double tot = 1.0;
-    for (int i = 0; i < cls; ++i) tot *= (double)s->d.npebf * (double)s->d.npebf;
+    for (int i = 0; i < (cls == 4 ? 2 : cls); ++i) tot *= (double)s->d.npebf * (double)s->d.npebf;
     tot *= (double)s->d.npbf * (double)s->d.npbf;
     if (start < 0 || count < 0 || (double)start + (double)count > tot)
         return fail(NEORX_E_ARG, "range [%lld,%lld) outside the %.0f tuples of class %lld", (long long)start,
@@ -507,6 +511,7 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
     hipStream_t st = (hipStream_t)stream;
     if (cls == 1) return launch_prims<1, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
     if (cls == 2) return launch_prims<2, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
+    if (cls == 4) return launch_prims<4, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
     return launch_prims<3, SRC_SORTED>(s->d, start, count, nullptr, d_ansE, d_ansS, st);
 }
 
@@ -788,13 +793,14 @@ static i64 slab_cap(const System *s, int cls)
 
 // Unsymmetrised contracted e-e-p values of (ip,jp) pairs [p0, p1) into X, Xs
 // (block-local, ne^4 per pair), through primitive slabs in Y, Ys.
+template <int CLS = 2>
 static int build2_block(System *s, i64 p0, i64 p1, double *X, double *Xs, double *Y, double *Ys, int flush,
                         hipStream_t st)
 {
     i64 ne = s->d.nebf, ne4 = ne * ne * ne * ne, cap;
     for (auto &sl : make_slabs(s, 2, p0, p1, cap)) {
         i64 R0 = slab_start(s, 2, sl.first), R1 = slab_start(s, 2, sl.second);
-        int rc = launch_prims<2, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, Ys, st);
+        int rc = launch_prims<CLS, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, Ys, st);
         if (rc) return rc;
         i64 n = (sl.second - sl.first) * ne4;
         hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y, Ys, X, Xs,
@@ -2072,5 +2078,264 @@ extern "C" int neorx_omg4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, 
     }
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(gm4, C.p, ng4 * 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+// ------------------------------------------------------------------
+// RXCHF-ne (source/rxchfne/): RXCHFne_GAM2_IC1 and the contractions of the
+// RXCHF-ne / RXCUHF-ne Fock builds
+// ------------------------------------------------------------------
+// GM2 = contracted ansE1 as is, GM2ex = (x12 + x21)/2 of contracted ansE2
+// (rxchfne_gam2_IC1_OMP.f:169-205)
+__global__ void __launch_bounds__(256) k_rxne_symm(i64 ne, i64 n, const double *__restrict__ X,
+                                                   const double *__restrict__ Xs, double *__restrict__ G,
+                                                   double *__restrict__ Gex)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 ne2 = ne * ne;
+    i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
+    i64 t21 = (b * ne2 + p2) * ne2 + p1;
+    G[t] = X[t];
+    Gex[t] = (Xs[t] + Xs[t21]) / 2.0;
+}
+
+extern "C" int neorx_rxchfne_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2ex, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (!d_gm2 || !d_gm2ex) return fail(NEORX_E_ARG, "null output");
+    i64 ne = s->d.nebf, np = s->d.npbf, ng2 = ne * ne * ne * ne * np * np;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap = slab_cap(s, 2);
+    DevBuf X, Xs, Y, Ys;
+    int rc = X.alloc(ng2 * 8);
+    if (!rc) rc = Xs.alloc(ng2 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = build2_block<4>(s, 0, np * np, X.d(), Xs.d(), Y.d(), Ys.d(), 0, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rxne_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2ex);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfne_gam2_ic1(int64_t handle, double *gm2icr, double *gm2exicr)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * ne * ne * np * np;
+    DevBuf G, Gx;
+    int rc = G.alloc(n * 8);
+    if (!rc) rc = Gx.alloc(n * 8);
+    if (!rc) rc = neorx_rxchfne_gam2_ic1_dev(handle, G.d(), Gx.d(), 0);
+    if (!rc) rc = copy_out(gm2icr, G.d(), n);
+    if (!rc) rc = copy_out(gm2exicr, Gx.d(), n);
+    return rc;
+}
+
+// Block = (ip,jp); thread = (i2,j2) column of the block's ne^2 x ne^2 matrix
+// G[(i1 j1),(i2 j2)].  Per block:
+//   v12(i2,j2) = sum_{i1 j1} DBE(i1,j1) G(i1 j1 i2 j2)
+//   v21(i2,j2) = sum_{i1 j1} DBE(i1,j1) Gx(i1 j2 i2 j1)
+//   R(i1,j1)   = sum_{i2 j2} DT(i2,j2) G(i1 j1 i2 j2) - DX(i2,j2) Gx(i1 j2 i2 j1)
+// (Gx, DX absent in the closed-shell routines).  part[pp] = [v12, v21, R].
+__global__ void __launch_bounds__(1024) k_rxne_slab(i64 ne, const double *__restrict__ DBE,
+                                                    const double *__restrict__ DT, const double *__restrict__ DX,
+                                                    const double *__restrict__ G, const double *__restrict__ Gx,
+                                                    double *__restrict__ part)
+{
+    extern __shared__ double R[]; // [ne^2]
+    i64 pp = blockIdx.x, ne2 = ne * ne, ne4 = ne2 * ne2;
+    const double *g = G + pp * ne4, *gx = Gx ? Gx + pp * ne4 : nullptr;
+    double *out = part + pp * 3 * ne2;
+    for (i64 k = threadIdx.x; k < ne2; k += blockDim.x) R[k] = 0.0;
+    __syncthreads();
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (i64 c0 = 0; c0 < ne2; c0 += blockDim.x) {
+        i64 c = c0 + threadIdx.x;
+        bool on = c < ne2;
+        i64 i2 = on ? c / ne : 0, j2 = on ? c % ne : 0;
+        double dt = on ? DT[j2 * ne + i2] : 0.0, dx = (on && DX) ? DX[j2 * ne + i2] : 0.0;
+        double v12 = 0.0, v21 = 0.0;
+        for (i64 i1 = 0; i1 < ne; ++i1)
+            for (i64 j1 = 0; j1 < ne; ++j1) {
+                double d1 = DBE[j1 * ne + i1];
+                double a = on ? g[(i1 * ne + j1) * ne2 + c] : 0.0;
+                double b = (on && gx) ? gx[((i1 * ne + j2) * ne + i2) * ne + j1] : 0.0;
+                v12 += d1 * a;
+                v21 += d1 * b;
+                double r = wave_sum(dt * a - dx * b);
+                if (lead) atomicAdd(&R[i1 * ne + j1], r);
+            }
+        if (on) {
+            out[c] = v12;
+            out[ne2 + c] = v21;
+        }
+    }
+    __syncthreads();
+    for (i64 k = threadIdx.x; k < ne2; k += blockDim.x) out[2 * ne2 + k] = R[k];
+}
+
+// mode 0: F1 += sum_pp DP (v12 - v21), F2 (may be null) += sum_pp DP v12;
+// mode 1: F1 = FP, FP(pp) += sum DBE R_pp;  mode 2: F1(i1,j1) += sum_pp DP R_pp;
+// every mode: E = sum_pp DP(pp) sum DBE R_pp
+__global__ void k_rxne_reduce(i64 ne, i64 np, int mode, const double *__restrict__ DP, const double *__restrict__ DBE,
+                              const double *__restrict__ part, double *__restrict__ F1, double *__restrict__ F2,
+                              double *__restrict__ E)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 ne2 = ne * ne, np2 = np * np;
+    auto dpv = [&](i64 pp) { return DP[(pp % np) * np + pp / np]; }; // DP(ip,jp), pp = ip*np + jp
+    auto fp_of = [&](i64 pp) {
+        const double *R = part + pp * 3 * ne2 + 2 * ne2;
+        double f = 0.0;
+        for (i64 k = 0; k < ne2; ++k) f += DBE[(k % ne) * ne + k / ne] * R[k]; // R(i1,j1) at i1*ne + j1
+        return f;
+    };
+    if (mode != 1 && t < ne2) {
+        i64 i = t / ne, j = t % ne; // output element (i,j)
+        double f = 0.0, f2 = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) {
+            const double *o = part + pp * 3 * ne2;
+            double dp = dpv(pp);
+            if (mode == 0) {
+                f += dp * (o[t] - o[ne2 + t]);
+                f2 += dp * o[t];
+            } else {
+                f += dp * o[2 * ne2 + t];
+            }
+        }
+        F1[j * ne + i] += f;
+        if (mode == 0 && F2) F2[j * ne + i] += f2;
+    }
+    if (mode == 1 && t < np2) {
+        i64 ip = t / np, jp = t % np;
+        F1[jp * np + ip] += fp_of(t);
+    }
+    if (t == 0) {
+        double e = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) e += dpv(pp) * fp_of(pp);
+        *E = e;
+    }
+}
+
+static int rxne_contract(int mode, i64 ne, i64 np, const double *dbe, const double *dt, const double *dx,
+                         const double *dp, const double *gm2, const double *gm2ex, double *f1, double *f2, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng2 = ne * ne * ne * ne * np * np, nf = mode == 1 ? np * np : ne * ne;
+    DevBuf dB, dT, dX, dP, dG, dGx, dF1, dF2, dE, part;
+    rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dT, dt, ne * ne);
+    if (!rc && dx) rc = h2d(dX, dx, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(dG, gm2, ng2);
+    if (!rc && gm2ex) rc = h2d(dGx, gm2ex, ng2);
+    if (!rc) rc = h2d(dF1, f1, nf);
+    if (!rc && f2) rc = h2d(dF2, f2, ne * ne);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc) rc = part.alloc(np * np * 3 * ne * ne * 8);
+    if (rc) return rc;
+    unsigned nt = (unsigned)std::min<i64>(((ne * ne + 63) / 64) * 64, 1024);
+    hipLaunchKernelGGL(k_rxne_slab, dim3((unsigned)(np * np)), dim3(nt), ne * ne * 8, 0, ne, dB.d(), dT.d(),
+                       dx ? dX.d() : nullptr, dG.d(), gm2ex ? dGx.d() : nullptr, part.d());
+    hipLaunchKernelGGL(k_rxne_reduce, dim3(nblocks(std::max(ne * ne, np * np), 256)), dim3(256), 0, 0, ne, np, mode,
+                       dP.d(), dB.d(), part.d(), dF1.d(), f2 ? dF2.d() : nullptr, dE.d());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f1, dF1.p, nf * 8, hipMemcpyDeviceToHost));
+    if (f2) HIPCHK(hipMemcpy(f2, dF2.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfne_fae_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                                      const double *dp, const double *gm2, double *fae, double *e)
+{
+    return rxne_contract(0, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fae, nullptr, e);
+}
+extern "C" int neorx_rxchfne_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                                     const double *dp, const double *gm2, double *fp, double *e)
+{
+    return rxne_contract(1, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fp, nullptr, e);
+}
+extern "C" int neorx_rxchfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
+                                      const double *dp, const double *gm2, double *fbe, double *e)
+{
+    return rxne_contract(2, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fbe, nullptr, e);
+}
+extern "C" int neorx_rxcuhfne_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
+                                      const double *dbe, const double *dp, const double *gm2, const double *gm2ex,
+                                      double *faalp, double *fabet, double *e)
+{
+    if (!gm2ex || !fabet) return fail(NEORX_E_ARG, "null argument");
+    return rxne_contract(0, nebf, npbf, dbe, datot, daalp, dp, gm2, gm2ex, faalp, fabet, e);
+}
+extern "C" int neorx_rxcuhfne_fp_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
+                                      const double *dbe, const double *dp, const double *gm2, const double *gm2ex,
+                                      double *fp, double *e)
+{
+    if (!gm2ex) return fail(NEORX_E_ARG, "null argument");
+    return rxne_contract(1, nebf, npbf, dbe, datot, daalp, dp, gm2, gm2ex, fp, nullptr, e);
+}
+extern "C" int neorx_rxcuhfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
+                                       const double *dbe, const double *dp, const double *gm2, const double *gm2ex,
+                                       double *fbe, double *e)
+{
+    if (!gm2ex) return fail(NEORX_E_ARG, "null argument");
+    return rxne_contract(2, nebf, npbf, dbe, datot, daalp, dp, gm2, gm2ex, fbe, nullptr, e);
+}
+
+// RXCHFne_FAE_GAMee (rxchfne_hfterms.f:2-60): lane per FAE(i1,j1);
+// part[t] = FAE(i1,j1) increment, energy from a second pass
+__global__ void k_rxne_gamee(i64 ne, const double *__restrict__ Gee, const double *__restrict__ DAE,
+                             double *__restrict__ F, double *__restrict__ xf)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * ne) return;
+    i64 i1 = t / ne, j1 = t % ne, ne2 = ne * ne;
+    double f = 0.0;
+    for (i64 i2 = 0; i2 < ne; ++i2)
+        for (i64 j2 = 0; j2 < ne; ++j2) {
+            double v1 = Gee[(i1 * ne + j1) * ne2 + i2 * ne + j2], v2 = Gee[(i1 * ne + j2) * ne2 + i2 * ne + j1];
+            f += DAE[j2 * ne + i2] * (v1 - 0.5 * v2);
+        }
+    xf[j1 * ne + i1] = f;
+    F[j1 * ne + i1] += f;
+}
+
+__global__ void k_rxne_gamee_e(i64 ne, const double *__restrict__ DAE, const double *__restrict__ xf,
+                               double *__restrict__ E)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double e = 0.0;
+        for (i64 k = 0; k < ne * ne; ++k) e += 0.5 * DAE[k] * xf[k];
+        *E = e;
+    }
+}
+
+extern "C" int neorx_rxchfne_fae_gamee(int64_t nebf, const double *gam_ee, const double *dae, double *fae, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    i64 ne = nebf;
+    if (ne <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    DevBuf dG, dD, dF, dX, dE;
+    rc = h2d(dG, gam_ee, ne * ne * ne * ne);
+    if (!rc) rc = h2d(dD, dae, ne * ne);
+    if (!rc) rc = h2d(dF, fae, ne * ne);
+    if (!rc) rc = dX.alloc(ne * ne * 8);
+    if (!rc) rc = dE.alloc(8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rxne_gamee, dim3(nblocks(ne * ne, 256)), dim3(256), 0, 0, ne, dG.d(), dD.d(), dF.d(), dX.d());
+    hipLaunchKernelGGL(k_rxne_gamee_e, dim3(1), dim3(64), 0, 0, ne, dD.d(), dX.d(), dE.d());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(fae, dF.p, ne * ne * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
     return NEORX_OK;
 }

@@ -148,6 +148,54 @@ module neorx_capi
       real(c_double) :: g2s(*), gee(*), g4(*)
       integer(c_int) :: neorx_omg4_icr
     end function
+    function neorx_rxchfne_gam2_ic1(h, g, gx) bind(C, name="neorx_rxchfne_gam2_ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g(*), gx(*)
+      integer(c_int) :: neorx_rxchfne_gam2_ic1
+    end function
+    function neorx_rxchfne_fae_omg2(ne, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfne_fae_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfne_fae_omg2
+    end function
+    function neorx_rxchfne_fp_omg2(ne, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfne_fp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfne_fp_omg2
+    end function
+    function neorx_rxchfne_fbe_omg2(ne, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfne_fbe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfne_fbe_omg2
+    end function
+    function neorx_rxcuhfne_fe_omg2(ne, np, daa, dat, db, dp, g, gx, fa, fb, e) bind(C, name="neorx_rxcuhfne_fe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: daa(*), dat(*), db(*), dp(*), g(*), gx(*), fa(*), fb(*), e
+      integer(c_int) :: neorx_rxcuhfne_fe_omg2
+    end function
+    function neorx_rxcuhfne_fp_omg2(ne, np, daa, dat, db, dp, g, gx, f, e) bind(C, name="neorx_rxcuhfne_fp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: daa(*), dat(*), db(*), dp(*), g(*), gx(*), f(*), e
+      integer(c_int) :: neorx_rxcuhfne_fp_omg2
+    end function
+    function neorx_rxcuhfne_fbe_omg2(ne, np, daa, dat, db, dp, g, gx, f, e) bind(C, name="neorx_rxcuhfne_fbe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: daa(*), dat(*), db(*), dp(*), g(*), gx(*), f(*), e
+      integer(c_int) :: neorx_rxcuhfne_fbe_omg2
+    end function
+    function neorx_rxchfne_fae_gamee(ne, gee, da, f, e) bind(C, name="neorx_rxchfne_fae_gamee")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne
+      real(c_double) :: gee(*), da(*), f(*), e
+      integer(c_int) :: neorx_rxchfne_fae_gamee
+    end function
   end interface
 contains
   subroutine neorx_check(rc, where)
@@ -464,4 +512,97 @@ subroutine OMG4_ICR(Nchunks, ne, np, ngee, ng2, ng4, GAM_2s, GAM_4)
   close(810)
   call neorx_check(neorx_omg4_icr(ne, np, GAM_2s, gee, GAM_4), 'OMG4_ICR')
   deallocate(gee)
+end subroutine
+
+! ---------------------------------------------------------------------
+! RXCHF-ne (source/rxchfne/)
+! ---------------------------------------------------------------------
+! RXCHFne_GAM2_IC1, source/rxchfne/rxchfne_gam2_IC1_OMP.f:2-243
+subroutine RXCHFne_GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                            KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                            GM2ICR, GM2exICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM2ICR(ng2), GM2exICR(ng2)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'RXCHFne_GAM2_IC1')
+  call neorx_check(neorx_rxchfne_gam2_ic1(h, GM2ICR, GM2exICR), 'RXCHFne_GAM2_IC1')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFne_GAM2_IC1')
+end subroutine
+
+! RXCHFne_FAE_OMG2, source/rxchfne/rxchfne_omg2.f:2-116
+subroutine RXCHFne_FAE_OMG2(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2ICR, FAE, E_AE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), FAE(nebf, nebf), E_AE_OMG2
+  call neorx_check(neorx_rxchfne_fae_omg2(nebf, npbf, DAE, DBE, DP, GM2ICR, FAE, E_AE_OMG2), 'RXCHFne_FAE_OMG2')
+end subroutine
+
+! RXCHFne_FP_OMG2, source/rxchfne/rxchfne_omg2.f:212-314
+subroutine RXCHFne_FP_OMG2(Nchunks, nebf, npbf, ng2, NAE, NBE, DAE, DBE, DP, GM2ICR, FP, E_P_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), FP(npbf, npbf), E_P_OMG2
+  call neorx_check(neorx_rxchfne_fp_omg2(nebf, npbf, DAE, DBE, DP, GM2ICR, FP, E_P_OMG2), 'RXCHFne_FP_OMG2')
+end subroutine
+
+! RXCHFne_FBE_OMG2, source/rxchfne/rxchfne_omg2.f:409-511
+subroutine RXCHFne_FBE_OMG2(Nchunks, nebf, npbf, ng2, NAE, NBE, DAE, DBE, DP, GM2ICR, FBE, E_BE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAE, NBE
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), FBE(nebf, nebf), E_BE_OMG2
+  call neorx_check(neorx_rxchfne_fbe_omg2(nebf, npbf, DAE, DBE, DP, GM2ICR, FBE, E_BE_OMG2), 'RXCHFne_FBE_OMG2')
+end subroutine
+
+! RXCUHFne_FE_OMG2, source/rxchfne/rxcuhfne_omg2.f:2-129 (DAbetE is not used by it)
+subroutine RXCUHFne_FE_OMG2(Nchunks, nebf, npbf, ng2, DAalpE, DAbetE, DAtotE, DBE, DP, GM2ICR, GM2exICR, &
+                            FAalpE, FAbetE, E_AE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAalpE(nebf, nebf), DAbetE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2ICR(ng2), GM2exICR(ng2), FAalpE(nebf, nebf), FAbetE(nebf, nebf), E_AE_OMG2
+  call neorx_check(neorx_rxcuhfne_fe_omg2(nebf, npbf, DAalpE, DAtotE, DBE, DP, GM2ICR, GM2exICR, FAalpE, FAbetE, &
+                   E_AE_OMG2), 'RXCUHFne_FE_OMG2')
+end subroutine
+
+! RXCUHFne_FP_OMG2, source/rxchfne/rxcuhfne_omg2.f:239-345
+subroutine RXCUHFne_FP_OMG2(Nchunks, nebf, npbf, ng2, NAalpE, NBE, DAalpE, DAtotE, DBE, DP, GM2ICR, GM2exICR, &
+                            FP, E_P_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAalpE, NBE
+  real(c_double) :: DAalpE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2ICR(ng2), GM2exICR(ng2), FP(npbf, npbf), E_P_OMG2
+  call neorx_check(neorx_rxcuhfne_fp_omg2(nebf, npbf, DAalpE, DAtotE, DBE, DP, GM2ICR, GM2exICR, FP, E_P_OMG2), &
+                   'RXCUHFne_FP_OMG2')
+end subroutine
+
+! RXCUHFne_FBE_OMG2, source/rxchfne/rxcuhfne_omg2.f:445-551
+subroutine RXCUHFne_FBE_OMG2(Nchunks, nebf, npbf, ng2, NAalpE, NBE, DAalpE, DAtotE, DBE, DP, GM2ICR, GM2exICR, &
+                             FBE, E_BE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAalpE, NBE
+  real(c_double) :: DAalpE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2ICR(ng2), GM2exICR(ng2), FBE(nebf, nebf), E_BE_OMG2
+  call neorx_check(neorx_rxcuhfne_fbe_omg2(nebf, npbf, DAalpE, DAtotE, DBE, DP, GM2ICR, GM2exICR, FBE, E_BE_OMG2), &
+                   'RXCUHFne_FBE_OMG2')
+end subroutine
+
+! RXCHFne_FAE_GAMee, source/rxchfne/rxchfne_hfterms.f:2-60
+subroutine RXCHFne_FAE_GAMee(nebf, ngee, GAM_ee, DAE, FAE, E_AE_GAMee)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, ngee
+  real(c_double) :: GAM_ee(ngee), DAE(nebf, nebf), FAE(nebf, nebf), E_AE_GAMee
+  call neorx_check(neorx_rxchfne_fae_gamee(nebf, GAM_ee, DAE, FAE, E_AE_GAMee), 'RXCHFne_FAE_GAMee')
 end subroutine

@@ -77,6 +77,14 @@ def load():
     for n in ("fae_omg3", "ufp_omg3", "fae_omg4", "ufp_omg4"):
         _sig("neorx_" + n, [_I64] * 4 + [_P] * 6)
     _sig("neorx_omg4_icr", [_I64, _I64, _P, _P, _P])
+    _sig("neorx_rxchfne_gam2_ic1", [_I64, _P, _P])
+    _sig("neorx_rxchfne_gam2_ic1_dev", [_I64, _P, _P, _P])
+    for n in ("fae", "fp", "fbe"):
+        _sig(f"neorx_rxchfne_{n}_omg2", [_I64, _I64] + [_P] * 6)
+    _sig("neorx_rxcuhfne_fe_omg2", [_I64, _I64] + [_P] * 9)
+    for n in ("fp", "fbe"):
+        _sig(f"neorx_rxcuhfne_{n}_omg2", [_I64, _I64] + [_P] * 8)
+    _sig("neorx_rxchfne_fae_gamee", [_I64] + [_P] * 4)
     _sig("neorx_gam2_dscf", [_I64] + [_P] * 8)
     _sig("neorx_gam3_dscf", [_I64] + [_P] * 5)
     _sig("neorx_gam4_dscf", [_I64, _I64] + [_P] * 7)
@@ -171,6 +179,15 @@ class System:
         g = _np.zeros(ne ** 4)
         _check(_lib.neorx_gam_ee(self.handle, _p(g)))
         return g
+
+    def RXCHFne_GAM2_IC1(self):
+        """RXCHFne_GAM2_IC1 (source/rxchfne/rxchfne_gam2_IC1_OMP.f:2):
+        returns (GM2ICR, GM2exICR)."""
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne ** 4 * npb * npb)
+        gx = _np.zeros_like(g)
+        _check(_lib.neorx_rxchfne_gam2_ic1(self.handle, _p(g), _p(gx)))
+        return g, gx
 
     # --- direct-SCF routines (host arrays) ---
     def GAM2_DSCF(self, DE, DP, focke, fockp, se2, sp2):
@@ -324,3 +341,34 @@ def OMG4_ICR(nebf, npbf, gam_2s, gam_ee):
     _check(_lib.neorx_omg4_icr(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
                                _p(g)))
     return g
+
+
+def RXCHFne_OMG2(which, nebf, npbf, DAE, DBE, DP, GM2ICR, F):
+    """RXCHFne_{FAE,FP,FBE}_OMG2 (source/rxchfne/rxchfne_omg2.f), which = 'fae',
+    'fp' or 'fbe'; F accumulates; returns E."""
+    load()
+    e = _np.zeros(1)
+    _check(getattr(_lib, f"neorx_rxchfne_{which}_omg2")(nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
+                                                          _p(_np.ascontiguousarray(GM2ICR)), _p(F), _p(e)))
+    return float(e[0])
+
+
+def RXCUHFne_OMG2(which, nebf, npbf, DAalp, DAtot, DBE, DP, GM2ICR, GM2exICR, F, F2=None):
+    """RXCUHFne_{FE,FP,FBE}_OMG2 (source/rxchfne/rxcuhfne_omg2.f), which = 'fe'
+    (F = FAalpE, F2 = FAbetE), 'fp' or 'fbe'; returns E."""
+    load()
+    e = _np.zeros(1)
+    a = [nebf, npbf, _p(_fort(DAalp)), _p(_fort(DAtot)), _p(_fort(DBE)), _p(_fort(DP)),
+         _p(_np.ascontiguousarray(GM2ICR)), _p(_np.ascontiguousarray(GM2exICR)), _p(F)]
+    if which == "fe":
+        a.append(_p(F2))
+    _check(getattr(_lib, f"neorx_rxcuhfne_{which}_omg2")(*a, _p(e)))
+    return float(e[0])
+
+
+def RXCHFne_FAE_GAMee(nebf, GAM_ee, DAE, FAE):
+    """RXCHFne_FAE_GAMee (source/rxchfne/rxchfne_hfterms.f:2); returns E."""
+    load()
+    e = _np.zeros(1)
+    _check(_lib.neorx_rxchfne_fae_gamee(nebf, _p(_np.ascontiguousarray(GAM_ee)), _p(_fort(DAE)), _p(FAE), _p(e)))
+    return float(e[0])

@@ -741,6 +741,31 @@ static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *
     *ansS = gsg;
 }
 
+/* RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): two outputs,
+ * ansE1 = gVEPg2 + gVEEg1 and ansE2 = gVEEg2, summed over all (k,l). */
+static void rxne_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE1, double *ansE2)
+{
+    const prim_t *b1 = &b[0], *b2 = &b[1], *b3 = &b[2], *k1 = &k[0], *k2 = &k[1], *k3 = &k[2];
+    double gVEPg2 = 0, gVEEg1 = 0, gVEEg2 = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il], bb = S->bcoef[ik] * S->bcoef[il];
+            /* G3_MD_xgVeeg(p, e2, e1): 1/r(p,e2), geminal p-e1 on bra (gk) and ket (gl) */
+            double vep = g3(OP_COUL, b3, b2, b1, k3, k2, k1, 0, gk, 0, 0, gl, 0, NULL, 1.0);
+            underflow(&vep);
+            vep = -vep;
+            double vee1 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, gl, 0, NULL, 1.0);
+            underflow(&vee1);
+            double vee2 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            underflow(&vee2);
+            gVEPg2 += bb * vep;
+            gVEEg1 += bb * vee1;
+            gVEEg2 += bb * vee2;
+        }
+    *ansE1 = gVEPg2 + gVEEg1;
+    *ansE2 = gVEEg2;
+}
+
 /* xcalc_GAM3_MD (gam_3_OMP.f:617-983): particles e1, e2, e3, p */
 static void gam3_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
 {
@@ -833,12 +858,14 @@ void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double
 {
     BAS_INIT;
     SYS_INIT;
-    int w = (int)(2 * (cls + 1));
+    /* cls 4: the RXCHF-ne e-e-p tuple (two electrons, like cls 2) */
+    int nel = cls == 4 ? 2 : (int)cls;
+    int w = 2 * (nel + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
         const i64 *x = idx + t * w;
         prim_t b[4], k[4];
-        int ne = (int)cls;
+        int ne = nel;
         for (int i = 0; i < ne; ++i) {
             b[i] = eprim(&B, x[i]);
             k[i] = eprim(&B, x[ne + 1 + i]);
@@ -850,6 +877,8 @@ void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double
             gam1_prim(&S, &b[0], &b[1], &k[0], &k[1], &e, &s);
         else if (cls == 2)
             gam2_prim(&S, b, k, &e, &s);
+        else if (cls == 4)
+            rxne_prim(&S, b, k, &e, &s);
         else
             gam3_prim(&S, b, k, &e);
         outE[t] = e;
@@ -1601,3 +1630,157 @@ void orc_omg4_icr(i64 nebf, i64 npbf, const double *GAM2s, const double *GAMee, 
 #undef DAm
 #undef DBm
 #undef DPm
+
+/* ------------------------------------------------------------------------
+ * RXCHF-ne (source/rxchfne/): two-output e-e-p tuples, their contracted
+ * tensors and the contractions of the RXCHF-ne / RXCUHF-ne Fock builds.
+ * ------------------------------------------------------------------------ */
+/* RXCHFne_GAM2_IC1 (rxchfne_gam2_IC1_OMP.f:2-243) with
+ * RXCHFne_contract_omega2_conv (rxchfne_gam2_CONV_OMP.f:2-200):
+ * GM2 = contracted ansE1 (not symmetrised), GM2ex = (x12 + x21)/2 of ansE2 */
+void orc_rxchfne_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2ex)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng2 = ne * ne * ne * ne * np * np;
+    double *X = (double *)malloc(sizeof(double) * ng2);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            double om = 0.0, omx = 0.0;
+                            for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                        for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2) {
+                                            prim_t b[3] = {eprim(&B, a1), eprim(&B, a2), pprim(&B, ip)};
+                                            prim_t k[3] = {eprim(&B, c1), eprim(&B, c2), pprim(&B, jp)};
+                                            double e1, e2;
+                                            rxne_prim(&S, b, k, &e1, &e2);
+                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                        agebfcc[a2] * agebfcc[c2];
+                                            om += e1 * cf;
+                                            omx += e2 * cf;
+                                        }
+                            i64 ia = ix2(ne, np, ip, jp, i1, j1, i2, j2);
+                            GM2[ia] = om;
+                            X[ia] = omx;
+                        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i2, j2, i1, j1);
+                            GM2ex[a12] = (X[a12] + X[a21]) / 2.0;
+                        }
+    free(X);
+}
+
+#define DAm(i, j) DAE[(j) * ne + (i)]
+#define DBm(i, j) DBE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+#define LOOP2                                                                                                      \
+    for (i64 ip = 0; ip < np; ++ip)                                                                                \
+        for (i64 jp = 0; jp < np; ++jp)                                                                            \
+            for (i64 i1 = 0; i1 < ne; ++i1)                                                                        \
+                for (i64 j1 = 0; j1 < ne; ++j1)                                                                    \
+                    for (i64 i2 = 0; i2 < ne; ++i2)                                                                \
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+
+/* RXCHFne_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2 (rxchfne_omg2.f:2-600); which = 0,
+ * 1, 2 selects the routine; F accumulates, E overwritten.  DAE: regular,
+ * DBE: special electrons. */
+void orc_rxchfne_omg2(i64 which, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                      const double *GM2, double *F, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+    LOOP2
+    {
+        double val = GM2[ix2(ne, np, ip, jp, i1, j1, i2, j2)];
+        if (which == 0) {
+            xf[j2 * ne + i2] += DPm(ip, jp) * DBm(i1, j1) * val;
+            e += DPm(ip, jp) * DAm(i2, j2) * DBm(i1, j1) * val;
+        } else if (which == 1) {
+            xf[jp * np + ip] += DAm(i2, j2) * DBm(i1, j1) * val;
+            e += DPm(ip, jp) * DAm(i2, j2) * DBm(i1, j1) * val;
+        } else {
+            xf[j1 * ne + i1] += DAm(i2, j2) * DPm(ip, jp) * val;
+            e += DPm(ip, jp) * DAm(i2, j2) * DBm(i1, j1) * val;
+        }
+    }
+    for (i64 k = 0; k < nf; ++k) F[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+#undef DAm
+
+/* RXCUHFne_FE_OMG2 / _FP_OMG2 / _FBE_OMG2 (rxcuhfne_omg2.f:2-650):
+ * val12 = GM2(i1 j1 i2 j2), val21 = GM2ex(i1 j2 i2 j1); DAa: regular alpha
+ * density, DAt: regular total density.  which = 0 fills FAa and FAb (alpha
+ * and beta regular Fock), 1 fills F = FP, 2 fills F = FBE. */
+void orc_rxcuhfne_omg2(i64 which, i64 nebf, i64 npbf, const double *DAa, const double *DAt, const double *DBE,
+                       const double *DP, const double *GM2, const double *GM2ex, double *F, double *F2, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double)), *xf2 = (double *)calloc(ne * ne, sizeof(double));
+    double e = 0.0;
+#define DAam(i, j) DAa[(j) * ne + (i)]
+#define DAtm(i, j) DAt[(j) * ne + (i)]
+    LOOP2
+    {
+        double v12 = GM2[ix2(ne, np, ip, jp, i1, j1, i2, j2)], v21 = GM2ex[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+        double t = DAtm(i2, j2) * v12 - DAam(i2, j2) * v21;
+        if (which == 0) {
+            xf[j2 * ne + i2] += DPm(ip, jp) * DBm(i1, j1) * (v12 - v21);
+            xf2[j2 * ne + i2] += DPm(ip, jp) * DBm(i1, j1) * v12;
+        } else if (which == 1) {
+            xf[jp * np + ip] += DBm(i1, j1) * t;
+        } else {
+            xf[j1 * ne + i1] += DPm(ip, jp) * t;
+        }
+        e += DPm(ip, jp) * DBm(i1, j1) * t;
+    }
+#undef DAam
+#undef DAtm
+    for (i64 k = 0; k < nf; ++k) F[k] += xf[k];
+    if (which == 0)
+        for (i64 k = 0; k < ne * ne; ++k) F2[k] += xf2[k];
+    *E = e;
+    free(xf);
+    free(xf2);
+}
+#undef DBm
+#undef DPm
+
+/* RXCHFne_FAE_GAMee (rxchfne_hfterms.f:2-60): HF Coulomb/exchange of the
+ * regular electrons from GAM_ee (pack_4D order) */
+void orc_rxchfne_fae_gamee(i64 nebf, const double *GAMee, const double *DAE, double *FAE, double *E)
+{
+    i64 ne = nebf;
+    double *xf = (double *)calloc(ne * ne, sizeof(double));
+    double e = 0.0;
+#define EE(a, b, c, d) GAMee[(((a) * ne + (b)) * ne + (c)) * ne + (d)]
+    for (i64 i2 = 0; i2 < ne; ++i2)
+        for (i64 j2 = 0; j2 < ne; ++j2)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1) {
+                    /* pack_4D(ne,ne,ne, je2,ie2,je1,ie1) and (je1,ie2,je2,ie1) */
+                    double v1 = EE(i1, j1, i2, j2), v2 = EE(i1, j2, i2, j1);
+                    double val = v1 - 0.5 * v2;
+                    xf[j1 * ne + i1] += DAE[j2 * ne + i2] * val;
+                    e += 0.5 * DAE[j1 * ne + i1] * DAE[j2 * ne + i2] * val;
+                }
+#undef EE
+    for (i64 k = 0; k < ne * ne; ++k) FAE[k] += xf[k];
+    *E = e;
+    free(xf);
+}

@@ -188,9 +188,11 @@ def _d(v):
 
 
 def ref_prim_list(s, cls, idx):
-    """xcalc_GAM{1,2,3}_MD of the reference for each tuple."""
+    """xcalc_GAM{1,2,3}_MD of the reference for each tuple (cls 4:
+    RXCHFne_xcalc_GAM2_MD, two outputs returned as E and S)."""
     R = ref()
-    fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_}[cls]
+    fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_}[cls]
+    nel = 2 if cls == 4 else cls
     am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
     c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
     cat = np.ascontiguousarray(s["cat"])
@@ -205,16 +207,16 @@ def ref_prim_list(s, cls, idx):
             keep.append(c)
             return [_i8(am[i, 0]), _i8(am[i, 1]), _i8(am[i, 2]), _d(ex[i]), _p(c)]
 
-        for i in range(cls):
+        for i in range(nel):
             args += prim(am_e, s["elcex"], c_e, x[i])
-        args += prim(am_p, s["nucex"], c_p, x[cls])
-        for i in range(cls):
-            args += prim(am_e, s["elcex"], c_e, x[cls + 1 + i])
-        args += prim(am_p, s["nucex"], c_p, x[2 * cls + 1])
+        args += prim(am_p, s["nucex"], c_p, x[nel])
+        for i in range(nel):
+            args += prim(am_e, s["elcex"], c_e, x[nel + 1 + i])
+        args += prim(am_p, s["nucex"], c_p, x[2 * nel + 1])
         e = C.c_double()
         sv = C.c_double()
         tail = [_i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g), C.byref(e)]
-        if cls < 3:
+        if cls != 3:
             tail.append(C.byref(sv))
         fn(*args, *tail)
         out_e[t] = e.value
@@ -478,3 +480,86 @@ def ref_omg4_icr(nebf, npbf, g2s, gee, workdir):
     _in_dir(workdir, lambda: ref().omg4_icr_(_i8(1), _i8(nebf), _i8(npbf), _i8(ngee), _i8(ng2), _i8(g4.size),
                                              _p(np.ascontiguousarray(g2s)), _p(g4)))
     return g4
+
+
+# ---------------------------------------------------------------------------
+# RXCHF-ne (oracle and reference)
+# ---------------------------------------------------------------------------
+def rxchfne_gam2_ic1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne ** 4 * npb * npb)
+    gx = np.zeros_like(g)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfne_gam2_ic1(*ba, *sa, _p(g), _p(gx))
+    return g, gx
+
+
+def rxchfne_omg2(which, ne, npb, DAE, DBE, DP, G, F):
+    """which: 0 RXCHFne_FAE_OMG2, 1 RXCHFne_FP_OMG2, 2 RXCHFne_FBE_OMG2."""
+    e = np.zeros(1)
+    lib().orc_rxchfne_omg2(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                           _p(np.ascontiguousarray(G)), _p(F), _p(e))
+    return float(e[0])
+
+
+def rxcuhfne_omg2(which, ne, npb, DAa, DAt, DBE, DP, G, Gx, F, F2=None):
+    """which: 0 RXCUHFne_FE_OMG2 (F = FAalpE, F2 = FAbetE), 1 _FP_OMG2, 2 _FBE_OMG2."""
+    e = np.zeros(1)
+    lib().orc_rxcuhfne_omg2(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAa), _f(DAt), _f(DBE), _f(DP),
+                            _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gx)), _p(F), _p(F2), _p(e))
+    return float(e[0])
+
+
+def rxchfne_fae_gamee(ne, gee, DAE, F):
+    e = np.zeros(1)
+    lib().orc_rxchfne_fae_gamee(C.c_int64(ne), _p(np.ascontiguousarray(gee)), _f(DAE), _p(F), _p(e))
+    return float(e[0])
+
+
+def ref_rxchfne_gam2_ic1(s):
+    """RXCHFne_GAM2_IC1 (source/rxchfne/rxchfne_gam2_IC1_OMP.f:2)."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng2 = ne ** 4 * npb * npb
+    g, gx = np.zeros(ng2), np.zeros(ng2)
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfne_gam2_ic1_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng2), _i8(0), _i8(s["nat"]),
+                            _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep],
+                            _p(g), _p(gx))
+    return g, gx
+
+
+def ref_rxchfne_omg2(which, ne, npb, DAE, DBE, DP, G, F, nae=1, nbe=2):
+    e = C.c_double()
+    ng2 = ne ** 4 * npb * npb
+    R = ref()
+    Gp = _p(np.ascontiguousarray(G))
+    if which == 0:
+        R.rxchfne_fae_omg2_(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _f(DAE), _f(DBE), _f(DP), Gp, _p(F), C.byref(e))
+    else:
+        fn = R.rxchfne_fp_omg2_ if which == 1 else R.rxchfne_fbe_omg2_
+        fn(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _i8(nae), _i8(nbe), _f(DAE), _f(DBE), _f(DP), Gp, _p(F), C.byref(e))
+    return e.value
+
+
+def ref_rxcuhfne_omg2(which, ne, npb, DAa, DAt, DBE, DP, G, Gx, F, F2=None, nab=1, nbe=1):
+    e = C.c_double()
+    ng2 = ne ** 4 * npb * npb
+    R = ref()
+    Gp, Gxp = _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(Gx))
+    if which == 0:
+        DAb = np.asfortranarray(np.asarray(DAt) - np.asarray(DAa))
+        R.rxcuhfne_fe_omg2_(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _f(DAa), _p(DAb), _f(DAt), _f(DBE), _f(DP), Gp,
+                            Gxp, _p(F), _p(F2), C.byref(e))
+    else:
+        fn = R.rxcuhfne_fp_omg2_ if which == 1 else R.rxcuhfne_fbe_omg2_
+        fn(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _i8(nab), _i8(nbe), _f(DAa), _f(DAt), _f(DBE), _f(DP), Gp, Gxp,
+           _p(F), C.byref(e))
+    return e.value
+
+
+def ref_rxchfne_fae_gamee(ne, gee, DAE, F):
+    e = C.c_double()
+    ref().rxchfne_fae_gamee_(_i8(ne), _i8(ne ** 4), _p(np.ascontiguousarray(gee)), _f(DAE), _p(F), C.byref(e))
+    return e.value

@@ -55,6 +55,7 @@ static void eval_tuple(const SysParams &S, int cls, const PairRec *q, int tier, 
     e = s = 0.0;
     if (cls == 1) eval_one<1>(S, qp, tier, e, s);
     else if (cls == 2) eval_one<2>(S, qp, tier, e, s);
+    else if (cls == 4) eval_one<4>(S, qp, tier, e, s);
     else eval_one<3>(S, qp, tier, e, s);
 }
 
@@ -65,13 +66,13 @@ extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const do
                               i64 tier)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = (int)cls, w = 2 * (ne + 1);
+    int ne = cls == 4 ? 2 : (int)cls, w = 2 * (ne + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];
         pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
         double e, s;
-        eval_tuple(S, ne, q, (int)tier, e, s);
+        eval_tuple(S, (int)cls, q, (int)tier, e, s);
         outE[t] = e;
         if (outS) outS[t] = s;
     }
@@ -118,13 +119,13 @@ extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, 
                                       const double *gamma, i64 cls, i64 ntup, const i64 *idx)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = (int)cls, w = 2 * (ne + 1);
+    int ne = cls == 4 ? 2 : (int)cls, w = 2 * (ne + 1);
     nrx_flops = 0.0;
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];
         pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
         double e, s;
-        eval_tuple(S, ne, q, 0, e, s);
+        eval_tuple(S, (int)cls, q, 0, e, s);
     }
     return nrx_flops / (double)ntup;
 }

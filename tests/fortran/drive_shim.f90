@@ -115,5 +115,23 @@ program drive_shim
   fp = 0.0d0
   call UFP_OMG4(one, nebf, npbf, ng4, nae, nbe, dae, dbe, dp, gm4, fp, e4)
   write(11) fp, e4
+  ! RXCHF-ne (rxchfne_fock.f:391, :446, :501; rxcuhfne_fock.f:376): DBE here
+  ! is the special-electron density (dbe), DAE the regular one (dae)
+  call RXCHFne_GAM2_IC1(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                        kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm2, gm2s)
+  write(11) gm2, gm2s
+  fe = 0.0d0
+  call RXCHFne_FAE_OMG2(one, nebf, npbf, ng2, dae, dbe, dp, gm2, fe, e2)
+  write(11) fe, e2
+  fp = 0.0d0
+  call RXCHFne_FP_OMG2(one, nebf, npbf, ng2, nae, nbe, dae, dbe, dp, gm2, fp, e2)
+  write(11) fp, e2
+  fe = 0.0d0
+  call RXCHFne_FBE_OMG2(one, nebf, npbf, ng2, nae, nbe, dae, dbe, dp, gm2, fe, e2)
+  write(11) fe, e2
+  fe = 0.0d0
+  se = 0.0d0
+  call RXCUHFne_FE_OMG2(one, nebf, npbf, ng2, dbe, dae, de, dbe, dp, gm2, gm2s, fe, se, e2)
+  write(11) fe, se, e2
   close(11)
 end program

@@ -78,3 +78,15 @@ def test_engine_boys_vs_reference_semantics():
                 acc += term
             exact = float(acc * (-Td).exp())
             assert abs(F[m] - exact) <= 4e-15 * exact, (m, T, F[m], exact)
+
+
+def test_engine_rxchfne_tuples_vs_oracle():
+    """RXCHFne_xcalc_GAM2_MD (two outputs) in the host build of the engine
+    against the oracle, both capacity tiers."""
+    s = synth.small(seed=47, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    idx = random_tuples(s, 2, 200, seed=8)
+    oe, os_ = oracle.prim_list(s, 4, idx)
+    for tier in (0, 1):
+        ee, es = engine_prim_list(s, 4, idx, tier)
+        check_close(ee, oe, what=f"RXCHF-ne ansE1 tier {tier}")
+        check_close(es, os_, what=f"RXCHF-ne ansE2 tier {tier}")

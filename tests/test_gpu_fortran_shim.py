@@ -79,7 +79,10 @@ def test_fortran_dropin_matches_oracle():
              ne * ne, ne * ne, 1, 1,  # FAE_OMG2: FAE, SAE2, E, S
              npb * npb, npb * npb, 1, 1,  # UFP_OMG2
              ne * ne, 1, npb * npb, 1,  # FAE_OMG3, UFP_OMG3
-             ne * ne, 1, npb * npb, 1]  # FAE_OMG4, UFP_OMG4
+             ne * ne, 1, npb * npb, 1,  # FAE_OMG4, UFP_OMG4
+             ng2, ng2,  # RXCHFne_GAM2_IC1
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFne_FAE/FP/FBE_OMG2
+             ne * ne, ne * ne, 1]  # RXCUHFne_FE_OMG2
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -144,3 +147,18 @@ def test_fortran_dropin_matches_oracle():
         e = fn(*a, G, F)
         check_close(parts[37 + 2 * k], F.ravel(order="F"), what=fn.__name__)
         assert parts[38 + 2 * k][0] == pytest.approx(e, rel=1e-11, abs=1e-14), fn.__name__
+    # RXCHF-ne pieces: regular density DAE = 2/3 DE, special DBE = 1/3 DE
+    rg, rgx = oracle.rxchfne_gam2_ic1(s)
+    check_close(parts[45], rg, what="RXCHFne_GAM2_IC1 GM2")
+    check_close(parts[46], rgx, what="RXCHFne_GAM2_IC1 GM2ex")
+    for k in range(3):
+        n = npb if k == 1 else ne
+        F = np.zeros((n, n), order="F")
+        e = oracle.rxchfne_omg2(k, ne, npb, DAE, DBE, DP, rg, F)
+        check_close(parts[47 + 2 * k], F.ravel(order="F"), what=f"RXCHFne omg2 {k}")
+        assert parts[48 + 2 * k][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    Fa, Fb = np.zeros((ne, ne), order="F"), np.zeros((ne, ne), order="F")
+    e = oracle.rxcuhfne_omg2(0, ne, npb, DBE, DE, DBE, DP, rg, rgx, Fa, Fb)  # DAalpE, DAtotE as the driver passes
+    check_close(parts[53], Fa.ravel(order="F"), what="RXCUHFne_FE_OMG2 alpha")
+    check_close(parts[54], Fb.ravel(order="F"), what="RXCUHFne_FE_OMG2 beta")
+    assert parts[55][0] == pytest.approx(e, rel=1e-11, abs=1e-14)

@@ -335,3 +335,52 @@ def test_omg4_icr(torch_dev):
     gee = oracle.gam_ee(s)
     check_close(neorx.OMG4_ICR(s["nebf"], s["npbf"], g2s, gee), oracle.omg4_icr(s["nebf"], s["npbf"], g2s, gee),
                 what="OMG4_ICR")
+
+
+def test_rxchfne(torch_dev):
+    """RXCHF-ne: class-4 tuples, RXCHFne_GAM2_IC1 and the RXCHF-ne /
+    RXCUHF-ne contractions through the C ABI against the oracle."""
+    torch = torch_dev
+    s = synth.small(seed=48, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    S = neorx.System(s)
+    idx = random_tuples(s, 2, 512, seed=9)
+    ge, gs = _run_list(torch, S, 4, idx)
+    oe, os_ = oracle.prim_list(s, 4, idx)
+    check_close(ge, oe, what="RXCHF-ne ansE1")
+    check_close(gs, os_, what="RXCHF-ne ansE2")
+    S.close()
+    t = synth.small(seed=49, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    T = neorx.System(t)
+    g, gx = T.RXCHFne_GAM2_IC1()
+    og, ogx = oracle.rxchfne_gam2_ic1(t)
+    check_close(g, og, what="RXCHFne_GAM2_IC1 GM2")
+    check_close(gx, ogx, what="RXCHFne_GAM2_IC1 GM2ex")
+    T.close()
+    ne, npb = t["nebf"], t["npbf"]
+    rng = np.random.default_rng(12)
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    DAa = synth.density(ne, 1.0, 4)
+    for k, which in enumerate(("fae", "fp", "fbe")):
+        n = npb if which == "fp" else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        eg = neorx.RXCHFne_OMG2(which, ne, npb, DA, DB, DP, og, A)
+        eo = oracle.rxchfne_omg2(k, ne, npb, DA, DB, DP, og, B)
+        check_close(A, B, what=f"RXCHFne_{which}_OMG2")
+        assert eg == pytest.approx(eo, rel=1e-11, abs=1e-15)
+        wu = {"fae": "fe", "fp": "fp", "fbe": "fbe"}[which]
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        A2 = np.asfortranarray(rng.standard_normal((ne, ne)))
+        B2 = A2.copy(order="F")
+        eg = neorx.RXCUHFne_OMG2(wu, ne, npb, DAa, DA, DB, DP, og, ogx, A, A2)
+        eo = oracle.rxcuhfne_omg2(k, ne, npb, DAa, DA, DB, DP, og, ogx, B, B2)
+        check_close(A, B, what=f"RXCUHFne_{wu}_OMG2")
+        if wu == "fe":
+            check_close(A2, B2, what="RXCUHFne_FE_OMG2 beta")
+        assert eg == pytest.approx(eo, rel=1e-11, abs=1e-15)
+    gee = oracle.gam_ee(t)
+    F0 = np.asfortranarray(rng.standard_normal((ne, ne)))
+    A, B = F0.copy(order="F"), F0.copy(order="F")
+    assert neorx.RXCHFne_FAE_GAMee(ne, gee, DA, A) == pytest.approx(oracle.rxchfne_fae_gamee(ne, gee, DA, B),
+                                                                   rel=1e-11)
+    check_close(A, B, what="RXCHFne_FAE_GAMee")

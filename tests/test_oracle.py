@@ -279,3 +279,46 @@ def test_oracle_omg4_icr_vs_reference(tmp_path):
     ue = oracle.gam_ee(u)
     check_close(oracle.omg4_icr(u["nebf"], u["npbf"], g2s, ue),
                 oracle.ref_omg4_icr(u["nebf"], u["npbf"], g2s, ue, str(tmp_path)), what="OMG4_ICR")
+
+
+@needs_ref
+def test_oracle_rxchfne_vs_reference():
+    """RXCHF-ne: the two-output e-e-p tuple, RXCHFne_GAM2_IC1 and the
+    RXCHF-ne / RXCUHF-ne contractions of the oracle against the reference's
+    own routines (source/rxchfne/)."""
+    s = synth.small(seed=44, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    idx = random_tuples(s, 2, 16, seed=4)
+    re1, re2 = oracle.ref_prim_list(s, 4, idx)
+    oe1, oe2 = oracle.prim_list(s, 4, idx)
+    check_close(oe1, re1, what="RXCHFne ansE1")
+    check_close(oe2, re2, what="RXCHFne ansE2")
+    t = synth.small(seed=45, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    g, gx = oracle.rxchfne_gam2_ic1(t)
+    rg, rgx = oracle.ref_rxchfne_gam2_ic1(t)
+    check_close(g, rg, what="RXCHFne_GAM2_IC1 GM2")
+    check_close(gx, rgx, what="RXCHFne_GAM2_IC1 GM2ex")
+    ne, npb = t["nebf"], t["npbf"]
+    rng = np.random.default_rng(3)
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    DAa = synth.density(ne, 1.0, 4)
+    for which in (0, 1, 2):
+        n = npb if which == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        assert oracle.rxchfne_omg2(which, ne, npb, DA, DB, DP, g, A) == pytest.approx(
+            oracle.ref_rxchfne_omg2(which, ne, npb, DA, DB, DP, g, B), rel=1e-12)
+        check_close(A, B, what=f"RXCHFne omg2 {which}")
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        A2 = np.asfortranarray(rng.standard_normal((ne, ne)))
+        B2 = A2.copy(order="F")
+        assert oracle.rxcuhfne_omg2(which, ne, npb, DAa, DA, DB, DP, g, gx, A, A2) == pytest.approx(
+            oracle.ref_rxcuhfne_omg2(which, ne, npb, DAa, DA, DB, DP, g, gx, B, B2), rel=1e-12)
+        check_close(A, B, what=f"RXCUHFne omg2 {which}")
+        if which == 0:
+            check_close(A2, B2, what="RXCUHFne FE beta")
+    gee = oracle.gam_ee(t)
+    F0 = np.asfortranarray(rng.standard_normal((ne, ne)))
+    A, B = F0.copy(order="F"), F0.copy(order="F")
+    assert oracle.rxchfne_fae_gamee(ne, gee, DA, A) == pytest.approx(oracle.ref_rxchfne_fae_gamee(ne, gee, DA, B),
+                                                                    rel=1e-12)
+    check_close(A, B, what="RXCHFne_FAE_GAMee")
