@@ -607,17 +607,19 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
     double pre = fac * 2.0 * pic / w;
     double tot = 0.0;
     for (int c = 0; c < nat; ++c) {
+        // product over directions, sized as it grows: x*y at capacity
+        // 2NL-1 (capped at NB), then *z at NB
+        constexpr int N2 = (2 * NL - 1 < NB) ? 2 * NL - 1 : NB;
+        double Dx = X[0] - cat[3 * c], Dy = X[1] - cat[3 * c + 1], Dz = X[2] - cat[3 * c + 2];
+        PolyT<NL> Px, Py;
+        hermite_coulomb(e[0], Dx, 0.0, w, Px);
+        hermite_coulomb(e[1], Dy, 0.0, w, Py);
+        PolyT<N2> cxy;
+        p_mul(Px, Py, cxy);
+        hermite_coulomb(e[2], Dz, 0.0, w, Px);
         PolyT<NB> cc;
-        p_const(cc, 1.0);
-        double D2 = 0.0;
-        NRX_UNROLL for (int d = 0; d < 3; ++d) {
-            double D = X[d] - cat[3 * c + d];
-            PolyT<NL> P;
-            hermite_coulomb(e[d], D, 0.0, w, P);
-            p_mul(cc, P, cc);
-            D2 += D * D;
-        }
-        double T = w * D2;
+        p_mul(cxy, Px, cc);
+        double T = w * (((0.0 + Dx * Dx) + Dy * Dy) + Dz * Dz);
         double v = pre * boys_dot(cc, T, s.bt);
         if (flush) underflow(v);
         tot += zan[c] * v;
