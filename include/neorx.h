@@ -237,6 +237,79 @@ int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const double *d_dp, 
 int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_gam2s, const double *d_gamee, const double *d_de,
                         const double *d_dp, double *d_fe, double *d_fp, double *d_e, void *stream);
 
+/* ---- RXCHFmult (reference semantics, source/rxchfmult/) -------------------
+ * More than one special electron: a regular electron basis (nebf functions,
+ * npebf primitives) and a special-electron basis (nebfBE, npebfBE).  Tensors
+ * use the RXCHFmult packings, rxchfmult/rxchfmult_basis.f:2-134: C row-major
+ * [ip][jp][ie1][je1] over the regular basis, then [ie][je] over the special
+ * basis for every further electron.  The routines that read regular-basis
+ * slots with special-basis indices (XCHF_GAM2, GAM4_INT) need
+ * nebfBE <= nebf, as the reference's arrays do. */
+/* System handle over both bases; the arguments are those of RXCHFmult_GAM2,
+ * rxchfmult/rxchfmult_gam2.f:2-12 (the _be arrays: special-electron basis). */
+int neorx_mult_system_create(int64_t nebf, int64_t npebf, int64_t nebfbe, int64_t npebfbe, int64_t npbf, int64_t nat,
+                             int64_t ngtg, double pmass, const double *cat, const double *zan, const double *bcoef,
+                             const double *gamma, const int64_t *kpestr, const int64_t *kpeend,
+                             const int64_t *ampeb2c, const double *agebfcc, const double *elcex,
+                             const int64_t *elcam, const double *elcbfc, const int64_t *kpestr_be,
+                             const int64_t *kpeend_be, const int64_t *ampeb2c_be, const double *agebfcc_be,
+                             const double *elcex_be, const int64_t *elcam_be, const double *elcbfc_be,
+                             const double *agnbfcc, const double *nucex, const int64_t *nucam,
+                             const double *nucbfc, int64_t *handle);
+/* Primitive tuples of a two-basis handle: primitive indices are over the
+ * regular primitives followed by the special ones (0-based).
+ *   cls 5 (e-e-p)   : RXCHFmult_GAM2_MD, rxchfmult/rxchfmult_gam2.f:627 -> ansE1, ansS, ansE2
+ *   cls 6 (e-e-e-p) : RXCHFmult_INT_GAM3_MD, rxchfmult/rxchfmult_gam3-int.f:661 -> ans
+ * Index rows as for cls 2 and cls 3.  neorx_prim_list_dev also accepts them
+ * (ansE2 dropped); d_ansE2 receives zero for the other classes. */
+int neorx_prim_list3_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
+                         double *d_ansS, double *d_ansE2, void *stream);
+/* RXCHFmult_GAM2, rxchfmult/rxchfmult_gam2.f:2: INT_GAM2 (npbf^2 nebf^2
+ * nebfBE^2, contracted ansE2, unsymmetrised), XCHF_GAM2 and XCHF_GAM2s
+ * (npbf^2 nebfBE^4, index_GAM_2PK packing: (x12 + x21)/2 of ansE1 and ansS) */
+int neorx_rxchfmult_gam2(int64_t handle, double *int_gam2, double *xchf_gam2, double *xchf_gam2s);
+int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, double *d_xchf_gam2, double *d_xchf_gam2s,
+                             void *stream);
+/* RXCHFmult_GAM3_INT, rxchfmult/rxchfmult_gam3-int.f:2: INT_GAM3
+ * (npbf^2 nebf^2 nebfBE^4), averaged over the exchange of electrons 2 and 3 */
+int neorx_rxchfmult_gam3_int(int64_t handle, double *int_gam3);
+int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, void *stream);
+/* RXCHFmult_GAM4_INT, rxchfmult/rxchfmult_gam4-int.f:2: INT_GAM4
+ * (npbf^2 nebf^2 nebfBE^6) from GAM_2s (= XCHF_GAM2s) and GAM_ee (nebf^4;
+ * the reference reads GAM_ee.ufm) */
+int neorx_rxchfmult_gam4_int(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *gam_2s, const double *gam_ee,
+                             double *gam4);
+int neorx_rxchfmult_gam4_int_dev(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *d_gam_2s,
+                                 const double *d_gam_ee, double *d_gam4, void *stream);
+/* RXCHFmult_FAE/FP/FBE_OMG2, rxchfmult/rxchfmult_omg2.f:2, :175, :348;
+ * _OMG3, rxchfmult_omg3.f:2, :201, :400; _OMG4, rxchfmult_omg4.f:2, :207,
+ * :412.  dae (nebf^2), dbe (nebfBE^2), dp (npbf^2); gm the INT tensor of
+ * the order; f (FAE, FP or FBE) accumulates (add2fock), e overwritten. */
+int neorx_rxchfmult_fae_omg2(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fae, double *e);
+int neorx_rxchfmult_fp_omg2(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                            const double *dp, const double *gm, double *fp, double *e);
+int neorx_rxchfmult_fbe_omg2(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fbe, double *e);
+int neorx_rxchfmult_fae_omg3(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fae, double *e);
+int neorx_rxchfmult_fp_omg3(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                            const double *dp, const double *gm, double *fp, double *e);
+int neorx_rxchfmult_fbe_omg3(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fbe, double *e);
+int neorx_rxchfmult_fae_omg4(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fae, double *e);
+int neorx_rxchfmult_fp_omg4(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                            const double *dp, const double *gm, double *fp, double *e);
+int neorx_rxchfmult_fbe_omg4(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
+                             const double *dp, const double *gm, double *fbe, double *e);
+/* The three contractions of one order k (1: OMG2, 2: OMG3, 3: OMG4) in one
+ * pass over the device tensor d_gm; any of d_fae, d_fp, d_fbe may be NULL.
+ * d_e receives the energy all three reference routines return. */
+int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, int64_t npbf, const double *d_dae,
+                            const double *d_dbe, const double *d_dp, const double *d_gm, double *d_fae,
+                            double *d_fp, double *d_fbe, double *d_e, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -838,12 +838,17 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
     ansS = 2.0 * ogx + ogg;
 }
 
-// xcalc_GAM2_MD (gam_2_OMP.f:546-1077).
-template <int NL>
-NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
-                       double &ansS)
+// xcalc_GAM2_MD (gam_2_OMP.f:546-1077).  With MULT set: RXCHFmult_GAM2_MD
+// (rxchfmult/rxchfmult_gam2.f:627-1162), where e1 and e2 come from different
+// bases.  It drops the single-geminal terms, places the geminal of gVEEg1 on
+// e2 (gamA23/gamB23, :1067-1068) instead of e1, and returns three values:
+// ansE = the two-geminal energy, ansE2 = gVEPg2 + gVEEg1, ansS = gsg.
+template <int NL, bool MULT = false>
+NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
+                      double &ansS, double &ansE2)
 {
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
+    if constexpr (!MULT)
     for (int ik = 0; ik < S.ngtg; ++ik) {
         double gk = S.gamma[ik], bk = S.bcoef[ik];
         Sys s = mksys(S, 3, e1, e2, p, 0, 0.0, gk, 0); // g(e2,p)
@@ -892,13 +897,22 @@ NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
             v = coul<NL>(s2, 0, 2);
             underflow(v);
             gVEPg2 -= w * v;
-            v = coul<NL>(s1, 0, 1);
+            v = coul<NL>(MULT ? s2 : s1, 0, 1);
             underflow(v);
             gVEEg1 += w * v;
         }
     ansE = 2.0 * (gTE + gVEC + gVEP) + 4.0 * gVEE * 0.5 + gTPg + gVPCg + gTEg1 + gTEg3 + 2.0 * (gVEPg1 + gVECg1) +
            gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
     ansS = gsg;
+    ansE2 = gVEPg2 + gVEEg1;
+}
+
+template <int NL>
+NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
+                       double &ansS)
+{
+    double unused;
+    gam2_core<NL, false>(S, e1, e2, p, ansE, ansS, unused);
 }
 
 // xcalc_GAM3_MD (gam_3_OMP.f:617-983).
@@ -973,6 +987,39 @@ NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e
     ansE2 = gVEEg2;
 }
 
+// RXCHFmult_INT_GAM3_MD (rxchfmult/rxchfmult_gam3-int.f:661-1054): e1 from
+// the regular basis, e2 and e3 from the special-electron basis.
+//   xgVEPg1 = -<1/r(e1,p)> with g(e2,p) = gamma_k, g(e3,p) = gamma_l
+//   xgVEEg1 = <g(e2,p) = gamma_k + gamma_l> * <1/r(e1,e3)>
+//   xgVEEg2 = <1/r(e1,e2)> with g(e2,p) = gamma_k, g(e3,p) = gamma_l
+//   ans = gVEPg1 + gVEEg1 + 2 gVEEg2
+template <int NL>
+NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3,
+                          const PairRec *p, double &ans)
+{
+    Sys s13 = mksys(S, 2, e1, e3, 0, 0, 0.0, 0, 0);
+    double vee13 = coul<NL>(s13, 0, 1);
+    underflow(vee13);
+    double gVEPg1 = 0, gVEEg1 = 0, gVEEg2 = 0;
+    for (int ik = 0; ik < S.ngtg; ++ik)
+        for (int il = 0; il < S.ngtg; ++il) {
+            double gk = S.gamma[ik], gl = S.gamma[il];
+            double bb = S.bcoef[ik] * S.bcoef[il];
+            Sys s4 = mksys(S, 4, e1, e2, e3, p, 0.0, gk, gl);
+            double zz = coul<NL>(s4, 0, 3);
+            underflow(zz);
+            gVEPg1 += bb * (zz * (-1.0));
+            gVEEg2 += bb * coul<NL>(s4, 0, 1);
+            if (il < ik) continue;
+            double w = (il == ik) ? bb : 2.0 * bb;
+            Sys s2 = mksys(S, 2, e2, p, 0, 0, gk + gl, 0, 0);
+            double xx = ovl<NL>(s2);
+            underflow(xx);
+            gVEEg1 += w * (xx * vee13);
+        }
+    ans = gVEPg1 + gVEEg1 + 2.0 * gVEEg2;
+}
+
 // Polynomial capacities per tuple class.  GEN covers every tuple allowed by
 // l <= 2 (per particle and direction bra+ket powers <= 4): e-p 4+4+2,
 // e-e-p 3*4+2, e-e-e-p 4*4.  SMALL covers tuples with tuple_degree <= SMALL-1,
@@ -995,10 +1042,20 @@ template <> struct Tier<4> {
     static constexpr int SMALL = 9, GEN = 13;
     static constexpr bool KIN = false;
 };
+// class 5: the RXCHFmult e-e-p tuple (two bases, kinetic terms as class 2)
+template <> struct Tier<5> {
+    static constexpr int SMALL = 9, GEN = 15;
+    static constexpr bool KIN = true;
+};
+// class 6: the RXCHFmult e-e-e-p interaction tuple (Coulomb and overlap: 4*4)
+template <> struct Tier<6> {
+    static constexpr int SMALL = 9, GEN = 17;
+    static constexpr bool KIN = false;
+};
 constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
 
 // electrons per tuple of a class
-template <int CLS> constexpr int nel() { return CLS == 4 ? 2 : CLS; }
+template <int CLS> constexpr int nel() { return (CLS == 4 || CLS == 5) ? 2 : CLS == 6 ? 3 : CLS; }
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
 template <int CLS, int NL>
@@ -1011,8 +1068,24 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
         gam2_tuple<NL>(S, q[0], q[1], q[2], e, s);
     else if constexpr (CLS == 4)
         gam2ne_tuple<NL>(S, q[0], q[1], q[2], e, s);
+    else if constexpr (CLS == 5) {
+        double x;
+        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x);
+    } else if constexpr (CLS == 6)
+        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
     else
         gam3_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
+}
+
+// three-output form: x is ansE2 of class 5, zero for the other classes
+template <int CLS, int NL>
+NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x)
+{
+    x = 0.0;
+    if constexpr (CLS == 5)
+        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x);
+    else
+        tuple_eval<CLS, NL>(S, q, e, s);
 }
 
 template <int CLS> NRX_HD bool tuple_small(const PairRec *const *q)

@@ -65,6 +65,7 @@ static int ensure_init()
 // ------------------------------------------------------------------
 struct DevSys {
     i64 nebf, npebf, npbf, nat, ngtg;
+    i64 nebf1, npebf1;        // first electron basis; the rest is the special-electron basis (RXCHFmult)
     double pmass;
     const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc;
     const i64 *kpestr, *kpeend, *ampeb2c;
@@ -209,6 +210,8 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
     DevSys &d = s->d;
     d.nebf = nebf;
     d.npebf = npebf;
+    d.nebf1 = nebf;
+    d.npebf1 = npebf;
     d.npbf = npbf;
     d.nat = nat;
     d.ngtg = ngtg;
@@ -310,6 +313,13 @@ template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x,
     tuple_eval<CLS, NL>(d_params(S), q, e, s);
 }
 
+template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s, double &xx)
+{
+    const PairRec *q[nel<CLS>() + 1];
+    tuple_pairs<CLS>(S, x, q);
+    tuple_eval<CLS, NL>(d_params(S), q, e, s, xx);
+}
+
 template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
 {
     const PairRec *q[nel<CLS>() + 1];
@@ -332,17 +342,25 @@ template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t
 
 // tuple r of the enumeration (ip,jp,ie1,je1,...,ieN,jeN), last index
 // fastest, in the system's sorted primitive order (SORTED) or in the
-// caller's original order
+// caller's original order.  The RXCHFmult classes (5, 6) enumerate e1 over
+// the first basis and the other electrons over the special-electron basis,
+// in original order only.
 template <int CLS, bool SORTED> NRX_DI void range_tuple(const DevSys &S, i64 r, i64 *x)
 {
-    i64 ne = S.npebf, np = S.npbf;
+    static_assert(CLS < 5 || !SORTED, "two-basis classes are enumerated in original order");
+    i64 np = S.npbf;
     constexpr int CLS_E = nel<CLS>();
     for (int i = CLS_E - 1; i >= 0; --i) {
+        i64 ne = S.npebf, off = 0;
+        if constexpr (CLS >= 5) {
+            ne = i == 0 ? S.npebf1 : S.npebf - S.npebf1;
+            off = i == 0 ? 0 : S.npebf1;
+        }
         i64 k = r % ne;
-        x[CLS_E + 1 + i] = SORTED ? S.eord[k] : k;
+        x[CLS_E + 1 + i] = SORTED ? S.eord[k] : off + k;
         r /= ne;
         k = r % ne;
-        x[i] = SORTED ? S.eord[k] : k;
+        x[i] = SORTED ? S.eord[k] : off + k;
         r /= ne;
     }
     i64 k = r % np;
@@ -370,7 +388,8 @@ template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *id
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
                                                     double *__restrict__ outE, double *__restrict__ outS,
-                                                    i64 *__restrict__ ovf, unsigned *__restrict__ novf)
+                                                    double *__restrict__ outX, i64 *__restrict__ ovf,
+                                                    unsigned *__restrict__ novf)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntup) return;
@@ -379,6 +398,7 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
         if (!load_tuple<CLS>(S, idx, t, x)) { // out-of-range primitive index: flag the entry
             outE[t] = __builtin_nan("");
             if (outS) outS[t] = __builtin_nan("");
+            if (outX) outX[t] = __builtin_nan("");
             return;
         }
     } else {
@@ -392,10 +412,11 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
             return;
         }
     }
-    double e, s;
-    eval_tuple<CLS, NS_>(S, x, e, s);
+    double e, s, xx;
+    eval_tuple<CLS, NS_>(S, x, e, s, xx);
     outE[t] = e;
     if (outS) outS[t] = s;
+    if (outX) outX[t] = xx;
 }
 
 // Second pass: the tuples listed by the first, general capacity, grid-stride
@@ -403,17 +424,19 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_gen(DevSys S, i64 start, const i64 *__restrict__ idx,
                                                   double *__restrict__ outE, double *__restrict__ outS,
-                                                  const i64 *__restrict__ ovf, const unsigned *__restrict__ novf)
+                                                  double *__restrict__ outX, const i64 *__restrict__ ovf,
+                                                  const unsigned *__restrict__ novf)
 {
     i64 n = *novf;
     for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
         i64 t = ovf[k];
         i64 x[2 * nel<CLS>() + 2];
         src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by the first pass
-        double e, s;
-        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s);
+        double e, s, xx;
+        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s, xx);
         outE[t] = e;
         if (outS) outS[t] = s;
+        if (outX) outX[t] = xx;
     }
 }
 
@@ -446,7 +469,8 @@ struct StreamBuf {
 };
 
 template <int CLS, int SRC>
-static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st)
+static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st,
+                        double *ox = nullptr)
 {
     constexpr bool tiered = Tier<CLS>::SMALL != Tier<CLS>::GEN;
     StreamBuf ws;
@@ -462,36 +486,52 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
     for (i64 off = 0; off < ntup; off += PRIM_CHUNK) {
         i64 n = std::min(ntup - off, PRIM_CHUNK);
         const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * nel<CLS>() + 2) : nullptr;
-        double *e = oe + off, *s = os ? os + off : nullptr;
+        double *e = oe + off, *s = os ? os + off : nullptr, *x = ox ? ox + off : nullptr;
         if (tiered) {
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, cnt);
             HIPCHK(hipGetLastError());
         }
         hipLaunchKernelGGL((k_prim_small<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e,
-                           s, ovf, cnt);
+                           s, x, ovf, cnt);
         HIPCHK(hipGetLastError());
         if (tiered) {
             hipLaunchKernelGGL((k_prim_gen<CLS, SRC>), dim3(GEN_BLOCKS), dim3(256), 0, st, d, start + off, ix, e, s,
-                               ovf, cnt);
+                               x, ovf, cnt);
             HIPCHK(hipGetLastError());
         }
     }
     return NEORX_OK;
 }
 
-extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
-                                   double *d_ansS, void *stream)
+static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE, double *d_ansS,
+                     double *d_ansX, void *stream)
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 4) return fail(NEORX_E_ARG, "class %lld not in 1..4", (long long)cls);
+    if (cls < 1 || cls > 6) return fail(NEORX_E_ARG, "class %lld not in 1..6", (long long)cls);
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
-    if (cls == 1) return launch_prims<1, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
-    if (cls == 2) return launch_prims<2, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
-    if (cls == 4) return launch_prims<4, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
-    return launch_prims<3, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st);
+    switch (cls) {
+    case 1: return launch_prims<1, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 2: return launch_prims<2, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 3: return launch_prims<3, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 4: return launch_prims<4, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 5: return launch_prims<5, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    default: return launch_prims<6, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    }
+}
+
+extern "C" int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
+                                   double *d_ansS, void *stream)
+{
+    return prim_list(handle, cls, ntup, d_idx, d_ansE, d_ansS, nullptr, stream);
+}
+
+extern "C" int neorx_prim_list3_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
+                                    double *d_ansS, double *d_ansE2, void *stream)
+{
+    return prim_list(handle, cls, ntup, d_idx, d_ansE, d_ansS, d_ansE2, stream);
 }
 
 extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, int64_t count, double *d_ansE,
@@ -758,13 +798,16 @@ static const i64 BLOCK_CAP = (i64)1 << 27;
 
 // first primitive tuple (original order) of leading index L of class cls
 // (e-e-p: L = ip*npbf + jp; e-e-e-p: L = (ip*npbf + jp)*nebf + i1); the
-// number of leading indices gives the end of the enumeration
+// number of leading indices gives the end of the enumeration.  The
+// RXCHFmult classes 5 and 6 count e1 over the first basis (pa primitives,
+// nebf1 functions) and the other electrons over the special basis (pb).
 static i64 slab_start(const System *s, int cls, i64 L)
 {
-    i64 pe = s->d.npebf, pe4 = pe * pe * pe * pe;
-    if (cls == 2) return L * pe4;
-    i64 ne = s->d.nebf, pp = L / ne, i1 = L % ne;
-    return (pp * pe + (s->kpestr[i1] - 1)) * pe4 * pe; // i1 == 0 past the end: pp * pe^6
+    bool two = cls >= 5;
+    i64 pa = two ? s->d.npebf1 : s->d.npebf, pb = two ? s->d.npebf - pa : pa, ne = two ? s->d.nebf1 : s->d.nebf;
+    if (cls == 2 || cls == 5) return L * pa * pa * pb * pb;
+    i64 pp = L / ne, i1 = L % ne;
+    return (pp * pa + (s->kpestr[i1] - 1)) * pa * pb * pb * pb * pb; // i1 == 0 past the end: pp * pa^2 pb^4
 }
 
 // consecutive slabs [L0, L1) covering [La, Lb); cap = largest primitive span
@@ -786,7 +829,7 @@ static std::vector<std::pair<i64, i64>> make_slabs(const System *s, int cls, i64
 static i64 slab_cap(const System *s, int cls)
 {
     i64 cap = 0;
-    i64 nL = s->d.npbf * s->d.npbf * (cls == 3 ? s->d.nebf : 1);
+    i64 nL = s->d.npbf * s->d.npbf * ((cls == 3 || cls == 6) ? s->d.nebf1 : 1);
     make_slabs(s, cls, 0, nL, cap);
     return cap;
 }
@@ -2339,3 +2382,533 @@ extern "C" int neorx_rxchfne_fae_gamee(int64_t nebf, const double *gam_ee, const
     HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
     return NEORX_OK;
 }
+
+// ------------------------------------------------------------------
+// RXCHFmult (rxchfmult/): more than one special electron.  One system handle
+// holds both electron bases: primitives [0, npebf) and contracted functions
+// [0, nebf) are the regular basis, the special-electron basis follows them.
+// Tensors use the RXCHFmult packings (rxchfmult/rxchfmult_basis.f:2-134):
+// C row-major [ip][jp][ie1][je1] over the regular basis, then [ie_k][je_k]
+// over the special basis for every further electron.
+// ------------------------------------------------------------------
+extern "C" int neorx_mult_system_create(
+    int64_t nebf, int64_t npebf, int64_t nebfbe, int64_t npebfbe, int64_t npbf, int64_t nat, int64_t ngtg, double pmass,
+    const double *cat, const double *zan, const double *bcoef, const double *gamma, const int64_t *kpestr,
+    const int64_t *kpeend, const int64_t *ampeb2c, const double *agebfcc, const double *elcex, const int64_t *elcam,
+    const double *elcbfc, const int64_t *kpestr_be, const int64_t *kpeend_be, const int64_t *ampeb2c_be,
+    const double *agebfcc_be, const double *elcex_be, const int64_t *elcam_be, const double *elcbfc_be,
+    const double *agnbfcc, const double *nucex, const int64_t *nucam, const double *nucbfc, int64_t *handle)
+{
+    if (nebf <= 0 || npebf <= 0 || nebfbe <= 0 || npebfbe <= 0)
+        return fail(NEORX_E_ARG, "bad basis sizes nebf=%lld npebf=%lld nebfBE=%lld npebfBE=%lld", (long long)nebf,
+                    (long long)npebf, (long long)nebfbe, (long long)npebfbe);
+    i64 n = npebf + npebfbe, nc = nebf + nebfbe;
+    std::vector<int64_t> kps(nc), kpe(nc), amp(n), am(3 * n);
+    std::vector<double> cc(n), ex(n), cen(3 * n);
+    for (i64 c = 0; c < nebf; ++c) {
+        kps[c] = kpestr[c];
+        kpe[c] = kpeend[c];
+    }
+    for (i64 c = 0; c < nebfbe; ++c) {
+        kps[nebf + c] = kpestr_be[c] + npebf;
+        kpe[nebf + c] = kpeend_be[c] + npebf;
+    }
+    for (i64 i = 0; i < npebf; ++i) {
+        amp[i] = ampeb2c[i];
+        cc[i] = agebfcc[i];
+        ex[i] = elcex[i];
+        for (int d = 0; d < 3; ++d) {
+            am[d * n + i] = elcam[d * npebf + i];
+            cen[d * n + i] = elcbfc[d * npebf + i];
+        }
+    }
+    for (i64 i = 0; i < npebfbe; ++i) {
+        amp[npebf + i] = ampeb2c_be[i] + nebf;
+        cc[npebf + i] = agebfcc_be[i];
+        ex[npebf + i] = elcex_be[i];
+        for (int d = 0; d < 3; ++d) {
+            am[d * n + npebf + i] = elcam_be[d * npebfbe + i];
+            cen[d * n + npebf + i] = elcbfc_be[d * npebfbe + i];
+        }
+    }
+    int rc = neorx_system_create(nc, n, npbf, nat, ngtg, pmass, cat, zan, bcoef, gamma, kps.data(), kpe.data(),
+                                 amp.data(), cc.data(), agnbfcc, ex.data(), nucex, am.data(), nucam, cen.data(),
+                                 nucbfc, handle);
+    if (rc) return rc;
+    System *s = get_sys(*handle);
+    s->d.nebf1 = nebf;
+    s->d.npebf1 = npebf;
+    return NEORX_OK;
+}
+
+static System *get_mult(int64_t handle, int &rc)
+{
+    System *s = get_sys(handle);
+    rc = NEORX_OK;
+    if (!s) rc = fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    else if (s->d.nebf1 == s->d.nebf) rc = fail(NEORX_E_ARG, "handle %lld has no special-electron basis "
+                                                "(create it with neorx_mult_system_create)", (long long)handle);
+    return rc ? nullptr : s;
+}
+
+// contract_omega2 (rxchfmult_gam2.f:413-626): lane per contracted
+// (L = ip*npbf + jp, i1, j1, i2, j2); the three primitive values ansE1, ansS,
+// ansE2 of the slab in Y, Ys, Yx (class 5 enumeration, from R0)
+__global__ void __launch_bounds__(256) k_mcontract2(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
+                                                    const double *__restrict__ Ys, const double *__restrict__ Yx,
+                                                    double *__restrict__ X1, double *__restrict__ Xs,
+                                                    double *__restrict__ X2)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 na = S.nebf1, nb = S.nebf - na, pa = S.npebf1, pb = S.npebf - pa, np = S.npbf;
+    i64 blk = na * na * nb * nb, pblk = pa * pa * pb * pb;
+    i64 L = L0 + t / blk, rem = t % blk;
+    i64 j2 = rem % nb, i2 = (rem / nb) % nb, j1 = (rem / (nb * nb)) % na, i1 = rem / (nb * nb * na);
+    i64 ip = L / np, jp = L % np;
+    double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
+    double o1 = 0.0, o2 = 0.0, os = 0.0;
+    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
+        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
+            for (i64 a2 = S.kpestr[na + i2] - 1; a2 < S.kpeend[na + i2]; ++a2)
+                for (i64 c2 = S.kpestr[na + j2] - 1; c2 < S.kpeend[na + j2]; ++c2) {
+                    i64 r = L * pblk + ((a1 * pa + c1) * pb + (a2 - pa)) * pb + (c2 - pa) - R0;
+                    double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
+                    o1 += Y[r] * cf;
+                    o2 += Yx[r] * cf;
+                    os += Ys[r] * cf;
+                }
+    i64 o = L * blk + rem;
+    X1[o] = o1;
+    X2[o] = o2;
+    Xs[o] = os;
+}
+
+// XCHF_GAM2 = (x12 + x21)/2 over special-basis indices (rxchfmult_gam2.f:173-228):
+// x12 at (ip,jp,i1,j1,i2,j2), x21 at (ip,jp,i2,j2,i1,j1) of the unsymmetrised
+// two-basis tensor, the special indices taken as regular-basis ones.
+__global__ void __launch_bounds__(256) k_mult_symm2(i64 na, i64 nb, i64 n, const double *__restrict__ X,
+                                                    const double *__restrict__ Xs, double *__restrict__ G,
+                                                    double *__restrict__ Gs)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 j2 = t % nb, i2 = (t / nb) % nb, j1 = (t / (nb * nb)) % nb, i1 = (t / (nb * nb * nb)) % nb,
+        pp = t / (nb * nb * nb * nb);
+    i64 a = ((pp * na + i1) * na + j1) * nb * nb + i2 * nb + j2;
+    i64 b = ((pp * na + i2) * na + j2) * nb * nb + i1 * nb + j1;
+    G[t] = (X[a] + X[b]) / 2.0;
+    Gs[t] = (Xs[a] + Xs[b]) / 2.0;
+}
+
+extern "C" int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, double *d_xchf_gam2,
+                                        double *d_xchf_gam2s, void *stream)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    if (!d_int_gam2 || !d_xchf_gam2 || !d_xchf_gam2s) return fail(NEORX_E_ARG, "null output");
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    if (nb > na) return fail(NEORX_E_ARG, "nebfBE=%lld > nebf=%lld: XCHF_GAM2 reads regular-basis slots with "
+                             "special-basis indices", (long long)nb, (long long)na);
+    i64 nint = np * np * na * na * nb * nb, nx = np * np * nb * nb * nb * nb;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap;
+    auto slabs = make_slabs(s, 5, 0, np * np, cap);
+    DevBuf X1, Xs, Y, Ys, Yx;
+    rc = X1.alloc(nint * 8);
+    if (!rc) rc = Xs.alloc(nint * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = Yx.alloc(cap * 8);
+    if (rc) return rc;
+    i64 blk = na * na * nb * nb;
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 5, sl.first), R1 = slab_start(s, 5, sl.second);
+        rc = launch_prims<5, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st, Yx.d());
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * blk;
+        hipLaunchKernelGGL(k_mcontract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
+                           Ys.d(), Yx.d(), X1.d(), Xs.d(), d_int_gam2);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mult_symm2, dim3(nblocks(nx, 256)), dim3(256), 0, st, na, nb, nx, X1.d(), Xs.d(),
+                       d_xchf_gam2, d_xchf_gam2s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam2(int64_t handle, double *int_gam2, double *xchf_gam2, double *xchf_gam2s)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    i64 nint = np * np * na * na * nb * nb, nx = np * np * nb * nb * nb * nb;
+    DevBuf A, B, C;
+    rc = A.alloc(nint * 8);
+    if (!rc) rc = B.alloc(nx * 8);
+    if (!rc) rc = C.alloc(nx * 8);
+    if (!rc) rc = neorx_rxchfmult_gam2_dev(handle, A.d(), B.d(), C.d(), 0);
+    if (!rc) rc = copy_out(int_gam2, A.d(), nint);
+    if (!rc) rc = copy_out(xchf_gam2, B.d(), nx);
+    if (!rc) rc = copy_out(xchf_gam2s, C.d(), nx);
+    return rc;
+}
+
+// contract_omega3_INT (rxchfmult_gam3-int.f:417-660): lane per contracted
+// (L = (ip*npbf + jp)*nebf + i1, j1, i2, j2, i3, j3)
+__global__ void __launch_bounds__(256) k_mcontract3(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
+                                                    double *__restrict__ X)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 na = S.nebf1, nb = S.nebf - na, pa = S.npebf1, pb = S.npebf - pa, np = S.npbf;
+    i64 nb2 = nb * nb, blk = na * nb2 * nb2, pb4 = pb * pb * pb * pb;
+    i64 L = L0 + t / blk, rem = t % blk;
+    i64 j3 = rem % nb, i3 = (rem / nb) % nb, j2 = (rem / nb2) % nb, i2 = (rem / (nb2 * nb)) % nb,
+        j1 = rem / (nb2 * nb2);
+    i64 i1 = L % na, pp = L / na, ip = pp / np, jp = pp % np;
+    double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
+    double om = 0.0;
+    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
+        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
+            for (i64 a2 = S.kpestr[na + i2] - 1; a2 < S.kpeend[na + i2]; ++a2)
+                for (i64 c2 = S.kpestr[na + j2] - 1; c2 < S.kpeend[na + j2]; ++c2)
+                    for (i64 a3 = S.kpestr[na + i3] - 1; a3 < S.kpeend[na + i3]; ++a3)
+                        for (i64 c3 = S.kpestr[na + j3] - 1; c3 < S.kpeend[na + j3]; ++c3) {
+                            i64 r = (pp * pa + a1) * pa * pb4 +
+                                    (((c1 * pb + (a2 - pa)) * pb + (c2 - pa)) * pb + (a3 - pa)) * pb + (c3 - pa) - R0;
+                            double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2] *
+                                        S.agebfcc[a3] * S.agebfcc[c3];
+                            om += Y[r] * cf;
+                        }
+    X[L * blk + rem] = om;
+}
+
+// INT_GAM3 = (x123 + x132)/2, electrons 2 and 3 exchanged (rxchfmult_gam3-int.f:173-218)
+__global__ void __launch_bounds__(256) k_mult_symm3(i64 nb, i64 n, const double *__restrict__ X,
+                                                    double *__restrict__ G)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 nb2 = nb * nb;
+    i64 p3 = t % nb2, p2 = (t / nb2) % nb2, head = t / (nb2 * nb2);
+    G[t] = (X[t] + X[(head * nb2 + p3) * nb2 + p2]) * 0.5;
+}
+
+extern "C" int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, void *stream)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    if (!d_int_gam3) return fail(NEORX_E_ARG, "null output");
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    i64 nb2 = nb * nb, blk = na * nb2 * nb2, n3 = np * np * na * blk;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap;
+    auto slabs = make_slabs(s, 6, 0, np * np * na, cap);
+    DevBuf X, Y;
+    rc = X.alloc(n3 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (rc) return rc;
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 6, sl.first), R1 = slab_start(s, 6, sl.second);
+        rc = launch_prims<6, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), nullptr, st);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * blk;
+        hipLaunchKernelGGL(k_mcontract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
+                           X.d());
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mult_symm3, dim3(nblocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam3_int(int64_t handle, double *int_gam3)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    i64 n3 = np * np * na * na * nb * nb * nb * nb;
+    DevBuf A;
+    rc = A.alloc(n3 * 8);
+    if (!rc) rc = neorx_rxchfmult_gam3_int_dev(handle, A.d(), 0);
+    if (!rc) rc = copy_out(int_gam3, A.d(), n3);
+    return rc;
+}
+
+// RXCHFmult_symm_gam4 / RXCHFmult_calc_GAM_4_integral (rxchfmult_gam4-int.f:290-486):
+// lane per INT_GAM4 entry; electron 1 fixed, kets of electrons 2..4 permuted
+// with weights 1, -1/2, -1/2, 1/4, 1/4, -1/2.  Each term is
+// (ee14 gs23 + ee13 gs24 + ee12 gs34)/3 from GAM_ee over the regular basis
+// (special indices taken as regular ones) and GAM_2s over the special basis.
+__global__ void __launch_bounds__(256) k_mult_gam4(i64 na, i64 nb, i64 np, i64 off, i64 n,
+                                                   const double *__restrict__ G2s, const double *__restrict__ Gee,
+                                                   double *__restrict__ G4)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = off + t;
+    i64 br[4], kt[4];
+    for (int k = 3; k >= 1; --k) {
+        kt[k] = r % nb; r /= nb;
+        br[k] = r % nb; r /= nb;
+    }
+    kt[0] = r % na; r /= na;
+    br[0] = r % na; r /= na;
+    i64 jp = r % np, ip = r / np;
+    auto ee = [&](i64 x, const i64 *k) {
+        double v = Gee[((br[0] * na + k[0]) * na + br[x]) * na + k[x]];
+        underflow(v);
+        return v;
+    };
+    auto gs = [&](i64 x, i64 y, const i64 *k) {
+        double v = G2s[((((ip * np + jp) * nb + br[x]) * nb + k[x]) * nb + br[y]) * nb + k[y]];
+        underflow(v);
+        return v;
+    };
+    auto term = [&](i64 a, i64 b, i64 c) {
+        i64 k[4] = {kt[0], kt[a], kt[b], kt[c]};
+        double x1 = ee(3, k) * gs(1, 2, k), x2 = ee(2, k) * gs(1, 3, k), x3 = ee(1, k) * gs(2, 3, k);
+        return (x1 + x2 + x3) / 3.0;
+    };
+    double ans = term(1, 2, 3) - term(1, 3, 2) / 2.0 - term(2, 1, 3) / 2.0 + term(2, 3, 1) / 4.0 +
+                 term(3, 1, 2) / 4.0 - term(3, 2, 1) / 2.0;
+    G4[off + t] = ans;
+}
+
+extern "C" int neorx_rxchfmult_gam4_int_dev(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *d_gam_2s,
+                                            const double *d_gam_ee, double *d_gam4, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || nebfbe <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (nebfbe > nebf) return fail(NEORX_E_ARG, "nebfBE=%lld > nebf=%lld: GAM_ee is read with special-basis indices",
+                                   (long long)nebfbe, (long long)nebf);
+    if (!d_gam_2s || !d_gam_ee || !d_gam4) return fail(NEORX_E_ARG, "null argument");
+    i64 na = nebf, nb = nebfbe, np = npbf, nb2 = nb * nb;
+    i64 n4 = np * np * na * na * nb2 * nb2 * nb2;
+    hipStream_t st = (hipStream_t)stream;
+    for (i64 off = 0; off < n4; off += CHUNK) {
+        i64 n = std::min(CHUNK, n4 - off);
+        hipLaunchKernelGGL(k_mult_gam4, dim3(nblocks(n, 256)), dim3(256), 0, st, na, nb, np, off, n, d_gam_2s,
+                           d_gam_ee, d_gam4);
+        HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam4_int(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *gam_2s,
+                                        const double *gam_ee, double *gam4)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || nebfbe <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 na = nebf, nb = nebfbe, np = npbf, nb2 = nb * nb;
+    i64 n4 = np * np * na * na * nb2 * nb2 * nb2;
+    DevBuf G2, Ge, G4;
+    rc = h2d(G2, gam_2s, np * np * nb2 * nb2);
+    if (!rc) rc = h2d(Ge, gam_ee, na * na * na * na);
+    if (!rc) rc = G4.alloc(n4 * 8);
+    if (!rc) rc = neorx_rxchfmult_gam4_int_dev(nebf, nebfbe, npbf, G2.d(), Ge.d(), G4.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    return copy_out(gam4, G4.d(), n4);
+}
+
+// ---- RXCHFmult Fock contractions (rxchfmult_omg{2,3,4}.f) --------------------
+// A tensor of order k (k = 1, 2, 3 special electrons besides electron 1) is a
+// row-major matrix G[R][c]: rows R = (ip,jp,i1,j1), columns c = the special
+// indices.  One pass over G gives
+//   rowsum[R] = sum_c W(c) G[R][c]                    (FAE, FP and E)
+//   HH[c]     = sum_R DP(ip,jp) DAE(i1,j1) G[R][c]    (FBE)
+// with column weights W = DBE(i2,j2) (k=1); DBE(p,q)DBE(r,s) - 1/2 DBE(p,s)DBE(r,q)
+// at c = (p,q,r,s) (k=2: the exchanged read val = G(..i2 j3 i3 j2)/2 of
+// thread_F*_OMG3, rxchfmult_omg3.f:187, moved onto the weight); the product
+// of three DBE (k=3).
+__global__ void __launch_bounds__(256) k_mult_weights(int k, i64 nb, i64 nc, const double *__restrict__ DBE,
+                                                      double *__restrict__ W)
+{
+    i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    auto D = [&](i64 i, i64 j) { return DBE[j * nb + i]; }; // DBE(i,j)
+    double w;
+    if (k == 1) {
+        w = D(c / nb, c % nb);
+    } else if (k == 2) {
+        i64 s = c % nb, r = (c / nb) % nb, q = (c / (nb * nb)) % nb, p = c / (nb * nb * nb);
+        w = D(p, q) * D(r, s) - 0.5 * D(p, s) * D(r, q);
+    } else {
+        i64 nb2 = nb * nb, c3 = c % nb2, c2 = (c / nb2) % nb2, c1 = c / (nb2 * nb2);
+        w = D(c1 / nb, c1 % nb) * D(c2 / nb, c2 % nb) * D(c3 / nb, c3 % nb);
+    }
+    W[c] = w;
+}
+
+__global__ void k_fill0(i64 n, double *__restrict__ p)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+static const i64 MULT_ROWS = 64; // rows of G per block of k_mult_rows
+
+// block = (run of MULT_ROWS rows, column tile of ntile); lane per column
+__global__ void __launch_bounds__(256) k_mult_rows(i64 na, i64 np, i64 nrows, i64 ncols, i64 ntile,
+                                                   const double *__restrict__ DAE, const double *__restrict__ DP,
+                                                   const double *__restrict__ W, const double *__restrict__ G,
+                                                   double *__restrict__ rowsum, double *__restrict__ HH)
+{
+    __shared__ double rs[MULT_ROWS];
+    i64 c = ((i64)blockIdx.x % ntile) * blockDim.x + threadIdx.x;
+    i64 R0 = ((i64)blockIdx.x / ntile) * MULT_ROWS, nr = std::min<i64>(MULT_ROWS, nrows - R0);
+    if (threadIdx.x < MULT_ROWS) rs[threadIdx.x] = 0.0;
+    __syncthreads();
+    bool on = c < ncols;
+    double w = on ? W[c] : 0.0, h = 0.0;
+    const bool lead = (threadIdx.x & 63) == 0;
+    i64 na2 = na * na;
+    for (i64 k = 0; k < nr; ++k) {
+        i64 R = R0 + k, pp = R / na2, a = R % na2;
+        double g = on ? G[R * ncols + c] : 0.0;
+        double wr = DP[(pp % np) * np + pp / np] * DAE[(a % na) * na + a / na];
+        h += wr * g;
+        double v = wave_sum(w * g);
+        if (lead) atomicAdd(&rs[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < nr) atomicAdd(&rowsum[R0 + threadIdx.x], rs[threadIdx.x]);
+    if (HH && on) atomicAdd(&HH[c], h);
+}
+
+// FAE(i1,j1) += sum_pp DP rowsum;  FP(ip,jp) += sum_a DAE rowsum;
+// FBE(i2,j2) += k * (HH contracted with DBE over the other special electrons);
+// E = sum_pp DP sum_a DAE rowsum
+__global__ void __launch_bounds__(256) k_mult_finish(int k, i64 na, i64 nb, i64 np, const double *__restrict__ DAE,
+                                                     const double *__restrict__ DBE, const double *__restrict__ DP,
+                                                     const double *__restrict__ rowsum, const double *__restrict__ HH,
+                                                     double *__restrict__ FAE, double *__restrict__ FP,
+                                                     double *__restrict__ FBE, double *__restrict__ E)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    i64 na2 = na * na, np2 = np * np, nb2 = nb * nb;
+    auto dp = [&](i64 pp) { return DP[(pp % np) * np + pp / np]; };
+    auto dae = [&](i64 a) { return DAE[(a % na) * na + a / na]; };
+    auto dbe = [&](i64 i, i64 j) { return DBE[j * nb + i]; };
+    auto fp_of = [&](i64 pp) {
+        double f = 0.0;
+        for (i64 a = 0; a < na2; ++a) f += dae(a) * rowsum[pp * na2 + a];
+        return f;
+    };
+    if (FAE && t < na2) {
+        double f = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) f += dp(pp) * rowsum[pp * na2 + t];
+        FAE[(t % na) * na + t / na] += f;
+    }
+    if (FP && t < np2) FP[(t % np) * np + t / np] += fp_of(t);
+    if (FBE && t < nb2) {
+        i64 i2 = t / nb, j2 = t % nb;
+        double f = 0.0;
+        if (k == 1) {
+            f = HH[t];
+        } else if (k == 2) {
+            for (i64 i3 = 0; i3 < nb; ++i3)
+                for (i64 j3 = 0; j3 < nb; ++j3)
+                    f += dbe(i3, j3) * (HH[t * nb2 + i3 * nb + j3] - 0.5 * HH[(i2 * nb + j3) * nb2 + i3 * nb + j2]);
+            f *= 2.0;
+        } else {
+            for (i64 c = 0; c < nb2 * nb2; ++c) {
+                i64 c3 = c % nb2, c2 = c / nb2;
+                f += dbe(c2 / nb, c2 % nb) * dbe(c3 / nb, c3 % nb) * HH[t * nb2 * nb2 + c];
+            }
+            f *= 3.0;
+        }
+        FBE[j2 * nb + i2] += f;
+    }
+    if (t == 0) {
+        double e = 0.0;
+        for (i64 pp = 0; pp < np2; ++pp) e += dp(pp) * fp_of(pp);
+        *E = e;
+    }
+}
+
+extern "C" int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, int64_t npbf, const double *d_dae,
+                                       const double *d_dbe, const double *d_dp, const double *d_gm, double *d_fae,
+                                       double *d_fp, double *d_fbe, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (k < 1 || k > 3) return fail(NEORX_E_ARG, "order %lld not in 1..3", (long long)k);
+    if (nebf <= 0 || nebfbe <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (!d_dae || !d_dbe || !d_dp || !d_gm || !d_e) return fail(NEORX_E_ARG, "null argument");
+    i64 na = nebf, nb = nebfbe, np = npbf, nrows = np * np * na * na, ncols = 1;
+    for (int i = 0; i < k; ++i) ncols *= nb * nb;
+    hipStream_t st = (hipStream_t)stream;
+    StreamBuf ws;
+    rc = ws.alloc((nrows + 2 * ncols) * 8, st);
+    if (rc) return rc;
+    double *rowsum = (double *)ws.p, *W = rowsum + nrows, *HH = d_fbe ? W + ncols : nullptr;
+    hipLaunchKernelGGL(k_fill0, dim3(nblocks(nrows, 256)), dim3(256), 0, st, nrows, rowsum);
+    if (HH) hipLaunchKernelGGL(k_fill0, dim3(nblocks(ncols, 256)), dim3(256), 0, st, ncols, HH);
+    hipLaunchKernelGGL(k_mult_weights, dim3(nblocks(ncols, 256)), dim3(256), 0, st, (int)k, nb, ncols, d_dbe, W);
+    HIPCHK(hipGetLastError());
+    unsigned nt = (unsigned)std::min<i64>(256, ((ncols + 63) / 64) * 64);
+    i64 ntile = (ncols + nt - 1) / nt, nblk = ntile * ((nrows + MULT_ROWS - 1) / MULT_ROWS);
+    if (nblk > 0x7fffffff) return fail(NEORX_E_UNSUPPORTED, "tensor too large for one launch");
+    hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae, d_dp, W,
+                       d_gm, rowsum, HH);
+    HIPCHK(hipGetLastError());
+    i64 nf = std::max(std::max(na * na, np * np), nb * nb);
+    hipLaunchKernelGGL(k_mult_finish, dim3(nblocks(nf, 256)), dim3(256), 0, st, (int)k, na, nb, np, d_dae, d_dbe, d_dp,
+                       rowsum, HH, d_fae, d_fp, d_fbe, d_e);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+// host form of one reference routine: which = 0 (FAE), 1 (FP), 2 (FBE)
+static int mult_omg_host(int k, int which, i64 na, i64 nb, i64 np, const double *dae, const double *dbe,
+                         const double *dp, const double *gm, double *f, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (na <= 0 || nb <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng = np * np * na * na, nf = which == 0 ? na * na : which == 1 ? np * np : nb * nb;
+    for (int i = 0; i < k; ++i) ng *= nb * nb;
+    DevBuf dA, dB, dP, dG, dF, dE;
+    rc = h2d(dA, dae, na * na);
+    if (!rc) rc = h2d(dB, dbe, nb * nb);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(dG, gm, ng);
+    if (!rc) rc = h2d(dF, f, nf);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc)
+        rc = neorx_rxchfmult_omg_dev(k, na, nb, np, dA.d(), dB.d(), dP.d(), dG.d(), which == 0 ? dF.d() : nullptr,
+                                     which == 1 ? dF.d() : nullptr, which == 2 ? dF.d() : nullptr, dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+#define NRX_MULT_OMG(name, k, which)                                                                              \
+    extern "C" int name(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,        \
+                        const double *dp, const double *gm, double *f, double *e)                                \
+    {                                                                                                             \
+        return mult_omg_host(k, which, nebf, nebfbe, npbf, dae, dbe, dp, gm, f, e);                              \
+    }
+NRX_MULT_OMG(neorx_rxchfmult_fae_omg2, 1, 0)
+NRX_MULT_OMG(neorx_rxchfmult_fp_omg2, 1, 1)
+NRX_MULT_OMG(neorx_rxchfmult_fbe_omg2, 1, 2)
+NRX_MULT_OMG(neorx_rxchfmult_fae_omg3, 2, 0)
+NRX_MULT_OMG(neorx_rxchfmult_fp_omg3, 2, 1)
+NRX_MULT_OMG(neorx_rxchfmult_fbe_omg3, 2, 2)
+NRX_MULT_OMG(neorx_rxchfmult_fae_omg4, 3, 0)
+NRX_MULT_OMG(neorx_rxchfmult_fp_omg4, 3, 1)
+NRX_MULT_OMG(neorx_rxchfmult_fbe_omg4, 3, 2)
+#undef NRX_MULT_OMG

@@ -196,6 +196,91 @@ module neorx_capi
       real(c_double) :: gee(*), da(*), f(*), e
       integer(c_int) :: neorx_rxchfne_fae_gamee
     end function
+    function neorx_mult_system_create(nebf, npebf, nebfbe, npebfbe, npbf, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                                      kpestr, kpeend, ampeb2c, agebfcc, elcex, elcam, elcbfc, kpestr_be, kpeend_be, &
+                                      ampeb2c_be, agebfcc_be, elcex_be, elcam_be, elcbfc_be, agnbfcc, nucex, nucam, &
+                                      nucbfc, handle) bind(C, name="neorx_mult_system_create")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nebf, npebf, nebfbe, npebfbe, npbf, nat, ngtg
+      real(c_double), value :: pmass
+      real(c_double) :: cat(*), zan(*), bcoef(*), gamma(*), agebfcc(*), elcex(*), elcbfc(*)
+      real(c_double) :: agebfcc_be(*), elcex_be(*), elcbfc_be(*), agnbfcc(*), nucex(*), nucbfc(*)
+      integer(c_int64_t) :: kpestr(*), kpeend(*), ampeb2c(*), elcam(*), kpestr_be(*), kpeend_be(*)
+      integer(c_int64_t) :: ampeb2c_be(*), elcam_be(*), nucam(*), handle
+      integer(c_int) :: neorx_mult_system_create
+    end function
+    function neorx_rxchfmult_gam2(h, gi, gx, gs) bind(C, name="neorx_rxchfmult_gam2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: gi(*), gx(*), gs(*)
+      integer(c_int) :: neorx_rxchfmult_gam2
+    end function
+    function neorx_rxchfmult_gam3_int(h, g) bind(C, name="neorx_rxchfmult_gam3_int")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g(*)
+      integer(c_int) :: neorx_rxchfmult_gam3_int
+    end function
+    function neorx_rxchfmult_gam4_int(na, nb, np, g2s, gee, g4) bind(C, name="neorx_rxchfmult_gam4_int")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: g2s(*), gee(*), g4(*)
+      integer(c_int) :: neorx_rxchfmult_gam4_int
+    end function
+    function neorx_rxchfmult_fae_omg2(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fae_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fae_omg2
+    end function
+    function neorx_rxchfmult_fp_omg2(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fp_omg2
+    end function
+    function neorx_rxchfmult_fbe_omg2(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fbe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fbe_omg2
+    end function
+    function neorx_rxchfmult_fae_omg3(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fae_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fae_omg3
+    end function
+    function neorx_rxchfmult_fp_omg3(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fp_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fp_omg3
+    end function
+    function neorx_rxchfmult_fbe_omg3(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fbe_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fbe_omg3
+    end function
+    function neorx_rxchfmult_fae_omg4(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fae_omg4")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fae_omg4
+    end function
+    function neorx_rxchfmult_fp_omg4(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fp_omg4")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fp_omg4
+    end function
+    function neorx_rxchfmult_fbe_omg4(na, nb, np, da, db, dp, g, f, e) bind(C, name="neorx_rxchfmult_fbe_omg4")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: na, nb, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fbe_omg4
+    end function
   end interface
 contains
   subroutine neorx_check(rc, where)
@@ -605,4 +690,165 @@ subroutine RXCHFne_FAE_GAMee(nebf, ngee, GAM_ee, DAE, FAE, E_AE_GAMee)
   integer(c_int64_t) :: nebf, ngee
   real(c_double) :: GAM_ee(ngee), DAE(nebf, nebf), FAE(nebf, nebf), E_AE_GAMee
   call neorx_check(neorx_rxchfne_fae_gamee(nebf, GAM_ee, DAE, FAE, E_AE_GAMee), 'RXCHFne_FAE_GAMee')
+end subroutine
+
+! ---- RXCHFmult (source/rxchfmult/) ------------------------------------------
+
+! RXCHFmult_GAM2, source/rxchfmult/rxchfmult_gam2.f:2-265
+subroutine RXCHFmult_GAM2(Nchunks, nebf, npebf, nebfBE, npebfBE, npbf, dimINT2, dimXCHF2, nat, ngtg1, pmass, cat, &
+                          zan, bcoef1, gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR_be, &
+                          KPEEND_be, AMPEB2C_be, AGEBFCC_be, ELCEX_be, ELCAM_be, ELCBFC_be, AGNBFCC, NUCEX, NUCAM, &
+                          NUCBFC, INT_GAM2, XCHF_GAM2, XCHF_GAM2s)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, nebfBE, npebfBE, npbf, dimINT2, dimXCHF2, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3)
+  integer(c_int64_t) :: KPESTR_be(nebfBE), KPEEND_be(nebfBE), AMPEB2C_be(npebfBE), ELCAM_be(npebfBE, 3)
+  integer(c_int64_t) :: NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), ELCEX(npebf), ELCBFC(npebf, 3)
+  real(c_double) :: AGEBFCC_be(npebfBE), ELCEX_be(npebfBE), ELCBFC_be(npebfBE, 3)
+  real(c_double) :: AGNBFCC(npbf), NUCEX(npbf), NUCBFC(npbf, 3)
+  real(c_double) :: INT_GAM2(dimINT2), XCHF_GAM2(dimXCHF2), XCHF_GAM2s(dimXCHF2)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_mult_system_create(nebf, npebf, nebfBE, npebfBE, npbf, nat, ngtg1, pmass, cat, zan, &
+                   bcoef1, gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR_be, KPEEND_be, &
+                   AMPEB2C_be, AGEBFCC_be, ELCEX_be, ELCAM_be, ELCBFC_be, AGNBFCC, NUCEX, NUCAM, NUCBFC, h), &
+                   'RXCHFmult_GAM2')
+  call neorx_check(neorx_rxchfmult_gam2(h, INT_GAM2, XCHF_GAM2, XCHF_GAM2s), 'RXCHFmult_GAM2')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM2')
+end subroutine
+
+! RXCHFmult_GAM3_INT, source/rxchfmult/rxchfmult_gam3-int.f:2-254
+subroutine RXCHFmult_GAM3_INT(Nchunks, nebf, npebf, nebfBE, npebfBE, npbf, ng3, nat, ngtg1, pmass, cat, zan, &
+                              bcoef1, gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR_be, &
+                              KPEEND_be, AMPEB2C_be, AGEBFCC_be, ELCEX_be, ELCAM_be, ELCBFC_be, AGNBFCC, NUCEX, &
+                              NUCAM, NUCBFC, INT_GAM3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, nebfBE, npebfBE, npbf, ng3, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3)
+  integer(c_int64_t) :: KPESTR_be(nebfBE), KPEEND_be(nebfBE), AMPEB2C_be(npebfBE), ELCAM_be(npebfBE, 3)
+  integer(c_int64_t) :: NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), ELCEX(npebf), ELCBFC(npebf, 3)
+  real(c_double) :: AGEBFCC_be(npebfBE), ELCEX_be(npebfBE), ELCBFC_be(npebfBE, 3)
+  real(c_double) :: AGNBFCC(npbf), NUCEX(npbf), NUCBFC(npbf, 3)
+  real(c_double) :: INT_GAM3(ng3)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_mult_system_create(nebf, npebf, nebfBE, npebfBE, npbf, nat, ngtg1, pmass, cat, zan, &
+                   bcoef1, gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR_be, KPEEND_be, &
+                   AMPEB2C_be, AGEBFCC_be, ELCEX_be, ELCAM_be, ELCBFC_be, AGNBFCC, NUCEX, NUCAM, NUCBFC, h), &
+                   'RXCHFmult_GAM3_INT')
+  call neorx_check(neorx_rxchfmult_gam3_int(h, INT_GAM3), 'RXCHFmult_GAM3_INT')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM3_INT')
+end subroutine
+
+! RXCHFmult_GAM4_INT, source/rxchfmult/rxchfmult_gam4-int.f:2-153: reads
+! GAM_ee.ufm (nebf^4, regular basis) like read_GAM_ee
+subroutine RXCHFmult_GAM4_INT(Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4, GAM_2s, GAM_4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4
+  real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
+  real(c_double), allocatable :: gee(:)
+  integer(c_int64_t) :: ia
+  allocate(gee(ngee))
+  open(811, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do ia = 1, ngee
+    read(811, rec=ia) gee(ia)
+  end do
+  close(811)
+  call neorx_check(neorx_rxchfmult_gam4_int(nebf, nebfBE, npbf, GAM_2s, gee, GAM_4), 'RXCHFmult_GAM4_INT')
+  deallocate(gee)
+end subroutine
+
+! RXCHFmult_FAE_OMG2, source/rxchfmult/rxchfmult_omg2.f:2-90: FAE accumulates, E overwritten
+subroutine RXCHFmult_FAE_OMG2(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fae_omg2(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FAE_OMG2')
+end subroutine
+
+! RXCHFmult_FP_OMG2, source/rxchfmult/rxchfmult_omg2.f:175-263: FP accumulates, E overwritten
+subroutine RXCHFmult_FP_OMG2(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(npbf, npbf), E
+  call neorx_check(neorx_rxchfmult_fp_omg2(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FP_OMG2')
+end subroutine
+
+! RXCHFmult_FBE_OMG2, source/rxchfmult/rxchfmult_omg2.f:348-436: FBE accumulates, E overwritten
+subroutine RXCHFmult_FBE_OMG2(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebfBE, nebfBE), E
+  call neorx_check(neorx_rxchfmult_fbe_omg2(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FBE_OMG2')
+end subroutine
+
+! RXCHFmult_FAE_OMG3, source/rxchfmult/rxchfmult_omg3.f:2-96: FAE accumulates, E overwritten
+subroutine RXCHFmult_FAE_OMG3(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fae_omg3(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FAE_OMG3')
+end subroutine
+
+! RXCHFmult_FP_OMG3, source/rxchfmult/rxchfmult_omg3.f:201-295: FP accumulates, E overwritten
+subroutine RXCHFmult_FP_OMG3(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(npbf, npbf), E
+  call neorx_check(neorx_rxchfmult_fp_omg3(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FP_OMG3')
+end subroutine
+
+! RXCHFmult_FBE_OMG3, source/rxchfmult/rxchfmult_omg3.f:400-494: FBE accumulates, E overwritten
+subroutine RXCHFmult_FBE_OMG3(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebfBE, nebfBE), E
+  call neorx_check(neorx_rxchfmult_fbe_omg3(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FBE_OMG3')
+end subroutine
+
+! RXCHFmult_FAE_OMG4, source/rxchfmult/rxchfmult_omg4.f:2-108: FAE accumulates, E overwritten
+subroutine RXCHFmult_FAE_OMG4(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fae_omg4(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FAE_OMG4')
+end subroutine
+
+! RXCHFmult_FP_OMG4, source/rxchfmult/rxchfmult_omg4.f:207-313: FP accumulates, E overwritten
+subroutine RXCHFmult_FP_OMG4(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(npbf, npbf), E
+  call neorx_check(neorx_rxchfmult_fp_omg4(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FP_OMG4')
+end subroutine
+
+! RXCHFmult_FBE_OMG4, source/rxchfmult/rxchfmult_omg4.f:412-518: FBE accumulates, E overwritten
+subroutine RXCHFmult_FBE_OMG4(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ng
+  real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebfBE, nebfBE), E
+  call neorx_check(neorx_rxchfmult_fbe_omg4(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
+                   'RXCHFmult_FBE_OMG4')
 end subroutine

@@ -91,6 +91,18 @@ def load():
     _sig("neorx_gam2_dscf_dev", [_I64] + [_P] * 8 + [_P])
     _sig("neorx_gam3_dscf_dev", [_I64] + [_P] * 5 + [_P])
     _sig("neorx_gam4_dscf_dev", [_I64, _I64] + [_P] * 7 + [_P])
+    _sig("neorx_mult_system_create", [_I64] * 7 + [_D] + [_P] * 22 + [_P])
+    _sig("neorx_prim_list3_dev", [_I64, _I64, _I64, _P, _P, _P, _P, _P])
+    _sig("neorx_rxchfmult_gam2", [_I64, _P, _P, _P])
+    _sig("neorx_rxchfmult_gam2_dev", [_I64, _P, _P, _P, _P])
+    _sig("neorx_rxchfmult_gam3_int", [_I64, _P])
+    _sig("neorx_rxchfmult_gam3_int_dev", [_I64, _P, _P])
+    _sig("neorx_rxchfmult_gam4_int", [_I64, _I64, _I64, _P, _P, _P])
+    _sig("neorx_rxchfmult_gam4_int_dev", [_I64, _I64, _I64, _P, _P, _P, _P])
+    for k in (2, 3, 4):
+        for n in ("fae", "fp", "fbe"):
+            _sig(f"neorx_rxchfmult_{n}_omg{k}", [_I64] * 3 + [_P] * 6)
+    _sig("neorx_rxchfmult_omg_dev", [_I64] * 4 + [_P] * 8 + [_P])
     return _lib
 
 
@@ -112,12 +124,18 @@ def init(device=0):
 class System:
     """Basis, nuclei and geminal resident in HBM (the arguments of GAM2_IC1,
     source/gam2_IC1_OMP.f:2-37).  ``sysd`` is a dict produced by
-    :mod:`neorx.synth` (Fortran-layout numpy arrays)."""
+    :mod:`neorx.synth` (Fortran-layout numpy arrays).  A dict that also holds a
+    special-electron basis (``nebfBE`` and the ``*_be`` arrays, as from
+    ``synth.mult``) gives a two-basis handle for the RXCHFmult routines
+    (the arguments of RXCHFmult_GAM2, source/rxchfmult/rxchfmult_gam2.f:2-12)."""
 
     def __init__(self, sysd):
         load()
         s = sysd
         self.s = s
+        if "nebfBE" in s:
+            self._init_mult(s)
+            return
         self._keep = [
             _np.ascontiguousarray(s[k])
             for k in ("cat", "zan", "bcoef", "gamma", "kpestr", "kpeend", "ampeb2c", "agebfcc", "agnbfcc",
@@ -127,6 +145,19 @@ class System:
         h = _C.c_int64()
         _check(_lib.neorx_system_create(s["nebf"], s["npebf"], s["npbf"], s["nat"], s["ngtg"], s["pmass"],
                                         *[_p(a) for a in self._keep], _C.byref(h)))
+        self.handle = h.value
+
+    def _init_mult(self, s):
+        c, f = _np.ascontiguousarray, _np.asfortranarray
+        self._keep = [c(s[k]) for k in ("cat", "zan", "bcoef", "gamma", "kpestr", "kpeend", "ampeb2c", "agebfcc",
+                                        "elcex")]
+        self._keep += [f(s["elcam"]), f(s["elcbfc"])]
+        self._keep += [c(s[k + "_be"]) for k in ("kpestr", "kpeend", "ampeb2c", "agebfcc", "elcex")]
+        self._keep += [f(s["elcam_be"]), f(s["elcbfc_be"])]
+        self._keep += [c(s["agnbfcc"]), c(s["nucex"]), f(s["nucam"]), f(s["nucbfc"])]
+        h = _C.c_int64()
+        _check(_lib.neorx_mult_system_create(s["nebf"], s["npebf"], s["nebfBE"], s["npebfBE"], s["npbf"], s["nat"],
+                                             s["ngtg"], s["pmass"], *[_p(a) for a in self._keep], _C.byref(h)))
         self.handle = h.value
 
     def close(self):
@@ -149,6 +180,26 @@ class System:
     # --- primitive tuples (device pointers) ---
     def prim_list_dev(self, cls, ntup, d_idx, d_e, d_s, stream=0):
         _check(_lib.neorx_prim_list_dev(self.handle, cls, ntup, d_idx, d_e, d_s, stream))
+
+    def prim_list3_dev(self, cls, ntup, d_idx, d_e, d_s, d_e2, stream=0):
+        _check(_lib.neorx_prim_list3_dev(self.handle, cls, ntup, d_idx, d_e, d_s, d_e2, stream))
+
+    # --- RXCHFmult builders (two-basis handle) ---
+    def RXCHFmult_GAM2(self):
+        """INT_GAM2, XCHF_GAM2, XCHF_GAM2s (rxchfmult/rxchfmult_gam2.f:2)."""
+        na, nb, npb = self.s["nebf"], self.s["nebfBE"], self.s["npbf"]
+        gi = _np.zeros(npb * npb * na * na * nb * nb)
+        gx = _np.zeros(npb * npb * nb ** 4)
+        gs = _np.zeros_like(gx)
+        _check(_lib.neorx_rxchfmult_gam2(self.handle, _p(gi), _p(gx), _p(gs)))
+        return gi, gx, gs
+
+    def RXCHFmult_GAM3_INT(self):
+        """INT_GAM3 (rxchfmult/rxchfmult_gam3-int.f:2)."""
+        na, nb, npb = self.s["nebf"], self.s["nebfBE"], self.s["npbf"]
+        g = _np.zeros(npb * npb * na * na * nb ** 4)
+        _check(_lib.neorx_rxchfmult_gam3_int(self.handle, _p(g)))
+        return g
 
     def prim_range_dev(self, cls, start, count, d_e, d_s, stream=0):
         _check(_lib.neorx_prim_range_dev(self.handle, cls, start, count, d_e, d_s, stream))
@@ -371,4 +422,28 @@ def RXCHFne_FAE_GAMee(nebf, GAM_ee, DAE, FAE):
     load()
     e = _np.zeros(1)
     _check(_lib.neorx_rxchfne_fae_gamee(nebf, _p(_np.ascontiguousarray(GAM_ee)), _p(_fort(DAE)), _p(FAE), _p(e)))
+    return float(e[0])
+
+
+# ---------------------------------------------------------------------------
+# RXCHFmult (source/rxchfmult/)
+# ---------------------------------------------------------------------------
+def RXCHFmult_GAM4_INT(nebf, nebfBE, npbf, gam_2s, gam_ee):
+    """INT_GAM4 (rxchfmult/rxchfmult_gam4-int.f:2) from XCHF_GAM2s and GAM_ee."""
+    g = _np.zeros(npbf * npbf * nebf * nebf * nebfBE ** 6)
+    load()
+    _check(_lib.neorx_rxchfmult_gam4_int(nebf, nebfBE, npbf, _p(_np.ascontiguousarray(gam_2s)),
+                                         _p(_np.ascontiguousarray(gam_ee)), _p(g)))
+    return g
+
+
+def RXCHFmult_OMG(which, k, nebf, nebfBE, npbf, DAE, DBE, DP, GM, F):
+    """RXCHFmult_{FAE,FP,FBE}_OMG{2,3,4} (rxchfmult/rxchfmult_omg{2,3,4}.f):
+    which 0/1/2 = FAE/FP/FBE, k = 2, 3, 4.  F (Fortran layout) accumulates;
+    returns the energy."""
+    load()
+    e = _np.zeros(1)
+    name = f"neorx_rxchfmult_{('fae', 'fp', 'fbe')[which]}_omg{k}"
+    _check(getattr(_lib, name)(nebf, nebfBE, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
+                               _p(_np.ascontiguousarray(GM)), _p(F), _p(e)))
     return float(e[0])

@@ -189,6 +189,55 @@ def small(seed=0, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 
     return _finish(prims, contr, nuc, catv, zan, b, g)
 
 
+BE_KEYS = ("kpestr", "kpeend", "ampeb2c", "agebfcc", "elcex", "elcam", "elcbfc")
+
+
+def mult(seed=0, e_shells=(0, 1), be_shells=(0, 1), be_centres=1, **kw):
+    """Two-basis system for the RXCHFmult routines: the regular electron basis
+    of small(seed, ...) plus a special-electron basis (keys nebfBE, npebfBE and
+    <key>_be) drawn from another seed on be_centres centres, sharing nuclei,
+    the proton basis and the geminal."""
+    a = small(seed=seed, e_shells=e_shells, **kw)
+    b = small(seed=seed + 7919, e_shells=be_shells, n_e_centres=be_centres,
+              **{k: v for k, v in kw.items() if k != "n_e_centres"})
+    out = dict(a)
+    out["nebfBE"], out["npebfBE"] = b["nebf"], b["npebf"]
+    for k in BE_KEYS:
+        out[k + "_be"] = b[k]
+    return out
+
+
+def be_part(sysd):
+    """The special-electron basis of a two-basis system as a one-basis system
+    (for the XCHF routines over that basis: GAM1_MD, GAM_ee, ...)."""
+    out = dict(sysd)
+    out["nebf"], out["npebf"] = sysd["nebfBE"], sysd["npebfBE"]
+    for k in BE_KEYS:
+        out[k] = sysd[k + "_be"]
+    return out
+
+
+def combined(sysd):
+    """The primitive list of a two-basis handle as a one-basis dict: regular
+    primitives first, then the special-electron ones (contracted functions and
+    KPESTR/KPEEND/AMPEB2C offset to match).  Primitive-tuple indices of the
+    RXCHFmult classes refer to this list."""
+    out = dict(sysd)
+    na, pa = sysd["nebf"], sysd["npebf"]
+    out["nebf"], out["npebf"] = na + sysd["nebfBE"], pa + sysd["npebfBE"]
+    out["kpestr"] = np.concatenate([sysd["kpestr"], sysd["kpestr_be"] + pa])
+    out["kpeend"] = np.concatenate([sysd["kpeend"], sysd["kpeend_be"] + pa])
+    out["ampeb2c"] = np.concatenate([sysd["ampeb2c"], sysd["ampeb2c_be"] + na])
+    for k in ("agebfcc", "elcex"):
+        out[k] = np.concatenate([sysd[k], sysd[k + "_be"]])
+    for k in ("elcam", "elcbfc"):
+        out[k] = np.asfortranarray(np.concatenate([sysd[k], sysd[k + "_be"]], axis=0))
+    for k in list(out):
+        if k.endswith("_be") or k in ("nebfBE", "npebfBE"):
+            del out[k]
+    return out
+
+
 def density(n, trace, seed=0):
     """Random symmetric positive semidefinite n x n matrix with given trace."""
     rng = np.random.default_rng(seed)

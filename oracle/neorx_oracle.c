@@ -664,12 +664,17 @@ static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const
     *ansS = 0.0 + 2.0 * ogx + ogg;
 }
 
-/* xcalc_GAM2_MD (gam_2_OMP.f:546-1077): particles e1, e2, p */
-static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE, double *ansS)
+/* xcalc_GAM2_MD (gam_2_OMP.f:546-1077): particles e1, e2, p.
+ * With mult set: RXCHFmult_GAM2_MD (rxchfmult/rxchfmult_gam2.f:627-1162), the
+ * same two-geminal loop without the one-geminal terms, xgVEEg1 with the
+ * geminals on e2-p (gamA23, gamB23, :1067-1068), and the outputs
+ * ansE1 (:1149-1155) in ansE, ansE2 = gVEPg2 + gVEEg1 (:1156), ansS = gsg (:1158). */
+static void gam2_prim_impl(const sys_t *S, const prim_t *b, const prim_t *k, int mult, double *ansE, double *ansS,
+                           double *ansE2)
 {
     const prim_t *b1 = &b[0], *b2 = &b[1], *b3 = &b[2], *k1 = &k[0], *k2 = &k[1], *k3 = &k[2];
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
-    for (i64 ik = 0; ik < S->ngtg; ++ik) {
+    for (i64 ik = 0; ik < (mult ? 0 : S->ngtg); ++ik) {
         double gk = S->gamma[ik], bk = S->bcoef[ik];
         double xgTE = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, 0, NULL, 1.0);
         double xgVEC = 0.0;
@@ -718,7 +723,8 @@ static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *
             double xgVEPg2 = g3(OP_COUL, b1, b3, b2, k1, k3, k2, 0, 0, gk, 0, 0, gl, NULL, 1.0);
             underflow(&xgVEPg2);
             xgVEPg2 = -xgVEPg2;
-            double xgVEEg1 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, gl, 0, NULL, 1.0);
+            double xgVEEg1 = mult ? g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, 0, gk, 0, 0, gl, NULL, 1.0)
+                                  : g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, gl, 0, NULL, 1.0);
             underflow(&xgVEEg1);
             double xgVEEg2 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
             underflow(&xgVEEg2);
@@ -736,9 +742,49 @@ static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *
             gVEEg2 += bb * xgVEEg2;
             gsg += bb * xgsg;
         }
-    *ansE = 2.0 * (gTE + gVEC + gVEP) + 4.0 * gVEE * 0.5 + gTPg + gVPCg + gTEg1 + gTEg3 + 2.0 * (gVEPg1 + gVECg1) +
-            gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
+    if (mult)
+        *ansE = gTPg + gVPCg + gTEg1 + gTEg3 + 2.0 * (gVEPg1 + gVECg1) + gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 +
+                2.0 * gVEEg2 * 0.5;
+    else
+        *ansE = 2.0 * (gTE + gVEC + gVEP) + 4.0 * gVEE * 0.5 + gTPg + gVPCg + gTEg1 + gTEg3 +
+                2.0 * (gVEPg1 + gVECg1) + gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
     *ansS = gsg;
+    if (ansE2) *ansE2 = gVEPg2 + gVEEg1;
+}
+
+static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE, double *ansS)
+{
+    gam2_prim_impl(S, b, k, 0, ansE, ansS, NULL);
+}
+
+/* RXCHFmult_INT_GAM3_MD (rxchfmult/rxchfmult_gam3-int.f:661-1054): particles
+ * e1 (regular basis), e2, e3 (special basis), p. */
+static void gam3int_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
+{
+    double gVEEg1 = 0, gVEEg2 = 0, gVEPg1 = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            /* G4_MD_xgVepg(e2, e3, e1, p; gamA14, gamB24): 1/r(e1,p), geminals e2-p, e3-p (:901) */
+            prim_t bv[4] = {b[1], b[2], b[0], b[3]}, kv[4] = {k[1], k[2], k[0], k[3]};
+            double zz = g4(1, bv, kv, gk, gl);
+            underflow(&zz);
+            double xgVEPg1 = zz * (-1.0);
+            /* G2_MD_xggs(e2, p; gamA, gamB) * gfvee(e1, e3) (:941, :954) */
+            double xx = g2(OP_OVL, &b[1], &b[3], &k[1], &k[3], gk, gl, NULL, 1.0);
+            double yy = gfvee(&b[0], &b[2], &k[0], &k[2]);
+            underflow(&xx);
+            underflow(&yy);
+            double xgVEEg1 = xx * yy;
+            /* G4_MD_xgVeeg(e2, e1, e3, p; gamA14, gamB34): 1/r(e2,e1) (:998), no underflow */
+            prim_t bw[4] = {b[1], b[0], b[2], b[3]}, kw[4] = {k[1], k[0], k[2], k[3]};
+            double xgVEEg2 = g4(0, bw, kw, gk, gl);
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            gVEEg1 = gVEEg1 + (bb * xgVEEg1);
+            gVEEg2 = gVEEg2 + (bb * xgVEEg2);
+            gVEPg1 = gVEPg1 + (bb * xgVEPg1);
+        }
+    *ans = gVEPg1 + gVEEg1 + 2.0 * gVEEg2;
 }
 
 /* RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): two outputs,
@@ -854,12 +900,12 @@ static prim_t pprim(const basis_t *B, i64 i)
  * primitive indices: cls 1 -> (ie1, ip, je1, jp); cls 2 -> (ie1,ie2,ip,je1,je2,jp);
  * cls 3 -> (ie1,ie2,ie3,ip,je1,je2,je3,jp).  Outputs ansE (and ansS for
  * cls 1,2). */
-void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
+static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup, const i64 *idx, double *outE,
+                           double *outS, double *outX)
 {
-    BAS_INIT;
-    SYS_INIT;
-    /* cls 4: the RXCHF-ne e-e-p tuple (two electrons, like cls 2) */
-    int nel = cls == 4 ? 2 : (int)cls;
+    /* cls 4: the RXCHF-ne e-e-p tuple; cls 5, 6: the RXCHFmult e-e-p and
+     * e-e-e-p tuples (primitive indices over one combined list) */
+    int nel = (cls == 4 || cls == 5) ? 2 : cls == 6 ? 3 : (int)cls;
     int w = 2 * (nel + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
@@ -867,23 +913,43 @@ void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double
         prim_t b[4], k[4];
         int ne = nel;
         for (int i = 0; i < ne; ++i) {
-            b[i] = eprim(&B, x[i]);
-            k[i] = eprim(&B, x[ne + 1 + i]);
+            b[i] = eprim(Bp, x[i]);
+            k[i] = eprim(Bp, x[ne + 1 + i]);
         }
-        b[ne] = pprim(&B, x[ne]);
-        k[ne] = pprim(&B, x[2 * ne + 1]);
-        double e = 0, s = 0;
+        b[ne] = pprim(Bp, x[ne]);
+        k[ne] = pprim(Bp, x[2 * ne + 1]);
+        double e = 0, s = 0, e2 = 0;
         if (cls == 1)
-            gam1_prim(&S, &b[0], &b[1], &k[0], &k[1], &e, &s);
+            gam1_prim(Sp, &b[0], &b[1], &k[0], &k[1], &e, &s);
         else if (cls == 2)
-            gam2_prim(&S, b, k, &e, &s);
+            gam2_prim(Sp, b, k, &e, &s);
         else if (cls == 4)
-            rxne_prim(&S, b, k, &e, &s);
+            rxne_prim(Sp, b, k, &e, &s);
+        else if (cls == 5)
+            gam2_prim_impl(Sp, b, k, 1, &e, &s, &e2);
+        else if (cls == 6)
+            gam3int_prim(Sp, b, k, &e);
         else
-            gam3_prim(&S, b, k, &e);
+            gam3_prim(Sp, b, k, &e);
         outE[t] = e;
         if (outS) outS[t] = s;
+        if (outX) outX[t] = e2;
     }
+}
+
+void orc_prim_list(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS)
+{
+    BAS_INIT;
+    SYS_INIT;
+    prim_list_impl(&B, &S, cls, ntup, idx, outE, outS, NULL);
+}
+
+/* Same with the third output (ansE2 of cls 5, zero otherwise). */
+void orc_prim_list3(BAS_ARGS, SYS_ARGS, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS, double *outX)
+{
+    BAS_INIT;
+    SYS_INIT;
+    prim_list_impl(&B, &S, cls, ntup, idx, outE, outS, outX);
 }
 
 /* Elementary integral probe: op (0 ovl,1 nuc,2 coul,3 kin), n particles,
@@ -1783,4 +1849,263 @@ void orc_rxchfne_fae_gamee(i64 nebf, const double *GAMee, const double *DAE, dou
     for (i64 k = 0; k < ne * ne; ++k) FAE[k] += xf[k];
     *E = e;
     free(xf);
+}
+
+/* ------------------------------------------------------------------ */
+/* RXCHFmult (rxchfmult/): regular basis B, special-electron basis BB   */
+/* ------------------------------------------------------------------ */
+#define MBAS_ARGS                                                                                                  \
+    i64 nebf, i64 npebf, i64 nebfBE, i64 npebfBE, i64 npbf, const i64 *kpestr, const i64 *kpeend,                 \
+        const double *agebfcc, const double *elcex, const i64 *elcam, const double *elcbfc, const i64 *kpestr_be,  \
+        const i64 *kpeend_be, const double *agebfcc_be, const double *elcex_be, const i64 *elcam_be,              \
+        const double *elcbfc_be, const double *agnbfcc, const double *nucex, const i64 *nucam, const double *nucbfc
+#define MBAS_INIT                                                                                                  \
+    basis_t B = {nebf, npebf, npbf, kpestr, kpeend, NULL, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc,     \
+                 nucbfc};                                                                                          \
+    basis_t BB = {nebfBE, npebfBE, npbf, kpestr_be, kpeend_be, NULL, agebfcc_be, agnbfcc, elcex_be, nucex,         \
+                  elcam_be, nucam, elcbfc_be, nucbfc}
+
+/* RXCHFmult_GAM_2PK / _3PK / _4PK (rxchfmult_basis.f:2-134), 0-based:
+ * [ip][jp][ie1][je1] over the regular basis (na), the rest over nb */
+static inline i64 mix2(i64 na, i64 nb, i64 np, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i64 j2)
+{
+    return ((((ip * np + jp) * na + i1) * na + j1) * nb + i2) * nb + j2;
+}
+static inline i64 mix3(i64 na, i64 nb, i64 np, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i64 j2, i64 i3, i64 j3)
+{
+    return (mix2(na, nb, np, ip, jp, i1, j1, i2, j2) * nb + i3) * nb + j3;
+}
+
+/* RXCHFmult_GAM2 (rxchfmult_gam2.f:2-265) with RXCHFmult_contract_omega2
+ * (:413-626): XG2_1ICR, XG2_2ICR, XG2sICR over (ip,jp, iec1,jec1 regular,
+ * iec2,jec2 special), then XCHF_GAM2, XCHF_GAM2s = (x12 + x21)/2 over special
+ * indices (x12, x21 read through the two-basis packing, :173-228) and
+ * INT_GAM2 = XG2_2ICR. */
+void orc_rxchfmult_gam2(MBAS_ARGS, SYS_ARGS, double *INT_GAM2, double *XCHF_GAM2, double *XCHF_GAM2s)
+{
+    MBAS_INIT;
+    SYS_INIT;
+    i64 na = nebf, nb = nebfBE, np = npbf, ng = np * np * na * na * nb * nb;
+    double *X1 = (double *)malloc(sizeof(double) * ng), *X2 = (double *)malloc(sizeof(double) * ng),
+           *Xs = (double *)malloc(sizeof(double) * ng);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < na; ++i1)
+                for (i64 j1 = 0; j1 < na; ++j1)
+                    for (i64 i2 = 0; i2 < nb; ++i2)
+                        for (i64 j2 = 0; j2 < nb; ++j2) {
+                            double o1 = 0.0, o2 = 0.0, os = 0.0;
+                            for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr_be[i2] - 1; a2 < kpeend_be[i2]; ++a2)
+                                        for (i64 c2 = kpestr_be[j2] - 1; c2 < kpeend_be[j2]; ++c2) {
+                                            prim_t b[3] = {eprim(&B, a1), eprim(&BB, a2), pprim(&B, ip)};
+                                            prim_t k[3] = {eprim(&B, c1), eprim(&BB, c2), pprim(&B, jp)};
+                                            double e1, e2, sv;
+                                            gam2_prim_impl(&S, b, k, 1, &e1, &sv, &e2);
+                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                        agebfcc_be[a2] * agebfcc_be[c2];
+                                            o1 += e1 * cf;
+                                            o2 += e2 * cf;
+                                            os += sv * cf;
+                                        }
+                            i64 ia = mix2(na, nb, np, ip, jp, i1, j1, i2, j2);
+                            X1[ia] = o1;
+                            X2[ia] = o2;
+                            Xs[ia] = os;
+                        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i2 = 0; i2 < nb; ++i2)
+                for (i64 j2 = 0; j2 < nb; ++j2) {
+                    for (i64 i1 = 0; i1 < nb; ++i1)
+                        for (i64 j1 = 0; j1 < nb; ++j1) {
+                            i64 a12 = mix2(na, nb, np, ip, jp, i1, j1, i2, j2);
+                            i64 a21 = mix2(na, nb, np, ip, jp, i2, j2, i1, j1);
+                            i64 ia = ix2(nb, np, ip, jp, i1, j1, i2, j2);
+                            XCHF_GAM2[ia] = (X1[a12] + X1[a21]) / 2.0;
+                            XCHF_GAM2s[ia] = (Xs[a12] + Xs[a21]) / 2.0;
+                        }
+                    for (i64 i1 = 0; i1 < na; ++i1)
+                        for (i64 j1 = 0; j1 < na; ++j1) {
+                            i64 ia = mix2(na, nb, np, ip, jp, i1, j1, i2, j2);
+                            INT_GAM2[ia] = X2[ia];
+                        }
+                }
+    free(X1);
+    free(X2);
+    free(Xs);
+}
+
+/* RXCHFmult_GAM3_INT (rxchfmult_gam3-int.f:2-254) with
+ * RXCHFmult_contract_omega3_INT (:417-660) and the average over the exchange
+ * of electrons 2 and 3 (:173-218). */
+void orc_rxchfmult_gam3_int(MBAS_ARGS, SYS_ARGS, double *INT_GAM3)
+{
+    MBAS_INIT;
+    SYS_INIT;
+    i64 na = nebf, nb = nebfBE, np = npbf, ng = np * np * na * na * nb * nb * nb * nb;
+    double *X = (double *)malloc(sizeof(double) * ng);
+#pragma omp parallel for collapse(3) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < na; ++i1)
+                for (i64 j1 = 0; j1 < na; ++j1)
+                    for (i64 i2 = 0; i2 < nb; ++i2)
+                        for (i64 j2 = 0; j2 < nb; ++j2)
+                            for (i64 i3 = 0; i3 < nb; ++i3)
+                                for (i64 j3 = 0; j3 < nb; ++j3) {
+                                    double om = 0.0;
+                                    for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                        for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                            for (i64 a2 = kpestr_be[i2] - 1; a2 < kpeend_be[i2]; ++a2)
+                                                for (i64 c2 = kpestr_be[j2] - 1; c2 < kpeend_be[j2]; ++c2)
+                                                    for (i64 a3 = kpestr_be[i3] - 1; a3 < kpeend_be[i3]; ++a3)
+                                                        for (i64 c3 = kpestr_be[j3] - 1; c3 < kpeend_be[j3]; ++c3) {
+                                                            prim_t b[4] = {eprim(&B, a1), eprim(&BB, a2),
+                                                                           eprim(&BB, a3), pprim(&B, ip)};
+                                                            prim_t k[4] = {eprim(&B, c1), eprim(&BB, c2),
+                                                                           eprim(&BB, c3), pprim(&B, jp)};
+                                                            double v;
+                                                            gam3int_prim(&S, b, k, &v);
+                                                            om += v * agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] *
+                                                                  agebfcc[c1] * agebfcc_be[a2] * agebfcc_be[c2] *
+                                                                  agebfcc_be[a3] * agebfcc_be[c3];
+                                                        }
+                                    X[mix3(na, nb, np, ip, jp, i1, j1, i2, j2, i3, j3)] = om;
+                                }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < na; ++i1)
+                for (i64 j1 = 0; j1 < na; ++j1)
+                    for (i64 i2 = 0; i2 < nb; ++i2)
+                        for (i64 j2 = 0; j2 < nb; ++j2)
+                            for (i64 i3 = 0; i3 < nb; ++i3)
+                                for (i64 j3 = 0; j3 < nb; ++j3) {
+                                    i64 a123 = mix3(na, nb, np, ip, jp, i1, j1, i2, j2, i3, j3);
+                                    i64 a132 = mix3(na, nb, np, ip, jp, i1, j1, i3, j3, i2, j2);
+                                    double xx = (X[a123] + X[a132]) * 0.5;
+                                    INT_GAM3[a123] = xx;
+                                    INT_GAM3[a132] = xx;
+                                }
+    free(X);
+}
+
+/* RXCHFmult_calc_GAM_4_integral (rxchfmult_gam4-int.f:392-486), 0-based
+ * indices: GAM_ee by pack_4D(nebf,nebf,nebf, je_x,ie_x,je1,ie1) over the
+ * regular basis, GAM_2s by index_GAM_2PK(nebfBE, ...). */
+static double mult_gam4_int(i64 na, i64 nb, i64 np, const double *G2s, const double *Gee, i64 ip, i64 jp, i64 ie1,
+                            i64 je1, i64 ie2, i64 je2, i64 ie3, i64 je3, i64 ie4, i64 je4)
+{
+#define EE(a, b, c, d) Gee[(((a) * na + (b)) * na + (c)) * na + (d)]
+    double ee14 = EE(ie1, je1, ie4, je4), ee13 = EE(ie1, je1, ie3, je3), ee12 = EE(ie1, je1, ie2, je2);
+#undef EE
+    underflow(&ee14);
+    underflow(&ee13);
+    underflow(&ee12);
+    double gs23 = G2s[ix2(nb, np, ip, jp, ie2, je2, ie3, je3)];
+    double gs24 = G2s[ix2(nb, np, ip, jp, ie2, je2, ie4, je4)];
+    double gs34 = G2s[ix2(nb, np, ip, jp, ie3, je3, ie4, je4)];
+    underflow(&gs23);
+    underflow(&gs24);
+    underflow(&gs34);
+    double x1 = ee14 * gs23, x2 = ee13 * gs24, x3 = ee12 * gs34;
+    return (x1 + x2 + x3) / 3.0;
+}
+
+/* RXCHFmult_GAM4_INT (rxchfmult_gam4-int.f:2-153) with RXCHFmult_symm_gam4
+ * (:290-352) and RXCHFmult_i10 (:353-391). */
+void orc_rxchfmult_gam4_int(i64 nebf, i64 nebfBE, i64 npbf, const double *GAM_2s, const double *GAM_ee, double *GAM_4)
+{
+    i64 na = nebf, nb = nebfBE, np = npbf;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 ii = 0; ii < na; ++ii)
+                for (i64 i = 0; i < na; ++i)
+                    for (i64 jj = 0; jj < nb; ++jj)
+                        for (i64 j = 0; j < nb; ++j)
+                            for (i64 kk = 0; kk < nb; ++kk)
+                                for (i64 k = 0; k < nb; ++k)
+                                    for (i64 ll = 0; ll < nb; ++ll)
+                                        for (i64 l = 0; l < nb; ++l) {
+#define I10(a, b, c, d) mult_gam4_int(na, nb, np, GAM_2s, GAM_ee, ip, jp, ii, a, jj, b, kk, c, ll, d)
+                                            double x1 = I10(i, j, k, l), x2 = I10(i, j, l, k), x3 = I10(i, k, j, l);
+                                            double x4 = I10(i, k, l, j), x5 = I10(i, l, j, k), x6 = I10(i, l, k, j);
+#undef I10
+                                            double ans = x1 - x2 / 2.0 - x3 / 2.0 + x4 / 4.0 + x5 / 4.0 - x6 / 2.0;
+                                            i64 ia = ((mix2(na, nb, np, ip, jp, ii, i, jj, j) * nb + kk) * nb + k) *
+                                                         nb * nb + ll * nb + l;
+                                            GAM_4[ia] = ans;
+                                        }
+}
+
+/* RXCHFmult_thread_{FAE,FP,FBE}_OMG{2,3,4} (rxchfmult_omg2.f:91-174,
+ * :264-347, :437-520; rxchfmult_omg3.f:97-200, :296-399, :495-599;
+ * rxchfmult_omg4.f:109-206, :314-411, :519-620) and add2fock.  order 2, 3, 4;
+ * which 0 FAE (nebf^2), 1 FP (npbf^2), 2 FBE (nebfBE^2). */
+void orc_rxchfmult_omg(i64 order, i64 which, i64 nebf, i64 nebfBE, i64 npbf, const double *DAE, const double *DBE,
+                       const double *DP, const double *G, double *F, double *E)
+{
+    i64 na = nebf, nb = nebfBE, np = npbf;
+    i64 nf = which == 0 ? na * na : which == 1 ? np * np : nb * nb;
+    double *XF = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+#define DAm(i, j) DAE[(j) * na + (i)]
+#define DBm(i, j) DBE[(j) * nb + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < na; ++i1)
+                for (i64 j1 = 0; j1 < na; ++j1)
+                    for (i64 i2 = 0; i2 < nb; ++i2)
+                        for (i64 j2 = 0; j2 < nb; ++j2) {
+                            if (order == 2) {
+                                double val = G[mix2(na, nb, np, ip, jp, i1, j1, i2, j2)];
+                                if (which == 0) XF[j1 * na + i1] += DPm(ip, jp) * DBm(i2, j2) * val;
+                                if (which == 1) XF[jp * np + ip] += DAm(i1, j1) * DBm(i2, j2) * val;
+                                if (which == 2) XF[j2 * nb + i2] += DAm(i1, j1) * DPm(ip, jp) * val;
+                                e += DPm(ip, jp) * DAm(i1, j1) * DBm(i2, j2) * val;
+                                continue;
+                            }
+                            for (i64 i3 = 0; i3 < nb; ++i3)
+                                for (i64 j3 = 0; j3 < nb; ++j3) {
+                                    if (order == 3) {
+                                        double val = G[mix3(na, nb, np, ip, jp, i1, j1, i2, j2, i3, j3)] -
+                                                     0.5 * G[mix3(na, nb, np, ip, jp, i1, j1, i2, j3, i3, j2)];
+                                        if (which == 0)
+                                            XF[j1 * na + i1] += DBm(i2, j2) * DBm(i3, j3) * DPm(ip, jp) * val;
+                                        if (which == 1)
+                                            XF[jp * np + ip] += DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) * val;
+                                        if (which == 2)
+                                            XF[j2 * nb + i2] += 2.0 * DAm(i1, j1) * DBm(i3, j3) * DPm(ip, jp) * val;
+                                        e += DPm(ip, jp) * DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) * val;
+                                        continue;
+                                    }
+                                    for (i64 i4 = 0; i4 < nb; ++i4)
+                                        for (i64 j4 = 0; j4 < nb; ++j4) {
+                                            i64 ia = (mix3(na, nb, np, ip, jp, i1, j1, i2, j2, i3, j3) * nb + i4) * nb +
+                                                     j4;
+                                            double val = G[ia];
+                                            if (which == 0)
+                                                XF[j1 * na + i1] += DPm(ip, jp) * DBm(i2, j2) * DBm(i3, j3) *
+                                                                    DBm(i4, j4) * val;
+                                            if (which == 1)
+                                                XF[jp * np + ip] += DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) *
+                                                                    DBm(i4, j4) * val;
+                                            if (which == 2)
+                                                XF[j2 * nb + i2] += 3.0 * DAm(i1, j1) * DBm(i3, j3) * DBm(i4, j4) *
+                                                                    DPm(ip, jp) * val;
+                                            e += DPm(ip, jp) * DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) *
+                                                 DBm(i4, j4) * val;
+                                        }
+                                }
+                        }
+#undef DAm
+#undef DBm
+#undef DPm
+    for (i64 t = 0; t < nf; ++t) F[t] += XF[t];
+    *E = e;
+    free(XF);
 }

@@ -56,14 +56,21 @@ def boys(m, T):
 def prim_list(s, cls, idx):
     """Primitive tuple values (xcalc_GAM{1,2,3}_MD semantics) for an
     (ntup, 2*(cls+1)) int64 array of 0-based primitive indices."""
+    e, sv, _ = prim_list3(s, cls, idx)
+    return e, sv
+
+
+def prim_list3(s, cls, idx):
+    """As prim_list, with the third output (ansE2 of the RXCHFmult e-e-p class
+    5).  For classes 5 and 6 s is a one-basis dict over the combined primitive
+    list (synth.combined)."""
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     n = idx.shape[0]
-    e = np.zeros(n)
-    sv = np.zeros(n)
+    e, sv, e2 = np.zeros(n), np.zeros(n), np.zeros(n)
     ba, k1 = _bas(s)
     sa, k2 = _sysargs(s)
-    lib().orc_prim_list(*ba, *sa, C.c_int64(cls), C.c_int64(n), _p(idx), _p(e), _p(sv))
-    return e, sv
+    lib().orc_prim_list3(*ba, *sa, C.c_int64(cls), C.c_int64(n), _p(idx), _p(e), _p(sv), _p(e2))
+    return e, sv, e2
 
 
 def gam1(s):
@@ -190,15 +197,24 @@ def _d(v):
 def ref_prim_list(s, cls, idx):
     """xcalc_GAM{1,2,3}_MD of the reference for each tuple (cls 4:
     RXCHFne_xcalc_GAM2_MD, two outputs returned as E and S)."""
+    e, sv, _ = ref_prim_list3(s, cls, idx)
+    return e, sv
+
+
+def ref_prim_list3(s, cls, idx):
+    """As ref_prim_list; cls 5: RXCHFmult_GAM2_MD (ansE1, ansS, ansE2),
+    cls 6: RXCHFmult_INT_GAM3_MD, over a combined one-basis dict."""
     R = ref()
-    fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_}[cls]
-    nel = 2 if cls == 4 else cls
+    fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_,
+          5: R.rxchfmult_gam2_md_, 6: R.rxchfmult_int_gam3_md_}[cls]
+    nel = {4: 2, 5: 2, 6: 3}.get(cls, cls)
     am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
     c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
     cat = np.ascontiguousarray(s["cat"])
     zan, b, g = (np.ascontiguousarray(s[k]) for k in ("zan", "bcoef", "gamma"))
     out_e = np.zeros(len(idx))
     out_s = np.zeros(len(idx))
+    out_x = np.zeros(len(idx))
     for t, x in enumerate(np.asarray(idx)):
         keep, args = [], []
 
@@ -213,15 +229,17 @@ def ref_prim_list(s, cls, idx):
         for i in range(nel):
             args += prim(am_e, s["elcex"], c_e, x[nel + 1 + i])
         args += prim(am_p, s["nucex"], c_p, x[2 * nel + 1])
-        e = C.c_double()
-        sv = C.c_double()
+        e, sv, xv = C.c_double(), C.c_double(), C.c_double()
         tail = [_i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g), C.byref(e)]
-        if cls != 3:
+        if cls == 5:  # ansE1, ansE2, ansS
+            tail += [C.byref(xv), C.byref(sv)]
+        elif cls not in (3, 6):
             tail.append(C.byref(sv))
         fn(*args, *tail)
         out_e[t] = e.value
         out_s[t] = sv.value
-    return out_e, out_s
+        out_x[t] = xv.value
+    return out_e, out_s, out_x
 
 
 def _ref_basis_args(s):
@@ -562,4 +580,107 @@ def ref_rxcuhfne_omg2(which, ne, npb, DAa, DAt, DBE, DP, G, Gx, F, F2=None, nab=
 def ref_rxchfne_fae_gamee(ne, gee, DAE, F):
     e = C.c_double()
     ref().rxchfne_fae_gamee_(_i8(ne), _i8(ne ** 4), _p(np.ascontiguousarray(gee)), _f(DAE), _p(F), C.byref(e))
+    return e.value
+
+
+# ---------------------------------------------------------------------------
+# RXCHFmult (oracle and reference); s is a two-basis dict (synth.mult)
+# ---------------------------------------------------------------------------
+def _mbas(s):
+    c, f = np.ascontiguousarray, np.asfortranarray
+    keep = [c(s["kpestr"]), c(s["kpeend"]), c(s["agebfcc"]), c(s["elcex"]), f(s["elcam"]), f(s["elcbfc"]),
+            c(s["kpestr_be"]), c(s["kpeend_be"]), c(s["agebfcc_be"]), c(s["elcex_be"]), f(s["elcam_be"]),
+            f(s["elcbfc_be"]), c(s["agnbfcc"]), c(s["nucex"]), f(s["nucam"]), f(s["nucbfc"])]
+    args = [C.c_int64(s[k]) for k in ("nebf", "npebf", "nebfBE", "npebfBE", "npbf")] + [_p(a) for a in keep]
+    return args, keep
+
+
+def _mdims(s):
+    return s["nebf"], s["nebfBE"], s["npbf"]
+
+
+def rxchfmult_gam2(s):
+    na, nb, npb = _mdims(s)
+    gi = np.zeros(npb * npb * na * na * nb * nb)
+    gx, gs = np.zeros(npb * npb * nb ** 4), np.zeros(npb * npb * nb ** 4)
+    ba, k1 = _mbas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam2(*ba, *sa, _p(gi), _p(gx), _p(gs))
+    return gi, gx, gs
+
+
+def rxchfmult_gam3_int(s):
+    na, nb, npb = _mdims(s)
+    g = np.zeros(npb * npb * na * na * nb ** 4)
+    ba, k1 = _mbas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam3_int(*ba, *sa, _p(g))
+    return g
+
+
+def rxchfmult_gam4_int(na, nb, npb, g2s, gee):
+    g = np.zeros(npb * npb * na * na * nb ** 6)
+    lib().orc_rxchfmult_gam4_int(C.c_int64(na), C.c_int64(nb), C.c_int64(npb), _p(np.ascontiguousarray(g2s)),
+                                 _p(np.ascontiguousarray(gee)), _p(g))
+    return g
+
+
+def rxchfmult_omg(order, which, na, nb, npb, DAE, DBE, DP, G, F):
+    """RXCHFmult_{FAE,FP,FBE}_OMG{order}; which 0/1/2; F accumulates."""
+    e = np.zeros(1)
+    lib().orc_rxchfmult_omg(C.c_int64(order), C.c_int64(which), C.c_int64(na), C.c_int64(nb), C.c_int64(npb),
+                            _f(DAE), _f(DBE), _f(DP), _p(np.ascontiguousarray(G)), _p(F), _p(e))
+    return float(e[0])
+
+
+def _ref_mbas(s):
+    c, f = np.ascontiguousarray, np.asfortranarray
+    return [c(s["kpestr"]), c(s["kpeend"]), c(s["ampeb2c"]), c(s["agebfcc"]), c(s["elcex"]), f(s["elcam"]),
+            f(s["elcbfc"]), c(s["kpestr_be"]), c(s["kpeend_be"]), c(s["ampeb2c_be"]), c(s["agebfcc_be"]),
+            c(s["elcex_be"]), f(s["elcam_be"]), f(s["elcbfc_be"]), c(s["agnbfcc"]), c(s["nucex"]), f(s["nucam"]),
+            f(s["nucbfc"])]
+
+
+def ref_rxchfmult_gam2(s):
+    """RXCHFmult_GAM2 (source/rxchfmult/rxchfmult_gam2.f:2)."""
+    na, nb, npb = _mdims(s)
+    nint, nx = npb * npb * na * na * nb * nb, npb * npb * nb ** 4
+    gi, gx, gs = np.zeros(nint), np.zeros(nx), np.zeros(nx)
+    keep = _ref_mbas(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfmult_gam2_(_i8(1), _i8(na), _i8(s["npebf"]), _i8(nb), _i8(s["npebfBE"]), _i8(npb), _i8(nint),
+                          _i8(nx), _i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm),
+                          *[_p(a) for a in keep], _p(gi), _p(gx), _p(gs))
+    return gi, gx, gs
+
+
+def ref_rxchfmult_gam3_int(s):
+    """RXCHFmult_GAM3_INT (source/rxchfmult/rxchfmult_gam3-int.f:2)."""
+    na, nb, npb = _mdims(s)
+    n3 = npb * npb * na * na * nb ** 4
+    g = np.zeros(n3)
+    keep = _ref_mbas(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfmult_gam3_int_(_i8(1), _i8(na), _i8(s["npebf"]), _i8(nb), _i8(s["npebfBE"]), _i8(npb), _i8(n3),
+                              _i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm),
+                              *[_p(a) for a in keep], _p(g))
+    return g
+
+
+def ref_rxchfmult_gam4_int(na, nb, npb, g2s, gee, workdir):
+    """RXCHFmult_GAM4_INT (source/rxchfmult/rxchfmult_gam4-int.f:2): reads
+    GAM_ee.ufm from workdir."""
+    np.ascontiguousarray(gee, dtype=np.float64).tofile(os.path.join(workdir, "GAM_ee.ufm"))
+    g4 = np.zeros(npb * npb * na * na * nb ** 6)
+    _in_dir(workdir, lambda: ref().rxchfmult_gam4_int_(_i8(1), _i8(na), _i8(nb), _i8(npb), _i8(na ** 4),
+                                                       _i8(npb * npb * nb ** 4), _i8(g4.size),
+                                                       _p(np.ascontiguousarray(g2s)), _p(g4)))
+    return g4
+
+
+def ref_rxchfmult_omg(order, which, na, nb, npb, DAE, DBE, DP, G, F):
+    e = C.c_double()
+    fn = getattr(ref(), f"rxchfmult_{('fae', 'fp', 'fbe')[which]}_omg{order}_")
+    fn(_i8(1), _i8(na), _i8(nb), _i8(npb), _i8(np.asarray(G).size), _f(DAE), _f(DBE), _f(DP),
+       _p(np.ascontiguousarray(G)), _p(F), C.byref(e))
     return e.value

@@ -56,15 +56,37 @@ def engine_host():
 def engine_prim_list(s, cls, idx, tier=0):
     """Host engine values; tier 0 picks the capacity as the device does,
     tier 1 forces the general capacity."""
+    e, sv, _ = engine_prim_list3(s, cls, idx, tier)
+    return e, sv
+
+
+def mult_tuples(sysd, cls, n, seed):
+    """Random primitive tuples of the RXCHFmult classes over the combined
+    primitive list of a two-basis system: electron 1 from the regular
+    primitives, the others from the special-electron ones."""
+    rng = np.random.default_rng(seed)
+    ne = 2 if cls == 5 else 3
+    pa, pb, npb = sysd["npebf"], sysd["npebfBE"], sysd["npbf"]
+    idx = np.zeros((n, 2 * (ne + 1)), dtype=np.int64)
+    for off in (0, ne + 1):
+        idx[:, off] = rng.integers(0, pa, n)
+        for i in range(1, ne):
+            idx[:, off + i] = pa + rng.integers(0, pb, n)
+        idx[:, off + ne] = rng.integers(0, npb, n)
+    return idx
+
+
+def engine_prim_list3(s, cls, idx, tier=0):
+    """As engine_prim_list with the third output (ansE2 of class 5)."""
     E = engine_host()
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     n = len(idx)
-    e, sv = np.zeros(n), np.zeros(n)
+    e, sv, xv = np.zeros(n), np.zeros(n), np.zeros(n)
     keep = [np.ascontiguousarray(s["elcex"]), np.ascontiguousarray(s["nucex"]), np.asfortranarray(s["elcam"]),
             np.asfortranarray(s["nucam"]), np.asfortranarray(s["elcbfc"]), np.asfortranarray(s["nucbfc"])]
     sk = [np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma")]
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
     E.eng_prim_list(C.c_int64(s["npebf"]), C.c_int64(s["npbf"]), *[P(a) for a in keep], C.c_int64(s["nat"]),
                     C.c_int64(s["ngtg"]), C.c_double(s["pmass"]), *[P(a) for a in sk], C.c_int64(cls), C.c_int64(n),
-                    P(idx), P(e), P(sv), C.c_int64(tier))
-    return e, sv
+                    P(idx), P(e), P(sv), P(xv), C.c_int64(tier))
+    return e, sv, xv

@@ -33,12 +33,13 @@ static Prim eprim(i64 npebf, const double *ex, const i64 *am, const double *cen,
 
 // tier: 0 = choose as the device does (small capacity when the tuple fits),
 // 1 = always the general capacity
-template <int CLS> static void eval_one(const SysParams &S, const PairRec *const *q, int tier, double &e, double &s)
+template <int CLS>
+static void eval_one(const SysParams &S, const PairRec *const *q, int tier, double &e, double &s, double &x)
 {
     if (tier == 0 && tuple_small<CLS>(q))
-        tuple_eval<CLS, Tier<CLS>::SMALL>(S, q, e, s);
+        tuple_eval<CLS, Tier<CLS>::SMALL>(S, q, e, s, x);
     else
-        tuple_eval<CLS, Tier<CLS>::GEN>(S, q, e, s);
+        tuple_eval<CLS, Tier<CLS>::GEN>(S, q, e, s, x);
 }
 
 static void pairs_of(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam, const i64 *nucam,
@@ -49,32 +50,35 @@ static void pairs_of(i64 npebf, i64 npbf, const double *elcex, const double *nuc
     make_pair(eprim(npbf, nucex, nucam, nucbfc, x[ne]), eprim(npbf, nucex, nucam, nucbfc, x[2 * ne + 1]), q[ne]);
 }
 
-static void eval_tuple(const SysParams &S, int cls, const PairRec *q, int tier, double &e, double &s)
+static void eval_tuple(const SysParams &S, int cls, const PairRec *q, int tier, double &e, double &s, double &x)
 {
     const PairRec *qp[4] = {&q[0], &q[1], &q[2], &q[3]};
-    e = s = 0.0;
-    if (cls == 1) eval_one<1>(S, qp, tier, e, s);
-    else if (cls == 2) eval_one<2>(S, qp, tier, e, s);
-    else if (cls == 4) eval_one<4>(S, qp, tier, e, s);
-    else eval_one<3>(S, qp, tier, e, s);
+    e = s = x = 0.0;
+    if (cls == 1) eval_one<1>(S, qp, tier, e, s, x);
+    else if (cls == 2) eval_one<2>(S, qp, tier, e, s, x);
+    else if (cls == 4) eval_one<4>(S, qp, tier, e, s, x);
+    else if (cls == 5) eval_one<5>(S, qp, tier, e, s, x);
+    else if (cls == 6) eval_one<6>(S, qp, tier, e, s, x);
+    else eval_one<3>(S, qp, tier, e, s, x);
 }
 
 extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam,
                               const i64 *nucam, const double *elcbfc, const double *nucbfc, i64 nat, i64 ngtg,
                               double pmass, const double *cat, const double *zan, const double *bcoef,
                               const double *gamma, i64 cls, i64 ntup, const i64 *idx, double *outE, double *outS,
-                              i64 tier)
+                              double *outX, i64 tier)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = cls == 4 ? 2 : (int)cls, w = 2 * (ne + 1);
+    int ne = (cls == 4 || cls == 5) ? 2 : cls == 6 ? 3 : (int)cls, w = 2 * (ne + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];
         pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
-        double e, s;
-        eval_tuple(S, (int)cls, q, (int)tier, e, s);
+        double e, s, x;
+        eval_tuple(S, (int)cls, q, (int)tier, e, s, x);
         outE[t] = e;
         if (outS) outS[t] = s;
+        if (outX) outX[t] = x;
     }
 }
 
@@ -119,13 +123,13 @@ extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, 
                                       const double *gamma, i64 cls, i64 ntup, const i64 *idx)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = cls == 4 ? 2 : (int)cls, w = 2 * (ne + 1);
+    int ne = (cls == 4 || cls == 5) ? 2 : cls == 6 ? 3 : (int)cls, w = 2 * (ne + 1);
     nrx_flops = 0.0;
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];
         pairs_of(npebf, npbf, elcex, nucex, elcam, nucam, elcbfc, nucbfc, ne, idx + t * w, q);
-        double e, s;
-        eval_tuple(S, (int)cls, q, 0, e, s);
+        double e, s, x;
+        eval_tuple(S, (int)cls, q, 0, e, s, x);
     }
     return nrx_flops / (double)ntup;
 }

@@ -133,5 +133,45 @@ program drive_shim
   se = 0.0d0
   call RXCUHFne_FE_OMG2(one, nebf, npbf, ng2, dbe, dae, de, dbe, dp, gm2, gm2s, fe, se, e2)
   write(11) fe, se, e2
+  ! RXCHFmult (rxchfmult_driver.f:398-633, rxchfmult_fock-int.f:366-577) with
+  ! the special-electron basis equal to the regular one (nebfBE = nebf):
+  ! INT_GAM2 in gm2, XCHF_GAM2 in gm2s, XCHF_GAM2s in gm3(1:ng2)
+  call RXCHFmult_GAM2(one, nebf, npebf, nebf, npebf, npbf, ng2, ng2, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                      kpestr, kpeend, ampeb2c, agebfcc, elcex, elcam, elcbfc, kpestr, kpeend, ampeb2c, agebfcc, &
+                      elcex, elcam, elcbfc, agnbfcc, nucex, nucam, nucbfc, gm2, gm2s, gm3)
+  write(11) gm2, gm2s, gm3(1:ng2)
+  gm2s(1:ng2) = gm3(1:ng2)
+  call RXCHFmult_GAM4_INT(one, nebf, nebf, npbf, ngee, ng2, ng4, gm2s, gm4)
+  call RXCHFmult_GAM3_INT(one, nebf, npebf, nebf, npebf, npbf, ng3, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                          kpestr, kpeend, ampeb2c, agebfcc, elcex, elcam, elcbfc, kpestr, kpeend, ampeb2c, agebfcc, &
+                          elcex, elcam, elcbfc, agnbfcc, nucex, nucam, nucbfc, gm3)
+  write(11) gm3, gm4
+  fe = 0.0d0
+  call RXCHFmult_FAE_OMG2(one, nebf, nebf, npbf, ng2, dae, dbe, dp, gm2, fe, e2)
+  write(11) fe, e2
+  fp = 0.0d0
+  call RXCHFmult_FP_OMG2(one, nebf, nebf, npbf, ng2, dae, dbe, dp, gm2, fp, e2)
+  write(11) fp, e2
+  fe = 0.0d0
+  call RXCHFmult_FBE_OMG2(one, nebf, nebf, npbf, ng2, dae, dbe, dp, gm2, fe, e2)
+  write(11) fe, e2
+  fe = 0.0d0
+  call RXCHFmult_FAE_OMG3(one, nebf, nebf, npbf, ng3, dae, dbe, dp, gm3, fe, e3)
+  write(11) fe, e3
+  fp = 0.0d0
+  call RXCHFmult_FP_OMG3(one, nebf, nebf, npbf, ng3, dae, dbe, dp, gm3, fp, e3)
+  write(11) fp, e3
+  fe = 0.0d0
+  call RXCHFmult_FBE_OMG3(one, nebf, nebf, npbf, ng3, dae, dbe, dp, gm3, fe, e3)
+  write(11) fe, e3
+  fe = 0.0d0
+  call RXCHFmult_FAE_OMG4(one, nebf, nebf, npbf, ng4, dae, dbe, dp, gm4, fe, e4)
+  write(11) fe, e4
+  fp = 0.0d0
+  call RXCHFmult_FP_OMG4(one, nebf, nebf, npbf, ng4, dae, dbe, dp, gm4, fp, e4)
+  write(11) fp, e4
+  fe = 0.0d0
+  call RXCHFmult_FBE_OMG4(one, nebf, nebf, npbf, ng4, dae, dbe, dp, gm4, fe, e4)
+  write(11) fe, e4
   close(11)
 end program

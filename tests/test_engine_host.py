@@ -90,3 +90,19 @@ def test_engine_rxchfne_tuples_vs_oracle():
         ee, es = engine_prim_list(s, 4, idx, tier)
         check_close(ee, oe, what=f"RXCHF-ne ansE1 tier {tier}")
         check_close(es, os_, what=f"RXCHF-ne ansE2 tier {tier}")
+
+
+def test_engine_rxchfmult_tuples_vs_oracle():
+    """RXCHFmult_GAM2_MD (three outputs) and RXCHFmult_INT_GAM3_MD in the host
+    build of the engine against the oracle, both capacity tiers."""
+    from _util import engine_prim_list3, mult_tuples
+
+    s = synth.mult(seed=54, e_shells=(0, 1, 2), be_shells=(0, 1, 2), p_shells=(0, 1, 2), be_centres=2)
+    cs = synth.combined(s)
+    for cls, n in ((5, 120), (6, 40)):
+        idx = mult_tuples(s, cls, n, seed=10 + cls)
+        ref = oracle.prim_list3(cs, cls, idx)
+        for tier in (0, 1):
+            got = engine_prim_list3(cs, cls, idx, tier)
+            for k, what in enumerate(("ansE1", "ansS", "ansE2") if cls == 5 else ("ans",)):
+                check_close(got[k], ref[k], what=f"cls{cls} {what} tier {tier}")

@@ -82,7 +82,12 @@ def test_fortran_dropin_matches_oracle():
              ne * ne, 1, npb * npb, 1,  # FAE_OMG4, UFP_OMG4
              ng2, ng2,  # RXCHFne_GAM2_IC1
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFne_FAE/FP/FBE_OMG2
-             ne * ne, ne * ne, 1]  # RXCUHFne_FE_OMG2
+             ne * ne, ne * ne, 1,  # RXCUHFne_FE_OMG2
+             ng2, ng2, ng2,  # RXCHFmult_GAM2: INT_GAM2, XCHF_GAM2, XCHF_GAM2s
+             ng3, ng4,  # RXCHFmult_GAM3_INT, RXCHFmult_GAM4_INT
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG2
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # ... _OMG3
+             ne * ne, 1, npb * npb, 1, ne * ne, 1]  # ... _OMG4
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -162,3 +167,25 @@ def test_fortran_dropin_matches_oracle():
     check_close(parts[53], Fa.ravel(order="F"), what="RXCUHFne_FE_OMG2 alpha")
     check_close(parts[54], Fb.ravel(order="F"), what="RXCUHFne_FE_OMG2 beta")
     assert parts[55][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+
+    # RXCHFmult pieces, special-electron basis = regular basis
+    m = dict(s)
+    m["nebfBE"], m["npebfBE"] = s["nebf"], s["npebf"]
+    for k in synth.BE_KEYS:
+        m[k + "_be"] = s[k]
+    mi, mx, mxs = oracle.rxchfmult_gam2(m)
+    check_close(parts[56], mi, what="RXCHFmult_GAM2 INT_GAM2")
+    check_close(parts[57], mx, what="RXCHFmult_GAM2 XCHF_GAM2")
+    check_close(parts[58], mxs, what="RXCHFmult_GAM2 XCHF_GAM2s")
+    m3 = oracle.rxchfmult_gam3_int(m)
+    check_close(parts[59], m3, what="RXCHFmult_GAM3_INT")
+    m4 = oracle.rxchfmult_gam4_int(ne, ne, npb, mxs, oee)
+    check_close(parts[60], m4, what="RXCHFmult_GAM4_INT")
+    for order, G in ((2, mi), (3, m3), (4, m4)):
+        for which in range(3):
+            n = npb if which == 1 else ne
+            F = np.zeros((n, n), order="F")
+            e = oracle.rxchfmult_omg(order, which, ne, ne, npb, DAE, DBE, DP, G, F)
+            at = 61 + 6 * (order - 2) + 2 * which
+            check_close(parts[at], F.ravel(order="F"), what=f"RXCHFmult omg{order} {which}")
+            assert parts[at + 1][0] == pytest.approx(e, rel=1e-11, abs=1e-14)

@@ -384,3 +384,62 @@ def test_rxchfne(torch_dev):
     assert neorx.RXCHFne_FAE_GAMee(ne, gee, DA, A) == pytest.approx(oracle.rxchfne_fae_gamee(ne, gee, DA, B),
                                                                    rel=1e-11)
     check_close(A, B, what="RXCHFne_FAE_GAMee")
+
+
+def _run_list3(torch, S, cls, idx):
+    n = len(idx)
+    d_idx = torch.from_numpy(np.ascontiguousarray(idx)).cuda()
+    out = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    S.prim_list3_dev(cls, n, d_idx.data_ptr(), *[o.data_ptr() for o in out],
+                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in out]
+
+
+def test_rxchfmult(torch_dev):
+    """RXCHFmult: class-5/6 tuples of a two-basis handle, RXCHFmult_GAM2,
+    _GAM3_INT, _GAM4_INT and the FAE/FP/FBE_OMG{2,3,4} contractions through
+    the C ABI against the oracle."""
+    from _util import mult_tuples
+
+    torch = torch_dev
+    s = synth.mult(seed=51, e_shells=(0, 1, 2), be_shells=(0, 1, 2), p_shells=(0, 1, 2), be_centres=2)
+    cs = synth.combined(s)
+    S = neorx.System(s)
+    for cls, n in ((5, 512), (6, 256)):
+        print(f"case rxchfmult prim_list3 cls={cls} n={n}", flush=True)
+        idx = mult_tuples(s, cls, n, seed=cls)
+        got = _run_list3(torch, S, cls, idx)
+        ref = oracle.prim_list3(cs, cls, idx)
+        for k, what in enumerate(("ansE1", "ansS", "ansE2") if cls == 5 else ("ans",)):
+            check_close(got[k], ref[k], what=f"cls{cls} {what}")
+    S.close()
+    t = synth.mult(seed=52, e_shells=(0, 1), be_shells=(0, 1), p_shells=(0, 0), n_e_centres=1, be_centres=1,
+                   contract=(1, 2), ngtg=2)
+    na, nb, npb = t["nebf"], t["nebfBE"], t["npbf"]
+    print(f"case rxchfmult builders na={na} nb={nb} npbf={npb}", flush=True)
+    T = neorx.System(t)
+    got = T.RXCHFmult_GAM2()
+    ref = oracle.rxchfmult_gam2(t)
+    for g, r, what in zip(got, ref, ("INT_GAM2", "XCHF_GAM2", "XCHF_GAM2s")):
+        check_close(g, r, what=what)
+    g3 = T.RXCHFmult_GAM3_INT()
+    r3 = oracle.rxchfmult_gam3_int(t)
+    check_close(g3, r3, what="INT_GAM3")
+    T.close()
+    gee = oracle.gam_ee(t)
+    g4 = neorx.RXCHFmult_GAM4_INT(na, nb, npb, ref[2], gee)
+    r4 = oracle.rxchfmult_gam4_int(na, nb, npb, ref[2], gee)
+    check_close(g4, r4, what="INT_GAM4")
+    rng = np.random.default_rng(14)
+    DA, DB, DP = synth.density(na, 2.0, 1), synth.density(nb, 1.0, 2), synth.density(npb, 1.0, 3)
+    for order, G in ((2, ref[0]), (3, r3), (4, r4)):
+        for which in (0, 1, 2):
+            print(f"case rxchfmult omg{order} which={which}", flush=True)
+            n = (na, npb, nb)[which]
+            F0 = np.asfortranarray(rng.standard_normal((n, n)))
+            A, B = F0.copy(order="F"), F0.copy(order="F")
+            eg = neorx.RXCHFmult_OMG(which, order, na, nb, npb, DA, DB, DP, G, A)
+            eo = oracle.rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, B)
+            check_close(A - F0, B - F0, what=f"RXCHFmult omg{order} {which}")
+            assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)

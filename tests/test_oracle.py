@@ -322,3 +322,46 @@ def test_oracle_rxchfne_vs_reference():
     assert oracle.rxchfne_fae_gamee(ne, gee, DA, A) == pytest.approx(oracle.ref_rxchfne_fae_gamee(ne, gee, DA, B),
                                                                     rel=1e-12)
     check_close(A, B, what="RXCHFne_FAE_GAMee")
+
+
+@needs_ref
+def test_oracle_rxchfmult_vs_reference(tmp_path):
+    """RXCHFmult: the two-basis e-e-p (three outputs) and e-e-e-p tuples,
+    RXCHFmult_GAM2, _GAM3_INT, _GAM4_INT and the FAE/FP/FBE_OMG{2,3,4}
+    contractions of the oracle against the reference's own routines
+    (source/rxchfmult/)."""
+    from _util import mult_tuples
+
+    s = synth.mult(seed=50, e_shells=(0, 1, 2), be_shells=(0, 1, 2), p_shells=(0, 1, 2), be_centres=2)
+    cs = synth.combined(s)
+    for cls, n in ((5, 12), (6, 4)):
+        idx = mult_tuples(s, cls, n, seed=cls)
+        ref = oracle.ref_prim_list3(cs, cls, idx)
+        got = oracle.prim_list3(cs, cls, idx)
+        for k, what in enumerate(("ansE1", "ansS", "ansE2") if cls == 5 else ("ans",)):
+            check_close(got[k], ref[k], what=f"cls{cls} {what}")
+    t = synth.mult(seed=53, e_shells=(0, 1), be_shells=(0, 1), p_shells=(0, 0), n_e_centres=1, be_centres=1,
+                   contract=(1,), ngtg=2)
+    na, nb, npb = t["nebf"], t["nebfBE"], t["npbf"]
+    assert nb <= na
+    got = oracle.rxchfmult_gam2(t)
+    ref = oracle.ref_rxchfmult_gam2(t)
+    for g, r, what in zip(got, ref, ("INT_GAM2", "XCHF_GAM2", "XCHF_GAM2s")):
+        check_close(g, r, what=what)
+    g3 = oracle.rxchfmult_gam3_int(t)
+    check_close(g3, oracle.ref_rxchfmult_gam3_int(t), what="INT_GAM3")
+    rng = np.random.default_rng(8)
+    gee = oracle.gam_ee(t)
+    g4 = oracle.rxchfmult_gam4_int(na, nb, npb, got[2], gee)
+    check_close(g4, oracle.ref_rxchfmult_gam4_int(na, nb, npb, got[2], gee, str(tmp_path)), what="INT_GAM4")
+    assert np.max(np.abs(g4)) > 0.0
+    DA, DB, DP = synth.density(na, 2.0, 1), synth.density(nb, 1.0, 2), synth.density(npb, 1.0, 3)
+    for order, G in ((2, got[0]), (3, g3), (4, g4)):
+        for which in (0, 1, 2):
+            n = (na, npb, nb)[which]
+            F0 = np.asfortranarray(rng.standard_normal((n, n)))
+            A, B = F0.copy(order="F"), F0.copy(order="F")
+            eo = oracle.rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, A)
+            er = oracle.ref_rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, B)
+            assert eo == pytest.approx(er, rel=1e-12, abs=1e-14)
+            check_close(A - F0, B - F0, what=f"RXCHFmult omg{order} {which}")
