@@ -260,8 +260,13 @@ int neorx_mult_system_create(int64_t nebf, int64_t npebf, int64_t nebfbe, int64_
  * regular primitives followed by the special ones (0-based).
  *   cls 5 (e-e-p)   : RXCHFmult_GAM2_MD, rxchfmult/rxchfmult_gam2.f:627 -> ansE1, ansS, ansE2
  *   cls 6 (e-e-e-p) : RXCHFmult_INT_GAM3_MD, rxchfmult/rxchfmult_gam3-int.f:661 -> ans
- * Index rows as for cls 2 and cls 3.  neorx_prim_list_dev also accepts them
- * (ansE2 dropped); d_ansE2 receives zero for the other classes. */
+ * and of any handle:
+ *   cls 7 (e-e-e-p) : RXCHFmult_XCHF_GAM3_MD, rxchfmult/rxchfmult_gam3-xchf.f:630 -> ans
+ *                     (electron labels as in xcalc_GAM3_MD; equal after the
+ *                     six-permutation average of RXCHFmult_GAM3_XCHF)
+ *   cls 8 (e-p)     : RXCHFmult_xcalc_GAM1_MD, rxchfmult/rxchfmult_gam1.f:399 -> ansE, ansS
+ * Index rows as for cls 2, 3, 3 and 1.  neorx_prim_list_dev also accepts
+ * them (ansE2 dropped); d_ansE2 receives zero for the classes other than 5. */
 int neorx_prim_list3_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
                          double *d_ansS, double *d_ansE2, void *stream);
 /* RXCHFmult_GAM2, rxchfmult/rxchfmult_gam2.f:2: INT_GAM2 (npbf^2 nebf^2
@@ -270,6 +275,18 @@ int neorx_prim_list3_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_
 int neorx_rxchfmult_gam2(int64_t handle, double *int_gam2, double *xchf_gam2, double *xchf_gam2s);
 int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, double *d_xchf_gam2, double *d_xchf_gam2s,
                              void *stream);
+/* RXCHFmult_GAM1_MD, rxchfmult/rxchfmult_gam1.f:2, over the basis of a
+ * one-basis handle (the driver passes the special-electron basis): GAM_1,
+ * GAM_1s as neorx_gam1 returns them (the reference writes GAM_1.ufm,
+ * GAM_1s.ufm). */
+int neorx_rxchfmult_gam1(int64_t handle, double *gam1, double *gam1s);
+/* RXCHFmult_GAM3_XCHF, rxchfmult/rxchfmult_gam3-xchf.f:2, over the basis of
+ * a one-basis handle: XCHF_GAM3 (nebf^6 npbf^2, index_GAM_3PK packing).
+ * RXCHFmult_GAM4_XCHF (rxchfmult/rxchfmult_gam4-xchf.f:2) is GAM4_ICR over
+ * the special basis with the leading nebfBE^4 block of GAM_ee: call
+ * neorx_gam4_icr(nebfBE, npbf, XCHF_GAM2s, GAM_eeBE, ...). */
+int neorx_rxchfmult_gam3_xchf(int64_t handle, double *xchf_gam3);
+int neorx_rxchfmult_gam3_xchf_dev(int64_t handle, double *d_xchf_gam3, void *stream);
 /* RXCHFmult_GAM3_INT, rxchfmult/rxchfmult_gam3-int.f:2: INT_GAM3
  * (npbf^2 nebf^2 nebfBE^4), averaged over the exchange of electrons 2 and 3 */
 int neorx_rxchfmult_gam3_int(int64_t handle, double *int_gam3);

@@ -797,7 +797,9 @@ NRX_HD int tuple_degree(const PairRec *const *q, int n, bool kinetic)
 
 // xcalc_GAM1_MD (gam_1_OMP.f:398-839).  e = electron, p = proton.
 // Terms that depend on gamma_k + gamma_l only are summed over k <= l.
-template <int NL>
+// With MULT set: RXCHFmult_xcalc_GAM1_MD (rxchfmult/rxchfmult_gam1.f:399-840),
+// the same value without the one-geminal terms.
+template <int NL, bool MULT = false>
 NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, double &ansE, double &ansS)
 {
     double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
@@ -805,19 +807,21 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
     double ogx = 0, ogg = 0;
     for (int k = 0; k < S.ngtg; ++k) {
         double gv = S.gamma[k], bv = S.bcoef[k];
-        Sys s1 = mksys(S, 2, e, p, 0, 0, gv, 0, 0);  // (e, p): hub = p
-        Sys s1r = mksys(S, 2, p, e, 0, 0, gv, 0, 0); // (p, e): hub = e
-        double gks[2] = {0.0, gv}, kk[2];
-        kin_multi<NL>(s1, 0, 1, gks, 2, 1.0, PI_G2, kk);
-        tegx += bv * kk[0];
-        texg += bv * kk[1];
-        kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
-        tpgx += bv * kk[0];
-        tpxg += bv * kk[1];
-        vcegx += bv * nuc_sum<NL>(s1, 0, S.nat, S.cat, S.zan, false);
-        vcpgx += bv * nuc_sum<NL>(s1, 1, S.nat, S.cat, S.zan, false);
-        vmgx += bv * coul<NL>(s1, 0, 1);
-        ogx += bv * ovl<NL>(s1);
+        if constexpr (!MULT) {
+            Sys s1 = mksys(S, 2, e, p, 0, 0, gv, 0, 0);  // (e, p): hub = p
+            Sys s1r = mksys(S, 2, p, e, 0, 0, gv, 0, 0); // (p, e): hub = e
+            double gks[2] = {0.0, gv}, kk[2];
+            kin_multi<NL>(s1, 0, 1, gks, 2, 1.0, PI_G2, kk);
+            tegx += bv * kk[0];
+            texg += bv * kk[1];
+            kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
+            tpgx += bv * kk[0];
+            tpxg += bv * kk[1];
+            vcegx += bv * nuc_sum<NL>(s1, 0, S.nat, S.cat, S.zan, false);
+            vcpgx += bv * nuc_sum<NL>(s1, 1, S.nat, S.cat, S.zan, false);
+            vmgx += bv * coul<NL>(s1, 0, 1);
+            ogx += bv * ovl<NL>(s1);
+        }
         for (int l = 0; l < S.ngtg; ++l) {
             double gw = S.gamma[l], bb = bv * S.bcoef[l];
             Sys s2 = mksys(S, 2, e, p, 0, 0, gv + gw, 0, 0);
@@ -915,8 +919,12 @@ NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
     gam2_core<NL, false>(S, e1, e2, p, ansE, ansS, unused);
 }
 
-// xcalc_GAM3_MD (gam_3_OMP.f:617-983).
-template <int NL>
+// xcalc_GAM3_MD (gam_3_OMP.f:617-983).  With MULT set:
+// RXCHFmult_XCHF_GAM3_MD (rxchfmult/rxchfmult_gam3-xchf.f:630-1023), which
+// drops the one-geminal gVEE term and relabels the electrons of the other
+// terms; the relabelling is undone by the six-permutation average its
+// builder applies, so the terms are evaluated here with the labels above.
+template <int NL, bool MULT = false>
 NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3, const PairRec *p,
                        double &ans)
 {
@@ -928,7 +936,7 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
     double hcore = kin<NL>(s3, 0, -1, 0.0, 1.0) - nuc_sum<NL>(s3, 0, S.nat, S.cat, S.zan, false);
     underflow(hcore);
     double gVEE = 0.0;
-    for (int ik = 0; ik < S.ngtg; ++ik) {
+    for (int ik = 0; ik < (MULT ? 0 : S.ngtg); ++ik) {
         Sys s = mksys(S, 2, e1, p, 0, 0, S.gamma[ik], 0, 0);
         double xx = ovl<NL>(s);
         underflow(xx);
@@ -955,7 +963,10 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
             underflow(xx);
             gVEEg1 += w * (xx * vee23);
         }
-    ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
+    if constexpr (MULT)
+        ans = gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
+    else
+        ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
 }
 
 // RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): the RXCHF-ne
@@ -1052,10 +1063,17 @@ template <> struct Tier<6> {
     static constexpr int SMALL = 9, GEN = 17;
     static constexpr bool KIN = false;
 };
+// class 7: the RXCHFmult XCHF e-e-e-p tuple (as class 3 without gVEE);
+// class 8: the RXCHFmult e-p tuple (as class 1 without the one-geminal terms)
+template <> struct Tier<7> : Tier<3> {};
+template <> struct Tier<8> : Tier<1> {};
 constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
 
 // electrons per tuple of a class
-template <int CLS> constexpr int nel() { return (CLS == 4 || CLS == 5) ? 2 : CLS == 6 ? 3 : CLS; }
+template <int CLS> constexpr int nel()
+{
+    return (CLS == 4 || CLS == 5) ? 2 : (CLS == 6 || CLS == 7) ? 3 : CLS == 8 ? 1 : CLS;
+}
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
 template <int CLS, int NL>
@@ -1064,6 +1082,10 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
     s = 0.0;
     if constexpr (CLS == 1)
         gam1_tuple<NL>(S, q[0], q[1], e, s);
+    else if constexpr (CLS == 8)
+        gam1_tuple<NL, true>(S, q[0], q[1], e, s);
+    else if constexpr (CLS == 7)
+        gam3_tuple<NL, true>(S, q[0], q[1], q[2], q[3], e);
     else if constexpr (CLS == 2)
         gam2_tuple<NL>(S, q[0], q[1], q[2], e, s);
     else if constexpr (CLS == 4)

@@ -340,19 +340,22 @@ template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t
     return ok;
 }
 
+// the RXCHFmult classes whose electrons come from two bases
+template <int CLS> constexpr bool two_basis() { return CLS == 5 || CLS == 6; }
+
 // tuple r of the enumeration (ip,jp,ie1,je1,...,ieN,jeN), last index
 // fastest, in the system's sorted primitive order (SORTED) or in the
-// caller's original order.  The RXCHFmult classes (5, 6) enumerate e1 over
+// caller's original order.  The two-basis classes (5, 6) enumerate e1 over
 // the first basis and the other electrons over the special-electron basis,
 // in original order only.
 template <int CLS, bool SORTED> NRX_DI void range_tuple(const DevSys &S, i64 r, i64 *x)
 {
-    static_assert(CLS < 5 || !SORTED, "two-basis classes are enumerated in original order");
+    static_assert(!two_basis<CLS>() || !SORTED, "two-basis classes are enumerated in original order");
     i64 np = S.npbf;
     constexpr int CLS_E = nel<CLS>();
     for (int i = CLS_E - 1; i >= 0; --i) {
         i64 ne = S.npebf, off = 0;
-        if constexpr (CLS >= 5) {
+        if constexpr (two_basis<CLS>()) {
             ne = i == 0 ? S.npebf1 : S.npebf - S.npebf1;
             off = i == 0 ? 0 : S.npebf1;
         }
@@ -473,6 +476,9 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
                         double *ox = nullptr)
 {
     constexpr bool tiered = Tier<CLS>::SMALL != Tier<CLS>::GEN;
+    if constexpr (two_basis<CLS>())
+        if (SRC != SRC_LIST && (d.npebf1 <= 0 || d.npebf1 >= d.npebf))
+            return fail(NEORX_E_ARG, "class %d enumerates two electron bases; this handle has one", CLS);
     StreamBuf ws;
     i64 *ovf = nullptr;
     unsigned *cnt = nullptr;
@@ -508,7 +514,7 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 6) return fail(NEORX_E_ARG, "class %lld not in 1..6", (long long)cls);
+    if (cls < 1 || cls > 8) return fail(NEORX_E_ARG, "class %lld not in 1..8", (long long)cls);
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -518,7 +524,9 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
     case 3: return launch_prims<3, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 4: return launch_prims<4, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 5: return launch_prims<5, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
-    default: return launch_prims<6, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 6: return launch_prims<6, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 7: return launch_prims<7, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    default: return launch_prims<8, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     }
 }
 
@@ -560,6 +568,8 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
 // ------------------------------------------------------------------
 // GAM1_OMP_MD: one lane per contracted (ip,jp,iec,jec), primitive sums in
 // the reference's loop order.  Every e-p tuple fits Tier<1>::GEN.
+// CLS 8: RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2-398), same loops.
+template <int CLS>
 __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, double *__restrict__ Gs)
 {
     i64 ne = S.nebf, np = S.npbf;
@@ -571,7 +581,7 @@ __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, 
         for (i64 je = S.kpestr[jc] - 1; je < S.kpeend[jc]; ++je) {
             i64 x[4] = {ie, ip, je, jp};
             double e, s;
-            eval_tuple<1, Tier<1>::GEN>(S, x, e, s);
+            eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s);
             double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[ie] * S.agebfcc[je];
             acc += e * cf;
             accs += s * cf;
@@ -803,7 +813,7 @@ static const i64 BLOCK_CAP = (i64)1 << 27;
 // nebf1 functions) and the other electrons over the special basis (pb).
 static i64 slab_start(const System *s, int cls, i64 L)
 {
-    bool two = cls >= 5;
+    bool two = cls == 5 || cls == 6;
     i64 pa = two ? s->d.npebf1 : s->d.npebf, pb = two ? s->d.npebf - pa : pa, ne = two ? s->d.nebf1 : s->d.nebf;
     if (cls == 2 || cls == 5) return L * pa * pa * pb * pb;
     i64 pp = L / ne, i1 = L % ne;
@@ -855,12 +865,13 @@ static int build2_block(System *s, i64 p0, i64 p1, double *X, double *Xs, double
 
 // Unsymmetrised contracted e-e-e-p values of (ip,jp) pairs [p0, p1) into X
 // (block-local, ne^6 per pair).
+template <int CLS = 3>
 static int build3_block(System *s, i64 p0, i64 p1, double *X, double *Y, hipStream_t st)
 {
     i64 ne = s->d.nebf, ne5 = ne * ne * ne * ne * ne, cap;
     for (auto &sl : make_slabs(s, 3, p0 * ne, p1 * ne, cap)) {
         i64 R0 = slab_start(s, 3, sl.first), R1 = slab_start(s, 3, sl.second);
-        int rc = launch_prims<3, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, nullptr, st);
+        int rc = launch_prims<CLS, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y, nullptr, st);
         if (rc) return rc;
         i64 n = (sl.second - sl.first) * ne5;
         hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y, X, p0 * ne);
@@ -915,7 +926,7 @@ static int copy_out(double *h, const double *d, i64 n)
     return NEORX_OK;
 }
 
-extern "C" int neorx_gam1(int64_t handle, double *gam1, double *gam1s)
+template <int CLS> static int gam1_host(int64_t handle, double *gam1, double *gam1s)
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
@@ -924,13 +935,15 @@ extern "C" int neorx_gam1(int64_t handle, double *gam1, double *gam1s)
     int rc = G.alloc(n * 8);
     if (!rc) rc = Gs.alloc(n * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gam1, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d(), Gs.d());
+    hipLaunchKernelGGL(k_gam1<CLS>, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d(), Gs.d());
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     rc = copy_out(gam1, G.d(), n);
     if (!rc) rc = copy_out(gam1s, Gs.d(), n);
     return rc;
 }
+
+extern "C" int neorx_gam1(int64_t handle, double *gam1, double *gam1s) { return gam1_host<1>(handle, gam1, gam1s); }
 
 extern "C" int neorx_gam2_ic1(int64_t handle, double *gm2icr, double *gm2sicr)
 {
@@ -2639,6 +2652,48 @@ extern "C" int neorx_rxchfmult_gam3_int(int64_t handle, double *int_gam3)
     rc = A.alloc(n3 * 8);
     if (!rc) rc = neorx_rxchfmult_gam3_int_dev(handle, A.d(), 0);
     if (!rc) rc = copy_out(int_gam3, A.d(), n3);
+    return rc;
+}
+
+// RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2-398) over the basis of a
+// one-basis handle (the driver passes the special-electron basis,
+// rxchfmult_driver.f:324-332)
+extern "C" int neorx_rxchfmult_gam1(int64_t handle, double *gam1, double *gam1s)
+{
+    return gam1_host<8>(handle, gam1, gam1s);
+}
+
+// RXCHFmult_GAM3_XCHF (rxchfmult/rxchfmult_gam3-xchf.f:2-268) over the basis
+// of a one-basis handle: contract_omega3_XCHF, then the six-permutation average
+extern "C" int neorx_rxchfmult_gam3_xchf_dev(int64_t handle, double *d_gm3, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (s->d.nebf1 != s->d.nebf) return fail(NEORX_E_ARG, "RXCHFmult_GAM3_XCHF takes a one-basis handle");
+    if (!d_gm3) return fail(NEORX_E_ARG, "null output");
+    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap = slab_cap(s, 3);
+    DevBuf X, Y;
+    int rc = X.alloc(ng3 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = build3_block<7>(s, 0, np * np, X.d(), Y.d(), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, ng3, X.d(), d_gm3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam3_xchf(int64_t handle, double *gm3)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    DevBuf A;
+    int rc = A.alloc(ng3 * 8);
+    if (!rc) rc = neorx_rxchfmult_gam3_xchf_dev(handle, A.d(), 0);
+    if (!rc) rc = copy_out(gm3, A.d(), ng3);
     return rc;
 }
 

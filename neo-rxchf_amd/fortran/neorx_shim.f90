@@ -215,6 +215,18 @@ module neorx_capi
       real(c_double) :: gi(*), gx(*), gs(*)
       integer(c_int) :: neorx_rxchfmult_gam2
     end function
+    function neorx_rxchfmult_gam1(h, g, gs) bind(C, name="neorx_rxchfmult_gam1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g(*), gs(*)
+      integer(c_int) :: neorx_rxchfmult_gam1
+    end function
+    function neorx_rxchfmult_gam3_xchf(h, g) bind(C, name="neorx_rxchfmult_gam3_xchf")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g(*)
+      integer(c_int) :: neorx_rxchfmult_gam3_xchf
+    end function
     function neorx_rxchfmult_gam3_int(h, g) bind(C, name="neorx_rxchfmult_gam3_int")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: h
@@ -717,6 +729,82 @@ subroutine RXCHFmult_GAM2(Nchunks, nebf, npebf, nebfBE, npebfBE, npbf, dimINT2, 
                    'RXCHFmult_GAM2')
   call neorx_check(neorx_rxchfmult_gam2(h, INT_GAM2, XCHF_GAM2, XCHF_GAM2s), 'RXCHFmult_GAM2')
   call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM2')
+end subroutine
+
+! RXCHFmult_GAM1_MD, source/rxchfmult/rxchfmult_gam1.f:2-398: as GAM1_OMP_MD
+! (GAM_1.ufm / GAM_1s.ufm), without the one-geminal terms
+subroutine RXCHFmult_GAM1_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                             AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf)
+  real(c_double), allocatable :: g(:), gs(:)
+  call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
+  allocate(g(ng1), gs(ng1))
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, kpestr, &
+                   kpeend, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), &
+                   'RXCHFmult_GAM1_MD')
+  call neorx_check(neorx_rxchfmult_gam1(h, g, gs), 'RXCHFmult_GAM1_MD')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM1_MD')
+  open(801, file='GAM_1.ufm', form='unformatted', status='unknown', access='direct', recl=8)
+  open(802, file='GAM_1s.ufm', form='unformatted', status='unknown', access='direct', recl=8)
+  do ia = 1, ng1
+    write(801, rec=ia) g(ia)
+    write(802, rec=ia) gs(ia)
+  end do
+  close(801)
+  close(802)
+  deallocate(g, gs)
+end subroutine
+
+! RXCHFmult_GAM3_XCHF, source/rxchfmult/rxchfmult_gam3-xchf.f:2-268 (the
+! driver passes the special-electron basis, rxchfmult_driver.f:548)
+subroutine RXCHFmult_GAM3_XCHF(Nchunks, nebf, npebf, npbf, ng3, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                               KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, AGNBFCC, NUCEX, NUCAM, &
+                               NUCBFC, XCHF_GAM3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng3, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: XCHF_GAM3(ng3)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), &
+                   'RXCHFmult_GAM3_XCHF')
+  call neorx_check(neorx_rxchfmult_gam3_xchf(h, XCHF_GAM3), 'RXCHFmult_GAM3_XCHF')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM3_XCHF')
+end subroutine
+
+! RXCHFmult_GAM4_XCHF, source/rxchfmult/rxchfmult_gam4-xchf.f:2-157: GAM4_ICR
+! over the special basis with the leading nebfBE^4 block of GAM_ee.ufm
+subroutine RXCHFmult_GAM4_XCHF(Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4, GAM_2s, GAM_4)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4
+  real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
+  real(c_double), allocatable :: gee(:,:,:,:), geebe(:,:,:,:)
+  integer(c_int64_t) :: i1, j1, i2, j2
+  allocate(gee(nebf, nebf, nebf, nebf), geebe(nebfBE, nebfBE, nebfBE, nebfBE))
+  open(812, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
+  do i1 = 1, nebf
+    do j1 = 1, nebf
+      do i2 = 1, nebf
+        do j2 = 1, nebf
+          read(812, rec=j2 + nebf * (i2 - 1 + nebf * (j1 - 1 + nebf * (i1 - 1)))) gee(j2, i2, j1, i1)
+        end do
+      end do
+    end do
+  end do
+  close(812)
+  geebe = gee(1:nebfBE, 1:nebfBE, 1:nebfBE, 1:nebfBE)
+  call neorx_check(neorx_gam4_icr(nebfBE, npbf, GAM_2s, geebe, GAM_4), 'RXCHFmult_GAM4_XCHF')
+  deallocate(gee, geebe)
 end subroutine
 
 ! RXCHFmult_GAM3_INT, source/rxchfmult/rxchfmult_gam3-int.f:2-254

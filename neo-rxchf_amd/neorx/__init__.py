@@ -96,6 +96,9 @@ def load():
     _sig("neorx_rxchfmult_gam2", [_I64, _P, _P, _P])
     _sig("neorx_rxchfmult_gam2_dev", [_I64, _P, _P, _P, _P])
     _sig("neorx_rxchfmult_gam3_int", [_I64, _P])
+    _sig("neorx_rxchfmult_gam1", [_I64, _P, _P])
+    _sig("neorx_rxchfmult_gam3_xchf", [_I64, _P])
+    _sig("neorx_rxchfmult_gam3_xchf_dev", [_I64, _P, _P])
     _sig("neorx_rxchfmult_gam3_int_dev", [_I64, _P, _P])
     _sig("neorx_rxchfmult_gam4_int", [_I64, _I64, _I64, _P, _P, _P])
     _sig("neorx_rxchfmult_gam4_int_dev", [_I64, _I64, _I64, _P, _P, _P, _P])
@@ -193,6 +196,23 @@ class System:
         gs = _np.zeros_like(gx)
         _check(_lib.neorx_rxchfmult_gam2(self.handle, _p(gi), _p(gx), _p(gs)))
         return gi, gx, gs
+
+    def RXCHFmult_GAM1_MD(self):
+        """GAM_1, GAM_1s of RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2)
+        over this handle's basis (one-basis handle)."""
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne * ne * npb * npb)
+        gs = _np.zeros_like(g)
+        _check(_lib.neorx_rxchfmult_gam1(self.handle, _p(g), _p(gs)))
+        return g, gs
+
+    def RXCHFmult_GAM3_XCHF(self):
+        """XCHF_GAM3 (rxchfmult/rxchfmult_gam3-xchf.f:2) over this handle's
+        basis (one-basis handle)."""
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g = _np.zeros(ne ** 6 * npb * npb)
+        _check(_lib.neorx_rxchfmult_gam3_xchf(self.handle, _p(g)))
+        return g
 
     def RXCHFmult_GAM3_INT(self):
         """INT_GAM3 (rxchfmult/rxchfmult_gam3-int.f:2)."""
@@ -435,6 +455,13 @@ def RXCHFmult_GAM4_INT(nebf, nebfBE, npbf, gam_2s, gam_ee):
     _check(_lib.neorx_rxchfmult_gam4_int(nebf, nebfBE, npbf, _p(_np.ascontiguousarray(gam_2s)),
                                          _p(_np.ascontiguousarray(gam_ee)), _p(g)))
     return g
+
+
+def RXCHFmult_GAM4_XCHF(nebf, nebfBE, npbf, gam_2s, gam_ee):
+    """XCHF_GAM4 (rxchfmult/rxchfmult_gam4-xchf.f:2): GAM4_ICR over the special
+    basis with the leading nebfBE^4 block of GAM_ee (regular basis)."""
+    ee = _np.ascontiguousarray(_np.asarray(gam_ee).reshape((nebf,) * 4)[:nebfBE, :nebfBE, :nebfBE, :nebfBE])
+    return GAM4_ICR(nebfBE, npbf, gam_2s, ee.ravel())
 
 
 def RXCHFmult_OMG(which, k, nebf, nebfBE, npbf, DAE, DBE, DP, GM, F):

@@ -613,28 +613,33 @@ typedef struct {
     const double *cat, *zan, *bcoef, *gamma;
 } sys_t;
 
-/* xcalc_GAM1_MD (gam_1_OMP.f:398-839): electron e1 = particle 1, proton = 2 */
-static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const prim_t *e1k, const prim_t *pk,
-                      double *ansE, double *ansS)
+/* xcalc_GAM1_MD (gam_1_OMP.f:398-839): electron e1 = particle 1, proton = 2.
+ * With mult set: RXCHFmult_xcalc_GAM1_MD (rxchfmult/rxchfmult_gam1.f:399-840),
+ * which has no one-geminal terms (tegx, tpgx, vcegx, vcpgx, vmgx, texg, tpxg,
+ * ogx stay zero). */
+static void gam1_prim_impl(const sys_t *S, int mult, const prim_t *e1b, const prim_t *pb, const prim_t *e1k,
+                           const prim_t *pk, double *ansE, double *ansS)
 {
     double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
     double vcegx = 0, vcegg = 0, vcpgx = 0, vcpgg = 0, vmgx = 0, vmgg = 0;
     for (i64 k = 0; k < S->ngtg; ++k) {
         double gv = S->gamma[k], bv = S->bcoef[k];
-        tegx += bv * pgike1(e1b, pb, e1k, pk, gv, 0.0, 1.0);
-        tpgx += bv * pgike1(pb, e1b, pk, e1k, gv, 0.0, S->pmass);
-        double vce = 0, vcp = 0;
-        for (i64 c = 0; c < S->nat; ++c) {
-            const double *C = S->cat + 3 * c;
-            vce += g2(OP_NUC, e1b, pb, e1k, pk, gv, 0.0, C, 1.0) * S->zan[c];
-            vcp += g2(OP_NUC, pb, e1b, pk, e1k, gv, 0.0, C, 1.0) * S->zan[c];
+        if (!mult) {
+            tegx += bv * pgike1(e1b, pb, e1k, pk, gv, 0.0, 1.0);
+            tpgx += bv * pgike1(pb, e1b, pk, e1k, gv, 0.0, S->pmass);
+            double vce = 0, vcp = 0;
+            for (i64 c = 0; c < S->nat; ++c) {
+                const double *C = S->cat + 3 * c;
+                vce += g2(OP_NUC, e1b, pb, e1k, pk, gv, 0.0, C, 1.0) * S->zan[c];
+                vcp += g2(OP_NUC, pb, e1b, pk, e1k, gv, 0.0, C, 1.0) * S->zan[c];
+            }
+            double vm = g2(OP_COUL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
+            vcegx += bv * vce;
+            vcpgx += bv * vcp;
+            vmgx += bv * vm;
+            texg += bv * pgike1(e1b, pb, e1k, pk, 0.0, gv, 1.0);
+            tpxg += bv * pgike1(pb, e1b, pk, e1k, 0.0, gv, S->pmass);
         }
-        double vm = g2(OP_COUL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
-        vcegx += bv * vce;
-        vcpgx += bv * vcp;
-        vmgx += bv * vm;
-        texg += bv * pgike1(e1b, pb, e1k, pk, 0.0, gv, 1.0);
-        tpxg += bv * pgike1(pb, e1b, pk, e1k, 0.0, gv, S->pmass);
         for (i64 l = 0; l < S->ngtg; ++l) {
             double gw = S->gamma[l], bw = S->bcoef[l];
             tegg += bv * bw * pgike1(e1b, pb, e1k, pk, gv, gw, 1.0);
@@ -657,11 +662,17 @@ static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const
     double ogx = 0, ogg = 0;
     for (i64 k = 0; k < S->ngtg; ++k) {
         double gv = S->gamma[k], bv = S->bcoef[k];
-        ogx += bv * g2(OP_OVL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
+        if (!mult) ogx += bv * g2(OP_OVL, e1b, pb, e1k, pk, gv, 0.0, NULL, 1.0);
         for (i64 l = 0; l < S->ngtg; ++l)
             ogg += bv * S->bcoef[l] * g2(OP_OVL, e1b, pb, e1k, pk, gv, S->gamma[l], NULL, 1.0);
     }
     *ansS = 0.0 + 2.0 * ogx + ogg;
+}
+
+static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const prim_t *e1k, const prim_t *pk,
+                      double *ansE, double *ansS)
+{
+    gam1_prim_impl(S, 0, e1b, pb, e1k, pk, ansE, ansS);
 }
 
 /* xcalc_GAM2_MD (gam_2_OMP.f:546-1077): particles e1, e2, p.
@@ -851,6 +862,42 @@ static void gam3_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *
     *ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
 }
 
+/* RXCHFmult_XCHF_GAM3_MD (rxchfmult/rxchfmult_gam3-xchf.f:630-1023): no
+ * one-geminal gVEE term; the Coulomb terms of the (k,l) loop with the
+ * electrons relabelled: G4_MD_xgVepg(e2, e3, e1, p) (:870), G2_MD_xggs(e2, p)
+ * times gfvee(e1, e3) (:910, :923), G4_MD_xgVeeg(e2, e1, e3, p) (:967). */
+static void gam3xchf_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
+{
+    double gHEg = 0, gVEEg1 = 0, gVEEg2 = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            double xke = g1(OP_KIN, &b[2], &k[2], NULL, 1.0);
+            double vc = 0.0;
+            for (i64 c = 0; c < S->nat; ++c) vc += S->zan[c] * g1(OP_NUC, &b[2], &k[2], S->cat + 3 * c, 1.0);
+            double xx = xke + vc * (-1.0);
+            double yy = g3(OP_OVL, &b[0], &b[1], &b[3], &k[0], &k[1], &k[3], 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            prim_t bv[4] = {b[1], b[2], b[0], b[3]}, kv[4] = {k[1], k[2], k[0], k[3]};
+            double zz = g4(1, bv, kv, gk, gl);
+            underflow(&xx);
+            underflow(&yy);
+            underflow(&zz);
+            double xgHEg = (xx * yy) + (zz * (-1.0));
+            xx = g2(OP_OVL, &b[1], &b[3], &k[1], &k[3], gk, gl, NULL, 1.0);
+            yy = gfvee(&b[0], &b[2], &k[0], &k[2]);
+            underflow(&xx);
+            underflow(&yy);
+            double xgVEEg1 = xx * yy;
+            prim_t bw[4] = {b[1], b[0], b[2], b[3]}, kw[4] = {k[1], k[0], k[2], k[3]};
+            double xgVEEg2 = g4(0, bw, kw, gk, gl);
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            gHEg += bb * xgHEg;
+            gVEEg1 += bb * xgVEEg1;
+            gVEEg2 += bb * xgVEEg2;
+        }
+    *ans = gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
+}
+
 /* ------------------------------------------------------------------ */
 /* Basis access (Fortran arrays of the reference)                      */
 /* ------------------------------------------------------------------ */
@@ -905,7 +952,7 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
 {
     /* cls 4: the RXCHF-ne e-e-p tuple; cls 5, 6: the RXCHFmult e-e-p and
      * e-e-e-p tuples (primitive indices over one combined list) */
-    int nel = (cls == 4 || cls == 5) ? 2 : cls == 6 ? 3 : (int)cls;
+    int nel = (cls == 4 || cls == 5) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls;
     int w = 2 * (nel + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
@@ -929,6 +976,10 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
             gam2_prim_impl(Sp, b, k, 1, &e, &s, &e2);
         else if (cls == 6)
             gam3int_prim(Sp, b, k, &e);
+        else if (cls == 7)
+            gam3xchf_prim(Sp, b, k, &e);
+        else if (cls == 8)
+            gam1_prim_impl(Sp, 1, &b[0], &b[1], &k[0], &k[1], &e, &s);
         else
             gam3_prim(Sp, b, k, &e);
         outE[t] = e;
@@ -993,10 +1044,13 @@ static inline i64 ix3(i64 ne, i64 np, i64 ip, i64 jp, i64 i1, i64 j1, i64 i2, i6
 
 /* GAM1_OMP_MD (gam_1_OMP.f:2-397): contracted e-p integrals, layout
  * pack_4D(nebf,nebf,npbf, jec1,iec1,jp,ip) */
-void orc_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
+static void gam1_impl(const basis_t *Bp, const sys_t *Sp, int mult, double *GAM1, double *GAM1s)
 {
-    BAS_INIT;
-    SYS_INIT;
+    basis_t B = *Bp;
+    sys_t S = *Sp;
+    i64 nebf = B.nebf, npebf = B.npebf, npbf = B.npbf;
+    const i64 *ampeb2c = B.ampeb2c;
+    const double *agebfcc = B.agebfcc, *agnbfcc = B.agnbfcc;
     i64 ng1 = nebf * nebf * npbf * npbf;
     memset(GAM1, 0, sizeof(double) * ng1);
     memset(GAM1s, 0, sizeof(double) * ng1);
@@ -1006,13 +1060,29 @@ void orc_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
                 for (i64 je = 0; je < npebf; ++je) {
                     prim_t eb = eprim(&B, ie), ek = eprim(&B, je), pb = pprim(&B, ip), pk = pprim(&B, jp);
                     double e, s;
-                    gam1_prim(&S, &eb, &pb, &ek, &pk, &e, &s);
+                    gam1_prim_impl(&S, mult, &eb, &pb, &ek, &pk, &e, &s);
                     i64 ic = ampeb2c[ie] - 1, jc = ampeb2c[je] - 1;
                     double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[ie] * agebfcc[je];
                     i64 ia = ((ip * npbf + jp) * nebf + ic) * nebf + jc;
                     GAM1[ia] += e * cf;
                     GAM1s[ia] += s * cf;
                 }
+}
+
+void orc_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    gam1_impl(&B, &S, 0, GAM1, GAM1s);
+}
+
+/* RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2-398): the loops of
+ * GAM1_OMP_MD over RXCHFmult_xcalc_GAM1_MD */
+void orc_rxchfmult_gam1(BAS_ARGS, SYS_ARGS, double *GAM1, double *GAM1s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    gam1_impl(&B, &S, 1, GAM1, GAM1s);
 }
 
 /* GAM2_IC1 (gam2_IC1_OMP.f:2-212) with contract_omega2_conv
@@ -1080,10 +1150,15 @@ void orc_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2, double *GM2s)
 
 /* GAM3_IC1 (gam3_IC1_OMP.f:2-263) with contract_omega3_conv
  * (gam3_CONV_OMP.f:479-705) and the six-permutation average. */
-void orc_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3)
+typedef void (*prim3_fn)(const sys_t *, const prim_t *, const prim_t *, double *);
+
+static void gam3_ic1_impl(const basis_t *Bp, const sys_t *Sp, prim3_fn prim, double *GM3)
 {
-    BAS_INIT;
-    SYS_INIT;
+    basis_t B = *Bp;
+    sys_t S = *Sp;
+    i64 nebf = B.nebf, npbf = B.npbf;
+    const i64 *kpestr = B.kpestr, *kpeend = B.kpeend;
+    const double *agebfcc = B.agebfcc, *agnbfcc = B.agnbfcc;
     i64 ne = nebf, np = npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
     double *X = (double *)malloc(sizeof(double) * ng3);
 #pragma omp parallel for collapse(2) schedule(dynamic, 1)
@@ -1107,7 +1182,7 @@ void orc_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3)
                                                             prim_t k[4] = {eprim(&B, c1), eprim(&B, c2), eprim(&B, c3),
                                                                            pprim(&B, jp)};
                                                             double e;
-                                                            gam3_prim(&S, b, k, &e);
+                                                            prim(&S, b, k, &e);
                                                             double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] *
                                                                         agebfcc[c1] * agebfcc[a2] * agebfcc[c2] *
                                                                         agebfcc[a3] * agebfcc[c3];
@@ -1132,6 +1207,22 @@ void orc_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3)
                                     GM3[ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3)] = x / 6.0;
                                 }
     free(X);
+}
+
+void orc_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3)
+{
+    BAS_INIT;
+    SYS_INIT;
+    gam3_ic1_impl(&B, &S, gam3_prim, GM3);
+}
+
+/* RXCHFmult_GAM3_XCHF (rxchfmult/rxchfmult_gam3-xchf.f:2-268) with
+ * RXCHFmult_contract_omega3_XCHF (:402-629) and the six-permutation average */
+void orc_rxchfmult_gam3_xchf(BAS_ARGS, SYS_ARGS, double *GM3)
+{
+    BAS_INIT;
+    SYS_INIT;
+    gam3_ic1_impl(&B, &S, gam3xchf_prim, GM3);
 }
 
 /* calc_GAM_ee (gam_ee.f:2-168): contracted ERIs, layout

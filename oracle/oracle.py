@@ -206,8 +206,9 @@ def ref_prim_list3(s, cls, idx):
     cls 6: RXCHFmult_INT_GAM3_MD, over a combined one-basis dict."""
     R = ref()
     fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_,
-          5: R.rxchfmult_gam2_md_, 6: R.rxchfmult_int_gam3_md_}[cls]
-    nel = {4: 2, 5: 2, 6: 3}.get(cls, cls)
+          5: R.rxchfmult_gam2_md_, 6: R.rxchfmult_int_gam3_md_, 7: R.rxchfmult_xchf_gam3_md_,
+          8: R.rxchfmult_xcalc_gam1_md_}[cls]
+    nel = {4: 2, 5: 2, 6: 3, 7: 3, 8: 1}.get(cls, cls)
     am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
     c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
     cat = np.ascontiguousarray(s["cat"])
@@ -233,7 +234,7 @@ def ref_prim_list3(s, cls, idx):
         tail = [_i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g), C.byref(e)]
         if cls == 5:  # ansE1, ansE2, ansS
             tail += [C.byref(xv), C.byref(sv)]
-        elif cls not in (3, 6):
+        elif cls not in (3, 6, 7):
             tail.append(C.byref(sv))
         fn(*args, *tail)
         out_e[t] = e.value
@@ -684,3 +685,63 @@ def ref_rxchfmult_omg(order, which, na, nb, npb, DAE, DBE, DP, G, F):
     fn(_i8(1), _i8(na), _i8(nb), _i8(npb), _i8(np.asarray(G).size), _f(DAE), _f(DBE), _f(DP),
        _p(np.ascontiguousarray(G)), _p(F), C.byref(e))
     return e.value
+
+
+def rxchfmult_gam1(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne * ne * npb * npb)
+    gs = np.zeros_like(g)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam1(*ba, *sa, _p(g), _p(gs))
+    return g, gs
+
+
+def rxchfmult_gam3_xchf(s):
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne ** 6 * npb * npb)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam3_xchf(*ba, *sa, _p(g))
+    return g
+
+
+def ref_rxchfmult_gam1(s, workdir):
+    """RXCHFmult_GAM1_MD (source/rxchfmult/rxchfmult_gam1.f:2): writes
+    GAM_1.ufm and GAM_1s.ufm into workdir."""
+    ne, npb = s["nebf"], s["npbf"]
+    keep = [np.ascontiguousarray(s[k]) for k in ("ampeb2c", "agebfcc", "agnbfcc", "elcex", "nucex")]
+    keep += [np.asfortranarray(s[k]) for k in ("elcam", "nucam", "elcbfc", "nucbfc")]
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    _in_dir(workdir, lambda: ref().rxchfmult_gam1_md_(_i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ne * ne * npb * npb),
+                                                      _i8(s["npebf"] ** 2 * npb * npb), _i8(s["nat"]),
+                                                      _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b),
+                                                      _p(gm), *[_p(a) for a in keep]))
+    g = np.fromfile(os.path.join(workdir, "GAM_1.ufm"), dtype=np.float64)
+    gs = np.fromfile(os.path.join(workdir, "GAM_1s.ufm"), dtype=np.float64)
+    return g, gs
+
+
+def ref_rxchfmult_gam3_xchf(s):
+    """RXCHFmult_GAM3_XCHF (source/rxchfmult/rxchfmult_gam3-xchf.f:2)."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng3 = ne ** 6 * npb * npb
+    g = np.zeros(ng3)
+    c, f = np.ascontiguousarray, np.asfortranarray
+    keep = [c(s["kpestr"]), c(s["kpeend"]), c(s["ampeb2c"]), c(s["agebfcc"]), c(s["elcex"]), f(s["elcam"]),
+            f(s["elcbfc"]), c(s["agnbfcc"]), c(s["nucex"]), f(s["nucam"]), f(s["nucbfc"])]
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfmult_gam3_xchf_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng3), _i8(s["nat"]), _i8(s["ngtg"]),
+                               _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep], _p(g))
+    return g
+
+
+def ref_rxchfmult_gam4_xchf(na, nb, npb, g2s, gee, workdir):
+    """RXCHFmult_GAM4_XCHF (source/rxchfmult/rxchfmult_gam4-xchf.f:2): reads
+    GAM_ee.ufm (regular basis) from workdir."""
+    np.ascontiguousarray(gee, dtype=np.float64).tofile(os.path.join(workdir, "GAM_ee.ufm"))
+    g4 = np.zeros(npb * npb * nb ** 8)
+    _in_dir(workdir, lambda: ref().rxchfmult_gam4_xchf_(_i8(1), _i8(na), _i8(nb), _i8(npb), _i8(na ** 4),
+                                                        _i8(npb * npb * nb ** 4), _i8(g4.size),
+                                                        _p(np.ascontiguousarray(g2s)), _p(g4)))
+    return g4

@@ -21,7 +21,7 @@ program drive_shim
   real(c_double), allocatable :: cat(:), zan(:), bcoef(:), gamma(:), agebfcc(:), agnbfcc(:), elcex(:), nucex(:)
   real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:), dae(:,:), dbe(:,:)
   integer(c_int64_t), allocatable :: kpestr(:), kpeend(:), ampeb2c(:), elcam(:), nucam(:)
-  real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:)
+  real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:), g1(:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -45,6 +45,8 @@ program drive_shim
 
   call GAM1_OMP_MD(nebf, npebf, npbf, ng1, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
                    ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc)
+  allocate(g1(ng1))
+  call read_ufm('GAM_1.ufm', ng1, g1)
   call calc_GAM_ee(nebf, npebf, ngee, ampeb2c, agebfcc, elcex, elcam, elcbfc)
   call GAM2_IC1(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, kpeend, &
                 ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm2, gm2s)
@@ -173,5 +175,28 @@ program drive_shim
   fe = 0.0d0
   call RXCHFmult_FBE_OMG4(one, nebf, nebf, npbf, ng4, dae, dbe, dp, gm4, fe, e4)
   write(11) fe, e4
+  ! RXCHFmult's XCHF-side builders over the special basis (= the regular one
+  ! here): GAM1_MD overwrites GAM_1.ufm, so the GAM1_OMP_MD file goes first
+  write(11) g1
+  call RXCHFmult_GAM1_MD(nebf, npebf, npbf, ng1, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                         ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc)
+  call read_ufm('GAM_1.ufm', ng1, g1)
+  write(11) g1
+  call RXCHFmult_GAM3_XCHF(one, nebf, npebf, npbf, ng3, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                           kpeend, ampeb2c, agebfcc, elcex, elcam, elcbfc, agnbfcc, nucex, nucam, nucbfc, gm3)
+  call RXCHFmult_GAM4_XCHF(one, nebf, nebf, npbf, ngee, ng2, ng4, gm2s, gm4)
+  write(11) gm3, gm4
   close(11)
+contains
+  subroutine read_ufm(name, n, a)
+    character(len=*), intent(in) :: name
+    integer(c_int64_t), intent(in) :: n
+    real(c_double), intent(out) :: a(n)
+    integer(c_int64_t) :: k
+    open(820, file=name, form='unformatted', status='old', access='direct', recl=8)
+    do k = 1, n
+      read(820, rec=k) a(k)
+    end do
+    close(820)
+  end subroutine
 end program

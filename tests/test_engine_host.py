@@ -106,3 +106,31 @@ def test_engine_rxchfmult_tuples_vs_oracle():
             got = engine_prim_list3(cs, cls, idx, tier)
             for k, what in enumerate(("ansE1", "ansS", "ansE2") if cls == 5 else ("ans",)):
                 check_close(got[k], ref[k], what=f"cls{cls} {what} tier {tier}")
+
+
+def test_engine_rxchfmult_xchf_side_vs_oracle():
+    """RXCHFmult_xcalc_GAM1_MD (class 8) and RXCHFmult_XCHF_GAM3_MD (class 7)
+    in the host engine against the oracle; class 7 keeps the labels of
+    xcalc_GAM3_MD, so it is compared after averaging each tuple over the six
+    permutations of its electrons."""
+    from itertools import permutations
+
+    from _util import engine_prim_list3
+
+    s = synth.small(seed=59, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    idx = random_tuples(s, 1, 300, seed=21)
+    ref = oracle.prim_list3(s, 8, idx)
+    for tier in (0, 1):
+        got = engine_prim_list3(s, 8, idx, tier)
+        check_close(got[0], ref[0], what=f"cls8 E tier {tier}")
+        check_close(got[1], ref[1], what=f"cls8 S tier {tier}")
+    base = random_tuples(s, 3, 20, seed=22)
+    rows = []
+    for x in base:
+        for p in permutations(range(3)):
+            rows.append([x[p[0]], x[p[1]], x[p[2]], x[3], x[4 + p[0]], x[4 + p[1]], x[4 + p[2]], x[7]])
+    idx = np.array(rows, dtype=np.int64)
+    ref = oracle.prim_list3(s, 7, idx)[0].reshape(-1, 6).mean(axis=1)
+    for tier in (0, 1):
+        got = engine_prim_list3(s, 7, idx, tier)[0].reshape(-1, 6).mean(axis=1)
+        check_close(got, ref, what=f"cls7 six-permutation average tier {tier}")

@@ -87,7 +87,9 @@ def test_fortran_dropin_matches_oracle():
              ng3, ng4,  # RXCHFmult_GAM3_INT, RXCHFmult_GAM4_INT
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG2
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # ... _OMG3
-             ne * ne, 1, npb * npb, 1, ne * ne, 1]  # ... _OMG4
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # ... _OMG4
+             ne * ne * npb * npb, ne * ne * npb * npb,  # GAM_1 of GAM1_OMP_MD, of RXCHFmult_GAM1_MD
+             ng3, ng4]  # RXCHFmult_GAM3_XCHF, RXCHFmult_GAM4_XCHF (in the driver gm2s holds XCHF_GAM2s)
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -114,7 +116,8 @@ def test_fortran_dropin_matches_oracle():
     check_close(parts[13], FE.ravel(order="F"), what="EG4ICR FE")
     assert parts[15][0] == pytest.approx(E4, rel=1e-11)
     og1, og1s = oracle.gam1(s)
-    check_close(gam1, og1, what="GAM_1 file")
+    check_close(parts[79], og1, what="GAM_1 file of GAM1_OMP_MD")
+    check_close(gam1, parts[80], what="GAM_1 file of RXCHFmult_GAM1_MD")
     # direct-SCF routines against the oracle's restatement of GAM*_DSCF
     F = [np.zeros((n, n), order="F") for n in (ne, npb, ne, npb)]
     E2, S2 = oracle.gam2_dscf(s, DE, DP, *F)
@@ -189,3 +192,6 @@ def test_fortran_dropin_matches_oracle():
             at = 61 + 6 * (order - 2) + 2 * which
             check_close(parts[at], F.ravel(order="F"), what=f"RXCHFmult omg{order} {which}")
             assert parts[at + 1][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    check_close(parts[80], oracle.rxchfmult_gam1(s)[0], what="RXCHFmult_GAM1_MD")
+    check_close(parts[81], oracle.rxchfmult_gam3_xchf(s), what="RXCHFmult_GAM3_XCHF")
+    check_close(parts[82], oracle.gam4_icr(ne, npb, mxs, oee), what="RXCHFmult_GAM4_XCHF")

@@ -443,3 +443,44 @@ def test_rxchfmult(torch_dev):
             eo = oracle.rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, B)
             check_close(A - F0, B - F0, what=f"RXCHFmult omg{order} {which}")
             assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)
+
+
+def test_rxchfmult_xchf_side(torch_dev):
+    """RXCHFmult's XCHF-side builders over one basis: class-8 and class-7
+    tuples, RXCHFmult_GAM1_MD, RXCHFmult_GAM3_XCHF and RXCHFmult_GAM4_XCHF."""
+    from itertools import permutations
+
+    torch = torch_dev
+    s = synth.small(seed=61, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    S = neorx.System(s)
+    idx = random_tuples(s, 1, 1024, seed=31)
+    print("case rxchfmult cls8 n=1024", flush=True)
+    got = _run_list3(torch, S, 8, idx)
+    ref = oracle.prim_list3(s, 8, idx)
+    check_close(got[0], ref[0], what="cls8 E")
+    check_close(got[1], ref[1], what="cls8 S")
+    base = random_tuples(s, 3, 40, seed=32)
+    rows = [[x[p[0]], x[p[1]], x[p[2]], x[3], x[4 + p[0]], x[4 + p[1]], x[4 + p[2]], x[7]]
+            for x in base for p in permutations(range(3))]
+    idx = np.array(rows, dtype=np.int64)
+    print("case rxchfmult cls7 n=240", flush=True)
+    got = _run_list3(torch, S, 7, idx)[0].reshape(-1, 6).mean(axis=1)
+    ref = oracle.prim_list3(s, 7, idx)[0].reshape(-1, 6).mean(axis=1)
+    check_close(got, ref, what="cls7 six-permutation average")
+    S.close()
+    t = synth.small(seed=62, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 0), ngtg=2)
+    print("case rxchfmult GAM1_MD / GAM3_XCHF", flush=True)
+    T = neorx.System(t)
+    g, gs = T.RXCHFmult_GAM1_MD()
+    og, ogs = oracle.rxchfmult_gam1(t)
+    check_close(g, og, what="RXCHFmult_GAM1_MD GAM_1")
+    check_close(gs, ogs, what="RXCHFmult_GAM1_MD GAM_1s")
+    check_close(T.RXCHFmult_GAM3_XCHF(), oracle.rxchfmult_gam3_xchf(t), what="RXCHFmult_GAM3_XCHF")
+    T.close()
+    reg = synth.small(seed=63, n_e_centres=1, e_shells=(0, 1, 0), contract=(1,), p_shells=(0,), ngtg=2)
+    na, nb, npb = reg["nebf"], 4, reg["npbf"]
+    gee = oracle.gam_ee(reg)
+    g2s = np.random.default_rng(3).standard_normal(npb * npb * nb ** 4)
+    eeb = np.ascontiguousarray(gee.reshape((na,) * 4)[:nb, :nb, :nb, :nb]).ravel()
+    check_close(neorx.RXCHFmult_GAM4_XCHF(na, nb, npb, g2s, gee), oracle.gam4_icr(nb, npb, g2s, eeb),
+                what="RXCHFmult_GAM4_XCHF")

@@ -365,3 +365,39 @@ def test_oracle_rxchfmult_vs_reference(tmp_path):
             er = oracle.ref_rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, B)
             assert eo == pytest.approx(er, rel=1e-12, abs=1e-14)
             check_close(A - F0, B - F0, what=f"RXCHFmult omg{order} {which}")
+
+
+@needs_ref
+def test_oracle_rxchfmult_xchf_side_vs_reference(tmp_path):
+    """RXCHFmult's XCHF-side builders over the special-electron basis:
+    RXCHFmult_xcalc_GAM1_MD / RXCHFmult_XCHF_GAM3_MD tuples, RXCHFmult_GAM1_MD,
+    RXCHFmult_GAM3_XCHF and RXCHFmult_GAM4_XCHF (GAM4_ICR over the special
+    basis with the leading block of GAM_ee) against the reference."""
+    s = synth.small(seed=55, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    for cls, n in ((8, 60), (7, 4)):
+        nel = 1 if cls == 8 else 3
+        idx = random_tuples(s, nel, n, seed=cls)
+        ref = oracle.ref_prim_list3(s, cls, idx)
+        got = oracle.prim_list3(s, cls, idx)
+        check_close(got[0], ref[0], what=f"cls{cls} E")
+        if cls == 8:
+            check_close(got[1], ref[1], what="cls8 S")
+    t = synth.small(seed=56, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 0), ngtg=2)
+    g, gs = oracle.rxchfmult_gam1(t)
+    rg, rgs = oracle.ref_rxchfmult_gam1(t, str(tmp_path))
+    check_close(g, rg, what="RXCHFmult_GAM1_MD GAM_1")
+    check_close(gs, rgs, what="RXCHFmult_GAM1_MD GAM_1s")
+    x1, _ = oracle.gam1(t)
+    assert np.max(np.abs(g - x1)) > 1e-6 * np.max(np.abs(x1))  # the one-geminal terms are really dropped
+    u = synth.small(seed=57, n_e_centres=1, e_shells=(0, 1), contract=(1,), p_shells=(0,), ngtg=2)
+    g3 = oracle.rxchfmult_gam3_xchf(u)
+    check_close(g3, oracle.ref_rxchfmult_gam3_xchf(u), what="RXCHFmult_GAM3_XCHF")
+    # GAM4_XCHF: special basis = the first nb functions of a larger regular basis
+    reg = synth.small(seed=58, n_e_centres=1, e_shells=(0, 1, 0), contract=(1,), p_shells=(0,), ngtg=2)
+    na, nb, npb = reg["nebf"], 4, reg["npbf"]
+    gee = oracle.gam_ee(reg)
+    rng = np.random.default_rng(2)
+    g2s = rng.standard_normal(npb * npb * nb ** 4)
+    eeb = np.ascontiguousarray(gee.reshape((na,) * 4)[:nb, :nb, :nb, :nb]).ravel()
+    g4 = oracle.gam4_icr(nb, npb, g2s, eeb)
+    check_close(g4, oracle.ref_rxchfmult_gam4_xchf(na, nb, npb, g2s, gee, str(tmp_path)), what="RXCHFmult_GAM4_XCHF")
