@@ -259,7 +259,9 @@ int neorx_mult_system_create(int64_t nebf, int64_t npebf, int64_t nebfbe, int64_
 /* Primitive tuples of a two-basis handle: primitive indices are over the
  * regular primitives followed by the special ones (0-based).
  *   cls 5 (e-e-p)   : RXCHFmult_GAM2_MD, rxchfmult/rxchfmult_gam2.f:627 -> ansE1, ansS, ansE2
- *   cls 6 (e-e-e-p) : RXCHFmult_INT_GAM3_MD, rxchfmult/rxchfmult_gam3-int.f:661 -> ans
+ *   cls 6 (e-e-e-p) : RXCHFmult_INT_GAM3_MD, rxchfmult/rxchfmult_gam3-int.f:661 -> ans, and
+ *                     ans3 = 2 gVEEg2, ans4 = gVEEg1 of RXCHFmult_xcalc_GAM3_MDex
+ *                     (rxchfmult/rxchfmult_gam3ex.f:712) in the S and E2 arrays
  * and of any handle:
  *   cls 7 (e-e-e-p) : RXCHFmult_XCHF_GAM3_MD, rxchfmult/rxchfmult_gam3-xchf.f:630 -> ans
  *                     (electron labels as in xcalc_GAM3_MD; equal after the
@@ -320,6 +322,39 @@ int neorx_rxchfmult_fp_omg4(int64_t nebf, int64_t nebfbe, int64_t npbf, const do
                             const double *dp, const double *gm, double *fp, double *e);
 int neorx_rxchfmult_fbe_omg4(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,
                              const double *dp, const double *gm, double *fbe, double *e);
+/* ---- RXCHFmult with exchange (LADDEXCH), one basis ------------------------
+ * The routines the driver calls with LADDEXCH set take the regular basis for
+ * every electron (nebfBE = nebf): the builders take a two-basis handle whose
+ * two bases are the same, the contractions one basis size. */
+/* RXCHFmult_GAM2_IC1ex, rxchfmult/rxchfmult_gam2ex.f:2: GM2_1ICR (XCHF_GAM2),
+ * GM2_2ICR (INT_GAM2), GM2_3ICR (INT_GAM2ex), GM2sICR (XCHF_GAM2s), ng2 each */
+int neorx_rxchfmult_gam2_ic1ex(int64_t handle, double *gm2_1, double *gm2_2, double *gm2_3, double *gm2s);
+int neorx_rxchfmult_gam2_ic1ex_dev(int64_t handle, double *d_gm2_1, double *d_gm2_2, double *d_gm2_3,
+                                   double *d_gm2s, void *stream);
+/* RXCHFmult_GAM3_IC1ex, rxchfmult/rxchfmult_gam3ex.f:2: INT_GAM3, INT_GAM3ex1,
+ * INT_GAM3ex2 (ng3 each) */
+int neorx_rxchfmult_gam3_ic1ex(int64_t handle, double *int_gam3, double *int_gam3ex1, double *int_gam3ex2);
+int neorx_rxchfmult_gam3_ic1ex_dev(int64_t handle, double *d_int_gam3, double *d_ex1, double *d_ex2, void *stream);
+/* RXCHFmult_FAE/FP/FBE_OMG2ex, rxchfmult/rxchfmult_omg2ex.f:2, :181, :360;
+ * _OMG3ex, rxchfmult_omg3ex.f:2, :217, :432.  f accumulates, e overwritten. */
+int neorx_rxchfmult_fae_omg2ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                               const double *gm2, const double *gm2ex, double *fae, double *e);
+int neorx_rxchfmult_fp_omg2ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                              const double *gm2, const double *gm2ex, double *fp, double *e);
+int neorx_rxchfmult_fbe_omg2ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                               const double *gm2, const double *gm2ex, double *fbe, double *e);
+int neorx_rxchfmult_fae_omg3ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                               const double *gm3, const double *gm3ex1, const double *gm3ex2, double *fae, double *e);
+int neorx_rxchfmult_fp_omg3ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                              const double *gm3, const double *gm3ex1, const double *gm3ex2, double *fp, double *e);
+int neorx_rxchfmult_fbe_omg3ex(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                               const double *gm3, const double *gm3ex1, const double *gm3ex2, double *fbe, double *e);
+/* The three exchange-form contractions of order 2 (OMG2ex: d_gx1 = GM2exICR)
+ * or 3 (OMG3ex: d_gx1, d_gx2 = GM3ex1ICR, GM3ex2ICR) on device arrays; any of
+ * d_fae, d_fp, d_fbe may be NULL. */
+int neorx_rxchfmult_omgex_dev(int64_t order, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                              const double *d_dp, const double *d_gm, const double *d_gx1, const double *d_gx2,
+                              double *d_fae, double *d_fp, double *d_fbe, double *d_e, void *stream);
 /* The three contractions of one order k (1: OMG2, 2: OMG3, 3: OMG4) in one
  * pass over the device tensor d_gm; any of d_fae, d_fp, d_fbe may be NULL.
  * d_e receives the energy all three reference routines return. */

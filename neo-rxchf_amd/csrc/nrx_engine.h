@@ -846,10 +846,12 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
 // (rxchfmult/rxchfmult_gam2.f:627-1162), where e1 and e2 come from different
 // bases.  It drops the single-geminal terms, places the geminal of gVEEg1 on
 // e2 (gamA23/gamB23, :1067-1068) instead of e1, and returns three values:
-// ansE = the two-geminal energy, ansE2 = gVEPg2 + gVEEg1, ansS = gsg.
+// ansE = the two-geminal energy, ansE2 = gVEPg2 + gVEEg1, ansS = gsg; its
+// exchange form RXCHFmult_xcalc_GAM2_MDex (rxchfmult_gam2ex.f:559-1097) adds
+// ansE3 = gVEEg2.
 template <int NL, bool MULT = false>
 NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
-                      double &ansS, double &ansE2)
+                      double &ansS, double &ansE2, double &ansE3)
 {
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
     if constexpr (!MULT)
@@ -909,14 +911,15 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
            gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
     ansS = gsg;
     ansE2 = gVEPg2 + gVEEg1;
+    ansE3 = gVEEg2;
 }
 
 template <int NL>
 NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
                        double &ansS)
 {
-    double unused;
-    gam2_core<NL, false>(S, e1, e2, p, ansE, ansS, unused);
+    double unused, unused3;
+    gam2_core<NL, false>(S, e1, e2, p, ansE, ansS, unused, unused3);
 }
 
 // xcalc_GAM3_MD (gam_3_OMP.f:617-983).  With MULT set:
@@ -1004,9 +1007,11 @@ NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e
 //   xgVEEg1 = <g(e2,p) = gamma_k + gamma_l> * <1/r(e1,e3)>
 //   xgVEEg2 = <1/r(e1,e2)> with g(e2,p) = gamma_k, g(e3,p) = gamma_l
 //   ans = gVEPg1 + gVEEg1 + 2 gVEEg2
+// and, for its exchange form RXCHFmult_xcalc_GAM3_MDex (rxchfmult_gam3ex.f:712-1111),
+// ans3 = 2 gVEEg2, ans4 = gVEEg1
 template <int NL>
 NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3,
-                          const PairRec *p, double &ans)
+                          const PairRec *p, double &ans, double &ans3, double &ans4)
 {
     Sys s13 = mksys(S, 2, e1, e3, 0, 0, 0.0, 0, 0);
     double vee13 = coul<NL>(s13, 0, 1);
@@ -1029,6 +1034,8 @@ NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *
             gVEEg1 += w * (xx * vee13);
         }
     ans = gVEPg1 + gVEEg1 + 2.0 * gVEEg2;
+    ans3 = 2.0 * gVEEg2;
+    ans4 = gVEEg1;
 }
 
 // Polynomial capacities per tuple class.  GEN covers every tuple allowed by
@@ -1091,23 +1098,36 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
     else if constexpr (CLS == 4)
         gam2ne_tuple<NL>(S, q[0], q[1], q[2], e, s);
     else if constexpr (CLS == 5) {
+        double x, y;
+        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x, y);
+    } else if constexpr (CLS == 6) {
         double x;
-        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x);
-    } else if constexpr (CLS == 6)
-        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
+        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e, s, x);
+    }
     else
         gam3_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
 }
 
-// three-output form: x is ansE2 of class 5, zero for the other classes
+// four-output form: (ansE1, ansS, ansE2, ansE3) of class 5, (ans, ans3, ans4,
+// 0) of class 6; x and y are zero for the other classes
+template <int CLS, int NL>
+NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x, double &y)
+{
+    x = 0.0;
+    y = 0.0;
+    if constexpr (CLS == 5)
+        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x, y);
+    else if constexpr (CLS == 6)
+        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e, s, x);
+    else
+        tuple_eval<CLS, NL>(S, q, e, s);
+}
+
 template <int CLS, int NL>
 NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x)
 {
-    x = 0.0;
-    if constexpr (CLS == 5)
-        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x);
-    else
-        tuple_eval<CLS, NL>(S, q, e, s);
+    double y;
+    tuple_eval<CLS, NL>(S, q, e, s, x, y);
 }
 
 template <int CLS> NRX_HD bool tuple_small(const PairRec *const *q)

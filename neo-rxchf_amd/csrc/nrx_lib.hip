@@ -313,11 +313,12 @@ template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x,
     tuple_eval<CLS, NL>(d_params(S), q, e, s);
 }
 
-template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s, double &xx)
+template <int CLS, int NL>
+NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s, double &xx, double &yy)
 {
     const PairRec *q[nel<CLS>() + 1];
     tuple_pairs<CLS>(S, x, q);
-    tuple_eval<CLS, NL>(d_params(S), q, e, s, xx);
+    tuple_eval<CLS, NL>(d_params(S), q, e, s, xx, yy);
 }
 
 template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
@@ -391,8 +392,8 @@ template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *id
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
                                                     double *__restrict__ outE, double *__restrict__ outS,
-                                                    double *__restrict__ outX, i64 *__restrict__ ovf,
-                                                    unsigned *__restrict__ novf)
+                                                    double *__restrict__ outX, double *__restrict__ outY,
+                                                    i64 *__restrict__ ovf, unsigned *__restrict__ novf)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntup) return;
@@ -402,6 +403,7 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
             outE[t] = __builtin_nan("");
             if (outS) outS[t] = __builtin_nan("");
             if (outX) outX[t] = __builtin_nan("");
+            if (outY) outY[t] = __builtin_nan("");
             return;
         }
     } else {
@@ -415,11 +417,12 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
             return;
         }
     }
-    double e, s, xx;
-    eval_tuple<CLS, NS_>(S, x, e, s, xx);
+    double e, s, xx, yy;
+    eval_tuple<CLS, NS_>(S, x, e, s, xx, yy);
     outE[t] = e;
     if (outS) outS[t] = s;
     if (outX) outX[t] = xx;
+    if (outY) outY[t] = yy;
 }
 
 // Second pass: the tuples listed by the first, general capacity, grid-stride
@@ -427,19 +430,20 @@ __global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntu
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_gen(DevSys S, i64 start, const i64 *__restrict__ idx,
                                                   double *__restrict__ outE, double *__restrict__ outS,
-                                                  double *__restrict__ outX, const i64 *__restrict__ ovf,
-                                                  const unsigned *__restrict__ novf)
+                                                  double *__restrict__ outX, double *__restrict__ outY,
+                                                  const i64 *__restrict__ ovf, const unsigned *__restrict__ novf)
 {
     i64 n = *novf;
     for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
         i64 t = ovf[k];
         i64 x[2 * nel<CLS>() + 2];
         src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by the first pass
-        double e, s, xx;
-        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s, xx);
+        double e, s, xx, yy;
+        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s, xx, yy);
         outE[t] = e;
         if (outS) outS[t] = s;
         if (outX) outX[t] = xx;
+        if (outY) outY[t] = yy;
     }
 }
 
@@ -473,7 +477,7 @@ struct StreamBuf {
 
 template <int CLS, int SRC>
 static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st,
-                        double *ox = nullptr)
+                        double *ox = nullptr, double *oy = nullptr)
 {
     constexpr bool tiered = Tier<CLS>::SMALL != Tier<CLS>::GEN;
     if constexpr (two_basis<CLS>())
@@ -493,16 +497,17 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
         i64 n = std::min(ntup - off, PRIM_CHUNK);
         const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * nel<CLS>() + 2) : nullptr;
         double *e = oe + off, *s = os ? os + off : nullptr, *x = ox ? ox + off : nullptr;
+        double *y = oy ? oy + off : nullptr;
         if (tiered) {
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, cnt);
             HIPCHK(hipGetLastError());
         }
         hipLaunchKernelGGL((k_prim_small<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e,
-                           s, x, ovf, cnt);
+                           s, x, y, ovf, cnt);
         HIPCHK(hipGetLastError());
         if (tiered) {
             hipLaunchKernelGGL((k_prim_gen<CLS, SRC>), dim3(GEN_BLOCKS), dim3(256), 0, st, d, start + off, ix, e, s,
-                               x, ovf, cnt);
+                               x, y, ovf, cnt);
             HIPCHK(hipGetLastError());
         }
     }
@@ -2465,12 +2470,15 @@ static System *get_mult(int64_t handle, int &rc)
 }
 
 // contract_omega2 (rxchfmult_gam2.f:413-626): lane per contracted
-// (L = ip*npbf + jp, i1, j1, i2, j2); the three primitive values ansE1, ansS,
-// ansE2 of the slab in Y, Ys, Yx (class 5 enumeration, from R0)
+// (L = ip*npbf + jp, i1, j1, i2, j2); the primitive values ansE1, ansS, ansE2
+// (and ansE3 of the exchange form, contract_omega2_convex,
+// rxchfmult_gam2ex.f:354-558) of the slab in Y, Ys, Yx, Yy (class 5
+// enumeration, from R0); Yy and X3 may be null
 __global__ void __launch_bounds__(256) k_mcontract2(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
                                                     const double *__restrict__ Ys, const double *__restrict__ Yx,
-                                                    double *__restrict__ X1, double *__restrict__ Xs,
-                                                    double *__restrict__ X2)
+                                                    const double *__restrict__ Yy, double *__restrict__ X1,
+                                                    double *__restrict__ Xs, double *__restrict__ X2,
+                                                    double *__restrict__ X3)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -2480,7 +2488,7 @@ __global__ void __launch_bounds__(256) k_mcontract2(DevSys S, i64 L0, i64 n, i64
     i64 j2 = rem % nb, i2 = (rem / nb) % nb, j1 = (rem / (nb * nb)) % na, i1 = rem / (nb * nb * na);
     i64 ip = L / np, jp = L % np;
     double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
-    double o1 = 0.0, o2 = 0.0, os = 0.0;
+    double o1 = 0.0, o2 = 0.0, o3 = 0.0, os = 0.0;
     for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
         for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
             for (i64 a2 = S.kpestr[na + i2] - 1; a2 < S.kpeend[na + i2]; ++a2)
@@ -2490,11 +2498,13 @@ __global__ void __launch_bounds__(256) k_mcontract2(DevSys S, i64 L0, i64 n, i64
                     o1 += Y[r] * cf;
                     o2 += Yx[r] * cf;
                     os += Ys[r] * cf;
+                    if (Yy) o3 += Yy[r] * cf;
                 }
     i64 o = L * blk + rem;
     X1[o] = o1;
     X2[o] = o2;
     Xs[o] = os;
+    if (X3) X3[o] = o3;
 }
 
 // XCHF_GAM2 = (x12 + x21)/2 over special-basis indices (rxchfmult_gam2.f:173-228):
@@ -2514,6 +2524,33 @@ __global__ void __launch_bounds__(256) k_mult_symm2(i64 na, i64 nb, i64 n, const
     Gs[t] = (Xs[a] + Xs[b]) / 2.0;
 }
 
+// The contracted e-e-p tensors of a two-basis handle: X1, Xs (unsymmetrised
+// ansE1, ansS), X2 (ansE2) and, if X3 is given, X3 (ansE3).
+static int mult_build2(System *s, double *X1, double *Xs, double *X2, double *X3, hipStream_t st)
+{
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    i64 cap;
+    auto slabs = make_slabs(s, 5, 0, np * np, cap);
+    DevBuf Y, Ys, Yx, Yy;
+    int rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = Yx.alloc(cap * 8);
+    if (!rc && X3) rc = Yy.alloc(cap * 8);
+    if (rc) return rc;
+    i64 blk = na * na * nb * nb;
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 5, sl.first), R1 = slab_start(s, 5, sl.second);
+        rc = launch_prims<5, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st, Yx.d(), X3 ? Yy.d() : nullptr);
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * blk;
+        hipLaunchKernelGGL(k_mcontract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
+                           Ys.d(), Yx.d(), X3 ? Yy.d() : nullptr, X1, Xs, X2, X3);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st)); // slab buffers are released on return
+    return NEORX_OK;
+}
+
 extern "C" int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, double *d_xchf_gam2,
                                         double *d_xchf_gam2s, void *stream)
 {
@@ -2526,30 +2563,66 @@ extern "C" int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, doub
                              "special-basis indices", (long long)nb, (long long)na);
     i64 nint = np * np * na * na * nb * nb, nx = np * np * nb * nb * nb * nb;
     hipStream_t st = (hipStream_t)stream;
-    i64 cap;
-    auto slabs = make_slabs(s, 5, 0, np * np, cap);
-    DevBuf X1, Xs, Y, Ys, Yx;
+    DevBuf X1, Xs;
     rc = X1.alloc(nint * 8);
     if (!rc) rc = Xs.alloc(nint * 8);
-    if (!rc) rc = Y.alloc(cap * 8);
-    if (!rc) rc = Ys.alloc(cap * 8);
-    if (!rc) rc = Yx.alloc(cap * 8);
+    if (!rc) rc = mult_build2(s, X1.d(), Xs.d(), d_int_gam2, nullptr, st);
     if (rc) return rc;
-    i64 blk = na * na * nb * nb;
-    for (auto &sl : slabs) {
-        i64 R0 = slab_start(s, 5, sl.first), R1 = slab_start(s, 5, sl.second);
-        rc = launch_prims<5, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st, Yx.d());
-        if (rc) return rc;
-        i64 n = (sl.second - sl.first) * blk;
-        hipLaunchKernelGGL(k_mcontract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
-                           Ys.d(), Yx.d(), X1.d(), Xs.d(), d_int_gam2);
-        HIPCHK(hipGetLastError());
-    }
     hipLaunchKernelGGL(k_mult_symm2, dim3(nblocks(nx, 256)), dim3(256), 0, st, na, nb, nx, X1.d(), Xs.d(),
                        d_xchf_gam2, d_xchf_gam2s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
+}
+
+// RXCHFmult_GAM2_IC1ex (rxchfmult_gam2ex.f:2-229, symmetrised at :160-186),
+// one basis (a two-basis handle whose two bases are the same, nebfBE = nebf):
+// GM2_1ICR, GM2_3ICR, GM2sICR = (x12 + x21)/2 of ansE1, ansE3, ansS;
+// GM2_2ICR = ansE2 as contracted
+extern "C" int neorx_rxchfmult_gam2_ic1ex_dev(int64_t handle, double *d_gm2_1, double *d_gm2_2, double *d_gm2_3,
+                                              double *d_gm2s, void *stream)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    if (!d_gm2_1 || !d_gm2_2 || !d_gm2_3 || !d_gm2s) return fail(NEORX_E_ARG, "null output");
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    if (nb != na) return fail(NEORX_E_ARG, "RXCHFmult_GAM2_IC1ex takes one basis: nebfBE=%lld != nebf=%lld",
+                              (long long)nb, (long long)na);
+    i64 ng2 = np * np * na * na * na * na;
+    hipStream_t st = (hipStream_t)stream;
+    DevBuf X1, Xs, X3;
+    rc = X1.alloc(ng2 * 8);
+    if (!rc) rc = Xs.alloc(ng2 * 8);
+    if (!rc) rc = X3.alloc(ng2 * 8);
+    if (!rc) rc = mult_build2(s, X1.d(), Xs.d(), d_gm2_2, X3.d(), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, na, ng2, X1.d(), Xs.d(), d_gm2_1,
+                       d_gm2s);
+    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, na, ng2, X3.d(), X3.d(), d_gm2_3,
+                       d_gm2_3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam2_ic1ex(int64_t handle, double *gm2_1, double *gm2_2, double *gm2_3, double *gm2s)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    i64 na = s->d.nebf1, np = s->d.npbf, ng2 = np * np * na * na * na * na;
+    DevBuf A, B, C, D;
+    rc = A.alloc(ng2 * 8);
+    if (!rc) rc = B.alloc(ng2 * 8);
+    if (!rc) rc = C.alloc(ng2 * 8);
+    if (!rc) rc = D.alloc(ng2 * 8);
+    if (!rc) rc = neorx_rxchfmult_gam2_ic1ex_dev(handle, A.d(), B.d(), C.d(), D.d(), 0);
+    if (!rc) rc = copy_out(gm2_1, A.d(), ng2);
+    if (!rc) rc = copy_out(gm2_2, B.d(), ng2);
+    if (!rc) rc = copy_out(gm2_3, C.d(), ng2);
+    if (!rc) rc = copy_out(gm2s, D.d(), ng2);
+    return rc;
 }
 
 extern "C" int neorx_rxchfmult_gam2(int64_t handle, double *int_gam2, double *xchf_gam2, double *xchf_gam2s)
@@ -2571,9 +2644,13 @@ extern "C" int neorx_rxchfmult_gam2(int64_t handle, double *int_gam2, double *xc
 }
 
 // contract_omega3_INT (rxchfmult_gam3-int.f:417-660): lane per contracted
-// (L = (ip*npbf + jp)*nebf + i1, j1, i2, j2, i3, j3)
+// (L = (ip*npbf + jp)*nebf + i1, j1, i2, j2, i3, j3); with Ys, Yx given also
+// ans3, ans4 of the exchange form (contract_omega3_convex,
+// rxchfmult_gam3ex.f:459-711) into X3, X4
 __global__ void __launch_bounds__(256) k_mcontract3(DevSys S, i64 L0, i64 n, i64 R0, const double *__restrict__ Y,
-                                                    double *__restrict__ X)
+                                                    const double *__restrict__ Ys, const double *__restrict__ Yx,
+                                                    double *__restrict__ X, double *__restrict__ X3,
+                                                    double *__restrict__ X4)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -2584,7 +2661,7 @@ __global__ void __launch_bounds__(256) k_mcontract3(DevSys S, i64 L0, i64 n, i64
         j1 = rem / (nb2 * nb2);
     i64 i1 = L % na, pp = L / na, ip = pp / np, jp = pp % np;
     double cp = S.agnbfcc[ip] * S.agnbfcc[jp];
-    double om = 0.0;
+    double om = 0.0, o3 = 0.0, o4 = 0.0;
     for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
         for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
             for (i64 a2 = S.kpestr[na + i2] - 1; a2 < S.kpeend[na + i2]; ++a2)
@@ -2596,8 +2673,16 @@ __global__ void __launch_bounds__(256) k_mcontract3(DevSys S, i64 L0, i64 n, i64
                             double cf = cp * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2] *
                                         S.agebfcc[a3] * S.agebfcc[c3];
                             om += Y[r] * cf;
+                            if (Ys) {
+                                o3 += Ys[r] * cf;
+                                o4 += Yx[r] * cf;
+                            }
                         }
     X[L * blk + rem] = om;
+    if (X3) {
+        X3[L * blk + rem] = o3;
+        X4[L * blk + rem] = o4;
+    }
 }
 
 // INT_GAM3 = (x123 + x132)/2, electrons 2 and 3 exchanged (rxchfmult_gam3-int.f:173-218)
@@ -2609,6 +2694,19 @@ __global__ void __launch_bounds__(256) k_mult_symm3(i64 nb, i64 n, const double 
     i64 nb2 = nb * nb;
     i64 p3 = t % nb2, p2 = (t / nb2) % nb2, head = t / (nb2 * nb2);
     G[t] = (X[t] + X[(head * nb2 + p3) * nb2 + p2]) * 0.5;
+}
+
+// one basis (ne): (x123 + x213)/2 (swap = 1) or (x123 + x321)/2 (swap = 2),
+// rxchfmult_gam3ex.f:248-269
+__global__ void __launch_bounds__(256) k_mult_symm3x(i64 ne, i64 n, int swap, const double *__restrict__ X,
+                                                     double *__restrict__ G)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 ne2 = ne * ne;
+    i64 p3 = t % ne2, p2 = (t / ne2) % ne2, p1 = (t / (ne2 * ne2)) % ne2, pp = t / (ne2 * ne2 * ne2);
+    i64 o = swap == 1 ? ((pp * ne2 + p2) * ne2 + p1) * ne2 + p3 : ((pp * ne2 + p3) * ne2 + p2) * ne2 + p1;
+    G[t] = (X[t] + X[o]) * 0.5;
 }
 
 extern "C" int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, void *stream)
@@ -2632,13 +2730,72 @@ extern "C" int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, 
         if (rc) return rc;
         i64 n = (sl.second - sl.first) * blk;
         hipLaunchKernelGGL(k_mcontract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
-                           X.d());
+                           nullptr, nullptr, X.d(), nullptr, nullptr);
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_mult_symm3, dim3(nblocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
+}
+
+// RXCHFmult_GAM3_IC1ex (rxchfmult_gam3ex.f:2-316), one basis (nebfBE = nebf):
+// INT_GAM3 = (x123 + x132)/2 of ans, INT_GAM3ex1 = (x123 + x213)/2 of
+// ans3 = 2 gVEEg2, INT_GAM3ex2 = (x123 + x321)/2 of ans4 = gVEEg1
+extern "C" int neorx_rxchfmult_gam3_ic1ex_dev(int64_t handle, double *d_int_gam3, double *d_ex1, double *d_ex2,
+                                              void *stream)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    if (!d_int_gam3 || !d_ex1 || !d_ex2) return fail(NEORX_E_ARG, "null output");
+    i64 na = s->d.nebf1, nb = s->d.nebf - na, np = s->d.npbf;
+    if (nb != na) return fail(NEORX_E_ARG, "RXCHFmult_GAM3_IC1ex takes one basis: nebfBE=%lld != nebf=%lld",
+                              (long long)nb, (long long)na);
+    i64 nb2 = nb * nb, blk = na * nb2 * nb2, n3 = np * np * na * blk;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap;
+    auto slabs = make_slabs(s, 6, 0, np * np * na, cap);
+    DevBuf X, X3, X4, Y, Ys, Yx;
+    rc = X.alloc(n3 * 8);
+    if (!rc) rc = X3.alloc(n3 * 8);
+    if (!rc) rc = X4.alloc(n3 * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (!rc) rc = Ys.alloc(cap * 8);
+    if (!rc) rc = Yx.alloc(cap * 8);
+    if (rc) return rc;
+    for (auto &sl : slabs) {
+        i64 R0 = slab_start(s, 6, sl.first), R1 = slab_start(s, 6, sl.second);
+        rc = launch_prims<6, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y.d(), Ys.d(), st, Yx.d());
+        if (rc) return rc;
+        i64 n = (sl.second - sl.first) * blk;
+        hipLaunchKernelGGL(k_mcontract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y.d(),
+                           Ys.d(), Yx.d(), X.d(), X3.d(), X4.d());
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mult_symm3, dim3(nblocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
+    hipLaunchKernelGGL(k_mult_symm3x, dim3(nblocks(n3, 256)), dim3(256), 0, st, na, n3, 1, X3.d(), d_ex1);
+    hipLaunchKernelGGL(k_mult_symm3x, dim3(nblocks(n3, 256)), dim3(256), 0, st, na, n3, 2, X4.d(), d_ex2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfmult_gam3_ic1ex(int64_t handle, double *int_gam3, double *ex1, double *ex2)
+{
+    int rc;
+    System *s = get_mult(handle, rc);
+    if (!s) return rc;
+    i64 na = s->d.nebf1, np = s->d.npbf, n3 = np * np * na * na * na * na * na * na;
+    DevBuf A, B, C;
+    rc = A.alloc(n3 * 8);
+    if (!rc) rc = B.alloc(n3 * 8);
+    if (!rc) rc = C.alloc(n3 * 8);
+    if (!rc) rc = neorx_rxchfmult_gam3_ic1ex_dev(handle, A.d(), B.d(), C.d(), 0);
+    if (!rc) rc = copy_out(int_gam3, A.d(), n3);
+    if (!rc) rc = copy_out(ex1, B.d(), n3);
+    if (!rc) rc = copy_out(ex2, C.d(), n3);
+    return rc;
 }
 
 extern "C" int neorx_rxchfmult_gam3_int(int64_t handle, double *int_gam3)
@@ -2798,6 +2955,9 @@ __global__ void __launch_bounds__(256) k_mult_weights(int k, i64 nb, i64 nc, con
     } else if (k == 2) {
         i64 s = c % nb, r = (c / nb) % nb, q = (c / (nb * nb)) % nb, p = c / (nb * nb * nb);
         w = D(p, q) * D(r, s) - 0.5 * D(p, s) * D(r, q);
+    } else if (k == 4) { // DBE(i2,j2) DBE(i3,j3): the exchange forms, exchange already in the tensor
+        i64 c2 = c % (nb * nb), c1 = c / (nb * nb);
+        w = D(c1 / nb, c1 % nb) * D(c2 / nb, c2 % nb);
     } else {
         i64 nb2 = nb * nb, c3 = c % nb2, c2 = (c / nb2) % nb2, c1 = c / (nb2 * nb2);
         w = D(c1 / nb, c1 % nb) * D(c2 / nb, c2 % nb) * D(c3 / nb, c3 % nb);
@@ -2825,7 +2985,7 @@ __global__ void __launch_bounds__(256) k_mult_rows(i64 na, i64 np, i64 nrows, i6
     if (threadIdx.x < MULT_ROWS) rs[threadIdx.x] = 0.0;
     __syncthreads();
     bool on = c < ncols;
-    double w = on ? W[c] : 0.0, h = 0.0;
+    double w = (on && W) ? W[c] : 0.0, h = 0.0;
     const bool lead = (threadIdx.x & 63) == 0;
     i64 na2 = na * na;
     for (i64 k = 0; k < nr; ++k) {
@@ -2833,11 +2993,13 @@ __global__ void __launch_bounds__(256) k_mult_rows(i64 na, i64 np, i64 nrows, i6
         double g = on ? G[R * ncols + c] : 0.0;
         double wr = DP[(pp % np) * np + pp / np] * DAE[(a % na) * na + a / na];
         h += wr * g;
-        double v = wave_sum(w * g);
-        if (lead) atomicAdd(&rs[k], v);
+        if (rowsum) {
+            double v = wave_sum(w * g);
+            if (lead) atomicAdd(&rs[k], v);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < nr) atomicAdd(&rowsum[R0 + threadIdx.x], rs[threadIdx.x]);
+    if (rowsum && threadIdx.x < nr) atomicAdd(&rowsum[R0 + threadIdx.x], rs[threadIdx.x]);
     if (HH && on) atomicAdd(&HH[c], h);
 }
 
@@ -2876,12 +3038,14 @@ __global__ void __launch_bounds__(256) k_mult_finish(int k, i64 na, i64 nb, i64 
                 for (i64 j3 = 0; j3 < nb; ++j3)
                     f += dbe(i3, j3) * (HH[t * nb2 + i3 * nb + j3] - 0.5 * HH[(i2 * nb + j3) * nb2 + i3 * nb + j2]);
             f *= 2.0;
-        } else {
+        } else if (k == 3) {
             for (i64 c = 0; c < nb2 * nb2; ++c) {
                 i64 c3 = c % nb2, c2 = c / nb2;
                 f += dbe(c2 / nb, c2 % nb) * dbe(c3 / nb, c3 % nb) * HH[t * nb2 * nb2 + c];
             }
             f *= 3.0;
+        } else { // k == 4: the FBE tensor of the exchange form carries its factors
+            for (i64 c = 0; c < nb2; ++c) f += dbe(c / nb, c % nb) * HH[t * nb2 + c];
         }
         FBE[j2 * nb + i2] += f;
     }
@@ -2890,6 +3054,42 @@ __global__ void __launch_bounds__(256) k_mult_finish(int k, i64 na, i64 nb, i64 
         for (i64 pp = 0; pp < np2; ++pp) e += dp(pp) * fp_of(pp);
         *E = e;
     }
+}
+
+// One pass (two for the exchange form of order 2, where FBE reads its own
+// tensor) over the device tensors: mode 1..3 = order k of the plain forms,
+// mode 4 = the exchange form of order 2 (G: the FAE/FP/E tensor, K: the FBE
+// tensor, both built by k_omg3ex_combine).
+static int mult_omg_core(int mode, i64 na, i64 nb, i64 np, const double *d_dae, const double *d_dbe,
+                         const double *d_dp, const double *G, const double *K, double *d_fae, double *d_fp,
+                         double *d_fbe, double *d_e, hipStream_t st)
+{
+    int k = mode == 4 ? 2 : mode;
+    i64 nrows = np * np * na * na, ncols = 1;
+    for (int i = 0; i < k; ++i) ncols *= nb * nb;
+    StreamBuf ws;
+    int rc = ws.alloc((nrows + 2 * ncols) * 8, st);
+    if (rc) return rc;
+    double *rowsum = (double *)ws.p, *W = rowsum + nrows, *HH = d_fbe ? W + ncols : nullptr;
+    hipLaunchKernelGGL(k_fill0, dim3(nblocks(nrows, 256)), dim3(256), 0, st, nrows, rowsum);
+    if (HH) hipLaunchKernelGGL(k_fill0, dim3(nblocks(ncols, 256)), dim3(256), 0, st, ncols, HH);
+    hipLaunchKernelGGL(k_mult_weights, dim3(nblocks(ncols, 256)), dim3(256), 0, st, mode, nb, ncols, d_dbe, W);
+    HIPCHK(hipGetLastError());
+    unsigned nt = (unsigned)std::min<i64>(256, ((ncols + 63) / 64) * 64);
+    i64 ntile = (ncols + nt - 1) / nt, nblk = ntile * ((nrows + MULT_ROWS - 1) / MULT_ROWS);
+    if (nblk > 0x7fffffff) return fail(NEORX_E_UNSUPPORTED, "tensor too large for one launch");
+    bool sep = K != G;
+    hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae, d_dp, W,
+                       G, rowsum, sep ? nullptr : HH);
+    if (sep && HH)
+        hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae,
+                           d_dp, nullptr, K, nullptr, HH);
+    HIPCHK(hipGetLastError());
+    i64 nf = std::max(std::max(na * na, np * np), nb * nb);
+    hipLaunchKernelGGL(k_mult_finish, dim3(nblocks(nf, 256)), dim3(256), 0, st, mode, na, nb, np, d_dae, d_dbe, d_dp,
+                       rowsum, HH, d_fae, d_fp, d_fbe, d_e);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
 }
 
 extern "C" int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, int64_t npbf, const double *d_dae,
@@ -2901,28 +3101,77 @@ extern "C" int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, 
     if (k < 1 || k > 3) return fail(NEORX_E_ARG, "order %lld not in 1..3", (long long)k);
     if (nebf <= 0 || nebfbe <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
     if (!d_dae || !d_dbe || !d_dp || !d_gm || !d_e) return fail(NEORX_E_ARG, "null argument");
-    i64 na = nebf, nb = nebfbe, np = npbf, nrows = np * np * na * na, ncols = 1;
-    for (int i = 0; i < k; ++i) ncols *= nb * nb;
-    hipStream_t st = (hipStream_t)stream;
-    StreamBuf ws;
-    rc = ws.alloc((nrows + 2 * ncols) * 8, st);
+    return mult_omg_core((int)k, nebf, nebfbe, npbf, d_dae, d_dbe, d_dp, d_gm, d_gm, d_fae, d_fp, d_fbe, d_e,
+                         (hipStream_t)stream);
+}
+
+// thread_F*_OMG2ex (rxchfmult_omg2ex.f:94-180): val = G(i1 j1 i2 j2) - G2ex(i1 j2 i2 j1)/2,
+// then the contractions of the plain order-1 form
+__global__ void __launch_bounds__(256) k_omg2ex_combine(i64 ne, i64 n, const double *__restrict__ G,
+                                                        const double *__restrict__ Gx, double *__restrict__ H)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = (t / (ne * ne * ne)) % ne,
+        pp = t / (ne * ne * ne * ne);
+    H[t] = G[t] - 0.5 * Gx[(((pp * ne + i1) * ne + j2) * ne + i2) * ne + j1];
+}
+
+// thread_F*_OMG3ex (rxchfmult_omg3ex.f:101-216, :316-431, :531-663): per entry
+//   val = G(123) - G(1,i2 j3,i3 j2)/2
+//   v2  = X1(i1 j2,i2 j1,i3 j3) - X1(i1 j3,i2 j1,i3 j2)/2 + X2(i1 j3,i2 j2,i3 j1) - X2(i1 j2,i2 j3,i3 j1)/2
+//   v3  = X1(i1 j3,i3 j1,i2 j2) - X1(i1 j2,i3 j1,i2 j3)/2 + X2(i1 j2,i3 j3,i2 j1) - X2(i1 j3,i3 j2,i2 j1)/2
+//   H = val - v2/2 (FAE, FP and every energy), K = 2 val - (v2 + v3)/2 (FBE)
+__global__ void __launch_bounds__(256) k_omg3ex_combine(i64 ne, i64 n, const double *__restrict__ G,
+                                                        const double *__restrict__ X1, const double *__restrict__ X2,
+                                                        double *__restrict__ H, double *__restrict__ K)
+{
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    i64 r = t;
+    i64 j3 = r % ne; r /= ne;
+    i64 i3 = r % ne; r /= ne;
+    i64 j2 = r % ne; r /= ne;
+    i64 i2 = r % ne; r /= ne;
+    i64 j1 = r % ne; r /= ne;
+    i64 i1 = r % ne, pp = r / ne;
+    auto ix = [&](i64 a, i64 b, i64 c, i64 d, i64 e, i64 f) {
+        return (((((pp * ne + a) * ne + b) * ne + c) * ne + d) * ne + e) * ne + f;
+    };
+    double val = G[t] - 0.5 * G[ix(i1, j1, i2, j3, i3, j2)];
+    double v2 = X1[ix(i1, j2, i2, j1, i3, j3)] - 0.5 * X1[ix(i1, j3, i2, j1, i3, j2)] + X2[ix(i1, j3, i2, j2, i3, j1)] -
+                0.5 * X2[ix(i1, j2, i2, j3, i3, j1)];
+    double v3 = X1[ix(i1, j3, i3, j1, i2, j2)] - 0.5 * X1[ix(i1, j2, i3, j1, i2, j3)] + X2[ix(i1, j2, i3, j3, i2, j1)] -
+                0.5 * X2[ix(i1, j3, i3, j2, i2, j1)];
+    H[t] = val - 0.5 * v2;
+    K[t] = 2.0 * val - 0.5 * (v2 + v3);
+}
+
+// order 2 or 3 (OMG2ex, OMG3ex), one basis; d_gx2 only for order 3
+extern "C" int neorx_rxchfmult_omgex_dev(int64_t order, int64_t nebf, int64_t npbf, const double *d_dae,
+                                         const double *d_dbe, const double *d_dp, const double *d_gm,
+                                         const double *d_gx1, const double *d_gx2, double *d_fae, double *d_fp,
+                                         double *d_fbe, double *d_e, void *stream)
+{
+    int rc = ensure_init();
     if (rc) return rc;
-    double *rowsum = (double *)ws.p, *W = rowsum + nrows, *HH = d_fbe ? W + ncols : nullptr;
-    hipLaunchKernelGGL(k_fill0, dim3(nblocks(nrows, 256)), dim3(256), 0, st, nrows, rowsum);
-    if (HH) hipLaunchKernelGGL(k_fill0, dim3(nblocks(ncols, 256)), dim3(256), 0, st, ncols, HH);
-    hipLaunchKernelGGL(k_mult_weights, dim3(nblocks(ncols, 256)), dim3(256), 0, st, (int)k, nb, ncols, d_dbe, W);
+    if (order != 2 && order != 3) return fail(NEORX_E_ARG, "order %lld not 2 or 3", (long long)order);
+    if (nebf <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (!d_dae || !d_dbe || !d_dp || !d_gm || !d_gx1 || (order == 3 && !d_gx2) || !d_e)
+        return fail(NEORX_E_ARG, "null argument");
+    i64 ne = nebf, np = npbf, n = np * np * ne * ne * ne * ne * (order == 3 ? ne * ne : 1);
+    hipStream_t st = (hipStream_t)stream;
+    StreamBuf tb;
+    rc = tb.alloc(n * 8 * (order == 3 ? 2 : 1), st);
+    if (rc) return rc;
+    double *H = (double *)tb.p, *K = order == 3 ? H + n : H;
+    if (order == 2)
+        hipLaunchKernelGGL(k_omg2ex_combine, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, H);
+    else
+        hipLaunchKernelGGL(k_omg3ex_combine, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, d_gx2, H,
+                           K);
     HIPCHK(hipGetLastError());
-    unsigned nt = (unsigned)std::min<i64>(256, ((ncols + 63) / 64) * 64);
-    i64 ntile = (ncols + nt - 1) / nt, nblk = ntile * ((nrows + MULT_ROWS - 1) / MULT_ROWS);
-    if (nblk > 0x7fffffff) return fail(NEORX_E_UNSUPPORTED, "tensor too large for one launch");
-    hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae, d_dp, W,
-                       d_gm, rowsum, HH);
-    HIPCHK(hipGetLastError());
-    i64 nf = std::max(std::max(na * na, np * np), nb * nb);
-    hipLaunchKernelGGL(k_mult_finish, dim3(nblocks(nf, 256)), dim3(256), 0, st, (int)k, na, nb, np, d_dae, d_dbe, d_dp,
-                       rowsum, HH, d_fae, d_fp, d_fbe, d_e);
-    HIPCHK(hipGetLastError());
-    return NEORX_OK;
+    return mult_omg_core(order == 2 ? 1 : 4, ne, ne, np, d_dae, d_dbe, d_dp, H, K, d_fae, d_fp, d_fbe, d_e, st);
 }
 
 // host form of one reference routine: which = 0 (FAE), 1 (FP), 2 (FBE)
@@ -2950,6 +3199,54 @@ static int mult_omg_host(int k, int which, i64 na, i64 nb, i64 np, const double 
     HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
     return NEORX_OK;
 }
+
+static int mult_omgex_host(int order, int which, i64 ne, i64 np, const double *dae, const double *dbe, const double *dp,
+                           const double *gm, const double *gx1, const double *gx2, double *f, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    i64 ng = np * np * ne * ne * ne * ne * (order == 3 ? ne * ne : 1), nf = which == 1 ? np * np : ne * ne;
+    DevBuf dA, dB, dP, dG, dX1, dX2, dF, dE;
+    rc = h2d(dA, dae, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(dG, gm, ng);
+    if (!rc) rc = h2d(dX1, gx1, ng);
+    if (!rc && order == 3) rc = h2d(dX2, gx2, ng);
+    if (!rc) rc = h2d(dF, f, nf);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc)
+        rc = neorx_rxchfmult_omgex_dev(order, ne, np, dA.d(), dB.d(), dP.d(), dG.d(), dX1.d(),
+                                       order == 3 ? dX2.d() : nullptr, which == 0 ? dF.d() : nullptr,
+                                       which == 1 ? dF.d() : nullptr, which == 2 ? dF.d() : nullptr, dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+#define NRX_MULT_OMG2EX(name, which)                                                                              \
+    extern "C" int name(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,       \
+                        const double *gm2, const double *gm2ex, double *f, double *e)                            \
+    {                                                                                                             \
+        return mult_omgex_host(2, which, nebf, npbf, dae, dbe, dp, gm2, gm2ex, nullptr, f, e);                   \
+    }
+#define NRX_MULT_OMG3EX(name, which)                                                                              \
+    extern "C" int name(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,       \
+                        const double *gm3, const double *gm3ex1, const double *gm3ex2, double *f, double *e)     \
+    {                                                                                                             \
+        return mult_omgex_host(3, which, nebf, npbf, dae, dbe, dp, gm3, gm3ex1, gm3ex2, f, e);                   \
+    }
+NRX_MULT_OMG2EX(neorx_rxchfmult_fae_omg2ex, 0)
+NRX_MULT_OMG2EX(neorx_rxchfmult_fp_omg2ex, 1)
+NRX_MULT_OMG2EX(neorx_rxchfmult_fbe_omg2ex, 2)
+NRX_MULT_OMG3EX(neorx_rxchfmult_fae_omg3ex, 0)
+NRX_MULT_OMG3EX(neorx_rxchfmult_fp_omg3ex, 1)
+NRX_MULT_OMG3EX(neorx_rxchfmult_fbe_omg3ex, 2)
+#undef NRX_MULT_OMG2EX
+#undef NRX_MULT_OMG3EX
 
 #define NRX_MULT_OMG(name, k, which)                                                                              \
     extern "C" int name(int64_t nebf, int64_t nebfbe, int64_t npbf, const double *dae, const double *dbe,        \

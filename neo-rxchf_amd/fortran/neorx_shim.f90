@@ -215,6 +215,54 @@ module neorx_capi
       real(c_double) :: gi(*), gx(*), gs(*)
       integer(c_int) :: neorx_rxchfmult_gam2
     end function
+    function neorx_rxchfmult_gam2_ic1ex(h, g1, g2, g3, gs) bind(C, name="neorx_rxchfmult_gam2_ic1ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g1(*), g2(*), g3(*), gs(*)
+      integer(c_int) :: neorx_rxchfmult_gam2_ic1ex
+    end function
+    function neorx_rxchfmult_gam3_ic1ex(h, g2, g3, g4) bind(C, name="neorx_rxchfmult_gam3_ic1ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g2(*), g3(*), g4(*)
+      integer(c_int) :: neorx_rxchfmult_gam3_ic1ex
+    end function
+    function neorx_rxchfmult_fae_omg2ex(ne, np, da, db, dp, g, gx, f, e) bind(C, name="neorx_rxchfmult_fae_omg2ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fae_omg2ex
+    end function
+    function neorx_rxchfmult_fae_omg3ex(ne, np, da, db, dp, g, gx1, gx2, f, e) bind(C, name="neorx_rxchfmult_fae_omg3ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx1(*), gx2(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fae_omg3ex
+    end function
+    function neorx_rxchfmult_fp_omg2ex(ne, np, da, db, dp, g, gx, f, e) bind(C, name="neorx_rxchfmult_fp_omg2ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fp_omg2ex
+    end function
+    function neorx_rxchfmult_fp_omg3ex(ne, np, da, db, dp, g, gx1, gx2, f, e) bind(C, name="neorx_rxchfmult_fp_omg3ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx1(*), gx2(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fp_omg3ex
+    end function
+    function neorx_rxchfmult_fbe_omg2ex(ne, np, da, db, dp, g, gx, f, e) bind(C, name="neorx_rxchfmult_fbe_omg2ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fbe_omg2ex
+    end function
+    function neorx_rxchfmult_fbe_omg3ex(ne, np, da, db, dp, g, gx1, gx2, f, e) bind(C, name="neorx_rxchfmult_fbe_omg3ex")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g(*), gx1(*), gx2(*), f(*), e
+      integer(c_int) :: neorx_rxchfmult_fbe_omg3ex
+    end function
     function neorx_rxchfmult_gam1(h, g, gs) bind(C, name="neorx_rxchfmult_gam1")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: h
@@ -939,4 +987,106 @@ subroutine RXCHFmult_FBE_OMG4(Nchunks, nebf, nebfBE, npbf, ng, DAE, DBE, DP, GM,
   real(c_double) :: DAE(nebf, nebf), DBE(nebfBE, nebfBE), DP(npbf, npbf), GM(ng), F(nebfBE, nebfBE), E
   call neorx_check(neorx_rxchfmult_fbe_omg4(nebf, nebfBE, npbf, DAE, DBE, DP, GM, F, E), &
                    'RXCHFmult_FBE_OMG4')
+end subroutine
+
+! RXCHFmult_GAM2_IC1ex, source/rxchfmult/rxchfmult_gam2ex.f:2-229 (one basis:
+! the handle takes the regular basis for both electrons)
+subroutine RXCHFmult_GAM2_IC1ex(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, &
+                                gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, &
+                                ELCBFC, NUCBFC, GM2_1ICR, GM2_2ICR, GM2_3ICR, GM2sICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM2_1ICR(ng2), GM2_2ICR(ng2), GM2_3ICR(ng2), GM2sICR(ng2)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_mult_system_create(nebf, npebf, nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, &
+                   gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR, KPEEND, AMPEB2C, &
+                   AGEBFCC, ELCEX, ELCAM, ELCBFC, AGNBFCC, NUCEX, NUCAM, NUCBFC, h), 'RXCHFmult_GAM2_IC1ex')
+  call neorx_check(neorx_rxchfmult_gam2_ic1ex(h, GM2_1ICR, GM2_2ICR, GM2_3ICR, GM2sICR), 'RXCHFmult_GAM2_IC1ex')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM2_IC1ex')
+end subroutine
+
+! RXCHFmult_GAM3_IC1ex, source/rxchfmult/rxchfmult_gam3ex.f:2-316
+subroutine RXCHFmult_GAM3_IC1ex(Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1, pmass, cat, zan, bcoef1, &
+                                gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, &
+                                ELCBFC, NUCBFC, GM3_2ICR, GM3_3ICR, GM3_4ICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM3_2ICR(ng3), GM3_3ICR(ng3), GM3_4ICR(ng3)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_mult_system_create(nebf, npebf, nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, &
+                   gamma1, KPESTR, KPEEND, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC, KPESTR, KPEEND, AMPEB2C, &
+                   AGEBFCC, ELCEX, ELCAM, ELCBFC, AGNBFCC, NUCEX, NUCAM, NUCBFC, h), 'RXCHFmult_GAM3_IC1ex')
+  call neorx_check(neorx_rxchfmult_gam3_ic1ex(h, GM3_2ICR, GM3_3ICR, GM3_4ICR), 'RXCHFmult_GAM3_IC1ex')
+  call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM3_IC1ex')
+end subroutine
+
+! RXCHFmult_FAE_OMG2ex, source/rxchfmult/rxchfmult_omg2ex.f:2-93
+subroutine RXCHFmult_FAE_OMG2ex(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2ICR, GM2exICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2exICR(ng2), F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fae_omg2ex(nebf, npbf, DAE, DBE, DP, GM2ICR, GM2exICR, F, E), &
+                   'RXCHFmult_FAE_OMG2ex')
+end subroutine
+
+! RXCHFmult_FAE_OMG3ex, source/rxchfmult/rxchfmult_omg3ex.f:2-100
+subroutine RXCHFmult_FAE_OMG3ex(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), GM3ex1ICR(ng3), GM3ex2ICR(ng3)
+  real(c_double) :: F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fae_omg3ex(nebf, npbf, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E), &
+                   'RXCHFmult_FAE_OMG3ex')
+end subroutine
+
+! RXCHFmult_FP_OMG2ex, source/rxchfmult/rxchfmult_omg2ex.f:181-272
+subroutine RXCHFmult_FP_OMG2ex(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2ICR, GM2exICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2exICR(ng2), F(npbf, npbf), E
+  call neorx_check(neorx_rxchfmult_fp_omg2ex(nebf, npbf, DAE, DBE, DP, GM2ICR, GM2exICR, F, E), &
+                   'RXCHFmult_FP_OMG2ex')
+end subroutine
+
+! RXCHFmult_FP_OMG3ex, source/rxchfmult/rxchfmult_omg3ex.f:217-315
+subroutine RXCHFmult_FP_OMG3ex(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), GM3ex1ICR(ng3), GM3ex2ICR(ng3)
+  real(c_double) :: F(npbf, npbf), E
+  call neorx_check(neorx_rxchfmult_fp_omg3ex(nebf, npbf, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E), &
+                   'RXCHFmult_FP_OMG3ex')
+end subroutine
+
+! RXCHFmult_FBE_OMG2ex, source/rxchfmult/rxchfmult_omg2ex.f:360-451
+subroutine RXCHFmult_FBE_OMG2ex(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2ICR, GM2exICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2ICR(ng2), GM2exICR(ng2), F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fbe_omg2ex(nebf, npbf, DAE, DBE, DP, GM2ICR, GM2exICR, F, E), &
+                   'RXCHFmult_FBE_OMG2ex')
+end subroutine
+
+! RXCHFmult_FBE_OMG3ex, source/rxchfmult/rxchfmult_omg3ex.f:432-530
+subroutine RXCHFmult_FBE_OMG3ex(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3ICR(ng3), GM3ex1ICR(ng3), GM3ex2ICR(ng3)
+  real(c_double) :: F(nebf, nebf), E
+  call neorx_check(neorx_rxchfmult_fbe_omg3ex(nebf, npbf, DAE, DBE, DP, GM3ICR, GM3ex1ICR, GM3ex2ICR, F, E), &
+                   'RXCHFmult_FBE_OMG3ex')
 end subroutine

@@ -106,6 +106,14 @@ def load():
         for n in ("fae", "fp", "fbe"):
             _sig(f"neorx_rxchfmult_{n}_omg{k}", [_I64] * 3 + [_P] * 6)
     _sig("neorx_rxchfmult_omg_dev", [_I64] * 4 + [_P] * 8 + [_P])
+    _sig("neorx_rxchfmult_gam2_ic1ex", [_I64] + [_P] * 4)
+    _sig("neorx_rxchfmult_gam2_ic1ex_dev", [_I64] + [_P] * 4 + [_P])
+    _sig("neorx_rxchfmult_gam3_ic1ex", [_I64] + [_P] * 3)
+    _sig("neorx_rxchfmult_gam3_ic1ex_dev", [_I64] + [_P] * 3 + [_P])
+    for n in ("fae", "fp", "fbe"):
+        _sig(f"neorx_rxchfmult_{n}_omg2ex", [_I64, _I64] + [_P] * 7)
+        _sig(f"neorx_rxchfmult_{n}_omg3ex", [_I64, _I64] + [_P] * 8)
+    _sig("neorx_rxchfmult_omgex_dev", [_I64] * 3 + [_P] * 10 + [_P])
     return _lib
 
 
@@ -213,6 +221,21 @@ class System:
         g = _np.zeros(ne ** 6 * npb * npb)
         _check(_lib.neorx_rxchfmult_gam3_xchf(self.handle, _p(g)))
         return g
+
+    def RXCHFmult_GAM2_IC1ex(self):
+        """GM2_1ICR, GM2_2ICR, GM2_3ICR, GM2sICR (rxchfmult/rxchfmult_gam2ex.f:2);
+        the two bases of the handle must be the same."""
+        na, npb = self.s["nebf"], self.s["npbf"]
+        out = [_np.zeros(npb * npb * na ** 4) for _ in range(4)]
+        _check(_lib.neorx_rxchfmult_gam2_ic1ex(self.handle, *[_p(a) for a in out]))
+        return out
+
+    def RXCHFmult_GAM3_IC1ex(self):
+        """INT_GAM3, INT_GAM3ex1, INT_GAM3ex2 (rxchfmult/rxchfmult_gam3ex.f:2)."""
+        na, npb = self.s["nebf"], self.s["npbf"]
+        out = [_np.zeros(npb * npb * na ** 6) for _ in range(3)]
+        _check(_lib.neorx_rxchfmult_gam3_ic1ex(self.handle, *[_p(a) for a in out]))
+        return out
 
     def RXCHFmult_GAM3_INT(self):
         """INT_GAM3 (rxchfmult/rxchfmult_gam3-int.f:2)."""
@@ -473,4 +496,18 @@ def RXCHFmult_OMG(which, k, nebf, nebfBE, npbf, DAE, DBE, DP, GM, F):
     name = f"neorx_rxchfmult_{('fae', 'fp', 'fbe')[which]}_omg{k}"
     _check(getattr(_lib, name)(nebf, nebfBE, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)),
                                _p(_np.ascontiguousarray(GM)), _p(F), _p(e)))
+    return float(e[0])
+
+
+def RXCHFmult_OMGex(which, order, nebf, npbf, DAE, DBE, DP, GM, GX1, GX2, F):
+    """RXCHFmult_{FAE,FP,FBE}_OMG{2,3}ex (rxchfmult/rxchfmult_omg{2,3}ex.f):
+    which 0/1/2, order 2 or 3 (GX2 only for 3).  F accumulates; returns E."""
+    load()
+    e = _np.zeros(1)
+    name = f"neorx_rxchfmult_{('fae', 'fp', 'fbe')[which]}_omg{order}ex"
+    args = [_p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(GM)),
+            _p(_np.ascontiguousarray(GX1))]
+    if order == 3:
+        args.append(_p(_np.ascontiguousarray(GX2)))
+    _check(getattr(_lib, name)(nebf, npbf, *args, _p(F), _p(e)))
     return float(e[0])

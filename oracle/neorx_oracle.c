@@ -681,7 +681,7 @@ static void gam1_prim(const sys_t *S, const prim_t *e1b, const prim_t *pb, const
  * geminals on e2-p (gamA23, gamB23, :1067-1068), and the outputs
  * ansE1 (:1149-1155) in ansE, ansE2 = gVEPg2 + gVEEg1 (:1156), ansS = gsg (:1158). */
 static void gam2_prim_impl(const sys_t *S, const prim_t *b, const prim_t *k, int mult, double *ansE, double *ansS,
-                           double *ansE2)
+                           double *ansE2, double *ansE3)
 {
     const prim_t *b1 = &b[0], *b2 = &b[1], *b3 = &b[2], *k1 = &k[0], *k2 = &k[1], *k3 = &k[2];
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
@@ -761,16 +761,19 @@ static void gam2_prim_impl(const sys_t *S, const prim_t *b, const prim_t *k, int
                 2.0 * (gVEPg1 + gVECg1) + gTEg2 + gVEPg2 + gVECg2 + 2.0 * gVEEg1 * 0.5 + 2.0 * gVEEg2 * 0.5;
     *ansS = gsg;
     if (ansE2) *ansE2 = gVEPg2 + gVEEg1;
+    if (ansE3) *ansE3 = gVEEg2; /* RXCHFmult_xcalc_GAM2_MDex (rxchfmult_gam2ex.f:559-1097) */
 }
 
 static void gam2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE, double *ansS)
 {
-    gam2_prim_impl(S, b, k, 0, ansE, ansS, NULL);
+    gam2_prim_impl(S, b, k, 0, ansE, ansS, NULL, NULL);
 }
 
 /* RXCHFmult_INT_GAM3_MD (rxchfmult/rxchfmult_gam3-int.f:661-1054): particles
- * e1 (regular basis), e2, e3 (special basis), p. */
-static void gam3int_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
+ * e1 (regular basis), e2, e3 (special basis), p.  ans3, ans4 (may be NULL):
+ * the extra outputs of RXCHFmult_xcalc_GAM3_MDex (rxchfmult_gam3ex.f:712-1111). */
+static void gam3int_prim_ex(const sys_t *S, const prim_t *b, const prim_t *k, double *ans, double *ans3,
+                            double *ans4)
 {
     double gVEEg1 = 0, gVEEg2 = 0, gVEPg1 = 0;
     for (i64 ik = 0; ik < S->ngtg; ++ik)
@@ -796,6 +799,13 @@ static void gam3int_prim(const sys_t *S, const prim_t *b, const prim_t *k, doubl
             gVEPg1 = gVEPg1 + (bb * xgVEPg1);
         }
     *ans = gVEPg1 + gVEEg1 + 2.0 * gVEEg2;
+    if (ans3) *ans3 = 2.0 * gVEEg2;
+    if (ans4) *ans4 = gVEEg1;
+}
+
+static void gam3int_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
+{
+    gam3int_prim_ex(S, b, k, ans, NULL, NULL);
 }
 
 /* RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): two outputs,
@@ -973,9 +983,9 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
         else if (cls == 4)
             rxne_prim(Sp, b, k, &e, &s);
         else if (cls == 5)
-            gam2_prim_impl(Sp, b, k, 1, &e, &s, &e2);
+            gam2_prim_impl(Sp, b, k, 1, &e, &s, &e2, NULL);
         else if (cls == 6)
-            gam3int_prim(Sp, b, k, &e);
+            gam3int_prim_ex(Sp, b, k, &e, &s, &e2);
         else if (cls == 7)
             gam3xchf_prim(Sp, b, k, &e);
         else if (cls == 8)
@@ -1994,7 +2004,7 @@ void orc_rxchfmult_gam2(MBAS_ARGS, SYS_ARGS, double *INT_GAM2, double *XCHF_GAM2
                                             prim_t b[3] = {eprim(&B, a1), eprim(&BB, a2), pprim(&B, ip)};
                                             prim_t k[3] = {eprim(&B, c1), eprim(&BB, c2), pprim(&B, jp)};
                                             double e1, e2, sv;
-                                            gam2_prim_impl(&S, b, k, 1, &e1, &sv, &e2);
+                                            gam2_prim_impl(&S, b, k, 1, &e1, &sv, &e2, NULL);
                                             double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
                                                         agebfcc_be[a2] * agebfcc_be[c2];
                                             o1 += e1 * cf;
@@ -2193,6 +2203,208 @@ void orc_rxchfmult_omg(i64 order, i64 which, i64 nebf, i64 nebfBE, i64 npbf, con
                                         }
                                 }
                         }
+#undef DAm
+#undef DBm
+#undef DPm
+    for (i64 t = 0; t < nf; ++t) F[t] += XF[t];
+    *E = e;
+    free(XF);
+}
+
+/* ---- RXCHFmult with exchange (LADDEXCH): one basis ---------------- */
+/* RXCHFmult_GAM2_IC1ex (rxchfmult_gam2ex.f:2-229) with
+ * RXCHFmult_contract_omega2_convex (:354-558): GM2_1ICR, GM2_3ICR, GM2sICR =
+ * (x12 + x21)/2 of ansE1, ansE3, ansS; GM2_2ICR = ansE2 (:160-186). */
+void orc_rxchfmult_gam2_ic1ex(BAS_ARGS, SYS_ARGS, double *GM2_1, double *GM2_2, double *GM2_3, double *GM2s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng = np * np * ne * ne * ne * ne;
+    double *X1 = (double *)malloc(sizeof(double) * ng), *X2 = (double *)malloc(sizeof(double) * ng),
+           *X3 = (double *)malloc(sizeof(double) * ng), *Xs = (double *)malloc(sizeof(double) * ng);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            double o1 = 0, o2 = 0, o3 = 0, os = 0;
+                            for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                        for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2) {
+                                            prim_t b[3] = {eprim(&B, a1), eprim(&B, a2), pprim(&B, ip)};
+                                            prim_t k[3] = {eprim(&B, c1), eprim(&B, c2), pprim(&B, jp)};
+                                            double e1, e2, e3, sv;
+                                            gam2_prim_impl(&S, b, k, 1, &e1, &sv, &e2, &e3);
+                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                        agebfcc[a2] * agebfcc[c2];
+                                            o1 += e1 * cf;
+                                            o2 += e2 * cf;
+                                            o3 += e3 * cf;
+                                            os += sv * cf;
+                                        }
+                            i64 ia = ix2(ne, np, ip, jp, i1, j1, i2, j2);
+                            X1[ia] = o1;
+                            X2[ia] = o2;
+                            X3[ia] = o3;
+                            Xs[ia] = os;
+                        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i2, j2, i1, j1);
+                            GM2_1[a12] = (X1[a12] + X1[a21]) / 2.0;
+                            GM2_2[a12] = X2[a12];
+                            GM2_3[a12] = (X3[a12] + X3[a21]) / 2.0;
+                            GM2s[a12] = (Xs[a12] + Xs[a21]) / 2.0;
+                        }
+    free(X1);
+    free(X2);
+    free(X3);
+    free(Xs);
+}
+
+/* RXCHFmult_GAM3_IC1ex (rxchfmult_gam3ex.f:2-316) with
+ * RXCHFmult_contract_omega3_convex (:459-711) and the pair averages (:248-269) */
+void orc_rxchfmult_gam3_ic1ex(BAS_ARGS, SYS_ARGS, double *GM3_2, double *GM3_3, double *GM3_4)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng = np * np * ne * ne * ne * ne * ne * ne;
+    double *X2 = (double *)malloc(sizeof(double) * ng), *X3 = (double *)malloc(sizeof(double) * ng),
+           *X4 = (double *)malloc(sizeof(double) * ng);
+#pragma omp parallel for collapse(3) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    double o2 = 0, o3 = 0, o4 = 0;
+                                    for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                        for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                            for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                                for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2)
+                                                    for (i64 a3 = kpestr[i3] - 1; a3 < kpeend[i3]; ++a3)
+                                                        for (i64 c3 = kpestr[j3] - 1; c3 < kpeend[j3]; ++c3) {
+                                                            prim_t b[4] = {eprim(&B, a1), eprim(&B, a2), eprim(&B, a3),
+                                                                           pprim(&B, ip)};
+                                                            prim_t k[4] = {eprim(&B, c1), eprim(&B, c2), eprim(&B, c3),
+                                                                           pprim(&B, jp)};
+                                                            double v2, v3, v4;
+                                                            gam3int_prim_ex(&S, b, k, &v2, &v3, &v4);
+                                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] *
+                                                                        agebfcc[c1] * agebfcc[a2] * agebfcc[c2] *
+                                                                        agebfcc[a3] * agebfcc[c3];
+                                                            o2 += v2 * cf;
+                                                            o3 += v3 * cf;
+                                                            o4 += v4 * cf;
+                                                        }
+                                    i64 ia = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3);
+                                    X2[ia] = o2;
+                                    X3[ia] = o3;
+                                    X4[ia] = o4;
+                                }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    i64 a123 = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3);
+                                    i64 a132 = ix3(ne, np, ip, jp, i1, j1, i3, j3, i2, j2);
+                                    i64 a213 = ix3(ne, np, ip, jp, i2, j2, i1, j1, i3, j3);
+                                    i64 a321 = ix3(ne, np, ip, jp, i3, j3, i2, j2, i1, j1);
+                                    double x = (X2[a123] + X2[a132]) * 0.5;
+                                    GM3_2[a123] = x;
+                                    GM3_2[a132] = x;
+                                    x = (X3[a123] + X3[a213]) * 0.5;
+                                    GM3_3[a123] = x;
+                                    GM3_3[a213] = x;
+                                    x = (X4[a123] + X4[a321]) * 0.5;
+                                    GM3_4[a123] = x;
+                                    GM3_4[a321] = x;
+                                }
+    free(X2);
+    free(X3);
+    free(X4);
+}
+
+/* RXCHFmult_thread_{FAE,FP,FBE}_OMG2ex (rxchfmult_omg2ex.f:94-180, :273-359,
+ * :452-538) and add2fock; which 0 FAE, 1 FP, 2 FBE. */
+void orc_rxchfmult_omg2ex(i64 which, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                          const double *G, const double *Gx, double *F, double *E)
+{
+    i64 ne = nebf, np = npbf, nf = which == 1 ? np * np : ne * ne;
+    double *XF = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+#define DAm(i, j) DAE[(j) * ne + (i)]
+#define DBm(i, j) DBE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            double val = G[ix2(ne, np, ip, jp, i1, j1, i2, j2)] -
+                                         0.5 * Gx[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+                            if (which == 0) XF[j1 * ne + i1] += DPm(ip, jp) * DBm(i2, j2) * val;
+                            if (which == 1) XF[jp * np + ip] += DAm(i1, j1) * DBm(i2, j2) * val;
+                            if (which == 2) XF[j2 * ne + i2] += DAm(i1, j1) * DPm(ip, jp) * val;
+                            e += DPm(ip, jp) * DAm(i1, j1) * DBm(i2, j2) * val;
+                        }
+    for (i64 t = 0; t < nf; ++t) F[t] += XF[t];
+    *E = e;
+    free(XF);
+}
+
+/* RXCHFmult_thread_{FAE,FP,FBE}_OMG3ex (rxchfmult_omg3ex.f:101-216, :316-431,
+ * :531-663) and add2fock. */
+void orc_rxchfmult_omg3ex(i64 which, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                          const double *G, const double *G1, const double *G2, double *F, double *E)
+{
+    i64 ne = nebf, np = npbf, nf = which == 1 ? np * np : ne * ne;
+    double *XF = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+#define X(A, a, b, c, d, f, g) A[ix3(ne, np, ip, jp, a, b, c, d, f, g)]
+                                    double val = X(G, i1, j1, i2, j2, i3, j3) - 0.5 * X(G, i1, j1, i2, j3, i3, j2);
+                                    double d = DPm(ip, jp);
+                                    if (which != 2) {
+                                        double val2 = X(G1, i1, j2, i2, j1, i3, j3) - 0.5 * X(G1, i1, j3, i2, j1, i3, j2);
+                                        double val3 = X(G2, i1, j3, i2, j2, i3, j1) - 0.5 * X(G2, i1, j2, i2, j3, i3, j1);
+                                        double v = val - 0.5 * (val2 + val3);
+                                        if (which == 0) XF[j1 * ne + i1] += v * DBm(i2, j2) * DBm(i3, j3) * d;
+                                        else XF[jp * np + ip] += v * DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3);
+                                        e += d * DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) * v;
+                                    } else {
+                                        double val2 = X(G1, i1, j2, i2, j1, i3, j3) - 0.5 * X(G1, i1, j3, i2, j1, i3, j2) +
+                                                      X(G2, i1, j3, i2, j2, i3, j1) - 0.5 * X(G2, i1, j2, i2, j3, i3, j1);
+                                        double val3 = X(G1, i1, j3, i3, j1, i2, j2) - 0.5 * X(G1, i1, j2, i3, j1, i2, j3) +
+                                                      X(G2, i1, j2, i3, j3, i2, j1) - 0.5 * X(G2, i1, j3, i3, j2, i2, j1);
+                                        XF[j2 * ne + i2] += (2.0 * val - 0.5 * (val2 + val3)) * DAm(i1, j1) *
+                                                            DBm(i3, j3) * d;
+                                        e += d * DAm(i1, j1) * DBm(i2, j2) * DBm(i3, j3) * (val - 0.5 * val2);
+                                    }
+#undef X
+                                }
 #undef DAm
 #undef DBm
 #undef DPm

@@ -745,3 +745,70 @@ def ref_rxchfmult_gam4_xchf(na, nb, npb, g2s, gee, workdir):
                                                         _i8(npb * npb * nb ** 4), _i8(g4.size),
                                                         _p(np.ascontiguousarray(g2s)), _p(g4)))
     return g4
+
+
+def rxchfmult_gam2_ic1ex(s):
+    """GM2_1ICR, GM2_2ICR, GM2_3ICR, GM2sICR of RXCHFmult_GAM2_IC1ex (one basis)."""
+    ne, npb = s["nebf"], s["npbf"]
+    out = [np.zeros(ne ** 4 * npb * npb) for _ in range(4)]
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam2_ic1ex(*ba, *sa, *[_p(a) for a in out])
+    return out
+
+
+def rxchfmult_gam3_ic1ex(s):
+    ne, npb = s["nebf"], s["npbf"]
+    out = [np.zeros(ne ** 6 * npb * npb) for _ in range(3)]
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchfmult_gam3_ic1ex(*ba, *sa, *[_p(a) for a in out])
+    return out
+
+
+def rxchfmult_omgex(order, which, ne, npb, DAE, DBE, DP, G, GX1, GX2, F):
+    e = np.zeros(1)
+    if order == 2:
+        lib().orc_rxchfmult_omg2ex(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                                   _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(GX1)), _p(F), _p(e))
+    else:
+        lib().orc_rxchfmult_omg3ex(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                                   _p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(GX1)),
+                                   _p(np.ascontiguousarray(GX2)), _p(F), _p(e))
+    return float(e[0])
+
+
+def ref_rxchfmult_gam2_ic1ex(s):
+    """RXCHFmult_GAM2_IC1ex (source/rxchfmult/rxchfmult_gam2ex.f:2)."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng2 = ne ** 4 * npb * npb
+    out = [np.zeros(ng2) for _ in range(4)]
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfmult_gam2_ic1ex_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng2), _i8(0), _i8(s["nat"]),
+                                _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm),
+                                *[_p(a) for a in keep], *[_p(a) for a in out])
+    return out
+
+
+def ref_rxchfmult_gam3_ic1ex(s):
+    """RXCHFmult_GAM3_IC1ex (source/rxchfmult/rxchfmult_gam3ex.f:2)."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng3 = ne ** 6 * npb * npb
+    out = [np.zeros(ng3) for _ in range(3)]
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchfmult_gam3_ic1ex_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng3), _i8(0), _i8(s["nat"]),
+                                _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm),
+                                *[_p(a) for a in keep], *[_p(a) for a in out])
+    return out
+
+
+def ref_rxchfmult_omgex(order, which, ne, npb, DAE, DBE, DP, G, GX1, GX2, F):
+    e = C.c_double()
+    fn = getattr(ref(), f"rxchfmult_{('fae', 'fp', 'fbe')[which]}_omg{order}ex_")
+    gs = [_p(np.ascontiguousarray(G)), _p(np.ascontiguousarray(GX1))]
+    if order == 3:
+        gs.append(_p(np.ascontiguousarray(GX2)))
+    fn(_i8(1), _i8(ne), _i8(npb), _i8(np.asarray(G).size), _f(DAE), _f(DBE), _f(DP), *gs, _p(F), C.byref(e))
+    return e.value

@@ -22,6 +22,7 @@ program drive_shim
   real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:), dae(:,:), dbe(:,:)
   integer(c_int64_t), allocatable :: kpestr(:), kpeend(:), ampeb2c(:), elcam(:), nucam(:)
   real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:), g1(:)
+  real(c_double), allocatable :: gx1(:), gx2(:), gx3(:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -186,6 +187,37 @@ program drive_shim
                            kpeend, ampeb2c, agebfcc, elcex, elcam, elcbfc, agnbfcc, nucex, nucam, nucbfc, gm3)
   call RXCHFmult_GAM4_XCHF(one, nebf, nebf, npbf, ngee, ng2, ng4, gm2s, gm4)
   write(11) gm3, gm4
+  ! RXCHFmult with exchange (LADDEXCH, rxchfmult_driver.f:384, :470;
+  ! rxchfmult_fock-int.f:366, :379, :459, :472, :552, :565): one basis
+  allocate(gx1(ng2), gx2(ng2))
+  call RXCHFmult_GAM2_IC1ex(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                            kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm2, &
+                            gm2s, gx1, gx2)
+  write(11) gm2, gm2s, gx1, gx2
+  fe = 0.0d0
+  call RXCHFmult_FAE_OMG2ex(one, nebf, npbf, ng2, dae, dbe, dp, gm2s, gx1, fe, e2)
+  write(11) fe, e2
+  fp = 0.0d0
+  call RXCHFmult_FP_OMG2ex(one, nebf, npbf, ng2, dae, dbe, dp, gm2s, gx1, fp, e2)
+  write(11) fp, e2
+  fe = 0.0d0
+  call RXCHFmult_FBE_OMG2ex(one, nebf, npbf, ng2, dae, dbe, dp, gm2s, gx1, fe, e2)
+  write(11) fe, e2
+  deallocate(gm4)
+  allocate(gm4(ng3), gx3(ng3))
+  call RXCHFmult_GAM3_IC1ex(one, nebf, npebf, npbf, ng3, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                            kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm3, &
+                            gm4, gx3)
+  write(11) gm3, gm4, gx3
+  fe = 0.0d0
+  call RXCHFmult_FAE_OMG3ex(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gm4, gx3, fe, e3)
+  write(11) fe, e3
+  fp = 0.0d0
+  call RXCHFmult_FP_OMG3ex(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gm4, gx3, fp, e3)
+  write(11) fp, e3
+  fe = 0.0d0
+  call RXCHFmult_FBE_OMG3ex(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gm4, gx3, fe, e3)
+  write(11) fe, e3
   close(11)
 contains
   subroutine read_ufm(name, n, a)

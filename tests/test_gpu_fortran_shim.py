@@ -89,7 +89,11 @@ def test_fortran_dropin_matches_oracle():
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # ... _OMG3
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # ... _OMG4
              ne * ne * npb * npb, ne * ne * npb * npb,  # GAM_1 of GAM1_OMP_MD, of RXCHFmult_GAM1_MD
-             ng3, ng4]  # RXCHFmult_GAM3_XCHF, RXCHFmult_GAM4_XCHF (in the driver gm2s holds XCHF_GAM2s)
+             ng3, ng4,  # RXCHFmult_GAM3_XCHF, RXCHFmult_GAM4_XCHF (in the driver gm2s holds XCHF_GAM2s)
+             ng2, ng2, ng2, ng2,  # RXCHFmult_GAM2_IC1ex
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG2ex
+             ng3, ng3, ng3,  # RXCHFmult_GAM3_IC1ex
+             ne * ne, 1, npb * npb, 1, ne * ne, 1]  # RXCHFmult_FAE/FP/FBE_OMG3ex
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -195,3 +199,22 @@ def test_fortran_dropin_matches_oracle():
     check_close(parts[80], oracle.rxchfmult_gam1(s)[0], what="RXCHFmult_GAM1_MD")
     check_close(parts[81], oracle.rxchfmult_gam3_xchf(s), what="RXCHFmult_GAM3_XCHF")
     check_close(parts[82], oracle.gam4_icr(ne, npb, mxs, oee), what="RXCHFmult_GAM4_XCHF")
+    # RXCHFmult with exchange (one basis)
+    x2 = oracle.rxchfmult_gam2_ic1ex(s)
+    for k, what in enumerate(("GM2_1ICR", "GM2_2ICR", "GM2_3ICR", "GM2sICR")):
+        check_close(parts[83 + k], x2[k], what=f"RXCHFmult_GAM2_IC1ex {what}")
+    for which in range(3):
+        n = npb if which == 1 else ne
+        F = np.zeros((n, n), order="F")
+        e = oracle.rxchfmult_omgex(2, which, ne, npb, DAE, DBE, DP, x2[1], x2[2], None, F)
+        check_close(parts[87 + 2 * which], F.ravel(order="F"), what=f"RXCHFmult omg2ex {which}")
+        assert parts[88 + 2 * which][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    x3 = oracle.rxchfmult_gam3_ic1ex(s)
+    for k, what in enumerate(("INT_GAM3", "INT_GAM3ex1", "INT_GAM3ex2")):
+        check_close(parts[93 + k], x3[k], what=f"RXCHFmult_GAM3_IC1ex {what}")
+    for which in range(3):
+        n = npb if which == 1 else ne
+        F = np.zeros((n, n), order="F")
+        e = oracle.rxchfmult_omgex(3, which, ne, npb, DAE, DBE, DP, x3[0], x3[1], x3[2], F)
+        check_close(parts[96 + 2 * which], F.ravel(order="F"), what=f"RXCHFmult omg3ex {which}")
+        assert parts[97 + 2 * which][0] == pytest.approx(e, rel=1e-11, abs=1e-14)

@@ -484,3 +484,36 @@ def test_rxchfmult_xchf_side(torch_dev):
     eeb = np.ascontiguousarray(gee.reshape((na,) * 4)[:nb, :nb, :nb, :nb]).ravel()
     check_close(neorx.RXCHFmult_GAM4_XCHF(na, nb, npb, g2s, gee), oracle.gam4_icr(nb, npb, g2s, eeb),
                 what="RXCHFmult_GAM4_XCHF")
+
+
+def test_rxchfmult_exchange(torch_dev):
+    """RXCHFmult with exchange (one basis): RXCHFmult_GAM2_IC1ex,
+    RXCHFmult_GAM3_IC1ex and the FAE/FP/FBE_OMG2ex/_OMG3ex contractions."""
+    s = synth.small(seed=66, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 0), ngtg=2)
+    m = dict(s)
+    m["nebfBE"], m["npebfBE"] = s["nebf"], s["npebf"]
+    for k in synth.BE_KEYS:
+        m[k + "_be"] = s[k]
+    print("case rxchfmult GAM2_IC1ex / GAM3_IC1ex", flush=True)
+    M = neorx.System(m)
+    got2, ref2 = M.RXCHFmult_GAM2_IC1ex(), oracle.rxchfmult_gam2_ic1ex(s)
+    for g, r, what in zip(got2, ref2, ("GM2_1ICR", "GM2_2ICR", "GM2_3ICR", "GM2sICR")):
+        check_close(g, r, what=f"RXCHFmult_GAM2_IC1ex {what}")
+    got3, ref3 = M.RXCHFmult_GAM3_IC1ex(), oracle.rxchfmult_gam3_ic1ex(s)
+    for g, r, what in zip(got3, ref3, ("INT_GAM3", "INT_GAM3ex1", "INT_GAM3ex2")):
+        check_close(g, r, what=f"RXCHFmult_GAM3_IC1ex {what}")
+    M.close()
+    ne, npb = s["nebf"], s["npbf"]
+    rng = np.random.default_rng(15)
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    DA[0, 1] += 0.05
+    for order, (G, X1, X2) in ((2, (ref2[1], ref2[2], None)), (3, ref3)):
+        for which in (0, 1, 2):
+            print(f"case rxchfmult omg{order}ex which={which}", flush=True)
+            n = npb if which == 1 else ne
+            F0 = np.asfortranarray(rng.standard_normal((n, n)))
+            A, B = F0.copy(order="F"), F0.copy(order="F")
+            eg = neorx.RXCHFmult_OMGex(which, order, ne, npb, DA, DB, DP, G, X1, X2, A)
+            eo = oracle.rxchfmult_omgex(order, which, ne, npb, DA, DB, DP, G, X1, X2, B)
+            check_close(A - F0, B - F0, what=f"RXCHFmult omg{order}ex {which}")
+            assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)

@@ -401,3 +401,33 @@ def test_oracle_rxchfmult_xchf_side_vs_reference(tmp_path):
     eeb = np.ascontiguousarray(gee.reshape((na,) * 4)[:nb, :nb, :nb, :nb]).ravel()
     g4 = oracle.gam4_icr(nb, npb, g2s, eeb)
     check_close(g4, oracle.ref_rxchfmult_gam4_xchf(na, nb, npb, g2s, gee, str(tmp_path)), what="RXCHFmult_GAM4_XCHF")
+
+
+@needs_ref
+def test_oracle_rxchfmult_exchange_vs_reference():
+    """RXCHFmult with exchange (one basis): RXCHFmult_GAM2_IC1ex,
+    RXCHFmult_GAM3_IC1ex and the FAE/FP/FBE_OMG2ex and _OMG3ex contractions
+    of the oracle against the reference (source/rxchfmult/)."""
+    s = synth.small(seed=64, n_e_centres=1, e_shells=(0, 1), contract=(1,), p_shells=(0, 0), ngtg=2)
+    got, ref = oracle.rxchfmult_gam2_ic1ex(s), oracle.ref_rxchfmult_gam2_ic1ex(s)
+    for g, r, what in zip(got, ref, ("GM2_1ICR", "GM2_2ICR", "GM2_3ICR", "GM2sICR")):
+        check_close(g, r, what=f"RXCHFmult_GAM2_IC1ex {what}")
+    u = synth.small(seed=65, n_e_centres=1, e_shells=(0, 1), contract=(1,), p_shells=(0,), ngtg=2)
+    g3, r3 = oracle.rxchfmult_gam3_ic1ex(u), oracle.ref_rxchfmult_gam3_ic1ex(u)
+    for g, r, what in zip(g3, r3, ("INT_GAM3", "INT_GAM3ex1", "INT_GAM3ex2")):
+        check_close(g, r, what=f"RXCHFmult_GAM3_IC1ex {what}")
+    rng = np.random.default_rng(9)
+    ne, npb = 3, 2
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    DA[0, 1] += 0.05  # unsymmetric densities tell the index orders apart
+    for order in (2, 3):
+        n = npb * npb * ne ** (2 * order)
+        G, X1, X2 = (rng.standard_normal(n) for _ in range(3))
+        for which in (0, 1, 2):
+            m = npb if which == 1 else ne
+            F0 = np.asfortranarray(rng.standard_normal((m, m)))
+            A, B = F0.copy(order="F"), F0.copy(order="F")
+            eo = oracle.rxchfmult_omgex(order, which, ne, npb, DA, DB, DP, G, X1, X2, A)
+            er = oracle.ref_rxchfmult_omgex(order, which, ne, npb, DA, DB, DP, G, X1, X2, B)
+            assert eo == pytest.approx(er, rel=1e-12, abs=1e-13)
+            check_close(A - F0, B - F0, what=f"RXCHFmult omg{order}ex {which}")
