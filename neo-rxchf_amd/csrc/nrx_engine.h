@@ -259,21 +259,21 @@ NRX_HD void boys_ref(int M, double T, double *F, const double *tab)
 }
 
 // sum_k c_k F_k(T) for a polynomial c of degree M = c.n - 1
-NRX_HD double boys_dot(const PolyT<NB> &c, double T, const double *tab)
+template <int N> NRX_HD double boys_dot(const PolyT<N> &c, double T, const double *tab)
 {
     int M = c.n - 1;
     double s = 0.0;
     if (T > 20.0) {
         double f = 0.88622692545275801365 / sqrt(T), it = 1.0 / T;
         s = c.c[0] * f;
-        NRX_UNROLL for (int l = 1; l < NB; ++l) {
+        NRX_UNROLL for (int l = 1; l < N; ++l) {
             f = f * (l - 0.5) * it;
             s += c.c[l] * f;
         }
     } else {
         double f = boys_top(M, T, tab);
         double ex = exp(-T), t2 = 2.0 * T;
-        NRX_UNROLL for (int k = NB - 1; k >= 0; --k) {
+        NRX_UNROLL for (int k = N - 1; k >= 0; --k) {
             double fn = (t2 * f + ex) * (1.0 / (2 * k + 1));
             f = (k < M) ? fn : f;
             s += c.c[k] * f; // c.c[k] = 0 for k > M
@@ -597,7 +597,7 @@ template <int NL> NRX_HD double nuc_core(const Sys &s, int a, int d, PolyT<NL> &
 
 // sum_c zan[c] <1/|r_a - C_c|>, each centre's value flushed below 1e-100 when
 // flush is set (the reference applies underflow() per centre)
-template <int NL>
+template <int NL, int NBC = NB>
 NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const double *zan, bool flush, double pic = PI)
 {
     NRX_FENCE();
@@ -608,8 +608,8 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
     double tot = 0.0;
     for (int c = 0; c < nat; ++c) {
         // product over directions, sized as it grows: x*y at capacity
-        // 2NL-1 (capped at NB), then *z at NB
-        constexpr int N2 = (2 * NL - 1 < NB) ? 2 * NL - 1 : NB;
+        // 2NL-1 (capped at NBC), then *z at NBC
+        constexpr int N2 = (2 * NL - 1 < NBC) ? 2 * NL - 1 : NBC;
         double Dx = X[0] - cat[3 * c], Dy = X[1] - cat[3 * c + 1], Dz = X[2] - cat[3 * c + 2];
         PolyT<NL> Px, Py;
         hermite_coulomb(e[0], Dx, 0.0, w, Px);
@@ -617,7 +617,7 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
         PolyT<N2> cxy;
         p_mul(Px, Py, cxy);
         hermite_coulomb(e[2], Dz, 0.0, w, Px);
-        PolyT<NB> cc;
+        PolyT<NBC> cc;
         p_mul(cxy, Px, cc);
         double T = w * (((0.0 + Dx * Dx) + Dy * Dy) + Dz * Dz);
         double v = pre * boys_dot(cc, T, s.bt);
@@ -639,11 +639,11 @@ template <int N> NRX_HD void conv_add(const PolyT<N> &ea, const PolyT<N> &eb, Po
 }
 
 // Coulomb <1/|r_a - r_b|>, a < b
-template <int NL> NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
+template <int NL, int NBC = NB> NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
 {
     NRX_FENCE();
     int h = s.n - 1;
-    PolyT<NB> cc; // product over directions of the polynomials in u^2
+    PolyT<NBC> cc; // product over directions of the polynomials in u^2
     p_const(cc, 1.0);
     double D2 = 0.0, fac = Kprod(s);
     double wa = 1.0, wb = 1.0, gab = 0.0;
@@ -799,7 +799,7 @@ NRX_HD int tuple_degree(const PairRec *const *q, int n, bool kinetic)
 // Terms that depend on gamma_k + gamma_l only are summed over k <= l.
 // With MULT set: RXCHFmult_xcalc_GAM1_MD (rxchfmult/rxchfmult_gam1.f:399-840),
 // the same value without the one-geminal terms.
-template <int NL, bool MULT = false>
+template <int NL, int NBC, bool MULT = false>
 NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, double &ansE, double &ansS)
 {
     double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
@@ -817,9 +817,9 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
             tpgx += bv * kk[0];
             tpxg += bv * kk[1];
-            vcegx += bv * nuc_sum<NL>(s1, 0, S.nat, S.cat, S.zan, false);
-            vcpgx += bv * nuc_sum<NL>(s1, 1, S.nat, S.cat, S.zan, false);
-            vmgx += bv * coul<NL>(s1, 0, 1);
+            vcegx += bv * nuc_sum<NL, NBC>(s1, 0, S.nat, S.cat, S.zan, false);
+            vcpgx += bv * nuc_sum<NL, NBC>(s1, 1, S.nat, S.cat, S.zan, false);
+            vmgx += bv * coul<NL, NBC>(s1, 0, 1);
             ogx += bv * ovl<NL>(s1);
         }
         for (int l = 0; l < S.ngtg; ++l) {
@@ -830,9 +830,9 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             tpgg += bb * kin<NL>(s2r, 0, 1, gw, S.pmass, PI_G2);
             if (l < k) continue;
             double w = (l == k) ? bb : 2.0 * bb;
-            vcegg += w * nuc_sum<NL>(s2, 0, S.nat, S.cat, S.zan, false);
-            vcpgg += w * nuc_sum<NL>(s2, 1, S.nat, S.cat, S.zan, false);
-            vmgg += w * coul<NL>(s2, 0, 1);
+            vcegg += w * nuc_sum<NL, NBC>(s2, 0, S.nat, S.cat, S.zan, false);
+            vcpgg += w * nuc_sum<NL, NBC>(s2, 1, S.nat, S.cat, S.zan, false);
+            vmgg += w * coul<NL, NBC>(s2, 0, 1);
             ogg += w * ovl<NL>(s2);
         }
     }
@@ -849,7 +849,7 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
 // ansE = the two-geminal energy, ansE2 = gVEPg2 + gVEEg1, ansS = gsg; its
 // exchange form RXCHFmult_xcalc_GAM2_MDex (rxchfmult_gam2ex.f:559-1097) adds
 // ansE3 = gVEEg2.
-template <int NL, bool MULT = false>
+template <int NL, int NBC, bool MULT = false>
 NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
                       double &ansS, double &ansE2, double &ansE3)
 {
@@ -859,11 +859,11 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
         double gk = S.gamma[ik], bk = S.bcoef[ik];
         Sys s = mksys(S, 3, e1, e2, p, 0, 0.0, gk, 0); // g(e2,p)
         double xgTE = kin<NL>(s, 0, -1, 0.0, 1.0);
-        double xgVEC = -nuc_sum<NL>(s, 0, S.nat, S.cat, S.zan, true);
-        double xgVEP = coul<NL>(s, 0, 2);
+        double xgVEC = -nuc_sum<NL, NBC>(s, 0, S.nat, S.cat, S.zan, true);
+        double xgVEP = coul<NL, NBC>(s, 0, 2);
         underflow(xgVEP);
         xgVEP = -xgVEP;
-        double xgVEE = coul<NL>(s, 0, 1);
+        double xgVEE = coul<NL, NBC>(s, 0, 1);
         underflow(xgVEE);
         gTE += bk * xgTE;
         gVEC += bk * xgVEC;
@@ -885,12 +885,12 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             gTEg1 += bb * kk[0];
             gTEg2 += bb * kk[1];
             gTPg += bb * kin<NL>(sb, 2, 1, gl, S.pmass);
-            gVPCg += bb * nuc_sum<NL>(sb, 2, S.nat, S.cat, S.zan, true);
-            gVECg1 -= bb * nuc_sum<NL>(sb, 0, S.nat, S.cat, S.zan, true);
-            double v = coul<NL>(sb, 0, 2);
+            gVPCg += bb * nuc_sum<NL, NBC>(sb, 2, S.nat, S.cat, S.zan, true);
+            gVECg1 -= bb * nuc_sum<NL, NBC>(sb, 0, S.nat, S.cat, S.zan, true);
+            double v = coul<NL, NBC>(sb, 0, 2);
             underflow(v);
             gVEPg1 -= bb * v;
-            v = coul<NL>(sb, 0, 1);
+            v = coul<NL, NBC>(sb, 0, 1);
             underflow(v);
             gVEEg2 += bb * v;
             gsg += bb * ovl<NL>(sb);
@@ -899,11 +899,11 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             Sys s2 = mksys(S, 3, e1, e2, p, 0, 0.0, gk + gl, 0); // g(e2,p)=gk+gl
             Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0); // g(e1,p)=gk+gl
             gTEg3 += w * kin<NL>(s2, 0, -1, 0.0, 1.0);
-            gVECg2 -= w * nuc_sum<NL>(s2, 0, S.nat, S.cat, S.zan, true);
-            v = coul<NL>(s2, 0, 2);
+            gVECg2 -= w * nuc_sum<NL, NBC>(s2, 0, S.nat, S.cat, S.zan, true);
+            v = coul<NL, NBC>(s2, 0, 2);
             underflow(v);
             gVEPg2 -= w * v;
-            v = coul<NL>(MULT ? s2 : s1, 0, 1);
+            v = coul<NL, NBC>(MULT ? s2 : s1, 0, 1);
             underflow(v);
             gVEEg1 += w * v;
         }
@@ -914,12 +914,12 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
     ansE3 = gVEEg2;
 }
 
-template <int NL>
+template <int NL, int NBC>
 NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
                        double &ansS)
 {
     double unused, unused3;
-    gam2_core<NL, false>(S, e1, e2, p, ansE, ansS, unused, unused3);
+    gam2_core<NL, NBC, false>(S, e1, e2, p, ansE, ansS, unused, unused3);
 }
 
 // xcalc_GAM3_MD (gam_3_OMP.f:617-983).  With MULT set:
@@ -927,16 +927,16 @@ NRX_HD void gam2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
 // drops the one-geminal gVEE term and relabels the electrons of the other
 // terms; the relabelling is undone by the six-permutation average its
 // builder applies, so the terms are evaluated here with the labels above.
-template <int NL, bool MULT = false>
+template <int NL, int NBC, bool MULT = false>
 NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3, const PairRec *p,
                        double &ans)
 {
     // geminal-independent factors
     Sys s23 = mksys(S, 2, e2, e3, 0, 0, 0.0, 0, 0);
-    double vee23 = coul<NL>(s23, 0, 1);
+    double vee23 = coul<NL, NBC>(s23, 0, 1);
     underflow(vee23);
     Sys s3 = mksys(S, 1, e3, 0, 0, 0, 0, 0, 0);
-    double hcore = kin<NL>(s3, 0, -1, 0.0, 1.0) - nuc_sum<NL>(s3, 0, S.nat, S.cat, S.zan, false);
+    double hcore = kin<NL>(s3, 0, -1, 0.0, 1.0) - nuc_sum<NL, NBC>(s3, 0, S.nat, S.cat, S.zan, false);
     underflow(hcore);
     double gVEE = 0.0;
     for (int ik = 0; ik < (MULT ? 0 : S.ngtg); ++ik) {
@@ -953,12 +953,12 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
             Sys s3p = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
             double yy = ovl<NL>(s3p);
             Sys s4a = mksys(S, 4, e1, e2, e3, p, gk, gl, 0.0); // 1/r34, g14 g24
-            double zz = coul<NL>(s4a, 2, 3);
+            double zz = coul<NL, NBC>(s4a, 2, 3);
             underflow(yy);
             underflow(zz);
             gHEg += bb * ((hcore * yy) + (zz * (-1.0)));
             Sys s4b = mksys(S, 4, e1, e2, e3, p, gk, 0.0, gl); // 1/r12, g14 g34
-            gVEEg2 += bb * coul<NL>(s4b, 0, 1);
+            gVEEg2 += bb * coul<NL, NBC>(s4b, 0, 1);
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 2, e1, p, 0, 0, gk + gl, 0, 0);
@@ -976,7 +976,7 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
 // e-e-p tuple, two outputs summed over all (k,l):
 //   ansE1 = -<1/r(e2,p)> with g(e1,p) = gamma_k + gamma_l  +  <1/r12> with the same geminal
 //   ansE2 =  <1/r12> with g(e1,p) = gamma_k, g(e2,p) = gamma_l
-template <int NL>
+template <int NL, int NBC>
 NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE1,
                          double &ansE2)
 {
@@ -986,12 +986,12 @@ NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e
             double gk = S.gamma[ik], gl = S.gamma[il];
             double bb = S.bcoef[ik] * S.bcoef[il];
             Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0);
-            double vep = coul<NL>(s1, 1, 2);
+            double vep = coul<NL, NBC>(s1, 1, 2);
             underflow(vep);
-            double vee1 = coul<NL>(s1, 0, 1);
+            double vee1 = coul<NL, NBC>(s1, 0, 1);
             underflow(vee1);
             Sys s2 = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
-            double vee2 = coul<NL>(s2, 0, 1);
+            double vee2 = coul<NL, NBC>(s2, 0, 1);
             underflow(vee2);
             gVEPg2 += bb * (-vep);
             gVEEg1 += bb * vee1;
@@ -1009,12 +1009,12 @@ NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e
 //   ans = gVEPg1 + gVEEg1 + 2 gVEEg2
 // and, for its exchange form RXCHFmult_xcalc_GAM3_MDex (rxchfmult_gam3ex.f:712-1111),
 // ans3 = 2 gVEEg2, ans4 = gVEEg1
-template <int NL>
+template <int NL, int NBC>
 NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3,
                           const PairRec *p, double &ans, double &ans3, double &ans4)
 {
     Sys s13 = mksys(S, 2, e1, e3, 0, 0, 0.0, 0, 0);
-    double vee13 = coul<NL>(s13, 0, 1);
+    double vee13 = coul<NL, NBC>(s13, 0, 1);
     underflow(vee13);
     double gVEPg1 = 0, gVEEg1 = 0, gVEEg2 = 0;
     for (int ik = 0; ik < S.ngtg; ++ik)
@@ -1022,10 +1022,10 @@ NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *
             double gk = S.gamma[ik], gl = S.gamma[il];
             double bb = S.bcoef[ik] * S.bcoef[il];
             Sys s4 = mksys(S, 4, e1, e2, e3, p, 0.0, gk, gl);
-            double zz = coul<NL>(s4, 0, 3);
+            double zz = coul<NL, NBC>(s4, 0, 3);
             underflow(zz);
             gVEPg1 += bb * (zz * (-1.0));
-            gVEEg2 += bb * coul<NL>(s4, 0, 1);
+            gVEEg2 += bb * coul<NL, NBC>(s4, 0, 1);
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 2, e2, p, 0, 0, gk + gl, 0, 0);
@@ -1038,40 +1038,42 @@ NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *
     ans4 = gVEEg1;
 }
 
-// Polynomial capacities per tuple class.  GEN covers every tuple allowed by
-// l <= 2 (per particle and direction bra+ket powers <= 4): e-p 4+4+2,
-// e-e-p 3*4+2, e-e-e-p 4*4.  SMALL covers tuples with tuple_degree <= SMALL-1,
-// which is almost all of them; the rest are evaluated by a second launch.
+// Capacity tiers per tuple class: tier k evaluates with polynomial capacity
+// NL[k] and Boys-order capacity NB[k].  A tuple runs in the first tier that
+// covers its degree (tuple_degree <= NL-1) and its total angular momentum
+// (tuple_L <= NB-1: the degree in u^2 of every Coulomb polynomial); the last
+// tier covers every tuple allowed by l <= 2 (per particle and direction
+// bra+ket powers <= 4): e-p 4+4+2 and L <= 8, e-e-p 3*4+2 and L <= 12,
+// e-e-e-p 4*4 and L <= 16.  Capacities past a tuple's need only add zero
+// terms, so every tier gives the same values.
 template <int CLS> struct Tier;
 template <> struct Tier<1> {
-    static constexpr int SMALL = 11, GEN = 11;
+    static constexpr int N = 2;
+    static constexpr int NL[N] = {7, 11}, NB[N] = {9, 9};
     static constexpr bool KIN = true;
 };
 template <> struct Tier<2> {
-    static constexpr int SMALL = 9, GEN = 15;
+    static constexpr int N = 3;
+    static constexpr int NL[N] = {7, 9, 15}, NB[N] = {9, 13, 13};
     static constexpr bool KIN = true;
 };
 template <> struct Tier<3> {
-    static constexpr int SMALL = 9, GEN = 17;
+    static constexpr int N = 3;
+    static constexpr int NL[N] = {7, 9, 17}, NB[N] = {11, 17, 17};
     static constexpr bool KIN = false;
 };
 // class 4: the RXCHF-ne e-e-p tuple (Coulomb terms only: 3*4)
 template <> struct Tier<4> {
-    static constexpr int SMALL = 9, GEN = 13;
+    static constexpr int N = 3;
+    static constexpr int NL[N] = {7, 9, 13}, NB[N] = {9, 13, 13};
     static constexpr bool KIN = false;
 };
-// class 5: the RXCHFmult e-e-p tuple (two bases, kinetic terms as class 2)
-template <> struct Tier<5> {
-    static constexpr int SMALL = 9, GEN = 15;
-    static constexpr bool KIN = true;
-};
-// class 6: the RXCHFmult e-e-e-p interaction tuple (Coulomb and overlap: 4*4)
-template <> struct Tier<6> {
-    static constexpr int SMALL = 9, GEN = 17;
-    static constexpr bool KIN = false;
-};
+// class 5: the RXCHFmult e-e-p tuple (two bases, kinetic terms as class 2);
+// class 6: the RXCHFmult e-e-e-p interaction tuple (Coulomb and overlap: 4*4);
 // class 7: the RXCHFmult XCHF e-e-e-p tuple (as class 3 without gVEE);
 // class 8: the RXCHFmult e-p tuple (as class 1 without the one-geminal terms)
+template <> struct Tier<5> : Tier<2> {};
+template <> struct Tier<6> : Tier<3> {};
 template <> struct Tier<7> : Tier<3> {};
 template <> struct Tier<8> : Tier<1> {};
 constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
@@ -1083,56 +1085,75 @@ template <int CLS> constexpr int nel()
 }
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
-template <int CLS, int NL>
+template <int CLS, int NL, int NBC = NB>
 NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s)
 {
     s = 0.0;
     if constexpr (CLS == 1)
-        gam1_tuple<NL>(S, q[0], q[1], e, s);
+        gam1_tuple<NL, NBC>(S, q[0], q[1], e, s);
     else if constexpr (CLS == 8)
-        gam1_tuple<NL, true>(S, q[0], q[1], e, s);
+        gam1_tuple<NL, NBC, true>(S, q[0], q[1], e, s);
     else if constexpr (CLS == 7)
-        gam3_tuple<NL, true>(S, q[0], q[1], q[2], q[3], e);
+        gam3_tuple<NL, NBC, true>(S, q[0], q[1], q[2], q[3], e);
     else if constexpr (CLS == 2)
-        gam2_tuple<NL>(S, q[0], q[1], q[2], e, s);
+        gam2_tuple<NL, NBC>(S, q[0], q[1], q[2], e, s);
     else if constexpr (CLS == 4)
-        gam2ne_tuple<NL>(S, q[0], q[1], q[2], e, s);
+        gam2ne_tuple<NL, NBC>(S, q[0], q[1], q[2], e, s);
     else if constexpr (CLS == 5) {
         double x, y;
-        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x, y);
+        gam2_core<NL, NBC, true>(S, q[0], q[1], q[2], e, s, x, y);
     } else if constexpr (CLS == 6) {
         double x;
-        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e, s, x);
+        gam3int_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s, x);
     }
     else
-        gam3_tuple<NL>(S, q[0], q[1], q[2], q[3], e);
+        gam3_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e);
 }
 
 // four-output form: (ansE1, ansS, ansE2, ansE3) of class 5, (ans, ans3, ans4,
 // 0) of class 6; x and y are zero for the other classes
-template <int CLS, int NL>
+template <int CLS, int NL, int NBC = NB>
 NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x, double &y)
 {
     x = 0.0;
     y = 0.0;
     if constexpr (CLS == 5)
-        gam2_core<NL, true>(S, q[0], q[1], q[2], e, s, x, y);
+        gam2_core<NL, NBC, true>(S, q[0], q[1], q[2], e, s, x, y);
     else if constexpr (CLS == 6)
-        gam3int_tuple<NL>(S, q[0], q[1], q[2], q[3], e, s, x);
+        gam3int_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s, x);
     else
-        tuple_eval<CLS, NL>(S, q, e, s);
+        tuple_eval<CLS, NL, NBC>(S, q, e, s);
 }
 
-template <int CLS, int NL>
+template <int CLS, int NL, int NBC = NB>
 NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x)
 {
     double y;
-    tuple_eval<CLS, NL>(S, q, e, s, x, y);
+    tuple_eval<CLS, NL, NBC>(S, q, e, s, x, y);
 }
 
-template <int CLS> NRX_HD bool tuple_small(const PairRec *const *q)
+// total angular momentum of a tuple: bra+ket powers summed over particles
+// and directions
+NRX_HD int tuple_L(const PairRec *const *q, int n)
 {
-    return tuple_degree(q, nel<CLS>() + 1, Tier<CLS>::KIN) <= Tier<CLS>::SMALL - 1;
+    int L = 0;
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) L += q[i]->la[d] + q[i]->lb[d];
+    return L;
 }
+
+// the capacity tier a tuple runs in
+template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
+{
+    using T = Tier<CLS>;
+    int deg = tuple_degree(q, nel<CLS>() + 1, T::KIN), L = tuple_L(q, nel<CLS>() + 1);
+    for (int k = 0; k + 1 < T::N; ++k)
+        if (deg <= T::NL[k] - 1 && L <= T::NB[k] - 1) return k;
+    return T::N - 1;
+}
+
+// the general tier of a class (covers every tuple)
+template <int CLS> constexpr int tier_nl() { return Tier<CLS>::NL[Tier<CLS>::N - 1]; }
+template <int CLS> constexpr int tier_nb() { return Tier<CLS>::NB[Tier<CLS>::N - 1]; }
 
 } // namespace nrx

@@ -306,26 +306,11 @@ template <int CLS> NRX_DI void tuple_pairs(const DevSys &S, const i64 *x, const 
     q[E] = S.pp + x[E] * S.npbf + x[2 * E + 1];
 }
 
-template <int CLS, int NL> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
+template <int CLS, int NL, int NBC> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
 {
     const PairRec *q[nel<CLS>() + 1];
     tuple_pairs<CLS>(S, x, q);
-    tuple_eval<CLS, NL>(d_params(S), q, e, s);
-}
-
-template <int CLS, int NL>
-NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s, double &xx, double &yy)
-{
-    const PairRec *q[nel<CLS>() + 1];
-    tuple_pairs<CLS>(S, x, q);
-    tuple_eval<CLS, NL>(d_params(S), q, e, s, xx, yy);
-}
-
-template <int CLS> NRX_DI bool small_tuple(const DevSys &S, const i64 *x)
-{
-    const PairRec *q[nel<CLS>() + 1];
-    tuple_pairs<CLS>(S, x, q);
-    return tuple_small<CLS>(q);
+    tuple_eval<CLS, NL, NBC>(d_params(S), q, e, s);
 }
 
 // index tuple t of a list, validated against the basis sizes
@@ -386,76 +371,91 @@ template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *id
         range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
 }
 
-// First pass over a list / range: tuples that fit the small capacity are
-// evaluated; the others are appended to ovf (their position t) for the
-// general-capacity pass.  With SMALL == GEN every tuple is evaluated here.
+// Tuples of a launch are binned by capacity tier (tuple_tier): k_prim_bin
+// appends each tuple's position t to its tier's list, one atomic per wave and
+// tier, and flags out-of-range list entries; k_prim_tier<K> then evaluates
+// tier K's list with that tier's capacities, one lane per listed tuple, so
+// every wave runs at the capacity its tuples need.
 template <int CLS, int SRC>
-__global__ void __launch_bounds__(256) k_prim_small(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
-                                                    double *__restrict__ outE, double *__restrict__ outS,
-                                                    double *__restrict__ outX, double *__restrict__ outY,
-                                                    i64 *__restrict__ ovf, unsigned *__restrict__ novf)
+__global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
+                                                  double *__restrict__ outE, double *__restrict__ outS,
+                                                  double *__restrict__ outX, double *__restrict__ outY,
+                                                  unsigned *__restrict__ lists, unsigned *__restrict__ counts, i64 cap)
 {
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntup) return;
-    i64 x[2 * nel<CLS>() + 2];
-    if constexpr (SRC == SRC_LIST) {
-        if (!load_tuple<CLS>(S, idx, t, x)) { // out-of-range primitive index: flag the entry
+    int tier = -1;
+    if (t < ntup) {
+        i64 x[2 * nel<CLS>() + 2];
+        bool ok = true;
+        if constexpr (SRC == SRC_LIST)
+            ok = load_tuple<CLS>(S, idx, t, x);
+        else
+            range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
+        if (ok) {
+            const PairRec *q[nel<CLS>() + 1];
+            tuple_pairs<CLS>(S, x, q);
+            tier = tuple_tier<CLS>(q);
+        } else { // out-of-range primitive index: flag the entry
             outE[t] = __builtin_nan("");
             if (outS) outS[t] = __builtin_nan("");
             if (outX) outX[t] = __builtin_nan("");
             if (outY) outY[t] = __builtin_nan("");
-            return;
         }
-    } else {
-        range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
     }
-    constexpr int NS_ = Tier<CLS>::SMALL, NG = Tier<CLS>::GEN;
-    if constexpr (NS_ != NG) {
-        if (!small_tuple<CLS>(S, x)) {
-            unsigned k = atomicAdd(novf, 1u);
-            ovf[k] = t;
-            return;
-        }
+    const int lane = __lane_id();
+    for (int k = 0; k < Tier<CLS>::N; ++k) {
+        unsigned long long m = __ballot(tier == k);
+        if (m == 0ull) continue;
+        int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&counts[k], (unsigned)__popcll(m));
+        base = __shfl(base, leader);
+        if (tier == k) lists[k * cap + base + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned)t;
+    }
+}
+
+// Tier K of a launch: lane per listed tuple.  The lane's pair records are
+// staged in LDS, which the tuple evaluation reads many times.
+template <int CLS, int SRC, int K>
+__global__ void __launch_bounds__(256) k_prim_tier(DevSys S, i64 start, const i64 *__restrict__ idx,
+                                                   double *__restrict__ outE, double *__restrict__ outS,
+                                                   double *__restrict__ outX, double *__restrict__ outY,
+                                                   const unsigned *__restrict__ list, const unsigned *__restrict__ count)
+{
+    constexpr int NP = nel<CLS>() + 1;
+    __shared__ PairRec stage[NP][256];
+    i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (i64)*count) return;
+    i64 t = list[k];
+    i64 x[2 * NP];
+    src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by k_prim_bin
+    const PairRec *g[NP];
+    tuple_pairs<CLS>(S, x, g);
+    const PairRec *q[NP];
+    NRX_UNROLL for (int i = 0; i < NP; ++i) {
+        stage[i][threadIdx.x] = *g[i];
+        q[i] = &stage[i][threadIdx.x];
     }
     double e, s, xx, yy;
-    eval_tuple<CLS, NS_>(S, x, e, s, xx, yy);
+    tuple_eval<CLS, Tier<CLS>::NL[K], Tier<CLS>::NB[K]>(d_params(S), q, e, s, xx, yy);
     outE[t] = e;
     if (outS) outS[t] = s;
     if (outX) outX[t] = xx;
     if (outY) outY[t] = yy;
 }
 
-// Second pass: the tuples listed by the first, general capacity, grid-stride
-// over the device-side count (every lane exits when the list is done).
-template <int CLS, int SRC>
-__global__ void __launch_bounds__(256) k_prim_gen(DevSys S, i64 start, const i64 *__restrict__ idx,
-                                                  double *__restrict__ outE, double *__restrict__ outS,
-                                                  double *__restrict__ outX, double *__restrict__ outY,
-                                                  const i64 *__restrict__ ovf, const unsigned *__restrict__ novf)
+__global__ void k_zero_u32n(unsigned *p, int n)
 {
-    i64 n = *novf;
-    for (i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
-        i64 t = ovf[k];
-        i64 x[2 * nel<CLS>() + 2];
-        src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by the first pass
-        double e, s, xx, yy;
-        eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s, xx, yy);
-        outE[t] = e;
-        if (outS) outS[t] = s;
-        if (outX) outX[t] = xx;
-        if (outY) outY[t] = yy;
-    }
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
-
-__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 
 static inline unsigned nblocks(i64 n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// tuples per launch of the primitive kernels (bounds the overflow list)
+// tuples per launch of the primitive kernels (bounds the tier lists, whose
+// entries are 32-bit positions)
 static const i64 PRIM_CHUNK = (i64)1 << 24;
-static const unsigned GEN_BLOCKS = 2048;
 
-// stream-ordered scratch: the overflow list and its counter
+// stream-ordered scratch: the tier lists and their counters
 struct StreamBuf {
     void *p = nullptr;
     hipStream_t st = 0;
@@ -475,41 +475,44 @@ struct StreamBuf {
     }
 };
 
+template <int CLS, int SRC, int K>
+static int launch_tiers(const DevSys &d, i64 start, i64 n, const i64 *ix, double *e, double *s, double *x, double *y,
+                        const unsigned *lists, const unsigned *counts, i64 cap, hipStream_t st)
+{
+    if constexpr (K < Tier<CLS>::N) {
+        hipLaunchKernelGGL((k_prim_tier<CLS, SRC, K>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start, ix, e, s, x,
+                           y, lists + K * cap, counts + K);
+        HIPCHK(hipGetLastError());
+        return launch_tiers<CLS, SRC, K + 1>(d, start, n, ix, e, s, x, y, lists, counts, cap, st);
+    }
+    return NEORX_OK;
+}
+
 template <int CLS, int SRC>
 static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st,
                         double *ox = nullptr, double *oy = nullptr)
 {
-    constexpr bool tiered = Tier<CLS>::SMALL != Tier<CLS>::GEN;
+    constexpr int NT = Tier<CLS>::N;
     if constexpr (two_basis<CLS>())
         if (SRC != SRC_LIST && (d.npebf1 <= 0 || d.npebf1 >= d.npebf))
             return fail(NEORX_E_ARG, "class %d enumerates two electron bases; this handle has one", CLS);
+    if (ntup <= 0) return NEORX_OK;
+    i64 cap = std::min(ntup, PRIM_CHUNK);
     StreamBuf ws;
-    i64 *ovf = nullptr;
-    unsigned *cnt = nullptr;
-    if (tiered) {
-        i64 cap = std::min(ntup, PRIM_CHUNK);
-        int rc = ws.alloc(cap * sizeof(i64) + 256, st);
-        if (rc) return rc;
-        cnt = (unsigned *)ws.p;
-        ovf = (i64 *)((char *)ws.p + 256);
-    }
+    int rc = ws.alloc(NT * cap * sizeof(unsigned) + 256, st);
+    if (rc) return rc;
+    unsigned *counts = (unsigned *)ws.p, *lists = (unsigned *)((char *)ws.p + 256);
     for (i64 off = 0; off < ntup; off += PRIM_CHUNK) {
         i64 n = std::min(ntup - off, PRIM_CHUNK);
         const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * nel<CLS>() + 2) : nullptr;
         double *e = oe + off, *s = os ? os + off : nullptr, *x = ox ? ox + off : nullptr;
         double *y = oy ? oy + off : nullptr;
-        if (tiered) {
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, cnt);
-            HIPCHK(hipGetLastError());
-        }
-        hipLaunchKernelGGL((k_prim_small<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e,
-                           s, x, y, ovf, cnt);
+        hipLaunchKernelGGL(k_zero_u32n, dim3(1), dim3(64), 0, st, counts, NT);
+        hipLaunchKernelGGL((k_prim_bin<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e, s,
+                           x, y, lists, counts, cap);
         HIPCHK(hipGetLastError());
-        if (tiered) {
-            hipLaunchKernelGGL((k_prim_gen<CLS, SRC>), dim3(GEN_BLOCKS), dim3(256), 0, st, d, start + off, ix, e, s,
-                               x, y, ovf, cnt);
-            HIPCHK(hipGetLastError());
-        }
+        rc = launch_tiers<CLS, SRC, 0>(d, start + off, n, ix, e, s, x, y, lists, counts, cap, st);
+        if (rc) return rc;
     }
     return NEORX_OK;
 }
@@ -572,7 +575,7 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
 // contracted builders
 // ------------------------------------------------------------------
 // GAM1_OMP_MD: one lane per contracted (ip,jp,iec,jec), primitive sums in
-// the reference's loop order.  Every e-p tuple fits Tier<1>::GEN.
+// the reference's loop order, at the general tier of the class.
 // CLS 8: RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2-398), same loops.
 template <int CLS>
 __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, double *__restrict__ Gs)
@@ -586,7 +589,7 @@ __global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, 
         for (i64 je = S.kpestr[jc] - 1; je < S.kpeend[jc]; ++je) {
             i64 x[4] = {ie, ip, je, jp};
             double e, s;
-            eval_tuple<CLS, Tier<CLS>::GEN>(S, x, e, s);
+            eval_tuple<CLS, tier_nl<CLS>(), tier_nb<CLS>()>(S, x, e, s);
             double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[ie] * S.agebfcc[je];
             acc += e * cf;
             accs += s * cf;

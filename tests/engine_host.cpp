@@ -31,15 +31,25 @@ static Prim eprim(i64 npebf, const double *ex, const i64 *am, const double *cen,
     return p;
 }
 
-// tier: 0 = choose as the device does (small capacity when the tuple fits),
-// 1 = always the general capacity
+template <int CLS, int K>
+static void eval_k(const SysParams &S, const PairRec *const *q, int k, double &e, double &s, double &x)
+{
+    if constexpr (K < Tier<CLS>::N) {
+        if (k == K) {
+            tuple_eval<CLS, Tier<CLS>::NL[K], Tier<CLS>::NB[K]>(S, q, e, s, x);
+            return;
+        }
+        eval_k<CLS, K + 1>(S, q, k, e, s, x);
+    }
+}
+
+// tier: 0 = choose the capacity tier as the device does, 1 = always the
+// general (last) tier
 template <int CLS>
 static void eval_one(const SysParams &S, const PairRec *const *q, int tier, double &e, double &s, double &x)
 {
-    if (tier == 0 && tuple_small<CLS>(q))
-        tuple_eval<CLS, Tier<CLS>::SMALL>(S, q, e, s, x);
-    else
-        tuple_eval<CLS, Tier<CLS>::GEN>(S, q, e, s, x);
+    int k = tier == 0 ? tuple_tier<CLS>(q) : Tier<CLS>::N - 1;
+    eval_k<CLS, 0>(S, q, k, e, s, x);
 }
 
 static void pairs_of(i64 npebf, i64 npbf, const double *elcex, const double *nucex, const i64 *elcam, const i64 *nucam,
