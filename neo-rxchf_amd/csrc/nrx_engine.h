@@ -528,9 +528,13 @@ template <int NL> NRX_HD double kin(const Sys &s, int a, int c, double gk, doubl
 }
 
 // P(u^2) = sum_k hk R_k(u^2), R_0 = 1, R_1 = -2 lam D,
-// R_{k+1} = -2 lam (D R_k + k R_{k-1}), lam = l0 + l1 u^2
-template <int N> NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, PolyT<N> &P)
+// R_{k+1} = -2 lam (D R_k + k R_{k-1}), lam = l0 + l1 u^2.  The factor -2 is
+// carried in m0 = -2 l0, m1 = -2 l1 (an exact scaling); ZERO0 drops the l0
+// terms of the nuclear attraction, where l0 = 0.
+template <int N, bool ZERO0 = false>
+NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, PolyT<N> &P)
 {
+    double m0 = -2.0 * l0, m1 = -2.0 * l1;
     double Rm[N], R[N];
     NRX_UNROLL for (int j = 0; j < N; ++j) {
         R[j] = (j == 0) ? 1.0 : 0.0;
@@ -542,9 +546,9 @@ template <int N> NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, doubl
         NRX_UNROLL for (int j = 0; j <= k; ++j) t[j] = D * R[j] + k * Rm[j];
         double Rn[N];
         NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
-            double v = (j <= k) ? l0 * t[j] : 0.0;
-            if (j >= 1) v += l1 * t[j - 1];
-            Rn[j] = -2.0 * v;
+            double v = (!ZERO0 && j <= k) ? m0 * t[j] : 0.0;
+            if (j >= 1) v = (ZERO0 || j > k) ? m1 * t[j - 1] : v + m1 * t[j - 1];
+            Rn[j] = v;
         }
         NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
             Rm[j] = (j <= k) ? R[j] : 0.0;
@@ -612,11 +616,11 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
         constexpr int N2 = (2 * NL - 1 < NBC) ? 2 * NL - 1 : NBC;
         double Dx = X[0] - cat[3 * c], Dy = X[1] - cat[3 * c + 1], Dz = X[2] - cat[3 * c + 2];
         PolyT<NL> Px, Py;
-        hermite_coulomb(e[0], Dx, 0.0, w, Px);
-        hermite_coulomb(e[1], Dy, 0.0, w, Py);
+        hermite_coulomb<NL, true>(e[0], Dx, 0.0, w, Px);
+        hermite_coulomb<NL, true>(e[1], Dy, 0.0, w, Py);
         PolyT<N2> cxy;
         p_mul(Px, Py, cxy);
-        hermite_coulomb(e[2], Dz, 0.0, w, Px);
+        hermite_coulomb<NL, true>(e[2], Dz, 0.0, w, Px);
         PolyT<NBC> cc;
         p_mul(cxy, Px, cc);
         double T = w * (((0.0 + Dx * Dx) + Dy * Dy) + Dz * Dz);

@@ -373,8 +373,8 @@ template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *id
 
 // Tuples of a launch are binned by capacity tier (tuple_tier): k_prim_bin
 // appends each tuple's position t to its tier's list, one atomic per wave and
-// tier, and flags out-of-range list entries; k_prim_tier<K> then evaluates
-// tier K's list with that tier's capacities, one lane per listed tuple, so
+// tier, and flags out-of-range list entries; k_prim_tiers then evaluates
+// each tier's list with that tier's capacities, one lane per listed tuple, so
 // every wave runs at the capacity its tuples need.
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
@@ -414,26 +414,22 @@ __global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup,
     }
 }
 
-// Tier K of a launch: lane per listed tuple.  The lane's pair records are
-// staged in LDS, which the tuple evaluation reads many times.
+// One listed tuple of tier K: its pair records are staged in LDS (the
+// evaluation reads them many times; at one wave per SIMD, the register use
+// of these kernels, the LDS footprint costs no occupancy), then it is
+// evaluated with the tier's capacities.
 template <int CLS, int SRC, int K>
-__global__ void __launch_bounds__(256) k_prim_tier(DevSys S, i64 start, const i64 *__restrict__ idx,
-                                                   double *__restrict__ outE, double *__restrict__ outS,
-                                                   double *__restrict__ outX, double *__restrict__ outY,
-                                                   const unsigned *__restrict__ list, const unsigned *__restrict__ count)
+NRX_DI void tier_item(const DevSys &S, i64 start, const i64 *__restrict__ idx, double *__restrict__ outE,
+                      double *__restrict__ outS, double *__restrict__ outX, double *__restrict__ outY, i64 t,
+                      PairRec (*stage)[256])
 {
     constexpr int NP = nel<CLS>() + 1;
-    __shared__ PairRec stage[NP][256];
-    i64 k = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (i64)*count) return;
-    i64 t = list[k];
     i64 x[2 * NP];
     src_tuple<CLS, SRC>(S, idx, start, t, x); // a list was validated by k_prim_bin
-    const PairRec *g[NP];
-    tuple_pairs<CLS>(S, x, g);
     const PairRec *q[NP];
+    tuple_pairs<CLS>(S, x, q);
     NRX_UNROLL for (int i = 0; i < NP; ++i) {
-        stage[i][threadIdx.x] = *g[i];
+        stage[i][threadIdx.x] = *q[i];
         q[i] = &stage[i][threadIdx.x];
     }
     double e, s, xx, yy;
@@ -442,6 +438,40 @@ __global__ void __launch_bounds__(256) k_prim_tier(DevSys S, i64 start, const i6
     if (outS) outS[t] = s;
     if (outX) outX[t] = xx;
     if (outY) outY[t] = yy;
+}
+
+// block b of the fused launch: the tiers' lists are laid out from the last
+// tier down (its tuples take longest, so they start first)
+template <int CLS, int SRC, int K>
+NRX_DI void tier_dispatch(const DevSys &S, i64 start, const i64 *__restrict__ idx, double *__restrict__ outE,
+                          double *__restrict__ outS, double *__restrict__ outX, double *__restrict__ outY,
+                          const unsigned *__restrict__ lists, const unsigned *__restrict__ counts, i64 cap, i64 b,
+                          PairRec (*stage)[256])
+{
+    if constexpr (K >= 0) {
+        i64 c = counts[K], nb = (c + 255) / 256;
+        if (b < nb) {
+            i64 k = b * 256 + threadIdx.x;
+            if (k < c)
+                tier_item<CLS, SRC, K>(S, start, idx, outE, outS, outX, outY, (i64)lists[K * cap + k], stage);
+            return;
+        }
+        tier_dispatch<CLS, SRC, K - 1>(S, start, idx, outE, outS, outX, outY, lists, counts, cap, b - nb, stage);
+    }
+}
+
+// Every tier of a launch in one kernel, so that the tiers' tails overlap:
+// lane per listed tuple, blocks past the lists exit.
+template <int CLS, int SRC>
+__global__ void __launch_bounds__(256) k_prim_tiers(DevSys S, i64 start, const i64 *__restrict__ idx,
+                                                    double *__restrict__ outE, double *__restrict__ outS,
+                                                    double *__restrict__ outX, double *__restrict__ outY,
+                                                    const unsigned *__restrict__ lists,
+                                                    const unsigned *__restrict__ counts, i64 cap)
+{
+    __shared__ PairRec stage[nel<CLS>() + 1][256];
+    tier_dispatch<CLS, SRC, Tier<CLS>::N - 1>(S, start, idx, outE, outS, outX, outY, lists, counts, cap,
+                                               (i64)blockIdx.x, stage);
 }
 
 __global__ void k_zero_u32n(unsigned *p, int n)
@@ -475,19 +505,6 @@ struct StreamBuf {
     }
 };
 
-template <int CLS, int SRC, int K>
-static int launch_tiers(const DevSys &d, i64 start, i64 n, const i64 *ix, double *e, double *s, double *x, double *y,
-                        const unsigned *lists, const unsigned *counts, i64 cap, hipStream_t st)
-{
-    if constexpr (K < Tier<CLS>::N) {
-        hipLaunchKernelGGL((k_prim_tier<CLS, SRC, K>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start, ix, e, s, x,
-                           y, lists + K * cap, counts + K);
-        HIPCHK(hipGetLastError());
-        return launch_tiers<CLS, SRC, K + 1>(d, start, n, ix, e, s, x, y, lists, counts, cap, st);
-    }
-    return NEORX_OK;
-}
-
 template <int CLS, int SRC>
 static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st,
                         double *ox = nullptr, double *oy = nullptr)
@@ -511,8 +528,9 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
         hipLaunchKernelGGL((k_prim_bin<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e, s,
                            x, y, lists, counts, cap);
         HIPCHK(hipGetLastError());
-        rc = launch_tiers<CLS, SRC, 0>(d, start + off, n, ix, e, s, x, y, lists, counts, cap, st);
-        if (rc) return rc;
+        hipLaunchKernelGGL((k_prim_tiers<CLS, SRC>), dim3(nblocks(n, 256) + NT), dim3(256), 0, st, d, start + off, ix, e,
+                           s, x, y, lists, counts, cap);
+        HIPCHK(hipGetLastError());
     }
     return NEORX_OK;
 }
