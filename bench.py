@@ -261,8 +261,9 @@ def main():
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP64_PEAK_TFLOPS, 5),
                 "traffic": TRAFFIC.get(dom) if args.nsample == (1 << 20) else None,
-                "kernel": {1: "k_prim_small<1,SRC_SORTED>", 2: "k_prim_small<2,SRC_LIST> + k_prim_gen<2,SRC_LIST>",
-                           3: "k_prim_small<3,SRC_LIST> + k_prim_gen<3,SRC_LIST>"}[dom],
+                "kernel": {1: "k_prim_bin<1,SRC_SORTED> + k_prim_tiers<1,SRC_SORTED>",
+                           2: "k_prim_bin<2,SRC_LIST> + k_prim_tiers<2,SRC_LIST>",
+                           3: "k_prim_bin<3,SRC_LIST> + k_prim_tiers<3,SRC_LIST>"}[dom],
                 "flops_per_tuple": flops_per}
 
     fock = None
@@ -300,7 +301,7 @@ def main():
 # make nonzero (tests/count_flops.py); traffic = HBM bytes per launch of the
 # dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), scaled
 # by the copy-kernel calibration.
-FLOPS_PER_TUPLE = {1: 99762.1, 2: 251422.5, 3: 46648.4}
+FLOPS_PER_TUPLE = {1: 92740.7, 2: 231752.4, 3: 46532.5}
 TRAFFIC = {2: 131.9e6}  # e-e-p launch group at the default 1,048,576 sampled tuples
 
 if __name__ == "__main__":

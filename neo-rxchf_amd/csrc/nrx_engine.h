@@ -557,7 +557,7 @@ NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, 
         }
     }
     P.n = hk.n;
-    NRX_CNT(4 * hk.n * hk.n);
+    NRX_CNT((ZERO0 ? 3 : 4) * hk.n * hk.n);
 }
 
 NRX_HD void underflow(double &x)
@@ -787,7 +787,9 @@ NRX_HD Sys mksys(const SysParams &S, int n, const PairRec *p0, const PairRec *p1
 
 // polynomial capacity needed by a tuple: the largest per-direction sum of
 // bra+ket powers over its particles, plus 2 where a multi-particle kinetic
-// operator raises one polynomial (e-p and e-e-p classes)
+// operator raises one polynomial (e-p and e-e-p classes).  Only the kinetic
+// terms need the 2: the tuple functions of those classes evaluate overlap,
+// nuclear attraction and Coulomb terms at capacity NL - 2.
 NRX_HD int tuple_degree(const PairRec *const *q, int n, bool kinetic)
 {
     int m = 0;
@@ -806,6 +808,7 @@ NRX_HD int tuple_degree(const PairRec *const *q, int n, bool kinetic)
 template <int NL, int NBC, bool MULT = false>
 NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, double &ansE, double &ansS)
 {
+    constexpr int NC = NL - 2; // capacity of the terms without a kinetic operator
     double tegx = 0, tpgx = 0, texg = 0, tpxg = 0, tegg = 0, tpgg = 0;
     double vcegx = 0, vcegg = 0, vcpgx = 0, vcpgg = 0, vmgx = 0, vmgg = 0;
     double ogx = 0, ogg = 0;
@@ -821,10 +824,10 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
             tpgx += bv * kk[0];
             tpxg += bv * kk[1];
-            vcegx += bv * nuc_sum<NL, NBC>(s1, 0, S.nat, S.cat, S.zan, false);
-            vcpgx += bv * nuc_sum<NL, NBC>(s1, 1, S.nat, S.cat, S.zan, false);
-            vmgx += bv * coul<NL, NBC>(s1, 0, 1);
-            ogx += bv * ovl<NL>(s1);
+            vcegx += bv * nuc_sum<NC, NBC>(s1, 0, S.nat, S.cat, S.zan, false);
+            vcpgx += bv * nuc_sum<NC, NBC>(s1, 1, S.nat, S.cat, S.zan, false);
+            vmgx += bv * coul<NC, NBC>(s1, 0, 1);
+            ogx += bv * ovl<NC>(s1);
         }
         for (int l = 0; l < S.ngtg; ++l) {
             double gw = S.gamma[l], bb = bv * S.bcoef[l];
@@ -834,10 +837,10 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             tpgg += bb * kin<NL>(s2r, 0, 1, gw, S.pmass, PI_G2);
             if (l < k) continue;
             double w = (l == k) ? bb : 2.0 * bb;
-            vcegg += w * nuc_sum<NL, NBC>(s2, 0, S.nat, S.cat, S.zan, false);
-            vcpgg += w * nuc_sum<NL, NBC>(s2, 1, S.nat, S.cat, S.zan, false);
-            vmgg += w * coul<NL, NBC>(s2, 0, 1);
-            ogg += w * ovl<NL>(s2);
+            vcegg += w * nuc_sum<NC, NBC>(s2, 0, S.nat, S.cat, S.zan, false);
+            vcpgg += w * nuc_sum<NC, NBC>(s2, 1, S.nat, S.cat, S.zan, false);
+            vmgg += w * coul<NC, NBC>(s2, 0, 1);
+            ogg += w * ovl<NC>(s2);
         }
     }
     double te = tegx + texg + tegg, tp = tpgx + tpxg + tpgg;
@@ -857,17 +860,18 @@ template <int NL, int NBC, bool MULT = false>
 NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE,
                       double &ansS, double &ansE2, double &ansE3)
 {
+    constexpr int NC = NL - 2; // capacity of the terms without a kinetic operator
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
     if constexpr (!MULT)
     for (int ik = 0; ik < S.ngtg; ++ik) {
         double gk = S.gamma[ik], bk = S.bcoef[ik];
         Sys s = mksys(S, 3, e1, e2, p, 0, 0.0, gk, 0); // g(e2,p)
         double xgTE = kin<NL>(s, 0, -1, 0.0, 1.0);
-        double xgVEC = -nuc_sum<NL, NBC>(s, 0, S.nat, S.cat, S.zan, true);
-        double xgVEP = coul<NL, NBC>(s, 0, 2);
+        double xgVEC = -nuc_sum<NC, NBC>(s, 0, S.nat, S.cat, S.zan, true);
+        double xgVEP = coul<NC, NBC>(s, 0, 2);
         underflow(xgVEP);
         xgVEP = -xgVEP;
-        double xgVEE = coul<NL, NBC>(s, 0, 1);
+        double xgVEE = coul<NC, NBC>(s, 0, 1);
         underflow(xgVEE);
         gTE += bk * xgTE;
         gVEC += bk * xgVEC;
@@ -889,25 +893,25 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             gTEg1 += bb * kk[0];
             gTEg2 += bb * kk[1];
             gTPg += bb * kin<NL>(sb, 2, 1, gl, S.pmass);
-            gVPCg += bb * nuc_sum<NL, NBC>(sb, 2, S.nat, S.cat, S.zan, true);
-            gVECg1 -= bb * nuc_sum<NL, NBC>(sb, 0, S.nat, S.cat, S.zan, true);
-            double v = coul<NL, NBC>(sb, 0, 2);
+            gVPCg += bb * nuc_sum<NC, NBC>(sb, 2, S.nat, S.cat, S.zan, true);
+            gVECg1 -= bb * nuc_sum<NC, NBC>(sb, 0, S.nat, S.cat, S.zan, true);
+            double v = coul<NC, NBC>(sb, 0, 2);
             underflow(v);
             gVEPg1 -= bb * v;
-            v = coul<NL, NBC>(sb, 0, 1);
+            v = coul<NC, NBC>(sb, 0, 1);
             underflow(v);
             gVEEg2 += bb * v;
-            gsg += bb * ovl<NL>(sb);
+            gsg += bb * ovl<NC>(sb);
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 3, e1, e2, p, 0, 0.0, gk + gl, 0); // g(e2,p)=gk+gl
             Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0); // g(e1,p)=gk+gl
             gTEg3 += w * kin<NL>(s2, 0, -1, 0.0, 1.0);
-            gVECg2 -= w * nuc_sum<NL, NBC>(s2, 0, S.nat, S.cat, S.zan, true);
-            v = coul<NL, NBC>(s2, 0, 2);
+            gVECg2 -= w * nuc_sum<NC, NBC>(s2, 0, S.nat, S.cat, S.zan, true);
+            v = coul<NC, NBC>(s2, 0, 2);
             underflow(v);
             gVEPg2 -= w * v;
-            v = coul<NL, NBC>(MULT ? s2 : s1, 0, 1);
+            v = coul<NC, NBC>(MULT ? s2 : s1, 0, 1);
             underflow(v);
             gVEEg1 += w * v;
         }
