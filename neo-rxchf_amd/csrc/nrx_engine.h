@@ -356,34 +356,44 @@ template <int N> struct VarT {
 template <int NLF, int NQ>
 NRX_HD double absorb_leaf(const PolyT<NLF> &LF, double Lw, double Lc, double g, double d, VarT<NQ> &Q, double pic)
 {
-    double W = Lw + g;
+    // one reciprocal per new exponent, the quotients as products
+    double W = Lw + g, iW = 1.0 / W;
     PolyT<NLF> h;
-    p_smooth(LF, 0.5 / W, h);
-    double fac = sqrt(pic / W);
+    p_smooth(LF, 0.5 * iW, h);
+    double fac = sqrt(pic * iW);
     if (g == 0.0) {
         double v = p_eval(h, Lc);
         NRX_UNROLL for (int k = 0; k < NQ; ++k) Q.F.c[k] *= v;
         return fac;
     }
-    double beta = g / W, m0 = (Lw * Lc - g * d) / W;
+    double beta = g * iW, m0 = (Lw * Lc - g * d) * iW;
     p_compose(h, m0, beta, h);
     p_mul(Q.F, h, Q.F);
-    double kap = Lw * g / W;
+    double kap = Lw * beta;
     double cl = d + Lc;
-    double Wq = Q.w + kap;
+    double Wq = Q.w + kap, iWq = 1.0 / Wq;
     double diff = Q.c - cl;
-    fac *= exp(-(Q.w * kap / Wq) * diff * diff);
-    Q.c = (Q.w * Q.c + kap * cl) / Wq;
+    fac *= exp(-(Q.w * kap * iWq) * diff * diff);
+    Q.c = (Q.w * Q.c + kap * cl) * iWq;
     Q.w = Wq;
     NRX_CNT(26);
     return fac;
 }
 
+// E[F(y)] over exp(-w (y - c)^2) / sqrt(pi/w), given iw = 1/w
+template <int N> NRX_HD double close_expect(const PolyT<N> &F, double iw, double c)
+{
+    PolyT<N> h;
+    p_smooth(F, 0.5 * iw, h);
+    return p_eval(h, c);
+}
+
 template <int N> NRX_HD double close_var(const VarT<N> &V, double pic)
 {
     PolyT<N> h;
-    p_smooth(V.F, 0.5 / V.w, h);
-    return sqrt(pic / V.w) * p_eval(h, V.c);
+    double iw = 1.0 / V.w;
+    p_smooth(V.F, 0.5 * iw, h);
+    return sqrt(pic * iw) * p_eval(h, V.c);
 }
 
 // Integral description: particles 0..n-1, hub = n-1, electron i linked to the
@@ -443,21 +453,103 @@ template <int NL> NRX_HD double ovl(const Sys &s, double pic = PI)
     return r;
 }
 
+// Particle i's polynomial f (in its own coordinate) as a polynomial in the
+// hub coordinate y after i is integrated out (the polynomial part of
+// absorb_leaf): smooth(f, 1/(2W))(m0 + beta y), W = p_i + g_i, beta = g_i / W,
+// m0 = -beta (P_i - P_hub).  Linear in f; the hub's own polynomial maps to
+// itself.
+template <int NF, int NR> NRX_HD void hub_map(const Sys &s, int i, int d, const PolyT<NF> &f, PolyT<NR> &r)
+{
+    const int h = s.n - 1;
+    if (i == h) {
+        p_copy(f, r);
+        return;
+    }
+    const PairRec &Q = *s.q[i];
+    double g = s.gam[i], iW = 1.0 / (Q.p + g);
+    PolyT<NF> t;
+    p_smooth(f, 0.5 * iW, t);
+    if (g == 0.0) {
+        p_const(r, p_eval(t, 0.0));
+        return;
+    }
+    double beta = g * iW;
+    p_compose(t, -beta * (Q.P[d] - s.q[h]->P[d]), beta, t);
+    p_copy(t, r);
+}
+
+template <int N> NRX_HD void p_add(const PolyT<N> &a, const PolyT<N> &b, PolyT<N> &r)
+{
+    NRX_UNROLL for (int k = 0; k < N; ++k) r.c[k] = a.c[k] + b.c[k];
+    r.n = a.n > b.n ? a.n : b.n;
+    NRX_CNT(r.n);
+}
+
 // Kinetic energy of particle a (-1/(2m) <bra| d^2/dx_a^2 |ket>) for nv <= 2
 // variants sharing one geminal configuration: variant v has a ket geminal of
 // exponent gk[v] between a and partner c (gk[v] == 0 or c < 0: none).
+// Per direction every 1-D integral of the product rule is
+// fac * sqrt(pi/w) * E[Q(y)] over one Gaussian exp(-w (y - c)^2) in the hub
+// coordinate (the Gaussian factors of the star do not depend on the
+// polynomials), with Q the product of the particles' polynomials mapped by
+// hub_map; the factors are computed once and the terms of a variant are
+// summed into one Q before the closing expectation.
 template <int NL>
-NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, double mass, double pic, double *out)
+NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, double mass, double pic, double *out,
+                      double *ovl_out = nullptr)
 {
     NRX_FENCE();
     // product rule over directions, accumulated direction by direction:
     // R_v = sum_d T_d prod_{d' != d} S_d',  Pr = prod_d S_d
     double Pr = 1.0, R[2] = {0.0, 0.0};
     const PairRec &A = *s.q[a];
-    PolyT<NK> dummy;
-    p_const(dummy, 1.0);
+    const int h = s.n - 1;
     NRX_DIRS for (int d = 0; d < 3; ++d) {
-        double Sd = ovl_dim<NL>(s, d, -1, dummy, -1, dummy, pic);
+        // Gaussian factors, leaf by leaf as absorb_leaf forms them
+        double w = s.q[h]->p, cq = 0.0, fac = 1.0;
+        NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
+            if (i >= h) continue;
+            const PairRec &Q = *s.q[i];
+            double g = s.gam[i], iW = 1.0 / (Q.p + g), fl = sqrt(pic * iW);
+            if (g != 0.0) {
+                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = 1.0 / Wq;
+                double diff = cq - cl;
+                fl *= exp(-(w * kap * iWq) * diff * diff);
+                cq = (w * cq + kap * cl) * iWq;
+                w = Wq;
+            }
+            fac *= fl;
+        }
+        const double iw = 1.0 / w, norm = fac * sqrt(pic * iw);
+        // the particles other than a and c
+        PolyT<NL> env;
+        p_const(env, 1.0);
+        NRX_UNROLL for (int i = 0; i < NPART; ++i) {
+            if (i > h || i == a || i == c) continue;
+            PolyT<NS> f;
+            fpoly(*s.q[i], d, f);
+            PolyT<NL> m;
+            hub_map(s, i, d, f, m);
+            p_mul(env, m, env);
+        }
+        PolyT<NL> base, q;
+        if (c >= 0) {
+            PolyT<NS> f;
+            fpoly(*s.q[c], d, f);
+            PolyT<NL> m;
+            hub_map(s, c, d, f, m);
+            p_mul(env, m, base);
+        } else {
+            base = env;
+        }
+        {
+            PolyT<NS> f;
+            fpoly(A, d, f);
+            PolyT<NL> m;
+            hub_map(s, a, d, f, m);
+            p_mul(base, m, q);
+        }
+        double Sd = norm * close_expect(q, iw, cq);
         int L = A.lb[d];
         double sA = A.sA[d], sB = A.sB[d];
         int la = A.la[d];
@@ -494,8 +586,11 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
                 p_mul_lin(F0, sA, i < la);
                 p_mul_lin(F1, sA, i < la);
             }
-            double t = ovl_dim<NL>(s, d, a, F0, -1, dummy, pic);
+            PolyT<NL> m0;
+            hub_map(s, a, d, F0, m0);
+            p_mul(base, m0, q);
             if (g != 0.0) {
+                // partner c's polynomial times X_c and X_c^2, X_c = y_c + sB_c
                 PolyT<NK> F2;
                 p_const(F2, 4.0 * g * g);
                 NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
@@ -503,11 +598,20 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
                 PolyT<NK> fc;
                 fpoly(*s.q[c], d, fc);
                 double xc = s.q[c]->sB[d];
+                PolyT<NL> m1, m2, c1, c2, u;
                 p_mul_lin(fc, xc);
-                t += ovl_dim<NL>(s, d, a, F1, c, fc, pic);
+                hub_map(s, c, d, fc, c1);
                 p_mul_lin(fc, xc);
-                t += ovl_dim<NL>(s, d, a, F2, c, fc, pic);
+                hub_map(s, c, d, fc, c2);
+                hub_map(s, a, d, F1, m1);
+                hub_map(s, a, d, F2, m2);
+                p_mul(m1, c1, u);
+                p_mul(m2, c2, m2);
+                p_add(u, m2, u);
+                p_mul(env, u, u);
+                p_add(q, u, q);
             }
+            double t = norm * close_expect(q, iw, cq);
             R[v] = R[v] * Sd + Pr * t;
         }
         Pr *= Sd;
@@ -517,6 +621,7 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
         if (v >= nv) continue;
         out[v] = -0.5 * R[v] / mass * kp;
     }
+    if (ovl_out) *ovl_out = kp * Pr; // the configuration's overlap, normalised with pic
 }
 
 template <int NL> NRX_HD double kin(const Sys &s, int a, int c, double gk, double mass, double pic = PI)
@@ -525,6 +630,34 @@ template <int NL> NRX_HD double kin(const Sys &s, int a, int c, double gk, doubl
     double g2[2] = {gk, gk};
     kin_multi<NL>(s, a, c, g2, 1, mass, pic, o);
     return o[0];
+}
+
+constexpr double factc(int n)
+{
+    double r = 1.0;
+    for (int i = 2; i <= n; ++i) r *= i;
+    return r;
+}
+
+// The l0 = 0 case of hermite_coulomb (nuclear attraction) in closed form:
+// R_t is the scaled Hermite polynomial
+//   R_t = sum_{j=ceil(t/2)}^{t} (-1)^j t! / ((t-j)! (2j-t)!) (2D)^{2j-t} l1^j u^{2j},
+// so P_j = (-l1)^j sum_{t=j}^{min(2j, N-1)} hk_t t! / ((t-j)! (2j-t)!) (2D)^{2j-t}.
+template <int N> NRX_HD void hermite_coulomb0(const PolyT<N> &hk, double D, double l1, PolyT<N> &P)
+{
+    double dp[N];
+    dp[0] = 1.0;
+    NRX_UNROLL for (int i = 1; i < N; ++i) dp[i] = dp[i - 1] * (2.0 * D);
+    double lp = 1.0;
+    NRX_UNROLL for (int j = 0; j < N; ++j) {
+        double acc = 0.0;
+        NRX_UNROLL for (int t = j; t < N && t <= 2 * j; ++t)
+            acc += hk.c[t] * (factc(t) / (factc(t - j) * factc(2 * j - t))) * dp[2 * j - t];
+        P.c[j] = acc * lp;
+        lp *= -l1;
+    }
+    P.n = hk.n;
+    NRX_CNT(hk.n * hk.n / 2 + 3 * hk.n);
 }
 
 // P(u^2) = sum_k hk R_k(u^2), R_0 = 1, R_1 = -2 lam D,
@@ -616,11 +749,11 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
         constexpr int N2 = (2 * NL - 1 < NBC) ? 2 * NL - 1 : NBC;
         double Dx = X[0] - cat[3 * c], Dy = X[1] - cat[3 * c + 1], Dz = X[2] - cat[3 * c + 2];
         PolyT<NL> Px, Py;
-        hermite_coulomb<NL, true>(e[0], Dx, 0.0, w, Px);
-        hermite_coulomb<NL, true>(e[1], Dy, 0.0, w, Py);
+        hermite_coulomb0(e[0], Dx, w, Px);
+        hermite_coulomb0(e[1], Dy, w, Py);
         PolyT<N2> cxy;
         p_mul(Px, Py, cxy);
-        hermite_coulomb<NL, true>(e[2], Dz, 0.0, w, Px);
+        hermite_coulomb0(e[2], Dz, w, Px);
         PolyT<NBC> cc;
         p_mul(cxy, Px, cc);
         double T = w * (((0.0 + Dx * Dx) + Dy * Dy) + Dz * Dz);
@@ -888,8 +1021,8 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             // (il,ik) (bra g(e2,p)=gl, ket g(e1,p)=gk) is the kinetic operator
             // on this configuration with ket partner exponent gk.
             Sys sb = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
-            double gks[2] = {0.0, gk}, kk[2];
-            kin_multi<NL>(sb, 0, 2, gks, 2, 1.0, PI, kk);
+            double gks[2] = {0.0, gk}, kk[2], ov;
+            kin_multi<NL>(sb, 0, 2, gks, 2, 1.0, PI, kk, &ov);
             gTEg1 += bb * kk[0];
             gTEg2 += bb * kk[1];
             gTPg += bb * kin<NL>(sb, 2, 1, gl, S.pmass);
@@ -901,7 +1034,7 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             v = coul<NC, NBC>(sb, 0, 1);
             underflow(v);
             gVEEg2 += bb * v;
-            gsg += bb * ovl<NC>(sb);
+            gsg += bb * ov;
             if (il < ik) continue;
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 3, e1, e2, p, 0, 0.0, gk + gl, 0); // g(e2,p)=gk+gl
