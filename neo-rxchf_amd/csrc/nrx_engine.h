@@ -88,6 +88,9 @@ constexpr int NPART = 4;
 constexpr double PI = 3.141592653589793238462643383279502884;
 // pgike1 (AC_Integrals/g2.f:216) normalises its overlaps with this value
 constexpr double PI_G2 = 3.141592650;
+// a two-particle overlap carries pi^3 (pi^(1/2) per Gaussian integration, two
+// per direction): this factor turns one normalised with PI_G2 into one with PI
+constexpr double G2_TO_PI3 = (PI / PI_G2) * (PI / PI_G2) * (PI / PI_G2);
 
 template <int N> struct PolyT {
     double c[N]; // c[k] = 0 for k >= n
@@ -950,8 +953,8 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
         if constexpr (!MULT) {
             Sys s1 = mksys(S, 2, e, p, 0, 0, gv, 0, 0);  // (e, p): hub = p
             Sys s1r = mksys(S, 2, p, e, 0, 0, gv, 0, 0); // (p, e): hub = e
-            double gks[2] = {0.0, gv}, kk[2];
-            kin_multi<NL>(s1, 0, 1, gks, 2, 1.0, PI_G2, kk);
+            double gks[2] = {0.0, gv}, kk[2], ov;
+            kin_multi<NL>(s1, 0, 1, gks, 2, 1.0, PI_G2, kk, &ov);
             tegx += bv * kk[0];
             texg += bv * kk[1];
             kin_multi<NL>(s1r, 0, 1, gks, 2, S.pmass, PI_G2, kk);
@@ -960,20 +963,26 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             vcegx += bv * nuc_sum<NC, NBC>(s1, 0, S.nat, S.cat, S.zan, false);
             vcpgx += bv * nuc_sum<NC, NBC>(s1, 1, S.nat, S.cat, S.zan, false);
             vmgx += bv * coul<NC, NBC>(s1, 0, 1);
-            ogx += bv * ovl<NC>(s1);
+            ogx += bv * (ov * G2_TO_PI3);
         }
-        for (int l = 0; l < S.ngtg; ++l) {
+        // (k,l) and (l,k) share the configuration gamma_k + gamma_l; their
+        // kinetic terms differ only in the ket-geminal exponent (gamma_l,
+        // gamma_k), two variants of one kinetic call
+        for (int l = k; l < S.ngtg; ++l) {
             double gw = S.gamma[l], bb = bv * S.bcoef[l];
             Sys s2 = mksys(S, 2, e, p, 0, 0, gv + gw, 0, 0);
             Sys s2r = mksys(S, 2, p, e, 0, 0, gv + gw, 0, 0);
-            tegg += bb * kin<NL>(s2, 0, 1, gw, 1.0, PI_G2);
-            tpgg += bb * kin<NL>(s2r, 0, 1, gw, S.pmass, PI_G2);
-            if (l < k) continue;
+            double g2[2] = {gw, gv}, kk[2], ov;
+            int nv = (l == k) ? 1 : 2;
+            kin_multi<NL>(s2, 0, 1, g2, nv, 1.0, PI_G2, kk, &ov);
+            tegg += (l == k) ? bb * kk[0] : bb * kk[0] + bb * kk[1];
+            kin_multi<NL>(s2r, 0, 1, g2, nv, S.pmass, PI_G2, kk);
+            tpgg += (l == k) ? bb * kk[0] : bb * kk[0] + bb * kk[1];
             double w = (l == k) ? bb : 2.0 * bb;
             vcegg += w * nuc_sum<NC, NBC>(s2, 0, S.nat, S.cat, S.zan, false);
             vcpgg += w * nuc_sum<NC, NBC>(s2, 1, S.nat, S.cat, S.zan, false);
             vmgg += w * coul<NC, NBC>(s2, 0, 1);
-            ogg += w * ovl<NC>(s2);
+            ogg += w * (ov * G2_TO_PI3);
         }
     }
     double te = tegx + texg + tegg, tp = tpgx + tpxg + tpgg;
