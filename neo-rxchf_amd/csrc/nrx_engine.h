@@ -1003,13 +1003,24 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
                       double &ansS, double &ansE2, double &ansE3)
 {
     constexpr int NC = NL - 2; // capacity of the terms without a kinetic operator
+    // Where e1 has no geminal (g(e1,p) = 0), its kinetic and nuclear-attraction
+    // integrals factor into e1's one-particle value times the (e2, p) overlap
+    // with the geminal: t1, v1 below, and ovl2(g).
+    Sys s1e = mksys(S, 1, e1, 0, 0, 0, 0.0, 0, 0);
+    const double t1 = kin<NL>(s1e, 0, -1, 0.0, 1.0);
+    const double v1 = nuc_sum<NC, NBC>(s1e, 0, S.nat, S.cat, S.zan, true);
+    auto ovl2 = [&](double g) {
+        Sys s2p = mksys(S, 2, e2, p, 0, 0, g, 0, 0);
+        return ovl<NC>(s2p);
+    };
     double gTE = 0, gVEC = 0, gVEE = 0, gVEP = 0;
     if constexpr (!MULT)
     for (int ik = 0; ik < S.ngtg; ++ik) {
         double gk = S.gamma[ik], bk = S.bcoef[ik];
         Sys s = mksys(S, 3, e1, e2, p, 0, 0.0, gk, 0); // g(e2,p)
-        double xgTE = kin<NL>(s, 0, -1, 0.0, 1.0);
-        double xgVEC = -nuc_sum<NC, NBC>(s, 0, S.nat, S.cat, S.zan, true);
+        double o2 = ovl2(gk);
+        double xgTE = t1 * o2;
+        double xgVEC = -(v1 * o2);
         double xgVEP = coul<NC, NBC>(s, 0, 2);
         underflow(xgVEP);
         xgVEP = -xgVEP;
@@ -1048,8 +1059,9 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             double w = (il == ik) ? bb : 2.0 * bb;
             Sys s2 = mksys(S, 3, e1, e2, p, 0, 0.0, gk + gl, 0); // g(e2,p)=gk+gl
             Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0); // g(e1,p)=gk+gl
-            gTEg3 += w * kin<NL>(s2, 0, -1, 0.0, 1.0);
-            gVECg2 -= w * nuc_sum<NC, NBC>(s2, 0, S.nat, S.cat, S.zan, true);
+            double o2 = ovl2(gk + gl);
+            gTEg3 += w * (t1 * o2);
+            gVECg2 -= w * (v1 * o2);
             v = coul<NC, NBC>(s2, 0, 2);
             underflow(v);
             gVEPg2 -= w * v;
