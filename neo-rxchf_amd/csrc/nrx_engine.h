@@ -848,11 +848,12 @@ template <int NL, int NBC = NB> NRX_HD double coul(const Sys &s, int a, int b, d
                 p_smooth(hs, 0.5 / W, hs);
                 double m0 = (H.w * H.c - ga * dpa - gb * dpb) / W, ba = ga / W, bb = gb / W;
                 // D = h^{(j)} / j!  (D_{j+1,k} = (k+1)/(j+1) D_{j,k+1}),
-                // Yb = B.F (bb yb)^j; the loop stays rolled, one term live
+                // Yb = B.F (bb yb)^j; the loop stays rolled, one term live,
+                // and ends after the hub polynomial's degree (D = 0 beyond)
                 PolyT<NS> D = hs;
                 PolyT<NL> Yb;
                 p_copy(B.F, Yb);
-                _Pragma("nounroll") for (int j = 0; j < NS; ++j) {
+                _Pragma("nounroll") for (int j = 0; j < hs.n; ++j) {
                     PolyT<NS> der;
                     p_compose(D, m0, ba, der);
                     PolyT<NL> Fa, Fb, ea, eb;
