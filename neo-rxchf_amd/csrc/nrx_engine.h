@@ -910,26 +910,32 @@ template <int N> NRX_HD void conv_add(const PolyT<N> &ea, const PolyT<N> &eb, Po
     NRX_CNT(2 * ea.n * eb.n);
 }
 
-// Coulomb <1/|r_a - r_b|>, a < b
-template <int NL, int NBC = NB> NRX_HD double coul(const Sys &s, int a, int b, double pic = PI)
+// Coulomb <1/|r_a - r_b|>, a < b.  With b the hub and hub_ovl set, also the
+// overlap of the star without particle a (the hub with every leaf but a
+// absorbed, which the Coulomb integral forms anyway).
+template <int NL, int NBC = NB>
+NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl = nullptr)
 {
     NRX_FENCE();
     int h = s.n - 1;
     PolyT<NBC> cc; // product over directions of the polynomials in u^2
     p_const(cc, 1.0);
-    double D2 = 0.0, fac = Kprod(s);
+    double D2 = 0.0, fac = Kprod(s), ho = 1.0;
     double wa = 1.0, wb = 1.0, gab = 0.0;
     NRX_DIRS for (int d = 0; d < 3; ++d) {
         VarT<NL> H;
         fpoly(*s.q[h], d, H.F);
         H.w = s.q[h]->p;
         H.c = 0.0;
+        double fh = 1.0;
         NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
             if (i >= h || i == a || i == b) continue;
             PolyT<NS> f;
             fpoly(*s.q[i], d, f);
-            fac *= absorb_leaf(f, s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
+            fh *= absorb_leaf(f, s.q[i]->p, 0.0, s.gam[i], s.q[i]->P[d] - s.q[h]->P[d], H, pic);
         }
+        fac *= fh;
+        if (hub_ovl) ho *= fh * close_var(H, pic);
         VarT<NL> A, B;
         fpoly(*s.q[a], d, A.F);
         A.w = s.q[a]->p;
@@ -1024,6 +1030,12 @@ template <int NL, int NBC = NB> NRX_HD double coul(const Sys &s, int a, int b, d
     double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
     double pre = pic / (wa + wb);
     pre = pre * sqrt(pre) * 2.0 * pic / (rho + gab);
+    if (hub_ovl) {
+        double k = 1.0;
+        NRX_UNROLL for (int i = 0; i < NPART; ++i)
+            if (i < s.n && i != a) k *= s.q[i]->K;
+        *hub_ovl = k * ho;
+    }
     return fac * pre * exp(-l0 * D2) * boys_dot(cc, l1 * D2, s.bt);
 }
 
@@ -1243,10 +1255,10 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
         for (int il = 0; il < S.ngtg; ++il) {
             double gk = S.gamma[ik], gl = S.gamma[il];
             double bb = S.bcoef[ik] * S.bcoef[il];
-            Sys s3p = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
-            double yy = ovl<NL>(s3p);
-            Sys s4a = mksys(S, 4, e1, e2, e3, p, gk, gl, 0.0); // 1/r34, g14 g24
-            double zz = coul<NL, NBC>(s4a, 2, 3);
+            // 1/r34, g14 g24; its hub star (e1, e2, p) is the overlap yy
+            Sys s4a = mksys(S, 4, e1, e2, e3, p, gk, gl, 0.0);
+            double yy;
+            double zz = coul<NL, NBC>(s4a, 2, 3, PI, &yy);
             underflow(yy);
             underflow(zz);
             gHEg += bb * ((hcore * yy) + (zz * (-1.0)));
