@@ -627,138 +627,6 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
     if (ovl_out) *ovl_out = kp * Pr; // the configuration's overlap, normalised with pic
 }
 
-// One kinetic operator of kin_pair: particle a, partner c (c < 0: none),
-// nv <= 2 variants with ket-geminal exponents gk[v], mass.
-struct KinOp {
-    int a, c, nv;
-    double gk[2], mass;
-};
-
-// Two kinetic operators on one configuration (for instance the electron's
-// and the proton's of an e-p star): the Gaussian factors, the particles'
-// polynomials mapped to the hub coordinate and the overlap of each direction
-// are computed once and shared, as are the product-rule overlaps S_d.
-// out1[v], out2[v] as kin_multi's out; ovl_out (if set) the overlap.
-template <int NL>
-NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic, double *out1, double *out2,
-                     double *ovl_out = nullptr)
-{
-    NRX_FENCE();
-    double Pr = 1.0, R[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    const int h = s.n - 1;
-    NRX_DIRS for (int d = 0; d < 3; ++d) {
-        double w = s.q[h]->p, cq = 0.0, fac = 1.0;
-        NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
-            if (i >= h) continue;
-            const PairRec &Q = *s.q[i];
-            double g = s.gam[i], iW = 1.0 / (Q.p + g), fl = sqrt(pic * iW);
-            if (g != 0.0) {
-                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = 1.0 / Wq;
-                double diff = cq - cl;
-                fl *= exp(-(w * kap * iWq) * diff * diff);
-                cq = (w * cq + kap * cl) * iWq;
-                w = Wq;
-            }
-            fac *= fl;
-        }
-        const double iw = 1.0 / w, norm = fac * sqrt(pic * iw);
-        // every particle's own polynomial in the hub coordinate
-        PolyT<NL> m[NPART];
-        NRX_UNROLL for (int i = 0; i < NPART; ++i) {
-            if (i > h) {
-                p_const(m[i], 1.0);
-                continue;
-            }
-            PolyT<NS> f;
-            fpoly(*s.q[i], d, f);
-            hub_map(s, i, d, f, m[i]);
-        }
-        PolyT<NL> q;
-        p_const(q, 1.0);
-        NRX_UNROLL for (int i = 0; i < NPART; ++i)
-            if (i <= h) p_mul(q, m[i], q);
-        const double Sd = norm * close_expect(q, iw, cq);
-        NRX_UNROLL for (int k = 0; k < 2; ++k) {
-            const KinOp &o = (k == 0) ? o1 : o2;
-            const int a = o.a, c = o.c;
-            const PairRec &A = *s.q[a];
-            PolyT<NL> env, base;
-            p_const(env, 1.0);
-            NRX_UNROLL for (int i = 0; i < NPART; ++i)
-                if (i <= h && i != a && i != c) p_mul(env, m[i], env);
-            if (c >= 0)
-                p_mul(env, m[c], base);
-            else
-                base = env;
-            int L = A.lb[d];
-            double sA = A.sA[d], sB = A.sB[d];
-            int la = A.la[d];
-            NRX_UNROLL for (int v = 0; v < 2; ++v) {
-                if (v >= o.nv) continue;
-                double g = (c >= 0) ? o.gk[v] : 0.0;
-                double al = A.beta + g;
-                double dl = (g != 0.0) ? g * (A.B[d] - s.q[c]->B[d]) : 0.0;
-                // the F0, F1, F2 of kin_multi
-                double cf0[5] = {(double)(L * (L - 1)), -4.0 * L * dl, -4.0 * L * al + 4.0 * dl * dl - 2.0 * al,
-                                 8.0 * al * dl, 4.0 * al * al};
-                double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
-                int j0 = (L >= 2) ? 0 : 2 - L;
-                PolyT<NK> F0, F1;
-                p_const(F0, cf0[4]);
-                p_const(F1, cf1[4]);
-                NRX_UNROLL for (int j = 3; j >= 0; --j) {
-                    bool on = j >= j0;
-                    p_mul_lin(F0, sB, on);
-                    p_mul_lin(F1, sB, on);
-                    F0.c[0] += on ? cf0[j] : 0.0;
-                    F1.c[0] += on ? cf1[j] : 0.0;
-                }
-                NRX_UNROLL for (int i = 0; i < 2; ++i) {
-                    p_mul_lin(F0, sB, i < L - 2);
-                    p_mul_lin(F1, sB, i < L - 2);
-                }
-                NRX_UNROLL for (int i = 0; i < 2; ++i) {
-                    p_mul_lin(F0, sA, i < la);
-                    p_mul_lin(F1, sA, i < la);
-                }
-                PolyT<NL> m0;
-                hub_map(s, a, d, F0, m0);
-                p_mul(base, m0, q);
-                if (g != 0.0) {
-                    PolyT<NK> F2;
-                    p_const(F2, 4.0 * g * g);
-                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
-                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
-                    PolyT<NK> fc;
-                    fpoly(*s.q[c], d, fc);
-                    double xc = s.q[c]->sB[d];
-                    PolyT<NL> m1, m2, c1, c2, u;
-                    p_mul_lin(fc, xc);
-                    hub_map(s, c, d, fc, c1);
-                    p_mul_lin(fc, xc);
-                    hub_map(s, c, d, fc, c2);
-                    hub_map(s, a, d, F1, m1);
-                    hub_map(s, a, d, F2, m2);
-                    p_mul(m1, c1, u);
-                    p_mul(m2, c2, m2);
-                    p_add(u, m2, u);
-                    p_mul(env, u, u);
-                    p_add(q, u, q);
-                }
-                double t = norm * close_expect(q, iw, cq);
-                R[k][v] = R[k][v] * Sd + Pr * t;
-            }
-        }
-        Pr *= Sd;
-    }
-    double kp = Kprod(s);
-    NRX_UNROLL for (int v = 0; v < 2; ++v) {
-        if (v < o1.nv) out1[v] = -0.5 * R[0][v] / o1.mass * kp;
-        if (v < o2.nv) out2[v] = -0.5 * R[1][v] / o2.mass * kp;
-    }
-    if (ovl_out) *ovl_out = kp * Pr;
-}
-
 template <int NL> NRX_HD double kin(const Sys &s, int a, int c, double gk, double mass, double pic = PI)
 {
     double o[2];
@@ -869,14 +737,12 @@ template <int NL> NRX_HD double nuc_core(const Sys &s, int a, int d, PolyT<NL> &
 
 // sum_c zan[c] <1/|r_a - C_c|>, each centre's value flushed below 1e-100 when
 // flush is set (the reference applies underflow() per centre)
-template <int NL, int NBC = NB>
-NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const double *zan, bool flush, double pic = PI)
+// sum_c zan[c] pre <1/|r - C_c|> over the core variable's Hermite
+// coefficients e[d], centre X[d] and exponent w (the centre loop of nuc_sum)
+template <int NL, int NBC>
+NRX_HD double nuc_centres(const PolyT<NL> *e, const double *X, double w, double pre, int nat, const double *cat,
+                          const double *zan, bool flush, const double *bt)
 {
-    NRX_FENCE();
-    PolyT<NL> e[3];
-    double X[3], w = 1.0, fac = Kprod(s);
-    NRX_UNROLL for (int d = 0; d < 3; ++d) fac *= nuc_core<NL>(s, a, d, e[d], X[d], w, pic);
-    double pre = fac * 2.0 * pic / w;
     double tot = 0.0;
     for (int c = 0; c < nat; ++c) {
         // product over directions, sized as it grows: x*y at capacity
@@ -892,11 +758,185 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
         PolyT<NBC> cc;
         p_mul(cxy, Px, cc);
         double T = w * (((0.0 + Dx * Dx) + Dy * Dy) + Dz * Dz);
-        double v = pre * boys_dot(cc, T, s.bt);
+        double v = pre * boys_dot(cc, T, bt);
         if (flush) underflow(v);
         tot += zan[c] * v;
     }
     return tot;
+}
+
+template <int NL, int NBC = NB>
+NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const double *zan, bool flush, double pic = PI)
+{
+    NRX_FENCE();
+    PolyT<NL> e[3];
+    double X[3], w = 1.0, fac = Kprod(s);
+    NRX_UNROLL for (int d = 0; d < 3; ++d) fac *= nuc_core<NL>(s, a, d, e[d], X[d], w, pic);
+    return nuc_centres<NL, NBC>(e, X, w, fac * 2.0 * pic / w, nat, cat, zan, flush, s.bt);
+}
+
+// One kinetic operator of kin_pair: particle a, partner c (c < 0: none),
+// nv <= 2 variants with ket-geminal exponents gk[v], mass.
+struct KinOp {
+    int a, c, nv;
+    double gk[2], mass;
+};
+
+// Two kinetic operators on one configuration (for instance the electron's
+// and the proton's of an e-p star): the Gaussian factors, the particles'
+// polynomials mapped to the hub coordinate and the overlap of each direction
+// are computed once and shared, as are the product-rule overlaps S_d.
+// out1[v], out2[v] as kin_multi's out; ovl_out (if set) the overlap.  With
+// NUC set, nuc_out = nuc_sum(s, hub, ...) at capacity NC: the hub's core
+// variable is the product polynomial of the overlap, on the same Gaussian.
+template <int NL, int NC = NL, int NBC = NB, bool NUC = false>
+NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic, double *out1, double *out2,
+                     double *ovl_out = nullptr, double *nuc_out = nullptr, int nat = 0, const double *cat = nullptr,
+                     const double *zan = nullptr, bool flush = false)
+{
+    NRX_FENCE();
+    const int h = s.n - 1;
+    double Pr = 1.0, R[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    PolyT<NC> ne[3]; // the hub's Hermite coefficients per direction (NUC)
+    double nX[3] = {0.0, 0.0, 0.0}, nfac = 1.0, nw = 1.0;
+    NRX_DIRS for (int d = 0; d < 3; ++d) {
+        double w = s.q[h]->p, cq = 0.0, fac = 1.0;
+        NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
+            if (i >= h) continue;
+            const PairRec &Q = *s.q[i];
+            double g = s.gam[i], iW = 1.0 / (Q.p + g), fl = sqrt(pic * iW);
+            if (g != 0.0) {
+                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = 1.0 / Wq;
+                double diff = cq - cl;
+                fl *= exp(-(w * kap * iWq) * diff * diff);
+                cq = (w * cq + kap * cl) * iWq;
+                w = Wq;
+            }
+            fac *= fl;
+        }
+        const double iw = 1.0 / w, norm = fac * sqrt(pic * iw);
+        // every particle's own polynomial in the hub coordinate
+        PolyT<NL> m[NPART];
+        NRX_UNROLL for (int i = 0; i < NPART; ++i) {
+            if (i > h) {
+                p_const(m[i], 1.0);
+                continue;
+            }
+            PolyT<NS> f;
+            fpoly(*s.q[i], d, f);
+            hub_map(s, i, d, f, m[i]);
+        }
+        PolyT<NL> q;
+        p_const(q, 1.0);
+        NRX_UNROLL for (int i = 0; i < NPART; ++i)
+            if (i <= h) p_mul(q, m[i], q);
+        const double Sd = norm * close_expect(q, iw, cq);
+        if constexpr (NUC) {
+            PolyT<NC> qc, ed;
+            p_copy(q, qc);
+            p_compose(qc, cq, 1.0, qc);
+            p_hermite(qc, w, ed);
+            // the direction loop stays rolled: store by uniform branches
+            if (d == 0)
+                ne[0] = ed;
+            else if (d == 1)
+                ne[1] = ed;
+            else
+                ne[2] = ed;
+            double x = s.q[h]->P[d] + cq;
+            nX[0] = (d == 0) ? x : nX[0];
+            nX[1] = (d == 1) ? x : nX[1];
+            nX[2] = (d == 2) ? x : nX[2];
+            nfac *= fac;
+            nw = w;
+        }
+        NRX_UNROLL for (int k = 0; k < 2; ++k) {
+            const KinOp &o = (k == 0) ? o1 : o2;
+            const int a = o.a, c = o.c;
+            const PairRec &A = *s.q[a];
+            PolyT<NL> env, base;
+            p_const(env, 1.0);
+            NRX_UNROLL for (int i = 0; i < NPART; ++i)
+                if (i <= h && i != a && i != c) p_mul(env, m[i], env);
+            if (c >= 0)
+                p_mul(env, m[c], base);
+            else
+                base = env;
+            int L = A.lb[d];
+            double sA = A.sA[d], sB = A.sB[d];
+            int la = A.la[d];
+            NRX_UNROLL for (int v = 0; v < 2; ++v) {
+                if (v >= o.nv) continue;
+                double g = (c >= 0) ? o.gk[v] : 0.0;
+                double al = A.beta + g;
+                double dl = (g != 0.0) ? g * (A.B[d] - s.q[c]->B[d]) : 0.0;
+                // the F0, F1, F2 of kin_multi
+                double cf0[5] = {(double)(L * (L - 1)), -4.0 * L * dl, -4.0 * L * al + 4.0 * dl * dl - 2.0 * al,
+                                 8.0 * al * dl, 4.0 * al * al};
+                double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
+                int j0 = (L >= 2) ? 0 : 2 - L;
+                PolyT<NK> F0, F1;
+                p_const(F0, cf0[4]);
+                p_const(F1, cf1[4]);
+                NRX_UNROLL for (int j = 3; j >= 0; --j) {
+                    bool on = j >= j0;
+                    p_mul_lin(F0, sB, on);
+                    p_mul_lin(F1, sB, on);
+                    F0.c[0] += on ? cf0[j] : 0.0;
+                    F1.c[0] += on ? cf1[j] : 0.0;
+                }
+                NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                    p_mul_lin(F0, sB, i < L - 2);
+                    p_mul_lin(F1, sB, i < L - 2);
+                }
+                NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                    p_mul_lin(F0, sA, i < la);
+                    p_mul_lin(F1, sA, i < la);
+                }
+                PolyT<NL> m0;
+                hub_map(s, a, d, F0, m0);
+                p_mul(base, m0, q);
+                if (g != 0.0) {
+                    PolyT<NK> F2;
+                    p_const(F2, 4.0 * g * g);
+                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
+                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
+                    PolyT<NK> fc;
+                    fpoly(*s.q[c], d, fc);
+                    double xc = s.q[c]->sB[d];
+                    PolyT<NL> m1, m2, c1, c2, u;
+                    p_mul_lin(fc, xc);
+                    hub_map(s, c, d, fc, c1);
+                    p_mul_lin(fc, xc);
+                    hub_map(s, c, d, fc, c2);
+                    hub_map(s, a, d, F1, m1);
+                    hub_map(s, a, d, F2, m2);
+                    p_mul(m1, c1, u);
+                    p_mul(m2, c2, m2);
+                    p_add(u, m2, u);
+                    p_mul(env, u, u);
+                    p_add(q, u, q);
+                }
+                double t = norm * close_expect(q, iw, cq);
+                R[k][v] = R[k][v] * Sd + Pr * t;
+            }
+        }
+        Pr *= Sd;
+    }
+    double kp = Kprod(s);
+    NRX_UNROLL for (int v = 0; v < 2; ++v) {
+        if (v < o1.nv) out1[v] = -0.5 * R[0][v] / o1.mass * kp;
+        if (v < o2.nv) out2[v] = -0.5 * R[1][v] / o2.mass * kp;
+    }
+    if (ovl_out) *ovl_out = kp * Pr;
+    if constexpr (NUC) {
+        // the Gaussian factors carry pic^(1/2) per leaf and direction; the
+        // nuclear attraction is normalised with PI
+        double r = PI / pic, sc = 1.0;
+        NRX_UNROLL for (int i = 0; i < NPART - 1; ++i)
+            if (i < h) sc *= r * sqrt(r);
+        *nuc_out = nuc_centres<NC, NBC>(ne, nX, nw, kp * nfac * sc * 2.0 * PI / nw, nat, cat, zan, flush, s.bt);
+    }
 }
 
 // hk[t+u] += ea[t] eb[u] (-1)^u
@@ -1099,14 +1139,15 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             Sys s1 = mksys(S, 2, e, p, 0, 0, gv, 0, 0);  // (e, p): hub = p
             // the electron's and the proton's kinetic operators on the same
             // two-particle star: one pass serves both
-            double ke[2], kp[2], ov;
-            kin_pair<NL>(s1, KinOp{0, 1, 2, {0.0, gv}, 1.0}, KinOp{1, 0, 2, {0.0, gv}, S.pmass}, PI_G2, ke, kp, &ov);
+            double ke[2], kp[2], ov, vp;
+            kin_pair<NL, NC, NBC, true>(s1, KinOp{0, 1, 2, {0.0, gv}, 1.0}, KinOp{1, 0, 2, {0.0, gv}, S.pmass}, PI_G2,
+                                        ke, kp, &ov, &vp, S.nat, S.cat, S.zan, false);
             tegx += bv * ke[0];
             texg += bv * ke[1];
             tpgx += bv * kp[0];
             tpxg += bv * kp[1];
             vcegx += bv * nuc_sum<NC, NBC>(s1, 0, S.nat, S.cat, S.zan, false);
-            vcpgx += bv * nuc_sum<NC, NBC>(s1, 1, S.nat, S.cat, S.zan, false);
+            vcpgx += bv * vp;
             vmgx += bv * coul<NC, NBC>(s1, 0, 1);
             ogx += bv * (ov * G2_TO_PI3);
         }
@@ -1117,13 +1158,14 @@ NRX_HD void gam1_tuple(const SysParams &S, const PairRec *e, const PairRec *p, d
             double gw = S.gamma[l], bb = bv * S.bcoef[l];
             Sys s2 = mksys(S, 2, e, p, 0, 0, gv + gw, 0, 0);
             int nv = (l == k) ? 1 : 2;
-            double ke[2], kp[2], ov;
-            kin_pair<NL>(s2, KinOp{0, 1, nv, {gw, gv}, 1.0}, KinOp{1, 0, nv, {gw, gv}, S.pmass}, PI_G2, ke, kp, &ov);
+            double ke[2], kp[2], ov, vp;
+            kin_pair<NL, NC, NBC, true>(s2, KinOp{0, 1, nv, {gw, gv}, 1.0}, KinOp{1, 0, nv, {gw, gv}, S.pmass}, PI_G2,
+                                        ke, kp, &ov, &vp, S.nat, S.cat, S.zan, false);
             tegg += (l == k) ? bb * ke[0] : bb * ke[0] + bb * ke[1];
             tpgg += (l == k) ? bb * kp[0] : bb * kp[0] + bb * kp[1];
             double w = (l == k) ? bb : 2.0 * bb;
             vcegg += w * nuc_sum<NC, NBC>(s2, 0, S.nat, S.cat, S.zan, false);
-            vcpgg += w * nuc_sum<NC, NBC>(s2, 1, S.nat, S.cat, S.zan, false);
+            vcpgg += w * vp;
             vmgg += w * coul<NC, NBC>(s2, 0, 1);
             ogg += w * (ov * G2_TO_PI3);
         }
@@ -1184,12 +1226,13 @@ NRX_HD void gam2_core(const SysParams &S, const PairRec *e1, const PairRec *e2, 
             // (il,ik) (bra g(e2,p)=gl, ket g(e1,p)=gk) is the kinetic operator
             // on this configuration with ket partner exponent gk.
             Sys sb = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
-            double kk[2], kq[2], ov;
-            kin_pair<NL>(sb, KinOp{0, 2, 2, {0.0, gk}, 1.0}, KinOp{2, 1, 1, {gl, gl}, S.pmass}, PI, kk, kq, &ov);
+            double kk[2], kq[2], ov, vp;
+            kin_pair<NL, NC, NBC, true>(sb, KinOp{0, 2, 2, {0.0, gk}, 1.0}, KinOp{2, 1, 1, {gl, gl}, S.pmass}, PI, kk,
+                                        kq, &ov, &vp, S.nat, S.cat, S.zan, true);
             gTEg1 += bb * kk[0];
             gTEg2 += bb * kk[1];
             gTPg += bb * kq[0];
-            gVPCg += bb * nuc_sum<NC, NBC>(sb, 2, S.nat, S.cat, S.zan, true);
+            gVPCg += bb * vp;
             gVECg1 -= bb * nuc_sum<NC, NBC>(sb, 0, S.nat, S.cat, S.zan, true);
             double v = coul<NC, NBC>(sb, 0, 2);
             underflow(v);
