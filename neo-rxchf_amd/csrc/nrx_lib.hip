@@ -371,19 +371,43 @@ template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *id
         range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
 }
 
-// Tuples of a launch are binned by capacity tier (tuple_tier): k_prim_bin
-// appends each tuple's position t to its tier's list, one atomic per wave and
-// tier, and flags out-of-range list entries; k_prim_tiers then evaluates
-// each tier's list with that tier's capacities, one lane per listed tuple, so
-// every wave runs at the capacity its tuples need.
+// Tuples of a launch are sorted by bin, a counting sort in two passes:
+// k_prim_bin gives each tuple its bin (capacity tier tuple_tier, and for the
+// classes whose Coulomb terms take the hub-cycle path also the proton pair's
+// largest per-direction degree, which bounds that path's series) and counts
+// the bins, one atomic per wave and bin; k_prim_scatter writes the tuple
+// positions bin by bin into one list; k_prim_tiers evaluates every bin with
+// its tier's capacities, one lane per listed tuple, so that every wave runs
+// at the capacity its tuples need.  Both passes count in LDS per block and
+// touch each global counter once per block.  Out-of-range list entries are
+// flagged (NaN results) and left out.
+template <int CLS> constexpr int hub_bins() { return (CLS == 2 || CLS == 4 || CLS == 5) ? 5 : 1; }
+template <int CLS> constexpr int n_bins() { return Tier<CLS>::N * hub_bins<CLS>(); }
+
+template <int CLS> NRX_DI int tuple_bin(const PairRec *const *q)
+{
+    int k = tuple_tier<CLS>(q);
+    if constexpr (hub_bins<CLS>() == 1) {
+        return k;
+    } else {
+        const PairRec &P = *q[nel<CLS>()];
+        int m = 0;
+        NRX_UNROLL for (int d = 0; d < 3; ++d) m = max(m, P.la[d] + P.lb[d]);
+        return k * hub_bins<CLS>() + min(m, hub_bins<CLS>() - 1);
+    }
+}
+
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup, const i64 *__restrict__ idx,
                                                   double *__restrict__ outE, double *__restrict__ outS,
                                                   double *__restrict__ outX, double *__restrict__ outY,
-                                                  unsigned *__restrict__ lists, unsigned *__restrict__ counts, i64 cap)
+                                                  unsigned char *__restrict__ code, unsigned *__restrict__ counts)
 {
+    __shared__ unsigned lc[32];
+    if (threadIdx.x < 32) lc[threadIdx.x] = 0u;
+    __syncthreads();
     i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    int tier = -1;
+    int bin = -1;
     if (t < ntup) {
         i64 x[2 * nel<CLS>() + 2];
         bool ok = true;
@@ -394,24 +418,50 @@ __global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup,
         if (ok) {
             const PairRec *q[nel<CLS>() + 1];
             tuple_pairs<CLS>(S, x, q);
-            tier = tuple_tier<CLS>(q);
+            bin = tuple_bin<CLS>(q);
         } else { // out-of-range primitive index: flag the entry
             outE[t] = __builtin_nan("");
             if (outS) outS[t] = __builtin_nan("");
             if (outX) outX[t] = __builtin_nan("");
             if (outY) outY[t] = __builtin_nan("");
         }
+        code[t] = (unsigned char)(bin < 0 ? 255 : bin);
     }
-    const int lane = __lane_id();
-    for (int k = 0; k < Tier<CLS>::N; ++k) {
-        unsigned long long m = __ballot(tier == k);
-        if (m == 0ull) continue;
-        int leader = __ffsll((long long)m) - 1;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&counts[k], (unsigned)__popcll(m));
-        base = __shfl(base, leader);
-        if (tier == k) lists[k * cap + base + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned)t;
+    if (bin >= 0) atomicAdd(&lc[bin], 1u);
+    __syncthreads();
+    if (threadIdx.x < n_bins<CLS>() && lc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], lc[threadIdx.x]);
+}
+
+// offset of bin b in the sorted list: the counts of the bins before it
+NRX_DI unsigned bin_offset(const unsigned *counts, int b)
+{
+    unsigned o = 0;
+    for (int j = 0; j < b; ++j) o += counts[j];
+    return o;
+}
+
+template <int CLS>
+__global__ void __launch_bounds__(256) k_prim_scatter(i64 ntup, const unsigned char *__restrict__ code,
+                                                      const unsigned *__restrict__ counts,
+                                                      unsigned *__restrict__ cursors, unsigned *__restrict__ list)
+{
+    __shared__ unsigned lc[32], lbase[32];
+    if (threadIdx.x < 32) lc[threadIdx.x] = 0u;
+    __syncthreads();
+    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    int bin = -1;
+    if (t < ntup) {
+        int c = code[t];
+        bin = (c == 255) ? -1 : c;
     }
+    unsigned loc = 0;
+    if (bin >= 0) loc = atomicAdd(&lc[bin], 1u); // rank within the block's share of the bin
+    __syncthreads();
+    if (threadIdx.x < n_bins<CLS>())
+        lbase[threadIdx.x] = bin_offset(counts, threadIdx.x) +
+                             (lc[threadIdx.x] ? atomicAdd(&cursors[threadIdx.x], lc[threadIdx.x]) : 0u);
+    __syncthreads();
+    if (bin >= 0) list[lbase[bin] + loc] = (unsigned)t;
 }
 
 // One listed tuple of tier K: its pair records are staged in LDS (the
@@ -440,38 +490,54 @@ NRX_DI void tier_item(const DevSys &S, i64 start, const i64 *__restrict__ idx, d
     if (outY) outY[t] = yy;
 }
 
-// block b of the fused launch: the tiers' lists are laid out from the last
-// tier down (its tuples take longest, so they start first)
+// block b of the fused launch: the bins are laid out from the last tier and
+// the highest proton degree down (their tuples take longest, so they start
+// first).  Locates the block's bin; -1 past the last bin.
+template <int CLS> NRX_DI int block_bin(const unsigned *__restrict__ counts, i64 b, i64 &k0)
+{
+    for (int bin = n_bins<CLS>() - 1; bin >= 0; --bin) {
+        i64 nb = ((i64)counts[bin] + 255) / 256;
+        if (b < nb) {
+            k0 = b * 256;
+            return bin;
+        }
+        b -= nb;
+    }
+    return -1;
+}
+
+// one inlined evaluation per tier: tier K of the bin's tuple
 template <int CLS, int SRC, int K>
-NRX_DI void tier_dispatch(const DevSys &S, i64 start, const i64 *__restrict__ idx, double *__restrict__ outE,
-                          double *__restrict__ outS, double *__restrict__ outX, double *__restrict__ outY,
-                          const unsigned *__restrict__ lists, const unsigned *__restrict__ counts, i64 cap, i64 b,
-                          PairRec (*stage)[256])
+NRX_DI void tier_eval(int tier, const DevSys &S, i64 start, const i64 *__restrict__ idx, double *__restrict__ outE,
+                      double *__restrict__ outS, double *__restrict__ outX, double *__restrict__ outY, i64 t,
+                      PairRec (*stage)[256])
 {
     if constexpr (K >= 0) {
-        i64 c = counts[K], nb = (c + 255) / 256;
-        if (b < nb) {
-            i64 k = b * 256 + threadIdx.x;
-            if (k < c)
-                tier_item<CLS, SRC, K>(S, start, idx, outE, outS, outX, outY, (i64)lists[K * cap + k], stage);
+        if (tier == K) {
+            tier_item<CLS, SRC, K>(S, start, idx, outE, outS, outX, outY, t, stage);
             return;
         }
-        tier_dispatch<CLS, SRC, K - 1>(S, start, idx, outE, outS, outX, outY, lists, counts, cap, b - nb, stage);
+        tier_eval<CLS, SRC, K - 1>(tier, S, start, idx, outE, outS, outX, outY, t, stage);
     }
 }
 
-// Every tier of a launch in one kernel, so that the tiers' tails overlap:
-// lane per listed tuple, blocks past the lists exit.
+// Every bin of a launch in one kernel, so that the bins' tails overlap:
+// lane per listed tuple, blocks past the bins exit.
 template <int CLS, int SRC>
 __global__ void __launch_bounds__(256) k_prim_tiers(DevSys S, i64 start, const i64 *__restrict__ idx,
                                                     double *__restrict__ outE, double *__restrict__ outS,
                                                     double *__restrict__ outX, double *__restrict__ outY,
-                                                    const unsigned *__restrict__ lists,
-                                                    const unsigned *__restrict__ counts, i64 cap)
+                                                    const unsigned *__restrict__ list,
+                                                    const unsigned *__restrict__ counts)
 {
     __shared__ PairRec stage[nel<CLS>() + 1][256];
-    tier_dispatch<CLS, SRC, Tier<CLS>::N - 1>(S, start, idx, outE, outS, outX, outY, lists, counts, cap,
-                                               (i64)blockIdx.x, stage);
+    i64 k0 = 0;
+    const int bin = block_bin<CLS>(counts, (i64)blockIdx.x, k0);
+    if (bin < 0) return;
+    const i64 k = k0 + threadIdx.x;
+    if (k >= (i64)counts[bin]) return;
+    const i64 t = (i64)list[bin_offset(counts, bin) + k];
+    tier_eval<CLS, SRC, Tier<CLS>::N - 1>(bin / hub_bins<CLS>(), S, start, idx, outE, outS, outX, outY, t, stage);
 }
 
 __global__ void k_zero_u32n(unsigned *p, int n)
@@ -481,11 +547,11 @@ __global__ void k_zero_u32n(unsigned *p, int n)
 
 static inline unsigned nblocks(i64 n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// tuples per launch of the primitive kernels (bounds the tier lists, whose
-// entries are 32-bit positions)
+// tuples per launch of the primitive kernels (bounds the sorted list, whose
+// entries are 32-bit positions, and the bin counters)
 static const i64 PRIM_CHUNK = (i64)1 << 24;
 
-// stream-ordered scratch: the tier lists and their counters
+// stream-ordered scratch: the sorted tuple list, bin codes and counters
 struct StreamBuf {
     void *p = nullptr;
     hipStream_t st = 0;
@@ -509,27 +575,32 @@ template <int CLS, int SRC>
 static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, double *oe, double *os, hipStream_t st,
                         double *ox = nullptr, double *oy = nullptr)
 {
-    constexpr int NT = Tier<CLS>::N;
+    constexpr int NBN = n_bins<CLS>();
+    static_assert(NBN <= 32, "bin counters and cursors fit the 256-byte header");
     if constexpr (two_basis<CLS>())
         if (SRC != SRC_LIST && (d.npebf1 <= 0 || d.npebf1 >= d.npebf))
             return fail(NEORX_E_ARG, "class %d enumerates two electron bases; this handle has one", CLS);
     if (ntup <= 0) return NEORX_OK;
     i64 cap = std::min(ntup, PRIM_CHUNK);
-    StreamBuf ws;
-    int rc = ws.alloc(NT * cap * sizeof(unsigned) + 256, st);
+    StreamBuf ws; // header: counts[32], cursors[32]; list: cap x u32; codes: cap x u8
+    int rc = ws.alloc(256 + cap * sizeof(unsigned) + cap, st);
     if (rc) return rc;
-    unsigned *counts = (unsigned *)ws.p, *lists = (unsigned *)((char *)ws.p + 256);
+    unsigned *counts = (unsigned *)ws.p, *cursors = counts + 32, *list = (unsigned *)((char *)ws.p + 256);
+    unsigned char *code = (unsigned char *)(list + cap);
     for (i64 off = 0; off < ntup; off += PRIM_CHUNK) {
         i64 n = std::min(ntup - off, PRIM_CHUNK);
         const i64 *ix = SRC == SRC_LIST ? idx + off * (2 * nel<CLS>() + 2) : nullptr;
         double *e = oe + off, *s = os ? os + off : nullptr, *x = ox ? ox + off : nullptr;
         double *y = oy ? oy + off : nullptr;
-        hipLaunchKernelGGL(k_zero_u32n, dim3(1), dim3(64), 0, st, counts, NT);
+        hipLaunchKernelGGL(k_zero_u32n, dim3(1), dim3(64), 0, st, counts, 64);
         hipLaunchKernelGGL((k_prim_bin<CLS, SRC>), dim3(nblocks(n, 256)), dim3(256), 0, st, d, start + off, n, ix, e, s,
-                           x, y, lists, counts, cap);
+                           x, y, code, counts);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL((k_prim_tiers<CLS, SRC>), dim3(nblocks(n, 256) + NT), dim3(256), 0, st, d, start + off, ix, e,
-                           s, x, y, lists, counts, cap);
+        hipLaunchKernelGGL((k_prim_scatter<CLS>), dim3(nblocks(n, 256)), dim3(256), 0, st, n, code, counts, cursors,
+                           list);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((k_prim_tiers<CLS, SRC>), dim3(nblocks(n, 256) + NBN), dim3(256), 0, st, d, start + off, ix,
+                           e, s, x, y, list, counts);
         HIPCHK(hipGetLastError());
     }
     return NEORX_OK;
