@@ -1396,18 +1396,18 @@ NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *
 // terms, so every tier gives the same values.
 template <int CLS> struct Tier;
 template <> struct Tier<1> {
-    static constexpr int N = 2;
-    static constexpr int NL[N] = {7, 11}, NB[N] = {9, 9};
+    static constexpr int N = 4;
+    static constexpr int NL[N] = {5, 6, 7, 11}, NB[N] = {7, 7, 9, 9};
     static constexpr bool KIN = true;
 };
 template <> struct Tier<2> {
-    static constexpr int N = 3;
-    static constexpr int NL[N] = {7, 9, 15}, NB[N] = {9, 13, 13};
+    static constexpr int N = 5;
+    static constexpr int NL[N] = {5, 6, 7, 9, 15}, NB[N] = {7, 9, 9, 13, 13};
     static constexpr bool KIN = true;
 };
 template <> struct Tier<3> {
-    static constexpr int N = 3;
-    static constexpr int NL[N] = {7, 9, 17}, NB[N] = {11, 17, 17};
+    static constexpr int N = 4;
+    static constexpr int NL[N] = {6, 7, 9, 17}, NB[N] = {9, 11, 17, 17};
     static constexpr bool KIN = false;
 };
 // class 4: the RXCHF-ne e-e-p tuple (Coulomb terms only: 3*4)
