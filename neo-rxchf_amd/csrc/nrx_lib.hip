@@ -1357,11 +1357,12 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                     const unsigned o0 = b * ne4 + d * ne2 + ef;
                     const double *dd = DE + d * ne, *db = DE + b * ne, *df = DE + f * ne;
                     unsigned c = 0;
-                    // eight independent loads in flight per lane before the FMAs
+                    // eight independent loads in flight per lane before the FMAs;
+                    // the slab is read once, so the loads are nontemporal
                     for (; c + 8 <= ne; c += 8) {
                         double gv[8];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) gv[k] = (double)g[o0 + (c + k) * ne3];
+                        for (int k = 0; k < 8; ++k) gv[k] = (double)__builtin_nontemporal_load(&g[o0 + (c + k) * ne3]);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             s1 += dd[c + k] * gv[k]; // DE(c,d)
@@ -1370,7 +1371,7 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                         }
                     }
                     for (; c < ne; ++c) {
-                        double gv = (double)g[o0 + c * ne3];
+                        double gv = (double)__builtin_nontemporal_load(&g[o0 + c * ne3]);
                         s1 += dd[c] * gv;
                         s2 += db[c] * gv;
                         s3 += df[c] * gv;
