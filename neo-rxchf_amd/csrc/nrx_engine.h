@@ -1062,7 +1062,10 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
         double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
         double Dd = (s.q[a]->P[d] + A.c) - (s.q[b]->P[d] + B.c);
         PolyT<NL> P;
-        hermite_coulomb(hk, Dd, l0, l1, P);
+        if (gab == 0.0) // l0 = 0: no geminal between a and b (uniform across a wave)
+            hermite_coulomb0(hk, Dd, l1, P);
+        else
+            hermite_coulomb(hk, Dd, l0, l1, P);
         p_mul(cc, P, cc);
         D2 += Dd * Dd;
     }
@@ -1076,7 +1079,7 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
             if (i < s.n && i != a) k *= s.q[i]->K;
         *hub_ovl = k * ho;
     }
-    return fac * pre * exp(-l0 * D2) * boys_dot(cc, l1 * D2, s.bt);
+    return fac * pre * (gab == 0.0 ? 1.0 : exp(-l0 * D2)) * boys_dot(cc, l1 * D2, s.bt);
 }
 
 // ------------------------------------------------------------------
