@@ -664,36 +664,15 @@ template <int N> NRX_HD void hermite_coulomb0(const PolyT<N> &hk, double D, doub
 }
 
 // P(u^2) = sum_k hk R_k(u^2), R_0 = 1, R_1 = -2 lam D,
-// R_{k+1} = -2 lam (D R_k + k R_{k-1}), lam = l0 + l1 u^2.  The factor -2 is
-// carried in m0 = -2 l0, m1 = -2 l1 (an exact scaling); ZERO0 drops the l0
-// terms of the nuclear attraction, where l0 = 0.
-template <int N, bool ZERO0 = false>
-NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, PolyT<N> &P)
+// R_{k+1} = -2 lam (D R_k + k R_{k-1}), lam = l0 + l1 u^2.  R_t is the same
+// scaled Hermite polynomial as above, in lam: P is hermite_coulomb0 with
+// l1 = 1 (a polynomial in lam) composed with lam = l0 + l1 u^2 (a Taylor
+// shift by l0, then the scaling by l1).
+template <int N> NRX_HD void hermite_coulomb(const PolyT<N> &hk, double D, double l0, double l1, PolyT<N> &P)
 {
-    double m0 = -2.0 * l0, m1 = -2.0 * l1;
-    double Rm[N], R[N];
-    NRX_UNROLL for (int j = 0; j < N; ++j) {
-        R[j] = (j == 0) ? 1.0 : 0.0;
-        Rm[j] = 0.0;
-        P.c[j] = (j == 0) ? hk.c[0] : 0.0;
-    }
-    NRX_UNROLL for (int k = 0; k + 1 < N; ++k) {
-        double t[N];
-        NRX_UNROLL for (int j = 0; j <= k; ++j) t[j] = D * R[j] + k * Rm[j];
-        double Rn[N];
-        NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
-            double v = (!ZERO0 && j <= k) ? m0 * t[j] : 0.0;
-            if (j >= 1) v = (ZERO0 || j > k) ? m1 * t[j - 1] : v + m1 * t[j - 1];
-            Rn[j] = v;
-        }
-        NRX_UNROLL for (int j = 0; j <= k + 1; ++j) {
-            Rm[j] = (j <= k) ? R[j] : 0.0;
-            R[j] = Rn[j];
-            P.c[j] += hk.c[k + 1] * Rn[j];
-        }
-    }
-    P.n = hk.n;
-    NRX_CNT((ZERO0 ? 3 : 4) * hk.n * hk.n);
+    PolyT<N> q;
+    hermite_coulomb0(hk, D, 1.0, q);
+    p_compose(q, l0, l1, P);
 }
 
 NRX_HD void underflow(double &x)
