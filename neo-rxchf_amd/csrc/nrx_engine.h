@@ -204,27 +204,34 @@ template <int N> NRX_HD void p_compose(const PolyT<N> &h, double m0, double beta
 }
 
 // Hermite-Gaussian expansion: f(s) exp(-w s^2) = sum_t e_t Lambda_t(s),
-// Lambda_t = d^t/dc^t exp(-w (s - c)^2) at c = 0;
-// s Lambda_t = Lambda_{t+1}/(2w) + t Lambda_{t-1}  (Horner in s)
+// Lambda_t = d^t/dc^t exp(-w (s - c)^2) at c = 0.  From
+// s Lambda_t = Lambda_{t+1}/(2w) + t Lambda_{t-1}, in closed form
+//   s^k exp(-w s^2) = sum_{t+2m=k} k! / (t! m! 2^m) a^(t+m) Lambda_t,  a = 1/(2w),
+// so e_t = a^t sum_m f_{t+2m} (t+2m)! / (t! m! 2^m) a^m.
+constexpr double hermite_coef(int t, int m)
+{
+    double r = 1.0;
+    for (int i = t + 1; i <= t + 2 * m; ++i) r *= i; // (t+2m)!/t!
+    for (int i = 2; i <= m; ++i) r /= i;             // / m!
+    for (int i = 0; i < m; ++i) r *= 0.5;            // / 2^m
+    return r;
+}
+
 template <int N> NRX_HD void p_hermite(const PolyT<N> &f, double w, PolyT<N> &e)
 {
-    double i2w = 0.5 / w;
-    double cur[N];
-    NRX_UNROLL for (int t = 0; t < N; ++t) cur[t] = 0.0;
-    cur[0] = f.c[N - 1];
-    NRX_UNROLL for (int k = N - 2; k >= 0; --k) {
-        double nw[N];
-        NRX_UNROLL for (int t = 0; t < N; ++t) {
-            double v = (t >= 1) ? cur[t - 1] * i2w : 0.0;
-            if (t + 1 < N) v += (t + 1) * cur[t + 1];
-            nw[t] = v;
-        }
-        nw[0] += f.c[k];
-        NRX_UNROLL for (int t = 0; t < N; ++t) cur[t] = nw[t];
+    double ap[N];
+    ap[0] = 1.0;
+    const double a = 0.5 / w;
+    NRX_UNROLL for (int i = 1; i < N; ++i) ap[i] = ap[i - 1] * a;
+    double t_[N];
+    NRX_UNROLL for (int t = 0; t < N; ++t) {
+        double acc = 0.0;
+        NRX_UNROLL for (int m = 0; t + 2 * m < N; ++m) acc += f.c[t + 2 * m] * hermite_coef(t, m) * ap[m];
+        t_[t] = acc * ap[t];
     }
-    NRX_UNROLL for (int t = 0; t < N; ++t) e.c[t] = cur[t];
+    NRX_UNROLL for (int t = 0; t < N; ++t) e.c[t] = t_[t];
     e.n = f.n;
-    NRX_CNT(2 * f.n * f.n);
+    NRX_CNT(f.n * f.n / 2 + 2 * f.n);
 }
 
 // ------------------------------------------------------------------
