@@ -509,6 +509,8 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
                       double *ovl_out = nullptr)
 {
     NRX_FENCE();
+    // the kinetic polynomials (degree <= tuple degree < NL) at capacity min(NK, NL)
+    constexpr int FK = NK < NL ? NK : NL;
     // product rule over directions, accumulated direction by direction:
     // R_v = sum_d T_d prod_{d' != d} S_d',  Pr = prod_d S_d
     double Pr = 1.0, R[2] = {0.0, 0.0};
@@ -578,7 +580,7 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
             double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
             // sum_j cf[j] X^{L-2+j} over L-2+j >= 0 by Horner in X, then X^{max(L-2,0)} u_A
             int j0 = (L >= 2) ? 0 : 2 - L;
-            PolyT<NK> F0, F1;
+            PolyT<FK> F0, F1;
             p_const(F0, cf0[4]);
             p_const(F1, cf1[4]);
             NRX_UNROLL for (int j = 3; j >= 0; --j) {
@@ -601,11 +603,11 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
             p_mul(base, m0, q);
             if (g != 0.0) {
                 // partner c's polynomial times X_c and X_c^2, X_c = y_c + sB_c
-                PolyT<NK> F2;
+                PolyT<FK> F2;
                 p_const(F2, 4.0 * g * g);
                 NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
                 NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
-                PolyT<NK> fc;
+                PolyT<FK> fc;
                 fpoly(*s.q[c], d, fc);
                 double xc = s.q[c]->sB[d];
                 PolyT<NL> m1, m2, c1, c2, u;
@@ -781,6 +783,8 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
                      const double *zan = nullptr, bool flush = false)
 {
     NRX_FENCE();
+    // the kinetic polynomials (degree <= tuple degree < NL) at capacity min(NK, NL)
+    constexpr int FK = NK < NL ? NK : NL;
     const int h = s.n - 1;
     double Pr = 1.0, R[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     PolyT<NC> ne[3]; // the hub's Hermite coefficients per direction (NUC)
@@ -861,7 +865,7 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
                                  8.0 * al * dl, 4.0 * al * al};
                 double cf1[5] = {0.0, 4.0 * L * g, -8.0 * dl * g, -8.0 * al * g, 0.0};
                 int j0 = (L >= 2) ? 0 : 2 - L;
-                PolyT<NK> F0, F1;
+                PolyT<FK> F0, F1;
                 p_const(F0, cf0[4]);
                 p_const(F1, cf1[4]);
                 NRX_UNROLL for (int j = 3; j >= 0; --j) {
@@ -883,11 +887,11 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
                 hub_map(s, a, d, F0, m0);
                 p_mul(base, m0, q);
                 if (g != 0.0) {
-                    PolyT<NK> F2;
+                    PolyT<FK> F2;
                     p_const(F2, 4.0 * g * g);
                     NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
                     NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
-                    PolyT<NK> fc;
+                    PolyT<FK> fc;
                     fpoly(*s.q[c], d, fc);
                     double xc = s.q[c]->sB[d];
                     PolyT<NL> m1, m2, c1, c2, u;
