@@ -261,9 +261,9 @@ def main():
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP64_PEAK_TFLOPS, 5),
                 "traffic": TRAFFIC.get(dom) if args.nsample == (1 << 20) else None,
-                "kernel": {1: "k_prim_bin<1,SRC_SORTED> + k_prim_tiers<1,SRC_SORTED>",
-                           2: "k_prim_bin<2,SRC_LIST> + k_prim_tiers<2,SRC_LIST>",
-                           3: "k_prim_bin<3,SRC_LIST> + k_prim_tiers<3,SRC_LIST>"}[dom],
+                "kernel": {1: "k_prim_bin + k_prim_scatter + k_prim_tiers<1,SRC_SORTED>",
+                           2: "k_prim_bin + k_prim_scatter + k_prim_tiers<2,SRC_LIST>",
+                           3: "k_prim_bin + k_prim_scatter + k_prim_tiers<3,SRC_LIST>"}[dom],
                 "flops_per_tuple": flops_per}
 
     fock = None
