@@ -111,6 +111,31 @@ NRX_HD constexpr double dfact(int n) // (n)!! for odd n, 1 for n <= 0
 }
 NRX_HD int imin(int a, int b) { return a < b ? a : b; }
 
+// 1/x and 1/sqrt(x) for the engine's positive, well-scaled exponents (no
+// zeros, infinities or denormals): the hardware estimate refined by two
+// Newton-Raphson steps, r <- r + r (1 - x r) and y <- y + y (1 - x y^2) / 2,
+// which leaves about an ulp; the host build uses the IEEE operations.
+NRX_HD double rcp(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+NRX_HD double rsqrt(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    y = __builtin_fma(0.5 * y, __builtin_fma(-x * y, y, 1.0), y);
+    return __builtin_fma(0.5 * y, __builtin_fma(-x * y, y, 1.0), y);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 template <int N> NRX_HD void p_const(PolyT<N> &p, double v)
 {
     NRX_UNROLL for (int k = 0; k < N; ++k) p.c[k] = 0.0;
@@ -221,7 +246,7 @@ template <int N> NRX_HD void p_hermite(const PolyT<N> &f, double w, PolyT<N> &e)
 {
     double ap[N];
     ap[0] = 1.0;
-    const double a = 0.5 / w;
+    const double a = 0.5 * rcp(w);
     NRX_UNROLL for (int i = 1; i < N; ++i) ap[i] = ap[i - 1] * a;
     double t_[N];
     NRX_UNROLL for (int t = 0; t < N; ++t) {
@@ -274,7 +299,7 @@ template <int N> NRX_HD double boys_dot(const PolyT<N> &c, double T, const doubl
     int M = c.n - 1;
     double s = 0.0;
     if (T > 20.0) {
-        double f = 0.88622692545275801365 / sqrt(T), it = 1.0 / T;
+        double rT = rsqrt(T), f = 0.88622692545275801365 * rT, it = rT * rT;
         s = c.c[0] * f;
         NRX_UNROLL for (int l = 1; l < N; ++l) {
             f = f * (l - 0.5) * it;
@@ -367,10 +392,10 @@ template <int NLF, int NQ>
 NRX_HD double absorb_leaf(const PolyT<NLF> &LF, double Lw, double Lc, double g, double d, VarT<NQ> &Q, double pic)
 {
     // one reciprocal per new exponent, the quotients as products
-    double W = Lw + g, iW = 1.0 / W;
+    double W = Lw + g, rW = rsqrt(W), iW = rW * rW;
     PolyT<NLF> h;
     p_smooth(LF, 0.5 * iW, h);
-    double fac = sqrt(pic * iW);
+    double fac = sqrt(pic) * rW;
     if (g == 0.0) {
         double v = p_eval(h, Lc);
         NRX_UNROLL for (int k = 0; k < NQ; ++k) Q.F.c[k] *= v;
@@ -381,7 +406,7 @@ NRX_HD double absorb_leaf(const PolyT<NLF> &LF, double Lw, double Lc, double g, 
     p_mul(Q.F, h, Q.F);
     double kap = Lw * beta;
     double cl = d + Lc;
-    double Wq = Q.w + kap, iWq = 1.0 / Wq;
+    double Wq = Q.w + kap, iWq = rcp(Wq);
     double diff = Q.c - cl;
     fac *= exp(-(Q.w * kap * iWq) * diff * diff);
     Q.c = (Q.w * Q.c + kap * cl) * iWq;
@@ -401,9 +426,9 @@ template <int N> NRX_HD double close_expect(const PolyT<N> &F, double iw, double
 template <int N> NRX_HD double close_var(const VarT<N> &V, double pic)
 {
     PolyT<N> h;
-    double iw = 1.0 / V.w;
+    double rw = rsqrt(V.w), iw = rw * rw;
     p_smooth(V.F, 0.5 * iw, h);
-    return sqrt(pic * iw) * p_eval(h, V.c);
+    return sqrt(pic) * rw * p_eval(h, V.c);
 }
 
 // Integral description: particles 0..n-1, hub = n-1, electron i linked to the
@@ -476,7 +501,7 @@ template <int NF, int NR> NRX_HD void hub_map(const Sys &s, int i, int d, const 
         return;
     }
     const PairRec &Q = *s.q[i];
-    double g = s.gam[i], iW = 1.0 / (Q.p + g);
+    double g = s.gam[i], iW = rcp(Q.p + g);
     PolyT<NF> t;
     p_smooth(f, 0.5 * iW, t);
     if (g == 0.0) {
@@ -522,9 +547,9 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
         NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
             if (i >= h) continue;
             const PairRec &Q = *s.q[i];
-            double g = s.gam[i], iW = 1.0 / (Q.p + g), fl = sqrt(pic * iW);
+            double g = s.gam[i], rW = rsqrt(Q.p + g), iW = rW * rW, fl = sqrt(pic) * rW;
             if (g != 0.0) {
-                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = 1.0 / Wq;
+                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = rcp(Wq);
                 double diff = cq - cl;
                 fl *= exp(-(w * kap * iWq) * diff * diff);
                 cq = (w * cq + kap * cl) * iWq;
@@ -532,7 +557,7 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
             }
             fac *= fl;
         }
-        const double iw = 1.0 / w, norm = fac * sqrt(pic * iw);
+        const double rw = rsqrt(w), iw = rw * rw, norm = fac * (sqrt(pic) * rw);
         // the particles other than a and c
         PolyT<NL> env;
         p_const(env, 1.0);
@@ -760,7 +785,7 @@ NRX_HD double nuc_sum(const Sys &s, int a, int nat, const double *cat, const dou
     PolyT<NL> e[3];
     double X[3], w = 1.0, fac = Kprod(s);
     NRX_UNROLL for (int d = 0; d < 3; ++d) fac *= nuc_core<NL>(s, a, d, e[d], X[d], w, pic);
-    return nuc_centres<NL, NBC>(e, X, w, fac * 2.0 * pic / w, nat, cat, zan, flush, s.bt);
+    return nuc_centres<NL, NBC>(e, X, w, fac * 2.0 * pic * rcp(w), nat, cat, zan, flush, s.bt);
 }
 
 // One kinetic operator of kin_pair: particle a, partner c (c < 0: none),
@@ -794,9 +819,9 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
         NRX_UNROLL for (int i = 0; i < NPART - 1; ++i) {
             if (i >= h) continue;
             const PairRec &Q = *s.q[i];
-            double g = s.gam[i], iW = 1.0 / (Q.p + g), fl = sqrt(pic * iW);
+            double g = s.gam[i], rW = rsqrt(Q.p + g), iW = rW * rW, fl = sqrt(pic) * rW;
             if (g != 0.0) {
-                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = 1.0 / Wq;
+                double kap = Q.p * (g * iW), cl = Q.P[d] - s.q[h]->P[d], Wq = w + kap, iWq = rcp(Wq);
                 double diff = cq - cl;
                 fl *= exp(-(w * kap * iWq) * diff * diff);
                 cq = (w * cq + kap * cl) * iWq;
@@ -804,7 +829,7 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
             }
             fac *= fl;
         }
-        const double iw = 1.0 / w, norm = fac * sqrt(pic * iw);
+        const double rw = rsqrt(w), iw = rw * rw, norm = fac * (sqrt(pic) * rw);
         // every particle's own polynomial in the hub coordinate
         PolyT<NL> m[NPART];
         NRX_UNROLL for (int i = 0; i < NPART; ++i) {
@@ -925,7 +950,7 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
         double r = PI / pic, sc = 1.0;
         NRX_UNROLL for (int i = 0; i < NPART - 1; ++i)
             if (i < h) sc *= r * sqrt(r);
-        *nuc_out = nuc_centres<NC, NBC>(ne, nX, nw, kp * nfac * sc * 2.0 * PI / nw, nat, cat, zan, flush, s.bt);
+        *nuc_out = nuc_centres<NC, NBC>(ne, nX, nw, kp * nfac * sc * 2.0 * PI * rcp(nw), nat, cat, zan, flush, s.bt);
     }
 }
 
@@ -990,31 +1015,31 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
             } else {
                 // hub linked to both: integrate the hub conditionally on (y_a, y_b)
                 cycle = true;
-                double W = H.w + ga + gb;
+                double W = H.w + ga + gb, rW = rsqrt(W), iW = rW * rW;
                 double dpa = s.q[h]->P[d] - s.q[a]->P[d], dpb = s.q[h]->P[d] - s.q[b]->P[d];
-                fac *= sqrt(pic / W);
-                double ka = H.w * ga / W, kb = H.w * gb / W;
+                fac *= sqrt(pic) * rW;
+                double ka = H.w * ga * iW, kb = H.w * gb * iW;
                 double ca = dpa + H.c, cb = dpb + H.c;
                 {
-                    double Wn = A.w + ka, df = A.c - ca;
-                    fac *= exp(-(A.w * ka / Wn) * df * df);
-                    A.c = (A.w * A.c + ka * ca) / Wn;
+                    double Wn = A.w + ka, iWn = rcp(Wn), df = A.c - ca;
+                    fac *= exp(-(A.w * ka * iWn) * df * df);
+                    A.c = (A.w * A.c + ka * ca) * iWn;
                     A.w = Wn;
                 }
                 {
-                    double Wn = B.w + kb, df = B.c - cb;
-                    fac *= exp(-(B.w * kb / Wn) * df * df);
-                    B.c = (B.w * B.c + kb * cb) / Wn;
+                    double Wn = B.w + kb, iWn = rcp(Wn), df = B.c - cb;
+                    fac *= exp(-(B.w * kb * iWn) * df * df);
+                    B.c = (B.w * B.c + kb * cb) * iWn;
                     B.w = Wn;
                 }
-                gab = ga * gb / W;
+                gab = ga * gb * iW;
                 // h(m0 + ba ya + bb yb) = sum_j (bb yb)^j / j! h^{(j)}(m0 + ba ya);
                 // the hub polynomial has degree <= NS - 1 here (its own
                 // polynomial only: every other electron is a or b)
                 PolyT<NS> hs;
                 p_copy(H.F, hs);
-                p_smooth(hs, 0.5 / W, hs);
-                double m0 = (H.w * H.c - ga * dpa - gb * dpb) / W, ba = ga / W, bb = gb / W;
+                p_smooth(hs, 0.5 * iW, hs);
+                double m0 = (H.w * H.c - ga * dpa - gb * dpb) * iW, ba = ga * iW, bb = gb * iW;
                 // D = h^{(j)} / j!  (D_{j+1,k} = (k+1)/(j+1) D_{j,k+1}),
                 // Yb = B.F (bb yb)^j; the loop stays rolled, one term live,
                 // and ends after the hub polynomial's degree (D = 0 beyond)
@@ -1031,7 +1056,7 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
                     p_hermite(Fa, A.w, ea);
                     p_hermite(Fb, B.w, eb);
                     conv_add(ea, eb, hk);
-                    double ij = 1.0 / (j + 1);
+                    double ij = rcp((double)(j + 1));
                     NRX_UNROLL for (int k = 0; k < NS; ++k) D.c[k] = (k + 1 < NS) ? D.c[k + 1] * ((k + 1) * ij) : 0.0;
                     NRX_UNROLL for (int k = NL - 1; k >= 0; --k) Yb.c[k] = (k >= 1) ? Yb.c[k - 1] * bb : 0.0;
                     Yb.n = imin(Yb.n + 1, NL);
@@ -1048,8 +1073,8 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
         }
         wa = A.w;
         wb = B.w;
-        double rho = wa * wb / (wa + wb);
-        double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
+        double rho = wa * wb * rcp(wa + wb), irg = rcp(rho + gab);
+        double l0 = rho * gab * irg, l1 = rho * rho * irg;
         double Dd = (s.q[a]->P[d] + A.c) - (s.q[b]->P[d] + B.c);
         PolyT<NL> P;
         if (gab == 0.0) // l0 = 0: no geminal between a and b (uniform across a wave)
@@ -1059,10 +1084,11 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
         p_mul(cc, P, cc);
         D2 += Dd * Dd;
     }
-    double rho = wa * wb / (wa + wb);
-    double l0 = rho * gab / (rho + gab), l1 = rho * rho / (rho + gab);
-    double pre = pic / (wa + wb);
-    pre = pre * sqrt(pre) * 2.0 * pic / (rho + gab);
+    double rab = rsqrt(wa + wb), iab = rab * rab;
+    double rho = wa * wb * iab, irg = rcp(rho + gab);
+    double l0 = rho * gab * irg, l1 = rho * rho * irg;
+    double pre = pic * iab;
+    pre = pre * (sqrt(pic) * rab) * 2.0 * pic * irg;
     if (hub_ovl) {
         double k = 1.0;
         NRX_UNROLL for (int i = 0; i < NPART; ++i)
@@ -1483,11 +1509,19 @@ NRX_HD int tuple_L(const PairRec *const *q, int n)
     return L;
 }
 
-// the capacity tier a tuple runs in
+// the capacity tier a tuple runs in.  The e-e-e-p tuples of classes 3 and 7
+// also apply a one-particle kinetic operator to e3 (their hcore term), whose
+// polynomial needs e3's own largest per-direction degree + 2.
 template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
 {
     using T = Tier<CLS>;
     int deg = tuple_degree(q, nel<CLS>() + 1, T::KIN), L = tuple_L(q, nel<CLS>() + 1);
+    if constexpr (CLS == 3 || CLS == 7) {
+        NRX_UNROLL for (int d = 0; d < 3; ++d) {
+            int k = q[2]->la[d] + q[2]->lb[d] + 2;
+            deg = k > deg ? k : deg;
+        }
+    }
     for (int k = 0; k + 1 < T::N; ++k)
         if (deg <= T::NL[k] - 1 && L <= T::NB[k] - 1) return k;
     return T::N - 1;

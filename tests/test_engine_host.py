@@ -134,3 +134,53 @@ def test_engine_rxchfmult_xchf_side_vs_oracle():
     for tier in (0, 1):
         got = engine_prim_list3(s, 7, idx, tier)[0].reshape(-1, 6).mean(axis=1)
         check_close(got, ref, what=f"cls7 six-permutation average tier {tier}")
+
+
+def _lone_am_tuples(s, cls):
+    """Tuples of class cls whose angular momentum sits on one particle: every
+    particle s-type except one, whose bra and ket are d-type along one axis
+    (or d and p).  These put one particle's polynomial at degree 4 with the
+    rest at 0, the case where a one-particle kinetic operator needs more
+    capacity than the summed degrees suggest."""
+    am, pm = np.asarray(s["elcam"]), np.asarray(s["nucam"])
+    ne = {1: 1, 2: 2, 3: 3, 4: 2, 7: 3, 8: 1}[cls]
+    e_s = int(np.flatnonzero(am.sum(1) == 0)[0])
+    p_s = int(np.flatnonzero(pm.sum(1) == 0)[0])
+    e_d = [int(i) for i in np.flatnonzero(am.max(1) == 2)[:3]]
+    e_p = [int(i) for i in np.flatnonzero(am.sum(1) == 1)[:3]]
+    p_d = [int(i) for i in np.flatnonzero(pm.max(1) == 2)[:3]]
+    rows = []
+    for j in range(ne + 1):
+        heavy = p_d if j == ne else e_d
+        other = e_p if j < ne else [int(i) for i in np.flatnonzero(pm.sum(1) == 1)[:3]]
+        for a in heavy:
+            for b in heavy + other:
+                bra = [e_s] * ne + [p_s]
+                ket = [e_s] * ne + [p_s]
+                bra[j], ket[j] = a, b
+                rows.append(bra + ket)
+    return np.asarray(rows, dtype=np.int64)
+
+
+@pytest.mark.parametrize("cls", [1, 2, 3, 4, 7, 8])
+def test_engine_tiers_lone_angular_momentum(cls):
+    """Every capacity tier gives the general tier's values (and the
+    oracle's) on tuples with all angular momentum on one particle, and on a
+    random sample; regression for a tier that lacked the +2 of the e-e-e-p
+    tuple's one-particle kinetic operator on e3."""
+    from _util import engine_prim_list3
+
+    s = synth.config2()
+    ne = {1: 1, 2: 2, 3: 3, 4: 2, 7: 3, 8: 1}[cls]
+    idx = np.concatenate([_lone_am_tuples(s, cls), random_tuples(s, ne, 150, seed=90 + cls)])
+    if cls in (7, 8):
+        t0 = engine_prim_list3(s, cls, idx, 0)[0]
+        t1 = engine_prim_list3(s, cls, idx, 1)[0]
+        ref = oracle.prim_list3(s, cls, idx)[0] if cls == 8 else None
+    else:
+        t0 = engine_prim_list(s, cls, idx, 0)[0]
+        t1 = engine_prim_list(s, cls, idx, 1)[0]
+        ref = oracle.prim_list(s, cls, idx)[0]
+    check_close(t0, t1, rtol=1e-13, atol_rel=1e-14, what=f"cls{cls} device tiers vs general tier")
+    if ref is not None:
+        check_close(t0, ref, what=f"cls{cls} device tiers vs oracle")

@@ -517,3 +517,22 @@ def test_rxchfmult_exchange(torch_dev):
             eo = oracle.rxchfmult_omgex(order, which, ne, npb, DA, DB, DP, G, X1, X2, B)
             check_close(A - F0, B - F0, what=f"RXCHFmult omg{order}ex {which}")
             assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cls", [1, 2, 3, 4])
+def test_prim_tiers_lone_angular_momentum(torch_dev, cls):
+    """The device's tier selection on tuples with all angular momentum on one
+    particle (the capacity boundary of the one-particle kinetic terms) and on
+    a larger random sample, against the oracle."""
+    from test_engine_host import _lone_am_tuples
+
+    print(f"case lone-am tiers cls={cls}", flush=True)
+    s = synth.config2()
+    S = neorx.System(s)
+    ne = {1: 1, 2: 2, 3: 3, 4: 2}[cls]
+    idx = np.concatenate([_lone_am_tuples(s, cls), random_tuples(s, ne, 512 if cls != 3 else 384, seed=700 + cls)])
+    ge, _ = _run_list(torch_dev, S, cls, idx)
+    oe, _ = oracle.prim_list(s, cls, idx)
+    check_close(ge, oe, what=f"lone-am cls{cls}")
+    S.close()
