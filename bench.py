@@ -301,7 +301,7 @@ def main():
 # make nonzero (tests/count_flops.py); traffic = HBM bytes per launch of the
 # dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), scaled
 # by the copy-kernel calibration.
-FLOPS_PER_TUPLE = {1: 56733.7, 2: 112808.3, 3: 33681.3}
+FLOPS_PER_TUPLE = {1: 56733.7, 2: 113053.3, 3: 33681.3}
 TRAFFIC = {2: 362.49e6}  # e-e-p launch group (bin + scatter + tiers) at the default 1,048,576 sampled tuples
 
 if __name__ == "__main__":

@@ -1041,25 +1041,40 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
                 p_smooth(hs, 0.5 * iW, hs);
                 double m0 = (H.w * H.c - ga * dpa - gb * dpb) * iW, ba = ga * iW, bb = gb * iW;
                 // D = h^{(j)} / j!  (D_{j+1,k} = (k+1)/(j+1) D_{j,k+1}),
-                // Yb = B.F (bb yb)^j; the loop stays rolled, one term live,
-                // and ends after the hub polynomial's degree (D = 0 beyond)
+                // Yb = B.F (bb yb)^j, kept as the Hermite coefficients eb of
+                // Yb(s + B.c) exp(-B.w s^2): the next term multiplies by
+                // bb (s + B.c), which s Lambda_t = Lambda_{t+1}/(2w) + t Lambda_{t-1}
+                // turns into a three-term update.  The loop stays rolled, one
+                // term live, and ends after the hub polynomial's degree (D = 0
+                // beyond, and Yb's degree stays within capacity until then).
                 PolyT<NS> D = hs;
-                PolyT<NL> Yb;
-                p_copy(B.F, Yb);
+                PolyT<NL> eb;
+                {
+                    PolyT<NL> Fb;
+                    p_compose(B.F, B.c, 1.0, Fb);
+                    p_hermite(Fb, B.w, eb);
+                }
+                const double i2wb = 0.5 * rcp(B.w);
                 _Pragma("nounroll") for (int j = 0; j < hs.n; ++j) {
                     PolyT<NS> der;
                     p_compose(D, m0, ba, der);
-                    PolyT<NL> Fa, Fb, ea, eb;
+                    PolyT<NL> Fa, ea;
                     p_mul(A.F, der, Fa);
                     p_compose(Fa, A.c, 1.0, Fa);
-                    p_compose(Yb, B.c, 1.0, Fb);
                     p_hermite(Fa, A.w, ea);
-                    p_hermite(Fb, B.w, eb);
                     conv_add(ea, eb, hk);
                     double ij = rcp((double)(j + 1));
                     NRX_UNROLL for (int k = 0; k < NS; ++k) D.c[k] = (k + 1 < NS) ? D.c[k + 1] * ((k + 1) * ij) : 0.0;
-                    NRX_UNROLL for (int k = NL - 1; k >= 0; --k) Yb.c[k] = (k >= 1) ? Yb.c[k - 1] * bb : 0.0;
-                    Yb.n = imin(Yb.n + 1, NL);
+                    double nb[NL];
+                    NRX_UNROLL for (int t = 0; t < NL; ++t) {
+                        double v = B.c * eb.c[t];
+                        if (t >= 1) v += eb.c[t - 1] * i2wb;
+                        if (t + 1 < NL) v += (t + 1) * eb.c[t + 1];
+                        nb[t] = bb * v;
+                    }
+                    NRX_UNROLL for (int t = 0; t < NL; ++t) eb.c[t] = nb[t];
+                    eb.n = imin(eb.n + 1, NL);
+                    NRX_CNT(5 * eb.n);
                 }
             }
         }
