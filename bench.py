@@ -302,7 +302,7 @@ def main():
 # dominant kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/), scaled
 # by the copy-kernel calibration.
 FLOPS_PER_TUPLE = {1: 56733.7, 2: 113053.3, 3: 33681.3}
-TRAFFIC = {2: 362.49e6}  # e-e-p launch group (bin + scatter + tiers) at the default 1,048,576 sampled tuples
+TRAFFIC = {2: 366.37e6}  # e-e-p launch group (bin + scatter + tiers) at the default 1,048,576 sampled tuples
 
 if __name__ == "__main__":
     main()
