@@ -536,3 +536,31 @@ def test_prim_tiers_lone_angular_momentum(torch_dev, cls):
     oe, _ = oracle.prim_list(s, cls, idx)
     check_close(ge, oe, what=f"lone-am cls{cls}")
     S.close()
+
+
+def test_rxchfmult_tiers_lone_angular_momentum(torch_dev):
+    """RXCHFmult's XCHF-side tuples (classes 8 and 7) through the device's
+    tier selection, on tuples with all angular momentum on one particle:
+    class 8 against the oracle, class 7 after the six-permutation average
+    (which moves the heavy electron through every position, e3 included)."""
+    from itertools import permutations
+
+    from test_engine_host import _lone_am_tuples
+
+    torch = torch_dev
+    s = synth.config2()
+    S = neorx.System(s)
+    idx = _lone_am_tuples(s, 8)
+    print(f"case rxchfmult lone-am cls8 n={len(idx)}", flush=True)
+    got = _run_list3(torch, S, 8, idx)
+    ref = oracle.prim_list3(s, 8, idx)
+    check_close(got[0], ref[0], what="lone-am cls8 E")
+    base = _lone_am_tuples(s, 7)
+    rows = [[x[p[0]], x[p[1]], x[p[2]], x[3], x[4 + p[0]], x[4 + p[1]], x[4 + p[2]], x[7]]
+            for x in base for p in permutations(range(3))]
+    idx = np.array(rows, dtype=np.int64)
+    print(f"case rxchfmult lone-am cls7 n={len(idx)}", flush=True)
+    got = _run_list3(torch, S, 7, idx)[0].reshape(-1, 6).mean(axis=1)
+    ref = oracle.prim_list3(s, 7, idx)[0].reshape(-1, 6).mean(axis=1)
+    check_close(got, ref, what="lone-am cls7 six-permutation average")
+    S.close()
