@@ -1362,18 +1362,31 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                     const unsigned o0 = b * ne4 + d * ne2 + ef;
                     const double *dd = DE + d * ne, *db = DE + b * ne, *df = DE + f * ne;
                     unsigned c = 0;
-                    // eight independent loads in flight per lane before the FMAs;
-                    // the slab is read once, so the loads are nontemporal
-                    for (; c + 8 <= ne; c += 8) {
+                    // twelve (then eight) independent loads in flight per lane
+                    // before the FMAs; the slab is read once, so the loads are
+                    // nontemporal
+                    for (; c + 12 <= ne; c += 12) {
+                        double gv[12];
+#pragma unroll
+                        for (int k = 0; k < 12; ++k) gv[k] = (double)__builtin_nontemporal_load(&g[o0 + (c + k) * ne3]);
+#pragma unroll
+                        for (int k = 0; k < 12; ++k) {
+                            s1 += dd[c + k] * gv[k]; // DE(c,d)
+                            s2 += db[c + k] * gv[k]; // DE(c,b)
+                            s3 += df[c + k] * gv[k]; // DE(c,f)
+                        }
+                    }
+                    if (c + 8 <= ne) {
                         double gv[8];
 #pragma unroll
                         for (int k = 0; k < 8; ++k) gv[k] = (double)__builtin_nontemporal_load(&g[o0 + (c + k) * ne3]);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            s1 += dd[c + k] * gv[k]; // DE(c,d)
-                            s2 += db[c + k] * gv[k]; // DE(c,b)
-                            s3 += df[c + k] * gv[k]; // DE(c,f)
+                            s1 += dd[c + k] * gv[k];
+                            s2 += db[c + k] * gv[k];
+                            s3 += df[c + k] * gv[k];
                         }
+                        c += 8;
                     }
                     for (; c < ne; ++c) {
                         double gv = (double)__builtin_nontemporal_load(&g[o0 + c * ne3]);
