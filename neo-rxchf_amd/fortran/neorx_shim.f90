@@ -376,6 +376,56 @@ contains
       kpeend(c) = i
     end do
   end subroutine
+  ! Record length of the .ufm direct-access files, in bytes.  The reference
+  ! opens them with RECL=8 (e.g. source/gam_ee.f:143-144): flang and gfortran
+  ! count RECL in bytes (8-byte records), ifort in 4-byte words (32-byte
+  ! records, each double followed by 24 zero bytes; the reference's own
+  ! tests/TEST_XCRHF/TEST_06/novlap.ufm is laid out so).  NEORX_UFM_RECL=32
+  ! makes these files match an ifort-built driver; 8 is the default.
+  integer(c_int64_t) function ufm_recl()
+    character(len=32) :: v
+    integer :: n, st
+    ufm_recl = 8
+    call get_environment_variable('NEORX_UFM_RECL', v, n, st)
+    if (st == 0 .and. n > 0) read(v, *) ufm_recl
+    if (ufm_recl /= 8 .and. ufm_recl /= 32) then
+      write(0, '(a,a)') 'neorx: NEORX_UFM_RECL must be 8 or 32, got ', trim(v)
+      error stop 1
+    end if
+  end function
+
+  ! write a(1:n) to fname, one double per record
+  subroutine ufm_write(u, fname, n, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: fname
+    integer(c_int64_t), intent(in) :: n
+    real(c_double), intent(in) :: a(n)
+    integer(c_int64_t) :: i, rl
+    rl = ufm_recl()
+    open(u, file=fname, form='unformatted', status='unknown', access='direct', recl=rl)
+    do i = 1, n
+      if (rl == 8) then
+        write(u, rec=i) a(i)
+      else
+        write(u, rec=i) a(i), 0.0d0, 0.0d0, 0.0d0
+      end if
+    end do
+    close(u)
+  end subroutine
+
+  ! read a(1:n) from fname, one double per record
+  subroutine ufm_read(u, fname, n, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: fname
+    integer(c_int64_t), intent(in) :: n
+    real(c_double), intent(out) :: a(n)
+    integer(c_int64_t) :: i
+    open(u, file=fname, form='unformatted', status='old', access='direct', recl=ufm_recl())
+    do i = 1, n
+      read(u, rec=i) a(i)
+    end do
+    close(u)
+  end subroutine
 end module neorx_capi
 
 ! GAM1_OMP_MD, source/gam_1_OMP.f:2-397: e-p integrals written to
@@ -389,7 +439,7 @@ subroutine GAM1_OMP_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, z
   real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
   integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
   real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
-  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf)
+  integer(c_int64_t) :: h, kpestr(nebf), kpeend(nebf)
   real(c_double), allocatable :: g(:), gs(:)
   call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
   allocate(g(ng1), gs(ng1))
@@ -397,14 +447,8 @@ subroutine GAM1_OMP_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, z
                    kpeend, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'GAM1_OMP_MD')
   call neorx_check(neorx_gam1(h, g, gs), 'GAM1_OMP_MD')
   call neorx_check(neorx_system_destroy(h), 'GAM1_OMP_MD')
-  open(801, file='GAM_1.ufm', form='unformatted', status='unknown', access='direct', recl=8)
-  open(802, file='GAM_1s.ufm', form='unformatted', status='unknown', access='direct', recl=8)
-  do ia = 1, ng1
-    write(801, rec=ia) g(ia)
-    write(802, rec=ia) gs(ia)
-  end do
-  close(801)
-  close(802)
+  call ufm_write(801, 'GAM_1.ufm', ng1, g)
+  call ufm_write(802, 'GAM_1s.ufm', ng1, gs)
   deallocate(g, gs)
 end subroutine
 
@@ -415,7 +459,7 @@ subroutine calc_GAM_ee(nebf, npebf, ngee, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC
   integer(c_int64_t) :: nebf, npebf, ngee
   integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3)
   real(c_double) :: AGEBFCC(npebf), ELCEX(npebf), ELCBFC(npebf, 3)
-  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf), nucam(3)
+  integer(c_int64_t) :: h, kpestr(nebf), kpeend(nebf), nucam(3)
   real(c_double) :: one(1), zero3(3), g(ngee)
   call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
   one = 1.0d0
@@ -426,11 +470,7 @@ subroutine calc_GAM_ee(nebf, npebf, ngee, AMPEB2C, AGEBFCC, ELCEX, ELCAM, ELCBFC
                    'calc_GAM_ee')
   call neorx_check(neorx_gam_ee(h, g), 'calc_GAM_ee')
   call neorx_check(neorx_system_destroy(h), 'calc_GAM_ee')
-  open(810, file='GAM_ee.ufm', form='unformatted', status='unknown', access='direct', recl=8)
-  do ia = 1, ngee
-    write(810, rec=ia) g(ia)
-  end do
-  close(810)
+  call ufm_write(810, 'GAM_ee.ufm', ngee, g)
 end subroutine
 
 ! GAM2_IC1, source/gam2_IC1_OMP.f:2-212
@@ -476,13 +516,8 @@ subroutine GAM4_ICR(Nchunks, ne, np, ngee, ng2, ng4, GAM_2s, GAM_4)
   integer(c_int64_t) :: Nchunks, ne, np, ngee, ng2, ng4
   real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
   real(c_double), allocatable :: gee(:)
-  integer(c_int64_t) :: ia
   allocate(gee(ngee))
-  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do ia = 1, ngee
-    read(810, rec=ia) gee(ia)
-  end do
-  close(810)
+  call ufm_read(810, 'GAM_ee.ufm', ngee, gee)
   call neorx_check(neorx_gam4_icr(ne, np, GAM_2s, gee, GAM_4), 'GAM4_ICR')
   deallocate(gee)
 end subroutine
@@ -561,18 +596,9 @@ subroutine GAM4_DSCF(NG4CHK, ne, np, ngee, ng2, ng4, DE, DP, focke, fockp, E_gam
   integer(c_int64_t) :: NG4CHK, ne, np, ngee, ng2, ng4
   real(c_double) :: DE(ne, ne), DP(np, np), focke(ne, ne), fockp(np, np), E_gam4
   real(c_double), allocatable :: gee(:), g2s(:)
-  integer(c_int64_t) :: ia
   allocate(gee(ngee), g2s(ng2))
-  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do ia = 1, ngee
-    read(810, rec=ia) gee(ia)
-  end do
-  close(810)
-  open(804, file='GAM_2s.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do ia = 1, ng2
-    read(804, rec=ia) g2s(ia)
-  end do
-  close(804)
+  call ufm_read(810, 'GAM_ee.ufm', ngee, gee)
+  call ufm_read(804, 'GAM_2s.ufm', ng2, g2s)
   call neorx_check(neorx_gam4_dscf(ne, np, g2s, gee, DE, DP, focke, fockp, E_gam4), 'GAM4_DSCF')
   deallocate(gee, g2s)
 end subroutine
@@ -648,13 +674,8 @@ subroutine OMG4_ICR(Nchunks, ne, np, ngee, ng2, ng4, GAM_2s, GAM_4)
   integer(c_int64_t) :: Nchunks, ne, np, ngee, ng2, ng4
   real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
   real(c_double), allocatable :: gee(:)
-  integer(c_int64_t) :: ia
   allocate(gee(ngee))
-  open(810, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do ia = 1, ngee
-    read(810, rec=ia) gee(ia)
-  end do
-  close(810)
+  call ufm_read(810, 'GAM_ee.ufm', ngee, gee)
   call neorx_check(neorx_omg4_icr(ne, np, GAM_2s, gee, GAM_4), 'OMG4_ICR')
   deallocate(gee)
 end subroutine
@@ -789,7 +810,7 @@ subroutine RXCHFmult_GAM1_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, 
   real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
   integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
   real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
-  integer(c_int64_t) :: h, ia, kpestr(nebf), kpeend(nebf)
+  integer(c_int64_t) :: h, kpestr(nebf), kpeend(nebf)
   real(c_double), allocatable :: g(:), gs(:)
   call neorx_kpe(nebf, npebf, AMPEB2C, kpestr, kpeend)
   allocate(g(ng1), gs(ng1))
@@ -798,14 +819,8 @@ subroutine RXCHFmult_GAM1_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, 
                    'RXCHFmult_GAM1_MD')
   call neorx_check(neorx_rxchfmult_gam1(h, g, gs), 'RXCHFmult_GAM1_MD')
   call neorx_check(neorx_system_destroy(h), 'RXCHFmult_GAM1_MD')
-  open(801, file='GAM_1.ufm', form='unformatted', status='unknown', access='direct', recl=8)
-  open(802, file='GAM_1s.ufm', form='unformatted', status='unknown', access='direct', recl=8)
-  do ia = 1, ng1
-    write(801, rec=ia) g(ia)
-    write(802, rec=ia) gs(ia)
-  end do
-  close(801)
-  close(802)
+  call ufm_write(801, 'GAM_1.ufm', ng1, g)
+  call ufm_write(802, 'GAM_1s.ufm', ng1, gs)
   deallocate(g, gs)
 end subroutine
 
@@ -837,19 +852,9 @@ subroutine RXCHFmult_GAM4_XCHF(Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4, GAM_
   integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4
   real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
   real(c_double), allocatable :: gee(:,:,:,:), geebe(:,:,:,:)
-  integer(c_int64_t) :: i1, j1, i2, j2
   allocate(gee(nebf, nebf, nebf, nebf), geebe(nebfBE, nebfBE, nebfBE, nebfBE))
-  open(812, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do i1 = 1, nebf
-    do j1 = 1, nebf
-      do i2 = 1, nebf
-        do j2 = 1, nebf
-          read(812, rec=j2 + nebf * (i2 - 1 + nebf * (j1 - 1 + nebf * (i1 - 1)))) gee(j2, i2, j1, i1)
-        end do
-      end do
-    end do
-  end do
-  close(812)
+  ! record j2 + nebf (i2-1 + nebf (j1-1 + nebf (i1-1))) holds gee(j2, i2, j1, i1)
+  call ufm_read(812, 'GAM_ee.ufm', nebf ** 4, gee)
   geebe = gee(1:nebfBE, 1:nebfBE, 1:nebfBE, 1:nebfBE)
   call neorx_check(neorx_gam4_icr(nebfBE, npbf, GAM_2s, geebe, GAM_4), 'RXCHFmult_GAM4_XCHF')
   deallocate(gee, geebe)
@@ -888,13 +893,8 @@ subroutine RXCHFmult_GAM4_INT(Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4, GAM_2
   integer(c_int64_t) :: Nchunks, nebf, nebfBE, npbf, ngee, ng2, ng4
   real(c_double) :: GAM_2s(ng2), GAM_4(ng4)
   real(c_double), allocatable :: gee(:)
-  integer(c_int64_t) :: ia
   allocate(gee(ngee))
-  open(811, file='GAM_ee.ufm', form='unformatted', status='old', access='direct', recl=8)
-  do ia = 1, ngee
-    read(811, rec=ia) gee(ia)
-  end do
-  close(811)
+  call ufm_read(811, 'GAM_ee.ufm', ngee, gee)
   call neorx_check(neorx_rxchfmult_gam4_int(nebf, nebfBE, npbf, GAM_2s, gee, GAM_4), 'RXCHFmult_GAM4_INT')
   deallocate(gee)
 end subroutine

@@ -4,6 +4,7 @@
 ! Input and output are raw stream files written/read by the GPU tests.
 program drive_shim
   use iso_c_binding
+  use neorx_capi, only: ufm_read, ufm_write
   implicit none
   interface
     subroutine GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
@@ -70,11 +71,9 @@ program drive_shim
   write(11) gm4, fe, fp, e4
   ! direct-SCF routines (xchf1_fock.f:247, :273, :290); GAM4_DSCF reads
   ! GAM_2s.ufm, which the reference's integral phase writes
-  open(804, file='GAM_2s.ufm', form='unformatted', status='replace', access='direct', recl=8)
-  do ia = 1, ng2
-    write(804, rec=ia) gm2s(ia)
-  end do
-  close(804)
+  ! (the driver's own .ufm I/O follows NEORX_UFM_RECL as the library does,
+  ! like a driver and library built with the same compiler)
+  call ufm_write(804, 'GAM_2s.ufm', ng2, gm2s)
   fe = 0.0d0
   fp = 0.0d0
   se = 0.0d0
@@ -224,11 +223,6 @@ contains
     character(len=*), intent(in) :: name
     integer(c_int64_t), intent(in) :: n
     real(c_double), intent(out) :: a(n)
-    integer(c_int64_t) :: k
-    open(820, file=name, form='unformatted', status='old', access='direct', recl=8)
-    do k = 1, n
-      read(820, rec=k) a(k)
-    end do
-    close(820)
+    call ufm_read(820, name, n, a)
   end subroutine
 end program

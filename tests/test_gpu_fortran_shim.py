@@ -43,7 +43,11 @@ def _write_input(path, s, DE, DP):
 
 
 @pytest.mark.gpu
-def test_fortran_dropin_matches_oracle():
+@pytest.mark.parametrize("recl", [8, 32])
+def test_fortran_dropin_matches_oracle(recl):
+    """recl: .ufm record length (NEORX_UFM_RECL): 8 bytes as flang/gfortran
+    write RECL=8, 32 as an ifort-built driver does (each double followed by 24
+    zero bytes); the driver's GAM4_ICR reads back the GAM_ee.ufm it wrote."""
     s = synth.small(seed=81, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,), ngtg=2)
     ne, npb = s["nebf"], s["npbf"]
     sub = dict(s)
@@ -63,12 +67,16 @@ def test_fortran_dropin_matches_oracle():
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         _write_input(fin, s, DE, DP)
-        print("case fortran drop-in driver", flush=True)
-        r = subprocess.run([DRIVER, fin, fout], cwd=td, capture_output=True, text=True, timeout=240)
+        print(f"case fortran drop-in driver recl={recl}", flush=True)
+        env = dict(os.environ, NEORX_UFM_RECL=str(recl))
+        r = subprocess.run([DRIVER, fin, fout], cwd=td, capture_output=True, text=True, timeout=240, env=env)
         assert r.returncode == 0, r.stderr
         out = np.fromfile(fout, dtype=np.float64)
-        gam1 = np.fromfile(os.path.join(td, "GAM_1.ufm"), dtype=np.float64)
-        gee = np.fromfile(os.path.join(td, "GAM_ee.ufm"), dtype=np.float64)
+        stride = recl // 8
+        raw1 = np.fromfile(os.path.join(td, "GAM_1.ufm"), dtype=np.float64).reshape(-1, stride)
+        rawe = np.fromfile(os.path.join(td, "GAM_ee.ufm"), dtype=np.float64).reshape(-1, stride)
+        assert rawe.shape[0] == ne ** 4 and not np.any(rawe[:, 1:]) and not np.any(raw1[:, 1:])
+        gam1, gee = raw1[:, 0].copy(), rawe[:, 0].copy()
     ng2, ng3, ng4 = ne ** 4 * npb ** 2, ne ** 6 * npb ** 2, ne ** 8 * npb ** 2
     sizes = [ng2, ng2, ne * ne, ne * ne, npb * npb, npb * npb, 1, 1, ng3, ne * ne, npb * npb, 1, ng4, ne * ne,
              npb * npb, 1,
