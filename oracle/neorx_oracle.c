@@ -2412,3 +2412,351 @@ void orc_rxchfmult_omg3ex(i64 which, i64 nebf, i64 npbf, const double *DAE, cons
     *E = e;
     free(XF);
 }
+
+/* ------------------------------------------------------------------------
+ * NEO-XCHF RHF SCF side (source/xchf1_independent/): the one-particle and
+ * e-p Coulomb integrals, the Fock build xchf1_fock with XCfock_correction,
+ * ROOTHAN and the SCF loop of xcscf.
+ * ------------------------------------------------------------------------ */
+static const double PI_AC = 3.141592653589793238462643383279502884; /* dacos(-1) of ke.f, vec.f, ovlap.f, vee.f */
+
+static double g1ac(int op, const prim_t *b, const prim_t *k, const double *C, double xmass)
+{
+    gint_t g;
+    gi_init(&g, 1);
+    g.bra[0] = *b;
+    g.ket[0] = *k;
+    g.op = op;
+    g.mass = xmass;
+    g.pi = PI_AC;
+    if (C) memcpy(g.C, C, sizeof(g.C));
+    return gint_eval(&g);
+}
+
+/* class_nuc_rep (class_nuc_rep.f:1-45) */
+double orc_enuc(i64 nat, const double *cat, const double *zan)
+{
+    double e = 0.0;
+    for (i64 i = 1; i < nat; ++i)
+        for (i64 j = 0; j < i; ++j) {
+            double r2 = 0.0;
+            for (int k = 0; k < 3; ++k) r2 += (cat[3 * i + k] - cat[3 * j + k]) * (cat[3 * i + k] - cat[3 * j + k]);
+            if (r2 != 0.0) e += zan[i] * zan[j] / sqrt(r2);
+        }
+    return e;
+}
+
+/* calc_GAM_epcore (gam_epcore.f:1-178) with xcalc_GAM_epcore (gfke + sum of
+ * coul_sign Z gfvec), and elec_ovlap / nuc_ovlap (pure_ovlap.f:2-204):
+ * He, Se in pack_2D(nebf) order [iec][jec], Hp, Sp in [ip][jp] */
+void orc_one_body(BAS_ARGS, SYS_ARGS, double *He, double *Se, double *Hp, double *Sp)
+{
+    BAS_INIT;
+    i64 ne = nebf, np = npbf;
+    memset(He, 0, sizeof(double) * ne * ne);
+    memset(Se, 0, sizeof(double) * ne * ne);
+    for (i64 ie = 0; ie < npebf; ++ie)
+        for (i64 je = 0; je < npebf; ++je) {
+            prim_t b = eprim(&B, ie), k = eprim(&B, je);
+            double ke = g1ac(OP_KIN, &b, &k, NULL, 1.0), v = 0.0;
+            for (i64 c = 0; c < nat; ++c) v += -1.0 * zan[c] * g1ac(OP_NUC, &b, &k, cat + 3 * c, 1.0);
+            i64 ia = (ampeb2c[ie] - 1) * ne + ampeb2c[je] - 1;
+            He[ia] += (ke + v) * agebfcc[ie] * agebfcc[je];
+            Se[ia] += g1ac(OP_OVL, &b, &k, NULL, 1.0) * agebfcc[ie] * agebfcc[je];
+        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp) {
+            prim_t b = pprim(&B, ip), k = pprim(&B, jp);
+            double ke = g1ac(OP_KIN, &b, &k, NULL, pmass), v = 0.0;
+            for (i64 c = 0; c < nat; ++c) v += 1.0 * zan[c] * g1ac(OP_NUC, &b, &k, cat + 3 * c, 1.0);
+            Hp[ip * np + jp] = (ke + v) * agnbfcc[ip] * agnbfcc[jp];
+            Sp[ip * np + jp] = g1ac(OP_OVL, &b, &k, NULL, 1.0) * agnbfcc[ip] * agnbfcc[jp];
+        }
+    (void)ngtg; (void)bcoef; (void)gamma; (void)kpestr; (void)kpeend;
+}
+
+/* calc_GAM_ep (gam_ep.f:1-141) with xcalc_GAM_ep = -gfvee:
+ * GAM_ep in pack_4D(nebf,nebf,npbf) order [ip][jp][iec][jec] */
+void orc_gam_ep(BAS_ARGS, double *Gep)
+{
+    BAS_INIT;
+    i64 ne = nebf, np = npbf;
+    memset(Gep, 0, sizeof(double) * ne * ne * np * np);
+#pragma omp parallel for collapse(2)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 ie = 0; ie < npebf; ++ie)
+                for (i64 je = 0; je < npebf; ++je) {
+                    prim_t b1 = eprim(&B, ie), k1 = eprim(&B, je), b2 = pprim(&B, ip), k2 = pprim(&B, jp);
+                    gint_t g;
+                    gi_init(&g, 2);
+                    g.bra[0] = b1;
+                    g.ket[0] = k1;
+                    g.bra[1] = b2;
+                    g.ket[1] = k2;
+                    g.op = OP_COUL;
+                    g.a = 0;
+                    g.b = 1;
+                    g.pi = PI_AC;
+                    double ans = -gint_eval(&g);
+                    i64 ia = ((ip * np + jp) * ne + ampeb2c[ie] - 1) * ne + ampeb2c[je] - 1;
+                    Gep[ia] += ans * agebfcc[ie] * agebfcc[je] * agnbfcc[ip] * agnbfcc[jp];
+                }
+    (void)kpestr; (void)kpeend;
+}
+
+/* xchf1_fock (xchf1_fock.f:2-360) for the in-core path of xcneo_HYBRID
+ * (LG2IC1, LG3IC1, LG4IC): E_from_GAM_ecore/pcore/ep/ee (:600-911),
+ * S_from_GAM_1s and E_from_GAM_1 (:990-1130, reading the arrays instead of
+ * GAM_1.ufm/GAM_1s.ufm), EG2IC1, EG3IC1, EG4ICR, the energy assembly and
+ * XCfock_correction (:728-784).  G4 may be NULL: the 5-particle values are
+ * then formed per entry from G2s and Gee (GAM4_DSCF).  comps as the C ABI's
+ * neorx_xchf_fock_dev returns them. */
+void orc_xchf_fock(i64 nebf, i64 npbf, i64 nelec, double enuc, const double *DE, const double *DP, const double *He,
+                   const double *Hp, const double *Gep, const double *Gee, const double *G1, const double *G1s,
+                   const double *G2, const double *G2s, const double *G3, const double *G4, double *FE, double *FP,
+                   double *comps)
+{
+    i64 ne = nebf, np = npbf;
+    double *se1 = calloc(ne * ne, 8), *se2 = calloc(ne * ne, 8), *sp1 = calloc(np * np, 8), *sp2 = calloc(np * np, 8);
+    double *xfe = calloc(ne * ne, 8), *xfp = calloc(np * np, 8);
+#define DEm(i, j) DE[(j) * ne + (i)]
+#define DPm(i, j) DP[(j) * np + (i)]
+    double e_ecore = 0, e_pcore = 0, e_ep = 0, e_ee = 0, e1 = 0, s1 = 0, e2 = 0, s2 = 0, e3 = 0, e4 = 0;
+    for (i64 i = 0; i < ne; ++i)
+        for (i64 j = 0; j < ne; ++j) {
+            FE[j * ne + i] = He[i * ne + j];
+            e_ecore += DEm(i, j) * He[i * ne + j];
+        }
+    for (i64 i = 0; i < np; ++i)
+        for (i64 j = 0; j < np; ++j) {
+            FP[j * np + i] = Hp[i * np + j];
+            e_pcore += DPm(i, j) * Hp[i * np + j];
+        }
+    /* E_from_GAM_ep */
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i = 0; i < ne; ++i)
+                for (i64 j = 0; j < ne; ++j) {
+                    double v = Gep[((ip * np + jp) * ne + i) * ne + j];
+                    xfe[j * ne + i] += DPm(ip, jp) * v;
+                    xfp[jp * np + ip] += DEm(i, j) * v;
+                    e_ep += DEm(i, j) * DPm(ip, jp) * v;
+                }
+    for (i64 t = 0; t < ne * ne; ++t) FE[t] += xfe[t];
+    for (i64 t = 0; t < np * np; ++t) FP[t] += xfp[t];
+    /* E_from_GAM_ee */
+    if (nelec > 1) {
+        memset(xfe, 0, 8 * ne * ne);
+        for (i64 i2 = 0; i2 < ne; ++i2)
+            for (i64 j2 = 0; j2 < ne; ++j2)
+                for (i64 i1 = 0; i1 < ne; ++i1)
+                    for (i64 j1 = 0; j1 < ne; ++j1) {
+                        double v1 = Gee[((i1 * ne + j1) * ne + i2) * ne + j2];
+                        double v2 = Gee[((i1 * ne + j2) * ne + i2) * ne + j1];
+                        double val = (v1 - 0.5 * v2) * 0.5;
+                        xfe[j1 * ne + i1] += 2.0 * DEm(i2, j2) * val;
+                        e_ee += DEm(i1, j1) * DEm(i2, j2) * val;
+                    }
+        for (i64 t = 0; t < ne * ne; ++t) FE[t] += xfe[t];
+    }
+    /* S_from_GAM_1s, E_from_GAM_1 */
+    memset(xfe, 0, 8 * ne * ne);
+    memset(xfp, 0, 8 * np * np);
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i = 0; i < ne; ++i)
+                for (i64 j = 0; j < ne; ++j) {
+                    i64 ia = ((ip * np + jp) * ne + i) * ne + j;
+                    se1[j * ne + i] += DPm(ip, jp) * G1s[ia];
+                    sp1[jp * np + ip] += DEm(i, j) * G1s[ia];
+                    s1 += DEm(i, j) * DPm(ip, jp) * G1s[ia];
+                    xfe[j * ne + i] += DPm(ip, jp) * G1[ia];
+                    xfp[jp * np + ip] += DEm(i, j) * G1[ia];
+                    e1 += DEm(i, j) * DPm(ip, jp) * G1[ia];
+                }
+    for (i64 t = 0; t < ne * ne; ++t) FE[t] += xfe[t];
+    for (i64 t = 0; t < np * np; ++t) FP[t] += xfp[t];
+#undef DEm
+#undef DPm
+    if (nelec > 1) {
+        eg2_impl(ne, np, DE, DP, G2, G2s, FE, se2, FP, sp2, &e2, &s2, 1);
+        if (nelec > 2) {
+            orc_eg3ic1(ne, np, DE, DP, G3, FE, FP, &e3);
+            if (G4)
+                orc_eg4icr(ne, np, DE, DP, G4, FE, FP, &e4);
+            else
+                orc_gam4_dscf(ne, np, G2s, Gee, DE, DP, FE, FP, &e4);
+        }
+    }
+    double S = 1.0 + s1 + s2;
+    double psi = (e_ecore + e_pcore + e_ep + e_ee + e1 + e2 + e3 + e4) / S;
+    double H = psi / S;
+    /* XCfock_correction */
+    for (i64 t = 0; t < ne * ne; ++t) FE[t] = (1.0 / S) * FE[t] - H * (se1[t] + 2.0 * se2[t]);
+    for (i64 t = 0; t < np * np; ++t) FP[t] = (1.0 / S) * FP[t] - H * (sp1[t] + sp2[t]);
+    double c[14] = {psi + enuc, enuc, e_ecore, e_pcore, e_ep, e_ee, e1, e2, e3, e4, S, s1, s2, H};
+    memcpy(comps, c, sizeof(c));
+    free(se1); free(se2); free(sp1); free(sp2); free(xfe); free(xfp);
+}
+
+/* cyclic Jacobi eigen-decomposition of the symmetric n x n column-major a
+ * (destroyed: eigenvalues on the diagonal), eigenvectors in the columns of v */
+static void jacobi_cpu(i64 n, double *a, double *v)
+{
+    for (i64 t = 0; t < n * n; ++t) v[t] = (t % n == t / n) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (i64 j = 0; j < n; ++j)
+            for (i64 i = 0; i < n; ++i) {
+                if (i == j)
+                    dia += a[j * n + i] * a[j * n + i];
+                else
+                    off += a[j * n + i] * a[j * n + i];
+            }
+        if (off <= 1e-30 * dia || off == 0.0) break;
+        for (i64 p = 0; p < n - 1; ++p)
+            for (i64 q = p + 1; q < n; ++q) {
+                double apq = a[q * n + p];
+                if (apq == 0.0) continue;
+                double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (i64 k = 0; k < n; ++k) { /* rows p, q */
+                    double x = a[k * n + p], y = a[k * n + q];
+                    a[k * n + p] = c * x - s * y;
+                    a[k * n + q] = s * x + c * y;
+                }
+                for (i64 k = 0; k < n; ++k) { /* columns p, q */
+                    double x = a[p * n + k], y = a[q * n + k];
+                    a[p * n + k] = c * x - s * y;
+                    a[q * n + k] = s * x + c * y;
+                    x = v[p * n + k];
+                    y = v[q * n + k];
+                    v[p * n + k] = c * x - s * y;
+                    v[q * n + k] = s * x + c * y;
+                }
+            }
+    }
+}
+
+/* ROOTHAN (xcscf.f:766-952): X = S^-1/2 (symmetric orthogonalisation),
+ * F' = X^T F X, its eigenvectors C' with eigenvalues ascending, C = X C',
+ * D = prefac sum_{k = kstart..klast} C(:,k) C(:,k)^T (1-based k). */
+void orc_roothan(i64 n, const double *S, const double *F, i64 kstart, i64 klast, double prefac, double *C,
+                 double *ev, double *D)
+{
+    double *a = malloc(8 * n * n), *v = malloc(8 * n * n), *X = malloc(8 * n * n), *t = malloc(8 * n * n);
+    i64 *ord = malloc(sizeof(i64) * n);
+    memcpy(a, S, 8 * n * n);
+    jacobi_cpu(n, a, v);
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) {
+            double x = 0.0;
+            for (i64 k = 0; k < n; ++k) x += v[k * n + i] * v[k * n + j] / sqrt(a[k * n + k]);
+            X[j * n + i] = x;
+        }
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) {
+            double x = 0.0;
+            for (i64 k = 0; k < n; ++k) x += F[k * n + i] * X[j * n + k];
+            t[j * n + i] = x;
+        }
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) {
+            double x = 0.0;
+            for (i64 k = 0; k < n; ++k) x += X[i * n + k] * t[j * n + k];
+            a[j * n + i] = x;
+        }
+    jacobi_cpu(n, a, v);
+    for (i64 i = 0; i < n; ++i) ord[i] = i;
+    for (i64 i = 1; i < n; ++i) /* insertion sort by eigenvalue */
+        for (i64 j = i; j > 0 && a[ord[j] * n + ord[j]] < a[ord[j - 1] * n + ord[j - 1]]; --j) {
+            i64 x = ord[j];
+            ord[j] = ord[j - 1];
+            ord[j - 1] = x;
+        }
+    for (i64 k = 0; k < n; ++k) {
+        ev[k] = a[ord[k] * n + ord[k]];
+        for (i64 i = 0; i < n; ++i) {
+            double x = 0.0;
+            for (i64 l = 0; l < n; ++l) x += X[l * n + i] * v[ord[k] * n + l];
+            C[k * n + i] = x;
+        }
+    }
+    for (i64 i = 0; i < n; ++i)
+        for (i64 j = 0; j < n; ++j) {
+            double x = 0.0;
+            for (i64 k = kstart - 1; k < klast; ++k) x += C[k * n + i] * C[k * n + j];
+            D[j * n + i] = prefac * x;
+        }
+    free(a); free(v); free(X); free(t); free(ord);
+}
+
+/* The SCF loop of xcscf (xcscf.f:2-560) with ROOTHAN for both particles
+ * (no SOSCF step): per iteration the nuclear Fock build with (DE, DP),
+ * ROOTHAN for DP, the electronic build with (DE, new DP), ROOTHAN for DE,
+ * then DENDIF on both; converged when max|dDE| < tole (and max|dDP| < tolp
+ * if npbf > 1).  ce_guess (column-major orbitals, NULL: the core guess of
+ * guess_elec) gives DE; guess_prot gives DP.  trace: per iteration
+ * [E_total, dE, max|dDE|, max|dDP|].  Returns the number of iterations
+ * (negative if not converged). */
+i64 orc_xcscf(i64 nebf, i64 npbf, i64 nelec, i64 nucst, double enuc, const double *Se, const double *Sp,
+              const double *He, const double *Hp, const double *Gep, const double *Gee, const double *G1,
+              const double *G1s, const double *G2, const double *G2s, const double *G3, const double *G4,
+              const double *ce_guess, i64 maxit, double tole, double tolp, double *comps_init, double *comps,
+              double *trace)
+{
+    i64 ne = nebf, np = npbf, nocc = nelec > 1 ? nelec / 2 : 1;
+    double pre = nelec > 1 ? 2.0 : 1.0;
+    double *DE = calloc(ne * ne, 8), *DP = calloc(np * np, 8), *DE0 = calloc(ne * ne, 8), *DP0 = calloc(np * np, 8);
+    double *FE = calloc(ne * ne, 8), *FP = calloc(np * np, 8), *FEd = calloc(ne * ne, 8), *FPd = calloc(np * np, 8);
+    double *CE = calloc(ne * ne, 8), *CP = calloc(np * np, 8), *evE = calloc(ne, 8), *evP = calloc(np, 8);
+    double *h = calloc(ne * ne + np * np, 8);
+    if (ce_guess) {
+        for (i64 i = 0; i < ne; ++i)
+            for (i64 j = 0; j < ne; ++j) {
+                double x = 0.0;
+                for (i64 k = 0; k < nocc; ++k) x += pre * ce_guess[k * ne + i] * ce_guess[k * ne + j];
+                DE[j * ne + i] = x;
+            }
+    } else {
+        for (i64 i = 0; i < ne; ++i)
+            for (i64 j = 0; j < ne; ++j) h[j * ne + i] = He[i * ne + j];
+        orc_roothan(ne, Se, h, 1, nocc, pre, CE, evE, DE);
+    }
+    for (i64 i = 0; i < np; ++i)
+        for (i64 j = 0; j < np; ++j) h[j * np + i] = Hp[i * np + j];
+    orc_roothan(np, Sp, h, nucst, nucst, 1.0, CP, evP, DP);
+    double eold = 0.0;
+    i64 it, done = 0;
+    for (it = 1; it <= maxit && !done; ++it) {
+        double c[14];
+        orc_xchf_fock(ne, np, nelec, enuc, DE, DP, He, Hp, Gep, Gee, G1, G1s, G2, G2s, G3, G4, FEd, FP, c);
+        if (it == 1 && comps_init) memcpy(comps_init, c, sizeof(c));
+        orc_roothan(np, Sp, FP, nucst, nucst, 1.0, CP, evP, DP);
+        orc_xchf_fock(ne, np, nelec, enuc, DE, DP, He, Hp, Gep, Gee, G1, G1s, G2, G2s, G3, G4, FE, FPd, c);
+        orc_roothan(ne, Se, FE, 1, nocc, pre, CE, evE, DE);
+        double de = 0.0, dp = 0.0;
+        for (i64 t = 0; t < ne * ne; ++t) {
+            de = fmax(de, fabs(DE0[t] - DE[t]));
+            DE0[t] = DE[t];
+        }
+        for (i64 t = 0; t < np * np; ++t) {
+            dp = fmax(dp, fabs(DP0[t] - DP[t]));
+            DP0[t] = DP[t];
+        }
+        if (trace) {
+            trace[4 * (it - 1)] = c[0];
+            trace[4 * (it - 1) + 1] = c[0] - eold;
+            trace[4 * (it - 1) + 2] = de;
+            trace[4 * (it - 1) + 3] = dp;
+        }
+        eold = c[0];
+        if (comps) memcpy(comps, c, sizeof(c));
+        done = np > 1 ? (de < tole && dp < tolp) : (de < tole);
+    }
+    free(DE); free(DP); free(DE0); free(DP0); free(FE); free(FP); free(FEd); free(FPd);
+    free(CE); free(CP); free(evE); free(evP); free(h);
+    return done ? it - 1 : -(it - 1);
+}

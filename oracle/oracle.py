@@ -812,3 +812,262 @@ def ref_rxchfmult_omgex(order, which, ne, npb, DAE, DBE, DP, G, GX1, GX2, F):
         gs.append(_p(np.ascontiguousarray(GX2)))
     fn(_i8(1), _i8(ne), _i8(npb), _i8(np.asarray(G).size), _f(DAE), _f(DBE), _f(DP), *gs, _p(F), C.byref(e))
     return e.value
+
+
+# ----------------------------------------------------------------------
+# NEO-XCHF RHF SCF side (restatement in neorx_oracle.c)
+# ----------------------------------------------------------------------
+def enuc(s):
+    """class_nuc_rep (source/xchf1_independent/class_nuc_rep.f:1)."""
+    f = lib().orc_enuc
+    f.restype = C.c_double
+    return f(C.c_int64(s["nat"]), _p(np.ascontiguousarray(s["cat"])), _p(np.ascontiguousarray(s["zan"])))
+
+
+def one_body(s):
+    """calc_GAM_epcore + elec_ovlap / nuc_ovlap: (GAM_ecore, S_e, GAM_pcore,
+    S_p) in pack_2D order."""
+    ne, npb = s["nebf"], s["npbf"]
+    He, Se, Hp, Sp = np.zeros(ne * ne), np.zeros(ne * ne), np.zeros(npb * npb), np.zeros(npb * npb)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_one_body(*ba, *sa, _p(He), _p(Se), _p(Hp), _p(Sp))
+    return He, Se, Hp, Sp
+
+
+def gam_ep(s):
+    """calc_GAM_ep (source/xchf1_independent/gam_ep.f:1): pack_4D order."""
+    ne, npb = s["nebf"], s["npbf"]
+    g = np.zeros(ne * ne * npb * npb)
+    ba, k1 = _bas(s)
+    lib().orc_gam_ep(*ba, _p(g))
+    return g
+
+
+def xchf_ints(s, gam4=True):
+    """Every integral of the XCHF energy for system s (oracle restatement):
+    dict with enuc, He, Se, Hp, Sp, Gep, Gee, G1, G1s, G2, G2s, G3, G4
+    (G4: the in-core 5-particle tensor, or None)."""
+    He, Se, Hp, Sp = one_body(s)
+    ints = dict(enuc=enuc(s), He=He, Se=Se, Hp=Hp, Sp=Sp, Gep=gam_ep(s), Gee=gam_ee(s))
+    ints["G1"], ints["G1s"] = gam1(s)
+    ints["G2"], ints["G2s"] = gam2_ic1(s)
+    ints["G3"] = gam3_ic1(s) if s.get("nelec", 3) > 2 else np.zeros(1)
+    ints["G4"] = gam4_icr(s["nebf"], s["npbf"], ints["G2s"], ints["Gee"]) if gam4 else None
+    return ints
+
+
+def _ints_args(ints):
+    keep = [np.ascontiguousarray(ints[k], dtype=np.float64) for k in ("He", "Hp", "Gep", "Gee", "G1", "G1s", "G2",
+                                                                      "G2s", "G3")]
+    g4 = None if ints.get("G4") is None else np.ascontiguousarray(ints["G4"])
+    return [_p(a) for a in keep] + [None if g4 is None else _p(g4)], keep + [g4]
+
+
+def xchf_fock(nebf, npbf, nelec, ints, DE, DP):
+    """xchf1_fock (source/xchf1_independent/xchf1_fock.f:2): (FE, FP, comps)."""
+    FE, FP, comps = np.zeros((nebf, nebf), order="F"), np.zeros((npbf, npbf), order="F"), np.zeros(14)
+    a, keep = _ints_args(ints)
+    de, dp = np.asfortranarray(DE), np.asfortranarray(DP)
+    lib().orc_xchf_fock(C.c_int64(nebf), C.c_int64(npbf), C.c_int64(nelec), C.c_double(ints["enuc"]), _p(de), _p(dp),
+                        *a, _p(FE), _p(FP), _p(comps))
+    return FE, FP, comps
+
+
+def roothan(S, F, kstart, klast, prefac):
+    """ROOTHAN (source/xchf1_independent/xcscf.f:766): (C, eigenvalues, D)."""
+    n = S.shape[0]
+    Cm, ev, D = np.zeros((n, n), order="F"), np.zeros(n), np.zeros((n, n), order="F")
+    s_, f_ = np.asfortranarray(S), np.asfortranarray(F)
+    lib().orc_roothan(C.c_int64(n), _p(s_), _p(f_), C.c_int64(kstart), C.c_int64(klast), C.c_double(prefac), _p(Cm),
+                      _p(ev), _p(D))
+    return Cm, ev, D
+
+
+def xcscf(s, ints, ce_guess=None, maxit=100, tole=1e-6, tolp=1e-4):
+    """The SCF loop of xcscf (source/xchf1_independent/xcscf.f:2) with ROOTHAN
+    steps only: dict(energy, niter, comps, comps_init, trace)."""
+    ne, npb = s["nebf"], s["npbf"]
+    a, keep = _ints_args(ints)
+    ci, cf, tr = np.zeros(14), np.zeros(14), np.zeros((maxit, 4))
+    ce = None if ce_guess is None else np.asfortranarray(ce_guess, dtype=np.float64)
+    se, sp = (np.ascontiguousarray(ints[k]) for k in ("Se", "Sp"))
+    # Se/Sp are symmetric: pack_2D order equals column-major
+    f = lib().orc_xcscf
+    f.restype = C.c_int64
+    n = f(C.c_int64(ne), C.c_int64(npb), C.c_int64(s["nelec"]), C.c_int64(s.get("nucst", 1)),
+          C.c_double(ints["enuc"]), _p(se), _p(sp), *a, None if ce is None else _p(ce), C.c_int64(maxit),
+          C.c_double(tole), C.c_double(tolp), _p(ci), _p(cf), _p(tr))
+    return dict(energy=cf[0], niter=int(n), comps=cf, comps_init=ci, trace=tr[:abs(int(n))])
+
+
+# ----------------------------------------------------------------------
+# The reference's own NEO-XCHF SCF (xcscf, source/xchf1_independent/xcscf.f:2)
+# run from oracle/_ref/libref.so in a child process: the integral routines
+# of xcneo_HYBRID.f:663-1002 write their .ufm files into a work directory,
+# the in-core GM2/GM3/GM4 are passed as arrays, and xcscf's printed log is
+# parsed.  Used to make golden vectors (tests/golden/make_xchf_scf.py).
+# ----------------------------------------------------------------------
+_XCSCF_CHILD = r"""
+import os, sys, json
+import numpy as np
+sys.path.insert(0, {odir!r})
+import oracle
+s = {{k: v for k, v in np.load("system.npz").items()}}
+for k in ("nebf", "npebf", "npbf", "nat", "ngtg", "nelec", "nucst"):
+    s[k] = int(s[k])
+s["pmass"] = float(s["pmass"])
+fd = os.open("xcscf.log", os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
+os.dup2(fd, 1)
+oracle._ref_xcscf_here(s, {read_ce}, {lsoscf})
+sys.stdout.flush()
+"""
+
+
+def _ref_xcscf_here(s, read_ce, lsoscf):
+    R = ref()
+    ne, npe, npb, nat, ngtg, nelec = (s[k] for k in ("nebf", "npebf", "npbf", "nat", "ngtg", "nelec"))
+    ne2, np2 = ne * ne, npb * npb
+    ng1, ng2, ng3, ng4 = ne2 * np2, ne2 * ne2 * np2, ne2 ** 3 * np2, ne2 ** 4 * np2
+    cat, zan, b, g = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    kpestr, kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc = _ref_basis_args(s)
+    wd = os.getcwd()
+    R.class_nuc_rep_(_i8(nat), _p(zan), _p(cat))
+    R.elec_ovlap_(_i8(npe), _i8(ne), _i8(ne2), _p(ampeb2c), _p(agebfcc), _p(elcex), _p(elcam), _p(elcbfc))
+    R.nuc_ovlap_(_i8(npb), _i8(np2), _p(agnbfcc), _p(nucex), _p(nucam), _p(nucbfc))
+    R.calc_gam_epcore_(_i8(ne), _i8(npe), _i8(npb), _i8(ne2), _i8(np2), _i8(nat), _d(s["pmass"]), _p(zan), _p(cat),
+                       _p(ampeb2c), _p(agebfcc), _p(agnbfcc), _p(elcex), _p(nucex), _p(elcam), _p(nucam), _p(elcbfc),
+                       _p(nucbfc))
+    R.calc_gam_ep_(_i8(ne), _i8(npe), _i8(npb), _i8(ng1), _p(ampeb2c), _p(agebfcc), _p(agnbfcc), _p(elcex),
+                   _p(nucex), _p(elcam), _p(nucam), _p(elcbfc), _p(nucbfc))
+    ref_gam_ee(s, wd)
+    ref_gam1(s, wd)
+    g2, g2s = ref_gam2_ic1(s)
+    g3 = ref_gam3_ic1(s) if nelec > 2 else np.zeros(1)
+    g4 = ref_gam4_icr(ne, npb, g2s, np.fromfile("GAM_ee.ufm", dtype=np.float64), wd) if nelec > 2 else np.zeros(1)
+    nocc = nelec // 2
+    npr = (ne - nocc) * nocc
+    T, F = _i8(1), _i8(0)
+    R.xcscf_(_i8(nelec), _i8(npr), _i8(ne * (ne + 1) // 2), _i8(s.get("nucst", 1)), _i8(npe), _i8(ne), _i8(ne2),
+             _i8(npb), _i8(np2), _i8(ne2 * ne2), _i8(ngtg), _i8(ng1), _i8(ng2), _i8(ng3), _i8(ng4), _i8(1), _i8(1),
+             _i8(1), T if read_ce else F, F, F, T, F, F, F, T if lsoscf else F, F,
+             _i8(npe ** 4 * np2), _i8(npe ** 6 * np2), _i8(nat), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g),
+             _p(kpestr), _p(kpeend), _p(ampeb2c), _p(agebfcc), _p(agnbfcc), _p(elcex), _p(nucex), _p(elcam),
+             _p(nucam), _p(elcbfc), _p(nucbfc), T, _i8(ng2), _p(g2), _p(g2s), T, _i8(g3.size), _p(g3), T,
+             _i8(g4.size), _p(g4))
+
+
+def ref_one_body(s, workdir):
+    """The reference's class_nuc_rep, elec_ovlap / nuc_ovlap, calc_GAM_epcore
+    and calc_GAM_ep, which write ENUCRP.dat, eovlap.ufm, novlap.ufm,
+    GAM_ecr.ufm, GAM_pcr.ufm and GAM_ep.ufm into workdir: returns (E_nuc,
+    GAM_ecore, S_e, GAM_pcore, S_p, GAM_ep) in the files' (pack) order."""
+    R = ref()
+    ne, npe, npb, nat = s["nebf"], s["npebf"], s["npbf"], s["nat"]
+    ne2, np2 = ne * ne, npb * npb
+    cat, zan = np.ascontiguousarray(s["cat"]), np.ascontiguousarray(s["zan"])
+    kpestr, kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc = _ref_basis_args(s)
+
+    def run():
+        R.class_nuc_rep_(_i8(nat), _p(zan), _p(cat))
+        R.elec_ovlap_(_i8(npe), _i8(ne), _i8(ne2), _p(ampeb2c), _p(agebfcc), _p(elcex), _p(elcam), _p(elcbfc))
+        R.nuc_ovlap_(_i8(npb), _i8(np2), _p(agnbfcc), _p(nucex), _p(nucam), _p(nucbfc))
+        R.calc_gam_epcore_(_i8(ne), _i8(npe), _i8(npb), _i8(ne2), _i8(np2), _i8(nat), _d(s["pmass"]), _p(zan),
+                           _p(cat), _p(ampeb2c), _p(agebfcc), _p(agnbfcc), _p(elcex), _p(nucex), _p(elcam),
+                           _p(nucam), _p(elcbfc), _p(nucbfc))
+        R.calc_gam_ep_(_i8(ne), _i8(npe), _i8(npb), _i8(ne2 * np2), _p(ampeb2c), _p(agebfcc), _p(agnbfcc),
+                       _p(elcex), _p(nucex), _p(elcam), _p(nucam), _p(elcbfc), _p(nucbfc))
+
+    _in_dir(workdir, run)
+    rd = lambda f: np.fromfile(os.path.join(workdir, f), dtype=np.float64)  # noqa: E731
+    e = float(open(os.path.join(workdir, "ENUCRP.dat")).read().split()[0].replace("D", "E"))
+    return e, rd("GAM_ecr.ufm"), rd("eovlap.ufm"), rd("GAM_pcr.ufm"), rd("novlap.ufm"), rd("GAM_ep.ufm")
+
+
+def ref_roothan(S, F, n_particle, i_particle, nucst=1):
+    """ROOTHAN (source/xchf1_independent/xcscf.f:766) of the reference:
+    (D, C, eigenvalues)."""
+    n = S.shape[0]
+    D, Cm, ev = np.zeros((n, n), order="F"), np.zeros((n, n), order="F"), np.zeros(n)
+    s_, f_ = np.asfortranarray(S, dtype=np.float64), np.asfortranarray(F, dtype=np.float64)
+    ref().roothan_(_p(D), _p(Cm), _p(ev), _p(s_), _p(f_), _i8(n), _i8(n_particle), _i8(i_particle), _i8(nucst))
+    return D, Cm, ev
+
+
+def ref_xchf_fock(s, ints, DE, DP, workdir):
+    """xchf1_fock (source/xchf1_independent/xchf1_fock.f:2) of the reference on
+    the in-core path (LG2IC1, LG3IC1, LG4IC) with the given integrals; GAM_1 and
+    GAM_1s go to workdir as the .ufm files E_from_GAM_1 / S_from_GAM_1s read.
+    Returns (FE, FP, comps) in the order of the C ABI's comps array."""
+    R = ref()
+    ne, npe, npb, nat, ngtg, nelec = (s[k] for k in ("nebf", "npebf", "npbf", "nat", "ngtg", "nelec"))
+    ne2, np2 = ne * ne, npb * npb
+    np.ascontiguousarray(ints["G1"]).tofile(os.path.join(workdir, "GAM_1.ufm"))
+    np.ascontiguousarray(ints["G1s"]).tofile(os.path.join(workdir, "GAM_1s.ufm"))
+    cat, zan, b, g = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    basis = _ref_basis_args(s)
+    arr = {k: np.ascontiguousarray(ints[k], dtype=np.float64) for k in ("He", "Hp", "Gep", "Gee", "G2", "G2s", "G3",
+                                                                        "G4")}
+    FE, FP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+    XFE, XFP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
+    de, dp = np.asfortranarray(DE, dtype=np.float64), np.asfortranarray(DP, dtype=np.float64)
+    out = [C.c_double() for _ in range(13)]
+    out[1].value = ints["enuc"]
+    T, F = _i8(1), _i8(0)
+    ng1, ng2, ng3, ng4 = ne2 * np2, ne2 * ne2 * np2, arr["G3"].size, arr["G4"].size
+
+    def run():
+        R.xchf1_fock_(F, T, F, T, F, T, F, T, F, _i8(1), _i8(1), _i8(1), _i8(ng4), _i8(ng3), _i8(ng2), _i8(npe),
+                      _i8(ne), _i8(ne2), _i8(npb), _i8(np2), _i8(nelec), _i8(ne2 * ne2), _i8(ng1), _i8(ng2),
+                      _i8(ng3), _i8(ng4), _p(de), _p(dp), _p(arr["He"]), _p(arr["Hp"]), _p(arr["Gep"]),
+                      _p(arr["Gee"]), _p(arr["G4"]), _p(arr["G3"]), _p(arr["G2"]), _p(arr["G2s"]),
+                      _i8(npe ** 4 * np2), _i8(ngtg), _i8(npe ** 6 * np2), _i8(nat), _d(s["pmass"]), _p(cat),
+                      _p(zan), _p(b), _p(g), *[_p(a) for a in basis], _p(FE), _p(FP), _p(XFE), _p(XFP),
+                      *[C.byref(o) for o in out])
+
+    _in_dir(workdir, run)
+    v = [o.value for o in out]  # E_total E_nuc E_ecore E_pcore E_ep E_ee E_gam1..4 S_total S_gam1 S_gam2
+    psi = v[0] - v[1]
+    comps = np.array(v[:10] + [v[10], v[11], v[12], psi / v[10]])
+    return FE, FP, comps
+
+
+_COMP_KEYS = ("E_NUC", "E_ECORE", "E_PCORE", "E_EP", "E_EE", "E_GAM1", "E_GAM2", "E_GAM3", "E_GAM4", "S_TOTAL",
+              "E_TOTAL")
+
+
+def parse_xcscf_log(text):
+    """Initial-guess and final energy components, the iteration table and the
+    final energy of an xcscf log (the formats 9000-9300 of xcscf.f)."""
+    import re
+    blocks = [dict(re.findall(r"^\s*(E_[A-Z0-9]+|S_TOTAL)=\s*(\S+)", b, re.M))
+              for b in text.split("NEOXCHF ENERGETIC COMPONENTS")[1:]]
+    comps = [{k: float(v) for k, v in b.items()} for b in blocks]
+    its = [tuple(float(x) for x in m) for m in
+           re.findall(r"^\s+(\d+)\s+(-?\d+\.\d+)\s+(-?\d+\.\d+)\s+(\d+\.\d+)\s+(\d+\.\d+)\s+(\d+\.\d+)\s*$", text, re.M)]
+    m = re.search(r"FINAL NEOXCHF ENERGY IS\s+(\S+)\s+AFTER\s+(\d+)", text)
+    return dict(initial=comps[0] if comps else None, final=comps[1] if len(comps) > 1 else None,
+                iterations=its, energy=float(m.group(1)) if m else None, niter=int(m.group(2)) if m else None)
+
+
+def ref_xcscf(s, workdir, ce_guess=None, lsoscf=True, timeout=3600):
+    """Run the reference's integrals + xcscf for system dict s (with nelec,
+    nucst) in workdir (a child process; OMP threads as the environment sets)
+    and return parse_xcscf_log of its output.  ce_guess: orbitals written as
+    guessCE.inp (read_CE = .true.), else the core guess."""
+    import subprocess
+    import sys
+    os.makedirs(workdir, exist_ok=True)
+    np.savez(os.path.join(workdir, "system.npz"), **{k: np.asarray(v) for k, v in s.items() if k != "name"})
+    if ce_guess is not None:
+        n = ce_guess.shape[0]
+        lines = []
+        for j in range(n):
+            for ic, i0 in enumerate(range(0, n, 5)):
+                vals = "".join(f"{v:15.8E}" for v in ce_guess[i0:i0 + 5, j])
+                lines.append(f"{(j + 1) % 100:2d}{(ic + 1) % 1000:3d}{vals}")
+        open(os.path.join(workdir, "guessCE.inp"), "w").write("\n".join(lines) + "\n")
+    code = _XCSCF_CHILD.format(odir=os.path.dirname(os.path.abspath(__file__)), read_ce=ce_guess is not None,
+                               lsoscf=lsoscf)
+    subprocess.run([sys.executable, "-c", code], cwd=workdir, check=True, timeout=timeout)
+    return parse_xcscf_log(open(os.path.join(workdir, "xcscf.log")).read())

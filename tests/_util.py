@@ -90,3 +90,32 @@ def engine_prim_list3(s, cls, idx, tier=0):
                     C.c_int64(s["ngtg"]), C.c_double(s["pmass"]), *[P(a) for a in sk], C.c_int64(cls), C.c_int64(n),
                     P(idx), P(e), P(sv), P(xv), C.c_int64(tier))
     return e, sv, xv
+
+
+def golden_scf():
+    """tests/golden/xchf_scf.json: reference xcscf runs (make_xchf_scf.py)."""
+    import json
+    return json.load(open(os.path.join(ROOT, "tests", "golden", "xchf_scf.json")))
+
+
+def golden_system(entry):
+    """System dict of a golden entry: the stored arrays, or the reference
+    input file it names (with its guess orbitals) -> (sysd, guess or None)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "neo-rxchf_amd"))
+    from neorx import xchf
+    if "input" in entry:
+        gd = os.path.join(ROOT, "tests", "golden")
+        s = xchf.read_input(os.path.join(gd, entry["input"]))
+        guess = xchf.read_mos(os.path.join(gd, entry["guess"]), s["nebf"]) if entry.get("guess") else None
+        return s, guess
+    s = {}
+    for k, v in entry["system"].items():
+        if isinstance(v, dict) and "data" in v:
+            a = np.array(v["data"], dtype=v["dtype"])
+            s[k] = a.reshape(v["shape"], order=v["order"]) if v["shape"] else a
+            if v["order"] == "F" and len(v["shape"]) > 1:
+                s[k] = np.asfortranarray(s[k])
+        else:
+            s[k] = v
+    return s, None

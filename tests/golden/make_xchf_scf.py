@@ -1,0 +1,114 @@
+"""Golden vectors for the NEO-XCHF RHF SCF: the reference's own integral
+routines and SCF (xcscf, source/xchf1_independent/xcscf.f:2, built from the
+reference's sources into oracle/_ref/libref.so by ``make -C oracle ref``) run
+in this container on
+
+* the reference's example ``examples/xchf`` (input and guess orbitals copied
+  to tests/golden/xchf_example/ as data); its answer log prints
+  E = -5.7800665094 after 8 iterations (examples/xchf/answer/XCHF.log:335),
+  which this run must reproduce;
+* small synthetic systems in the geometry of the C1 benchmark system
+  (classical nuclei at z = 0 and 2.1 bohr, the quantum proton at 4.3 bohr)
+  with s/p/d electrons and s/p nuclear functions, started from the core
+  guess.
+
+Writes tests/golden/xchf_scf.json: per system the input arrays, the
+initial-guess energy components, the final components and the iteration
+table, as xcscf prints them (10 decimals).
+
+    OMP_NUM_THREADS=8 python tests/golden/make_xchf_scf.py [name ...]
+"""
+import json
+import math
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "neo-rxchf_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle  # noqa: E402
+from neorx import synth, xchf  # noqa: E402
+
+SHELL = synth._SHELL
+
+
+def mini(name, centres, nuc_shells, zan, nelec, seed):
+    """centres: list of (z, [(l, nprim), ...]) on the bond axis (bohr);
+    nuc_shells: list of l on the proton at z = 4.3 bohr."""
+    rng = np.random.default_rng(seed)
+    prims, contr = [], []
+    for z, shells in centres:
+        c = np.array([0.0, 0.0, z])
+        for l, n in shells:
+            ex = np.sort(np.exp(rng.uniform(math.log(0.1), math.log(50.0), n)))[::-1]
+            cf = rng.uniform(0.1, 1.0, n)
+            for l3 in SHELL[l]:
+                for a, f in zip(ex, cf):
+                    prims.append((l3, float(a), c, float(f)))
+                contr.append(n)
+    hpos = np.array([0.0, 0.0, 4.3])
+    nuc = []
+    for l in nuc_shells:
+        a = float(np.exp(rng.uniform(math.log(4.0), math.log(32.0))))
+        for l3 in SHELL[l]:
+            nuc.append((l3, a, hpos))
+    b = rng.uniform(0.1, 0.5, 3)
+    g = np.exp(rng.uniform(0.0, math.log(30.0), 3))
+    s = synth._finish(prims, contr, nuc, [np.array([0.0, 0.0, z]) for z, _ in centres], zan, b, g, pmass=1836.0)
+    s["nelec"], s["nucst"], s["name"] = nelec, 1, name
+    return s
+
+
+SYSTEMS = {
+    "c1mini_sp": lambda: mini("c1mini_sp", [(0.0, [(0, 2), (0, 1), (1, 1)]), (2.1, [(0, 1), (1, 1)])], [0],
+                              [2.0, 1.0], 4, 11),
+    "c1mini_d": lambda: mini("c1mini_d", [(0.0, [(0, 2), (2, 1)])], [0], [2.0, 1.0], 4, 13),
+    # small enough for the oracle's own integrals in the CPU test suite
+    "tiny_s": lambda: mini("tiny_s", [(0.0, [(0, 2), (0, 1)]), (2.1, [(0, 1)])], [0], [2.0, 1.0], 4, 14),
+    "tiny_pnuc": lambda: mini("tiny_pnuc", [(0.0, [(0, 2), (0, 1)]), (2.1, [(0, 1)])], [0, 1], [2.0, 1.0], 4, 15),
+}
+
+
+def jsonable(s):
+    out = {}
+    for k, v in s.items():
+        if isinstance(v, np.ndarray):
+            out[k] = {"shape": list(v.shape), "order": "F" if v.flags.f_contiguous and v.ndim > 1 else "C",
+                      "dtype": str(v.dtype), "data": v.ravel(order="F" if v.flags.f_contiguous and v.ndim > 1
+                                                            else "C").tolist()}
+        else:
+            out[k] = v
+    return out
+
+
+def main(names):
+    """names: entries to (re)compute, all when empty; others are kept."""
+    path = os.path.join(HERE, "xchf_scf.json")
+    res = json.load(open(path)) if os.path.exists(path) else {}
+    if not names or "example" in names:
+        ex = os.path.join(HERE, "xchf_example")
+        s = xchf.read_input(os.path.join(ex, "exam_xchf.inp"))
+        C = xchf.read_mos(os.path.join(ex, "guessCE.inp"), s["nebf"])
+        with tempfile.TemporaryDirectory() as wd:
+            r = oracle.ref_xcscf(s, wd, ce_guess=C)
+        assert abs(r["energy"] - (-5.7800665094)) < 1e-10 and r["niter"] == 8, r
+        res["example"] = dict(input="xchf_example/exam_xchf.inp", guess="xchf_example/guessCE.inp", **r)
+        print("example", r["energy"], r["niter"], flush=True)
+        json.dump(res, open(path, "w"), indent=1)
+    for name, mk in SYSTEMS.items():
+        if names and name not in names:
+            continue
+        s = mk()
+        with tempfile.TemporaryDirectory() as wd:
+            r = oracle.ref_xcscf(s, wd)
+        print(s["name"], s["nebf"], s["npebf"], s["npbf"], r["energy"], r["niter"], flush=True)
+        res[s["name"]] = dict(system=jsonable(s), **r)
+        json.dump(res, open(path, "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

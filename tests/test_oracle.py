@@ -431,3 +431,95 @@ def test_oracle_rxchfmult_exchange_vs_reference():
             er = oracle.ref_rxchfmult_omgex(order, which, ne, npb, DA, DB, DP, G, X1, X2, B)
             assert eo == pytest.approx(er, rel=1e-12, abs=1e-13)
             check_close(A - F0, B - F0, what=f"RXCHFmult omg{order}ex {which}")
+
+
+# ----------------------------------------------------------------------
+# NEO-XCHF RHF SCF side
+# ----------------------------------------------------------------------
+def _xsys(seed, nelec=4, **kw):
+    s = synth.small(seed=seed, **kw)
+    s["nelec"] = nelec
+    return s
+
+
+@needs_ref
+def test_oracle_one_body_vs_reference(tmp_path):
+    """class_nuc_rep, elec_ovlap / nuc_ovlap, calc_GAM_epcore and calc_GAM_ep
+    (s/p/d on both particles) against the reference's routines."""
+    s = _xsys(91, n_e_centres=2, e_shells=(0, 1, 2), contract=(1, 2), p_shells=(0, 1, 2))
+    e, He, Se, Hp, Sp, Gep = oracle.ref_one_body(s, str(tmp_path))
+    oHe, oSe, oHp, oSp = oracle.one_body(s)
+    assert oracle.enuc(s) == pytest.approx(e, rel=1e-14)
+    for o, r, nm in ((oHe, He, "GAM_ecore"), (oSe, Se, "S_e"), (oHp, Hp, "GAM_pcore"), (oSp, Sp, "S_p"),
+                     (oracle.gam_ep(s), Gep, "GAM_ep")):
+        check_close(o, r, what=nm)
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [1, 5, 12])
+def test_oracle_roothan_vs_reference(n):
+    """ROOTHAN: eigenvalues and the density (electrons: the nocc lowest
+    orbitals doubly occupied; nucleus: orbital NUCST singly)."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    S = A @ A.T / n + np.eye(n)
+    F = rng.standard_normal((n, n))
+    F = F + F.T
+    for npart, ipart, nucst, k1, k2, pre in ((2 * max(1, n // 2), 1, 1, 1, max(1, n // 2), 2.0),
+                                             (1, 2, min(2, n), min(2, n), min(2, n), 1.0)):
+        D, Cm, ev = oracle.ref_roothan(S, F, npart, ipart, nucst)
+        oC, oev, oD = oracle.roothan(S, F, k1, k2, pre)
+        check_close(oev, ev, rtol=1e-12, atol_rel=1e-13, what="eigenvalues")
+        check_close(oD, D, rtol=1e-10, atol_rel=1e-12, what="density")
+
+
+@needs_ref
+@pytest.mark.parametrize("nelec", [1, 2, 4])
+def test_oracle_xchf_fock_vs_reference(tmp_path, nelec):
+    """xchf1_fock (energy components, FE and FP after XCfock_correction) on
+    the in-core path against the reference routine, with random densities."""
+    s = _xsys(92, nelec=nelec, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,))
+    ne, npb = s["nebf"], s["npbf"]
+    ints = oracle.xchf_ints(s)
+    DE, DP = synth.density(ne, float(nelec), 5), synth.density(npb, 1.0, 6)
+    FE, FP, c = oracle.ref_xchf_fock(s, ints, DE, DP, str(tmp_path))
+    oFE, oFP, oc = oracle.xchf_fock(ne, npb, nelec, ints, DE, DP)
+    check_close(oc, c, what="components")
+    check_close(oFE, FE, what="FE")
+    check_close(oFP, FP, what="FP")
+
+
+@pytest.mark.parametrize("name", ["tiny_s", "tiny_pnuc"])
+def test_oracle_xcscf_vs_reference_run(name):
+    """The oracle's SCF loop (ROOTHAN steps) converges to the energy the
+    reference's xcscf (SOSCF steps) printed for the same system and starts
+    from the same initial-guess components (tests/golden/xchf_scf.json)."""
+    from _util import golden_scf, golden_system
+    g = golden_scf()[name]
+    s, guess = golden_system(g)
+    r = oracle.xcscf(s, oracle.xchf_ints(s), ce_guess=guess)
+    assert r["niter"] > 0
+    assert abs(r["energy"] - g["energy"]) <= 1e-8, (r["energy"], g["energy"])
+    names = ("E_TOTAL", "E_NUC", "E_ECORE", "E_PCORE", "E_EP", "E_EE", "E_GAM1", "E_GAM2", "E_GAM3", "E_GAM4",
+             "S_TOTAL")
+    for k, v in zip(names, r["comps_init"]):
+        assert abs(v - g["initial"][k]) <= 1.5e-10 + 1e-12 * abs(v), k
+
+
+def test_input_and_orbital_files():
+    """The reference input reader reproduces the normalised coefficients and
+    bohr coordinates examples/xchf/answer/XCHF.log prints (6 decimals), and
+    the PUSQLF orbital file round-trips byte for byte."""
+    from neorx import xchf
+    d = os.path.join(HERE, "golden", "xchf_example")
+    s = xchf.read_input(os.path.join(d, "exam_xchf.inp"))
+    np.testing.assert_allclose(s["agebfcc"][:4], [0.441353, 0.693497, 0.664184, 0.287167], atol=5e-7)
+    np.testing.assert_allclose(s["agebfcc"][8:], [0.163548, 0.277418, 0.315749, 0.147123], atol=5e-7)
+    np.testing.assert_allclose(s["agnbfcc"], [2.077622], atol=5e-7)
+    np.testing.assert_allclose(s["cat"][:, 2], [-1.75250354, 1.75250354], atol=5e-9)
+    assert list(s["kpestr"]) == [1, 4, 5, 8, 9, 12] and list(s["kpeend"]) == [3, 4, 7, 8, 11, 12]
+    C = xchf.read_mos(os.path.join(d, "guessCE.inp"), 6)
+    out = os.path.join(HERE, "_build", "guess_roundtrip.dat")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    xchf.write_mos(out, C)
+    assert open(out).read() == open(os.path.join(d, "guessCE.inp")).read()
