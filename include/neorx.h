@@ -362,6 +362,85 @@ int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, int64_t npb
                             const double *d_dbe, const double *d_dp, const double *d_gm, double *d_fae,
                             double *d_fp, double *d_fbe, double *d_e, void *stream);
 
+/* ---- autotuner (no counterpart in the reference) ---------------------------
+ * Times the EG3IC1 slab kernel with 8, 12, 16 and 24 loads in flight per lane
+ * on the caller's tensor (dtype 0 FP64, 1 FP32, 2 BF16 storage; reps timed
+ * launches each) and records the fastest for (dtype, nebf): every later EG3
+ * contraction of that storage type and basis size in this process uses it.
+ * best_unroll, best_ms and all_ms[4] (per candidate) may be NULL. */
+int neorx_autotune_eg3_dev(int64_t nebf, int64_t npbf, int64_t dtype, const double *d_de, const void *d_gm3,
+                           int64_t reps, int64_t *best_unroll, double *best_ms, double *all_ms);
+
+/* ---- NEO-XCHF RHF on the device (source/xchf1_independent/) --------------
+ * A context holds every integral of the XCHF energy in HBM, built on the
+ * device from a one-basis system handle in the order of
+ * xcneo_HYBRID.f:663-1002: class_nuc_rep (class_nuc_rep.f:1), elec_ovlap /
+ * nuc_ovlap (pure_ovlap.f:2, :107), calc_GAM_epcore (gam_epcore.f:1),
+ * calc_GAM_ep (gam_ep.f:1), calc_GAM_ee, GAM1_OMP_MD, GAM2_IC1 and GAM3_IC1
+ * (the last two only for nelec > 1, > 2).  GAM_4 (GAM4_ICR) is never stored:
+ * the 5-particle contraction is evaluated in factorized form from GAM_ee and
+ * GAM_2s, with the same result as EG4ICR on the in-core tensor.
+ * store: bit mask of the storage types GAM_3 is kept in (1 FP64, 2 FP32,
+ * 4 BF16); 0 means FP64; bit 8 also autotunes the EG3 contraction for each
+ * stored type (neorx_autotune_eg3_dev).  The integrals are only built here:
+ * these entry points read no .ufm file. */
+int neorx_xchf_create(int64_t handle, int64_t nelec, int64_t store, int64_t *ctx);
+int neorx_xchf_destroy(int64_t ctx);
+/* Host copies of the context's integrals in the reference packings (any
+ * pointer may be NULL): E_nuc (ENUCRP.dat), S_e (eovlap.ufm), S_p
+ * (novlap.ufm), GAM_ecore (GAM_ecr.ufm), GAM_pcore (GAM_pcr.ufm), GAM_ep
+ * (GAM_ep.ufm, pack_4D(nebf,nebf,npbf)), GAM_ee (GAM_ee.ufm). */
+int neorx_xchf_integrals(int64_t ctx, double *enuc, double *se, double *sp, double *gam_ecore, double *gam_pcore,
+                         double *gam_ep, double *gam_ee);
+/* xchf1_fock, source/xchf1_independent/xchf1_fock.f:2, on device arrays, for
+ * the in-core path of xcneo_HYBRID (LG2IC1, LG3IC1, LG4IC): d_focke, d_fockp
+ * receive the Fock matrices after XCfock_correction (xchf1_fock.f:728-784),
+ * d_comps 14 doubles: E_total, E_nuc, E_ecore, E_pcore, E_ep, E_ee, E_gam1,
+ * E_gam2, E_gam3, E_gam4, S_total, S_gam1, S_gam2, H_expect.  prec selects
+ * the stored GAM_3 read (0 FP64, 1 FP32, 2 BF16); all arithmetic is FP64.
+ * Enqueued on stream, not synchronised. */
+int neorx_xchf_fock_dev(int64_t ctx, const double *d_de, const double *d_dp, int64_t prec, double *d_focke,
+                        double *d_fockp, double *d_comps, void *stream);
+/* The FP64 MFMA GEMM of the factorized 5-particle contraction (no
+ * counterpart in the reference): C (M x N, row-major) = A B with
+ * A(m,k) = d_a[m*sam + k*sak], B(k,n) = d_b[k*sbk + n*sbn]; split over K
+ * (stream-ordered scratch) when C has few 16x16 tiles.  Not synchronised. */
+int neorx_gemm_f64_dev(int64_t M, int64_t N, int64_t K, const double *d_a, int64_t sam, int64_t sak, const double *d_b,
+                       int64_t sbk, int64_t sbn, double *d_c, void *stream);
+/* ROOTHAN, source/xchf1_independent/xcscf.f:766, on device arrays (n <= 128):
+ * X = S^-1/2, diagonalisation of X^T F X (eigenvalues ascending), C = X C',
+ * D = prefac sum_{k = kstart..klast} C(:,k) C(:,k)^T (1-based k; the
+ * reference uses 1..nelec/2 with prefac 2 for electrons and NUCST..NUCST
+ * with 1 for the nucleus).  Stream-ordered scratch; not synchronised. */
+int neorx_roothan_dev(int64_t n, const double *d_s, const double *d_f, int64_t kstart, int64_t klast, double prefac,
+                      double *d_c, double *d_ev, double *d_d, void *stream);
+/* xcscf, source/xchf1_independent/xcscf.f:2-560: the NEO-XCHF RHF SCF on the
+ * device.  Per iteration, as xcscf: the Fock build with (DE, DP), ROOTHAN for
+ * the nucleus, the build with (DE, new DP), ROOTHAN for the electrons, DENDIF;
+ * converged when max|dDE| < tole (and max|dDP| < tolp if npbf > 1).  The
+ * reference's second-order step (SOSCF) is not restated; flags bit 2 turns on
+ * DIIS extrapolation of the electronic Fock matrix instead.  The two builds
+ * of one iteration share one pass over GM3/GM2 (all terms are linear in DP).
+ *   ce_guess: nebf x nebf column-major orbitals (read_elec_density of
+ *             guessCE.inp), or NULL for the core guess (guess_elec);
+ *   nucst:    occupied nuclear orbital (guess_prot, ROOTHAN);
+ *   mode:     0 FP64 GAM_3; 1 FP32 until the switch signal < sw_lo, then
+ *             FP64; 2 BF16 until < sw_hi, then FP32 until < sw_lo, then FP64
+ *             (signal: the DIIS error max|X^T(FDS - SDF)X| with DIIS, else
+ *             max|dDE|); convergence is only declared on an FP64 iteration;
+ *   flags:    bit 1: capture one iteration as a hipGraph and replay it;
+ *             bit 2: DIIS;
+ *   comps_init / comps: 14 components (as neorx_xchf_fock_dev) of the first
+ *             build of iteration 1 (the reference's INITIAL GUESS ENERGETICS)
+ *             and of the last electronic build (the final energy);
+ *   ce, cp, ev_e, ev_p: final orbitals and eigenvalues (FinalCE.dat, ...);
+ *   trace:    maxit x 8 doubles per iteration: E_total, dE, max|dDE|,
+ *             max|dDP|, switch signal, GAM_3 storage (0/1/2), wall ms, 0.
+ * Returns NEORX_E_ARG (after filling the outputs) if not converged. */
+int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst, int64_t maxit, double tole, double tolp,
+                   int64_t mode, int64_t flags, double sw_hi, double sw_lo, double *comps_init, double *comps,
+                   double *ce, double *cp, double *ev_e, double *ev_p, double *trace, int64_t *niter);
+
 #ifdef __cplusplus
 }
 #endif

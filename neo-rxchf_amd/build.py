@@ -2,8 +2,10 @@
 
     python neo-rxchf_amd/build.py          # or __graft_entry__.build()
 
-hipcc cross-compiles for gfx950 without a GPU present.
+hipcc cross-compiles for gfx950 without a GPU present.  Each HIP translation
+unit of csrc/ is compiled to an object in parallel, then linked.
 """
+import glob
 import os
 import subprocess
 import sys
@@ -14,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
 
 HIPFLAGS = [
-    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
     # keep the compiler from emitting packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -26,10 +28,23 @@ def run(cmd, cwd=None):
 
 
 def build_hip():
-    os.makedirs(os.path.join(HERE, "lib"), exist_ok=True)
-    out = os.path.join(HERE, "lib", "libneorx_hip.so")
+    lib = os.path.join(HERE, "lib")
+    objdir = os.path.join(lib, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    procs, objs = [], []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + HIPFLAGS + ["-c", "-o", obj, src]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    bad = [cmd for cmd, p in procs if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
+    out = os.path.join(lib, "libneorx_hip.so")
     tmp = out + ".tmp"
-    run([HIPCC] + HIPFLAGS + ["-o", tmp, os.path.join(HERE, "csrc", "nrx_lib.hip")])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     return out
 
@@ -55,7 +70,7 @@ def build_shim_driver():
     outdir = os.path.join(ROOT, "tests", "_build")
     os.makedirs(outdir, exist_ok=True)
     out = os.path.join(outdir, "drive_shim")
-    run([FLANG, "-O2", "-fdefault-integer-8", "-J", outdir, "-o", out, src, "-L" + lib, "-lneorx_fshim",
+    run([FLANG, "-O2", "-fdefault-integer-8", "-J", outdir, "-I", lib, "-o", out, src, "-L" + lib, "-lneorx_fshim",
          "-lneorx_hip", "-Wl,-rpath,$ORIGIN/../../neo-rxchf_amd/lib"])
     return out
 
@@ -67,5 +82,8 @@ def build_all():
 
 
 if __name__ == "__main__":
-    build_all()
+    if len(sys.argv) > 1 and sys.argv[1] == "hip":
+        build_hip()
+    else:
+        build_all()
     sys.exit(0)

@@ -9,24 +9,11 @@
 //     wavefront share loop trip counts;
 //   * contracted tensors use the reference packings (C row-major
 //     [ip][jp][ie1][je1]...).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
 #include <map>
 #include <mutex>
 #include <numeric>
-#include <string>
-#include <vector>
 
-#include "../../include/neorx.h"
-#include "nrx_engine.h"
-
-typedef int64_t i64;
-#define NRX_DI __device__ inline __attribute__((always_inline))
-using namespace nrx;
+#include "nrx_internal.h"
 
 // ------------------------------------------------------------------
 // error handling
@@ -37,7 +24,9 @@ static int g_device = -1;
 static char g_arch[64] = "";
 static double *g_boys_dev = nullptr; // Boys table in HBM, one per process
 
-static int fail(int code, const char *fmt, ...)
+i64 g_grid_cap = (i64)1 << 20;       // most blocks of a grid-stride launch
+
+int fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -48,13 +37,7 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(x)                                                                                                  \
-    do {                                                                                                           \
-        hipError_t _e = (x);                                                                                       \
-        if (_e != hipSuccess) return fail(NEORX_E_HIP, "%s failed: %s", #x, hipGetErrorString(_e));              \
-    } while (0)
-
-static int ensure_init()
+int ensure_init()
 {
     if (g_device >= 0) return NEORX_OK;
     return neorx_init(0);
@@ -63,25 +46,6 @@ static int ensure_init()
 // ------------------------------------------------------------------
 // system handle
 // ------------------------------------------------------------------
-struct DevSys {
-    i64 nebf, npebf, npbf, nat, ngtg;
-    i64 nebf1, npebf1;        // first electron basis; the rest is the special-electron basis (RXCHFmult)
-    double pmass;
-    const double *cat, *zan, *bcoef, *gamma, *agebfcc, *agnbfcc;
-    const i64 *kpestr, *kpeend, *ampeb2c;
-    const i64 *eord, *pord;   // enumeration order -> original primitive index
-    const PairRec *ep, *pp;   // bra*ket tables [i][j] over primitives (npebf^2, npbf^2)
-    const double *bt;         // Boys table (BT_N x BT_M)
-};
-
-struct System {
-    DevSys d;
-    std::vector<void *> allocs;
-    std::vector<i64> eord, pord;
-    std::vector<i64> kpestr; // host copy (1-based)
-    int maxl;
-};
-
 static std::map<i64, System *> g_sys;
 static i64 g_next = 1;
 
@@ -96,7 +60,7 @@ template <class T> static int upload(System *s, const T *h, size_t n, const T **
     return NEORX_OK;
 }
 
-static System *get_sys(i64 h)
+System *get_sys(i64 h)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_sys.find(h);
@@ -121,6 +85,12 @@ extern "C" int neorx_init(int64_t device)
         boys_table_fill(tab.data());
         HIPCHK(hipMalloc(&g_boys_dev, tab.size() * sizeof(double)));
         HIPCHK(hipMemcpy(g_boys_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    g_grid_cap = (i64)1 << 20;
+    if (const char *cap = getenv("NEORX_GRID_CAP")) {
+        long long v = atoll(cap);
+        if (v < 1) return fail(NEORX_E_ARG, "NEORX_GRID_CAP=%s: expected a positive block count", cap);
+        g_grid_cap = std::min<i64>(v, g_grid_cap);
     }
     g_device = (int)device;
     return NEORX_OK;
@@ -550,7 +520,6 @@ __global__ void k_zero_u32n(unsigned *p, int n)
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-static inline unsigned nblocks(i64 n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // tuples per launch of the primitive kernels (bounds the sorted list, whose
 // entries are 32-bit positions, and the bin counters)
@@ -740,13 +709,13 @@ __global__ void __launch_bounds__(256) k_gam2_symm(i64 ne, i64 n, const double *
                                                    const double *__restrict__ Xs, double *__restrict__ G,
                                                    double *__restrict__ Gs)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 ne2 = ne * ne;
-    i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
-    i64 t21 = (b * ne2 + p2) * ne2 + p1;
-    G[t] = (X[t] + X[t21]) / 2.0;
-    Gs[t] = (Xs[t] + Xs[t21]) / 2.0;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 ne2 = ne * ne;
+        i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
+        i64 t21 = (b * ne2 + p2) * ne2 + p1;
+        G[t] = (X[t] + X[t21]) / 2.0;
+        Gs[t] = (Xs[t] + Xs[t21]) / 2.0;
+    }
 }
 
 // e-e-e-p: lane per contracted (L, j1, i2, j2, i3, j3), L = (ip*npbf + jp)*nebf + i1
@@ -783,16 +752,16 @@ __global__ void __launch_bounds__(256) k_gam3_symm(i64 ne, i64 n, const double *
                                                    double *__restrict__ G)
 {
     i64 ne2 = ne * ne, blk = ne2 * ne2 * ne2;
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 b = t / blk, r = t % blk;
-    i64 p3 = r % ne2, p2 = (r / ne2) % ne2, p1 = r / (ne2 * ne2);
-    const double *Xb = X + b * blk;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 b = t / blk, r = t % blk;
+        i64 p3 = r % ne2, p2 = (r / ne2) % ne2, p1 = r / (ne2 * ne2);
+        const double *Xb = X + b * blk;
 #define IX(a, c, e) (((a) * ne2 + (c)) * ne2 + (e))
-    double s = Xb[IX(p1, p2, p3)] + Xb[IX(p1, p3, p2)] + Xb[IX(p2, p1, p3)] + Xb[IX(p2, p3, p1)] +
-               Xb[IX(p3, p1, p2)] + Xb[IX(p3, p2, p1)];
+        double s = Xb[IX(p1, p2, p3)] + Xb[IX(p1, p3, p2)] + Xb[IX(p2, p1, p3)] + Xb[IX(p2, p3, p1)] +
+                   Xb[IX(p3, p1, p2)] + Xb[IX(p3, p2, p1)];
 #undef IX
-    G[t] = s / 6.0;
+        G[t] = s / 6.0;
+    }
 }
 
 // calc_GAM_ee: lane per contracted quadruple, primitive sums in the
@@ -874,23 +843,6 @@ __global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, co
 // ------------------------------------------------------------------
 // host helpers for builders
 // ------------------------------------------------------------------
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t bytes)
-    {
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(NEORX_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-        }
-        return NEORX_OK;
-    }
-    double *d() { return (double *)p; }
-};
 
 static int h2d(DevBuf &b, const double *h, i64 n)
 {
@@ -997,9 +949,32 @@ extern "C" int neorx_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2s,
     if (!rc) rc = Ys.alloc(cap * 8);
     if (!rc) rc = build2_block(s, 0, np * np, X.d(), Xs.d(), Y.d(), Ys.d(), 0, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2s);
+    hipLaunchKernelGGL(k_gam2_symm, dim3(grid_blocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st)); // scratch buffers are released on return
+    return NEORX_OK;
+}
+
+// GAM3_IC1 (class 3) and RXCHFmult_GAM3_XCHF (class 7): contracted e-e-e-p
+// values per (ip,jp) block, then the six-permutation average into d_gm3
+template <int CLS> static int gam3_build_dev(System *s, double *d_gm3, void *stream)
+{
+    i64 ne = s->d.nebf, np = s->d.npbf, blk = ne * ne * ne * ne * ne * ne;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap = slab_cap(s, 3);
+    // one (ip,jp) block of unsymmetrised values at a time: the scratch is
+    // nebf^6 doubles, not the whole tensor again
+    DevBuf X, Y;
+    int rc = X.alloc(blk * 8);
+    if (!rc) rc = Y.alloc(cap * 8);
+    if (rc) return rc;
+    for (i64 b = 0; b < np * np; ++b) {
+        rc = build3_block<CLS>(s, b, b + 1, X.d(), Y.d(), st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_gam3_symm, dim3(grid_blocks(blk, 256)), dim3(256), 0, st, ne, blk, X.d(), d_gm3 + b * blk);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
 }
 
@@ -1008,18 +983,7 @@ extern "C" int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3, void *stream)
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
     if (!d_gm3) return fail(NEORX_E_ARG, "null output");
-    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
-    hipStream_t st = (hipStream_t)stream;
-    i64 cap = slab_cap(s, 3);
-    DevBuf X, Y;
-    int rc = X.alloc(ng3 * 8);
-    if (!rc) rc = Y.alloc(cap * 8);
-    if (!rc) rc = build3_block(s, 0, np * np, X.d(), Y.d(), st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, ng3, X.d(), d_gm3);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    return NEORX_OK;
+    return gam3_build_dev<3>(s, d_gm3, stream);
 }
 
 static int copy_out(double *h, const double *d, i64 n)
@@ -1085,6 +1049,22 @@ extern "C" int neorx_gam_ee(int64_t handle, double *gam_ee)
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     return copy_out(gam_ee, G.d(), n);
+}
+
+int nrx_gam_ee_launch(System *s, double *d_gee, hipStream_t st)
+{
+    i64 ne = s->d.nebf, n = ne * ne * ne * ne;
+    hipLaunchKernelGGL(k_gam_ee, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_gee);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+int nrx_gam1_launch(System *s, double *d_g1, double *d_g1s, hipStream_t st)
+{
+    i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * np * np;
+    hipLaunchKernelGGL(k_gam1<1>, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_g1, d_g1s);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
 }
 
 extern "C" int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, const double *gam_ee, double *gm4icr)
@@ -1160,6 +1140,15 @@ __global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, i64 np, const double *_
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 4 * ne; k += NT) part[blk * 4 * ne + k] = sm[k];
+}
+
+int nrx_eg2_slabs(i64 ne, i64 nblk, const double *d_de, const double *g, const double *gs, double *part,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL((k_eg2_slab<256, double>), dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, (i64)0, d_de,
+                       g, gs, part);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
 }
 
 // reduce partials: FE(a,b) += 2 sum_{ip,jp} DP(ip,jp) [dir_a[b] - ex_a[b]/2] etc.
@@ -1328,22 +1317,45 @@ NRX_DI double wave_sum(double v)
 //   F[f] += 3 ( 1/4 DE(c,b)DE(e,d) - 1/2 DE(c,d)DE(e,b) ) g
 // (F = FE row a before the DP(ip,jp) weight).  Each thread owns one (e,f)
 // column pair, so every read of the slab is a contiguous row of ne^2 values;
-// the sums over c run in registers, F[b] and F[d] are wave-reduced before one
-// LDS atomic per wavefront.  T is the storage type of GM3 (double, or float
-// for the mixed-precision path); all arithmetic is FP64.
+// the sums over c run in registers.  F[b] and F[d] are wave-reduced and added
+// by lane 0 into its wavefront's own LDS row; F[f] is kept per (e,f) column;
+// the rows and columns are summed in a fixed order at the end, so the result
+// does not depend on wavefront timing.  T is the storage type of GM3 (double,
+// or float for the mixed-precision path); all arithmetic is FP64.
 // part[blk][0..ne-1] = F, part[blk][ne] = sum_x DE(a,x) F[x] / 3.
-template <class T>
+// BF16 storage: the upper 16 bits of the FP32 value (round to nearest even)
+struct bf16_t {
+    uint16_t bits;
+};
+
+// nontemporal load of one stored tensor entry, widened to FP64
+NRX_DI double ld_entry(const double *p) { return __builtin_nontemporal_load(p); }
+NRX_DI double ld_entry(const float *p) { return (double)__builtin_nontemporal_load(p); }
+NRX_DI double ld_entry(const bf16_t *p)
+{
+    unsigned short b = __builtin_nontemporal_load((const unsigned short *)p);
+    return (double)__uint_as_float((unsigned)b << 16);
+}
+
+// loads in flight per lane in the EG3 slab loop when no tuned value is
+// recorded: 16 for FP64 storage and 12 for FP32 (both measured best at
+// nebf = 24), 16 for BF16
+template <class T> constexpr int eg3_unroll() { return sizeof(T) == 4 ? 12 : 16; }
+
+template <class T, int U = eg3_unroll<T>()>
 __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__restrict__ DE, const T *__restrict__ G,
                                                    double *__restrict__ part)
 {
     // offsets inside one slab are 32-bit (ne^5 < 2^31 is checked at launch)
-    extern __shared__ double F[]; // [ne]
+    extern __shared__ double sm[]; // Fw[nw][ne] (per-wavefront rows), Ff[ne^2] (per-column)
     const unsigned ne = (unsigned)ne_;
     i64 blk = blockIdx.x;
     unsigned a = (unsigned)(blk % ne);
     const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne;
+    const unsigned nw = blockDim.x >> 6;
+    double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne;
     const T *g = G + blk * (i64)ne4 * ne;
-    for (int k = threadIdx.x; k < ne; k += blockDim.x) F[k] = 0.0;
+    for (unsigned k = threadIdx.x; k < nw * ne + ne2; k += blockDim.x) sm[k] = 0.0;
     __syncthreads();
     const bool lead = (threadIdx.x & 63) == 0;
     for (unsigned ef0 = 0; ef0 < ne2; ef0 += blockDim.x) {
@@ -1363,14 +1375,12 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                     const double *dd = DE + d * ne, *db = DE + b * ne, *df = DE + f * ne;
                     unsigned c = 0;
                     // U (then eight) independent loads in flight per lane before
-                    // the FMAs, U = 16 for FP64 storage and 12 for FP32 (both
-                    // measured best); the slab is read once, so the loads are
+                    // the FMAs; the slab is read once, so the loads are
                     // nontemporal
-                    constexpr int U = sizeof(T) == 8 ? 16 : 12;
                     for (; c + U <= ne; c += U) {
                         double gv[U];
 #pragma unroll
-                        for (int k = 0; k < U; ++k) gv[k] = (double)__builtin_nontemporal_load(&g[o0 + (c + k) * ne3]);
+                        for (int k = 0; k < U; ++k) gv[k] = ld_entry(&g[o0 + (c + k) * ne3]);
 #pragma unroll
                         for (int k = 0; k < U; ++k) {
                             s1 += dd[c + k] * gv[k]; // DE(c,d)
@@ -1381,7 +1391,7 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                     if (c + 8 <= ne) {
                         double gv[8];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) gv[k] = (double)__builtin_nontemporal_load(&g[o0 + (c + k) * ne3]);
+                        for (int k = 0; k < 8; ++k) gv[k] = ld_entry(&g[o0 + (c + k) * ne3]);
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
                             s1 += dd[c + k] * gv[k];
@@ -1391,7 +1401,7 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                         c += 8;
                     }
                     for (; c < ne; ++c) {
-                        double gv = (double)__builtin_nontemporal_load(&g[o0 + c * ne3]);
+                        double gv = ld_entry(&g[o0 + c * ne3]);
                         s1 += dd[c] * gv;
                         s2 += db[c] * gv;
                         s3 += df[c] * gv;
@@ -1400,22 +1410,30 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                 accb += de_ef * s1 - 0.5 * de_ed * s3;
                 accf += 0.25 * de_ed * s2 - 0.5 * de_eb * s1;
                 double vd = wave_sum(-0.5 * de_ef * s2 + 0.25 * de_eb * s3);
-                if (lead) atomicAdd(&F[d], 3.0 * vd);
+                if (lead) Fw[d] += 3.0 * vd;
             }
             accb = wave_sum(accb);
-            if (lead) atomicAdd(&F[b], 3.0 * accb);
+            if (lead) Fw[b] += 3.0 * accb;
         }
-        if (on) atomicAdd(&F[f], 3.0 * accf);
+        if (on) Ff[ef] = 3.0 * accf; // each (e,f) column belongs to one lane
     }
     __syncthreads();
     if (threadIdx.x < 64) {
         double p = 0.0;
-        for (unsigned x = threadIdx.x; x < ne; x += 64) p += DE[x * ne + a] * F[x];
+        for (unsigned x = threadIdx.x; x < ne; x += 64) {
+            double fx = 0.0;
+            for (unsigned w = 0; w < nw; ++w) fx += sm[w * ne + x];
+            for (unsigned e = 0; e < ne; ++e) fx += Ff[e * ne + x];
+            part[blk * (ne + 1) + x] = fx;
+            p += DE[x * ne + a] * fx;
+        }
         p = wave_sum(p);
-        for (unsigned x = threadIdx.x; x < ne; x += 64) part[blk * (ne + 1) + x] = F[x];
         if (threadIdx.x == 0) part[blk * (ne + 1) + ne] = p / 3.0;
     }
 }
+
+// LDS bytes of k_eg3_slab: per-wavefront rows and per-column partials
+static inline size_t eg3_lds(i64 ne, unsigned threads) { return (size_t)((threads / 64) * ne + ne * ne) * 8; }
 
 // largest nebf whose e-e-e-p slab (nebf^5 entries) has 32-bit offsets
 static const i64 EG3_MAX_NE = 73;
@@ -1424,6 +1442,96 @@ static inline unsigned eg3_threads(i64 ne)
 {
     i64 t = ((ne * ne + 63) / 64) * 64;
     return (unsigned)std::min<i64>(t, 1024);
+}
+
+// ------------------------------------------------------------------
+// autotuned EG3 slab launches: the loads-in-flight depth U of k_eg3_slab per
+// (storage type, nebf), chosen by neorx_autotune_eg3_dev from timed launches
+// on the caller's tensor; 0 = not tuned (eg3_unroll's default)
+// ------------------------------------------------------------------
+static const int EG3_UNROLLS[4] = {8, 12, 16, 24};
+static int g_eg3_tuned[3][EG3_MAX_NE + 1] = {};
+
+template <class T>
+static int eg3_launch_u(int u, i64 ne, i64 nblk, const double *d_de, const T *g, double *part, hipStream_t st)
+{
+    const unsigned nt = eg3_threads(ne);
+    const size_t lds = eg3_lds(ne, nt);
+    const dim3 grid((unsigned)nblk), blk(nt);
+    switch (u) {
+    case 8: hipLaunchKernelGGL((k_eg3_slab<T, 8>), grid, blk, lds, st, ne, d_de, g, part); break;
+    case 12: hipLaunchKernelGGL((k_eg3_slab<T, 12>), grid, blk, lds, st, ne, d_de, g, part); break;
+    case 16: hipLaunchKernelGGL((k_eg3_slab<T, 16>), grid, blk, lds, st, ne, d_de, g, part); break;
+    case 24: hipLaunchKernelGGL((k_eg3_slab<T, 24>), grid, blk, lds, st, ne, d_de, g, part); break;
+    default: return fail(NEORX_E_ARG, "EG3 slabs: no variant with %d loads in flight", u);
+    }
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int eg3_launch(int dtype, int u, i64 ne, i64 nblk, const double *d_de, const void *g, double *part,
+                      hipStream_t st)
+{
+    switch (dtype) {
+    case NRX_F64: return eg3_launch_u<double>(u ? u : eg3_unroll<double>(), ne, nblk, d_de, (const double *)g, part, st);
+    case NRX_F32: return eg3_launch_u<float>(u ? u : eg3_unroll<float>(), ne, nblk, d_de, (const float *)g, part, st);
+    case NRX_BF16: return eg3_launch_u<bf16_t>(u ? u : eg3_unroll<bf16_t>(), ne, nblk, d_de, (const bf16_t *)g, part, st);
+    default: return fail(NEORX_E_ARG, "EG3 slabs: unknown storage type %d", dtype);
+    }
+}
+
+int nrx_eg3_slabs(i64 ne, i64 nblk, const double *d_de, const void *g, int dtype, double *part, hipStream_t st)
+{
+    if (ne <= 0 || ne > EG3_MAX_NE) return fail(NEORX_E_UNSUPPORTED, "EG3 slabs: nebf=%lld not in 1..%lld",
+                                                (long long)ne, (long long)EG3_MAX_NE);
+    if (dtype < 0 || dtype > 2) return fail(NEORX_E_ARG, "EG3 slabs: unknown storage type %d", dtype);
+    return eg3_launch(dtype, g_eg3_tuned[dtype][ne], ne, nblk, d_de, g, part, st);
+}
+
+extern "C" int neorx_autotune_eg3_dev(int64_t nebf, int64_t npbf, int64_t dtype, const double *d_de, const void *d_gm3,
+                                      int64_t reps, int64_t *best_unroll, double *best_ms, double *all_ms)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf;
+    if (ne <= 0 || ne > EG3_MAX_NE || np <= 0) return fail(NEORX_E_ARG, "autotune EG3: bad sizes");
+    if (dtype < 0 || dtype > 2) return fail(NEORX_E_ARG, "autotune EG3: unknown storage type %lld", (long long)dtype);
+    if (!d_de || !d_gm3 || reps < 1) return fail(NEORX_E_ARG, "autotune EG3: bad arguments");
+    const i64 nblk = np * np * ne;
+    DevBuf part;
+    rc = part.alloc(nblk * (ne + 1) * 8);
+    if (rc) return rc;
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreate(&st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = 0;
+    float bt = 0.0f;
+    for (int k = 0; k < 4 && !rc; ++k) {
+        const int u = EG3_UNROLLS[k];
+        rc = eg3_launch((int)dtype, u, ne, nblk, d_de, d_gm3, part.d(), st); // warm-up
+        if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventRecord failed");
+        for (i64 r = 0; r < reps && !rc; ++r) rc = eg3_launch((int)dtype, u, ne, nblk, d_de, d_gm3, part.d(), st);
+        if (!rc) rc = hipEventRecord(e1, st) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventRecord failed");
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventSynchronize failed");
+        float ms = 0.0f;
+        if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (all_ms) all_ms[k] = ms;
+        if (!rc && (best == 0 || ms < bt)) {
+            best = u;
+            bt = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    g_eg3_tuned[dtype][ne] = best;
+    if (best_unroll) *best_unroll = best;
+    if (best_ms) *best_ms = bt;
+    return NEORX_OK;
 }
 
 __global__ void k_eg3_reduce(i64 ne, i64 np, const double *__restrict__ DP, const double *__restrict__ part,
@@ -1462,9 +1570,8 @@ static int eg3_dev(int64_t nebf, int64_t npbf, const double *d_de, const double 
     rc = part.alloc(nblk * (ne + 1) * 8);
     if (!rc) rc = fpd.alloc(np * np * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_eg3_slab<T>, dim3((unsigned)nblk), dim3(eg3_threads(ne)), ne * 8, st, ne, d_de, d_gm3,
-                       part.d());
-    HIPCHK(hipGetLastError());
+    rc = nrx_eg3_slabs(ne, nblk, d_de, d_gm3, sizeof(T) == 8 ? NRX_F64 : NRX_F32, part.d(), st);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(),
                        d_fe, fpd.d());
     HIPCHK(hipGetLastError());
@@ -1716,7 +1823,7 @@ extern "C" int neorx_gam2_dscf_dev(int64_t handle, const double *d_de, const dou
         i64 p1 = std::min(npp, p0 + nb), n = (p1 - p0) * ne4;
         rc = build2_block(s, p0, p1, X.d(), Xs.d(), Y.d(), Ys.d(), 1, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), Xs.d(), G.d(), Gs.d());
+        hipLaunchKernelGGL(k_gam2_symm, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), Xs.d(), G.d(), Gs.d());
         hipLaunchKernelGGL((k_eg2_slab<256, double>), dim3((unsigned)((p1 - p0) * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
                            G.d(), Gs.d(), part.d() + p0 * ne * 4 * ne);
         HIPCHK(hipGetLastError());
@@ -1753,9 +1860,9 @@ extern "C" int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const dou
         i64 p1 = std::min(npp, p0 + nb), n = (p1 - p0) * ne6;
         rc = build3_block(s, p0, p1, X.d(), Y.d(), st);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), G.d());
-        hipLaunchKernelGGL(k_eg3_slab<double>, dim3((unsigned)((p1 - p0) * ne)), dim3(eg3_threads(ne)), ne * 8, st, ne,
-                           d_de, G.d(), part.d() + p0 * ne * (ne + 1));
+        hipLaunchKernelGGL(k_gam3_symm, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), G.d());
+        hipLaunchKernelGGL(k_eg3_slab<double>, dim3((unsigned)((p1 - p0) * ne)), dim3(eg3_threads(ne)),
+                           eg3_lds(ne, eg3_threads(ne)), st, ne, d_de, G.d(), part.d() + p0 * ne * (ne + 1));
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
@@ -2265,13 +2372,13 @@ __global__ void __launch_bounds__(256) k_rxne_symm(i64 ne, i64 n, const double *
                                                    const double *__restrict__ Xs, double *__restrict__ G,
                                                    double *__restrict__ Gex)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 ne2 = ne * ne;
-    i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
-    i64 t21 = (b * ne2 + p2) * ne2 + p1;
-    G[t] = X[t];
-    Gex[t] = (Xs[t] + Xs[t21]) / 2.0;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 ne2 = ne * ne;
+        i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
+        i64 t21 = (b * ne2 + p2) * ne2 + p1;
+        G[t] = X[t];
+        Gex[t] = (Xs[t] + Xs[t21]) / 2.0;
+    }
 }
 
 extern "C" int neorx_rxchfne_gam2_ic1_dev(int64_t handle, double *d_gm2, double *d_gm2ex, void *stream)
@@ -2289,7 +2396,7 @@ extern "C" int neorx_rxchfne_gam2_ic1_dev(int64_t handle, double *d_gm2, double 
     if (!rc) rc = Ys.alloc(cap * 8);
     if (!rc) rc = build2_block<4>(s, 0, np * np, X.d(), Xs.d(), Y.d(), Ys.d(), 0, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_rxne_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2ex);
+    hipLaunchKernelGGL(k_rxne_symm, dim3(grid_blocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X.d(), Xs.d(), d_gm2, d_gm2ex);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
@@ -2627,14 +2734,14 @@ __global__ void __launch_bounds__(256) k_mult_symm2(i64 na, i64 nb, i64 n, const
                                                     const double *__restrict__ Xs, double *__restrict__ G,
                                                     double *__restrict__ Gs)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 j2 = t % nb, i2 = (t / nb) % nb, j1 = (t / (nb * nb)) % nb, i1 = (t / (nb * nb * nb)) % nb,
-        pp = t / (nb * nb * nb * nb);
-    i64 a = ((pp * na + i1) * na + j1) * nb * nb + i2 * nb + j2;
-    i64 b = ((pp * na + i2) * na + j2) * nb * nb + i1 * nb + j1;
-    G[t] = (X[a] + X[b]) / 2.0;
-    Gs[t] = (Xs[a] + Xs[b]) / 2.0;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 j2 = t % nb, i2 = (t / nb) % nb, j1 = (t / (nb * nb)) % nb, i1 = (t / (nb * nb * nb)) % nb,
+            pp = t / (nb * nb * nb * nb);
+        i64 a = ((pp * na + i1) * na + j1) * nb * nb + i2 * nb + j2;
+        i64 b = ((pp * na + i2) * na + j2) * nb * nb + i1 * nb + j1;
+        G[t] = (X[a] + X[b]) / 2.0;
+        Gs[t] = (Xs[a] + Xs[b]) / 2.0;
+    }
 }
 
 // The contracted e-e-p tensors of a two-basis handle: X1, Xs (unsymmetrised
@@ -2681,7 +2788,7 @@ extern "C" int neorx_rxchfmult_gam2_dev(int64_t handle, double *d_int_gam2, doub
     if (!rc) rc = Xs.alloc(nint * 8);
     if (!rc) rc = mult_build2(s, X1.d(), Xs.d(), d_int_gam2, nullptr, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_mult_symm2, dim3(nblocks(nx, 256)), dim3(256), 0, st, na, nb, nx, X1.d(), Xs.d(),
+    hipLaunchKernelGGL(k_mult_symm2, dim3(grid_blocks(nx, 256)), dim3(256), 0, st, na, nb, nx, X1.d(), Xs.d(),
                        d_xchf_gam2, d_xchf_gam2s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -2710,9 +2817,9 @@ extern "C" int neorx_rxchfmult_gam2_ic1ex_dev(int64_t handle, double *d_gm2_1, d
     if (!rc) rc = X3.alloc(ng2 * 8);
     if (!rc) rc = mult_build2(s, X1.d(), Xs.d(), d_gm2_2, X3.d(), st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, na, ng2, X1.d(), Xs.d(), d_gm2_1,
+    hipLaunchKernelGGL(k_gam2_symm, dim3(grid_blocks(ng2, 256)), dim3(256), 0, st, na, ng2, X1.d(), Xs.d(), d_gm2_1,
                        d_gm2s);
-    hipLaunchKernelGGL(k_gam2_symm, dim3(nblocks(ng2, 256)), dim3(256), 0, st, na, ng2, X3.d(), X3.d(), d_gm2_3,
+    hipLaunchKernelGGL(k_gam2_symm, dim3(grid_blocks(ng2, 256)), dim3(256), 0, st, na, ng2, X3.d(), X3.d(), d_gm2_3,
                        d_gm2_3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -2802,11 +2909,11 @@ __global__ void __launch_bounds__(256) k_mcontract3(DevSys S, i64 L0, i64 n, i64
 __global__ void __launch_bounds__(256) k_mult_symm3(i64 nb, i64 n, const double *__restrict__ X,
                                                     double *__restrict__ G)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 nb2 = nb * nb;
-    i64 p3 = t % nb2, p2 = (t / nb2) % nb2, head = t / (nb2 * nb2);
-    G[t] = (X[t] + X[(head * nb2 + p3) * nb2 + p2]) * 0.5;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 nb2 = nb * nb;
+        i64 p3 = t % nb2, p2 = (t / nb2) % nb2, head = t / (nb2 * nb2);
+        G[t] = (X[t] + X[(head * nb2 + p3) * nb2 + p2]) * 0.5;
+    }
 }
 
 // one basis (ne): (x123 + x213)/2 (swap = 1) or (x123 + x321)/2 (swap = 2),
@@ -2814,12 +2921,12 @@ __global__ void __launch_bounds__(256) k_mult_symm3(i64 nb, i64 n, const double 
 __global__ void __launch_bounds__(256) k_mult_symm3x(i64 ne, i64 n, int swap, const double *__restrict__ X,
                                                      double *__restrict__ G)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 ne2 = ne * ne;
-    i64 p3 = t % ne2, p2 = (t / ne2) % ne2, p1 = (t / (ne2 * ne2)) % ne2, pp = t / (ne2 * ne2 * ne2);
-    i64 o = swap == 1 ? ((pp * ne2 + p2) * ne2 + p1) * ne2 + p3 : ((pp * ne2 + p3) * ne2 + p2) * ne2 + p1;
-    G[t] = (X[t] + X[o]) * 0.5;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 ne2 = ne * ne;
+        i64 p3 = t % ne2, p2 = (t / ne2) % ne2, p1 = (t / (ne2 * ne2)) % ne2, pp = t / (ne2 * ne2 * ne2);
+        i64 o = swap == 1 ? ((pp * ne2 + p2) * ne2 + p1) * ne2 + p3 : ((pp * ne2 + p3) * ne2 + p2) * ne2 + p1;
+        G[t] = (X[t] + X[o]) * 0.5;
+    }
 }
 
 extern "C" int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, void *stream)
@@ -2846,7 +2953,7 @@ extern "C" int neorx_rxchfmult_gam3_int_dev(int64_t handle, double *d_int_gam3, 
                            nullptr, nullptr, X.d(), nullptr, nullptr);
         HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_mult_symm3, dim3(nblocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
+    hipLaunchKernelGGL(k_mult_symm3, dim3(grid_blocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
@@ -2886,9 +2993,9 @@ extern "C" int neorx_rxchfmult_gam3_ic1ex_dev(int64_t handle, double *d_int_gam3
                            Ys.d(), Yx.d(), X.d(), X3.d(), X4.d());
         HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_mult_symm3, dim3(nblocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
-    hipLaunchKernelGGL(k_mult_symm3x, dim3(nblocks(n3, 256)), dim3(256), 0, st, na, n3, 1, X3.d(), d_ex1);
-    hipLaunchKernelGGL(k_mult_symm3x, dim3(nblocks(n3, 256)), dim3(256), 0, st, na, n3, 2, X4.d(), d_ex2);
+    hipLaunchKernelGGL(k_mult_symm3, dim3(grid_blocks(n3, 256)), dim3(256), 0, st, nb, n3, X.d(), d_int_gam3);
+    hipLaunchKernelGGL(k_mult_symm3x, dim3(grid_blocks(n3, 256)), dim3(256), 0, st, na, n3, 1, X3.d(), d_ex1);
+    hipLaunchKernelGGL(k_mult_symm3x, dim3(grid_blocks(n3, 256)), dim3(256), 0, st, na, n3, 2, X4.d(), d_ex2);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
@@ -2941,18 +3048,7 @@ extern "C" int neorx_rxchfmult_gam3_xchf_dev(int64_t handle, double *d_gm3, void
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
     if (s->d.nebf1 != s->d.nebf) return fail(NEORX_E_ARG, "RXCHFmult_GAM3_XCHF takes a one-basis handle");
     if (!d_gm3) return fail(NEORX_E_ARG, "null output");
-    i64 ne = s->d.nebf, np = s->d.npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
-    hipStream_t st = (hipStream_t)stream;
-    i64 cap = slab_cap(s, 3);
-    DevBuf X, Y;
-    int rc = X.alloc(ng3 * 8);
-    if (!rc) rc = Y.alloc(cap * 8);
-    if (!rc) rc = build3_block<7>(s, 0, np * np, X.d(), Y.d(), st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_gam3_symm, dim3(nblocks(ng3, 256)), dim3(256), 0, st, ne, ng3, X.d(), d_gm3);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    return NEORX_OK;
+    return gam3_build_dev<7>(s, d_gm3, stream);
 }
 
 extern "C" int neorx_rxchfmult_gam3_xchf(int64_t handle, double *gm3)
@@ -3223,11 +3319,11 @@ extern "C" int neorx_rxchfmult_omg_dev(int64_t k, int64_t nebf, int64_t nebfbe, 
 __global__ void __launch_bounds__(256) k_omg2ex_combine(i64 ne, i64 n, const double *__restrict__ G,
                                                         const double *__restrict__ Gx, double *__restrict__ H)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = (t / (ne * ne * ne)) % ne,
-        pp = t / (ne * ne * ne * ne);
-    H[t] = G[t] - 0.5 * Gx[(((pp * ne + i1) * ne + j2) * ne + i2) * ne + j1];
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = (t / (ne * ne * ne)) % ne,
+            pp = t / (ne * ne * ne * ne);
+        H[t] = G[t] - 0.5 * Gx[(((pp * ne + i1) * ne + j2) * ne + i2) * ne + j1];
+    }
 }
 
 // thread_F*_OMG3ex (rxchfmult_omg3ex.f:101-216, :316-431, :531-663): per entry
@@ -3239,25 +3335,25 @@ __global__ void __launch_bounds__(256) k_omg3ex_combine(i64 ne, i64 n, const dou
                                                         const double *__restrict__ X1, const double *__restrict__ X2,
                                                         double *__restrict__ H, double *__restrict__ K)
 {
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    i64 r = t;
-    i64 j3 = r % ne; r /= ne;
-    i64 i3 = r % ne; r /= ne;
-    i64 j2 = r % ne; r /= ne;
-    i64 i2 = r % ne; r /= ne;
-    i64 j1 = r % ne; r /= ne;
-    i64 i1 = r % ne, pp = r / ne;
-    auto ix = [&](i64 a, i64 b, i64 c, i64 d, i64 e, i64 f) {
-        return (((((pp * ne + a) * ne + b) * ne + c) * ne + d) * ne + e) * ne + f;
-    };
-    double val = G[t] - 0.5 * G[ix(i1, j1, i2, j3, i3, j2)];
-    double v2 = X1[ix(i1, j2, i2, j1, i3, j3)] - 0.5 * X1[ix(i1, j3, i2, j1, i3, j2)] + X2[ix(i1, j3, i2, j2, i3, j1)] -
-                0.5 * X2[ix(i1, j2, i2, j3, i3, j1)];
-    double v3 = X1[ix(i1, j3, i3, j1, i2, j2)] - 0.5 * X1[ix(i1, j2, i3, j1, i2, j3)] + X2[ix(i1, j2, i3, j3, i2, j1)] -
-                0.5 * X2[ix(i1, j3, i3, j2, i2, j1)];
-    H[t] = val - 0.5 * v2;
-    K[t] = 2.0 * val - 0.5 * (v2 + v3);
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        i64 r = t;
+        i64 j3 = r % ne; r /= ne;
+        i64 i3 = r % ne; r /= ne;
+        i64 j2 = r % ne; r /= ne;
+        i64 i2 = r % ne; r /= ne;
+        i64 j1 = r % ne; r /= ne;
+        i64 i1 = r % ne, pp = r / ne;
+        auto ix = [&](i64 a, i64 b, i64 c, i64 d, i64 e, i64 f) {
+            return (((((pp * ne + a) * ne + b) * ne + c) * ne + d) * ne + e) * ne + f;
+        };
+        double val = G[t] - 0.5 * G[ix(i1, j1, i2, j3, i3, j2)];
+        double v2 = X1[ix(i1, j2, i2, j1, i3, j3)] - 0.5 * X1[ix(i1, j3, i2, j1, i3, j2)] + X2[ix(i1, j3, i2, j2, i3, j1)] -
+                    0.5 * X2[ix(i1, j2, i2, j3, i3, j1)];
+        double v3 = X1[ix(i1, j3, i3, j1, i2, j2)] - 0.5 * X1[ix(i1, j2, i3, j1, i2, j3)] + X2[ix(i1, j2, i3, j3, i2, j1)] -
+                    0.5 * X2[ix(i1, j3, i3, j2, i2, j1)];
+        H[t] = val - 0.5 * v2;
+        K[t] = 2.0 * val - 0.5 * (v2 + v3);
+    }
 }
 
 // order 2 or 3 (OMG2ex, OMG3ex), one basis; d_gx2 only for order 3
@@ -3279,9 +3375,9 @@ extern "C" int neorx_rxchfmult_omgex_dev(int64_t order, int64_t nebf, int64_t np
     if (rc) return rc;
     double *H = (double *)tb.p, *K = order == 3 ? H + n : H;
     if (order == 2)
-        hipLaunchKernelGGL(k_omg2ex_combine, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, H);
+        hipLaunchKernelGGL(k_omg2ex_combine, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, H);
     else
-        hipLaunchKernelGGL(k_omg3ex_combine, dim3(nblocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, d_gx2, H,
+        hipLaunchKernelGGL(k_omg3ex_combine, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, d_gm, d_gx1, d_gx2, H,
                            K);
     HIPCHK(hipGetLastError());
     return mult_omg_core(order == 2 ? 1 : 4, ne, ne, np, d_dae, d_dbe, d_dp, H, K, d_fae, d_fp, d_fbe, d_e, st);

@@ -13,6 +13,7 @@ import os as _os
 import numpy as _np
 
 from . import synth  # noqa: F401  (synthetic systems of the benchmark configurations)
+from . import xchf  # noqa: F401  (NEO-XCHF input files, orbital files and the device SCF)
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 LIBDIR = _os.path.join(_os.path.dirname(_HERE), "lib")
@@ -114,6 +115,14 @@ def load():
         _sig(f"neorx_rxchfmult_{n}_omg2ex", [_I64, _I64] + [_P] * 7)
         _sig(f"neorx_rxchfmult_{n}_omg3ex", [_I64, _I64] + [_P] * 8)
     _sig("neorx_rxchfmult_omgex_dev", [_I64] * 3 + [_P] * 10 + [_P])
+    _sig("neorx_autotune_eg3_dev", [_I64, _I64, _I64, _P, _P, _I64, _P, _P, _P])
+    _sig("neorx_xchf_create", [_I64, _I64, _I64, _P])
+    _sig("neorx_xchf_destroy", [_I64])
+    _sig("neorx_xchf_integrals", [_I64] + [_P] * 7)
+    _sig("neorx_xchf_fock_dev", [_I64, _P, _P, _I64, _P, _P, _P, _P])
+    _sig("neorx_roothan_dev", [_I64, _P, _P, _I64, _I64, _D, _P, _P, _P, _P])
+    _sig("neorx_gemm_f64_dev", [_I64, _I64, _I64, _P, _I64, _I64, _P, _I64, _I64, _P, _P])
+    _sig("neorx_xchf_scf", [_I64, _P, _I64, _I64, _D, _D, _I64, _I64, _D, _D] + [_P] * 8)
     return _lib
 
 
@@ -511,3 +520,12 @@ def RXCHFmult_OMGex(which, order, nebf, npbf, DAE, DBE, DP, GM, GX1, GX2, F):
         args.append(_p(_np.ascontiguousarray(GX2)))
     _check(getattr(_lib, name)(nebf, npbf, *args, _p(F), _p(e)))
     return float(e[0])
+
+
+def autotune_eg3_dev(nebf, npbf, dtype, d_de, d_gm3, reps=3):
+    """neorx_autotune_eg3_dev: (best loads-in-flight, its ms, ms of the 8, 12,
+    16 and 24 variants) for storage dtype 0 FP64, 1 FP32, 2 BF16."""
+    load()
+    b, ms, allms = _C.c_int64(), _C.c_double(), (_C.c_double * 4)()
+    _check(_lib.neorx_autotune_eg3_dev(nebf, npbf, dtype, d_de, d_gm3, reps, _C.byref(b), _C.byref(ms), allms))
+    return b.value, ms.value, list(allms)

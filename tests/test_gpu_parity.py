@@ -98,13 +98,49 @@ def test_gam1_gam2_builders(torch_dev):
     S.close()
 
 
-def test_gam3_builder(torch_dev):
-    s = synth.small(seed=72, n_e_centres=3, e_shells=(0,), contract=(1,), p_shells=(0, 1))
+@pytest.mark.parametrize("kw", [
+    dict(seed=72, n_e_centres=3, e_shells=(0,), contract=(1,), p_shells=(0, 1)),
+    dict(seed=74, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,)),
+    dict(seed=75, n_e_centres=1, e_shells=(2,), contract=(1,), p_shells=(1,)),
+], ids=["s-shells", "sp-contracted", "d-electrons-p-nucleus"])
+def test_gam3_builder(torch_dev, kw):
+    """GAM3_IC1: primitive tuples, contraction over primitives of length 1-2
+    and the six-permutation average, s/p/d electrons."""
+    s = synth.small(**kw)
+    print(f"case gam3 builder nebf={s['nebf']} npebf={s['npebf']} npbf={s['npbf']}", flush=True)
     S = neorx.System(s)
     g3 = S.GAM3_IC1()
     o3 = oracle.gam3_ic1(s)
     check_close(g3, o3, what="GM3ICR")
     S.close()
+
+
+def test_builders_grid_stride(torch_dev):
+    """The full-tensor symmetrisation kernels stride over their entries when
+    the grid is capped: GAM2_IC1 and GAM3_IC1 with a one-block grid still
+    equal the oracle."""
+    import os
+
+    s = synth.small(seed=76, n_e_centres=1, e_shells=(0, 1), contract=(1,), p_shells=(0, 1))
+    ne, npb = s["nebf"], s["npbf"]
+    print(f"case grid cap nebf={ne} npbf={npb}", flush=True)
+    # one 256-lane block: GM2 (ne^4 npbf^2 entries) and each (ip,jp) block of
+    # GM3 (ne^6 entries) take several passes of the stride loop
+    assert ne ** 4 * npb ** 2 >= 8 * 256 and ne ** 6 >= 8 * 256
+    os.environ["NEORX_GRID_CAP"] = "1"
+    try:
+        neorx.init(0)
+        S = neorx.System(s)
+        g2, g2s = S.GAM2_IC1()
+        g3 = S.GAM3_IC1()
+        S.close()
+    finally:
+        del os.environ["NEORX_GRID_CAP"]
+        neorx.init(0)
+    o2, o2s = oracle.gam2_ic1(s)
+    check_close(g2, o2, what="GM2ICR (capped grid)")
+    check_close(g2s, o2s, what="GM2sICR (capped grid)")
+    check_close(g3, oracle.gam3_ic1(s), what="GM3ICR (capped grid)")
 
 
 def test_gam_ee_and_gam4(torch_dev):
@@ -269,6 +305,44 @@ def test_mixed_precision_contractions(torch_dev, ne, npb):
         check_close(fe.cpu().numpy(), FE.ravel(order="F"), what="EG4 f32 FE")
         check_close(fp.cpu().numpy(), FP.ravel(order="F"), what="EG4 f32 FP")
         assert float(e.cpu()[0]) == pytest.approx(E4, rel=1e-11)
+
+
+@pytest.mark.parametrize("ne,npb", [(16, 1), (24, 1), (25, 1), (16, 2)])
+def test_eg3_wide_slabs(torch_dev, ne, npb):
+    """EG3IC1 at basis sizes whose slab loop runs the 16-wide (FP64) and
+    12-wide (FP32) load blocks, the 8-wide block and the scalar remainder:
+    nebf 16 = 16 (FP64) / 12+4 (FP32), 24 = 16+8 / 12+12, 25 = 16+8+1 /
+    12+12+1.  FP64 and FP32 storage against the oracle, and two runs of the
+    same call bitwise equal (fixed-order reduction)."""
+    torch = torch_dev
+    print(f"case eg3 wide ne={ne} np={npb}", flush=True)
+    rng = np.random.default_rng(100 + ne + npb)
+    DE, DP = synth.density(ne, 10.0, ne), synth.density(npb, 1.0, 3)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    d_de, d_dp = _dev(torch, DE.ravel(order="F")), _dev(torch, DP.ravel(order="F"))
+    d_g3 = _dev(torch, g3)
+    d_g3f = torch.empty(g3.size, dtype=torch.float32, device="cuda")
+    neorx.call_dev("neorx_convert_f32_dev", g3.size, d_g3.data_ptr(), d_g3f.data_ptr(), 0)
+    torch.cuda.synchronize()
+    FE0 = rng.standard_normal((ne, ne))
+    for name, gp, gh in (("neorx_eg3ic1_dev", d_g3, g3),
+                         ("neorx_eg3ic1_f32_dev", d_g3f, g3.astype(np.float32).astype(np.float64))):
+        outs = []
+        for rep in range(2):
+            fe = _dev(torch, FE0.ravel(order="F"))
+            fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+            e = torch.zeros(1, dtype=torch.float64, device="cuda")
+            neorx.call_dev(name, ne, npb, d_de.data_ptr(), d_dp.data_ptr(), gp.data_ptr(), fe.data_ptr(),
+                           fp.data_ptr(), e.data_ptr(), 0)
+            torch.cuda.synchronize()
+            outs.append((fe.cpu().numpy(), fp.cpu().numpy(), e.cpu().numpy()))
+        for a, b in zip(outs[0], outs[1]):
+            assert np.array_equal(a, b), f"{name}: two runs differ"
+        FE, FP = np.asfortranarray(FE0.copy()), np.zeros((npb, npb), order="F")
+        E = oracle.eg3ic1(ne, npb, DE, DP, gh, FE, FP)
+        check_close(outs[0][0], FE.ravel(order="F"), what=f"{name} FE")
+        check_close(outs[0][1], FP.ravel(order="F"), what=f"{name} FP")
+        assert outs[0][2][0] == pytest.approx(E, rel=1e-11)
 
 
 def test_precision_switch(torch_dev):

@@ -1,0 +1,217 @@
+"""GPU parity of the device-resident NEO-XCHF RHF path (nrx_scf.hip) through
+the C ABI: the SCF's integrals, the Fock build of xchf1_fock with the
+factorized 5-particle term, ROOTHAN, and whole SCF runs against the energies
+of the reference's own xcscf (tests/golden/xchf_scf.json, made by
+tests/golden/make_xchf_scf.py from the reference's sources)."""
+import numpy as np
+import pytest
+
+import neorx
+import oracle
+from neorx import synth, xchf
+from _util import check_close, golden_scf, golden_system
+
+pytestmark = pytest.mark.gpu
+
+# the golden values are printed with 10 decimals
+PRINT_TOL = 1.5e-10
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available()
+    torch.cuda.init()
+    arch = neorx.init(0)
+    assert arch.startswith("gfx950"), arch
+    return torch
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _small(nelec, **kw):
+    s = synth.small(**kw)
+    s["nelec"] = nelec
+    return s
+
+
+def test_xchf_integrals(torch_dev):
+    """class_nuc_rep, elec/nuc_ovlap, calc_GAM_epcore, calc_GAM_ep, calc_GAM_ee
+    on the device against the oracle (p and d functions on both particles)."""
+    s = _small(4, seed=81, n_e_centres=2, e_shells=(0, 1, 2), contract=(1, 2), p_shells=(0, 1, 2))
+    print(f"case xchf integrals nebf={s['nebf']} npbf={s['npbf']}", flush=True)
+    S = neorx.System(s)
+    X = xchf.XCHF(S, 2)
+    enuc, Se, Sp, He, Hp, Gep, Gee = X.integrals()
+    oHe, oSe, oHp, oSp = oracle.one_body(s)
+    assert enuc == pytest.approx(oracle.enuc(s), rel=1e-14)
+    for g, o, nm in ((Se, oSe, "S_e"), (Sp, oSp, "S_p"), (He, oHe, "GAM_ecore"), (Hp, oHp, "GAM_pcore"),
+                     (Gep, oracle.gam_ep(s), "GAM_ep"), (Gee, oracle.gam_ee(s), "GAM_ee")):
+        check_close(g, o, what=nm)
+    X.close()
+    S.close()
+
+
+@pytest.mark.parametrize("kw,sym", [
+    (dict(seed=82, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,)), True),
+    (dict(seed=83, n_e_centres=3, e_shells=(0,), contract=(1, 2), p_shells=(0, 1)), False),
+], ids=["sp-symmetric-D", "s-nonsymmetric-D"])
+def test_xchf_fock(torch_dev, kw, sym):
+    """One Fock build (xchf1_fock + XCfock_correction) against the oracle's
+    restatement, which contracts the in-core GAM_4 of GAM4_ICR as EG4ICR does:
+    checks the factorized 5-particle term, the shared-DE phase split and the
+    assembly.  Also with a non-symmetric DE (the factorization does not
+    assume symmetry)."""
+    torch = torch_dev
+    s = _small(4, **kw)
+    ne, npb = s["nebf"], s["npbf"]
+    print(f"case xchf fock nebf={ne} npbf={npb} sym={sym}", flush=True)
+    rng = np.random.default_rng(ne)
+    DE = synth.density(ne, 4.0, 3)
+    if not sym:
+        DE = np.asfortranarray(DE + 0.05 * rng.standard_normal((ne, ne)))
+    DP = synth.density(npb, 1.0, 4)
+    ints = oracle.xchf_ints(s)
+    oFE, oFP, oc = oracle.xchf_fock(ne, npb, 4, ints, DE, DP)
+    S = neorx.System(s)
+    X = xchf.XCHF(S, 4, store=("f64", "f32", "bf16"))
+    d_de, d_dp = _dev(torch, DE.ravel(order="F")), _dev(torch, DP.ravel(order="F"))
+    fe = torch.zeros(ne * ne, dtype=torch.float64, device="cuda")
+    fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+    c = torch.zeros(14, dtype=torch.float64, device="cuda")
+    X.fock_dev(d_de.data_ptr(), d_dp.data_ptr(), "f64", fe.data_ptr(), fp.data_ptr(), c.data_ptr())
+    torch.cuda.synchronize()
+    gc = c.cpu().numpy()
+    for k, nm in enumerate(xchf.COMPS):
+        print(f"  {nm:9s} gpu {gc[k]: .15e} oracle {oc[k]: .15e} diff {gc[k] - oc[k]: .3e}")
+    for k, nm in enumerate(xchf.COMPS):
+        assert gc[k] == pytest.approx(oc[k], rel=1e-11, abs=1e-12 * abs(oc[0])), nm
+    check_close(fe.cpu().numpy(), oFE.ravel(order="F"), what="FE")
+    check_close(fp.cpu().numpy(), oFP.ravel(order="F"), what="FP")
+    # reduced-precision storage of GAM_3 moves only E_GAM3 and the matrices,
+    # by the storage rounding
+    for prec, tol in (("f32", 1e-6), ("bf16", 1e-2)):
+        X.fock_dev(d_de.data_ptr(), d_dp.data_ptr(), prec, fe.data_ptr(), fp.data_ptr(), c.data_ptr())
+        torch.cuda.synchronize()
+        gl = c.cpu().numpy()
+        assert abs(gl[8] - oc[8]) <= tol * abs(oc[8]), (prec, gl[8], oc[8])
+        assert gl[7] == pytest.approx(oc[7], rel=1e-11) and gl[9] == pytest.approx(oc[9], rel=1e-11)
+    X.close()
+    S.close()
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(16, 16, 4, False, False), (24, 13824, 24, False, False),
+                                         (24, 24, 13824, False, True), (5, 37, 9, True, False),
+                                         (33, 7, 70, True, True)])
+def test_gemm_f64(torch_dev, M, N, K, ta, tb):
+    """The FP64 MFMA GEMM (v_mfma_f64_16x16x4_f64 lane layout, edge tiles,
+    strided operands, split-K) against numpy."""
+    torch = torch_dev
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    As = np.ascontiguousarray(A.T) if ta else A
+    Bs = np.ascontiguousarray(B.T) if tb else B
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    dA, dB = _dev(torch, As.ravel()), _dev(torch, Bs.ravel())
+    dC = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+    neorx.call_dev("neorx_gemm_f64_dev", M, N, K, dA.data_ptr(), sam, sak, dB.data_ptr(), sbk, sbn, dC.data_ptr(), 0)
+    torch.cuda.synchronize()
+    ref = A @ B
+    got = dC.cpu().numpy().reshape(M, N)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * K * np.max(np.abs(ref)), np.max(np.abs(got - ref))
+
+
+@pytest.mark.parametrize("n", [1, 7, 24, 70])
+def test_roothan(torch_dev, n):
+    """ROOTHAN on the device (Jacobi, LDS for n <= 63, global memory above)
+    against the oracle's restatement: eigenvalues, the density, and the
+    orbitals up to their sign; F C = S C e and C^T S C = 1."""
+    torch = torch_dev
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    Sm = np.asfortranarray(A @ A.T / n + np.eye(n))
+    Fm = rng.standard_normal((n, n))
+    Fm = np.asfortranarray(Fm + Fm.T)
+    k1, k2 = (1, max(1, n // 3)) if n > 1 else (1, 1)
+    oC, oev, oD = oracle.roothan(Sm, Fm, k1, k2, 2.0)
+    d = [torch.zeros(m, dtype=torch.float64, device="cuda") for m in (n * n, n, n * n)]
+    dS, dF = _dev(torch, Sm.ravel(order="F")), _dev(torch, Fm.ravel(order="F"))  # alive until synchronised
+    xchf.roothan_dev(n, dS.data_ptr(), dF.data_ptr(), k1, k2, 2.0, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+    torch.cuda.synchronize()
+    C, ev, D = (x.cpu().numpy() for x in d)
+    C = C.reshape(n, n, order="F")
+    check_close(ev, oev, rtol=1e-12, atol_rel=1e-13, what="eigenvalues")
+    check_close(D, oD.ravel(order="F"), rtol=1e-10, atol_rel=1e-12, what="density")
+    sgn = np.sign(np.sum(C * oC, axis=0))
+    check_close(C * sgn, oC, rtol=1e-9, atol_rel=1e-11, what="orbitals")
+    assert np.max(np.abs(C.T @ Sm @ C - np.eye(n))) < 1e-12
+    assert np.max(np.abs(Fm @ C - Sm @ C * ev)) < 1e-11 * np.max(np.abs(Fm))
+
+
+def _run(name, **kw):
+    g = golden_scf()[name]
+    s, guess = golden_system(g)
+    S = neorx.System(s)
+    X = xchf.XCHF(S, s["nelec"], store=kw.pop("store", ("f64",)))
+    r = X.scf(ce_guess=guess, nucst=s.get("nucst", 1), **kw)
+    X.close()
+    S.close()
+    return g, r
+
+
+def _check_golden(g, r, skip=()):
+    E = r["comps"]["E_TOTAL"]
+    print(f"  E = {E:.12f} (reference {g['energy']:.10f}) in {r['niter']} iterations", flush=True)
+    assert abs(E - g["energy"]) <= 1e-8, (E, g["energy"])
+    for k, v in g["initial"].items():
+        if k not in skip:
+            assert abs(r["comps_init"][k] - v) <= PRINT_TOL + 1e-12 * abs(v), (k, r["comps_init"][k], v)
+
+
+@pytest.mark.parametrize("graph,diis", [(True, False), (True, True), (False, True)])
+def test_scf_example(torch_dev, graph, diis):
+    """examples/xchf of the reference (He2 with a quantum proton, guess orbitals
+    from guessCE.inp): the initial-guess energy components and the final
+    energy -5.7800665094 of examples/xchf/answer/XCHF.log:335, FP64."""
+    print(f"case scf example graph={graph} diis={diis}", flush=True)
+    g, r = _run("example", graph=graph, diis=diis)
+    _check_golden(g, r)
+
+
+@pytest.mark.parametrize("name", ["c1mini_sp", "c1mini_d", "tiny_s", "tiny_pnuc"])
+def test_scf_golden_minis(torch_dev, name):
+    """Small systems in the C1 geometry (s/p/d electrons, s/p nuclear
+    functions), core guess: the reference xcscf's initial components and
+    final energy."""
+    print(f"case scf {name}", flush=True)
+    g, r = _run(name, diis=True)
+    _check_golden(g, r)
+
+
+@pytest.mark.parametrize("mode", ["f32", "bf16"])
+def test_scf_mixed_precision(torch_dev, mode):
+    """Mixed precision: GAM_3 read in FP32 (or BF16, then FP32) until the DIIS
+    error falls below 1e-5, then FP64; the final energy is the FP64 one."""
+    print(f"case scf mixed {mode}", flush=True)
+    g, r = _run("c1mini_sp", mode=mode, diis=True, store=("f64", "f32", "bf16"))
+    prec = r["trace"][:, 5]
+    assert prec[0] == {"f32": 1, "bf16": 2}[mode] and prec[-1] == 0, prec
+    # the first build reads GAM_3 in the reduced storage: E_GAM3 (and the
+    # total) of the initial guess move by the storage rounding, the rest not
+    tol = {"f32": 1e-6, "bf16": 1e-2}[mode]
+    assert abs(r["comps_init"]["E_GAM3"] - g["initial"]["E_GAM3"]) <= tol * abs(g["initial"]["E_GAM3"]) + PRINT_TOL
+    _check_golden(g, r, skip=("E_GAM3", "E_TOTAL"))
+
+
+def test_scf_graph_matches_eager(torch_dev):
+    """The hipGraph replay of an iteration and the eager launches give the
+    same energies bit for bit (every reduction has a fixed order)."""
+    _, a = _run("tiny_pnuc", graph=True, diis=True)
+    _, b = _run("tiny_pnuc", graph=False, diis=True)
+    assert a["niter"] == b["niter"]
+    assert np.array_equal(a["trace"][:, :5], b["trace"][:, :5])
