@@ -162,6 +162,63 @@ def fock_bench(np, torch, neorx, synth, ne, npb, reps):
     }
 
 
+def scf_bench(np, torch, neorx, synth, maxit=80):
+    """NEO-XCHF RHF SCF of the C1 system (neorx.synth.config1: C/N classical,
+    quantum proton, 24 electronic functions from 36 s/p primitives, 8 nuclear
+    s/p/d primitives, 3-term geminal, 14 electrons) on the device: every
+    integral built on the GPU (GM3 = 1.22e10 entries in FP64, FP32 and BF16
+    copies), then the SCF in FP64 and in mixed precision (GAM_3 read in BF16
+    until the DIIS error < 1e-3, then FP32 until < 1e-5, then FP64), each
+    iteration a captured hipGraph.  SCF s/iter = wall time per iteration from
+    the host's clock (graph launch to the read-back of the convergence data)."""
+    from neorx import xchf
+
+    sysd = synth.config1()
+    S = neorx.System(sysd)
+    t0 = time.perf_counter()
+    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "f32", "bf16"))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    ne, npe, npb = sysd["nebf"], sysd["npebf"], sysd["npbf"]
+    prims = npe ** 6 * npb ** 2 + npe ** 4 * npb ** 2 + npe ** 2 * npb ** 2
+    print(f"# C1 integrals built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    runs = {}
+    for mode in ("f64", "bf16", "f32"):
+        t0 = time.perf_counter()
+        r = X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-5)
+        wall = time.perf_counter() - t0
+        tr = r["trace"]
+        per = {}
+        for code, nm in ((0, "f64"), (1, "f32"), (2, "bf16")):
+            m = tr[:, 5] == code
+            if m.any():
+                # the first iteration of each storage type includes its graph capture
+                ms = tr[m, 6][1:] if m.sum() > 1 else tr[m, 6]
+                per[nm] = {"iterations": int(m.sum()), "ms_per_iter": float(np.median(ms))}
+        runs[mode] = {"energy": r["comps"]["E_TOTAL"], "iterations": r["niter"], "wall_s": wall,
+                      "ms_per_iter_all": float(tr[:, 6].sum() / len(tr)), "by_storage": per}
+        print(f"# SCF {mode}: E={r['comps']['E_TOTAL']:.10f} it={r['niter']} "
+              + " ".join(f"{k}:{v['iterations']}x{v['ms_per_iter']:.2f}ms" for k, v in per.items()),
+              file=sys.stderr, flush=True)
+    X.close()
+    S.close()
+    f64 = runs["f64"]["by_storage"]["f64"]["ms_per_iter"]
+    out = {
+        "system": "C1 NEO-XCHF RHF: nebf 24 (36 s/p prims), npbf 8 (s/p/d), 3 geminals, 14 electrons",
+        "integral_build_s": t_build, "primitive_tuples_built": prims,
+        "scf_s_per_iter_fp64": f64 / 1e3,
+        "runs": runs,
+    }
+    for mode in ("bf16", "f32"):
+        lo = runs[mode]["by_storage"].get(mode)
+        if lo:
+            out[f"scf_s_per_iter_{mode}_phase"] = lo["ms_per_iter"] / 1e3
+            out[f"speedup_{mode}_phase_vs_fp64"] = f64 / lo["ms_per_iter"]
+        out[f"dE_{mode}_vs_fp64_Eh"] = abs(runs[mode]["energy"] - runs["f64"]["energy"])
+        out[f"scf_wall_ratio_fp64_over_{mode}"] = runs["f64"]["wall_s"] / runs[mode]["wall_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +229,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fock", type=str, default="24,8",
                     help="nebf,npbf of the Fock-build contraction timing (FP64 vs mixed); 'off' to skip")
+    ap.add_argument("--scf", type=str, default="off", choices=("off", "c1"),
+                    help="SCF s/iter of the C1 system, FP64 vs mixed precision (builds C1's integrals first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -258,7 +317,7 @@ def main():
     if flops_per:
         ntup = n1 if dom == 1 else ns
         ach = flops_per * ntup / (np.mean(per_class_ms[dom]) / 1e3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64_valu", "achieved": round(ach, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP64_PEAK_TFLOPS, 5),
                 "traffic": TRAFFIC.get(dom) if args.nsample == (1 << 20) else None,
                 "kernel": {1: "k_prim_bin + k_prim_scatter + k_prim_tiers<1,SRC_SORTED>",
@@ -271,6 +330,11 @@ def main():
         fne, fnp = (int(x) for x in args.fock.split(","))
         print(f"# Fock-build contractions nebf={fne} npbf={fnp}", file=sys.stderr, flush=True)
         fock = fock_bench(np, torch, neorx, synth, fne, fnp, 3)
+
+    scf = None
+    if args.scf == "c1":
+        print("# C1 SCF (FP64 vs mixed precision)", file=sys.stderr, flush=True)
+        scf = scf_bench(np, torch, neorx, synth)
 
     cpu = None
     if not args.no_cpu:
@@ -291,7 +355,7 @@ def main():
                                "geminal; all e-p tuples + seeded samples of e-e-p and e-e-e-p tuples",
                    "e_p_tuples": n1, "e_e_p_tuples": ns, "e_e_e_p_tuples": ns, "device": arch},
         "per_class_integrals_per_s": {"e-p": rates[1], "e-e-p": rates[2], "e-e-e-p": rates[3]},
-        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock,
+        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock, "scf": scf,
     }
     print(json.dumps(out))
 
