@@ -32,6 +32,9 @@ NRX_HIDDEN int fail(int code, const char *fmt, ...);
         if (_e != hipSuccess) return fail(NEORX_E_HIP, "%s failed: %s", #x, hipGetErrorString(_e));              \
     } while (0)
 
+// NEORX_VERBOSE > 0: long builds report their progress on stderr
+NRX_HIDDEN extern int g_verbose;
+
 // neorx_init(0) unless a device is already selected
 NRX_HIDDEN int ensure_init();
 

@@ -9,6 +9,7 @@
 //     wavefront share loop trip counts;
 //   * contracted tensors use the reference packings (C row-major
 //     [ip][jp][ie1][je1]...).
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -25,6 +26,7 @@ static char g_arch[64] = "";
 static double *g_boys_dev = nullptr; // Boys table in HBM, one per process
 
 i64 g_grid_cap = (i64)1 << 20;       // most blocks of a grid-stride launch
+int g_verbose = 0;                   // NEORX_VERBOSE: progress of long builds on stderr
 
 int fail(int code, const char *fmt, ...)
 {
@@ -86,6 +88,7 @@ extern "C" int neorx_init(int64_t device)
         HIPCHK(hipMalloc(&g_boys_dev, tab.size() * sizeof(double)));
         HIPCHK(hipMemcpy(g_boys_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    g_verbose = getenv("NEORX_VERBOSE") ? atoi(getenv("NEORX_VERBOSE")) : 0;
     g_grid_cap = (i64)1 << 20;
     if (const char *cap = getenv("NEORX_GRID_CAP")) {
         long long v = atoll(cap);
@@ -968,11 +971,18 @@ template <int CLS> static int gam3_build_dev(System *s, double *d_gm3, void *str
     int rc = X.alloc(blk * 8);
     if (!rc) rc = Y.alloc(cap * 8);
     if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
     for (i64 b = 0; b < np * np; ++b) {
         rc = build3_block<CLS>(s, b, b + 1, X.d(), Y.d(), st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gam3_symm, dim3(grid_blocks(blk, 256)), dim3(256), 0, st, ne, blk, X.d(), d_gm3 + b * blk);
         HIPCHK(hipGetLastError());
+        if (g_verbose && ((b + 1) % 8 == 0 || b + 1 == np * np)) {
+            HIPCHK(hipStreamSynchronize(st));
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fprintf(stderr, "neorx: e-e-e-p build, %lld of %lld (ip,jp) blocks in %.1f s\n", (long long)(b + 1),
+                    (long long)(np * np), el);
+        }
     }
     HIPCHK(hipStreamSynchronize(st));
     return NEORX_OK;
