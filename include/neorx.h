@@ -112,6 +112,18 @@ int neorx_eg2ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const doubl
 int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm3,
                      double *d_fe, double *d_fp, double *d_e, void *stream);
 
+/* neorx_eg3ic1_dev without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a static device array of 2^21 doubles (16 MiB per
+ * device, held until the process ends), so nebf and npbf are limited to
+ * npbf^2*nebf*(nebf+1) + npbf^2 <= 2^21 (NEORX_E_ARG beyond).  Same results
+ * as neorx_eg3ic1_dev; enqueued on the stream and not synchronised.  The
+ * array is shared by every call on a device: calls that overlap on one device
+ * race.  The device must have been set up by neorx_init. */
+int neorx_eg3ic1_noalloc_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                             const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream);
+
 /* EG4ICR on device arrays. */
 int neorx_eg4icr_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const double *d_gm4,
                      double *d_fe, double *d_fp, double *d_e, void *stream);

@@ -25,6 +25,17 @@ static int g_device = -1;
 static char g_arch[64] = "";
 static double *g_boys_dev = nullptr; // Boys table in HBM, one per process
 
+// Scratch of the allocation-free entry point neorx_eg3ic1_noalloc_dev: a
+// file-scope device array of SCRATCH_DOUBLES doubles (16 MiB on every device
+// the library runs on, held until the process ends).  In host code the
+// array's name is not a device address: neorx_init looks its address up once
+// per device (hipGetSymbolAddress, with that device current) and every host
+// use goes through g_scratch[device].
+static const i64 SCRATCH_DOUBLES = (i64)1 << 21;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+static const int MAX_DEVICES = 64;
+static double *g_scratch[MAX_DEVICES] = {};
+
 i64 g_grid_cap = (i64)1 << 20;       // most blocks of a grid-stride launch
 int g_verbose = 0;                   // NEORX_VERBOSE: progress of long builds on stderr
 
@@ -88,6 +99,13 @@ extern "C" int neorx_init(int64_t device)
         HIPCHK(hipMalloc(&g_boys_dev, tab.size() * sizeof(double)));
         HIPCHK(hipMemcpy(g_boys_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    if (device < MAX_DEVICES && !g_scratch[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NEORX_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch[device] = (double *)p;
+    }
     g_verbose = getenv("NEORX_VERBOSE") ? atoi(getenv("NEORX_VERBOSE")) : 0;
     g_grid_cap = (i64)1 << 20;
     if (const char *cap = getenv("NEORX_GRID_CAP")) {
@@ -109,6 +127,7 @@ extern "C" int neorx_finalize(void)
     g_sys.clear();
     if (g_boys_dev) (void)hipFree(g_boys_dev);
     g_boys_dev = nullptr;
+    std::fill(g_scratch, g_scratch + MAX_DEVICES, nullptr); // looked up again by the next neorx_init
     g_device = -1;
     return NEORX_OK;
 }
@@ -1597,6 +1616,47 @@ extern "C" int neorx_eg3ic1_dev(int64_t nebf, int64_t npbf, const double *d_de, 
                                 const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
 {
     return eg3_dev<double>(nebf, npbf, d_de, d_dp, d_gm3, d_fe, d_fp, d_e, stream);
+}
+
+// EG3IC1 without any allocation: the slab partials and the nuclear partial
+// sums live in the current device's scratch array.  Every scratch element the
+// kernels read is written earlier in the same call, so nothing is cleared and
+// the previous call's contents never reach the results.  Enqueued on the
+// stream and not synchronised (the existing entry point synchronises because
+// it frees its buffers).  Calls that overlap on one device share the array
+// and race.
+extern "C" int neorx_eg3ic1_noalloc_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,
+                                        const double *d_gm3, double *d_fe, double *d_fp, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (ne > EG3_MAX_NE) return fail(NEORX_E_UNSUPPORTED, "EG3IC1: nebf=%lld > %lld", (long long)ne, (long long)EG3_MAX_NE);
+    if (!d_de || !d_dp || !d_gm3 || !d_fe || !d_fp || !d_e) return fail(NEORX_E_ARG, "null device pointer");
+    const i64 nblk = np * np * ne;
+    if (np > SCRATCH_DOUBLES || nblk > SCRATCH_DOUBLES / (ne + 1) ||
+        nblk * (ne + 1) + np * np > SCRATCH_DOUBLES)
+        return fail(NEORX_E_ARG,
+                    "neorx_eg3ic1_noalloc_dev: nebf=%lld npbf=%lld needs npbf^2*nebf*(nebf+1)+npbf^2 scratch doubles, "
+                    "more than the %lld the library holds",
+                    (long long)ne, (long long)np, (long long)SCRATCH_DOUBLES);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES || !g_scratch[dev])
+        return fail(NEORX_E_NODEVICE, "neorx_eg3ic1_noalloc_dev: device %d was not set up by neorx_init", dev);
+    double *part = g_scratch[dev], *fpd = part + nblk * (ne + 1);
+    hipStream_t st = (hipStream_t)stream;
+    rc = nrx_eg3_slabs(ne, nblk, d_de, d_gm3, NRX_F64, part, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part,
+                       d_fe, fpd);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, st, np, d_dp, fpd, nullptr, d_e, nullptr);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(np * np, 256)), dim3(256), 0, st, np * np, fpd, d_fp);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
 }
 
 extern "C" int neorx_eg3ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp,

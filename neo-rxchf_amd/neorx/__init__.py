@@ -67,6 +67,7 @@ def load():
     _sig("neorx_eg4icr", [_I64, _I64] + [_P] * 6)
     _sig("neorx_eg2ic1_dev", [_I64, _I64] + [_P] * 10 + [_P])
     _sig("neorx_eg3ic1_dev", [_I64, _I64] + [_P] * 6 + [_P])
+    _sig("neorx_eg3ic1_noalloc_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_eg4icr_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_convert_f32_dev", [_I64, _P, _P, _P])
     _sig("neorx_eg2ic1_f32_dev", [_I64, _I64] + [_P] * 10 + [_P])
