@@ -1122,62 +1122,119 @@ extern "C" int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, 
 // ------------------------------------------------------------------
 // Fock contractions
 // ------------------------------------------------------------------
+// sum over the 64 lanes of a wavefront (every lane gets the total)
+NRX_DI double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // EG2IC1.  Block (ip,jp,a) reads the ne^3 slab G[ip][jp][a][b][c][d] once and
 // forms dir[b] = sum_cd DE(c,d) G[b][c][d] and ex[d] = sum_bc DE(c,b) G[b][c][d]
 // (the exchange read G(a,j2,i2,j1) of thread_EG2ICR re-indexed onto the same
-// slab), for G and Gs.  Partial vectors go to a scratch array that a second
-// kernel reduces in a fixed order (deterministic).
-template <int NT, class T>
-__global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, i64 np, const double *__restrict__ DE,
-                                                 const T *__restrict__ G, const T *__restrict__ Gs,
-                                                 double *__restrict__ part)
+// slab), for G and Gs.  Lane t owns the (c,d) columns cd = t + k NT (k < K)
+// for every b: its ex partials stay in registers and meet in LDS once, dir[b]
+// is summed over the wave by shuffles and over the waves in a fixed order, so
+// no two runs differ in rounding.  Partial vectors [dir, ex, dirS, exS] go to
+// a scratch array that k_eg2_reduce sums in a fixed order.
+template <int NT, int K, class T>
+__global__ void __launch_bounds__(NT) k_eg2_slab(i64 ne, const double *__restrict__ DE, const T *__restrict__ G,
+                                                 const T *__restrict__ Gs, double *__restrict__ part)
 {
-    extern __shared__ double sm[]; // [4][ne]: dir, ex, dirS, exS
-    i64 blk = blockIdx.x; // (ip*np + jp)*ne + a
-    i64 ne3 = ne * ne * ne;
+    constexpr int NW = NT / 64;
+    extern __shared__ double sm[]; // wd[NW][2 ne] (dir, dirS per wave), xc[2][ne^2] (ex column partials)
+    const i64 blk = blockIdx.x, ne2 = ne * ne, ne3 = ne2 * ne;
+    const int w = threadIdx.x >> 6;
+    const bool lead = (threadIdx.x & 63) == 0;
     const T *g = G + blk * ne3;
     const T *gs = Gs ? Gs + blk * ne3 : nullptr;
-    for (int k = threadIdx.x; k < 4 * ne; k += NT) sm[k] = 0.0;
-    __syncthreads();
-    // each lane owns one b (row) at a time and strides over (c,d)
+    double *wd = sm, *xc = sm + 2 * NW * ne;
+    double dcd[K], xa[K], xs[K];
+    int cc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const i64 cd = threadIdx.x + (i64)k * NT;
+        const bool on = cd < ne2;
+        cc[k] = on ? (int)(cd / ne) : 0;
+        dcd[k] = on ? DE[(cd % ne) * ne + cc[k]] : 0.0; // DE(c,d), column-major
+        xa[k] = xs[k] = 0.0;
+    }
     for (i64 b = 0; b < ne; ++b) {
-        double dsum = 0.0, dsums = 0.0;
-        const T *row = g + b * ne * ne;
-        const T *rows = gs ? gs + b * ne * ne : nullptr;
-        for (i64 cd = threadIdx.x; cd < ne * ne; cd += NT) {
-            i64 c = cd / ne, d = cd % ne;
-            double gv = (double)row[cd];
-            double de_cd = DE[d * ne + c]; // DE(c,d), column-major
-            double de_cb = DE[b * ne + c]; // DE(c,b)
-            dsum += de_cd * gv;
-            atomicAdd(&sm[ne + d], de_cb * gv);
-            if (rows) {
-                double gsv = (double)rows[cd];
-                dsums += de_cd * gsv;
-                atomicAdd(&sm[3 * ne + d], de_cb * gsv);
+        const T *row = g + b * ne2;
+        const T *rows = gs ? gs + b * ne2 : nullptr;
+        const double *deb = DE + b * ne; // DE(c,b) = deb[c]
+        double ds = 0.0, dss = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const i64 cd = threadIdx.x + (i64)k * NT;
+            if (cd < ne2) {
+                const double gv = (double)row[cd], de_cb = deb[cc[k]];
+                ds += dcd[k] * gv;
+                xa[k] += de_cb * gv;
+                if (rows) {
+                    const double gsv = (double)rows[cd];
+                    dss += dcd[k] * gsv;
+                    xs[k] += de_cb * gsv;
+                }
             }
         }
-        // block reduce dsum -> sm[b]
-        for (int o = 32; o > 0; o >>= 1) {
-            dsum += __shfl_down(dsum, o);
-            dsums += __shfl_down(dsums, o);
+        ds = wave_sum(ds);
+        dss = wave_sum(dss);
+        if (lead) {
+            wd[w * 2 * ne + b] = ds;
+            wd[w * 2 * ne + ne + b] = dss;
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&sm[b], dsum);
-            atomicAdd(&sm[2 * ne + b], dsums);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const i64 cd = threadIdx.x + (i64)k * NT;
+        if (cd < ne2) {
+            xc[cd] = xa[k];
+            xc[ne2 + cd] = xs[k];
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < 4 * ne; k += NT) part[blk * 4 * ne + k] = sm[k];
+    for (i64 t = threadIdx.x; t < 4 * ne; t += NT) {
+        const i64 q = t / ne, x = t % ne;
+        double v = 0.0;
+        if (q == 0 || q == 2) {
+            for (int u = 0; u < NW; ++u) v += wd[u * 2 * ne + (q == 2 ? ne : 0) + x];
+        } else {
+            const double *col = xc + (q == 3 ? ne2 : 0);
+            for (i64 c = 0; c < ne; ++c) v += col[c * ne + x];
+        }
+        part[blk * 4 * ne + t] = v;
+    }
+}
+
+// largest nebf of the EG2 slab kernel: ne^2 columns over 1024 lanes x 8
+static const i64 EG2_MAX_NE = 90;
+
+// k_eg2_slab over nblk slabs: the smallest (lanes, columns per lane) that
+// holds ne^2 columns
+template <class T>
+static int eg2_slab_launch(i64 ne, i64 nblk, const double *d_de, const T *g, const T *gs, double *part,
+                           hipStream_t st)
+{
+    if (ne <= 0 || ne > EG2_MAX_NE)
+        return fail(NEORX_E_UNSUPPORTED, "EG2 slabs: nebf=%lld not in 1..%lld", (long long)ne, (long long)EG2_MAX_NE);
+    const i64 ne2 = ne * ne;
+    const dim3 grid((unsigned)nblk);
+    auto lds = [&](int nt) { return (size_t)(2 * (nt / 64) * ne + 2 * ne2) * 8; };
+    if (ne2 <= 256 * 4)
+        hipLaunchKernelGGL((k_eg2_slab<256, 4, T>), grid, dim3(256), lds(256), st, ne, d_de, g, gs, part);
+    else if (ne2 <= 256 * 8)
+        hipLaunchKernelGGL((k_eg2_slab<256, 8, T>), grid, dim3(256), lds(256), st, ne, d_de, g, gs, part);
+    else
+        hipLaunchKernelGGL((k_eg2_slab<1024, 8, T>), grid, dim3(1024), lds(1024), st, ne, d_de, g, gs, part);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
 }
 
 int nrx_eg2_slabs(i64 ne, i64 nblk, const double *d_de, const double *g, const double *gs, double *part,
                   hipStream_t st)
 {
-    hipLaunchKernelGGL((k_eg2_slab<256, double>), dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, (i64)0, d_de,
-                       g, gs, part);
-    HIPCHK(hipGetLastError());
-    return NEORX_OK;
+    return eg2_slab_launch<double>(ne, nblk, d_de, g, gs, part, st);
 }
 
 // reduce partials: FE(a,b) += 2 sum_{ip,jp} DP(ip,jp) [dir_a[b] - ex_a[b]/2] etc.
@@ -1258,9 +1315,8 @@ static int eg2_dev(int64_t nebf, int64_t npbf, const double *d_de, const double 
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(fpd.p, 0, np * np * 8, st));
     HIPCHK(hipMemsetAsync(fpsd.p, 0, np * np * 8, st));
-    hipLaunchKernelGGL((k_eg2_slab<256, T>), dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, np, d_de, d_gm2,
-                       d_gm2s, part.d());
-    HIPCHK(hipGetLastError());
+    rc = eg2_slab_launch<T>(ne, nblk, d_de, d_gm2, d_gm2s, part.d(), st);
+    if (rc) return rc;
     int withS = d_gm2s != nullptr;
     // FE/SE and the per-(ip,jp) proton terms into fpd/fpsd; then add fpd to FP
     hipLaunchKernelGGL(k_eg2_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_de, d_dp,
@@ -1328,13 +1384,6 @@ extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const 
     *e_gam2 = es[0];
     *s_gam2 = es[1];
     return NEORX_OK;
-}
-
-// sum over the 64 lanes of a wavefront (every lane gets the total)
-NRX_DI double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
 }
 
 // EG3IC1 (thread_EG3ICR, gam3_IC1_OMP.f:537-725).  Block = slab (ip,jp,i1=a)
@@ -1693,42 +1742,53 @@ extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const 
     return NEORX_OK;
 }
 
-// EG4ICR: lane per entry, direct port of the reduction structure
-// (FE row per slab (ip,jp,i1) in LDS).
+// EG4ICR (thread_EG4ICR, gam4_ICR_OMP.f:390-490).  Block (ip,jp,i1=a) of
+// GM4, entries G[a][j1][i2][j2][i3][j3][i4][j4]: for every j1 the block sums
+// F(j1) = sum DE(i2,j2) DE(i3,j3) DE(i4,j4) G over the ne^6 entries behind it
+// (lanes stride over them, the wave sums by shuffles, the waves meet in LDS
+// slots summed in a fixed order), then writes the partial Fock row 4 F(j1)
+// and the proton term sum_j1 DE(j1,a) F(j1).  ONFLY evaluates symm_gam4/16 per
+// entry from GAM_2s and GAM_ee (thread_gam4_DSCF, gam4_DSCF_OMP.f:322-420)
+// instead of reading the in-core tensor.
 template <int NT, bool ONFLY, class T>
 __global__ void __launch_bounds__(NT) k_eg4_slab(i64 ne, i64 np, const double *__restrict__ DE,
                                                  const T *__restrict__ G, const double *__restrict__ G2s,
                                                  const double *__restrict__ Gee, double *__restrict__ part)
 {
-    // block (ip, jp, i1): ONFLY evaluates symm_gam4/16 per entry from GAM_2s
-    // and GAM_ee (thread_gam4_DSCF, gam4_DSCF_OMP.f:322-420) instead of
-    // reading the in-core tensor
-    extern __shared__ double sm[];
-    i64 blk = blockIdx.x, a = blk % ne, pp = blk / ne, ip = pp / np, jp = pp % np;
-    i64 ne7 = ne * ne * ne * ne * ne * ne * ne;
-    const T *g = ONFLY ? nullptr : G + blk * ne7;
-    for (int k = threadIdx.x; k < ne + 1; k += NT) sm[k] = 0.0;
-    __syncthreads();
-    double pacc = 0.0;
-    for (i64 r = threadIdx.x; r < ne7; r += NT) {
-        i64 j4 = r % ne, i4 = (r / ne) % ne, j3 = (r / (ne * ne)) % ne, i3 = (r / (ne * ne * ne)) % ne;
-        i64 j2 = (r / (ne * ne * ne * ne)) % ne, i2 = (r / (ne * ne * ne * ne * ne)) % ne,
-            j1 = r / (ne * ne * ne * ne * ne * ne);
-        double gv;
-        if constexpr (ONFLY) {
-            i64 br[4] = {a, i2, i3, i4}, kt[4] = {j1, j2, j3, j4};
-            gv = gam4_value(ne, np, G2s, Gee, ip, jp, br, kt);
-        } else {
-            gv = (double)g[r];
+    constexpr int NW = NT / 64;
+    extern __shared__ double sm[]; // [NW][ne] per-wave sums
+    const i64 blk = blockIdx.x, a = blk % ne, pp = blk / ne, ip = pp / np, jp = pp % np;
+    const i64 ne2 = ne * ne, ne4 = ne2 * ne2, ne6 = ne4 * ne2;
+    const int w = threadIdx.x >> 6;
+    for (i64 j1 = 0; j1 < ne; ++j1) {
+        double acc = 0.0;
+        for (i64 r = threadIdx.x; r < ne6; r += NT) {
+            const i64 j4 = r % ne, i4 = (r / ne) % ne, j3 = (r / ne2) % ne, i3 = (r / (ne2 * ne)) % ne;
+            const i64 j2 = (r / ne4) % ne, i2 = r / (ne4 * ne);
+            double gv;
+            if constexpr (ONFLY) {
+                i64 br[4] = {a, i2, i3, i4}, kt[4] = {j1, j2, j3, j4};
+                gv = gam4_value(ne, np, G2s, Gee, ip, jp, br, kt);
+            } else {
+                gv = (double)G[(blk * ne + j1) * ne6 + r];
+            }
+            acc += DE[j2 * ne + i2] * DE[j3 * ne + i3] * DE[j4 * ne + i4] * gv;
         }
-        double ddd = DE[j2 * ne + i2] * DE[j3 * ne + i3] * DE[j4 * ne + i4];
-        atomicAdd(&sm[j1], 4.0 * ddd * gv);
-        pacc += DE[j1 * ne + a] * ddd * gv;
+        acc = wave_sum(acc);
+        if ((threadIdx.x & 63) == 0) sm[w * ne + j1] = acc;
     }
-    for (int o = 32; o > 0; o >>= 1) pacc += __shfl_down(pacc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&sm[ne], pacc);
     __syncthreads();
-    for (int k = threadIdx.x; k < ne + 1; k += NT) part[blk * (ne + 1) + k] = sm[k];
+    if (threadIdx.x < 64) {
+        double p = 0.0;
+        for (i64 j1 = threadIdx.x; j1 < ne; j1 += 64) {
+            double f = 0.0;
+            for (int u = 0; u < NW; ++u) f += sm[u * ne + j1];
+            part[blk * (ne + 1) + j1] = 4.0 * f;
+            p += DE[j1 * ne + a] * f;
+        }
+        p = wave_sum(p);
+        if (threadIdx.x == 0) part[blk * (ne + 1) + ne] = p;
+    }
 }
 
 template <class T>
@@ -1745,7 +1805,7 @@ static int eg4_dev(int64_t nebf, int64_t npbf, const double *d_de, const double 
     rc = part.alloc(nblk * (ne + 1) * 8);
     if (!rc) rc = fpd.alloc(np * np * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_eg4_slab<256, false, T>), dim3((unsigned)nblk), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+    hipLaunchKernelGGL((k_eg4_slab<256, false, T>), dim3((unsigned)nblk), dim3(256), 4 * ne * 8, st, ne, np, d_de,
                        d_gm4, nullptr, nullptr, part.d());
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + np * np, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(),
                        d_fe, fpd.d());
@@ -1894,9 +1954,8 @@ extern "C" int neorx_gam2_dscf_dev(int64_t handle, const double *d_de, const dou
         rc = build2_block(s, p0, p1, X.d(), Xs.d(), Y.d(), Ys.d(), 1, st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gam2_symm, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), Xs.d(), G.d(), Gs.d());
-        hipLaunchKernelGGL((k_eg2_slab<256, double>), dim3((unsigned)((p1 - p0) * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
-                           G.d(), Gs.d(), part.d() + p0 * ne * 4 * ne);
-        HIPCHK(hipGetLastError());
+        rc = eg2_slab_launch<double>(ne, (p1 - p0) * ne, d_de, G.d(), Gs.d(), part.d() + p0 * ne * 4 * ne, st);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_eg2_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_de, d_dp, part.d(),
                        1, 1, d_fe, d_se2, fpd.d(), fpsd.d(), d_e, d_s);
@@ -1958,7 +2017,7 @@ extern "C" int neorx_gam4_dscf_dev(int64_t nebf, int64_t npbf, const double *d_g
     rc = part.alloc(npp * ne * (ne + 1) * 8);
     if (!rc) rc = fpd.alloc(npp * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_eg4_slab<256, true, double>), dim3((unsigned)(npp * ne)), dim3(256), (ne + 1) * 8, st, ne, np, d_de,
+    hipLaunchKernelGGL((k_eg4_slab<256, true, double>), dim3((unsigned)(npp * ne)), dim3(256), 4 * ne * 8, st, ne, np, d_de,
                        nullptr, d_gam2s, d_gamee, part.d());
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,
                        fpd.d());

@@ -203,6 +203,48 @@ def test_eg3_eg4(torch_dev):
     assert e1 == pytest.approx(e2, rel=1e-11)
 
 
+@pytest.mark.parametrize("ne", [33, 46])
+def test_eg2_eg4_fixed_order(torch_dev, ne):
+    """EG2IC1 at basis sizes that take the wider launch shapes of its slab
+    kernel (nebf 33: 256 lanes x 8 columns; 46: 1024 lanes), and EG4ICR:
+    against the oracle, and two runs of the same call bitwise equal (every
+    partial sum has a fixed order, no floating-point atomics)."""
+    print(f"case eg2/eg4 fixed order ne={ne}", flush=True)
+    rng = np.random.default_rng(ne)
+    npb = 1
+    G, Gs = rng.standard_normal(ne ** 4), rng.standard_normal(ne ** 4)
+    DE, DP = synth.density(ne, 14.0, 1), synth.density(npb, 1.0, 2)
+    FE0 = np.asfortranarray(rng.standard_normal((ne, ne)))
+    runs = []
+    for _ in range(2):
+        FE, FP = FE0.copy(order="F"), np.zeros((npb, npb), order="F")
+        r = neorx.EG2IC1(ne, npb, DE, DP, G, Gs, FE, FP)
+        runs.append([FE, FP, r[0], r[1], np.array(r[2:4])])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b), "EG2IC1: two runs differ"
+    FE, FP = FE0.copy(order="F"), np.zeros((npb, npb), order="F")
+    o = oracle.eg2ic1(ne, npb, DE, DP, G, Gs, FE, FP)
+    check_close(runs[0][0], FE, what="FE")
+    check_close(runs[0][1], FP, what="FP")
+    check_close(runs[0][2], o[0], what="SE")
+    assert runs[0][4][0] == pytest.approx(o[2], rel=1e-11)
+    ne4, npb4 = 4, 2
+    G4 = rng.standard_normal(ne4 ** 8 * npb4 * npb4)
+    DE4, DP4 = synth.density(ne4, 3.0, 5), synth.density(npb4, 1.0, 4)
+    runs = []
+    for _ in range(2):
+        FE, FP = np.zeros((ne4, ne4), order="F"), np.zeros((npb4, npb4), order="F")
+        e = neorx.EG4ICR(ne4, npb4, DE4, DP4, G4, FE, FP)
+        runs.append([FE, FP, np.array([e])])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b), "EG4ICR: two runs differ"
+    FE, FP = np.zeros((ne4, ne4), order="F"), np.zeros((npb4, npb4), order="F")
+    e = oracle.eg4icr(ne4, npb4, DE4, DP4, G4, FE, FP)
+    check_close(runs[0][0], FE, what="EG4 FE")
+    check_close(runs[0][1], FP, what="EG4 FP")
+    assert runs[0][2][0] == pytest.approx(e, rel=1e-11)
+
+
 def test_direct_scf_routines(torch_dev):
     """GAM2/3/4_DSCF through the C ABI against the oracle's restatement of the
     reference routines; Fock matrices start non-zero to check accumulation."""

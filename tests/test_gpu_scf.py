@@ -211,7 +211,8 @@ def test_scf_mixed_precision(torch_dev, mode):
 def test_scf_graph_matches_eager(torch_dev):
     """The hipGraph replay of an iteration and the eager launches give the
     same energies bit for bit (every reduction has a fixed order)."""
-    _, a = _run("tiny_pnuc", graph=True, diis=True)
-    _, b = _run("tiny_pnuc", graph=False, diis=True)
+    print("case scf graph vs eager c1mini_sp", flush=True)
+    _, a = _run("c1mini_sp", graph=True, diis=True)
+    _, b = _run("c1mini_sp", graph=False, diis=True)
     assert a["niter"] == b["niter"]
     assert np.array_equal(a["trace"][:, :5], b["trace"][:, :5])
