@@ -1406,75 +1406,102 @@ struct bf16_t {
     uint16_t bits;
 };
 
-// nontemporal load of one stored tensor entry, widened to FP64
-NRX_DI double ld_entry(const double *p) { return __builtin_nontemporal_load(p); }
-NRX_DI double ld_entry(const float *p) { return (double)__builtin_nontemporal_load(p); }
-NRX_DI double ld_entry(const bf16_t *p)
-{
-    unsigned short b = __builtin_nontemporal_load((const unsigned short *)p);
-    return (double)__uint_as_float((unsigned)b << 16);
-}
+// nontemporal load of one stored tensor entry in its storage bits
+// (ld_raw), and the stored value widened to FP64 (widen)
+NRX_DI double ld_raw(const double *p) { return __builtin_nontemporal_load(p); }
+NRX_DI float ld_raw(const float *p) { return __builtin_nontemporal_load(p); }
+NRX_DI unsigned short ld_raw(const bf16_t *p) { return __builtin_nontemporal_load((const unsigned short *)p); }
+NRX_DI double widen(double v) { return v; }
+NRX_DI double widen(float v) { return (double)v; }
+NRX_DI double widen(unsigned short b) { return (double)__uint_as_float((unsigned)b << 16); }
+template <class T> NRX_DI double ld_entry(const T *p) { return widen(ld_raw(p)); }
+template <class T> using raw_t = decltype(ld_raw((const T *)nullptr));
+
+// Compiler barrier between a batch of independent loads and their first
+// use: memory operations cannot move across it, so every load of the batch
+// is issued before the widening and FMAs that consume them (left to itself
+// the scheduler issued narrow loads one at a time, each followed by a wait).
+#define NRX_LOADS_ISSUED() asm volatile("" ::: "memory")
 
 // loads in flight per lane in the EG3 slab loop when no tuned value is
 // recorded: 16 for FP64 storage and 12 for FP32 (both measured best at
 // nebf = 24), 16 for BF16
 template <class T> constexpr int eg3_unroll() { return sizeof(T) == 4 ? 12 : 16; }
 
+// most lanes of one EG3 slab block (its launch bound)
+constexpr unsigned EG3_MAX_THREADS = 256;
+
 template <class T, int U = eg3_unroll<T>()>
-__global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__restrict__ DE, const T *__restrict__ G,
-                                                   double *__restrict__ part)
+__global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const double *__restrict__ DE,
+                                                              const T *__restrict__ G, double *__restrict__ part)
 {
     // offsets inside one slab are 32-bit (ne^5 < 2^31 is checked at launch)
-    extern __shared__ double sm[]; // Fw[nw][ne] (per-wavefront rows), Ff[ne^2] (per-column)
+    // LDS: Fw[nw][ne] (per-wavefront rows), Ff[ne^2] (per-column partials),
+    // Dt[ne^2] with Dt[c*ne + f] = DE(c,f): the lane-dependent density factors
+    // come from LDS, where consecutive lanes read consecutive words, instead
+    // of per-lane global gathers whose latency would sit between the loads
+    extern __shared__ double sm[];
     const unsigned ne = (unsigned)ne_;
     i64 blk = blockIdx.x;
     unsigned a = (unsigned)(blk % ne);
     const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne;
     const unsigned nw = blockDim.x >> 6;
-    double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne;
+    double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne, *Dt = Ff + ne2;
     const T *g = G + blk * (i64)ne4 * ne;
     for (unsigned k = threadIdx.x; k < nw * ne + ne2; k += blockDim.x) sm[k] = 0.0;
+    for (unsigned k = threadIdx.x; k < ne2; k += blockDim.x) Dt[k] = DE[(k % ne) * ne + k / ne];
     __syncthreads();
     const bool lead = (threadIdx.x & 63) == 0;
     for (unsigned ef0 = 0; ef0 < ne2; ef0 += blockDim.x) {
         unsigned ef = ef0 + threadIdx.x;
         bool on = ef < ne2;
         unsigned e = on ? ef / ne : 0, f = on ? ef % ne : 0;
-        double de_ef = on ? DE[f * ne + e] : 0.0; // DE(e,f), column-major
+        const double *De = Dt + e * ne; // De[x] = DE(e,x)
+        double de_ef = on ? De[f] : 0.0;
         double accf = 0.0;
         for (unsigned b = 0; b < ne; ++b) {
-            double de_eb = on ? DE[b * ne + e] : 0.0;
+            double de_eb = on ? De[b] : 0.0;
             double accb = 0.0;
             for (unsigned d = 0; d < ne; ++d) {
-                double de_ed = on ? DE[d * ne + e] : 0.0;
+                double de_ed = on ? De[d] : 0.0;
                 double s1 = 0.0, s2 = 0.0, s3 = 0.0;
                 if (on) {
                     const unsigned o0 = b * ne4 + d * ne2 + ef;
-                    const double *dd = DE + d * ne, *db = DE + b * ne, *df = DE + f * ne;
+                    const double *dd = DE + d * ne, *db = DE + b * ne, *df = Dt + f; // df[c*ne] = DE(c,f)
                     unsigned c = 0;
                     // U (then eight) independent loads in flight per lane before
                     // the FMAs; the slab is read once, so the loads are
                     // nontemporal
                     for (; c + U <= ne; c += U) {
-                        double gv[U];
+                        raw_t<T> r[U];
+                        double fv[U];
 #pragma unroll
-                        for (int k = 0; k < U; ++k) gv[k] = ld_entry(&g[o0 + (c + k) * ne3]);
+                        for (int k = 0; k < U; ++k) r[k] = ld_raw(&g[o0 + (c + k) * ne3]);
+#pragma unroll
+                        for (int k = 0; k < U; ++k) fv[k] = df[(c + k) * ne]; // DE(c,f)
+                        NRX_LOADS_ISSUED();
 #pragma unroll
                         for (int k = 0; k < U; ++k) {
-                            s1 += dd[c + k] * gv[k]; // DE(c,d)
-                            s2 += db[c + k] * gv[k]; // DE(c,b)
-                            s3 += df[c + k] * gv[k]; // DE(c,f)
+                            const double gv = widen(r[k]);
+                            s1 += dd[c + k] * gv; // DE(c,d)
+                            s2 += db[c + k] * gv; // DE(c,b)
+                            s3 += fv[k] * gv;
                         }
                     }
                     if (c + 8 <= ne) {
-                        double gv[8];
+                        raw_t<T> r[8];
+                        double fv[8];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) gv[k] = ld_entry(&g[o0 + (c + k) * ne3]);
+                        for (int k = 0; k < 8; ++k) r[k] = ld_raw(&g[o0 + (c + k) * ne3]);
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) fv[k] = df[(c + k) * ne];
+                        NRX_LOADS_ISSUED();
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            s1 += dd[c + k] * gv[k];
-                            s2 += db[c + k] * gv[k];
-                            s3 += df[c + k] * gv[k];
+                            const double gv = widen(r[k]);
+                            s1 += dd[c + k] * gv;
+                            s2 += db[c + k] * gv;
+                            s3 += fv[k] * gv;
                         }
                         c += 8;
                     }
@@ -1482,7 +1509,7 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
                         double gv = ld_entry(&g[o0 + c * ne3]);
                         s1 += dd[c] * gv;
                         s2 += db[c] * gv;
-                        s3 += df[c] * gv;
+                        s3 += df[c * ne] * gv;
                     }
                 }
                 accb += de_ef * s1 - 0.5 * de_ed * s3;
@@ -1510,16 +1537,29 @@ __global__ void __launch_bounds__(1024) k_eg3_slab(i64 ne_, const double *__rest
     }
 }
 
-// LDS bytes of k_eg3_slab: per-wavefront rows and per-column partials
-static inline size_t eg3_lds(i64 ne, unsigned threads) { return (size_t)((threads / 64) * ne + ne * ne) * 8; }
+// LDS bytes of k_eg3_slab: per-wavefront rows, per-column partials and the
+// transposed density
+static inline size_t eg3_lds(i64 ne, unsigned threads) { return (size_t)((threads / 64) * ne + 2 * ne * ne) * 8; }
 
 // largest nebf whose e-e-e-p slab (nebf^5 entries) has 32-bit offsets
 static const i64 EG3_MAX_NE = 73;
 
+// lanes per slab block: at most EG3_MAX_THREADS (the block loops over its
+// nebf^2 columns), the multiple of 64 that leaves the fewest idle lanes in
+// the last pass, the larger one on a tie
 static inline unsigned eg3_threads(i64 ne)
 {
-    i64 t = ((ne * ne + 63) / 64) * 64;
-    return (unsigned)std::min<i64>(t, 1024);
+    const i64 n = ne * ne;
+    unsigned best = 64;
+    i64 waste = ((n + 63) / 64) * 64 - n;
+    for (unsigned t = 128; t <= EG3_MAX_THREADS; t += 64) {
+        const i64 w = ((n + t - 1) / t) * t - n;
+        if (w <= waste) {
+            best = t;
+            waste = w;
+        }
+    }
+    return best;
 }
 
 // ------------------------------------------------------------------

@@ -387,6 +387,47 @@ def test_eg3_wide_slabs(torch_dev, ne, npb):
         assert outs[0][2][0] == pytest.approx(E, rel=1e-11)
 
 
+def test_autotune_eg3(torch_dev):
+    """neorx_autotune_eg3_dev times the 8, 12, 16 and 24 loads-in-flight
+    variants of the EG3 slab kernel for each storage type and records the
+    fastest; the slab sums run over c in the same order whatever the depth,
+    so the tuned launch gives bitwise the results of the default one."""
+    torch = torch_dev
+    ne, npb = 16, 2
+    print(f"case autotune eg3 ne={ne} np={npb}", flush=True)
+    rng = np.random.default_rng(77)
+    DE, DP = synth.density(ne, 6.0, 7), synth.density(npb, 1.0, 8)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    d_de, d_dp, d_g3 = _dev(torch, DE.ravel(order="F")), _dev(torch, DP.ravel(order="F")), _dev(torch, g3)
+    d_g3f = torch.empty(g3.size, dtype=torch.float32, device="cuda")
+    neorx.call_dev("neorx_convert_f32_dev", g3.size, d_g3.data_ptr(), d_g3f.data_ptr(), 0)
+    torch.cuda.synchronize()
+
+    def run(name, g):
+        fe = torch.zeros(ne * ne, dtype=torch.float64, device="cuda")
+        fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+        e = torch.zeros(1, dtype=torch.float64, device="cuda")
+        neorx.call_dev(name, ne, npb, d_de.data_ptr(), d_dp.data_ptr(), g.data_ptr(), fe.data_ptr(), fp.data_ptr(),
+                       e.data_ptr(), 0)
+        torch.cuda.synchronize()
+        return [x.cpu().numpy() for x in (fe, fp, e)]
+
+    for dtype, name, g in ((0, "neorx_eg3ic1_dev", d_g3), (1, "neorx_eg3ic1_f32_dev", d_g3f)):
+        before = run(name, g)
+        best, ms, allms = neorx.autotune_eg3_dev(ne, npb, dtype, d_de.data_ptr(), g.data_ptr(), reps=2)
+        print(f"  dtype {dtype}: best {best} ({ms:.3f} ms), all {allms}", flush=True)
+        assert best in (8, 12, 16, 24) and ms > 0 and all(m > 0 for m in allms)
+        assert ms == pytest.approx(min(allms), rel=1e-6)
+        after = run(name, g)
+        for a, b in zip(before, after):
+            assert np.array_equal(a, b)
+    d_g3h = torch.from_numpy(g3.astype(np.float32)).cuda().to(torch.bfloat16)
+    best, ms, allms = neorx.autotune_eg3_dev(ne, npb, 2, d_de.data_ptr(), d_g3h.data_ptr(), reps=2)
+    assert best in (8, 12, 16, 24) and ms > 0
+    with pytest.raises(neorx.NeorxError):
+        neorx.autotune_eg3_dev(ne, npb, 3, d_de.data_ptr(), d_g3.data_ptr(), reps=1)
+
+
 def test_precision_switch(torch_dev):
     """FP64 is selected once successive densities differ by less than the
     threshold, and stays selected."""
