@@ -9,8 +9,10 @@ in this container on
   which this run must reproduce;
 * small synthetic systems in the geometry of the C1 benchmark system
   (classical nuclei at z = 0 and 2.1 bohr, the quantum proton at 4.3 bohr)
-  with s/p/d electrons and s/p nuclear functions, started from the core
-  guess.
+  with s/p electrons and s/p nuclear functions, started from the core
+  guess.  (A d-electron system was dropped: the reference's xcscf did not
+  finish on it within 50 minutes on 8 threads; d functions are covered by the
+  Fock-build and integral tests against the oracle.)
 
 Writes tests/golden/xchf_scf.json: per system the input arrays, the
 initial-guess energy components, the final components and the iteration
@@ -66,11 +68,17 @@ def mini(name, centres, nuc_shells, zan, nelec, seed):
 SYSTEMS = {
     "c1mini_sp": lambda: mini("c1mini_sp", [(0.0, [(0, 2), (0, 1), (1, 1)]), (2.1, [(0, 1), (1, 1)])], [0],
                               [2.0, 1.0], 4, 11),
-    "c1mini_d": lambda: mini("c1mini_d", [(0.0, [(0, 2), (2, 1)])], [0], [2.0, 1.0], 4, 13),
     # small enough for the oracle's own integrals in the CPU test suite
     "tiny_s": lambda: mini("tiny_s", [(0.0, [(0, 2), (0, 1)]), (2.1, [(0, 1)])], [0], [2.0, 1.0], 4, 14),
     "tiny_pnuc": lambda: mini("tiny_pnuc", [(0.0, [(0, 2), (0, 1)]), (2.1, [(0, 1)])], [0, 1], [2.0, 1.0], 4, 15),
 }
+
+
+# With its second-order (SOSCF) step the reference's xcscf did not finish
+# within an hour on these three-function systems (8 threads busy, no output);
+# they are run with its ROOTHAN steps (LSOSCF = .false.), which converge to
+# the same fixed point.
+NO_SOSCF = {"tiny_s", "tiny_pnuc"}
 
 
 def jsonable(s):
@@ -103,10 +111,11 @@ def main(names):
         if names and name not in names:
             continue
         s = mk()
+        lsoscf = name not in NO_SOSCF
         with tempfile.TemporaryDirectory() as wd:
-            r = oracle.ref_xcscf(s, wd)
+            r = oracle.ref_xcscf(s, wd, lsoscf=lsoscf, timeout=3600 if lsoscf else 900)
         print(s["name"], s["nebf"], s["npebf"], s["npbf"], r["energy"], r["niter"], flush=True)
-        res[s["name"]] = dict(system=jsonable(s), **r)
+        res[s["name"]] = dict(system=jsonable(s), lsoscf=lsoscf, **r)
         json.dump(res, open(path, "w"), indent=1)
 
 

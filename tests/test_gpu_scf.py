@@ -183,11 +183,11 @@ def test_scf_example(torch_dev, graph, diis):
     _check_golden(g, r)
 
 
-@pytest.mark.parametrize("name", ["c1mini_sp", "c1mini_d", "tiny_s", "tiny_pnuc"])
+@pytest.mark.parametrize("name", ["c1mini_sp", "tiny_s", "tiny_pnuc"])
 def test_scf_golden_minis(torch_dev, name):
-    """Small systems in the C1 geometry (s/p/d electrons, s/p nuclear
+    """Small systems in the C1 geometry (s/p electrons, s/p nuclear
     functions), core guess: the reference xcscf's initial components and
-    final energy."""
+    final energy (tiny_s and tiny_pnuc from its ROOTHAN steps)."""
     print(f"case scf {name}", flush=True)
     g, r = _run(name, diis=True)
     _check_golden(g, r)
