@@ -170,13 +170,14 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     copies), then the SCF in FP64 and in mixed precision (GAM_3 read in BF16
     until the DIIS error < 1e-3, then FP32 until < 1e-5, then FP64), each
     iteration a captured hipGraph.  SCF s/iter = wall time per iteration from
-    the host's clock (graph launch to the read-back of the convergence data)."""
+    the host's clock (graph launch to the read-back of the convergence data).
+    The EG3 slab depth is autotuned per storage type on the built tensors."""
     from neorx import xchf
 
     sysd = synth.config1()
     S = neorx.System(sysd)
     t0 = time.perf_counter()
-    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "f32", "bf16"))
+    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "f32", "bf16"), autotune=True)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     ne, npe, npb = sysd["nebf"], sysd["npebf"], sysd["npbf"]

@@ -242,50 +242,69 @@ __global__ void k_pair_transpose(i64 ne, const double *__restrict__ X, double *_
 }
 
 // ------------------------------------------------------------------
-// the 24 ket permutations of symm_gam4 (gam_4_OMP.f:788-856): electron m
-// takes ket perm[s][m]; its weights; tau = the inverse permutation
-// ------------------------------------------------------------------
-__constant__ int c_tau[24][4];
-__constant__ double c_w32[24]; // weight / 32
-
-const int h_perm4[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0, 2, 1, 3}, {1, 2, 0, 3}, {2, 1, 0, 3},
-                            {3, 1, 0, 2}, {1, 3, 0, 2}, {0, 3, 1, 2}, {3, 0, 1, 2}, {1, 0, 3, 2}, {0, 1, 3, 2},
-                            {0, 2, 3, 1}, {2, 0, 3, 1}, {3, 0, 2, 1}, {0, 3, 2, 1}, {2, 3, 0, 1}, {3, 2, 0, 1},
-                            {3, 2, 1, 0}, {2, 3, 1, 0}, {1, 3, 2, 0}, {3, 1, 2, 0}, {2, 1, 3, 0}, {1, 2, 3, 0}};
-const double h_w4[24] = {16, -8, 4, -8, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2, 4, -8, 4, -2, 4, -2, 4, -8, 4, -2};
-
-// adjoint of the permutation sum:
+// adjoint of the 24-permutation sum of symm_gam4 (gam_4_OMP.f:788-856, ket
+// permutations sigma with weights 16, -8, 4, -8, 4, -8, 4, -2, ...):
 //   Y(a0,a1,a2,a3) = sum_s w_s/32 sum_{b : b_t0=a0, b0=a1, b_t1=a2, b1=a3} X(b_t2, b2, b_t3, b3)
-// (t = tau[s]); Y = dE4/dA for X = B and, by the symmetry of the sum under
+// (t = sigma^-1); Y = dE4/dA for X = B and, by the symmetry of the sum under
 // exchanging the electron pairs (0,1) and (2,3), Y = dE4/dB for X = A.
-// Lane per entry of Y; per permutation 0, 1, ne or ne^2 reads of X.
-__global__ void __launch_bounds__(256) k_g4_adjoint(i64 ne, const double *__restrict__ X, double *__restrict__ Y)
+// Resolving the constraints permutation by permutation: four permutations
+// leave no free index (Y reads one permuted entry of X), sixteen fix one of
+// b2, b3 only up to a Kronecker delta between two a's and sum the other over
+// a trace of X, and four leave two deltas and a double trace.  With the
+// partial traces (k_g4_traces)
+//   R0(p,q) = sum_y X(p,q,y,y)   R1(p,q) = sum_y X(y,p,q,y)
+//   R2(p,q) = sum_x X(p,x,x,q)   R3(p,q) = sum_x X(x,x,p,q)
+//   T1 = sum_x R0(x,x)           T2 = sum_x R2(x,x)
+// the sum is the closed form evaluated by k_g4_adjoint; a lane per entry
+// makes four reads of X and at most sixteen of the ne^2 traces.
+// ------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_g4_traces(i64 ne, const double *__restrict__ X, double *__restrict__ R)
 {
-    GRID_LOOP(t, ne * ne * ne * ne)
+    const i64 ne2 = ne * ne;
+    GRID_LOOP(t, 4 * ne2)
     {
-        const i64 a3 = t % ne, a2 = (t / ne) % ne, a1 = (t / (ne * ne)) % ne, a0 = t / (ne * ne * ne);
-        double sum = 0.0;
-        for (int s = 0; s < 24; ++s) {
-            const int t0 = c_tau[s][0], t1 = c_tau[s][1], t2 = c_tau[s][2], t3 = c_tau[s][3];
-            i64 b[4] = {a1, a3, -1, -1};
-            if (b[t0] >= 0 && b[t0] != a0) continue;
-            b[t0] = a0;
-            if (b[t1] >= 0 && b[t1] != a2) continue;
-            b[t1] = a2;
-            const bool f2 = b[2] < 0, f3 = b[3] < 0;
-            double acc = 0.0;
-            const i64 n2lo = f2 ? 0 : b[2], n2hi = f2 ? ne : b[2] + 1;
-            const i64 n3lo = f3 ? 0 : b[3], n3hi = f3 ? ne : b[3] + 1;
-            for (i64 x2 = n2lo; x2 < n2hi; ++x2) {
-                b[2] = x2;
-                for (i64 x3 = n3lo; x3 < n3hi; ++x3) {
-                    b[3] = x3;
-                    acc += X[((b[t2] * ne + b[2]) * ne + b[t3]) * ne + b[3]];
-                }
-            }
-            sum += c_w32[s] * acc;
+        const i64 k = t / ne2, p = (t % ne2) / ne, q = t % ne;
+        double acc = 0.0;
+        for (i64 y = 0; y < ne; ++y) {
+            const i64 i = k == 0 ? p : k == 1 ? y : k == 2 ? p : y;
+            const i64 j = k == 0 ? q : k == 1 ? p : y;
+            const i64 l = k == 0 ? y : k == 1 ? q : k == 2 ? y : p;
+            const i64 m = k == 0 ? y : k == 1 ? y : q;
+            acc += X[((i * ne + j) * ne + l) * ne + m];
         }
-        Y[t] = sum;
+        R[t] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_g4_adjoint(i64 ne, const double *__restrict__ X,
+                                                    const double *__restrict__ R, double *__restrict__ Y)
+{
+    const i64 ne2 = ne * ne;
+    const double *R0 = R, *R1 = R + ne2, *R2 = R + 2 * ne2, *R3 = R + 3 * ne2;
+    GRID_LOOP(t, ne2 * ne2)
+    {
+        const i64 a3 = t % ne, a2 = (t / ne) % ne, a1 = (t / ne2) % ne, a0 = t / (ne2 * ne);
+        auto x = [&](i64 i, i64 j, i64 k, i64 l) { return X[((i * ne + j) * ne + k) * ne + l]; };
+        auto r = [&](const double *Rk, i64 p, i64 q) { return Rk[p * ne + q]; };
+        // no free index: sigma = (2,3,0,1), (3,2,0,1), (3,2,1,0), (2,3,1,0) as tau
+        double s = 4.0 * x(a1, a0, a3, a2) - 2.0 * x(a3, a0, a1, a2) + 4.0 * x(a3, a2, a1, a0) - 2.0 * x(a1, a2, a3, a0);
+        // one delta, one trace
+        if (a0 == a1) s += -8.0 * r(R0, a3, a2) + 4.0 * r(R1, a2, a3) + 4.0 * r(R2, a3, a2) - 8.0 * r(R3, a3, a2);
+        if (a0 == a3) s += 4.0 * r(R0, a1, a2) - 2.0 * r(R1, a2, a1) - 2.0 * r(R2, a1, a2) + 4.0 * r(R3, a1, a2);
+        if (a2 == a1) s += 4.0 * r(R0, a3, a0) - 2.0 * r(R1, a0, a3) - 2.0 * r(R2, a3, a0) + 4.0 * r(R3, a3, a0);
+        if (a2 == a3) s += -8.0 * r(R0, a1, a0) + 4.0 * r(R1, a0, a1) + 4.0 * r(R2, a1, a0) - 8.0 * r(R3, a1, a0);
+        // two deltas, double traces
+        const bool d01 = a0 == a1 && a2 == a3, d03 = a0 == a3 && a2 == a1;
+        if (d01 || d03) {
+            double t1 = 0.0, t2 = 0.0;
+            for (i64 z = 0; z < ne; ++z) {
+                t1 += r(R0, z, z);
+                t2 += r(R2, z, z);
+            }
+            if (d01) s += 16.0 * t1 - 8.0 * t2;
+            if (d03) s += -8.0 * t1 + 4.0 * t2;
+        }
+        Y[t] = s / 32.0;
     }
 }
 
@@ -368,44 +387,47 @@ __global__ void __launch_bounds__(256) k_ep_rows(i64 ne, i64 np, const double *_
 }
 
 // E_from_GAM_ee (xchf1_fock.f:852-911): FEee(a,b) = sum_cd DE(c,d)
-// [(ab|cd) - (ad|cb)/2] (column-major), GAM_ee = [a][b][c][d]
-__global__ void __launch_bounds__(256) k_fock_ee(i64 ne, const double *__restrict__ DE, const double *__restrict__ Gee,
-                                                 double *__restrict__ FEee)
+// [(ab|cd) - (ad|cb)/2] (column-major), GAM_ee = [a][b][c][d].  A wavefront
+// per (a,b): the lanes stride over (c,d) (the (ab|cd) row is contiguous) and
+// meet in a fixed-order wave sum.
+__global__ void __launch_bounds__(64) k_fock_ee(i64 ne, const double *__restrict__ DE, const double *__restrict__ Gee,
+                                                double *__restrict__ FEee)
 {
-    GRID_LOOP(t, ne * ne)
-    {
+    const i64 ne2 = ne * ne;
+    for (i64 t = blockIdx.x; t < ne2; t += gridDim.x) {
         const i64 a = t % ne, b = t / ne;
+        const double *row = Gee + (a * ne + b) * ne2;
         double acc = 0.0;
-        for (i64 c = 0; c < ne; ++c)
-            for (i64 d = 0; d < ne; ++d) {
-                const double v1 = Gee[((a * ne + b) * ne + c) * ne + d], v2 = Gee[((a * ne + d) * ne + c) * ne + b];
-                acc += DE[d * ne + c] * (v1 - 0.5 * v2);
-            }
-        FEee[b * ne + a] = acc;
+        for (i64 cd = threadIdx.x; cd < ne2; cd += 64) {
+            const i64 c = cd / ne, d = cd % ne;
+            acc += DE[d * ne + c] * (row[cd] - 0.5 * Gee[((a * ne + d) * ne + c) * ne + b]);
+        }
+        acc = wsum(acc);
+        if (threadIdx.x == 0) FEee[b * ne + a] = acc;
     }
 }
 
 // ------------------------------------------------------------------
 // phase B: weighting by DP and the assembly of xchf1_fock
 // ------------------------------------------------------------------
-// lane per (a,b) of the electronic matrices and per (ip,jp) of the nuclear:
+// a wavefront per (a,b) of the electronic matrices (lanes over (ip,jp)) and
+// per (ip,jp) of the nuclear ones (lanes over (a,b)), fixed-order wave sums:
 //   FEs(a,b) = sum_pq DP(pq) [GAM_ep + GAM_1][pq][a][b] + EG2 + EG3 rows,
 //   SEs(a,b) = sum_pq DP(pq) GAM_1s[pq][a][b] + 2 SE2 (se1 + 2 se2),
 //   FPs(pq) = EG2 + EG3 nuclear sums; SP2(pq)
-__global__ void __launch_bounds__(256) k_phaseB_rows(i64 ne, i64 np, const double *__restrict__ DE,
-                                                     const double *__restrict__ DP, const double *__restrict__ Gep,
-                                                     const double *__restrict__ G1, const double *__restrict__ G1s,
-                                                     const double *__restrict__ part2, const double *__restrict__ part3,
-                                                     int has2, int has3, double *__restrict__ FEs, double *__restrict__ SEs,
-                                                     double *__restrict__ FP23, double *__restrict__ SP2)
+__global__ void __launch_bounds__(64) k_phaseB_rows(i64 ne, i64 np, const double *__restrict__ DE,
+                                                    const double *__restrict__ DP, const double *__restrict__ Gep,
+                                                    const double *__restrict__ G1, const double *__restrict__ G1s,
+                                                    const double *__restrict__ part2, const double *__restrict__ part3,
+                                                    int has2, int has3, double *__restrict__ FEs, double *__restrict__ SEs,
+                                                    double *__restrict__ FP23, double *__restrict__ SP2)
 {
     const i64 ne2 = ne * ne, np2 = np * np;
-    GRID_LOOP(t, ne2 + np2)
-    {
+    for (i64 t = blockIdx.x; t < ne2 + np2; t += gridDim.x) {
         if (t < ne2) {
             const i64 a = t % ne, b = t / ne; // FE(a,b) at b*ne + a
             double f = 0.0, s1 = 0.0, f2 = 0.0, s2 = 0.0, f3 = 0.0;
-            for (i64 pq = 0; pq < np2; ++pq) {
+            for (i64 pq = threadIdx.x; pq < np2; pq += 64) {
                 const double dp = DP[(pq % np) * np + pq / np];
                 const i64 u = pq * ne2 + a * ne + b;
                 f += dp * (Gep[u] + G1[u]);
@@ -417,26 +439,37 @@ __global__ void __launch_bounds__(256) k_phaseB_rows(i64 ne, i64 np, const doubl
                 }
                 if (has3) f3 += dp * part3[(pq * ne + a) * (ne + 1) + b];
             }
-            FEs[t] = f + f2 + f3;
-            SEs[t] = s1 + 2.0 * s2;
+            f = wsum(f);
+            s1 = wsum(s1);
+            f2 = wsum(f2);
+            s2 = wsum(s2);
+            f3 = wsum(f3);
+            if (threadIdx.x == 0) {
+                FEs[t] = f + f2 + f3;
+                SEs[t] = s1 + 2.0 * s2;
+            }
         } else {
             const i64 pq = t - ne2;
             double f = 0.0, fs = 0.0, f3 = 0.0;
-            for (i64 a = 0; a < ne; ++a) {
+            for (i64 ab = threadIdx.x; ab < ne2; ab += 64) {
+                const i64 a = ab / ne, b = ab % ne;
                 if (has2) {
                     const double *pr = part2 + (pq * ne + a) * 4 * ne;
-                    for (i64 b = 0; b < ne; ++b) {
-                        const double dab = DE[b * ne + a];
-                        f += dab * (pr[b] - 0.5 * pr[ne + b]);
-                        fs += dab * (pr[2 * ne + b] - 0.5 * pr[3 * ne + b]);
-                    }
+                    const double dab = DE[b * ne + a];
+                    f += dab * (pr[b] - 0.5 * pr[ne + b]);
+                    fs += dab * (pr[2 * ne + b] - 0.5 * pr[3 * ne + b]);
                 }
-                if (has3) f3 += part3[(pq * ne + a) * (ne + 1) + ne];
+                if (has3 && b == 0) f3 += part3[(pq * ne + a) * (ne + 1) + ne];
             }
-            FP23[pq] = f + f3;
-            FP23[np2 + pq] = f;  // EG2 alone (E_gam2)
-            FP23[2 * np2 + pq] = f3; // EG3 alone (E_gam3)
-            SP2[pq] = fs;
+            f = wsum(f);
+            fs = wsum(fs);
+            f3 = wsum(f3);
+            if (threadIdx.x == 0) {
+                FP23[pq] = f + f3;
+                FP23[np2 + pq] = f;      // EG2 alone (E_gam2)
+                FP23[2 * np2 + pq] = f3; // EG3 alone (E_gam3)
+                SP2[pq] = fs;
+            }
         }
     }
 }
@@ -867,6 +900,7 @@ struct Xchf {
     double *part2 = nullptr, *part3 = nullptr, *FPx = nullptr, *FEee = nullptr;
     double *A4 = nullptr, *HA4 = nullptr, *Bb4 = nullptr, *GPb4 = nullptr, *FP4 = nullptr, *t4 = nullptr;
     double *GP4 = nullptr, *HB4 = nullptr, *B4 = nullptr, *Ab4 = nullptr, *FE4 = nullptr, *E4 = nullptr;
+    double *R4 = nullptr; // partial traces of the adjoint (4 nebf^2)
     double *zpart = nullptr, *rows = nullptr;
     i64 nz_fe = 1, nch_fp = 1, nz_dot = 1;
     double *FEs = nullptr, *SEs = nullptr, *FP23 = nullptr, *SP2 = nullptr;
@@ -909,23 +943,6 @@ Xchf *get_x(i64 h)
     std::lock_guard<std::mutex> lk(g_xmu);
     auto it = g_xchf.find(h);
     return it == g_xchf.end() ? nullptr : it->second;
-}
-
-bool g_consts_set = false;
-
-int set_consts()
-{
-    if (g_consts_set) return NEORX_OK;
-    int tau[24][4];
-    double w[24];
-    for (int s = 0; s < 24; ++s) {
-        for (int m = 0; m < 4; ++m) tau[s][h_perm4[s][m]] = m;
-        w[s] = h_w4[s] / 32.0;
-    }
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tau), tau, sizeof(tau)));
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_w32), w, sizeof(w)));
-    g_consts_set = true;
-    return NEORX_OK;
 }
 
 // C (M x N) = A(M x K) B(K x N) through the MFMA GEMM, split over K when the
@@ -985,7 +1002,7 @@ int phase_a(Xchf *x, const double *DE, int dtype)
     hipLaunchKernelGGL(k_ep_rows, dim3((unsigned)np2), dim3(256), 0, st, ne, np, DE, x->Gep, x->G1, x->G1s, x->FPx);
     HIPCHK(hipGetLastError());
     if (x->nelec > 1) {
-        hipLaunchKernelGGL(k_fock_ee, dim3(grid_blocks(ne * ne, 64)), dim3(64), 0, st, ne, DE, x->Gee, x->FEee);
+        hipLaunchKernelGGL(k_fock_ee, dim3(grid_blocks(ne * ne, 1)), dim3(64), 0, st, ne, DE, x->Gee, x->FEee);
         int rc = nrx_eg2_slabs(ne, np2 * ne, DE, x->G2, x->G2s, x->part2, st);
         if (rc) return rc;
     }
@@ -998,7 +1015,8 @@ int phase_a(Xchf *x, const double *DE, int dtype)
         // GPbar = Bbar carried back, FP4 = G2s . GPbar
         rc = both_bras(x, DE, x->GeeU, x->HA4, x->A4);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_g4_adjoint, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->A4, x->Bb4);
+        hipLaunchKernelGGL(k_g4_traces, dim3(grid_blocks(4 * ne * ne, 256)), dim3(256), 0, st, ne, x->A4, x->R4);
+        hipLaunchKernelGGL(k_g4_adjoint, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->A4, x->R4, x->Bb4);
         // GPbar(i2,b2,i3,b3) = sum D(i2,c2) D(i3,c3) Bbar(c2,b2,c3,b3)
         hipLaunchKernelGGL(k_pair_transpose, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->Bb4, x->t4);
         rc = lead_transform_inv(x, DE, x->t4, x->GPb4, ne * ne * ne);
@@ -1020,7 +1038,7 @@ int phase_b(Xchf *x, const double *DE, const double *DP, double *FE, double *FP,
 {
     hipStream_t st = x->st;
     const i64 ne = x->ne, np = x->np, ne2 = ne * ne, np2 = np * np, ne3 = ne2 * ne, ne4 = ne3 * ne;
-    hipLaunchKernelGGL(k_phaseB_rows, dim3(nblocks(ne2 + np2, 64)), dim3(64), 0, st, ne, np, DE, DP, x->Gep, x->G1,
+    hipLaunchKernelGGL(k_phaseB_rows, dim3(grid_blocks(ne2 + np2, 1)), dim3(64), 0, st, ne, np, DE, DP, x->Gep, x->G1,
                        x->G1s, x->part2, x->part3, (int)(x->nelec > 1), (int)(x->nelec > 2), x->FEs, x->SEs, x->FP23,
                        x->SP2);
     HIPCHK(hipGetLastError());
@@ -1030,7 +1048,8 @@ int phase_b(Xchf *x, const double *DE, const double *DP, double *FE, double *FP,
         hipLaunchKernelGGL(k_gp, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne4, np, DP, x->G2sU, x->GP4);
         int rc = both_bras(x, DE, x->GP4, x->HB4, x->B4);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_g4_adjoint, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->B4, x->Ab4);
+        hipLaunchKernelGGL(k_g4_traces, dim3(grid_blocks(4 * ne2, 256)), dim3(256), 0, st, ne, x->B4, x->R4);
+        hipLaunchKernelGGL(k_g4_adjoint, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->B4, x->R4, x->Ab4);
         hipLaunchKernelGGL(k_dot_part, dim3((unsigned)x->nz_dot), dim3(256), 0, st, ne4, x->A4, x->Ab4, x->rows);
         hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, st, (i64)1, x->nz_dot, x->rows, x->E4);
         // FE4 = dE4/dDE: C1[x][y] = sum_r HA[x][r] Abar[y][r] (the bra slots of
@@ -1170,6 +1189,7 @@ int xchf_build(Xchf *x, i64 handle, i64 store)
     rc |= x->alloc(x->FEee, ne2);
     const i64 n4 = x->nelec > 2 ? ne4 : 1;
     rc |= x->alloc(x->A4, n4);
+    rc |= x->alloc(x->R4, 4 * ne2);
     rc |= x->alloc(x->HA4, n4);
     rc |= x->alloc(x->Bb4, n4);
     rc |= x->alloc(x->GPb4, n4);
@@ -1281,8 +1301,6 @@ extern "C" int neorx_xchf_create(int64_t handle, int64_t nelec, int64_t store, i
     if (nelec > 2 && !(store & 7)) return fail(NEORX_E_ARG, "store must keep GAM_3 in at least one storage type");
     if (s->d.nebf > RH_MAXN || s->d.npbf > RH_MAXN)
         return fail(NEORX_E_UNSUPPORTED, "ROOTHAN on the device supports at most %d basis functions", RH_MAXN);
-    rc = set_consts();
-    if (rc) return rc;
     Xchf *x = new Xchf();
     x->s = s;
     x->ne = s->d.nebf;

@@ -58,7 +58,8 @@ def test_xchf_integrals(torch_dev):
 @pytest.mark.parametrize("kw,sym", [
     (dict(seed=82, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,)), True),
     (dict(seed=83, n_e_centres=3, e_shells=(0,), contract=(1, 2), p_shells=(0, 1)), False),
-], ids=["sp-symmetric-D", "s-nonsymmetric-D"])
+    (dict(seed=84, n_e_centres=1, e_shells=(0, 2), contract=(1, 1), p_shells=(0,), ngtg=2), True),
+], ids=["sp-symmetric-D", "s-nonsymmetric-D", "sd-electrons"])
 def test_xchf_fock(torch_dev, kw, sym):
     """One Fock build (xchf1_fock + XCfock_correction) against the oracle's
     restatement, which contracts the in-core GAM_4 of GAM4_ICR as EG4ICR does:
