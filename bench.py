@@ -166,58 +166,59 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     """NEO-XCHF RHF SCF of the C1 system (neorx.synth.config1: C/N classical,
     quantum proton, 24 electronic functions from 36 s/p primitives, 8 nuclear
     s/p/d primitives, 3-term geminal, 14 electrons) on the device: every
-    integral built on the GPU (GM3 = 1.22e10 entries in FP64, FP32 and BF16
-    copies), then the SCF in FP64 and in mixed precision (GAM_3 read in BF16
-    until the DIIS error < 1e-3, then FP32 until < 1e-5, then FP64), each
-    iteration a captured hipGraph.  SCF s/iter = wall time per iteration from
-    the host's clock (graph launch to the read-back of the convergence data).
-    The EG3 slab depth is autotuned per storage type on the built tensors."""
+    integral built on the GPU (GM3 = 1.22e10 entries, kept in FP64 and BF16),
+    then the SCF with DIIS, each iteration one replay of a captured hipGraph:
+      * FP64: GAM_3 read in FP64 every iteration;
+      * mixed (incremental): GAM_3 read in BF16 until the DIIS error < 1e-3,
+        then an FP64 build kept as the reference and BF16-stored increments
+        T(DE - DEref, DE + DEref) from it, a new FP64 reference once the DIIS
+        error < 1e-5, and a last FP64 build that confirms convergence.
+    Each mode runs once untimed (graph capture) and once timed; SCF s/iter =
+    the timed run's wall time (host clock, first graph launch to the last
+    read-back) / its iterations.  The EG3 slab depth is autotuned per storage
+    type on the built tensors."""
     from neorx import xchf
 
     sysd = synth.config1()
     S = neorx.System(sysd)
     t0 = time.perf_counter()
-    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "f32", "bf16"), autotune=True)
+    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "bf16"), autotune=True)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     ne, npe, npb = sysd["nebf"], sysd["npebf"], sysd["npbf"]
     prims = npe ** 6 * npb ** 2 + npe ** 4 * npb ** 2 + npe ** 2 * npb ** 2
     print(f"# C1 integrals built in {t_build:.1f} s", file=sys.stderr, flush=True)
     runs = {}
-    for mode in ("f64", "bf16", "f32"):
+    names = {0: "f64", 1: "f32", 2: "bf16"}
+    for mode in ("f64", "inc"):
+        X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-5)  # captures the graphs
         t0 = time.perf_counter()
         r = X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-5)
         wall = time.perf_counter() - t0
         tr = r["trace"]
-        per = {}
-        for code, nm in ((0, "f64"), (1, "f32"), (2, "bf16")):
-            m = tr[:, 5] == code
-            if m.any():
-                # the first iteration of each storage type includes its graph capture
-                ms = tr[m, 6][1:] if m.sum() > 1 else tr[m, 6]
-                per[nm] = {"iterations": int(m.sum()), "ms_per_iter": float(np.median(ms))}
+        kinds = {}
+        for k in range(len(tr)):
+            nm = names[int(tr[k, 5])] + ("-increment" if tr[k, 7] == 2 else "-reference" if tr[k, 7] == 1 else "")
+            kinds.setdefault(nm, []).append(float(tr[k, 6]))
+        per = {k: {"iterations": len(v), "ms_per_iter": float(np.median(v))} for k, v in kinds.items()}
         runs[mode] = {"energy": r["comps"]["E_TOTAL"], "iterations": r["niter"], "wall_s": wall,
-                      "ms_per_iter_all": float(tr[:, 6].sum() / len(tr)), "by_storage": per}
-        print(f"# SCF {mode}: E={r['comps']['E_TOTAL']:.10f} it={r['niter']} "
+                      "s_per_iter": wall / r["niter"], "by_build": per}
+        print(f"# SCF {mode}: E={r['comps']['E_TOTAL']:.10f} it={r['niter']} {wall / r['niter'] * 1e3:.2f} ms/iter "
               + " ".join(f"{k}:{v['iterations']}x{v['ms_per_iter']:.2f}ms" for k, v in per.items()),
               file=sys.stderr, flush=True)
     X.close()
     S.close()
-    f64 = runs["f64"]["by_storage"]["f64"]["ms_per_iter"]
-    out = {
+    f64, mix = runs["f64"], runs["inc"]
+    return {
         "system": "C1 NEO-XCHF RHF: nebf 24 (36 s/p prims), npbf 8 (s/p/d), 3 geminals, 14 electrons",
         "integral_build_s": t_build, "primitive_tuples_built": prims,
-        "scf_s_per_iter_fp64": f64 / 1e3,
+        "scf_s_per_iter_fp64": f64["s_per_iter"], "scf_s_per_iter_mixed": mix["s_per_iter"],
+        "speedup_s_per_iter_mixed_vs_fp64": f64["s_per_iter"] / mix["s_per_iter"],
+        "scf_wall_s_fp64": f64["wall_s"], "scf_wall_s_mixed": mix["wall_s"],
+        "speedup_scf_wall_mixed_vs_fp64": f64["wall_s"] / mix["wall_s"],
+        "energy_fp64_Eh": f64["energy"], "dE_mixed_vs_fp64_Eh": abs(mix["energy"] - f64["energy"]),
         "runs": runs,
     }
-    for mode in ("bf16", "f32"):
-        lo = runs[mode]["by_storage"].get(mode)
-        if lo:
-            out[f"scf_s_per_iter_{mode}_phase"] = lo["ms_per_iter"] / 1e3
-            out[f"speedup_{mode}_phase_vs_fp64"] = f64 / lo["ms_per_iter"]
-        out[f"dE_{mode}_vs_fp64_Eh"] = abs(runs[mode]["energy"] - runs["f64"]["energy"])
-        out[f"scf_wall_ratio_fp64_over_{mode}"] = runs["f64"]["wall_s"] / runs[mode]["wall_s"]
-    return out
 
 
 def main():

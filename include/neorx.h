@@ -42,6 +42,22 @@ const char *neorx_last_error(void);
 /* name of the device the library runs on (e.g. "gfx950") */
 const char *neorx_device_arch(void);
 
+/* Resident tensors (no counterpart in the reference: the reference keeps
+ * its in-core tensors in host memory).  The host-pointer builders
+ * neorx_gam2_ic1, neorx_gam3_ic1 and neorx_gam4_icr fill the caller's array
+ * and keep their device copy, keyed by the array's address and size; a later
+ * host-pointer call given the same array as an input tensor (EG2IC1, EG3IC1,
+ * EG4ICR, the XCUHF contractions, GAM4_ICR) reads that copy instead of
+ * copying the tensor to the device again.  4096 entries sampled over the
+ * array are compared with the values kept at registration on every use, and
+ * a difference drops the copy.  Copies use at most 60 % of the device memory
+ * (the oldest are released first).  NEORX_RESIDENT=0 in the environment
+ * turns this off.  neorx_release_resident frees every copy (neorx_finalize
+ * does too).  neorx_transfer_bytes reports the bytes of tensors these entry
+ * points copied host->device and device->host since the process started. */
+int neorx_release_resident(void);
+int neorx_transfer_bytes(int64_t *h2d_bytes, int64_t *d2h_bytes);
+
 /* ---- system handle: basis, nuclei and geminal resident in HBM ---------- */
 /* Arguments are the basis and system arrays that the reference passes to
  * GAM2_IC1 (source/gam2_IC1_OMP.f:2-37); the handle keeps device copies. */
@@ -408,9 +424,14 @@ int neorx_xchf_integrals(int64_t ctx, double *enuc, double *se, double *sp, doub
  * the in-core path of xcneo_HYBRID (LG2IC1, LG3IC1, LG4IC): d_focke, d_fockp
  * receive the Fock matrices after XCfock_correction (xchf1_fock.f:728-784),
  * d_comps 14 doubles: E_total, E_nuc, E_ecore, E_pcore, E_ep, E_ee, E_gam1,
- * E_gam2, E_gam3, E_gam4, S_total, S_gam1, S_gam2, H_expect.  prec selects
- * the stored GAM_3 read (0 FP64, 1 FP32, 2 BF16); all arithmetic is FP64.
- * Enqueued on stream, not synchronised. */
+ * E_gam2, E_gam3, E_gam4, S_total, S_gam1, S_gam2, H_expect.  prec & 15
+ * selects the stored GAM_3 read (0 FP64, 1 FP32, 2 BF16); prec >> 4 the pass
+ * over it: 0 the rows of DE; 1 the same, kept as the reference rows (FP64
+ * only, prec 0x10); 2 the reference rows plus the increment
+ * T(DE - DEref, DE + DEref) read in the selected storage, where T is GAM_3's
+ * symmetric bilinear form in its two contracted densities (its storage error
+ * scales with |DE - DEref|).  All arithmetic is FP64.  Enqueued on stream,
+ * not synchronised. */
 int neorx_xchf_fock_dev(int64_t ctx, const double *d_de, const double *d_dp, int64_t prec, double *d_focke,
                         double *d_fockp, double *d_comps, void *stream);
 /* The FP64 MFMA GEMM of the factorized 5-particle contraction (no
@@ -439,7 +460,11 @@ int neorx_roothan_dev(int64_t n, const double *d_s, const double *d_f, int64_t k
  *   mode:     0 FP64 GAM_3; 1 FP32 until the switch signal < sw_lo, then
  *             FP64; 2 BF16 until < sw_hi, then FP32 until < sw_lo, then FP64
  *             (signal: the DIIS error max|X^T(FDS - SDF)X| with DIIS, else
- *             max|dDE|); convergence is only declared on an FP64 iteration;
+ *             max|dDE|); 3 (incremental) BF16 until < sw_hi, then an FP64
+ *             build taken as the reference and BF16-stored increments from
+ *             it, a new FP64 reference once < sw_lo, and an FP64 build to
+ *             confirm convergence; convergence is only declared on an FP64
+ *             (non-incremental) build;
  *   flags:    bit 1: capture one iteration as a hipGraph and replay it;
  *             bit 2: DIIS;
  *   comps_init / comps: 14 components (as neorx_xchf_fock_dev) of the first
@@ -447,7 +472,8 @@ int neorx_roothan_dev(int64_t n, const double *d_s, const double *d_f, int64_t k
  *             and of the last electronic build (the final energy);
  *   ce, cp, ev_e, ev_p: final orbitals and eigenvalues (FinalCE.dat, ...);
  *   trace:    maxit x 8 doubles per iteration: E_total, dE, max|dDE|,
- *             max|dDP|, switch signal, GAM_3 storage (0/1/2), wall ms, 0.
+ *             max|dDP|, switch signal, GAM_3 storage (0/1/2), wall ms, GAM_3
+ *             pass (0 full, 1 full kept as the reference, 2 increment).
  * Returns NEORX_E_ARG (after filling the outputs) if not converged. */
 int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst, int64_t maxit, double tole, double tolp,
                    int64_t mode, int64_t flags, double sw_hi, double sw_lo, double *comps_init, double *comps,

@@ -79,12 +79,24 @@ extern thread_local double nrx_flops;
 #define NRX_CNT(x) ((void)0)
 #endif
 
+// Largest angular momentum per primitive the engine is compiled for.  The
+// library compiles it twice: for l <= 2 (the tier kernels of nrx_lib.hip)
+// and for l <= 3 (nrx_wide.hip, the tuples that need f functions).  The
+// capacity-dependent part lives in an inline namespace per build, so the two
+// builds never share a symbol.
+#ifndef NRX_LMAX
+#define NRX_LMAX 2
+#endif
+#if NRX_LMAX == 2
+#define NRX_CAP_NS cap2
+#elif NRX_LMAX == 3
+#define NRX_CAP_NS cap3
+#else
+#error "NRX_LMAX must be 2 or 3"
+#endif
+
 namespace nrx {
 
-constexpr int NS = 5;  // per-particle polynomial capacity (degree <= 4, l <= 2 on both sides)
-constexpr int NK = 7;  // per-particle polynomial inside the kinetic operator (degree <= 6)
-constexpr int NB = 17; // Boys-order polynomial (total angular momentum <= 16)
-constexpr int NPART = 4;
 constexpr double PI = 3.141592653589793238462643383279502884;
 // pgike1 (AC_Integrals/g2.f:216) normalises its overlaps with this value
 constexpr double PI_G2 = 3.141592650;
@@ -176,13 +188,13 @@ template <int N> NRX_HD void p_mul_lin(PolyT<N> &a, double s, bool on = true)
     a.n = imin(a.n + (on ? 1 : 0), N);
 }
 
-// (y + sa)^la (y + sb)^lb with la, lb <= 2 (l <= 2 per Cartesian component
-// is checked when a system is created)
+// (y + sa)^la (y + sb)^lb with la, lb <= NRX_LMAX (the largest angular
+// momentum of a system is checked when it is created)
 template <int N> NRX_HD void p_pair(int la, double sa, int lb, double sb, PolyT<N> &r)
 {
     p_const(r, 1.0);
-    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(r, sa, i < la);
-    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(r, sb, i < lb);
+    NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(r, sa, i < la);
+    NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(r, sb, i < lb);
 }
 
 // h(s) = E[f(s + e)], e ~ N(0, var):  h_j = sum_i f_{j+2i} C(j+2i, j) (2i-1)!! var^i
@@ -431,6 +443,25 @@ template <int N> NRX_HD double close_var(const VarT<N> &V, double pic)
     return sqrt(pic) * rw * p_eval(h, V.c);
 }
 
+// system-wide inputs of a tuple evaluation
+struct SysParams {
+    int nat, ngtg;
+    double pmass;
+    const double *cat, *zan, *bcoef, *gamma;
+    const double *bt; // Boys table
+};
+
+inline namespace NRX_CAP_NS {
+
+constexpr int NS = 2 * NRX_LMAX + 1; // per-particle polynomial capacity (bra + ket powers <= 2 lmax)
+constexpr int NK = NS + 2;           // per-particle polynomial inside the kinetic operator
+constexpr int NB = 8 * NRX_LMAX + 1; // Boys-order polynomial (total angular momentum <= 8 lmax)
+constexpr int NPART = 4;
+// trip count of the kinetic operator's X^(L-2) factor loop (at most
+// NRX_LMAX - 2 factors; the l <= 2 build keeps its historical two predicated
+// steps, which never fire there)
+constexpr int KIN_LM2 = NRX_LMAX > 2 ? NRX_LMAX - 2 : 2;
+
 // Integral description: particles 0..n-1, hub = n-1, electron i linked to the
 // hub by gam[i] (bra+ket).
 struct Sys {
@@ -615,11 +646,11 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
                 F0.c[0] += on ? cf0[j] : 0.0;
                 F1.c[0] += on ? cf1[j] : 0.0;
             }
-            NRX_UNROLL for (int i = 0; i < 2; ++i) {
+            NRX_UNROLL for (int i = 0; i < KIN_LM2; ++i) {
                 p_mul_lin(F0, sB, i < L - 2);
                 p_mul_lin(F1, sB, i < L - 2);
             }
-            NRX_UNROLL for (int i = 0; i < 2; ++i) {
+            NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) {
                 p_mul_lin(F0, sA, i < la);
                 p_mul_lin(F1, sA, i < la);
             }
@@ -630,8 +661,8 @@ NRX_HD void kin_multi(const Sys &s, int a, int c, const double *gk, int nv, doub
                 // partner c's polynomial times X_c and X_c^2, X_c = y_c + sB_c
                 PolyT<FK> F2;
                 p_const(F2, 4.0 * g * g);
-                NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
-                NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
+                NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(F2, sB, i < L);
+                NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(F2, sA, i < la);
                 PolyT<FK> fc;
                 fpoly(*s.q[c], d, fc);
                 double xc = s.q[c]->sB[d];
@@ -900,11 +931,11 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
                     F0.c[0] += on ? cf0[j] : 0.0;
                     F1.c[0] += on ? cf1[j] : 0.0;
                 }
-                NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                NRX_UNROLL for (int i = 0; i < KIN_LM2; ++i) {
                     p_mul_lin(F0, sB, i < L - 2);
                     p_mul_lin(F1, sB, i < L - 2);
                 }
-                NRX_UNROLL for (int i = 0; i < 2; ++i) {
+                NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) {
                     p_mul_lin(F0, sA, i < la);
                     p_mul_lin(F1, sA, i < la);
                 }
@@ -914,8 +945,8 @@ NRX_HD void kin_pair(const Sys &s, const KinOp &o1, const KinOp &o2, double pic,
                 if (g != 0.0) {
                     PolyT<FK> F2;
                     p_const(F2, 4.0 * g * g);
-                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sB, i < L);
-                    NRX_UNROLL for (int i = 0; i < 2; ++i) p_mul_lin(F2, sA, i < la);
+                    NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(F2, sB, i < L);
+                    NRX_UNROLL for (int i = 0; i < NRX_LMAX; ++i) p_mul_lin(F2, sA, i < la);
                     PolyT<FK> fc;
                     fpoly(*s.q[c], d, fc);
                     double xc = s.q[c]->sB[d];
@@ -1116,12 +1147,6 @@ NRX_HD double coul(const Sys &s, int a, int b, double pic = PI, double *hub_ovl 
 // ------------------------------------------------------------------
 // Term assembly of one primitive tuple
 // ------------------------------------------------------------------
-struct SysParams {
-    int nat, ngtg;
-    double pmass;
-    const double *cat, *zan, *bcoef, *gamma;
-    const double *bt; // Boys table
-};
 
 NRX_HD Sys mksys(const SysParams &S, int n, const PairRec *p0, const PairRec *p1, const PairRec *p2,
                  const PairRec *p3, double g0, double g1, double g2)
@@ -1423,11 +1448,14 @@ NRX_HD void gam3int_tuple(const SysParams &S, const PairRec *e1, const PairRec *
 // Capacity tiers per tuple class: tier k evaluates with polynomial capacity
 // NL[k] and Boys-order capacity NB[k].  A tuple runs in the first tier that
 // covers its degree (tuple_degree <= NL-1) and its total angular momentum
-// (tuple_L <= NB-1: the degree in u^2 of every Coulomb polynomial); the last
-// tier covers every tuple allowed by l <= 2 (per particle and direction
-// bra+ket powers <= 4): e-p 4+4+2 and L <= 8, e-e-p 3*4+2 and L <= 12,
-// e-e-e-p 4*4 and L <= 16.  Capacities past a tuple's need only add zero
-// terms, so every tier gives the same values.
+// (tuple_L <= NB-1: the degree in u^2 of every Coulomb polynomial).
+// Capacities past a tuple's need only add zero terms, so every tier gives
+// the same values.
+#if NRX_LMAX == 2
+// The last tier covers every tuple allowed by l <= 2 (per particle and
+// direction bra+ket powers <= 4): e-p 4+4+2 and L <= 8, e-e-p 3*4+2 and
+// L <= 12, e-e-e-p 4*4 and L <= 16.  A tuple beyond it (an f function) has
+// the wide tier N: the l <= 3 build evaluates it.
 template <int CLS> struct Tier;
 template <> struct Tier<1> {
     static constexpr int N = 4;
@@ -1450,6 +1478,34 @@ template <> struct Tier<4> {
     static constexpr int NL[N] = {7, 9, 13}, NB[N] = {9, 13, 13};
     static constexpr bool KIN = false;
 };
+constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
+#else
+// One tier covering every tuple allowed by l <= 3 (bra+ket powers <= 6 per
+// particle and direction): e-p 6+6+2 and L <= 12, e-e-p 3*6+2 and L <= 18,
+// e-e-e-p 4*6 and L <= 24, RXCHF-ne 3*6 and L <= 18.
+template <int CLS> struct Tier;
+template <> struct Tier<1> {
+    static constexpr int N = 1;
+    static constexpr int NL[N] = {15}, NB[N] = {13};
+    static constexpr bool KIN = true;
+};
+template <> struct Tier<2> {
+    static constexpr int N = 1;
+    static constexpr int NL[N] = {21}, NB[N] = {19};
+    static constexpr bool KIN = true;
+};
+template <> struct Tier<3> {
+    static constexpr int N = 1;
+    static constexpr int NL[N] = {25}, NB[N] = {25};
+    static constexpr bool KIN = false;
+};
+template <> struct Tier<4> {
+    static constexpr int N = 1;
+    static constexpr int NL[N] = {19}, NB[N] = {19};
+    static constexpr bool KIN = false;
+};
+constexpr int NL_EE = 13; // two-electron repulsion only: 6+6
+#endif
 // class 5: the RXCHFmult e-e-p tuple (two bases, kinetic terms as class 2);
 // class 6: the RXCHFmult e-e-e-p interaction tuple (Coulomb and overlap: 4*4);
 // class 7: the RXCHFmult XCHF e-e-e-p tuple (as class 3 without gVEE);
@@ -1458,7 +1514,6 @@ template <> struct Tier<5> : Tier<2> {};
 template <> struct Tier<6> : Tier<3> {};
 template <> struct Tier<7> : Tier<3> {};
 template <> struct Tier<8> : Tier<1> {};
-constexpr int NL_EE = 9; // two-electron repulsion only: 4+4
 
 // electrons per tuple of a class
 template <int CLS> constexpr int nel()
@@ -1530,6 +1585,14 @@ NRX_HD int tuple_L(const PairRec *const *q, int n)
 template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
 {
     using T = Tier<CLS>;
+    // a Cartesian power above this build's l needs the other build
+    int lc = 0;
+    NRX_UNROLL for (int i = 0; i < nel<CLS>() + 1; ++i)
+        NRX_UNROLL for (int d = 0; d < 3; ++d) {
+            const int m = q[i]->la[d] > q[i]->lb[d] ? q[i]->la[d] : q[i]->lb[d];
+            lc = m > lc ? m : lc;
+        }
+    if (lc > NRX_LMAX) return T::N;
     int deg = tuple_degree(q, nel<CLS>() + 1, T::KIN), L = tuple_L(q, nel<CLS>() + 1);
     if constexpr (CLS == 3 || CLS == 7) {
         NRX_UNROLL for (int d = 0; d < 3; ++d) {
@@ -1537,13 +1600,14 @@ template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
             deg = k > deg ? k : deg;
         }
     }
-    for (int k = 0; k + 1 < T::N; ++k)
+    for (int k = 0; k < T::N; ++k)
         if (deg <= T::NL[k] - 1 && L <= T::NB[k] - 1) return k;
-    return T::N - 1;
+    return T::N; // beyond this build's capacities
 }
 
 // the general tier of a class (covers every tuple)
 template <int CLS> constexpr int tier_nl() { return Tier<CLS>::NL[Tier<CLS>::N - 1]; }
 template <int CLS> constexpr int tier_nb() { return Tier<CLS>::NB[Tier<CLS>::N - 1]; }
 
+} // inline namespace NRX_CAP_NS
 } // namespace nrx

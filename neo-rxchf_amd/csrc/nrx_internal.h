@@ -50,6 +50,7 @@ struct DevSys {
     const i64 *eord, *pord;   // enumeration order -> original primitive index
     const PairRec *ep, *pp;   // bra*ket tables [i][j] over primitives (npebf^2, npbf^2)
     const double *bt;         // Boys table (BT_N x BT_M)
+    int lmax;                 // largest primitive angular momentum (3: f functions present)
 };
 
 struct System {
@@ -61,6 +62,21 @@ struct System {
 };
 
 NRX_HIDDEN System *get_sys(i64 h);
+
+// ------------------------------------------------------------------
+// the l <= 3 build of the engine (nrx_wide.hip)
+// ------------------------------------------------------------------
+// Tuples beyond the l <= 2 capacities (an f function on some particle) are
+// binned by the tier kernels into the wide bin wbin of a launch's sorted
+// list; this evaluates that bin (class cls, tuple source src) on stream st.
+// n bounds the bin's size (the launch's tuple count).
+NRX_HIDDEN int nrx_wide_prims(int cls, int src, const DevSys &d, i64 start, const i64 *idx, double *oe, double *os,
+                              double *ox, double *oy, const unsigned *list, const unsigned *counts, int wbin, i64 n,
+                              hipStream_t st);
+// GAM1_OMP_MD (cls 1) / RXCHFmult_GAM1_MD (cls 8) and calc_GAM_ee at l <= 3
+// capacities
+NRX_HIDDEN int nrx_wide_gam1(int cls, const DevSys &d, double *G, double *Gs, hipStream_t st);
+NRX_HIDDEN int nrx_wide_gam_ee(const DevSys &d, double *G, hipStream_t st);
 
 // most blocks of a grid-stride launch (NEORX_GRID_CAP lowers it)
 NRX_HIDDEN extern i64 g_grid_cap;
@@ -76,9 +92,10 @@ static inline unsigned grid_blocks(i64 n, int bs) { return (unsigned)std::min<i6
 // device buffer released when it goes out of scope
 struct DevBuf {
     void *p = nullptr;
+    bool own = true; // false: a resident tensor's memory, borrowed (not freed)
     ~DevBuf()
     {
-        if (p) (void)hipFree(p);
+        if (p && own) (void)hipFree(p);
     }
     int alloc(size_t bytes)
     {
@@ -108,6 +125,11 @@ NRX_HIDDEN int nrx_gam1_launch(System *s, double *d_g1, double *d_g1s, hipStream
 // g of storage type dtype
 NRX_HIDDEN int nrx_eg3_slabs(i64 ne, i64 nblk, const double *d_de, const void *g, int dtype, double *part,
                              hipStream_t st);
+// the same rows as the symmetric bilinear form T(X, Y) of two densities (X on
+// the (i2,j2) pair, Y on the (i3,j3) pair; T(DE, DE) are EG3IC1's rows), and
+// part[blk][ne] = sum_x Z(i1,x) F[x] / 3, or 0 when d_z is null
+NRX_HIDDEN int nrx_eg3_slabs2(i64 ne, i64 nblk, const double *d_x, const double *d_y, const double *d_z, const void *g,
+                              int dtype, double *part, hipStream_t st);
 // EG2IC1 slab partials of nblk slabs (ip,jp,i1): part[blk][4][ne] = dir, ex,
 // dirS, exS (gs may be NULL: the S rows stay zero)
 NRX_HIDDEN int nrx_eg2_slabs(i64 ne, i64 nblk, const double *d_de, const double *g, const double *gs, double *part,

@@ -9,12 +9,14 @@
 //     wavefront share loop trip counts;
 //   * contracted tensors use the reference packings (C row-major
 //     [ip][jp][ie1][je1]...).
+#include <atomic>
 #include <chrono>
 #include <map>
 #include <mutex>
 #include <numeric>
 
 #include "nrx_internal.h"
+#include "nrx_tuples.h"
 
 // ------------------------------------------------------------------
 // error handling
@@ -119,6 +121,7 @@ extern "C" int neorx_init(int64_t device)
 
 extern "C" int neorx_finalize(void)
 {
+    (void)neorx_release_resident();
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto &kv : g_sys) {
         for (void *p : kv.second->allocs) (void)hipFree(p);
@@ -177,8 +180,8 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
             if (nucam[d * npbf + i] < 0) return fail(NEORX_E_ARG, "negative angular momentum");
         maxl = std::max(maxl, (int)(nucam[i] + nucam[npbf + i] + nucam[2 * npbf + i]));
     }
-    if (maxl > 2)
-        return fail(NEORX_E_UNSUPPORTED, "angular momentum l=%d > 2 is not supported by this build", maxl);
+    if (maxl > 3)
+        return fail(NEORX_E_UNSUPPORTED, "angular momentum l=%d > 3 is not supported by this build", maxl);
     for (i64 c = 0; c < nebf; ++c)
         if (kpestr[c] < 1 || kpeend[c] < kpestr[c] || kpeend[c] > npebf)
             return fail(NEORX_E_ARG, "KPESTR/KPEEND out of range for contracted function %lld", (long long)c + 1);
@@ -198,6 +201,7 @@ extern "C" int neorx_system_create(int64_t nebf, int64_t npebf, int64_t npbf, in
 
     System *s = new System();
     s->maxl = maxl;
+    s->d.lmax = maxl;
     s->kpestr.assign(kpestr, kpestr + nebf);
     DevSys &d = s->d;
     d.nebf = nebf;
@@ -272,97 +276,6 @@ extern "C" int neorx_system_order(int64_t handle, int64_t *eorder, int64_t *pord
     return NEORX_OK;
 }
 
-// ------------------------------------------------------------------
-// device helpers
-// ------------------------------------------------------------------
-NRX_DI SysParams d_params(const DevSys &S)
-{
-    SysParams P;
-    P.nat = (int)S.nat;
-    P.ngtg = (int)S.ngtg;
-    P.pmass = S.pmass;
-    P.cat = S.cat;
-    P.zan = S.zan;
-    P.bcoef = S.bcoef;
-    P.gamma = S.gamma;
-    P.bt = S.bt;
-    return P;
-}
-
-// pair records of one tuple given original primitive indices
-// (bra e..., ip, ket e..., jp)
-template <int CLS> NRX_DI void tuple_pairs(const DevSys &S, const i64 *x, const PairRec **q)
-{
-    constexpr int E = nel<CLS>();
-    for (int i = 0; i < E; ++i) q[i] = S.ep + x[i] * S.npebf + x[E + 1 + i];
-    q[E] = S.pp + x[E] * S.npbf + x[2 * E + 1];
-}
-
-template <int CLS, int NL, int NBC> NRX_DI void eval_tuple(const DevSys &S, const i64 *x, double &e, double &s)
-{
-    const PairRec *q[nel<CLS>() + 1];
-    tuple_pairs<CLS>(S, x, q);
-    tuple_eval<CLS, NL, NBC>(d_params(S), q, e, s);
-}
-
-// index tuple t of a list, validated against the basis sizes
-template <int CLS> NRX_DI bool load_tuple(const DevSys &S, const i64 *idx, i64 t, i64 *x)
-{
-    bool ok = true;
-    constexpr int E = nel<CLS>();
-    for (int k = 0; k < 2 * E + 2; ++k) {
-        x[k] = idx[t * (2 * E + 2) + k];
-        i64 lim = (k == E || k == 2 * E + 1) ? S.npbf : S.npebf;
-        ok = ok && x[k] >= 0 && x[k] < lim;
-    }
-    return ok;
-}
-
-// the RXCHFmult classes whose electrons come from two bases
-template <int CLS> constexpr bool two_basis() { return CLS == 5 || CLS == 6; }
-
-// tuple r of the enumeration (ip,jp,ie1,je1,...,ieN,jeN), last index
-// fastest, in the system's sorted primitive order (SORTED) or in the
-// caller's original order.  The two-basis classes (5, 6) enumerate e1 over
-// the first basis and the other electrons over the special-electron basis,
-// in original order only.
-template <int CLS, bool SORTED> NRX_DI void range_tuple(const DevSys &S, i64 r, i64 *x)
-{
-    static_assert(!two_basis<CLS>() || !SORTED, "two-basis classes are enumerated in original order");
-    i64 np = S.npbf;
-    constexpr int CLS_E = nel<CLS>();
-    for (int i = CLS_E - 1; i >= 0; --i) {
-        i64 ne = S.npebf, off = 0;
-        if constexpr (two_basis<CLS>()) {
-            ne = i == 0 ? S.npebf1 : S.npebf - S.npebf1;
-            off = i == 0 ? 0 : S.npebf1;
-        }
-        i64 k = r % ne;
-        x[CLS_E + 1 + i] = SORTED ? S.eord[k] : off + k;
-        r /= ne;
-        k = r % ne;
-        x[i] = SORTED ? S.eord[k] : off + k;
-        r /= ne;
-    }
-    i64 k = r % np;
-    x[2 * CLS_E + 1] = SORTED ? S.pord[k] : k;
-    r /= np;
-    k = r % np;
-    x[CLS_E] = SORTED ? S.pord[k] : k;
-}
-
-// Tuple sources: a caller's index list, or a range of the enumeration in
-// sorted or in original primitive order.
-enum { SRC_LIST = 0, SRC_SORTED = 1, SRC_ORIG = 2 };
-
-template <int CLS, int SRC> NRX_DI void src_tuple(const DevSys &S, const i64 *idx, i64 start, i64 t, i64 *x)
-{
-    if constexpr (SRC == SRC_LIST)
-        load_tuple<CLS>(S, idx, t, x);
-    else
-        range_tuple<CLS, SRC == SRC_SORTED>(S, start + t, x);
-}
-
 // Tuples of a launch are sorted by bin, a counting sort in two passes:
 // k_prim_bin gives each tuple its bin (capacity tier tuple_tier, and for the
 // classes whose Coulomb terms take the hub-cycle path also the proton pair's
@@ -379,6 +292,7 @@ template <int CLS> constexpr int n_bins() { return Tier<CLS>::N * hub_bins<CLS>(
 template <int CLS> NRX_DI int tuple_bin(const PairRec *const *q)
 {
     int k = tuple_tier<CLS>(q);
+    if (k == Tier<CLS>::N) return n_bins<CLS>(); // the wide bin (l <= 3 build)
     if constexpr (hub_bins<CLS>() == 1) {
         return k;
     } else {
@@ -421,15 +335,7 @@ __global__ void __launch_bounds__(256) k_prim_bin(DevSys S, i64 start, i64 ntup,
     }
     if (bin >= 0) atomicAdd(&lc[bin], 1u);
     __syncthreads();
-    if (threadIdx.x < n_bins<CLS>() && lc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], lc[threadIdx.x]);
-}
-
-// offset of bin b in the sorted list: the counts of the bins before it
-NRX_DI unsigned bin_offset(const unsigned *counts, int b)
-{
-    unsigned o = 0;
-    for (int j = 0; j < b; ++j) o += counts[j];
-    return o;
+    if (threadIdx.x < n_bins<CLS>() + 1 && lc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], lc[threadIdx.x]);
 }
 
 template <int CLS>
@@ -449,7 +355,7 @@ __global__ void __launch_bounds__(256) k_prim_scatter(i64 ntup, const unsigned c
     unsigned loc = 0;
     if (bin >= 0) loc = atomicAdd(&lc[bin], 1u); // rank within the block's share of the bin
     __syncthreads();
-    if (threadIdx.x < n_bins<CLS>())
+    if (threadIdx.x < n_bins<CLS>() + 1)
         lbase[threadIdx.x] = bin_offset(counts, threadIdx.x) +
                              (lc[threadIdx.x] ? atomicAdd(&cursors[threadIdx.x], lc[threadIdx.x]) : 0u);
     __syncthreads();
@@ -572,7 +478,7 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
                         double *ox = nullptr, double *oy = nullptr)
 {
     constexpr int NBN = n_bins<CLS>();
-    static_assert(NBN <= 32, "bin counters and cursors fit the 256-byte header");
+    static_assert(NBN + 1 <= 32, "bin counters (with the wide bin) and cursors fit the 256-byte header");
     if constexpr (two_basis<CLS>())
         if (SRC != SRC_LIST && (d.npebf1 <= 0 || d.npebf1 >= d.npebf))
             return fail(NEORX_E_ARG, "class %d enumerates two electron bases; this handle has one", CLS);
@@ -598,6 +504,10 @@ static int launch_prims(const DevSys &d, i64 start, i64 ntup, const i64 *idx, do
         hipLaunchKernelGGL((k_prim_tiers<CLS, SRC>), dim3(nblocks(n, TIER_BS) + NBN), dim3(TIER_BS), 0, st, d, start + off, ix,
                            e, s, x, y, list, counts);
         HIPCHK(hipGetLastError());
+        if (d.lmax > 2) { // tuples with f functions: the wide bin
+            rc = nrx_wide_prims(CLS, SRC, d, start + off, ix, e, s, x, y, list, counts, NBN, n, st);
+            if (rc) return rc;
+        }
     }
     return NEORX_OK;
 }
@@ -659,30 +569,6 @@ extern "C" int neorx_prim_range_dev(int64_t handle, int64_t cls, int64_t start, 
 // ------------------------------------------------------------------
 // contracted builders
 // ------------------------------------------------------------------
-// GAM1_OMP_MD: one lane per contracted (ip,jp,iec,jec), primitive sums in
-// the reference's loop order, at the general tier of the class.
-// CLS 8: RXCHFmult_GAM1_MD (rxchfmult/rxchfmult_gam1.f:2-398), same loops.
-template <int CLS>
-__global__ void __launch_bounds__(256) k_gam1(DevSys S, double *__restrict__ G, double *__restrict__ Gs)
-{
-    i64 ne = S.nebf, np = S.npbf;
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne * ne * np * np) return;
-    i64 jc = t % ne, ic = (t / ne) % ne, jp = (t / (ne * ne)) % np, ip = t / (ne * ne * np);
-    double acc = 0.0, accs = 0.0;
-    for (i64 ie = S.kpestr[ic] - 1; ie < S.kpeend[ic]; ++ie)
-        for (i64 je = S.kpestr[jc] - 1; je < S.kpeend[jc]; ++je) {
-            i64 x[4] = {ie, ip, je, jp};
-            double e, s;
-            eval_tuple<CLS, tier_nl<CLS>(), tier_nb<CLS>()>(S, x, e, s);
-            double cf = S.agnbfcc[ip] * S.agnbfcc[jp] * S.agebfcc[ie] * S.agebfcc[je];
-            acc += e * cf;
-            accs += s * cf;
-        }
-    G[t] = acc;
-    Gs[t] = accs;
-}
-
 // Contracted tuples of GAM2_IC1 / GAM3_IC1 are built in two steps: the
 // primitive kernels above fill a slab of primitive values in original order,
 // then a contraction kernel sums each contracted tuple over its primitives
@@ -786,27 +672,6 @@ __global__ void __launch_bounds__(256) k_gam3_symm(i64 ne, i64 n, const double *
     }
 }
 
-// calc_GAM_ee: lane per contracted quadruple, primitive sums in the
-// reference's loop order (ie1, je1, ie2, je2)
-__global__ void __launch_bounds__(256) k_gam_ee(DevSys S, double *__restrict__ G)
-{
-    i64 ne = S.nebf;
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne * ne * ne * ne) return;
-    i64 j2 = t % ne, i2 = (t / ne) % ne, j1 = (t / (ne * ne)) % ne, i1 = t / (ne * ne * ne);
-    SysParams P = d_params(S);
-    double acc = 0.0;
-    for (i64 a1 = S.kpestr[i1] - 1; a1 < S.kpeend[i1]; ++a1)
-        for (i64 c1 = S.kpestr[j1] - 1; c1 < S.kpeend[j1]; ++c1)
-            for (i64 a2 = S.kpestr[i2] - 1; a2 < S.kpeend[i2]; ++a2)
-                for (i64 c2 = S.kpestr[j2] - 1; c2 < S.kpeend[j2]; ++c2) {
-                    const PairRec *q0 = S.ep + a1 * S.npebf + c1, *q1 = S.ep + a2 * S.npebf + c2;
-                    Sys s = mksys(P, 2, q0, q1, 0, 0, 0.0, 0, 0);
-                    acc += coul<NL_EE>(s, 0, 1) * S.agebfcc[a1] * S.agebfcc[c1] * S.agebfcc[a2] * S.agebfcc[c2];
-                }
-    G[t] = acc;
-}
-
 // GAM4_ICR / symm_gam4 / calc_GAM_4_integral (gam_4_OMP.f:712-1192)
 __constant__ int c_perm4[24][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {2, 0, 1, 3}, {0, 2, 1, 3}, {1, 2, 0, 3}, {2, 1, 0, 3},
                                    {3, 1, 0, 2}, {1, 3, 0, 2}, {0, 3, 1, 2}, {3, 0, 1, 2}, {1, 0, 3, 2}, {0, 1, 3, 2},
@@ -866,11 +731,124 @@ __global__ void __launch_bounds__(256) k_gam4(i64 ne, i64 np, i64 off, i64 n, co
 // host helpers for builders
 // ------------------------------------------------------------------
 
+// ------------------------------------------------------------------
+// Resident tensors.  The host-pointer builders (GAM2_IC1, GAM3_IC1,
+// GAM4_ICR) fill the caller's array and keep their device copy, keyed by the
+// host array's address and size; a later host-pointer contraction given the
+// same array reads that copy instead of copying the tensor over PCIe again
+// (C1's GM3 is 97.8 GB).  The caller's array may have changed in between:
+// 4096 entries sampled over it are compared bit for bit with the values kept
+// at registration, and any difference drops the copy.  Resident copies use
+// at most 60 % of the device memory (the oldest are released first);
+// NEORX_RESIDENT=0 turns the cache off; neorx_release_resident frees it.
+// ------------------------------------------------------------------
+namespace {
+struct Resident {
+    const void *host;
+    i64 n; // doubles
+    void *dev;
+    std::vector<double> probe;
+};
+std::mutex g_res_mu;
+std::vector<Resident> g_res; // oldest first
+std::atomic<long long> g_h2d_bytes{0}, g_d2h_bytes{0};
+constexpr i64 RES_PROBES = 4096;
+
+i64 probe_pos(i64 n, i64 k) { return n <= RES_PROBES ? k : (k * (n - 1)) / (RES_PROBES - 1); }
+
+std::vector<double> take_probe(const double *h, i64 n)
+{
+    const i64 m = std::min(n, RES_PROBES);
+    std::vector<double> v(m);
+    for (i64 k = 0; k < m; ++k) v[k] = h[probe_pos(n, k)];
+    return v;
+}
+
+bool resident_on()
+{
+    const char *e = getenv("NEORX_RESIDENT");
+    return !(e && atoi(e) == 0);
+}
+
+void res_drop_locked(size_t i)
+{
+    (void)hipFree(g_res[i].dev);
+    g_res.erase(g_res.begin() + (long)i);
+}
+
+// device copy kept for host array h of n doubles, or null
+const double *res_find(const double *h, i64 n)
+{
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    for (size_t i = 0; i < g_res.size(); ++i) {
+        Resident &r = g_res[i];
+        if (r.host != (const void *)h) continue;
+        if (r.n != n) return nullptr;
+        const i64 m = (i64)r.probe.size();
+        for (i64 k = 0; k < m; ++k)
+            if (__builtin_memcmp(&r.probe[k], &h[probe_pos(n, k)], 8) != 0) {
+                res_drop_locked(i); // the caller changed its array
+                return nullptr;
+            }
+        return (const double *)r.dev;
+    }
+    return nullptr;
+}
+
+// keep the device buffer b (n doubles, equal to host array h) as h's resident
+// copy; b gives up its memory either way
+void res_keep(DevBuf &b, const double *h, i64 n)
+{
+    if (!b.p || !b.own) return;
+    if (!resident_on()) return;
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    for (size_t i = 0; i < g_res.size(); ++i)
+        if (g_res[i].host == (const void *)h) {
+            res_drop_locked(i);
+            break;
+        }
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return;
+    const long double cap = 0.6L * (long double)total;
+    long double held = 0.0L;
+    for (auto &r : g_res) held += (long double)r.n * 8;
+    while (!g_res.empty() && held + (long double)n * 8 > cap) {
+        held -= (long double)g_res.front().n * 8;
+        res_drop_locked(0);
+    }
+    if (held + (long double)n * 8 > cap) return; // larger than the budget: not kept
+    g_res.push_back(Resident{h, n, b.p, take_probe(h, n)});
+    b.p = nullptr;
+}
+} // namespace
+
+extern "C" int neorx_release_resident(void)
+{
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    while (!g_res.empty()) res_drop_locked(g_res.size() - 1);
+    return NEORX_OK;
+}
+
+extern "C" int neorx_transfer_bytes(int64_t *h2d_bytes, int64_t *d2h_bytes)
+{
+    if (h2d_bytes) *h2d_bytes = g_h2d_bytes.load();
+    if (d2h_bytes) *d2h_bytes = g_d2h_bytes.load();
+    return NEORX_OK;
+}
+
+// input tensor h (n doubles) on the device: its resident copy (borrowed by b)
+// or a fresh copy over PCIe
 static int h2d(DevBuf &b, const double *h, i64 n)
 {
+    if (const double *r = res_find(h, n)) {
+        b.p = (void *)r;
+        b.own = false;
+        return NEORX_OK;
+    }
     int rc = b.alloc(n * 8);
     if (rc) return rc;
     HIPCHK(hipMemcpy(b.p, h, n * 8, hipMemcpyHostToDevice));
+    g_h2d_bytes += n * 8;
     return NEORX_OK;
 }
 
@@ -1018,6 +996,7 @@ extern "C" int neorx_gam3_ic1_dev(int64_t handle, double *d_gm3, void *stream)
 static int copy_out(double *h, const double *d, i64 n)
 {
     HIPCHK(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    g_d2h_bytes += n * 8;
     return NEORX_OK;
 }
 
@@ -1030,7 +1009,12 @@ template <int CLS> static int gam1_host(int64_t handle, double *gam1, double *ga
     int rc = G.alloc(n * 8);
     if (!rc) rc = Gs.alloc(n * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gam1<CLS>, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d(), Gs.d());
+    if (s->maxl > 2) {
+        rc = nrx_wide_gam1(CLS, s->d, G.d(), Gs.d(), 0);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL((k_gam1<CLS, 2>), dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d(), Gs.d());
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     rc = copy_out(gam1, G.d(), n);
@@ -1051,6 +1035,10 @@ extern "C" int neorx_gam2_ic1(int64_t handle, double *gm2icr, double *gm2sicr)
     if (!rc) rc = neorx_gam2_ic1_dev(handle, G.d(), Gs.d(), 0);
     if (!rc) rc = copy_out(gm2icr, G.d(), n);
     if (!rc) rc = copy_out(gm2sicr, Gs.d(), n);
+    if (!rc) {
+        res_keep(G, gm2icr, n);
+        res_keep(Gs, gm2sicr, n);
+    }
     return rc;
 }
 
@@ -1063,6 +1051,7 @@ extern "C" int neorx_gam3_ic1(int64_t handle, double *gm3icr)
     int rc = G.alloc(n * 8);
     if (!rc) rc = neorx_gam3_ic1_dev(handle, G.d(), 0);
     if (!rc) rc = copy_out(gm3icr, G.d(), n);
+    if (!rc) res_keep(G, gm3icr, n);
     return rc;
 }
 
@@ -1074,8 +1063,8 @@ extern "C" int neorx_gam_ee(int64_t handle, double *gam_ee)
     DevBuf G;
     int rc = G.alloc(n * 8);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gam_ee, dim3(nblocks(n, 256)), dim3(256), 0, 0, s->d, G.d());
-    HIPCHK(hipGetLastError());
+    rc = nrx_gam_ee_launch(s, G.d(), 0);
+    if (rc) return rc;
     HIPCHK(hipDeviceSynchronize());
     return copy_out(gam_ee, G.d(), n);
 }
@@ -1083,7 +1072,8 @@ extern "C" int neorx_gam_ee(int64_t handle, double *gam_ee)
 int nrx_gam_ee_launch(System *s, double *d_gee, hipStream_t st)
 {
     i64 ne = s->d.nebf, n = ne * ne * ne * ne;
-    hipLaunchKernelGGL(k_gam_ee, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_gee);
+    if (s->maxl > 2) return nrx_wide_gam_ee(s->d, d_gee, st);
+    hipLaunchKernelGGL(k_gam_ee<2>, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_gee);
     HIPCHK(hipGetLastError());
     return NEORX_OK;
 }
@@ -1091,7 +1081,8 @@ int nrx_gam_ee_launch(System *s, double *d_gee, hipStream_t st)
 int nrx_gam1_launch(System *s, double *d_g1, double *d_g1s, hipStream_t st)
 {
     i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * np * np;
-    hipLaunchKernelGGL(k_gam1<1>, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_g1, d_g1s);
+    if (s->maxl > 2) return nrx_wide_gam1(1, s->d, d_g1, d_g1s, st);
+    hipLaunchKernelGGL((k_gam1<1, 2>), dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, d_g1, d_g1s);
     HIPCHK(hipGetLastError());
     return NEORX_OK;
 }
@@ -1104,19 +1095,19 @@ extern "C" int neorx_gam4_icr(int64_t nebf, int64_t npbf, const double *gam_2s, 
     if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
     i64 ng2 = ne * ne * ne * ne * np * np, ngee = ne * ne * ne * ne, ng4 = ngee * ngee * np * np;
     DevBuf A, B, C;
-    rc = A.alloc(ng2 * 8);
-    if (!rc) rc = B.alloc(ngee * 8);
+    rc = h2d(A, gam_2s, ng2); // GM2sICR of GAM2_IC1: resident when it built it
+    if (!rc) rc = h2d(B, gam_ee, ngee);
     if (!rc) rc = C.alloc(ng4 * 8);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(A.p, gam_2s, ng2 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B.p, gam_ee, ngee * 8, hipMemcpyHostToDevice));
     for (i64 off = 0; off < ng4; off += CHUNK) {
         i64 n = std::min(CHUNK, ng4 - off);
         hipLaunchKernelGGL(k_gam4, dim3(nblocks(n, 256)), dim3(256), 0, 0, ne, np, off, n, A.d(), B.d(), C.d());
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipDeviceSynchronize());
-    return copy_out(gm4icr, C.d(), ng4);
+    rc = copy_out(gm4icr, C.d(), ng4);
+    if (!rc) res_keep(C, gm4icr, ng4);
+    return rc;
 }
 
 // ------------------------------------------------------------------
@@ -1357,8 +1348,6 @@ extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const 
     DevBuf dDE, dDP, dG, dGs, dFE, dSE, dFP, dSP, dE;
     rc = dDE.alloc(ne * ne * 8);
     if (!rc) rc = dDP.alloc(np * np * 8);
-    if (!rc) rc = dG.alloc(ng2 * 8);
-    if (!rc) rc = dGs.alloc(ng2 * 8);
     if (!rc) rc = dFE.alloc(ne * ne * 8);
     if (!rc) rc = dSE.alloc(ne * ne * 8);
     if (!rc) rc = dFP.alloc(np * np * 8);
@@ -1367,8 +1356,9 @@ extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const 
     if (rc) return rc;
     HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dG.p, gm2icr, ng2 * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dGs.p, gm2sicr, ng2 * 8, hipMemcpyHostToDevice));
+    if (!rc) rc = h2d(dG, gm2icr, ng2);
+    if (!rc) rc = h2d(dGs, gm2sicr, ng2);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(dFE.p, fe, ne * ne * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dFP.p, fp, np * np * 8, hipMemcpyHostToDevice));
     rc = neorx_eg2ic1_dev(ne, np, dDE.d(), dDP.d(), dG.d(), dGs.d(), dFE.d(), dSE.d(), dFP.d(), dSP.d(), dE.d(),
@@ -1388,19 +1378,28 @@ extern "C" int neorx_eg2ic1(int64_t nebf, int64_t npbf, const double *de, const 
 
 // EG3IC1 (thread_EG3ICR, gam3_IC1_OMP.f:537-725).  Block = slab (ip,jp,i1=a)
 // of GM3, entries g = G[a][b][c][d][e][f] (i1, j-slot 1, i2, j-slot 2, i3,
-// j-slot 3).  thread_EG3ICR's six ket-permuted reads, weighted
-// (8,-4,2,-4,2,-4)/8 and times 3 for FE, become, per stored entry,
-//   F[b] += 3 ( DE(c,d)DE(e,f) - 1/2 DE(c,f)DE(e,d) ) g
-//   F[d] += 3 (-1/2 DE(c,b)DE(e,f) + 1/4 DE(c,f)DE(e,b) ) g
-//   F[f] += 3 ( 1/4 DE(c,b)DE(e,d) - 1/2 DE(c,d)DE(e,b) ) g
-// (F = FE row a before the DP(ip,jp) weight).  Each thread owns one (e,f)
-// column pair, so every read of the slab is a contiguous row of ne^2 values;
-// the sums over c run in registers.  F[b] and F[d] are wave-reduced and added
-// by lane 0 into its wavefront's own LDS row; F[f] is kept per (e,f) column;
-// the rows and columns are summed in a fixed order at the end, so the result
-// does not depend on wavefront timing.  T is the storage type of GM3 (double,
-// or float for the mixed-precision path); all arithmetic is FP64.
-// part[blk][0..ne-1] = F, part[blk][ne] = sum_x DE(a,x) F[x] / 3.
+// j-slot 3), stored [a][b][c][d][e][f] with (e,f) fastest.  thread_EG3ICR's
+// six ket-permuted reads, weighted (8,-4,2,-4,2,-4)/8 and times 3 for FE,
+// become, per stored entry,
+//   F[b] += 3 ( X(c,d)Y(e,f) - 1/2 X(c,f)Y(e,d) ) g
+//   F[d] += 3 (-1/2 X(c,b)Y(e,f) + 1/4 X(c,f)Y(e,b) ) g
+//   F[f] += 3 ( 1/4 X(c,b)Y(e,d) - 1/2 X(c,d)Y(e,b) ) g
+// with X = Y = DE for EG3IC1 (F = FE row a before the DP(ip,jp) weight).
+// Because GM3 is symmetric under exchanging its electron pairs (c,d) and
+// (e,f), the rows are a symmetric bilinear form T(X,Y) of the two densities,
+// so the difference of two Fock builds is one pass:
+//   T(D,D) - T(D0,D0) = T(D - D0, D + D0)
+// (the incremental build of the mixed-precision SCF).
+// Each thread owns one (e,f) column pair, so every read of the slab is a
+// contiguous row of ne^2 values whose base is the same for the whole
+// wavefront (a scalar base plus the lane's 32-bit offset); the sums over c run
+// in registers.  F[b] and F[d] are wave-reduced and added by lane 0 into its
+// wavefront's own LDS row; F[f] is kept per (e,f) column; the rows and columns
+// are summed in a fixed order at the end, so the result does not depend on
+// wavefront timing.  T is the storage type of GM3 (double, float or BF16);
+// all arithmetic is FP64.
+// part[blk][0..ne-1] = F, part[blk][ne] = sum_x Z(a,x) F[x] / 3 (0 when Z is
+// null).
 // BF16 storage: the upper 16 bits of the FP32 value (round to nearest even)
 struct bf16_t {
     uint16_t bits;
@@ -1417,6 +1416,29 @@ NRX_DI double widen(unsigned short b) { return (double)__uint_as_float((unsigned
 template <class T> NRX_DI double ld_entry(const T *p) { return widen(ld_raw(p)); }
 template <class T> using raw_t = decltype(ld_raw((const T *)nullptr));
 
+// Raw buffer loads of stored entries: a buffer resource over one run of the
+// tensor (base and byte size, in scalar registers), a per-lane byte offset
+// and a wave-uniform byte offset that the scalar unit computes, so a row
+// address costs no vector instruction.  Aux bit 1 marks the load
+// nontemporal.  Bytes past the resource's size read as zero.
+constexpr int BUF_WORD3 = 0x00020000; // raw (untyped) buffer, gfx9 family
+template <class T> NRX_DI __amdgpu_buffer_rsrc_t buf_rsrc(const T *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, BUF_WORD3);
+}
+NRX_DI double buf_ld(const double *, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+NRX_DI float buf_ld(const float *, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+NRX_DI unsigned short buf_ld(const bf16_t *, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 2);
+}
+
 // Compiler barrier between a batch of independent loads and their first
 // use: memory operations cannot move across it, so every load of the batch
 // is issued before the widening and FMAs that consume them (left to itself
@@ -1429,98 +1451,138 @@ template <class T> using raw_t = decltype(ld_raw((const T *)nullptr));
 template <class T> constexpr int eg3_unroll() { return sizeof(T) == 4 ? 12 : 16; }
 
 // most lanes of one EG3 slab block (its launch bound)
-constexpr unsigned EG3_MAX_THREADS = 256;
+constexpr unsigned EG3_MAX_THREADS = 512;
 
-template <class T, int U = eg3_unroll<T>()>
-__global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const double *__restrict__ DE,
+// P adjacent (e,f) columns of one stored row in one buffer load: P = 1 for
+// every storage type, P = 2 for BF16 (a 4-byte load, so a wavefront keeps
+// twice the bytes in flight per load; the lower address is the low half)
+template <class T, int P> struct Pack;
+template <class T> struct Pack<T, 1> {
+    using raw = raw_t<T>;
+    static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+    {
+        return buf_ld((const T *)nullptr, r, voff, soff);
+    }
+    static NRX_DI double get(raw v, int) { return widen(v); }
+};
+template <> struct Pack<bf16_t, 2> {
+    using raw = unsigned;
+    static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2);
+    }
+    static NRX_DI double get(raw v, int p) { return (double)__uint_as_float(p ? (v & 0xffff0000u) : (v << 16)); }
+};
+
+// R rows of the slab, c = c0 .. c0+R-1, for one (b,d) and the lane's P
+// columns: raw loads first (the buffer covers row block b, each row's byte
+// offset is wave-uniform), then the widening and three FMAs per entry
+template <class T, int R, int P>
+NRX_DI void eg3_rows(__amdgpu_buffer_rsrc_t rb, unsigned row_d, unsigned row_c, unsigned voff, unsigned c0,
+                     const double *__restrict__ xd, const double *__restrict__ xb, const double *const *xf,
+                     double *s1, double *s2, double *s3)
+{
+    typename Pack<T, P>::raw r[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) r[k] = Pack<T, P>::ld(rb, voff, row_d + (c0 + k) * row_c);
+    NRX_LOADS_ISSUED();
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double gv = Pack<T, P>::get(r[k], p);
+            s1[p] += xd[c0 + k] * gv;      // X(c,d)
+            s2[p] += xb[c0 + k] * gv;      // X(c,b)
+            s3[p] += xf[p][c0 + k] * gv;   // X(c,f)
+        }
+    }
+}
+
+// X, Y: the two densities (column-major ne x ne); Z: the density of the
+// nuclear-row term, or null.  P columns per lane (P divides ne).
+template <class T, int U, int P>
+__global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const double *__restrict__ X,
+                                                              const double *__restrict__ Y, const double *__restrict__ Z,
                                                               const T *__restrict__ G, double *__restrict__ part)
 {
-    // offsets inside one slab are 32-bit (ne^5 < 2^31 is checked at launch)
     // LDS: Fw[nw][ne] (per-wavefront rows), Ff[ne^2] (per-column partials),
-    // Dt[ne^2] with Dt[c*ne + f] = DE(c,f): the lane-dependent density factors
-    // come from LDS, where consecutive lanes read consecutive words, instead
-    // of per-lane global gathers whose latency would sit between the loads
+    // Xf[ne][ne+1] with Xf[f*(ne+1) + c] = X(c,f) (a lane reads its own
+    // column f at consecutive c; the odd stride spreads the lanes' columns
+    // over the banks), Yt[ne^2] with Yt[e*ne + x] = Y(e,x)
     extern __shared__ double sm[];
-    const unsigned ne = (unsigned)ne_;
-    i64 blk = blockIdx.x;
-    unsigned a = (unsigned)(blk % ne);
-    const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne;
+    const unsigned ne = (unsigned)ne_, xs = ne + 1;
+    const i64 blk = blockIdx.x;
+    const unsigned a = (unsigned)(blk % ne);
+    const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne, ncol = ne2 / P;
     const unsigned nw = blockDim.x >> 6;
-    double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne, *Dt = Ff + ne2;
+    double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne, *Xf = Ff + ne2, *Yt = Xf + ne * xs;
     const T *g = G + blk * (i64)ne4 * ne;
     for (unsigned k = threadIdx.x; k < nw * ne + ne2; k += blockDim.x) sm[k] = 0.0;
-    for (unsigned k = threadIdx.x; k < ne2; k += blockDim.x) Dt[k] = DE[(k % ne) * ne + k / ne];
+    for (unsigned k = threadIdx.x; k < ne2; k += blockDim.x) {
+        const unsigned r = k / ne, q = k % ne;
+        Xf[r * xs + q] = X[k];     // X(q, r)
+        Yt[k] = Y[q * ne + r];     // Y(r, q)
+    }
     __syncthreads();
     const bool lead = (threadIdx.x & 63) == 0;
-    for (unsigned ef0 = 0; ef0 < ne2; ef0 += blockDim.x) {
-        unsigned ef = ef0 + threadIdx.x;
-        bool on = ef < ne2;
-        unsigned e = on ? ef / ne : 0, f = on ? ef % ne : 0;
-        const double *De = Dt + e * ne; // De[x] = DE(e,x)
-        double de_ef = on ? De[f] : 0.0;
-        double accf = 0.0;
+    for (unsigned q0 = 0; q0 < ncol; q0 += blockDim.x) {
+        const unsigned qc = q0 + threadIdx.x;
+        const bool on = qc < ncol;
+        const unsigned ef = on ? qc * P : 0;
+        const unsigned e = ef / ne, f = ef % ne; // the P columns share e
+        const double *Ye = Yt + e * ne;          // Ye[x] = Y(e,x)
+        const double *xf[P];
+        double y_ef[P], accf[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xf[p] = Xf + (f + p) * xs; // xf[p][c] = X(c,f+p)
+            y_ef[p] = on ? Ye[f + p] : 0.0;
+            accf[p] = 0.0;
+        }
+        const unsigned voff = ef * (unsigned)sizeof(T);
         for (unsigned b = 0; b < ne; ++b) {
-            double de_eb = on ? De[b] : 0.0;
+            const double y_eb = on ? Ye[b] : 0.0;
+            // row block b of the slab: ne^4 entries, at most 2^31 bytes for
+            // nebf <= EG3_MAX_NE
+            const __amdgpu_buffer_rsrc_t rb = buf_rsrc(g + (size_t)b * ne4, ne4 * (unsigned)sizeof(T));
+            const double *xb = X + b * ne;
             double accb = 0.0;
             for (unsigned d = 0; d < ne; ++d) {
-                double de_ed = on ? De[d] : 0.0;
-                double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                const double y_ed = on ? Ye[d] : 0.0;
+                double s1[P], s2[P], s3[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) s1[p] = s2[p] = s3[p] = 0.0;
                 if (on) {
-                    const unsigned o0 = b * ne4 + d * ne2 + ef;
-                    const double *dd = DE + d * ne, *db = DE + b * ne, *df = Dt + f; // df[c*ne] = DE(c,f)
+                    const unsigned row_d = d * ne2 * (unsigned)sizeof(T), row_c = ne3 * (unsigned)sizeof(T);
+                    const double *xd = X + d * ne;
                     unsigned c = 0;
-                    // U (then eight) independent loads in flight per lane before
-                    // the FMAs; the slab is read once, so the loads are
+                    // U (then eight) independent loads in flight per lane
+                    // before the FMAs; the slab is read once, so the loads are
                     // nontemporal
-                    for (; c + U <= ne; c += U) {
-                        raw_t<T> r[U];
-                        double fv[U];
-#pragma unroll
-                        for (int k = 0; k < U; ++k) r[k] = ld_raw(&g[o0 + (c + k) * ne3]);
-#pragma unroll
-                        for (int k = 0; k < U; ++k) fv[k] = df[(c + k) * ne]; // DE(c,f)
-                        NRX_LOADS_ISSUED();
-#pragma unroll
-                        for (int k = 0; k < U; ++k) {
-                            const double gv = widen(r[k]);
-                            s1 += dd[c + k] * gv; // DE(c,d)
-                            s2 += db[c + k] * gv; // DE(c,b)
-                            s3 += fv[k] * gv;
-                        }
-                    }
+                    for (; c + U <= ne; c += U) eg3_rows<T, U, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                     if (c + 8 <= ne) {
-                        raw_t<T> r[8];
-                        double fv[8];
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) r[k] = ld_raw(&g[o0 + (c + k) * ne3]);
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) fv[k] = df[(c + k) * ne];
-                        NRX_LOADS_ISSUED();
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const double gv = widen(r[k]);
-                            s1 += dd[c + k] * gv;
-                            s2 += db[c + k] * gv;
-                            s3 += fv[k] * gv;
-                        }
+                        eg3_rows<T, 8, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                         c += 8;
                     }
-                    for (; c < ne; ++c) {
-                        double gv = ld_entry(&g[o0 + c * ne3]);
-                        s1 += dd[c] * gv;
-                        s2 += db[c] * gv;
-                        s3 += df[c * ne] * gv;
-                    }
+                    for (; c < ne; ++c) eg3_rows<T, 1, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                 }
-                accb += de_ef * s1 - 0.5 * de_ed * s3;
-                accf += 0.25 * de_ed * s2 - 0.5 * de_eb * s1;
-                double vd = wave_sum(-0.5 * de_ef * s2 + 0.25 * de_eb * s3);
+                double vd = 0.0;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    accb += y_ef[p] * s1[p] - 0.5 * y_ed * s3[p];
+                    accf[p] += 0.25 * y_ed * s2[p] - 0.5 * y_eb * s1[p];
+                    vd += -0.5 * y_ef[p] * s2[p] + 0.25 * y_eb * s3[p];
+                }
+                vd = wave_sum(vd);
                 if (lead) Fw[d] += 3.0 * vd;
             }
             accb = wave_sum(accb);
             if (lead) Fw[b] += 3.0 * accb;
         }
-        if (on) Ff[ef] = 3.0 * accf; // each (e,f) column belongs to one lane
+        if (on) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) Ff[ef + p] = 3.0 * accf[p]; // each column belongs to one lane
+        }
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -1530,7 +1592,7 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
             for (unsigned w = 0; w < nw; ++w) fx += sm[w * ne + x];
             for (unsigned e = 0; e < ne; ++e) fx += Ff[e * ne + x];
             part[blk * (ne + 1) + x] = fx;
-            p += DE[x * ne + a] * fx;
+            if (Z) p += Z[x * ne + a] * fx;
         }
         p = wave_sum(p);
         if (threadIdx.x == 0) part[blk * (ne + 1) + ne] = p / 3.0;
@@ -1538,18 +1600,20 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
 }
 
 // LDS bytes of k_eg3_slab: per-wavefront rows, per-column partials and the
-// transposed density
-static inline size_t eg3_lds(i64 ne, unsigned threads) { return (size_t)((threads / 64) * ne + 2 * ne * ne) * 8; }
+// two density layouts
+static inline size_t eg3_lds(i64 ne, unsigned threads)
+{
+    return (size_t)((threads / 64) * ne + 2 * ne * ne + ne * (ne + 1)) * 8;
+}
 
 // largest nebf whose e-e-e-p slab (nebf^5 entries) has 32-bit offsets
 static const i64 EG3_MAX_NE = 73;
 
-// lanes per slab block: at most EG3_MAX_THREADS (the block loops over its
-// nebf^2 columns), the multiple of 64 that leaves the fewest idle lanes in
-// the last pass, the larger one on a tie
-static inline unsigned eg3_threads(i64 ne)
+// lanes per slab block for n lane columns: at most EG3_MAX_THREADS (the block
+// loops over its columns), the multiple of 64 that leaves the fewest idle
+// lanes in the last pass, the larger one on a tie
+static inline unsigned eg3_threads_n(i64 n)
 {
-    const i64 n = ne * ne;
     unsigned best = 64;
     i64 waste = ((n + 63) / 64) * 64 - n;
     for (unsigned t = 128; t <= EG3_MAX_THREADS; t += 64) {
@@ -1561,6 +1625,11 @@ static inline unsigned eg3_threads(i64 ne)
     }
     return best;
 }
+static inline unsigned eg3_threads(i64 ne) { return eg3_threads_n(ne * ne); }
+
+// columns per lane: two for BF16 storage when nebf is even (the pair shares
+// e and its first entry lies on a 4-byte boundary), else one
+static inline int eg3_pack(int dtype, i64 ne) { return dtype == NRX_BF16 && ne % 2 == 0 ? 2 : 1; }
 
 // ------------------------------------------------------------------
 // autotuned EG3 slab launches: the loads-in-flight depth U of k_eg3_slab per
@@ -1570,40 +1639,55 @@ static inline unsigned eg3_threads(i64 ne)
 static const int EG3_UNROLLS[4] = {8, 12, 16, 24};
 static int g_eg3_tuned[3][EG3_MAX_NE + 1] = {};
 
-template <class T>
-static int eg3_launch_u(int u, i64 ne, i64 nblk, const double *d_de, const T *g, double *part, hipStream_t st)
+// the densities of one launch: X on the (c,d) pair, Y on the (e,f) pair, Z
+// for the nuclear-row term (or null)
+struct Eg3Dens {
+    const double *X, *Y, *Z;
+};
+
+template <class T, int P>
+static int eg3_launch_u(int u, i64 ne, i64 nblk, Eg3Dens D, const T *g, double *part, hipStream_t st)
 {
-    const unsigned nt = eg3_threads(ne);
+    const unsigned nt = eg3_threads_n(ne * ne / P);
     const size_t lds = eg3_lds(ne, nt);
     const dim3 grid((unsigned)nblk), blk(nt);
     switch (u) {
-    case 8: hipLaunchKernelGGL((k_eg3_slab<T, 8>), grid, blk, lds, st, ne, d_de, g, part); break;
-    case 12: hipLaunchKernelGGL((k_eg3_slab<T, 12>), grid, blk, lds, st, ne, d_de, g, part); break;
-    case 16: hipLaunchKernelGGL((k_eg3_slab<T, 16>), grid, blk, lds, st, ne, d_de, g, part); break;
-    case 24: hipLaunchKernelGGL((k_eg3_slab<T, 24>), grid, blk, lds, st, ne, d_de, g, part); break;
+    case 8: hipLaunchKernelGGL((k_eg3_slab<T, 8, P>), grid, blk, lds, st, ne, D.X, D.Y, D.Z, g, part); break;
+    case 12: hipLaunchKernelGGL((k_eg3_slab<T, 12, P>), grid, blk, lds, st, ne, D.X, D.Y, D.Z, g, part); break;
+    case 16: hipLaunchKernelGGL((k_eg3_slab<T, 16, P>), grid, blk, lds, st, ne, D.X, D.Y, D.Z, g, part); break;
+    case 24: hipLaunchKernelGGL((k_eg3_slab<T, 24, P>), grid, blk, lds, st, ne, D.X, D.Y, D.Z, g, part); break;
     default: return fail(NEORX_E_ARG, "EG3 slabs: no variant with %d loads in flight", u);
     }
     HIPCHK(hipGetLastError());
     return NEORX_OK;
 }
 
-static int eg3_launch(int dtype, int u, i64 ne, i64 nblk, const double *d_de, const void *g, double *part,
-                      hipStream_t st)
+static int eg3_launch(int dtype, int u, i64 ne, i64 nblk, Eg3Dens D, const void *g, double *part, hipStream_t st)
 {
     switch (dtype) {
-    case NRX_F64: return eg3_launch_u<double>(u ? u : eg3_unroll<double>(), ne, nblk, d_de, (const double *)g, part, st);
-    case NRX_F32: return eg3_launch_u<float>(u ? u : eg3_unroll<float>(), ne, nblk, d_de, (const float *)g, part, st);
-    case NRX_BF16: return eg3_launch_u<bf16_t>(u ? u : eg3_unroll<bf16_t>(), ne, nblk, d_de, (const bf16_t *)g, part, st);
+    case NRX_F64:
+        return eg3_launch_u<double, 1>(u ? u : eg3_unroll<double>(), ne, nblk, D, (const double *)g, part, st);
+    case NRX_F32: return eg3_launch_u<float, 1>(u ? u : eg3_unroll<float>(), ne, nblk, D, (const float *)g, part, st);
+    case NRX_BF16:
+        if (eg3_pack(dtype, ne) == 2)
+            return eg3_launch_u<bf16_t, 2>(u ? u : eg3_unroll<bf16_t>(), ne, nblk, D, (const bf16_t *)g, part, st);
+        return eg3_launch_u<bf16_t, 1>(u ? u : eg3_unroll<bf16_t>(), ne, nblk, D, (const bf16_t *)g, part, st);
     default: return fail(NEORX_E_ARG, "EG3 slabs: unknown storage type %d", dtype);
     }
 }
 
-int nrx_eg3_slabs(i64 ne, i64 nblk, const double *d_de, const void *g, int dtype, double *part, hipStream_t st)
+int nrx_eg3_slabs2(i64 ne, i64 nblk, const double *d_x, const double *d_y, const double *d_z, const void *g, int dtype,
+                   double *part, hipStream_t st)
 {
     if (ne <= 0 || ne > EG3_MAX_NE) return fail(NEORX_E_UNSUPPORTED, "EG3 slabs: nebf=%lld not in 1..%lld",
                                                 (long long)ne, (long long)EG3_MAX_NE);
     if (dtype < 0 || dtype > 2) return fail(NEORX_E_ARG, "EG3 slabs: unknown storage type %d", dtype);
-    return eg3_launch(dtype, g_eg3_tuned[dtype][ne], ne, nblk, d_de, g, part, st);
+    return eg3_launch(dtype, g_eg3_tuned[dtype][ne], ne, nblk, Eg3Dens{d_x, d_y, d_z}, g, part, st);
+}
+
+int nrx_eg3_slabs(i64 ne, i64 nblk, const double *d_de, const void *g, int dtype, double *part, hipStream_t st)
+{
+    return nrx_eg3_slabs2(ne, nblk, d_de, d_de, d_de, g, dtype, part, st);
 }
 
 extern "C" int neorx_autotune_eg3_dev(int64_t nebf, int64_t npbf, int64_t dtype, const double *d_de, const void *d_gm3,
@@ -1628,9 +1712,10 @@ extern "C" int neorx_autotune_eg3_dev(int64_t nebf, int64_t npbf, int64_t dtype,
     float bt = 0.0f;
     for (int k = 0; k < 4 && !rc; ++k) {
         const int u = EG3_UNROLLS[k];
-        rc = eg3_launch((int)dtype, u, ne, nblk, d_de, d_gm3, part.d(), st); // warm-up
+        rc = eg3_launch((int)dtype, u, ne, nblk, Eg3Dens{d_de, d_de, d_de}, d_gm3, part.d(), st); // warm-up
         if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventRecord failed");
-        for (i64 r = 0; r < reps && !rc; ++r) rc = eg3_launch((int)dtype, u, ne, nblk, d_de, d_gm3, part.d(), st);
+        for (i64 r = 0; r < reps && !rc; ++r)
+            rc = eg3_launch((int)dtype, u, ne, nblk, Eg3Dens{d_de, d_de, d_de}, d_gm3, part.d(), st);
         if (!rc) rc = hipEventRecord(e1, st) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventRecord failed");
         if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : fail(NEORX_E_HIP, "hipEventSynchronize failed");
         float ms = 0.0f;
@@ -1763,14 +1848,14 @@ extern "C" int neorx_eg3ic1(int64_t nebf, int64_t npbf, const double *de, const 
     DevBuf dDE, dDP, dG, dFE, dFP, dE;
     rc = dDE.alloc(ne * ne * 8);
     if (!rc) rc = dDP.alloc(np * np * 8);
-    if (!rc) rc = dG.alloc(ng3 * 8);
     if (!rc) rc = dFE.alloc(ne * ne * 8);
     if (!rc) rc = dFP.alloc(np * np * 8);
     if (!rc) rc = dE.alloc(8);
     if (rc) return rc;
     HIPCHK(hipMemcpy(dDE.p, de, ne * ne * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dDP.p, dp, np * np * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dG.p, gm3icr, ng3 * 8, hipMemcpyHostToDevice));
+    rc = h2d(dG, gm3icr, ng3);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(dFE.p, focke, ne * ne * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dFP.p, fockp, np * np * 8, hipMemcpyHostToDevice));
     rc = neorx_eg3ic1_dev(ne, np, dDE.d(), dDP.d(), dG.d(), dFE.d(), dFP.d(), dE.d(), 0);
@@ -2030,8 +2115,8 @@ extern "C" int neorx_gam3_dscf_dev(int64_t handle, const double *d_de, const dou
         rc = build3_block(s, p0, p1, X.d(), Y.d(), st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_gam3_symm, dim3(grid_blocks(n, 256)), dim3(256), 0, st, ne, n, X.d(), G.d());
-        hipLaunchKernelGGL(k_eg3_slab<double>, dim3((unsigned)((p1 - p0) * ne)), dim3(eg3_threads(ne)),
-                           eg3_lds(ne, eg3_threads(ne)), st, ne, d_de, G.d(), part.d() + p0 * ne * (ne + 1));
+        hipLaunchKernelGGL((k_eg3_slab<double, 16, 1>), dim3((unsigned)((p1 - p0) * ne)), dim3(eg3_threads(ne)),
+                           eg3_lds(ne, eg3_threads(ne)), st, ne, d_de, d_de, d_de, G.d(), part.d() + p0 * ne * (ne + 1));
         HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_eg3_reduce, dim3(nblocks(ne * ne + npp, 256)), dim3(256), 0, st, ne, np, d_dp, part.d(), d_fe,

@@ -882,6 +882,38 @@ __global__ void k_copy(i64 n, const double *__restrict__ x, double *__restrict__
     GRID_LOOP(t, n) y[t] = x[t];
 }
 
+// the two densities of an incremental EG3 pass: X = DE - DEref, Y = DE + DEref
+__global__ void k_inc_dens(i64 n, const double *__restrict__ D, const double *__restrict__ D0, double *__restrict__ X,
+                           double *__restrict__ Y)
+{
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        X[i] = D[i] - D0[i];
+        Y[i] = D[i] + D0[i];
+    }
+}
+
+// EG3 slab rows of DE from the increment: part[blk][x] += ref[blk][x] (the
+// rows of the reference density), then part[blk][ne] = sum_x DE(a,x) F[x] / 3
+// with a = blk % ne, as the slab kernel forms it; a wavefront per slab, fixed
+// order
+__global__ void __launch_bounds__(64) k_inc_rows(i64 ne, i64 nblk, const double *__restrict__ DE,
+                                                 const double *__restrict__ ref, double *__restrict__ part)
+{
+    const i64 blk = blockIdx.x;
+    if (blk >= nblk) return;
+    const i64 a = blk % ne;
+    double *r = part + blk * (ne + 1);
+    const double *q = ref + blk * (ne + 1);
+    double p = 0.0;
+    for (i64 x = threadIdx.x; x < ne; x += 64) {
+        const double v = r[x] + q[x];
+        r[x] = v;
+        p += DE[x * ne + a] * v;
+    }
+    p = wsum(p);
+    if (threadIdx.x == 0) r[ne] = p / 3.0;
+}
+
 // ------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------
@@ -902,6 +934,9 @@ struct Xchf {
     double *GP4 = nullptr, *HB4 = nullptr, *B4 = nullptr, *Ab4 = nullptr, *FE4 = nullptr, *E4 = nullptr;
     double *R4 = nullptr; // partial traces of the adjoint (4 nebf^2)
     double *zpart = nullptr, *rows = nullptr;
+    // incremental EG3 builds: rows of the reference density, the reference
+    // density and the pass's two densities
+    double *part3ref = nullptr, *DEref = nullptr, *DXi = nullptr, *DYi = nullptr;
     i64 nz_fe = 1, nch_fp = 1, nz_dot = 1;
     double *FEs = nullptr, *SEs = nullptr, *FP23 = nullptr, *SP2 = nullptr;
     // SCF state
@@ -910,8 +945,9 @@ struct Xchf {
     double *stat = nullptr, *rwork = nullptr;
     double *Fh = nullptr, *Eh = nullptr, *Bd = nullptr, *hist = nullptr, *dw1 = nullptr, *dw2 = nullptr;
     double *hstat = nullptr; // pinned
-    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    int graph_key[3] = {-1, -1, -1};
+    static constexpr int NGRAPH = 6;
+    hipGraphExec_t graph[NGRAPH] = {};
+    int graph_key[NGRAPH] = {-1, -1, -1, -1, -1, -1};
     hipStream_t st = nullptr;
 
     template <class T> int alloc(T *&p, i64 n)
@@ -994,8 +1030,15 @@ int both_bras(Xchf *x, const double *D, const double *G, double *H, double *T)
     return lead_transform(x, D, H, T, ne3);
 }
 
+// how one phase A reads GAM_3
+enum G3Pass {
+    G3_FULL = 0,    // the rows of DE
+    G3_REFRESH = 1, // the rows of DE, kept as the reference of later increments
+    G3_INC = 2,     // the reference rows plus the increment T(DE - DEref, DE + DEref)
+};
+
 // phase A (depends on DE): dtype = storage of GAM_3 read
-int phase_a(Xchf *x, const double *DE, int dtype)
+int phase_a(Xchf *x, const double *DE, int dtype, int pass = G3_FULL)
 {
     hipStream_t st = x->st;
     const i64 ne = x->ne, np = x->np, np2 = np * np, ne4 = ne * ne * ne * ne;
@@ -1009,8 +1052,25 @@ int phase_a(Xchf *x, const double *DE, int dtype)
     if (x->nelec > 2) {
         const void *g = dtype == NRX_F64 ? (const void *)x->G3 : dtype == NRX_F32 ? x->G3f : x->G3h;
         if (!g) return fail(NEORX_E_ARG, "GAM_3 is not held in storage type %d (see neorx_xchf_create store)", dtype);
-        int rc = nrx_eg3_slabs(ne, np2 * ne, DE, g, dtype, x->part3, st);
-        if (rc) return rc;
+        const i64 nblk = np2 * ne;
+        int rc = 0;
+        if (pass == G3_INC) {
+            hipLaunchKernelGGL(k_inc_dens, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref, x->DXi,
+                               x->DYi);
+            rc = nrx_eg3_slabs2(ne, nblk, x->DXi, x->DYi, nullptr, g, dtype, x->part3, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_inc_rows, dim3((unsigned)nblk), dim3(64), 0, st, ne, nblk, DE, x->part3ref, x->part3);
+            HIPCHK(hipGetLastError());
+        } else {
+            rc = nrx_eg3_slabs(ne, nblk, DE, g, dtype, x->part3, st);
+            if (rc) return rc;
+            if (pass == G3_REFRESH) {
+                hipLaunchKernelGGL(k_copy, dim3(nblocks(nblk * (ne + 1), 256)), dim3(256), 0, st, nblk * (ne + 1),
+                                   x->part3, x->part3ref);
+                hipLaunchKernelGGL(k_copy, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref);
+                HIPCHK(hipGetLastError());
+            }
+        }
         // 5-particle term, DE side: A (both bras), HA (one bra), Bbar = adj(A),
         // GPbar = Bbar carried back, FP4 = G2s . GPbar
         rc = both_bras(x, DE, x->GeeU, x->HA4, x->A4);
@@ -1185,6 +1245,10 @@ int xchf_build(Xchf *x, i64 handle, i64 store)
     x->nz_dot = 256;
     rc |= x->alloc(x->part2, x->nelec > 1 ? np2 * ne * 4 * ne : 1);
     rc |= x->alloc(x->part3, x->nelec > 2 ? np2 * ne * (ne + 1) : 1);
+    rc |= x->alloc(x->part3ref, x->nelec > 2 ? np2 * ne * (ne + 1) : 1);
+    rc |= x->alloc(x->DEref, ne2);
+    rc |= x->alloc(x->DXi, ne2);
+    rc |= x->alloc(x->DYi, ne2);
     rc |= x->alloc(x->FPx, 3 * np2);
     rc |= x->alloc(x->FEee, ne2);
     const i64 n4 = x->nelec > 2 ? ne4 : 1;
@@ -1259,13 +1323,13 @@ int launch_roothan(Xchf *x, int n, const double *X, const double *F, int first, 
 
 // stat layout: [0] diffE [1] diffP [2] DIIS error [3..16] comps of the
 // electronic Fock build [17..30] comps of the nuclear one
-int scf_iteration(Xchf *x, int dtype, bool diis, int nucst)
+int scf_iteration(Xchf *x, int dtype, bool diis, int nucst, int pass)
 {
     hipStream_t st = x->st;
     const int ne = (int)x->ne, np = (int)x->np;
     const int nocc = x->nelec > 1 ? x->nelec / 2 : 1;
     const double pre = x->nelec > 1 ? 2.0 : 1.0;
-    int rc = phase_a(x, x->DE, dtype);
+    int rc = phase_a(x, x->DE, dtype, pass);
     if (!rc) rc = phase_b(x, x->DE, x->DP, x->FEx, x->FP, x->stat + 17);
     if (!rc) rc = launch_roothan(x, np, x->Xp, x->FP, nucst - 1, nucst - 1, 1.0, x->CP, x->evP, x->DP);
     if (!rc) rc = phase_b(x, x->DE, x->DP, x->FE, x->FEx, x->stat + 3);
@@ -1299,6 +1363,10 @@ extern "C" int neorx_xchf_create(int64_t handle, int64_t nelec, int64_t store, i
     if (!ctx || nelec < 1) return fail(NEORX_E_ARG, "neorx_xchf_create: nelec >= 1 and a handle pointer are required");
     if (s->d.nebf1 != s->d.nebf) return fail(NEORX_E_ARG, "neorx_xchf_create takes a one-basis handle");
     if (nelec > 2 && !(store & 7)) return fail(NEORX_E_ARG, "store must keep GAM_3 in at least one storage type");
+    if (s->maxl > 2)
+        return fail(NEORX_E_UNSUPPORTED, "the device XCHF SCF supports l <= 2 (this basis has l = %d); the integral "
+                                         "builders and contractions take f functions",
+                    s->maxl);
     if (s->d.nebf > RH_MAXN || s->d.npbf > RH_MAXN)
         return fail(NEORX_E_UNSUPPORTED, "ROOTHAN on the device supports at most %d basis functions", RH_MAXN);
     Xchf *x = new Xchf();
@@ -1360,10 +1428,14 @@ extern "C" int neorx_xchf_fock_dev(int64_t ctx, const double *d_de, const double
     Xchf *x = get_x(ctx);
     if (!x) return fail(NEORX_E_ARG, "unknown XCHF context %lld", (long long)ctx);
     if (!d_de || !d_dp || !d_focke || !d_fockp || !d_comps) return fail(NEORX_E_ARG, "null argument");
-    if (prec < 0 || prec > 2) return fail(NEORX_E_ARG, "prec %lld not in 0..2", (long long)prec);
+    const int dtype = (int)(prec & 15), pass = (int)(prec >> 4);
+    if (prec < 0 || dtype > 2 || pass > 2)
+        return fail(NEORX_E_ARG, "prec %lld: storage not in 0..2 or pass not in 0..2", (long long)prec);
+    if (x->nelec > 2 && pass == G3_REFRESH && dtype != NRX_F64)
+        return fail(NEORX_E_ARG, "a reference build reads GAM_3 in FP64 (prec 0x10)");
     hipStream_t keep = x->st;
     if (stream) x->st = (hipStream_t)stream;
-    int rc = phase_a(x, d_de, (int)prec);
+    int rc = phase_a(x, d_de, dtype, pass);
     if (!rc) rc = phase_b(x, d_de, d_dp, d_focke, d_fockp, d_comps);
     x->st = keep;
     return rc;
@@ -1416,12 +1488,12 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
     const int ne = (int)x->ne, np = (int)x->np;
     if (nucst < 1 || nucst > np) return fail(NEORX_E_ARG, "NUCST=%lld not in 1..%d", (long long)nucst, np);
     if (maxit < 1) return fail(NEORX_E_ARG, "maxit < 1");
-    if (mode < 0 || mode > 2) return fail(NEORX_E_ARG, "mode %lld not in 0..2", (long long)mode);
+    if (mode < 0 || mode > 3) return fail(NEORX_E_ARG, "mode %lld not in 0..3", (long long)mode);
     const bool use_graph = flags & 1, diis = flags & 2;
     if (x->nelec > 2) {
         if (!x->G3 && mode == 0) return fail(NEORX_E_ARG, "FP64 mode needs GAM_3 in FP64 (store bit 1)");
-        if (mode >= 1 && !x->G3f) return fail(NEORX_E_ARG, "mixed mode needs GAM_3 in FP32 (store bit 2)");
-        if (mode == 2 && !x->G3h) return fail(NEORX_E_ARG, "mode 2 needs GAM_3 in BF16 (store bit 4)");
+        if ((mode == 1 || mode == 2) && !x->G3f) return fail(NEORX_E_ARG, "mixed mode needs GAM_3 in FP32 (store bit 2)");
+        if (mode >= 2 && !x->G3h) return fail(NEORX_E_ARG, "mode %lld needs GAM_3 in BF16 (store bit 4)", (long long)mode);
         if (mode >= 1 && !x->G3) return fail(NEORX_E_ARG, "mixed mode switches to FP64: store bit 1 is needed");
     }
     int rc = scf_state(x);
@@ -1472,20 +1544,28 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
 
     // precision of GAM_3 per iteration: mode 0 FP64; mode 1 FP32 until the
     // switch signal (DIIS error with DIIS, else max|dDE|) < sw_lo; mode 2
-    // BF16 until < sw_hi, then FP32 until < sw_lo; FP64 from then on
+    // BF16 until < sw_hi, then FP32 until < sw_lo; FP64 from then on.
+    // Mode 3 (incremental): BF16 builds until the signal < sw_hi; then an FP64
+    // build whose rows become the reference, and BF16-stored increments from
+    // it (T(DE - DEref, DE + DEref), whose storage error shrinks with the
+    // distance to the reference); a new FP64 reference once the signal <
+    // sw_lo, and a last FP64 build when the increments meet the convergence
+    // test.  Every mode converges only on an FP64 build.
     int dtype = mode == 0 ? NRX_F64 : mode == 1 ? NRX_F32 : NRX_BF16;
+    int pass = G3_FULL;
+    double ref_sig = 0.0; // the switch signal when the last reference was asked for
     double eold = 0.0;
     i64 it = 0;
     bool conv = false;
     for (it = 1; it <= maxit && !conv; ++it) {
         const auto t0 = std::chrono::steady_clock::now();
-        const int key = dtype * 2 + (diis ? 1 : 0);
+        const int key = ((dtype * 3 + pass) * 2) + (diis ? 1 : 0);
         if (use_graph) {
             int slot = -1;
-            for (int g = 0; g < 3; ++g)
+            for (int g = 0; g < Xchf::NGRAPH; ++g)
                 if (x->graph_key[g] == key) slot = g;
             if (slot < 0) {
-                for (int g = 0; g < 3 && slot < 0; ++g)
+                for (int g = 0; g < Xchf::NGRAPH && slot < 0; ++g)
                     if (x->graph_key[g] < 0) slot = g;
                 if (slot < 0) slot = 0;
                 if (x->graph[slot]) (void)hipGraphExecDestroy(x->graph[slot]);
@@ -1493,7 +1573,7 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
                 x->graph_key[slot] = -1;
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                rc = scf_iteration(x, dtype, diis, (int)nucst);
+                rc = scf_iteration(x, dtype, diis, (int)nucst, pass);
                 hipError_t e = hipStreamEndCapture(st, &g);
                 if (rc) return rc;
                 if (e != hipSuccess) return fail(NEORX_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
@@ -1504,7 +1584,7 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
             }
             HIPCHK(hipGraphLaunch(x->graph[slot], st));
         } else {
-            rc = scf_iteration(x, dtype, diis, (int)nucst);
+            rc = scf_iteration(x, dtype, diis, (int)nucst, pass);
             if (rc) return rc;
         }
         HIPCHK(hipMemcpyAsync(x->hstat, x->stat, 32 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1522,15 +1602,29 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
             r[4] = diis ? derr : dife;
             r[5] = dtype;
             r[6] = ms;
-            r[7] = 0.0;
+            r[7] = pass;
         }
         eold = etot;
         // converged (xcscf.f:515-519) only on an iteration with FP64 GAM_3
-        conv = (np > 1) ? (dife < tole && difp < tolp) : (dife < tole);
-        if (x->nelec > 2 && dtype != NRX_F64) conv = false;
+        const bool crit = (np > 1) ? (dife < tole && difp < tolp) : (dife < tole);
+        conv = crit && (x->nelec <= 2 || (dtype == NRX_F64 && pass != G3_INC));
         const double sig = diis ? derr : dife;
-        if (dtype == NRX_BF16 && sig < sw_hi) dtype = NRX_F32;
-        if (dtype == NRX_F32 && sig < sw_lo) dtype = NRX_F64;
+        if (mode == 3) {
+            // next build: a reference (FP64) or an increment (BF16)
+            const bool want_ref = pass == G3_INC ? (crit || (ref_sig >= sw_lo && sig < sw_lo))
+                                                 : (pass == G3_FULL && dtype == NRX_BF16 && (sig < sw_hi || crit));
+            if (want_ref) {
+                ref_sig = sig;
+                dtype = NRX_F64;
+                pass = G3_REFRESH;
+            } else if (pass == G3_REFRESH || pass == G3_INC) {
+                dtype = NRX_BF16;
+                pass = G3_INC;
+            }
+        } else {
+            if (dtype == NRX_BF16 && sig < sw_hi) dtype = NRX_F32;
+            if (dtype == NRX_F32 && sig < sw_lo) dtype = NRX_F64;
+        }
     }
     if (niter) *niter = it - 1;
     HIPCHK(hipStreamSynchronize(st));

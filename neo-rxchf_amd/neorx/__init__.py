@@ -50,6 +50,8 @@ def load():
     _lib.neorx_device_arch.restype = _C.c_char_p
     _sig("neorx_init", [_I64])
     _sig("neorx_finalize", [])
+    _sig("neorx_release_resident", [])
+    _sig("neorx_transfer_bytes", [_P, _P])
     _sig("neorx_system_create", [_I64, _I64, _I64, _I64, _I64, _D] + [_P] * 15 + [_P])
     _sig("neorx_system_destroy", [_I64])
     _sig("neorx_system_order", [_I64, _P, _P])
@@ -331,6 +333,21 @@ def GAM4_DSCF(nebf, npbf, gam_2s, gam_ee, DE, DP, focke, fockp):
     _check(_lib.neorx_gam4_dscf(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
                                 _p(DE), _p(DP), _p(focke), _p(fockp), _p(e)))
     return float(e[0])
+
+
+def transfer_bytes():
+    """(host->device, device->host) bytes of in-core tensors copied by the
+    host-pointer entry points so far (neorx_transfer_bytes)."""
+    load()
+    h, d = _C.c_int64(), _C.c_int64()
+    _check(_lib.neorx_transfer_bytes(_C.byref(h), _C.byref(d)))
+    return h.value, d.value
+
+
+def release_resident():
+    """Free the device copies kept for host tensors (neorx_release_resident)."""
+    load()
+    _check(_lib.neorx_release_resident())
 
 
 def EG2IC1(nebf, npbf, DE, DP, GM2ICR, GM2SICR, FE, FP):

@@ -18,6 +18,8 @@ _SHELL = {
     0: [(0, 0, 0)],
     1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
     2: [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
+    3: [(3, 0, 0), (0, 3, 0), (0, 0, 3), (2, 1, 0), (2, 0, 1), (1, 2, 0), (0, 2, 1), (1, 0, 2), (0, 1, 2),
+        (1, 1, 1)],
 }
 
 
@@ -162,8 +164,8 @@ def config2(seed=2):
 
 
 def small(seed=0, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=3, nat=2, maxl=None):
-    """Small random system for parity tests: s/p/d shells on random centres,
-    contracted functions of 1-2 primitives."""
+    """Small random system for parity tests: s/p/d/f shells on random
+    centres, contracted functions of 1-2 primitives."""
     rng = np.random.default_rng(seed)
     cents = [rng.uniform(-1.5, 1.5, 3) for _ in range(n_e_centres)]
     prims, contr = [], []

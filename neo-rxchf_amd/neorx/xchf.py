@@ -239,20 +239,26 @@ class XCHF:
         self._check(self._lib.neorx_xchf_integrals(self.ctx, _C.byref(e), *[x.ctypes.data_as(_C.c_void_p) for x in a]))
         return (e.value, *a)
 
-    def fock_dev(self, d_de, d_dp, prec, d_fe, d_fp, d_comps, stream=0):
+    def fock_dev(self, d_de, d_dp, prec, d_fe, d_fp, d_comps, stream=0, g3pass=0):
         """xchf1_fock (xchf1_fock.f:2) on device arrays: FE, FP (after
-        XCfock_correction) and the energy components (COMPS order)."""
-        self._check(self._lib.neorx_xchf_fock_dev(self.ctx, d_de, d_dp, PREC[prec] if isinstance(prec, str) else prec,
-                                                  d_fe, d_fp, d_comps, stream))
+        XCfock_correction) and the energy components (COMPS order).  g3pass:
+        0 full; 1 full, kept as the reference (FP64); 2 the reference plus the
+        increment from it, read in the storage prec."""
+        p = PREC[prec] if isinstance(prec, str) else prec
+        self._check(self._lib.neorx_xchf_fock_dev(self.ctx, d_de, d_dp, p | (g3pass << 4), d_fe, d_fp, d_comps,
+                                                  stream))
 
     def scf(self, ce_guess=None, nucst=1, maxit=100, tole=1e-6, tolp=1e-4, mode="f64", graph=True, diis=False,
             sw_hi=1e-3, sw_lo=1e-5):
         """xcscf (xcscf.f:2): returns a dict with the final and initial-guess
         energy components, the orbitals, eigenvalues and the per-iteration
         trace [E_total, dE, max|dDE|, max|dDP|, switch signal, GAM_3 storage
-        (0 FP64, 1 FP32, 2 BF16), ms]."""
+        (0 FP64, 1 FP32, 2 BF16), ms, GAM_3 pass (0 full, 1 full kept as the
+        reference, 2 increment)].  mode: "f64"; "f32" (FP32 then FP64);
+        "bf16" (BF16, FP32, FP64); "inc" (BF16, then BF16-stored increments
+        from FP64 reference builds)."""
         ne, npb = self.ne, self.np
-        m = {"f64": 0, "f32": 1, "bf16": 2}[mode]
+        m = {"f64": 0, "f32": 1, "bf16": 2, "inc": 3}[mode]
         ce0 = None if ce_guess is None else np.asfortranarray(ce_guess, dtype=np.float64)
         ci, cf = np.zeros(14), np.zeros(14)
         CE, CP = np.zeros((ne, ne), order="F"), np.zeros((npb, npb), order="F")
@@ -265,7 +271,7 @@ class XCHF:
                                              p(CE), p(CP), p(evE), p(evP), p(tr), _C.byref(nit)))
         n = nit.value
         return dict(comps=dict(zip(COMPS, cf)), comps_init=dict(zip(COMPS, ci)), CE=CE, CP=CP, evE=evE, evP=evP,
-                    trace=tr[:n, :7], niter=n)
+                    trace=tr[:n, :8], niter=n)
 
 
 def roothan_dev(n, d_s, d_f, kstart, klast, prefac, d_c, d_ev, d_d, stream=0):

@@ -41,22 +41,23 @@ def random_tuples(sysd, cls, n, seed):
     return idx
 
 
-_eng = None
+_eng = {}
 
 
-def engine_host():
-    """Host build of the device integral engine (tests/_build)."""
-    global _eng
-    if _eng is None:
-        _eng = C.CDLL(os.path.join(ROOT, "tests", "_build", "libengine_host.so"))
-        _eng.eng_gint.restype = C.c_double
-    return _eng
+def engine_host(lmax=2):
+    """Host build of the device integral engine (tests/_build) at the
+    capacities of l <= 2 (the tier kernels) or l <= 3 (the wide bin)."""
+    if lmax not in _eng:
+        name = "libengine_host.so" if lmax == 2 else "libengine_host3.so"
+        _eng[lmax] = C.CDLL(os.path.join(ROOT, "tests", "_build", name))
+        _eng[lmax].eng_gint.restype = C.c_double
+    return _eng[lmax]
 
 
-def engine_prim_list(s, cls, idx, tier=0):
+def engine_prim_list(s, cls, idx, tier=0, lmax=2):
     """Host engine values; tier 0 picks the capacity as the device does,
     tier 1 forces the general capacity."""
-    e, sv, _ = engine_prim_list3(s, cls, idx, tier)
+    e, sv, _ = engine_prim_list3(s, cls, idx, tier, lmax)
     return e, sv
 
 
@@ -76,9 +77,9 @@ def mult_tuples(sysd, cls, n, seed):
     return idx
 
 
-def engine_prim_list3(s, cls, idx, tier=0):
+def engine_prim_list3(s, cls, idx, tier=0, lmax=2):
     """As engine_prim_list with the third output (ansE2 of class 5)."""
-    E = engine_host()
+    E = engine_host(lmax)
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     n = len(idx)
     e, sv, xv = np.zeros(n), np.zeros(n), np.zeros(n)

@@ -184,3 +184,43 @@ def test_engine_tiers_lone_angular_momentum(cls):
     check_close(t0, t1, rtol=1e-13, atol_rel=1e-14, what=f"cls{cls} device tiers vs general tier")
     if ref is not None:
         check_close(t0, ref, what=f"cls{cls} device tiers vs oracle")
+
+
+@pytest.mark.parametrize("cls,n", [(1, 300), (2, 120), (3, 40), (4, 80), (7, 20), (8, 200)])
+def test_engine_f_tuples_vs_oracle(cls, n):
+    """The l <= 3 build of the engine (the wide bin of the device kernels) on
+    s/p/d/f electrons and an s/f nucleus; the l <= 2 build's tier choice sends
+    every tuple with an f power to that bin.  Class 7 equals the reference
+    after the average over the six permutations of its electron pairs."""
+    from itertools import permutations
+
+    s = synth.small(seed=130 + cls, n_e_centres=2, e_shells=(0, 2, 3), p_shells=(0, 3))
+    idx = random_tuples(s, {4: 2, 7: 3, 8: 1}.get(cls, cls), n, seed=cls)
+    if cls == 7:
+        idx = np.array([[x[p[0]], x[p[1]], x[p[2]], x[3], x[4 + p[0]], x[4 + p[1]], x[4 + p[2]], x[7]]
+                        for x in idx for p in permutations(range(3))], dtype=np.int64)
+    ee, es = engine_prim_list(s, cls, idx, lmax=3)
+    oe, os_ = oracle.prim_list(s, cls, idx)
+    if cls == 7:
+        ee, oe = ee.reshape(-1, 6).mean(axis=1), oe.reshape(-1, 6).mean(axis=1)
+    # absolute floor 1e-11 of the batch maximum: the oracle keeps the
+    # reference's truncated Boys series (~5e-12 relative at the orders f
+    # functions reach), the engine the exact function; cancelling terms show it
+    check_close(ee, oe, atol_rel=1e-11, what=f"l<=3 engine vs oracle cls{cls} out0")
+    if cls not in (3, 7):
+        check_close(es, os_, atol_rel=1e-11, what=f"l<=3 engine vs oracle cls{cls} out1")
+
+
+def test_engine_f_rxchfmult_vs_oracle():
+    """Two-basis classes 5 and 6 with f functions, l <= 3 build."""
+    from _util import engine_prim_list3, mult_tuples
+
+    s = synth.mult(seed=137, e_shells=(0, 3), be_shells=(0, 1, 3), p_shells=(0, 1), be_centres=1)
+    cs = synth.combined(s)
+    for cls, n in ((5, 60), (6, 24)):
+        idx = mult_tuples(s, cls, n, seed=cls)
+        got = engine_prim_list3(cs, cls, idx, 0, lmax=3)
+        ref = oracle.prim_list3(cs, cls, idx)
+        for k in range(3 if cls == 5 else 1):
+            check_close(got[k], ref[k], atol_rel=1e-11, what=f"l<=3 engine cls{cls} out{k}")
+

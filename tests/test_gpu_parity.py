@@ -98,6 +98,46 @@ def test_gam1_gam2_builders(torch_dev):
     S.close()
 
 
+def test_resident_tensors(torch_dev):
+    """The host-pointer builders keep GM2ICR/GM2sICR/GM3ICR on the device: the
+    host-pointer contractions given the same arrays copy no tensor over PCIe
+    and agree with the oracle; changing the caller's array drops the copy (the
+    next call copies the new values and agrees with the oracle on them)."""
+    s = synth.small(seed=77, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1))
+    ne, npb = s["nebf"], s["npbf"]
+    print(f"case resident tensors nebf={ne} npbf={npb}", flush=True)
+    S = neorx.System(s)
+    g2, g2s = S.GAM2_IC1()
+    g3 = S.GAM3_IC1()
+    S.close()
+    DE, DP = synth.density(ne, 4.0, 3), synth.density(npb, 1.0, 4)
+    F0e = np.asfortranarray(np.random.default_rng(1).standard_normal((ne, ne)))
+    F0p = np.asfortranarray(np.random.default_rng(2).standard_normal((npb, npb)))
+
+    def both(g, fn):
+        A, B = F0e.copy(order="F"), F0e.copy(order="F")
+        P, Q = F0p.copy(order="F"), F0p.copy(order="F")
+        h0 = neorx.transfer_bytes()[0]
+        eg = fn(neorx, A, P)
+        dh = neorx.transfer_bytes()[0] - h0
+        eo = fn(oracle, B, Q)
+        check_close(A, B, what="FE")
+        check_close(P, Q, what="FP")
+        return eg, eo, dh
+
+    eg, eo, dh = both(g3, lambda m, A, P: m.eg3ic1(ne, npb, DE, DP, g3, A, P) if m is oracle
+                      else m.EG3IC1(ne, npb, DE, DP, g3, A, P))
+    assert eg == pytest.approx(eo, rel=1e-11) and dh == 0, (eg, eo, dh)
+    r = both(g2, lambda m, A, P: m.eg2ic1(ne, npb, DE, DP, g2, g2s, A, P)[2] if m is oracle
+             else m.EG2IC1(ne, npb, DE, DP, g2, g2s, A, P)[2])
+    assert r[0] == pytest.approx(r[1], rel=1e-11) and r[2] == 0, r
+    g3[0] += 0.5  # a probed entry
+    eg, eo, dh = both(g3, lambda m, A, P: m.eg3ic1(ne, npb, DE, DP, g3, A, P) if m is oracle
+                      else m.EG3IC1(ne, npb, DE, DP, g3, A, P))
+    assert eg == pytest.approx(eo, rel=1e-11) and dh == g3.size * 8, (eg, eo, dh)
+    neorx.release_resident()
+
+
 @pytest.mark.parametrize("kw", [
     dict(seed=72, n_e_centres=3, e_shells=(0,), contract=(1,), p_shells=(0, 1)),
     dict(seed=74, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,)),

@@ -32,6 +32,16 @@ def _dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+def _round_storage(g, prec):
+    """GAM_3 as the device stores it in FP32 or BF16, widened back to FP64."""
+    f = np.asarray(g, dtype=np.float64).astype(np.float32)
+    if prec == "f32":
+        return f.astype(np.float64)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def _small(nelec, **kw):
     s = synth.small(**kw)
     s["nelec"] = nelec
@@ -93,13 +103,65 @@ def test_xchf_fock(torch_dev, kw, sym):
     check_close(fe.cpu().numpy(), oFE.ravel(order="F"), what="FE")
     check_close(fp.cpu().numpy(), oFP.ravel(order="F"), what="FP")
     # reduced-precision storage of GAM_3 moves only E_GAM3 and the matrices,
-    # by the storage rounding
+    # by the storage rounding; the build equals the oracle's FP64 build over
+    # the same rounded GAM_3 (FP32: round to nearest; BF16: the FP32 value
+    # rounded to nearest even in its upper 16 bits)
     for prec, tol in (("f32", 1e-6), ("bf16", 1e-2)):
         X.fock_dev(d_de.data_ptr(), d_dp.data_ptr(), prec, fe.data_ptr(), fp.data_ptr(), c.data_ptr())
         torch.cuda.synchronize()
         gl = c.cpu().numpy()
         assert abs(gl[8] - oc[8]) <= tol * abs(oc[8]), (prec, gl[8], oc[8])
         assert gl[7] == pytest.approx(oc[7], rel=1e-11) and gl[9] == pytest.approx(oc[9], rel=1e-11)
+        rints = dict(ints, G3=_round_storage(ints["G3"], prec))
+        rFE, rFP, rc = oracle.xchf_fock(ne, npb, 4, rints, DE, DP)
+        for k, nm in enumerate(xchf.COMPS):
+            assert gl[k] == pytest.approx(rc[k], rel=1e-11, abs=1e-12 * abs(rc[0])), (prec, nm)
+        check_close(fe.cpu().numpy(), rFE.ravel(order="F"), what=f"FE {prec}")
+        check_close(fp.cpu().numpy(), rFP.ravel(order="F"), what=f"FP {prec}")
+    X.close()
+    S.close()
+
+
+def test_xchf_fock_incremental(torch_dev):
+    """The incremental build: rows of a reference density D0 (an FP64 build
+    kept as the reference) plus T(D - D0, D + D0) equal the full build at D.
+    Read in FP64 this is the exact bilinear identity (parity with the oracle
+    at D to 1e-11); read in BF16 the increment's storage error scales with
+    |D - D0|, far below that of a full BF16 build."""
+    torch = torch_dev
+    s = _small(4, seed=85, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1))
+    ne, npb = s["nebf"], s["npbf"]
+    print(f"case xchf fock incremental nebf={ne} npbf={npb}", flush=True)
+    rng = np.random.default_rng(7)
+    D0 = synth.density(ne, 4.0, 3)
+    E = rng.standard_normal((ne, ne))
+    D = np.asfortranarray(D0 + 1e-3 * (E + E.T))
+    DP = synth.density(npb, 1.0, 4)
+    oFE, oFP, oc = oracle.xchf_fock(ne, npb, 4, oracle.xchf_ints(s), D, DP)
+    S = neorx.System(s)
+    X = xchf.XCHF(S, 4, store=("f64", "bf16"))
+    d0, d1, dp = (_dev(torch, a.ravel(order="F")) for a in (D0, D, DP))
+    fe = torch.zeros(ne * ne, dtype=torch.float64, device="cuda")
+    fp = torch.zeros(npb * npb, dtype=torch.float64, device="cuda")
+    c = torch.zeros(14, dtype=torch.float64, device="cuda")
+    X.fock_dev(d0.data_ptr(), dp.data_ptr(), "f64", fe.data_ptr(), fp.data_ptr(), c.data_ptr(), g3pass=1)
+    X.fock_dev(d1.data_ptr(), dp.data_ptr(), "f64", fe.data_ptr(), fp.data_ptr(), c.data_ptr(), g3pass=2)
+    torch.cuda.synchronize()
+    gc = c.cpu().numpy()
+    for k, nm in enumerate(xchf.COMPS):
+        assert gc[k] == pytest.approx(oc[k], rel=1e-11, abs=1e-12 * abs(oc[0])), nm
+    check_close(fe.cpu().numpy(), oFE.ravel(order="F"), what="FE")
+    check_close(fp.cpu().numpy(), oFP.ravel(order="F"), what="FP")
+    # BF16 increment vs a full BF16 build at D
+    X.fock_dev(d1.data_ptr(), dp.data_ptr(), "bf16", fe.data_ptr(), fp.data_ptr(), c.data_ptr(), g3pass=2)
+    torch.cuda.synchronize()
+    e_inc = c.cpu().numpy()[8]
+    X.fock_dev(d1.data_ptr(), dp.data_ptr(), "bf16", fe.data_ptr(), fp.data_ptr(), c.data_ptr())
+    torch.cuda.synchronize()
+    e_full = c.cpu().numpy()[8]
+    err_inc, err_full = abs(e_inc - oc[8]), abs(e_full - oc[8])
+    print(f"  E_GAM3 {oc[8]:.12e}: BF16 increment error {err_inc:.2e}, full BF16 build error {err_full:.2e}")
+    assert err_inc <= 1e-2 * err_full + 1e-14 * abs(oc[8]), (err_inc, err_full)
     X.close()
     S.close()
 
@@ -194,17 +256,23 @@ def test_scf_golden_minis(torch_dev, name):
     _check_golden(g, r)
 
 
-@pytest.mark.parametrize("mode", ["f32", "bf16"])
+@pytest.mark.parametrize("mode", ["f32", "bf16", "inc"])
 def test_scf_mixed_precision(torch_dev, mode):
     """Mixed precision: GAM_3 read in FP32 (or BF16, then FP32) until the DIIS
-    error falls below 1e-5, then FP64; the final energy is the FP64 one."""
+    error falls below 1e-5, then FP64; or (inc) BF16 builds, then BF16-stored
+    increments from FP64 reference builds.  Every mode ends on an FP64 build,
+    so the final energy is the FP64 one."""
     print(f"case scf mixed {mode}", flush=True)
     g, r = _run("c1mini_sp", mode=mode, diis=True, store=("f64", "f32", "bf16"))
-    prec = r["trace"][:, 5]
-    assert prec[0] == {"f32": 1, "bf16": 2}[mode] and prec[-1] == 0, prec
+    prec, kind = r["trace"][:, 5], r["trace"][:, 7]
+    assert prec[0] == {"f32": 1, "bf16": 2, "inc": 2}[mode] and prec[-1] == 0, prec
+    if mode == "inc":
+        # BF16 builds first, increments in between, an FP64 reference last
+        assert kind[0] == 0 and kind[-1] == 1 and (kind == 2).any(), kind
+        assert np.all(prec[kind == 2] == 2) and np.all(prec[kind == 1] == 0), (prec, kind)
     # the first build reads GAM_3 in the reduced storage: E_GAM3 (and the
     # total) of the initial guess move by the storage rounding, the rest not
-    tol = {"f32": 1e-6, "bf16": 1e-2}[mode]
+    tol = {"f32": 1e-6, "bf16": 1e-2, "inc": 1e-2}[mode]
     assert abs(r["comps_init"]["E_GAM3"] - g["initial"]["E_GAM3"]) <= tol * abs(g["initial"]["E_GAM3"]) + PRINT_TOL
     _check_golden(g, r, skip=("E_GAM3", "E_TOTAL"))
 

@@ -523,3 +523,24 @@ def test_input_and_orbital_files():
     os.makedirs(os.path.dirname(out), exist_ok=True)
     xchf.write_mos(out, C)
     assert open(out).read() == open(os.path.join(d, "guessCE.inp")).read()
+
+
+@pytest.mark.parametrize("name", ["f_elec", "f_nuc"])
+def test_oracle_f_tuples_vs_reference_golden(name):
+    """f functions (l = 3): the oracle against the reference's xcalc_GAM{1,2,3}_MD
+    values stored by tests/golden/make_f_tuples.py (the reference takes tens of
+    seconds per f tuple, so they are not recomputed here)."""
+    import json
+    import os
+
+    from _util import golden_system
+
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "f_tuples.json")))[name]
+    s, _ = golden_system(g)
+    for cls, ent in g["tuples"].items():
+        idx = np.array(ent["idx"], dtype=np.int64)
+        oe, os_ = oracle.prim_list(s, int(cls), idx)
+        check_close(oe, np.array(ent["E"]), what=f"{name} cls{cls} E")
+        if ent["S"] is not None:
+            check_close(os_, np.array(ent["S"]), what=f"{name} cls{cls} S")
+
