@@ -169,10 +169,11 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     integral built on the GPU (GM3 = 1.22e10 entries, kept in FP64 and BF16),
     then the SCF with DIIS, each iteration one replay of a captured hipGraph:
       * FP64: GAM_3 read in FP64 every iteration;
-      * mixed (incremental): GAM_3 read in BF16 until the DIIS error < 1e-3,
-        then an FP64 build kept as the reference and BF16-stored increments
-        T(DE - DEref, DE + DEref) from it, a new FP64 reference once the DIIS
-        error < 1e-5, and a last FP64 build that confirms convergence.
+      * mixed (incremental): GAM_3 read in BF16 (sums over its stored
+        entries in FP32) until the DIIS error < 1e-3, then an FP64 build kept
+        as the reference and BF16-stored increments T(DE - DEref, DE + DEref)
+        from it, a new FP64 reference if the DIIS error reaches 1e-6 before
+        convergence, and a last FP64 build that confirms convergence.
     Each mode runs once untimed (graph capture) and once timed; SCF s/iter =
     the timed run's wall time (host clock, first graph launch to the last
     read-back) / its iterations.  The EG3 slab depth is autotuned per storage
@@ -191,9 +192,9 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     runs = {}
     names = {0: "f64", 1: "f32", 2: "bf16"}
     for mode in ("f64", "inc"):
-        X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-5)  # captures the graphs
+        X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-6)  # captures the graphs
         t0 = time.perf_counter()
-        r = X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-5)
+        r = X.scf(mode=mode, diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-6)
         wall = time.perf_counter() - t0
         tr = r["trace"]
         kinds = {}
@@ -214,6 +215,9 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
         "integral_build_s": t_build, "primitive_tuples_built": prims,
         "scf_s_per_iter_fp64": f64["s_per_iter"], "scf_s_per_iter_mixed": mix["s_per_iter"],
         "speedup_s_per_iter_mixed_vs_fp64": f64["s_per_iter"] / mix["s_per_iter"],
+        "speedup_bf16_increment_iteration_vs_fp64": (f64["by_build"]["f64"]["ms_per_iter"] /
+                                                     mix["by_build"]["bf16-increment"]["ms_per_iter"]
+                                                     if "bf16-increment" in mix["by_build"] else None),
         "scf_wall_s_fp64": f64["wall_s"], "scf_wall_s_mixed": mix["wall_s"],
         "speedup_scf_wall_mixed_vs_fp64": f64["wall_s"] / mix["wall_s"],
         "energy_fp64_Eh": f64["energy"], "dE_mixed_vs_fp64_Eh": abs(mix["energy"] - f64["energy"]),
@@ -231,7 +235,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fock", type=str, default="24,8",
                     help="nebf,npbf of the Fock-build contraction timing (FP64 vs mixed); 'off' to skip")
-    ap.add_argument("--scf", type=str, default="off", choices=("off", "c1"),
+    ap.add_argument("--scf", type=str, default="c1", choices=("off", "c1"),
                     help="SCF s/iter of the C1 system, FP64 vs mixed precision (builds C1's integrals first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
@@ -245,6 +249,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 8}))
         return
+    if args.scf != "off":
+        os.environ.setdefault("NEORX_VERBOSE", "1")  # the C1 integral build reports its progress (~4 min)
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
               "(the tuple space shards with no communication, see DESIGN.md)", file=sys.stderr)

@@ -1383,6 +1383,76 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
 // e-e-p tuple, two outputs summed over all (k,l):
 //   ansE1 = -<1/r(e2,p)> with g(e1,p) = gamma_k + gamma_l  +  <1/r12> with the same geminal
 //   ansE2 =  <1/r12> with g(e1,p) = gamma_k, g(e2,p) = gamma_l
+// RXCHF_xcalc_GAM2_MD (rxchf/rxchf_gam_2_OMP.f:546-1120): e1 the special
+// electron, e2 a regular one, p the proton; only the two-geminal terms, in
+// three outputs.  Configuration sb: g(e1,p) = gk on the bra, g(e2,p) = gl on
+// the ket; configuration s1: g(e1,p) = gk + gl (e2 unlinked, so its kinetic
+// and nuclear-attraction integrals factor into e2's one-particle value times
+// the (e1, p) overlap):
+//   ansE1 = gTEg3 + gVECg3 + gVEPg3 + gVEEg1       (s1: e2's one-body terms,
+//           V(e2,p), V(e1,e2))
+//   ansE2 = gTPg + gVPCg + gTEg1 + gVECg1 + gTEg2 + gVECg2 + gVEPg1 + gVEPg2
+//           + gVEEg2                                (sb)
+//   ansS  = gsg                                     (sb overlap)
+template <int NL, int NBC>
+NRX_HD void rxchf2_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE1,
+                         double &ansE2, double &ansS)
+{
+    constexpr int NC = NL - 2; // capacity of the terms without a kinetic operator
+    Sys s2e = mksys(S, 1, e2, 0, 0, 0, 0.0, 0, 0);
+    const double t2 = kin<NL>(s2e, 0, -1, 0.0, 1.0);
+    const double v2 = nuc_sum<NC, NBC>(s2e, 0, S.nat, S.cat, S.zan, true);
+    auto ovl1 = [&](double g) {
+        Sys s1p = mksys(S, 2, e1, p, 0, 0, g, 0, 0);
+        return ovl<NC>(s1p);
+    };
+    double gTPg = 0, gTEg1 = 0, gTEg2 = 0, gTEg3 = 0, gVPCg = 0, gVECg1 = 0, gVECg2 = 0, gVECg3 = 0;
+    double gVEPg1 = 0, gVEPg2 = 0, gVEPg3 = 0, gVEEg1 = 0, gVEEg2 = 0, gsg = 0;
+    for (int ik = 0; ik < S.ngtg; ++ik)
+        for (int il = 0; il < S.ngtg; ++il) {
+            const double gk = S.gamma[ik], gl = S.gamma[il];
+            const double bb = S.bcoef[ik] * S.bcoef[il];
+            Sys sb = mksys(S, 3, e1, e2, p, 0, gk, gl, 0);
+            // e1's kinetic energy (no ket geminal on e1) and the proton's (ket
+            // geminal gl to e2) share the configuration's pass, with the
+            // proton's nuclear attraction and the overlap
+            double kk[2], kq[2], ov, vp;
+            kin_pair<NL, NC, NBC, true>(sb, KinOp{0, 2, 1, {0.0, 0.0}, 1.0}, KinOp{2, 1, 1, {gl, gl}, S.pmass}, PI, kk,
+                                        kq, &ov, &vp, S.nat, S.cat, S.zan, true);
+            gTEg1 += bb * kk[0];
+            gTPg += bb * kq[0];
+            gVPCg += bb * vp;
+            gsg += bb * ov;
+            gTEg2 += bb * kin<NL>(sb, 1, 2, gl, 1.0); // e2, ket geminal gl to p
+            gVECg1 -= bb * nuc_sum<NC, NBC>(sb, 0, S.nat, S.cat, S.zan, true);
+            gVECg2 -= bb * nuc_sum<NC, NBC>(sb, 1, S.nat, S.cat, S.zan, true);
+            double v = coul<NC, NBC>(sb, 0, 2);
+            underflow(v);
+            gVEPg1 -= bb * v;
+            v = coul<NC, NBC>(sb, 1, 2);
+            underflow(v);
+            gVEPg2 -= bb * v;
+            v = coul<NC, NBC>(sb, 0, 1);
+            underflow(v);
+            gVEEg2 += bb * v;
+            if (il < ik) continue;
+            const double w = (il == ik) ? bb : 2.0 * bb; // s1 depends on gk + gl only
+            Sys s1 = mksys(S, 3, e1, e2, p, 0, gk + gl, 0.0, 0);
+            const double o1 = ovl1(gk + gl);
+            gTEg3 += w * (t2 * o1);
+            gVECg3 -= w * (v2 * o1);
+            v = coul<NC, NBC>(s1, 1, 2);
+            underflow(v);
+            gVEPg3 -= w * v;
+            v = coul<NC, NBC>(s1, 0, 1);
+            underflow(v);
+            gVEEg1 += w * v;
+        }
+    ansE1 = gTEg3 + gVECg3 + gVEPg3 + gVEEg1;
+    ansE2 = gTPg + gVPCg + gTEg1 + gVECg1 + gTEg2 + gVECg2 + gVEPg1 + gVEPg2 + gVEEg2;
+    ansS = gsg;
+}
+
 template <int NL, int NBC>
 NRX_HD void gam2ne_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *p, double &ansE1,
                          double &ansE2)
@@ -1514,11 +1584,13 @@ template <> struct Tier<5> : Tier<2> {};
 template <> struct Tier<6> : Tier<3> {};
 template <> struct Tier<7> : Tier<3> {};
 template <> struct Tier<8> : Tier<1> {};
+// class 9: the RXCHF (NBE = 1) e-e-p tuple (kinetic terms as class 2)
+template <> struct Tier<9> : Tier<2> {};
 
 // electrons per tuple of a class
 template <int CLS> constexpr int nel()
 {
-    return (CLS == 4 || CLS == 5) ? 2 : (CLS == 6 || CLS == 7) ? 3 : CLS == 8 ? 1 : CLS;
+    return (CLS == 4 || CLS == 5 || CLS == 9) ? 2 : (CLS == 6 || CLS == 7) ? 3 : CLS == 8 ? 1 : CLS;
 }
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
@@ -1539,6 +1611,9 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
     else if constexpr (CLS == 5) {
         double x, y;
         gam2_core<NL, NBC, true>(S, q[0], q[1], q[2], e, s, x, y);
+    } else if constexpr (CLS == 9) {
+        double x;
+        rxchf2_tuple<NL, NBC>(S, q[0], q[1], q[2], e, x, s);
     } else if constexpr (CLS == 6) {
         double x;
         gam3int_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s, x);
@@ -1548,7 +1623,8 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
 }
 
 // four-output form: (ansE1, ansS, ansE2, ansE3) of class 5, (ans, ans3, ans4,
-// 0) of class 6; x and y are zero for the other classes
+// 0) of class 6, (ansE1, ansS, ansE2, 0) of class 9; x and y are zero for the
+// other classes
 template <int CLS, int NL, int NBC = NB>
 NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, double &s, double &x, double &y)
 {
@@ -1556,6 +1632,8 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
     y = 0.0;
     if constexpr (CLS == 5)
         gam2_core<NL, NBC, true>(S, q[0], q[1], q[2], e, s, x, y);
+    else if constexpr (CLS == 9)
+        rxchf2_tuple<NL, NBC>(S, q[0], q[1], q[2], e, x, s);
     else if constexpr (CLS == 6)
         gam3int_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s, x);
     else

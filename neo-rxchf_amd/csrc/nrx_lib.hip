@@ -286,7 +286,7 @@ extern "C" int neorx_system_order(int64_t handle, int64_t *eorder, int64_t *pord
 // at the capacity its tuples need.  Both passes count in LDS per block and
 // touch each global counter once per block.  Out-of-range list entries are
 // flagged (NaN results) and left out.
-template <int CLS> constexpr int hub_bins() { return (CLS == 2 || CLS == 4 || CLS == 5) ? 5 : 1; }
+template <int CLS> constexpr int hub_bins() { return (CLS == 2 || CLS == 4 || CLS == 5 || CLS == 9) ? 5 : 1; }
 template <int CLS> constexpr int n_bins() { return Tier<CLS>::N * hub_bins<CLS>(); }
 
 template <int CLS> NRX_DI int tuple_bin(const PairRec *const *q)
@@ -517,7 +517,7 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 8) return fail(NEORX_E_ARG, "class %lld not in 1..8", (long long)cls);
+    if (cls < 1 || cls > 9) return fail(NEORX_E_ARG, "class %lld not in 1..9", (long long)cls);
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -529,6 +529,7 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
     case 5: return launch_prims<5, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 6: return launch_prims<6, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 7: return launch_prims<7, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 9: return launch_prims<9, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     default: return launch_prims<8, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     }
 }
@@ -1465,22 +1466,40 @@ template <class T> struct Pack<T, 1> {
     }
     static NRX_DI double get(raw v, int) { return widen(v); }
 };
+template <> struct Pack<bf16_t, 1> {
+    using raw = unsigned short;
+    static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+    {
+        return buf_ld((const bf16_t *)nullptr, r, voff, soff);
+    }
+    static NRX_DI float get(raw v, int) { return __uint_as_float((unsigned)v << 16); }
+};
 template <> struct Pack<bf16_t, 2> {
     using raw = unsigned;
     static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
     {
         return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2);
     }
-    static NRX_DI double get(raw v, int p) { return (double)__uint_as_float(p ? (v & 0xffff0000u) : (v << 16)); }
+    static NRX_DI float get(raw v, int p) { return __uint_as_float(p ? (v & 0xffff0000u) : (v << 16)); }
+};
+
+// Arithmetic type of the sums over c: FP64, except for BF16 storage, whose
+// sums run in FP32 (the storage rounding, 2^-9 relative, dwarfs FP32's; the
+// pass is bound by instruction issue, and FP32 saves the widening to FP64).
+// The sums are widened to FP64 before they are weighted and accumulated.
+template <class T> struct SlabAcc {
+    using type = double;
+};
+template <> struct SlabAcc<bf16_t> {
+    using type = float;
 };
 
 // R rows of the slab, c = c0 .. c0+R-1, for one (b,d) and the lane's P
 // columns: raw loads first (the buffer covers row block b, each row's byte
 // offset is wave-uniform), then the widening and three FMAs per entry
-template <class T, int R, int P>
+template <class T, int R, int P, class A>
 NRX_DI void eg3_rows(__amdgpu_buffer_rsrc_t rb, unsigned row_d, unsigned row_c, unsigned voff, unsigned c0,
-                     const double *__restrict__ xd, const double *__restrict__ xb, const double *const *xf,
-                     double *s1, double *s2, double *s3)
+                     const A *__restrict__ xd, const A *__restrict__ xb, const A *const *xf, A *s1, A *s2, A *s3)
 {
     typename Pack<T, P>::raw r[R];
 #pragma unroll
@@ -1490,7 +1509,7 @@ NRX_DI void eg3_rows(__amdgpu_buffer_rsrc_t rb, unsigned row_d, unsigned row_c, 
     for (int k = 0; k < R; ++k) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const double gv = Pack<T, P>::get(r[k], p);
+            const A gv = (A)Pack<T, P>::get(r[k], p);
             s1[p] += xd[c0 + k] * gv;      // X(c,d)
             s2[p] += xb[c0 + k] * gv;      // X(c,b)
             s3[p] += xf[p][c0 + k] * gv;   // X(c,f)
@@ -1508,7 +1527,10 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
     // LDS: Fw[nw][ne] (per-wavefront rows), Ff[ne^2] (per-column partials),
     // Xf[ne][ne+1] with Xf[f*(ne+1) + c] = X(c,f) (a lane reads its own
     // column f at consecutive c; the odd stride spreads the lanes' columns
-    // over the banks), Yt[ne^2] with Yt[e*ne + x] = Y(e,x)
+    // over the banks), Yt[ne^2] with Yt[e*ne + x] = Y(e,x), and Xs, Xf in
+    // FP32 for the FP32 sums of BF16 storage
+    using A = typename SlabAcc<T>::type;
+    constexpr bool F32 = sizeof(A) == 4;
     extern __shared__ double sm[];
     const unsigned ne = (unsigned)ne_, xs = ne + 1;
     const i64 blk = blockIdx.x;
@@ -1516,14 +1538,20 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
     const unsigned ne2 = ne * ne, ne3 = ne2 * ne, ne4 = ne3 * ne, ncol = ne2 / P;
     const unsigned nw = blockDim.x >> 6;
     double *Fw = sm + (threadIdx.x >> 6) * ne, *Ff = sm + nw * ne, *Xf = Ff + ne2, *Yt = Xf + ne * xs;
+    float *Xs = (float *)(Yt + ne2);
     const T *g = G + blk * (i64)ne4 * ne;
     for (unsigned k = threadIdx.x; k < nw * ne + ne2; k += blockDim.x) sm[k] = 0.0;
     for (unsigned k = threadIdx.x; k < ne2; k += blockDim.x) {
         const unsigned r = k / ne, q = k % ne;
         Xf[r * xs + q] = X[k];     // X(q, r)
+        Xs[r * xs + q] = (float)X[k];
         Yt[k] = Y[q * ne + r];     // Y(r, q)
     }
     __syncthreads();
+    // the X columns of the sums: global FP64 (wave-uniform, scalar loads) or
+    // LDS FP32
+    const A *Xa = F32 ? (const A *)(const void *)Xs : (const A *)(const void *)X;
+    const unsigned xstride = F32 ? xs : ne;
     const bool lead = (threadIdx.x & 63) == 0;
     for (unsigned q0 = 0; q0 < ncol; q0 += blockDim.x) {
         const unsigned qc = q0 + threadIdx.x;
@@ -1531,11 +1559,12 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
         const unsigned ef = on ? qc * P : 0;
         const unsigned e = ef / ne, f = ef % ne; // the P columns share e
         const double *Ye = Yt + e * ne;          // Ye[x] = Y(e,x)
-        const double *xf[P];
+        const A *xf[P];
         double y_ef[P], accf[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            xf[p] = Xf + (f + p) * xs; // xf[p][c] = X(c,f+p)
+            // xf[p][c] = X(c,f+p)
+            xf[p] = F32 ? (const A *)(const void *)(Xs + (f + p) * xs) : (const A *)(const void *)(Xf + (f + p) * xs);
             y_ef[p] = on ? Ye[f + p] : 0.0;
             accf[p] = 0.0;
         }
@@ -1545,33 +1574,35 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
             // row block b of the slab: ne^4 entries, at most 2^31 bytes for
             // nebf <= EG3_MAX_NE
             const __amdgpu_buffer_rsrc_t rb = buf_rsrc(g + (size_t)b * ne4, ne4 * (unsigned)sizeof(T));
-            const double *xb = X + b * ne;
+            const A *xb = Xa + b * xstride;
             double accb = 0.0;
             for (unsigned d = 0; d < ne; ++d) {
                 const double y_ed = on ? Ye[d] : 0.0;
-                double s1[P], s2[P], s3[P];
+                A s1[P], s2[P], s3[P];
 #pragma unroll
-                for (int p = 0; p < P; ++p) s1[p] = s2[p] = s3[p] = 0.0;
+                for (int p = 0; p < P; ++p) s1[p] = s2[p] = s3[p] = (A)0;
                 if (on) {
                     const unsigned row_d = d * ne2 * (unsigned)sizeof(T), row_c = ne3 * (unsigned)sizeof(T);
-                    const double *xd = X + d * ne;
+                    const A *xd = Xa + d * xstride;
                     unsigned c = 0;
                     // U (then eight) independent loads in flight per lane
                     // before the FMAs; the slab is read once, so the loads are
                     // nontemporal
-                    for (; c + U <= ne; c += U) eg3_rows<T, U, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
+                    for (; c + U <= ne; c += U)
+                        eg3_rows<T, U, P, A>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                     if (c + 8 <= ne) {
-                        eg3_rows<T, 8, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
+                        eg3_rows<T, 8, P, A>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                         c += 8;
                     }
-                    for (; c < ne; ++c) eg3_rows<T, 1, P>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
+                    for (; c < ne; ++c) eg3_rows<T, 1, P, A>(rb, row_d, row_c, voff, c, xd, xb, xf, s1, s2, s3);
                 }
                 double vd = 0.0;
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
-                    accb += y_ef[p] * s1[p] - 0.5 * y_ed * s3[p];
-                    accf[p] += 0.25 * y_ed * s2[p] - 0.5 * y_eb * s1[p];
-                    vd += -0.5 * y_ef[p] * s2[p] + 0.25 * y_eb * s3[p];
+                    const double t1 = s1[p], t2 = s2[p], t3 = s3[p];
+                    accb += y_ef[p] * t1 - 0.5 * y_ed * t3;
+                    accf[p] += 0.25 * y_ed * t2 - 0.5 * y_eb * t1;
+                    vd += -0.5 * y_ef[p] * t2 + 0.25 * y_eb * t3;
                 }
                 vd = wave_sum(vd);
                 if (lead) Fw[d] += 3.0 * vd;
@@ -1603,7 +1634,7 @@ __global__ void __launch_bounds__(EG3_MAX_THREADS) k_eg3_slab(i64 ne_, const dou
 // two density layouts
 static inline size_t eg3_lds(i64 ne, unsigned threads)
 {
-    return (size_t)((threads / 64) * ne + 2 * ne * ne + ne * (ne + 1)) * 8;
+    return (size_t)((threads / 64) * ne + 2 * ne * ne + ne * (ne + 1)) * 8 + (size_t)ne * (ne + 1) * 4;
 }
 
 // largest nebf whose e-e-e-p slab (nebf^5 entries) has 32-bit offsets

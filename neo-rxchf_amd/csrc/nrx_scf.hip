@@ -1220,8 +1220,14 @@ int xchf_build(Xchf *x, i64 handle, i64 store)
             const void *g[3] = {x->G3, x->G3f, x->G3h};
             for (int t = 0; t < 3; ++t)
                 if (g[t]) {
-                    rc = neorx_autotune_eg3_dev(ne, np, t, x->Se, g[t], 3, nullptr, nullptr, nullptr);
+                    int64_t best = 0;
+                    double bms = 0.0, all[4] = {};
+                    rc = neorx_autotune_eg3_dev(ne, np, t, x->Se, g[t], 3, &best, &bms, all);
                     if (rc) return rc;
+                    if (g_verbose)
+                        fprintf(stderr, "neorx: EG3 slab depth for storage %d: %lld (%.3f ms; 8/12/16/24: %.3f %.3f %.3f "
+                                        "%.3f ms)\n",
+                                t, (long long)best, bms, all[0], all[1], all[2], all[3]);
                 }
         }
         if (!(store & 1)) {
