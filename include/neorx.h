@@ -74,6 +74,8 @@ int neorx_system_destroy(int64_t handle);
  *   cls 2 (e-e-p)   : xcalc_GAM2_MD, source/gam_2_OMP.f:546   -> ansE, ansS
  *   cls 3 (e-e-e-p) : xcalc_GAM3_MD, source/gam_3_OMP.f:617   -> ansE
  *   cls 4 (e-e-p)   : RXCHFne_xcalc_GAM2_MD, rxchfne/rxchfne_gam_2_OMP.f:2 -> ansE1, ansE2
+ *   cls 9 (e-e-p)   : RXCHF_xcalc_GAM2_MD, rxchf/rxchf_gam_2_OMP.f:546 (with neorx_prim_list3_dev)
+ *                     -> ansE1, ansS, ansE2
  * d_idx holds 0-based primitive indices per tuple, 2*(n+1) int64 each for n
  * electrons (n = cls, 2 for cls 4), in the order (ie1..ie_n, ip, je1..je_n, jp). */
 int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
@@ -238,6 +240,32 @@ int neorx_rxcuhfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *daalp, con
                             const double *dp, const double *gm2, const double *gm2ex, double *fbe, double *e);
 /* RXCHFne_FAE_GAMee, source/rxchfne/rxchfne_hfterms.f:2 */
 int neorx_rxchfne_fae_gamee(int64_t nebf, const double *gam_ee, const double *dae, double *fae, double *e);
+
+/* ---- RXCHF, one special electron (reference semantics, source/rxchf/) -----
+ * The RXCHF e-e-p tuple RXCHF_xcalc_GAM2_MD (rxchf/rxchf_gam_2_OMP.f:546) is
+ * class 9 of neorx_prim_list3_dev: outputs (ansE1, ansS, ansE2); electron 1
+ * is the special electron. */
+/* RXCHF_GAM2_IC1, source/rxchf/rxchf_gam2_IC1_OMP.f:2: GM2_1ICR (contracted
+ * ansE1), GM2_2ICR and GM2sICR ((x12 + x21)/2 of the contracted ansE2, ansS) */
+int neorx_rxchf_gam2_ic1(int64_t handle, double *gm2_1, double *gm2_2, double *gm2s);
+int neorx_rxchf_gam2_ic1_dev(int64_t handle, double *d_gm2_1, double *d_gm2_2, double *d_gm2s, void *stream);
+/* RXCHF_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2, source/rxchf/rxchf_omg2.f:2, :231,
+ * :449 (dae: regular, dbe: special electron density): F and SF accumulate,
+ * e and s (the energy and overlap terms) overwritten */
+int neorx_rxchf_fae_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                         const double *gm2_1, const double *gm2_2, const double *gm2s, double *fae, double *sae,
+                         double *e, double *s);
+int neorx_rxchf_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                        const double *gm2_1, const double *gm2_2, const double *gm2s, double *fp, double *sp,
+                        double *e, double *s);
+int neorx_rxchf_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                         const double *gm2_1, const double *gm2_2, const double *gm2s, double *fbe, double *sbe,
+                         double *e, double *s);
+/* the same on device arrays: which 0 FAE, 1 FP, 2 FBE; d_es[0] = E,
+ * d_es[1] = S; stream-ordered scratch, not synchronised */
+int neorx_rxchf_omg2_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                         const double *d_dp, const double *d_gm2_1, const double *d_gm2_2, const double *d_gm2s,
+                         double *d_f, double *d_sf, double *d_es, void *stream);
 
 /* ---- direct-SCF routines (reference semantics) -------------------------- */
 /* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per

@@ -2907,6 +2907,285 @@ extern "C" int neorx_rxchfne_fae_gamee(int64_t nebf, const double *gam_ee, const
 }
 
 // ------------------------------------------------------------------
+// RXCHF (rxchf/): one special electron.  Tuple class 9 is
+// RXCHF_xcalc_GAM2_MD (ansE1, ansS, ansE2); RXCHF_GAM2_IC1 contracts the
+// three outputs (RXCHF_contract_omega2_conv, rxchf_gam2_CONV_OMP.f:388-580)
+// and keeps GM2_1 as is and GM2_2, GM2s as pair-swap averages
+// (rxchf_gam2_IC1_OMP.f:158-190).
+// ------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_rxchf_symm(i64 ne, i64 n, const double *__restrict__ X1,
+                                                    const double *__restrict__ X2, const double *__restrict__ XS,
+                                                    double *__restrict__ G1, double *__restrict__ G2,
+                                                    double *__restrict__ GS)
+{
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (i64)gridDim.x * blockDim.x) {
+        const i64 ne2 = ne * ne;
+        const i64 b = t / (ne2 * ne2), p1 = (t / ne2) % ne2, p2 = t % ne2;
+        const i64 t21 = (b * ne2 + p2) * ne2 + p1;
+        G1[t] = X1[t];
+        G2[t] = (X2[t] + X2[t21]) / 2.0;
+        GS[t] = (XS[t] + XS[t21]) / 2.0;
+    }
+}
+
+extern "C" int neorx_rxchf_gam2_ic1_dev(int64_t handle, double *d_gm2_1, double *d_gm2_2, double *d_gm2s, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (!d_gm2_1 || !d_gm2_2 || !d_gm2s) return fail(NEORX_E_ARG, "null output");
+    if (s->d.nebf1 != s->d.nebf) return fail(NEORX_E_ARG, "RXCHF_GAM2_IC1 takes a one-basis handle");
+    const i64 ne = s->d.nebf, np = s->d.npbf, ne4 = ne * ne * ne * ne, ng2 = ne4 * np * np;
+    hipStream_t st = (hipStream_t)stream;
+    i64 cap = slab_cap(s, 2);
+    DevBuf X1, X2, XS, Y1, YS, Y2;
+    int rc = X1.alloc(ng2 * 8);
+    if (!rc) rc = X2.alloc(ng2 * 8);
+    if (!rc) rc = XS.alloc(ng2 * 8);
+    if (!rc) rc = Y1.alloc(cap * 8);
+    if (!rc) rc = YS.alloc(cap * 8);
+    if (!rc) rc = Y2.alloc(cap * 8);
+    if (rc) return rc;
+    for (auto &sl : make_slabs(s, 2, 0, np * np, cap)) {
+        const i64 R0 = slab_start(s, 2, sl.first), R1 = slab_start(s, 2, sl.second);
+        // primitive values of the slab: ansE1, ansS, ansE2
+        rc = launch_prims<9, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y1.d(), YS.d(), st, Y2.d());
+        if (rc) return rc;
+        const i64 n = (sl.second - sl.first) * ne4;
+        hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y1.d(), YS.d(),
+                           X1.d(), XS.d(), (i64)0, 0);
+        // the third output through the same loops (both of its outputs are X2)
+        hipLaunchKernelGGL(k_contract2, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y2.d(), Y2.d(),
+                           X2.d(), X2.d(), (i64)0, 0);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rxchf_symm, dim3(grid_blocks(ng2, 256)), dim3(256), 0, st, ne, ng2, X1.d(), X2.d(), XS.d(),
+                       d_gm2_1, d_gm2_2, d_gm2s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchf_gam2_ic1(int64_t handle, double *gm2_1, double *gm2_2, double *gm2s)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    const i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * ne * ne * np * np;
+    DevBuf G1, G2, GS;
+    int rc = G1.alloc(n * 8);
+    if (!rc) rc = G2.alloc(n * 8);
+    if (!rc) rc = GS.alloc(n * 8);
+    if (!rc) rc = neorx_rxchf_gam2_ic1_dev(handle, G1.d(), G2.d(), GS.d(), 0);
+    if (!rc) rc = copy_out(gm2_1, G1.d(), n);
+    if (!rc) rc = copy_out(gm2_2, G2.d(), n);
+    if (!rc) rc = copy_out(gm2s, GS.d(), n);
+    if (!rc) {
+        res_keep(G1, gm2_1, n);
+        res_keep(G2, gm2_2, n);
+        res_keep(GS, gm2s, n);
+    }
+    return rc;
+}
+
+// RXCHF_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2 (rxchf_omg2.f:2-660).  Per entry of
+// block (ip,jp): w1 = val1 - val2/2 and ws = -Sval/2 with val1 =
+// GM2_1(i1 j1 i2 j2), val2 = GM2_2(i1 j2 i2 j1), Sval = GM2s(i1 j2 i2 j1).
+// Row sums over (i2,j2) weighted by DAE and column sums over (i1,j1) weighted
+// by DBE, per block, in a fixed order; then
+//   FAE(i2,j2) += 2 sum_pq DP C1,   SAE += -2 sum_pq DP Cs   (FAE's 2 val1 - val2, SAE's Sval)
+//   FP(pq)     += sum_r DBE R1,     SP  += sum_r DBE Rs
+//   FBE(i1,j1) += sum_pq DP R1,     SBE += sum_pq DP Rs
+//   E = sum_pq DP sum_r DBE R1,     S   = sum_pq DP sum_r DBE Rs.
+// rows[pq][r][2] (r = i1*ne + j1), cols[pq][c][2] (c = i2*ne + j2)
+__global__ void __launch_bounds__(64) k_rxchf_rows(i64 ne, i64 np, const double *__restrict__ DAE,
+                                                   const double *__restrict__ G1, const double *__restrict__ G2,
+                                                   const double *__restrict__ GS, double *__restrict__ rows)
+{
+    const i64 ne2 = ne * ne, row = blockIdx.x; // (pq, i1, j1)
+    if (row >= np * np * ne2) return;
+    const i64 pq = row / ne2, i1 = (row / ne) % ne, j1 = row % ne;
+    const double *g1 = G1 + row * ne2, *g2 = G2 + pq * ne2 * ne2, *gs = GS + pq * ne2 * ne2;
+    double r1 = 0.0, rs = 0.0;
+    for (i64 c = threadIdx.x; c < ne2; c += 64) {
+        const i64 i2 = c / ne, j2 = c % ne;
+        const i64 x21 = ((i1 * ne + j2) * ne + i2) * ne + j1;
+        const double dae = DAE[j2 * ne + i2];
+        r1 += dae * (g1[c] - 0.5 * g2[x21]);
+        rs += dae * (-0.5 * gs[x21]);
+    }
+    r1 = wave_sum(r1);
+    rs = wave_sum(rs);
+    if (threadIdx.x == 0) {
+        rows[2 * row] = r1;
+        rows[2 * row + 1] = rs;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rxchf_cols(i64 ne, i64 np, const double *__restrict__ DBE,
+                                                    const double *__restrict__ G1, const double *__restrict__ G2,
+                                                    const double *__restrict__ GS, double *__restrict__ cols)
+{
+    const i64 ne2 = ne * ne;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; // (pq, i2, j2)
+    if (t >= np * np * ne2) return;
+    const i64 pq = t / ne2, c = t % ne2, i2 = c / ne, j2 = c % ne;
+    const double *g1 = G1 + pq * ne2 * ne2, *g2 = G2 + pq * ne2 * ne2, *gs = GS + pq * ne2 * ne2;
+    double c1 = 0.0, cs = 0.0;
+    for (i64 i1 = 0; i1 < ne; ++i1)
+        for (i64 j1 = 0; j1 < ne; ++j1) {
+            const i64 x12 = (i1 * ne + j1) * ne2 + c, x21 = ((i1 * ne + j2) * ne + i2) * ne + j1;
+            const double dbe = DBE[j1 * ne + i1];
+            c1 += dbe * (g1[x12] - 0.5 * g2[x21]);
+            cs += dbe * (-0.5 * gs[x21]);
+        }
+    cols[2 * t] = c1;
+    cols[2 * t + 1] = cs;
+}
+
+// which: 0 FAE/SAE, 1 FP/SP, 2 FBE/SBE; F, SF accumulate; es[0] = E, es[1] = S
+__global__ void __launch_bounds__(64) k_rxchf_finish(int which, i64 ne, i64 np, const double *__restrict__ DBE,
+                                                     const double *__restrict__ DP, const double *__restrict__ rows,
+                                                     const double *__restrict__ cols, double *__restrict__ F,
+                                                     double *__restrict__ SF, double *__restrict__ es)
+{
+    const i64 ne2 = ne * ne, np2 = np * np;
+    if (which == 0) {
+        for (i64 c = threadIdx.x; c < ne2; c += 64) {
+            double f = 0.0, sf = 0.0;
+            for (i64 pq = 0; pq < np2; ++pq) {
+                const double dp = DP[(pq % np) * np + pq / np];
+                f += dp * cols[2 * (pq * ne2 + c)];
+                sf += dp * cols[2 * (pq * ne2 + c) + 1];
+            }
+            const i64 i2 = c / ne, j2 = c % ne;
+            F[j2 * ne + i2] += 2.0 * f;
+            SF[j2 * ne + i2] += -2.0 * sf;
+        }
+    } else if (which == 1) {
+        for (i64 pq = threadIdx.x; pq < np2; pq += 64) {
+            double f = 0.0, sf = 0.0;
+            for (i64 r = 0; r < ne2; ++r) {
+                const double dbe = DBE[(r % ne) * ne + r / ne];
+                f += dbe * rows[2 * (pq * ne2 + r)];
+                sf += dbe * rows[2 * (pq * ne2 + r) + 1];
+            }
+            const i64 ip = pq / np, jp = pq % np;
+            F[jp * np + ip] += f;
+            SF[jp * np + ip] += sf;
+        }
+    } else {
+        for (i64 r = threadIdx.x; r < ne2; r += 64) {
+            double f = 0.0, sf = 0.0;
+            for (i64 pq = 0; pq < np2; ++pq) {
+                const double dp = DP[(pq % np) * np + pq / np];
+                f += dp * rows[2 * (pq * ne2 + r)];
+                sf += dp * rows[2 * (pq * ne2 + r) + 1];
+            }
+            const i64 i1 = r / ne, j1 = r % ne;
+            F[j1 * ne + i1] += f;
+            SF[j1 * ne + i1] += sf;
+        }
+    }
+    // energies: lane-strided partial sums over pq, then a fixed-order wave sum
+    double e = 0.0, sv = 0.0;
+    for (i64 pq = threadIdx.x; pq < np2; pq += 64) {
+        const double dp = DP[(pq % np) * np + pq / np];
+        double a = 0.0, b = 0.0;
+        for (i64 r = 0; r < ne2; ++r) {
+            const double dbe = DBE[(r % ne) * ne + r / ne];
+            a += dbe * rows[2 * (pq * ne2 + r)];
+            b += dbe * rows[2 * (pq * ne2 + r) + 1];
+        }
+        e += dp * a;
+        sv += dp * b;
+    }
+    e = wave_sum(e);
+    sv = wave_sum(sv);
+    if (threadIdx.x == 0) {
+        es[0] = e;
+        es[1] = sv;
+    }
+}
+
+extern "C" int neorx_rxchf_omg2_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dae,
+                                    const double *d_dbe, const double *d_dp, const double *d_gm2_1,
+                                    const double *d_gm2_2, const double *d_gm2s, double *d_f, double *d_sf,
+                                    double *d_es, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0 || which < 0 || which > 2) return fail(NEORX_E_ARG, "RXCHF OMG2: bad arguments");
+    if (!d_dae || !d_dbe || !d_dp || !d_gm2_1 || !d_gm2_2 || !d_gm2s || !d_f || !d_sf || !d_es)
+        return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const i64 nrow = np * np * ne * ne;
+    StreamBuf w;
+    rc = w.alloc((size_t)(4 * nrow) * 8, st);
+    if (rc) return rc;
+    double *rows = (double *)w.p, *cols = rows + 2 * nrow;
+    hipLaunchKernelGGL(k_rxchf_rows, dim3((unsigned)nrow), dim3(64), 0, st, ne, np, d_dae, d_gm2_1, d_gm2_2, d_gm2s,
+                       rows);
+    hipLaunchKernelGGL(k_rxchf_cols, dim3(nblocks(nrow, 256)), dim3(256), 0, st, ne, np, d_dbe, d_gm2_1, d_gm2_2,
+                       d_gm2s, cols);
+    hipLaunchKernelGGL(k_rxchf_finish, dim3(1), dim3(64), 0, st, (int)which, ne, np, d_dbe, d_dp, rows, cols, d_f,
+                       d_sf, d_es);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int rxchf_omg2_host(int which, i64 ne, i64 np, const double *dae, const double *dbe, const double *dp,
+                           const double *g1, const double *g2, const double *gs, double *f, double *sf, double *e,
+                           double *sv)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ng2 = ne * ne * ne * ne * np * np, nf = which == 1 ? np * np : ne * ne;
+    DevBuf dA, dB, dP, d1, d2, dS, dF, dSF, dE;
+    rc = h2d(dA, dae, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(d1, g1, ng2);
+    if (!rc) rc = h2d(d2, g2, ng2);
+    if (!rc) rc = h2d(dS, gs, ng2);
+    if (!rc) rc = dF.alloc(nf * 8);
+    if (!rc) rc = dSF.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(16);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dF.p, f, nf * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dSF.p, sf, nf * 8, hipMemcpyHostToDevice));
+    rc = neorx_rxchf_omg2_dev(which, ne, np, dA.d(), dB.d(), dP.d(), d1.d(), d2.d(), dS.d(), dF.d(), dSF.d(), dE.d(),
+                              0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    double es[2];
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sf, dSF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e = es[0];
+    *sv = es[1];
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchf_fae_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                    const double *gm2_1, const double *gm2_2, const double *gm2s, double *fae,
+                                    double *sae, double *e, double *s)
+{
+    return rxchf_omg2_host(0, nebf, npbf, dae, dbe, dp, gm2_1, gm2_2, gm2s, fae, sae, e, s);
+}
+extern "C" int neorx_rxchf_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                   const double *gm2_1, const double *gm2_2, const double *gm2s, double *fp, double *sp,
+                                   double *e, double *s)
+{
+    return rxchf_omg2_host(1, nebf, npbf, dae, dbe, dp, gm2_1, gm2_2, gm2s, fp, sp, e, s);
+}
+extern "C" int neorx_rxchf_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                    const double *gm2_1, const double *gm2_2, const double *gm2s, double *fbe,
+                                    double *sbe, double *e, double *s)
+{
+    return rxchf_omg2_host(2, nebf, npbf, dae, dbe, dp, gm2_1, gm2_2, gm2s, fbe, sbe, e, s);
+}
+
+// ------------------------------------------------------------------
 // RXCHFmult (rxchfmult/): more than one special electron.  One system handle
 // holds both electron bases: primitives [0, npebf) and contracted functions
 // [0, nebf) are the regular basis, the special-electron basis follows them.

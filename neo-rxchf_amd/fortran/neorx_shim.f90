@@ -148,6 +148,30 @@ module neorx_capi
       real(c_double) :: g2s(*), gee(*), g4(*)
       integer(c_int) :: neorx_omg4_icr
     end function
+    function neorx_rxchf_gam2_ic1(h, g1, g2, gs) bind(C, name="neorx_rxchf_gam2_ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g1(*), g2(*), gs(*)
+      integer(c_int) :: neorx_rxchf_gam2_ic1
+    end function
+    function neorx_rxchf_fae_omg2(ne, np, da, db, dp, g1, g2, gs, f, sf, e, sv) bind(C, name="neorx_rxchf_fae_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxchf_fae_omg2
+    end function
+    function neorx_rxchf_fp_omg2(ne, np, da, db, dp, g1, g2, gs, f, sf, e, sv) bind(C, name="neorx_rxchf_fp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxchf_fp_omg2
+    end function
+    function neorx_rxchf_fbe_omg2(ne, np, da, db, dp, g1, g2, gs, f, sf, e, sv) bind(C, name="neorx_rxchf_fbe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxchf_fbe_omg2
+    end function
     function neorx_rxchfne_gam2_ic1(h, g, gx) bind(C, name="neorx_rxchfne_gam2_ic1")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: h
@@ -699,6 +723,60 @@ subroutine RXCHFne_GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1,
                    KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'RXCHFne_GAM2_IC1')
   call neorx_check(neorx_rxchfne_gam2_ic1(h, GM2ICR, GM2exICR), 'RXCHFne_GAM2_IC1')
   call neorx_check(neorx_system_destroy(h), 'RXCHFne_GAM2_IC1')
+end subroutine
+
+! RXCHF_GAM2_IC1, source/rxchf/rxchf_gam2_IC1_OMP.f:2-250
+subroutine RXCHF_GAM2_IC1(Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                          KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                          GM2_1ICR, GM2_2ICR, GM2sICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng2, ng2prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'RXCHF_GAM2_IC1')
+  call neorx_check(neorx_rxchf_gam2_ic1(h, GM2_1ICR, GM2_2ICR, GM2sICR), 'RXCHF_GAM2_IC1')
+  call neorx_check(neorx_system_destroy(h), 'RXCHF_GAM2_IC1')
+end subroutine
+
+! RXCHF_FAE_OMG2, source/rxchf/rxchf_omg2.f:2-110
+subroutine RXCHF_FAE_OMG2(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FAE, SAE, &
+                          E_AE_OMG2, S_AE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2)
+  real(c_double) :: FAE(nebf, nebf), SAE(nebf, nebf), E_AE_OMG2, S_AE_OMG2
+  call neorx_check(neorx_rxchf_fae_omg2(nebf, npbf, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FAE, SAE, E_AE_OMG2, &
+                   S_AE_OMG2), 'RXCHF_FAE_OMG2')
+end subroutine
+
+! RXCHF_FP_OMG2, source/rxchf/rxchf_omg2.f:231-330
+subroutine RXCHF_FP_OMG2(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FP, SP, E_P_OMG2, &
+                         S_P_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2)
+  real(c_double) :: FP(npbf, npbf), SP(npbf, npbf), E_P_OMG2, S_P_OMG2
+  call neorx_check(neorx_rxchf_fp_omg2(nebf, npbf, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FP, SP, E_P_OMG2, &
+                   S_P_OMG2), 'RXCHF_FP_OMG2')
+end subroutine
+
+! RXCHF_FBE_OMG2, source/rxchf/rxchf_omg2.f:449-548
+subroutine RXCHF_FBE_OMG2(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FBE, SBE, &
+                          E_BE_OMG2, S_BE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2)
+  real(c_double) :: FBE(nebf, nebf), SBE(nebf, nebf), E_BE_OMG2, S_BE_OMG2
+  call neorx_check(neorx_rxchf_fbe_omg2(nebf, npbf, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FBE, SBE, E_BE_OMG2, &
+                   S_BE_OMG2), 'RXCHF_FBE_OMG2')
 end subroutine
 
 ! RXCHFne_FAE_OMG2, source/rxchfne/rxchfne_omg2.f:2-116

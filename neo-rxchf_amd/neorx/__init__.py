@@ -51,6 +51,9 @@ def load():
     _sig("neorx_init", [_I64])
     _sig("neorx_finalize", [])
     _sig("neorx_release_resident", [])
+    _sig("neorx_rxchf_gam2_ic1", [_I64, _P, _P, _P])
+    for nm in ("neorx_rxchf_fae_omg2", "neorx_rxchf_fp_omg2", "neorx_rxchf_fbe_omg2"):
+        _sig(nm, [_I64, _I64] + [_P] * 10)
     _sig("neorx_transfer_bytes", [_P, _P])
     _sig("neorx_system_create", [_I64, _I64, _I64, _I64, _I64, _D] + [_P] * 15 + [_P])
     _sig("neorx_system_destroy", [_I64])
@@ -286,6 +289,15 @@ class System:
         _check(_lib.neorx_gam_ee(self.handle, _p(g)))
         return g
 
+    def RXCHF_GAM2_IC1(self):
+        """RXCHF_GAM2_IC1 (source/rxchf/rxchf_gam2_IC1_OMP.f:2): (GM2_1ICR,
+        GM2_2ICR, GM2sICR)."""
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g1 = _np.zeros(ne ** 4 * npb * npb)
+        g2, gs = _np.zeros_like(g1), _np.zeros_like(g1)
+        _check(_lib.neorx_rxchf_gam2_ic1(self.handle, _p(g1), _p(g2), _p(gs)))
+        return g1, g2, gs
+
     def RXCHFne_GAM2_IC1(self):
         """RXCHFne_GAM2_IC1 (source/rxchfne/rxchfne_gam2_IC1_OMP.f:2):
         returns (GM2ICR, GM2exICR)."""
@@ -333,6 +345,18 @@ def GAM4_DSCF(nebf, npbf, gam_2s, gam_ee, DE, DP, focke, fockp):
     _check(_lib.neorx_gam4_dscf(nebf, npbf, _p(_np.ascontiguousarray(gam_2s)), _p(_np.ascontiguousarray(gam_ee)),
                                 _p(DE), _p(DP), _p(focke), _p(fockp), _p(e)))
     return float(e[0])
+
+
+def RXCHF_OMG2(which, nebf, npbf, DAE, DBE, DP, G1, G2, Gs, F, SF):
+    """RXCHF_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2 (source/rxchf/rxchf_omg2.f:2,
+    :231, :449) for which = 0, 1, 2: F and SF accumulate in place; returns
+    (E, S)."""
+    load()
+    fn = (_lib.neorx_rxchf_fae_omg2, _lib.neorx_rxchf_fp_omg2, _lib.neorx_rxchf_fbe_omg2)[which]
+    e, sv = _np.zeros(1), _np.zeros(1)
+    _check(fn(nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(G1)),
+              _p(_np.ascontiguousarray(G2)), _p(_np.ascontiguousarray(Gs)), _p(F), _p(SF), _p(e), _p(sv)))
+    return float(e[0]), float(sv[0])
 
 
 def transfer_bytes():

@@ -833,6 +833,84 @@ static void rxne_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *
     *ansE2 = gVEEg2;
 }
 
+/* RXCHF_xcalc_GAM2_MD (rxchf/rxchf_gam_2_OMP.f:546-1120): particle 1 the
+ * special electron, 2 a regular electron, 3 the proton; the two-geminal
+ * terms only (:700-1106), in three outputs (:1108-1118). */
+static void rxchf2_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ansE1, double *ansE2,
+                        double *ansS)
+{
+    const prim_t *b1 = &b[0], *b2 = &b[1], *b3 = &b[2], *k1 = &k[0], *k2 = &k[1], *k3 = &k[2];
+    double gTPg = 0, gTEg1 = 0, gTEg2 = 0, gTEg3 = 0, gVPCg = 0, gVECg1 = 0, gVECg2 = 0, gVECg3 = 0;
+    double gVEPg1 = 0, gVEPg2 = 0, gVEPg3 = 0, gVEEg1 = 0, gVEEg2 = 0, gsg = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            /* G3_MD_KE(3,1,2; gamA12, gamB13), pmass (:722) */
+            double xgTPg = g3(OP_KIN, b3, b1, b2, k3, k1, k2, gk, 0, 0, 0, gl, 0, NULL, S->pmass);
+            /* G3_MD_KE(1,2,3; gamA13, gamB23) (:737) */
+            double xgTEg1 = g3(OP_KIN, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            /* G3_MD_KE(2,1,3; gamA23, gamB13) (:754) */
+            double xgTEg2 = g3(OP_KIN, b2, b1, b3, k2, k1, k3, 0, 0, gk, 0, gl, 0, NULL, 1.0);
+            /* G3_MD_KE(2,1,3; gamA23, gamB23) (:770) */
+            double xgTEg3 = g3(OP_KIN, b2, b1, b3, k2, k1, k3, 0, 0, gk, 0, 0, gl, NULL, 1.0);
+            double xgVPCg = 0, xgVECg1 = 0, xgVECg2 = 0, xgVECg3 = 0;
+            for (i64 c = 0; c < S->nat; ++c) {
+                const double *C = S->cat + 3 * c;
+                double z = S->zan[c], v;
+                v = g3(OP_NUC, b3, b1, b2, k3, k1, k2, gk, 0, 0, 0, gl, 0, C, 1.0); /* :802 */
+                underflow(&v);
+                xgVPCg += z * v;
+                v = g3(OP_NUC, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, C, 1.0); /* :821 */
+                underflow(&v);
+                xgVECg1 -= z * v;
+                v = g3(OP_NUC, b2, b1, b3, k2, k1, k3, 0, 0, gk, 0, gl, 0, C, 1.0); /* :838 */
+                underflow(&v);
+                xgVECg2 -= z * v;
+                v = g3(OP_NUC, b2, b1, b3, k2, k1, k3, 0, 0, gk, 0, 0, gl, C, 1.0); /* :857 */
+                underflow(&v);
+                xgVECg3 -= z * v;
+            }
+            /* G3_MD_xgVeeg(1,3,2; gamA12, gamB23): 1/r(e1,p) (:876) */
+            double xgVEPg1 = g3(OP_COUL, b1, b3, b2, k1, k3, k2, gk, 0, 0, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEPg1);
+            xgVEPg1 = -xgVEPg1;
+            /* G3_MD_xgVeeg(2,3,1; gamA23, gamB12): 1/r(e2,p) (:893) */
+            double xgVEPg2 = g3(OP_COUL, b2, b3, b1, k2, k3, k1, 0, 0, gk, gl, 0, 0, NULL, 1.0);
+            underflow(&xgVEPg2);
+            xgVEPg2 = -xgVEPg2;
+            /* G3_MD_xgVeeg(2,3,1; gamA23, gamB23) (:910) */
+            double xgVEPg3 = g3(OP_COUL, b2, b3, b1, k2, k3, k1, 0, 0, gk, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEPg3);
+            xgVEPg3 = -xgVEPg3;
+            /* G3_MD_xgVeeg(1,2,3; gamA13, gamB13): 1/r(e1,e2) (:929) */
+            double xgVEEg1 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, gl, 0, NULL, 1.0);
+            underflow(&xgVEEg1);
+            /* G3_MD_xgVeeg(1,2,3; gamA13, gamB23) (:945) */
+            double xgVEEg2 = g3(OP_COUL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            underflow(&xgVEEg2);
+            /* G3_MD_xggs(1,2,3; gamA13, gamB23) (:962) */
+            double xgsg = g3(OP_OVL, b1, b2, b3, k1, k2, k3, 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            gTPg += bb * xgTPg;
+            gTEg1 += bb * xgTEg1;
+            gTEg2 += bb * xgTEg2;
+            gTEg3 += bb * xgTEg3;
+            gVPCg += bb * xgVPCg;
+            gVECg1 += bb * xgVECg1;
+            gVECg2 += bb * xgVECg2;
+            gVECg3 += bb * xgVECg3;
+            gVEPg1 += bb * xgVEPg1;
+            gVEPg2 += bb * xgVEPg2;
+            gVEPg3 += bb * xgVEPg3;
+            gVEEg1 += bb * xgVEEg1;
+            gVEEg2 += bb * xgVEEg2;
+            gsg += bb * xgsg;
+        }
+    *ansE1 = gTEg3 + gVECg3 + gVEPg3 + gVEEg1;
+    *ansE2 = gTPg + gVPCg + gTEg1 + gVECg1 + gTEg2 + gVECg2 + gVEPg1 + gVEPg2 + gVEEg2;
+    *ansS = gsg;
+}
+
 /* xcalc_GAM3_MD (gam_3_OMP.f:617-983): particles e1, e2, e3, p */
 static void gam3_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans)
 {
@@ -962,7 +1040,7 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
 {
     /* cls 4: the RXCHF-ne e-e-p tuple; cls 5, 6: the RXCHFmult e-e-p and
      * e-e-e-p tuples (primitive indices over one combined list) */
-    int nel = (cls == 4 || cls == 5) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls;
+    int nel = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls;
     int w = 2 * (nel + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
@@ -990,6 +1068,8 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
             gam3xchf_prim(Sp, b, k, &e);
         else if (cls == 8)
             gam1_prim_impl(Sp, 1, &b[0], &b[1], &k[0], &k[1], &e, &s);
+        else if (cls == 9) /* (ansE1, ansS, ansE2) */
+            rxchf2_prim(Sp, b, k, &e, &e2, &s);
         else
             gam3_prim(Sp, b, k, &e);
         outE[t] = e;
@@ -2760,3 +2840,110 @@ i64 orc_xcscf(i64 nebf, i64 npbf, i64 nelec, i64 nucst, double enuc, const doubl
     free(CE); free(CP); free(evE); free(evP); free(h);
     return done ? it - 1 : -(it - 1);
 }
+
+/* ------------------------------------------------------------------------
+ * RXCHF (source/rxchf/, one special electron): the e-e-p tensors and their
+ * Fock contractions.
+ * ------------------------------------------------------------------------ */
+/* RXCHF_GAM2_IC1 (rxchf_gam2_IC1_OMP.f:2-250) with RXCHF_contract_omega2_conv
+ * (rxchf_gam2_CONV_OMP.f:388-580): the contracted ansE1 as is (GM2_1), the
+ * pair-swap averages (x12 + x21)/2 of the contracted ansE2 (GM2_2) and ansS
+ * (GM2s) */
+void orc_rxchf_gam2_ic1(BAS_ARGS, SYS_ARGS, double *GM2_1, double *GM2_2, double *GM2s)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng2 = ne * ne * ne * ne * np * np;
+    double *X2 = (double *)malloc(sizeof(double) * ng2), *XS = (double *)malloc(sizeof(double) * ng2);
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            double o1 = 0.0, o2 = 0.0, os = 0.0;
+                            for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                        for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2) {
+                                            prim_t b[3] = {eprim(&B, a1), eprim(&B, a2), pprim(&B, ip)};
+                                            prim_t k[3] = {eprim(&B, c1), eprim(&B, c2), pprim(&B, jp)};
+                                            double e1, e2, sv;
+                                            rxchf2_prim(&S, b, k, &e1, &e2, &sv);
+                                            double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                        agebfcc[a2] * agebfcc[c2];
+                                            o1 += e1 * cf;
+                                            o2 += e2 * cf;
+                                            os += sv * cf;
+                                        }
+                            i64 ia = ix2(ne, np, ip, jp, i1, j1, i2, j2);
+                            GM2_1[ia] = o1;
+                            X2[ia] = o2;
+                            XS[ia] = os;
+                        }
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2) {
+                            i64 a12 = ix2(ne, np, ip, jp, i1, j1, i2, j2), a21 = ix2(ne, np, ip, jp, i2, j2, i1, j1);
+                            GM2_2[a12] = (X2[a12] + X2[a21]) / 2.0;
+                            GM2s[a12] = (XS[a12] + XS[a21]) / 2.0;
+                        }
+    free(X2);
+    free(XS);
+}
+
+#define RXA(i, j) DAE[(j) * ne + (i)]
+#define RXB(i, j) DBE[(j) * ne + (i)]
+#define RXP(i, j) DP[(j) * np + (i)]
+/* RXCHF_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2 (rxchf_omg2.f:2-660); which = 0, 1,
+ * 2 selects the routine.  Per entry (rxchf_omg2.f:193-227, :411-445,
+ * :629-663): val1 = GM2_1(i1 j1 i2 j2), val2 = GM2_2(i1 j2 i2 j1),
+ * Sval = GM2s(i1 j2 i2 j1).  F and SF accumulate; E, S overwritten. */
+void orc_rxchf_omg2(i64 which, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                    const double *GM2_1, const double *GM2_2, const double *GM2s, double *F, double *SF, double *E,
+                    double *Sv)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double)), *xs = (double *)calloc(nf, sizeof(double));
+    double e = 0.0, sv = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1)
+    for (i64 i2 = 0; i2 < ne; ++i2)
+    for (i64 j2 = 0; j2 < ne; ++j2) {
+        double val1 = GM2_1[ix2(ne, np, ip, jp, i1, j1, i2, j2)];
+        double val2 = GM2_2[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+        double sval = GM2s[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+        double w = RXP(ip, jp) * RXA(i2, j2) * RXB(i1, j1);
+        if (which == 0) {
+            xf[j2 * ne + i2] += RXP(ip, jp) * RXB(i1, j1) * (2.0 * val1 - val2);
+            xs[j2 * ne + i2] += RXP(ip, jp) * RXB(i1, j1) * sval;
+        } else if (which == 1) {
+            xf[jp * np + ip] += RXA(i2, j2) * RXB(i1, j1) * (val1 - 0.5 * val2);
+            xs[jp * np + ip] -= 0.5 * RXA(i2, j2) * RXB(i1, j1) * sval;
+        } else {
+            xf[j1 * ne + i1] += RXA(i2, j2) * RXP(ip, jp) * (val1 - 0.5 * val2);
+            xs[j1 * ne + i1] -= 0.5 * RXA(i2, j2) * RXP(ip, jp) * sval;
+        }
+        e += w * (val1 - 0.5 * val2);
+        sv += w * (-0.5) * sval;
+    }
+    for (i64 k = 0; k < nf; ++k) {
+        F[k] += xf[k];
+        SF[k] += xs[k];
+    }
+    *E = e;
+    *Sv = sv;
+    free(xf);
+    free(xs);
+}
+#undef RXA
+#undef RXB
+#undef RXP
+

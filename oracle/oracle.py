@@ -203,12 +203,13 @@ def ref_prim_list(s, cls, idx):
 
 def ref_prim_list3(s, cls, idx):
     """As ref_prim_list; cls 5: RXCHFmult_GAM2_MD (ansE1, ansS, ansE2),
-    cls 6: RXCHFmult_INT_GAM3_MD, over a combined one-basis dict."""
+    cls 6: RXCHFmult_INT_GAM3_MD, over a combined one-basis dict; cls 9:
+    RXCHF_xcalc_GAM2_MD (ansE1, ansS, ansE2)."""
     R = ref()
     fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_,
           5: R.rxchfmult_gam2_md_, 6: R.rxchfmult_int_gam3_md_, 7: R.rxchfmult_xchf_gam3_md_,
-          8: R.rxchfmult_xcalc_gam1_md_}[cls]
-    nel = {4: 2, 5: 2, 6: 3, 7: 3, 8: 1}.get(cls, cls)
+          8: R.rxchfmult_xcalc_gam1_md_, 9: R.rxchf_xcalc_gam2_md_}[cls]
+    nel = {4: 2, 5: 2, 6: 3, 7: 3, 8: 1, 9: 2}.get(cls, cls)
     am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
     c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
     cat = np.ascontiguousarray(s["cat"])
@@ -232,7 +233,7 @@ def ref_prim_list3(s, cls, idx):
         args += prim(am_p, s["nucex"], c_p, x[2 * nel + 1])
         e, sv, xv = C.c_double(), C.c_double(), C.c_double()
         tail = [_i8(s["nat"]), _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(g), C.byref(e)]
-        if cls == 5:  # ansE1, ansE2, ansS
+        if cls in (5, 9):  # ansE1, ansE2, ansS
             tail += [C.byref(xv), C.byref(sv)]
         elif cls not in (3, 6, 7):
             tail.append(C.byref(sv))
@@ -1071,3 +1072,53 @@ def ref_xcscf(s, workdir, ce_guess=None, lsoscf=True, timeout=3600):
                                lsoscf=lsoscf)
     subprocess.run([sys.executable, "-c", code], cwd=workdir, check=True, timeout=timeout)
     return parse_xcscf_log(open(os.path.join(workdir, "xcscf.log")).read())
+
+
+# ---------------------------------------------------------------------------
+# RXCHF, one special electron (oracle and reference)
+# ---------------------------------------------------------------------------
+def rxchf_gam2_ic1(s):
+    """RXCHF_GAM2_IC1 restated: (GM2_1ICR, GM2_2ICR, GM2sICR)."""
+    ne, npb = s["nebf"], s["npbf"]
+    g1 = np.zeros(ne ** 4 * npb * npb)
+    g2, gs = np.zeros_like(g1), np.zeros_like(g1)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchf_gam2_ic1(*ba, *sa, _p(g1), _p(g2), _p(gs))
+    return g1, g2, gs
+
+
+def rxchf_omg2(which, ne, npb, DAE, DBE, DP, G1, G2, Gs, F, SF):
+    """which: 0 RXCHF_FAE_OMG2, 1 RXCHF_FP_OMG2, 2 RXCHF_FBE_OMG2; F and SF
+    accumulate in place; returns (E, S)."""
+    e, sv = np.zeros(1), np.zeros(1)
+    lib().orc_rxchf_omg2(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                         _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G2)), _p(np.ascontiguousarray(Gs)),
+                         _p(F), _p(SF), _p(e), _p(sv))
+    return float(e[0]), float(sv[0])
+
+
+def ref_rxchf_gam2_ic1(s):
+    """RXCHF_GAM2_IC1 (source/rxchf/rxchf_gam2_IC1_OMP.f:2) of the reference."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng2 = ne ** 4 * npb * npb
+    g1, g2, gs = np.zeros(ng2), np.zeros(ng2), np.zeros(ng2)
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchf_gam2_ic1_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng2), _i8(0), _i8(s["nat"]),
+                          _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep],
+                          _p(g1), _p(g2), _p(gs))
+    return g1, g2, gs
+
+
+def ref_rxchf_omg2(which, ne, npb, DAE, DBE, DP, G1, G2, Gs, F, SF):
+    """RXCHF_FAE_OMG2 / _FP_OMG2 / _FBE_OMG2 (source/rxchf/rxchf_omg2.f:2,
+    :231, :449) of the reference; returns (E, S)."""
+    e, sv = C.c_double(), C.c_double()
+    ng2 = ne ** 4 * npb * npb
+    R = ref()
+    fn = (R.rxchf_fae_omg2_, R.rxchf_fp_omg2_, R.rxchf_fbe_omg2_)[which]
+    fn(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _f(DAE), _f(DBE), _f(DP), _p(np.ascontiguousarray(G1)),
+       _p(np.ascontiguousarray(G2)), _p(np.ascontiguousarray(Gs)), _p(F), _p(SF), C.byref(e), C.byref(sv))
+    return e.value, sv.value
+

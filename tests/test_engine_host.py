@@ -224,3 +224,20 @@ def test_engine_f_rxchfmult_vs_oracle():
         for k in range(3 if cls == 5 else 1):
             check_close(got[k], ref[k], atol_rel=1e-11, what=f"l<=3 engine cls{cls} out{k}")
 
+
+@pytest.mark.parametrize("lmax,shells", [(2, (0, 1, 2)), (3, (0, 3))])
+def test_engine_rxchf_tuples_vs_oracle(lmax, shells):
+    """RXCHF_xcalc_GAM2_MD (class 9: ansE1, ansS, ansE2), every capacity tier
+    (tier 0) and the general one (tier 1); the l <= 3 build with f
+    functions."""
+    from _util import engine_prim_list3
+
+    s = synth.small(seed=142 + lmax, e_shells=shells, p_shells=(0, 1, 2))
+    idx = random_tuples(s, 2, 150, seed=5)
+    ref = oracle.prim_list3(s, 9, idx)
+    for tier in (0, 1):
+        got = engine_prim_list3(s, 9, idx, tier, lmax=lmax)
+        for k in range(3):
+            check_close(got[k], ref[k], atol_rel=1e-12 if lmax == 2 else 1e-11,
+                        what=f"class 9 out{k} tier {tier} lmax {lmax}")
+

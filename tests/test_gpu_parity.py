@@ -642,6 +642,46 @@ def test_rxchfmult(torch_dev):
             assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)
 
 
+def test_rxchf(torch_dev):
+    """RXCHF with one special electron (source/rxchf/): class-9 tuples
+    (RXCHF_xcalc_GAM2_MD), RXCHF_GAM2_IC1 and RXCHF_FAE/FP/FBE_OMG2 through
+    the C ABI against the oracle (itself checked against the reference's
+    routines in tests/test_oracle.py)."""
+    torch = torch_dev
+    s = synth.small(seed=150, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    S = neorx.System(s)
+    idx = random_tuples(s, 2, 1024, seed=19)
+    print("case rxchf cls9 n=1024", flush=True)
+    got = _run_list3(torch, S, 9, idx)
+    ref = oracle.prim_list3(s, 9, idx)
+    for k, what in enumerate(("ansE1", "ansS", "ansE2")):
+        check_close(got[k], ref[k], what=f"cls9 {what}")
+    S.close()
+    t = synth.small(seed=151, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    ne, npb = t["nebf"], t["npbf"]
+    print(f"case rxchf builder nebf={ne} npbf={npb}", flush=True)
+    T = neorx.System(t)
+    g = T.RXCHF_GAM2_IC1()
+    T.close()
+    r = oracle.rxchf_gam2_ic1(t)
+    for a, b, what in zip(g, r, ("GM2_1ICR", "GM2_2ICR", "GM2sICR")):
+        check_close(a, b, what=f"RXCHF_GAM2_IC1 {what}")
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    rng = np.random.default_rng(23)
+    for w in range(3):
+        print(f"case rxchf omg2 which={w}", flush=True)
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        S0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B, C2, D = F0.copy(order="F"), F0.copy(order="F"), S0.copy(order="F"), S0.copy(order="F")
+        eg = neorx.RXCHF_OMG2(w, ne, npb, DA, DB, DP, *g, A, C2)
+        eo = oracle.rxchf_omg2(w, ne, npb, DA, DB, DP, *r, B, D)
+        check_close(A - F0, B - F0, what=f"RXCHF OMG2 {w} F")
+        check_close(C2 - S0, D - S0, what=f"RXCHF OMG2 {w} S")
+        assert eg[0] == pytest.approx(eo[0], rel=1e-11) and eg[1] == pytest.approx(eo[1], rel=1e-11), (eg, eo)
+    neorx.release_resident()
+
+
 def test_rxchfmult_xchf_side(torch_dev):
     """RXCHFmult's XCHF-side builders over one basis: class-8 and class-7
     tuples, RXCHFmult_GAM1_MD, RXCHFmult_GAM3_XCHF and RXCHFmult_GAM4_XCHF."""

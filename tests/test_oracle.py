@@ -544,3 +544,34 @@ def test_oracle_f_tuples_vs_reference_golden(name):
         if ent["S"] is not None:
             check_close(os_, np.array(ent["S"]), what=f"{name} cls{cls} S")
 
+
+@needs_ref
+def test_oracle_rxchf_vs_reference():
+    """RXCHF (one special electron, source/rxchf/): the oracle's
+    RXCHF_xcalc_GAM2_MD (class 9), RXCHF_GAM2_IC1 and RXCHF_FAE/FP/FBE_OMG2
+    against the reference's own routines."""
+    s = synth.small(seed=140, e_shells=(0, 1), p_shells=(0, 1))
+    idx = random_tuples(s, 2, 6, seed=9)
+    ref = oracle.ref_prim_list3(s, 9, idx)
+    got = oracle.prim_list3(s, 9, idx)
+    for k, what in enumerate(("ansE1", "ansS", "ansE2")):
+        check_close(got[k], ref[k], what=f"RXCHF_xcalc_GAM2_MD {what}")
+    t = synth.small(seed=141, n_e_centres=1, e_shells=(0, 1), contract=(1, 2), p_shells=(0,), ngtg=2)
+    ne, npb = t["nebf"], t["npbf"]
+    g = oracle.rxchf_gam2_ic1(t)
+    rg = oracle.ref_rxchf_gam2_ic1(t)
+    for a, b, what in zip(g, rg, ("GM2_1ICR", "GM2_2ICR", "GM2sICR")):
+        check_close(a, b, what=f"RXCHF_GAM2_IC1 {what}")
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    rng = np.random.default_rng(3)
+    for w in range(3):
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        S0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B, C2, D = F0.copy(order="F"), F0.copy(order="F"), S0.copy(order="F"), S0.copy(order="F")
+        eo = oracle.rxchf_omg2(w, ne, npb, DA, DB, DP, *g, A, C2)
+        er = oracle.ref_rxchf_omg2(w, ne, npb, DA, DB, DP, *rg, B, D)
+        check_close(A - F0, B - F0, what=f"RXCHF OMG2 {w} F")
+        check_close(C2 - S0, D - S0, what=f"RXCHF OMG2 {w} S")
+        assert eo[0] == pytest.approx(er[0], rel=1e-12) and eo[1] == pytest.approx(er[1], rel=1e-12)
+
