@@ -162,6 +162,66 @@ def fock_bench(np, torch, neorx, synth, ne, npb, reps):
     }
 
 
+def c3_bench(np, torch, neorx, synth, reps=5):
+    """Config 3: RXCHF-ne closed shell with 2 special electrons (48/12
+    functions, 3 geminals).  RXCHFne_GAM2_IC1 is built on the device in FP64;
+    the three closed-shell contractions (RXCHFne_FAE/FP/FBE_OMG2, one pass over
+    GM2) then read it stored in FP64, FP32 and BF16.  Reports the pass time per
+    storage type and the energy and Fock-matrix error of the reduced-precision
+    passes against the FP64 one, with the same densities (random PSD: DAE of
+    trace 20, DBE of trace 2, DP of trace 1)."""
+    f64 = torch.float64
+    sysd = synth.config3()
+    ne, npb = sysd["nebf"], sysd["npbf"]
+    ng2 = ne ** 4 * npb ** 2
+    S = neorx.System(sysd)
+    g2 = torch.empty(ng2, dtype=f64, device="cuda")
+    gx = torch.empty(ng2, dtype=f64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    neorx.call_dev("neorx_rxchfne_gam2_ic1_dev", S.handle, g2.data_ptr(), gx.data_ptr(), 0)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    S.close()
+    del gx
+    torch.cuda.empty_cache()
+    store = {0: g2, 1: torch.empty(ng2, dtype=torch.float32, device="cuda"),
+             2: torch.empty(ng2, dtype=torch.int16, device="cuda")}
+    neorx.call_dev("neorx_convert_f32_dev", ng2, g2.data_ptr(), store[1].data_ptr(), 0)
+    neorx.call_dev("neorx_convert_bf16_dev", ng2, g2.data_ptr(), store[2].data_ptr(), 0)
+    dens = [torch.from_numpy(np.ascontiguousarray(synth.density(n, tr, sd).ravel(order="F"))).cuda()
+            for n, tr, sd in ((ne, 20.0, 31), (ne, 2.0, 32), (npb, 1.0, 33))]
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    res = {}
+    for dt in (0, 1, 2):
+        outs = None
+        ts = []
+        for r in range(reps + 1):
+            outs = [torch.zeros(n * n, dtype=f64, device="cuda") for n in (ne, npb, ne)] + \
+                   [torch.zeros(1, dtype=f64, device="cuda")]
+            ev[0].record()
+            neorx.call_dev("neorx_rxchfne_omg2_dev", ne, npb, *[d.data_ptr() for d in dens], store[dt].data_ptr(),
+                           dt, *[o.data_ptr() for o in outs], 0)
+            ev[1].record()
+            torch.cuda.synchronize()
+            if r:
+                ts.append(ev[0].elapsed_time(ev[1]))
+        res[dt] = (min(ts), [o.cpu().numpy() for o in outs])
+    out = {"system": sysd["name"], "gm2_entries": ng2, "gam2_ic1_build_s": t_build}
+    ref = res[0][1]
+    for dt, nm in ((0, "fp64"), (1, "fp32"), (2, "bf16")):
+        ms, o = res[dt]
+        out[f"pass_ms_{nm}"] = ms
+        out[f"pass_GBps_{nm}"] = ng2 * (8, 4, 2)[dt] / ms / 1e6
+        out[f"energy_{nm}_Eh"] = float(o[3][0])
+        if dt:
+            out[f"speedup_{nm}_vs_fp64"] = res[0][0] / ms
+            out[f"dE_{nm}_vs_fp64_Eh"] = abs(float(o[3][0]) - float(ref[3][0]))
+            out[f"dF_{nm}_vs_fp64_rel"] = max(float(np.abs(o[k] - ref[k]).max() / np.abs(ref[k]).max())
+                                             for k in range(3))
+    return out
+
+
 def scf_bench(np, torch, neorx, synth, maxit=80):
     """NEO-XCHF RHF SCF of the C1 system (neorx.synth.config1: C/N classical,
     quantum proton, 24 electronic functions from 36 s/p primitives, 8 nuclear
@@ -237,6 +297,8 @@ def main():
                     help="nebf,npbf of the Fock-build contraction timing (FP64 vs mixed); 'off' to skip")
     ap.add_argument("--scf", type=str, default="c1", choices=("off", "c1"),
                     help="SCF s/iter of the C1 system, FP64 vs mixed precision (builds C1's integrals first)")
+    ap.add_argument("--c3", type=str, default="on", choices=("off", "on"),
+                    help="config 3: RXCHF-ne contractions at 48/12 functions, FP64 vs FP32 vs BF16 storage")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -339,6 +401,13 @@ def main():
         print(f"# Fock-build contractions nebf={fne} npbf={fnp}", file=sys.stderr, flush=True)
         fock = fock_bench(np, torch, neorx, synth, fne, fnp, 3)
 
+    c3 = None
+    if args.c3 == "on":
+        print("# C3 RXCHF-ne contractions (FP64 vs FP32 vs BF16 storage)", file=sys.stderr, flush=True)
+        c3 = c3_bench(np, torch, neorx, synth)
+        print(f"# C3 GM2 built in {c3['gam2_ic1_build_s']:.1f} s; pass {c3['pass_ms_fp64']:.3f} / "
+              f"{c3['pass_ms_fp32']:.3f} / {c3['pass_ms_bf16']:.3f} ms", file=sys.stderr, flush=True)
+
     scf = None
     if args.scf == "c1":
         print("# C1 SCF (FP64 vs mixed precision)", file=sys.stderr, flush=True)
@@ -363,7 +432,7 @@ def main():
                                "geminal; all e-p tuples + seeded samples of e-e-p and e-e-e-p tuples",
                    "e_p_tuples": n1, "e_e_p_tuples": ns, "e_e_e_p_tuples": ns, "device": arch},
         "per_class_integrals_per_s": {"e-p": rates[1], "e-e-p": rates[2], "e-e-e-p": rates[3]},
-        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock, "scf": scf,
+        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock, "scf": scf, "c3_rxchfne": c3,
     }
     print(json.dumps(out))
 

@@ -152,6 +152,9 @@ int neorx_eg4icr_dev(int64_t nebf, int64_t npbf, const double *d_de, const doubl
  * products and sums stay FP64, so the only rounding is that of the stored
  * values.  Same semantics as the FP64 entry points above. */
 int neorx_convert_f32_dev(int64_t n, const double *d_src, float *d_dst, void *stream);
+/* BF16 copy of an FP64 tensor: the upper 16 bits of the FP32 value, rounded
+ * to nearest even (for the storage types that take dtype 2). */
+int neorx_convert_bf16_dev(int64_t n, const double *d_src, uint16_t *d_dst, void *stream);
 int neorx_eg2ic1_f32_dev(int64_t nebf, int64_t npbf, const double *d_de, const double *d_dp, const float *d_gm2,
                          const float *d_gm2s, double *d_fe, double *d_se, double *d_fp, double *d_sp, double *d_e,
                          double *d_s, void *stream);
@@ -229,6 +232,17 @@ int neorx_rxchfne_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const d
                           const double *gm2, double *fp, double *e);
 int neorx_rxchfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
                            const double *gm2, double *fbe, double *e);
+/* The three closed-shell RXCHF-ne contractions above in one pass over GM2
+ * (device pointers), with GM2 stored as dtype 0 (FP64), 1 (FP32, FP64 sums)
+ * or 2 (BF16, FP32 sums; neorx_convert_bf16_dev).  d_fae (nebf^2), d_fp
+ * (npbf^2) and d_fbe (nebf^2) are added into, each may be null; d_e = the
+ * energy of all three (overwritten).  Stream-ordered scratch; not
+ * synchronised.  No counterpart of its own in the reference: it computes
+ * what RXCHFne_FAE_OMG2, _FP_OMG2 and _FBE_OMG2 compute (rxchfne_omg2.f:2,
+ * :212, :409), from one read of the tensor, in the storage type asked for. */
+int neorx_rxchfne_omg2_dev(int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                           const double *d_dp, const void *d_gm2, int dtype, double *d_fae, double *d_fp,
+                           double *d_fbe, double *d_e, void *stream);
 /* RXCUHFne_FE_OMG2 / _FP_OMG2 / _FBE_OMG2, source/rxchfne/rxcuhfne_omg2.f:2,
  * :239, :445 (daalp: regular alpha, datot: regular total density) */
 int neorx_rxcuhfne_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,

@@ -1483,6 +1483,29 @@ template <> struct Pack<bf16_t, 2> {
     static NRX_DI float get(raw v, int p) { return __uint_as_float(p ? (v & 0xffff0000u) : (v << 16)); }
 };
 
+// 8-byte loads of two FP32 or four BF16 entries (the lower address in the
+// low bits)
+template <> struct Pack<float, 2> {
+    using raw = unsigned long long;
+    static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+    {
+        return __builtin_bit_cast(raw, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+    }
+    static NRX_DI double get(raw v, int p) { return (double)__uint_as_float(p ? (unsigned)(v >> 32) : (unsigned)v); }
+};
+template <> struct Pack<bf16_t, 4> {
+    using raw = unsigned long long;
+    static NRX_DI raw ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+    {
+        return __builtin_bit_cast(raw, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+    }
+    static NRX_DI float get(raw v, int p)
+    {
+        const unsigned w = p < 2 ? (unsigned)v : (unsigned)(v >> 32);
+        return __uint_as_float((p & 1) ? (w & 0xffff0000u) : (w << 16));
+    }
+};
+
 // Arithmetic type of the sums over c: FP64, except for BF16 storage, whose
 // sums run in FP32 (the storage rounding, 2^-9 relative, dwarfs FP32's; the
 // pass is bound by instruction issue, and FP32 saves the widening to FP64).
@@ -2819,20 +2842,332 @@ static int rxne_contract(int mode, i64 ne, i64 np, const double *dbe, const doub
     return NEORX_OK;
 }
 
+// Closed-shell RXCHF-ne contractions (RXCHFne_FAE/FP/FBE_OMG2,
+// rxchfne_omg2.f:2, :212, :409) in one pass over GM2 stored as T (FP64,
+// FP32 or BF16).  Row r = (i1,j1) of the ne^2 x ne^2 block G[pp] is the
+// special electron's pair, column c = (i2,j2) the regular electron's.
+// Block (pp, s) takes split s of the rows; wave w of the block the rows
+// r0 + w, r0 + w + RX_WAVES, ...  A lane loads V adjacent columns in one
+// 8-byte buffer load (V = 1 for FP64, 2 for FP32, 4 for BF16, when a row's
+// byte length is a multiple of 8; else V = 1), so every wave load moves 512
+// bytes.  Per chunk of 64 KC V columns lane l owns the vectors at
+// c = c0 + (l + 64 k) V (k < KC) and per row adds
+//   vc[k][v] += DBE(r) G(r,c+v)          (column sums, in registers)
+//   R(r)     += sum_c DAE(c) G(r,c)      (one wave reduction per row and chunk)
+//   fp       += DBE(r) R(r)              (the FP sum of the wave's rows)
+// Two rows per step keep 2 KC loads in flight per lane; each row's byte
+// offset is wave-uniform (scalar).  Lanes past ne^2 read the row's first
+// entries and weight them by zero.  Outputs:
+//   vpart[pp][s][c]    = sum over the split's rows of DBE(r) G(r,c)
+//   rpart[pp][ch][r]   = sum over chunk ch's columns of DAE(c) G(r,c)
+//   fpart[pp][s][w]    = wave w's sum of DBE(r) R(r)
+// Sums over a row run in SlabAcc<T> (FP32 for BF16 storage, else FP64); the
+// column partials meet in LDS wave by wave, in a fixed order.
+constexpr int RX_WAVES = 4;
+
+template <class T, int KC, int V>
+__global__ void __launch_bounds__(64 * RX_WAVES) k_rxne_pass(i64 ne, int nsplit, const double *__restrict__ DBE,
+                                                             const double *__restrict__ DAE,
+                                                             const T *__restrict__ G, double *__restrict__ vpart,
+                                                             double *__restrict__ rpart, double *__restrict__ fpart)
+{
+    using A = typename SlabAcc<T>::type;
+    using P = Pack<T, V>;
+    constexpr int COLS = 64 * KC * V;
+    __shared__ double cs[COLS];
+    const i64 ne2 = ne * ne, pp = blockIdx.x;
+    const int s = blockIdx.y, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const i64 rpb = (ne2 + nsplit - 1) / nsplit, r0 = s * rpb, r1 = min(ne2, r0 + rpb);
+    const int nch = (int)((ne2 + COLS - 1) / COLS);
+    const auto rs = buf_rsrc(G + pp * ne2 * ne2, (unsigned)(ne2 * ne2 * (i64)sizeof(T)));
+    double fp = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const i64 c0 = (i64)ch * COLS;
+        A dt[KC][V], vc[KC][V];
+        unsigned voff[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const i64 cb = c0 + (i64)(64 * k + lane) * V;
+            const bool on = cb < ne2;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const i64 c = cb + v;
+                dt[k][v] = on ? (A)DAE[(c % ne) * ne + c / ne] : (A)0;
+                vc[k][v] = (A)0;
+            }
+            voff[k] = on ? (unsigned)(cb * (i64)sizeof(T)) : 0u;
+        }
+        i64 r = r0 + w;
+        for (; r + RX_WAVES < r1; r += 2 * RX_WAVES) {
+            typename P::raw va[KC], vb[KC];
+            const i64 rb = r + RX_WAVES;
+            const unsigned sa = (unsigned)(r * ne2 * (i64)sizeof(T)), sb = (unsigned)(rb * ne2 * (i64)sizeof(T));
+#pragma unroll
+            for (int k = 0; k < KC; ++k) va[k] = P::ld(rs, voff[k], sa);
+#pragma unroll
+            for (int k = 0; k < KC; ++k) vb[k] = P::ld(rs, voff[k], sb);
+            NRX_LOADS_ISSUED();
+            const double dad = DBE[(r % ne) * ne + r / ne], dbd = DBE[(rb % ne) * ne + rb / ne];
+            const A da = (A)dad, db = (A)dbd;
+            A acca = (A)0, accb = (A)0;
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const A a = (A)P::get(va[k], v), b = (A)P::get(vb[k], v);
+                    vc[k][v] += da * a;
+                    vc[k][v] += db * b;
+                    acca += dt[k][v] * a;
+                    accb += dt[k][v] * b;
+                }
+            const double ta = wave_sum((double)acca), tb = wave_sum((double)accb);
+            fp += dad * ta;
+            fp += dbd * tb;
+            if (lane == 0) {
+                rpart[(pp * nch + ch) * ne2 + r] = ta;
+                rpart[(pp * nch + ch) * ne2 + rb] = tb;
+            }
+        }
+        if (r < r1) {
+            typename P::raw va[KC];
+            const unsigned sa = (unsigned)(r * ne2 * (i64)sizeof(T));
+#pragma unroll
+            for (int k = 0; k < KC; ++k) va[k] = P::ld(rs, voff[k], sa);
+            NRX_LOADS_ISSUED();
+            const double dad = DBE[(r % ne) * ne + r / ne];
+            const A da = (A)dad;
+            A acca = (A)0;
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const A a = (A)P::get(va[k], v);
+                    vc[k][v] += da * a;
+                    acca += dt[k][v] * a;
+                }
+            const double ta = wave_sum((double)acca);
+            fp += dad * ta;
+            if (lane == 0) rpart[(pp * nch + ch) * ne2 + r] = ta;
+        }
+        // column partials: the waves add into one LDS row in turn
+        for (int q = 0; q < RX_WAVES; ++q) {
+            if (w == q)
+#pragma unroll
+                for (int k = 0; k < KC; ++k)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const int j = (64 * k + lane) * V + v;
+                        cs[j] = q ? cs[j] + (double)vc[k][v] : (double)vc[k][v];
+                    }
+            __syncthreads();
+        }
+        for (int j = threadIdx.x; j < COLS; j += 64 * RX_WAVES) {
+            const i64 c = c0 + j;
+            if (c < ne2) vpart[(pp * nsplit + s) * ne2 + c] = cs[j];
+        }
+        __syncthreads();
+    }
+    if (lane == 0) fpart[(pp * nsplit + s) * RX_WAVES + w] = fp;
+}
+
+// Weighted column sums of a row-major nrow x ncol matrix P, first stage:
+// block (cb, g) sums the rows of group g (rows g*rpg .. +rpg) for the 64
+// columns of block cb, each row q weighted by DP(pp) with pp = q / per
+// (DP column-major npbf x npbf, pp = ip*npbf + jp); 4 waves split the rows
+// and meet in LDS in a fixed order.  out1[g][c].
+constexpr int WCS_GROUPS = 32;
+__global__ void __launch_bounds__(256) k_wcolsum1(i64 nrow, i64 ncol, i64 per, i64 np, const double *__restrict__ DP,
+                                                  const double *__restrict__ P, double *__restrict__ out1)
+{
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const i64 c = (i64)blockIdx.x * 64 + lane, g = blockIdx.y;
+    const i64 rpg = (nrow + WCS_GROUPS - 1) / WCS_GROUPS, q0 = g * rpg, q1 = min(nrow, q0 + rpg);
+    double acc = 0.0;
+    if (c < ncol)
+        for (i64 q = q0 + w; q < q1; q += 4) {
+            const i64 pp = q / per;
+            acc += DP[(pp % np) * np + pp / np] * P[q * ncol + c];
+        }
+    red[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && c < ncol) out1[g * ncol + c] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+// second stage: F(i,j) += sum_g out1[g][c], c = i*ne + j, F column-major
+__global__ void k_wcolsum2(i64 ne, const double *__restrict__ out1, double *__restrict__ F)
+{
+    const i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x, ne2 = ne * ne;
+    if (c >= ne2) return;
+    double t = 0.0;
+    for (int g = 0; g < WCS_GROUPS; ++g) t += out1[g * ne2 + c];
+    F[(c % ne) * ne + c / ne] += t;
+}
+
+// FP(ip,jp) += sum_{s,w} fpart[pp][s][w] when FP is given;
+// E = sum_pp DP(pp) FP-increment(pp) (overwritten)
+__global__ void __launch_bounds__(64) k_rxne_e(i64 np, int nq, const double *__restrict__ DP,
+                                               const double *__restrict__ fpart, double *__restrict__ FP,
+                                               double *__restrict__ E)
+{
+    double acc = 0.0;
+    for (i64 pp = threadIdx.x; pp < np * np; pp += 64) {
+        double f = 0.0;
+        for (int q = 0; q < nq; ++q) f += fpart[pp * nq + q];
+        const i64 at = (pp % np) * np + pp / np;
+        acc += DP[at] * f;
+        if (FP) FP[at] += f;
+    }
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) *E = acc;
+}
+
+// vector slots per lane and chunk (KC): of the candidates for the vector
+// width V (at most 16 entries per lane and row), the one whose chunks of
+// 64 KC V columns pad the ne^2 columns least (the larger on a tie)
+static int rxne_kc(i64 ne2, int V)
+{
+    static const int c1[] = {16, 12, 9, 8}, c2[] = {8, 6, 4}, c4[] = {4, 3, 2};
+    const int *c = V == 1 ? c1 : V == 2 ? c2 : c4;
+    const int n = V == 1 ? 4 : 3;
+    int best = c[0];
+    i64 pad = -1;
+    for (int i = 0; i < n; ++i) {
+        const i64 cols = 64LL * c[i] * V, p = (ne2 + cols - 1) / cols * cols - ne2;
+        if (pad < 0 || p < pad) best = c[i], pad = p;
+    }
+    return best;
+}
+
+template <class T, int V>
+static void rxne_pass_kernel(int kc, i64 ne, i64 np2, int nsplit, const double *dbe, const double *dae, const T *g,
+                             double *vpart, double *rpart, double *fpart, hipStream_t st)
+{
+    const dim3 grid((unsigned)np2, (unsigned)nsplit), block(64 * RX_WAVES);
+#define NRX_RXNE_KC(K)                                                                                                \
+    case K:                                                                                                           \
+        hipLaunchKernelGGL((k_rxne_pass<T, K, V>), grid, block, 0, st, ne, nsplit, dbe, dae, g, vpart, rpart, fpart); \
+        break
+    if constexpr (V == 1) {
+        switch (kc) {
+            NRX_RXNE_KC(8);
+            NRX_RXNE_KC(9);
+            NRX_RXNE_KC(12);
+            NRX_RXNE_KC(16);
+        }
+    } else if constexpr (V == 2) {
+        switch (kc) {
+            NRX_RXNE_KC(4);
+            NRX_RXNE_KC(6);
+            NRX_RXNE_KC(8);
+        }
+    } else {
+        switch (kc) {
+            NRX_RXNE_KC(2);
+            NRX_RXNE_KC(3);
+            NRX_RXNE_KC(4);
+        }
+    }
+#undef NRX_RXNE_KC
+}
+
+template <class T>
+static int rxne_pass_launch(i64 ne, i64 np, const double *dae, const double *dbe, const double *dp, const T *g,
+                            double *fae, double *fp, double *fbe, double *e, hipStream_t st)
+{
+    const i64 ne2 = ne * ne, np2 = np * np;
+    if (ne2 * ne2 * (i64)sizeof(T) > 0xffffffffLL)
+        return fail(NEORX_E_ARG, "RXCHF-ne contraction: a GM2 block of nebf %lld exceeds 4 GiB", (long long)ne);
+    // entries per 8-byte load when every row starts on 8 bytes
+    const int V = (ne2 * (i64)sizeof(T)) % 8 == 0 ? (int)(8 / sizeof(T)) : 1;
+    const int kc = rxne_kc(ne2, V);
+    const int nch = (int)((ne2 + 64LL * kc * V - 1) / (64LL * kc * V));
+    // about 2048 blocks, at least 2 RX_WAVES rows each
+    const int nsplit = (int)std::max<i64>(1, std::min<i64>(ne2 / (2 * RX_WAVES), (2048 + np2 - 1) / np2));
+    StreamBuf w;
+    const i64 nv = np2 * nsplit * ne2, nr = np2 * nch * ne2, n1 = (i64)WCS_GROUPS * ne2,
+              nf = np2 * nsplit * RX_WAVES;
+    int rc = w.alloc((nv + nr + n1 + nf) * 8, st);
+    if (rc) return rc;
+    double *vpart = (double *)w.p, *rpart = vpart + nv, *o1 = rpart + nr, *fpart = o1 + n1;
+    if constexpr (sizeof(T) == 8) {
+        rxne_pass_kernel<T, 1>(kc, ne, np2, nsplit, dbe, dae, g, vpart, rpart, fpart, st);
+    } else {
+        if (V == 1)
+            rxne_pass_kernel<T, 1>(kc, ne, np2, nsplit, dbe, dae, g, vpart, rpart, fpart, st);
+        else
+            rxne_pass_kernel<T, 8 / sizeof(T)>(kc, ne, np2, nsplit, dbe, dae, g, vpart, rpart, fpart, st);
+    }
+    const dim3 g1((unsigned)((ne2 + 63) / 64), WCS_GROUPS);
+    if (fae) {
+        hipLaunchKernelGGL(k_wcolsum1, g1, dim3(256), 0, st, np2 * nsplit, ne2, (i64)nsplit, np, dp, vpart, o1);
+        hipLaunchKernelGGL(k_wcolsum2, dim3(nblocks(ne2, 256)), dim3(256), 0, st, ne, o1, fae);
+    }
+    if (fbe) {
+        hipLaunchKernelGGL(k_wcolsum1, g1, dim3(256), 0, st, np2 * nch, ne2, (i64)nch, np, dp, rpart, o1);
+        hipLaunchKernelGGL(k_wcolsum2, dim3(nblocks(ne2, 256)), dim3(256), 0, st, ne, o1, fbe);
+    }
+    hipLaunchKernelGGL(k_rxne_e, dim3(1), dim3(64), 0, st, np, nsplit * RX_WAVES, dp, fpart, fp, e);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchfne_omg2_dev(int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                                      const double *d_dp, const void *d_gm2, int dtype, double *d_fae,
+                                      double *d_fp, double *d_fbe, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || npbf <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    if (!d_dae || !d_dbe || !d_dp || !d_gm2 || !d_e) return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    switch (dtype) {
+    case 0: return rxne_pass_launch<double>(nebf, npbf, d_dae, d_dbe, d_dp, (const double *)d_gm2, d_fae, d_fp, d_fbe, d_e, st);
+    case 1: return rxne_pass_launch<float>(nebf, npbf, d_dae, d_dbe, d_dp, (const float *)d_gm2, d_fae, d_fp, d_fbe, d_e, st);
+    case 2: return rxne_pass_launch<bf16_t>(nebf, npbf, d_dae, d_dbe, d_dp, (const bf16_t *)d_gm2, d_fae, d_fp, d_fbe, d_e, st);
+    default: return fail(NEORX_E_ARG, "storage type %d (0 FP64, 1 FP32, 2 BF16)", dtype);
+    }
+}
+
+// host-pointer closed-shell routines: one matrix of the one-pass contraction
+static int rxne_closed(int mode, i64 ne, i64 np, const double *dae, const double *dbe, const double *dp,
+                       const double *gm2, double *f, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    const i64 ng2 = ne * ne * ne * ne * np * np, nf = mode == 1 ? np * np : ne * ne;
+    DevBuf dA, dB, dP, dG, dF, dE;
+    rc = h2d(dA, dae, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(dG, gm2, ng2);
+    if (!rc) rc = h2d(dF, f, nf);
+    if (!rc) rc = dE.alloc(8);
+    if (!rc)
+        rc = rxne_pass_launch<double>(ne, np, dA.d(), dB.d(), dP.d(), dG.d(), mode == 0 ? dF.d() : nullptr,
+                                      mode == 1 ? dF.d() : nullptr, mode == 2 ? dF.d() : nullptr, dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
 extern "C" int neorx_rxchfne_fae_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
                                       const double *dp, const double *gm2, double *fae, double *e)
 {
-    return rxne_contract(0, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fae, nullptr, e);
+    return rxne_closed(0, nebf, npbf, dae, dbe, dp, gm2, fae, e);
 }
 extern "C" int neorx_rxchfne_fp_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
                                      const double *dp, const double *gm2, double *fp, double *e)
 {
-    return rxne_contract(1, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fp, nullptr, e);
+    return rxne_closed(1, nebf, npbf, dae, dbe, dp, gm2, fp, e);
 }
 extern "C" int neorx_rxchfne_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const double *dbe,
                                       const double *dp, const double *gm2, double *fbe, double *e)
 {
-    return rxne_contract(2, nebf, npbf, dbe, dae, nullptr, dp, gm2, nullptr, fbe, nullptr, e);
+    return rxne_closed(2, nebf, npbf, dae, dbe, dp, gm2, fbe, e);
 }
 extern "C" int neorx_rxcuhfne_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
                                       const double *dbe, const double *dp, const double *gm2, const double *gm2ex,

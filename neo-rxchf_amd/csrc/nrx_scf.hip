@@ -1447,6 +1447,17 @@ extern "C" int neorx_xchf_fock_dev(int64_t ctx, const double *d_de, const double
     return rc;
 }
 
+extern "C" int neorx_convert_bf16_dev(int64_t n, const double *d_src, uint16_t *d_dst, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!d_src || !d_dst))) return fail(NEORX_E_ARG, "bad conversion arguments");
+    if (n == 0) return NEORX_OK;
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n, d_src, d_dst);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
 extern "C" int neorx_gemm_f64_dev(int64_t M, int64_t N, int64_t K, const double *d_a, int64_t sam, int64_t sak,
                                   const double *d_b, int64_t sbk, int64_t sbn, double *d_c, void *stream)
 {

@@ -75,6 +75,8 @@ def load():
     _sig("neorx_eg3ic1_noalloc_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_eg4icr_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_convert_f32_dev", [_I64, _P, _P, _P])
+    _sig("neorx_convert_bf16_dev", [_I64, _P, _P, _P])
+    _sig("neorx_rxchfne_omg2_dev", [_I64, _I64] + [_P] * 4 + [_C.c_int] + [_P] * 5)
     _sig("neorx_eg2ic1_f32_dev", [_I64, _I64] + [_P] * 10 + [_P])
     _sig("neorx_eg3ic1_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])
     _sig("neorx_eg4icr_f32_dev", [_I64, _I64] + [_P] * 6 + [_P])

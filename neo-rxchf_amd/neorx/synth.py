@@ -163,6 +163,38 @@ def config2(seed=2):
     return sysd
 
 
+def config3(seed=3):
+    """RXCHF-ne closed-shell system with 2 special electrons: 20 regular plus
+    2 special electrons; 48 electronic functions (one primitive each) on 5
+    random centres in a 6-bohr box, per centre 3 s shells and 1 p shell, and
+    a Cartesian d shell on each of the first 3 centres (5 x 6 + 3 x 6 = 48),
+    exponents log-uniform in [0.08, 80]; 12 nuclear functions (3 s, 1 p and
+    1 d shell, exponents log-uniform in [4, 32]) on centre 0; 3-term geminal,
+    b in [0.1, 0.5], gamma log-uniform in [1, 30]; a classical nucleus of
+    charge 4 at each centre."""
+    rng = np.random.default_rng(seed)
+    cents = [rng.uniform(0.0, 6.0, 3) for _ in range(5)]
+    prims, contr = [], []
+    for k, c in enumerate(cents):
+        for l in [0, 0, 0, 1] + ([2] if k < 3 else []):
+            a = float(_logu(rng, 0.08, 80.0))
+            for l3 in _SHELL[l]:
+                prims.append((l3, a, c, 1.0))
+                contr.append(1)
+    nuc = []
+    for l in [0, 0, 0, 1, 2]:
+        a = float(_logu(rng, 4.0, 32.0))
+        for l3 in _SHELL[l]:
+            nuc.append((l3, a, cents[0]))
+    b = rng.uniform(0.1, 0.5, 3)
+    g = _logu(rng, 1.0, 30.0, 3)
+    sysd = _finish(prims, contr, nuc, cents, [4.0] * 5, b, g)
+    sysd["nelec"] = 22
+    sysd["nbe"] = 2
+    sysd["name"] = "C3 RXCHF-ne closed shell (48 e-functions s/p/d on 5 centres, 12 p-functions, 3 geminals)"
+    return sysd
+
+
 def small(seed=0, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=3, nat=2, maxl=None):
     """Small random system for parity tests: s/p/d/f shells on random
     centres, contracted functions of 1-2 primitives."""

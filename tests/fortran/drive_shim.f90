@@ -217,6 +217,23 @@ program drive_shim
   fe = 0.0d0
   call RXCHFmult_FBE_OMG3ex(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gm4, gx3, fe, e3)
   write(11) fe, e3
+  ! RXCHF with one special electron (rxchf_fock.f:495, :580, :665): DAE the
+  ! regular density, DBE the special electron's
+  call RXCHF_GAM2_IC1(one, nebf, npebf, npbf, ng2, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                      kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm2, gx1, gx2)
+  write(11) gm2, gx1, gx2
+  fe = 0.0d0
+  se = 0.0d0
+  call RXCHF_FAE_OMG2(one, nebf, npbf, ng2, dae, dbe, dp, gm2, gx1, gx2, fe, se, e2, s2)
+  write(11) fe, se, e2, s2
+  fp = 0.0d0
+  sp = 0.0d0
+  call RXCHF_FP_OMG2(one, nebf, npbf, ng2, dae, dbe, dp, gm2, gx1, gx2, fp, sp, e2, s2)
+  write(11) fp, sp, e2, s2
+  fe = 0.0d0
+  se = 0.0d0
+  call RXCHF_FBE_OMG2(one, nebf, npbf, ng2, dae, dbe, dp, gm2, gx1, gx2, fe, se, e2, s2)
+  write(11) fe, se, e2, s2
   close(11)
 contains
   subroutine read_ufm(name, n, a)

@@ -101,7 +101,9 @@ def test_fortran_dropin_matches_oracle(recl):
              ng2, ng2, ng2, ng2,  # RXCHFmult_GAM2_IC1ex
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG2ex
              ng3, ng3, ng3,  # RXCHFmult_GAM3_IC1ex
-             ne * ne, 1, npb * npb, 1, ne * ne, 1]  # RXCHFmult_FAE/FP/FBE_OMG3ex
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG3ex
+             ng2, ng2, ng2,  # RXCHF_GAM2_IC1: GM2_1ICR, GM2_2ICR, GM2sICR
+             ne * ne, ne * ne, 1, 1, npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1]  # RXCHF_FAE/FP/FBE_OMG2
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -226,3 +228,16 @@ def test_fortran_dropin_matches_oracle(recl):
         e = oracle.rxchfmult_omgex(3, which, ne, npb, DAE, DBE, DP, x3[0], x3[1], x3[2], F)
         check_close(parts[96 + 2 * which], F.ravel(order="F"), what=f"RXCHFmult omg3ex {which}")
         assert parts[97 + 2 * which][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    # RXCHF with one special electron: DAE regular, DBE special
+    r1 = oracle.rxchf_gam2_ic1(s)
+    for k, what in enumerate(("GM2_1ICR", "GM2_2ICR", "GM2sICR")):
+        check_close(parts[102 + k], r1[k], what=f"RXCHF_GAM2_IC1 {what}")
+    for which in range(3):
+        n = npb if which == 1 else ne
+        F, SF = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        e, sv = oracle.rxchf_omg2(which, ne, npb, DAE, DBE, DP, *r1, F, SF)
+        at = 105 + 4 * which
+        check_close(parts[at], F.ravel(order="F"), what=f"RXCHF omg2 {which} F")
+        check_close(parts[at + 1], SF.ravel(order="F"), what=f"RXCHF omg2 {which} S")
+        assert parts[at + 2][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+        assert parts[at + 3][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)

@@ -583,6 +583,71 @@ def test_rxchfne(torch_dev):
     check_close(A, B, what="RXCHFne_FAE_GAMee")
 
 
+def _round_storage(g, prec):
+    """A tensor as the device stores it in FP32 or BF16 (round to nearest even
+    through FP32), widened back to FP64."""
+    f = np.asarray(g, dtype=np.float64).astype(np.float32)
+    if prec == 1:
+        return f.astype(np.float64)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("ne,npb", [(36, 2), (7, 3), (24, 2), (32, 1)])
+def test_rxchfne_one_pass(torch_dev, ne, npb):
+    """neorx_rxchfne_omg2_dev: RXCHFne_FAE/FP/FBE_OMG2 from one pass over GM2
+    stored in FP64, FP32 and BF16, against the oracle's three routines over
+    the same rounded tensor.  The chunk widths differ: nebf 36 takes two
+    chunks of 12 columns per lane (the second partial) and many row splits,
+    nebf 7 one partial chunk of 8, nebf 24 one chunk of 9, nebf 32 one of 16
+    with a single nuclear function.  Bars: 1e-11 for FP64 and FP32 storage
+    (FP64 sums); for BF16, whose sums run in FP32, 1e-5 of the same
+    contraction over absolute values, element by element."""
+    torch = torch_dev
+    rng = np.random.default_rng(ne)
+    ng2 = ne ** 4 * npb ** 2
+    G = rng.standard_normal(ng2) * 0.1
+    DA, DB, DP = synth.density(ne, 20.0, 1), synth.density(ne, 2.0, 2), synth.density(npb, 1.0, 3)
+    dG = _dev(torch, G)
+    dA, dB, dP = (_dev(torch, x.ravel(order="F")) for x in (DA, DB, DP))
+    F0 = [np.asfortranarray(rng.standard_normal((n, n))) for n in (ne, npb, ne)]
+    for dtype in (0, 1, 2):
+        print(f"case rxchfne one pass nebf={ne} npbf={npb} storage={dtype}", flush=True)
+        if dtype == 0:
+            store, Gr = dG, G
+        elif dtype == 1:
+            store = torch.empty(ng2, dtype=torch.float32, device="cuda")
+            neorx.call_dev("neorx_convert_f32_dev", ng2, dG.data_ptr(), store.data_ptr(), 0)
+            Gr = _round_storage(G, 1)
+        else:
+            store = torch.empty(ng2, dtype=torch.int16, device="cuda")
+            neorx.call_dev("neorx_convert_bf16_dev", ng2, dG.data_ptr(), store.data_ptr(), 0)
+            Gr = _round_storage(G, 2)
+        dF = [_dev(torch, f.ravel(order="F")) for f in F0]
+        dE = torch.zeros(1, dtype=torch.float64, device="cuda")
+        neorx.call_dev("neorx_rxchfne_omg2_dev", ne, npb, dA.data_ptr(), dB.data_ptr(), dP.data_ptr(),
+                       store.data_ptr(), dtype, *[f.data_ptr() for f in dF], dE.data_ptr(), 0)
+        torch.cuda.synchronize()
+        for k in range(3):
+            n = npb if k == 1 else ne
+            F = F0[k].copy(order="F")
+            e = oracle.rxchfne_omg2(k, ne, npb, DA, DB, DP, Gr, F)
+            got = dF[k].cpu().numpy().reshape(F.shape, order="F")
+            if dtype < 2:
+                check_close(got - F0[k], F - F0[k], what=f"RXCHF-ne one pass {k} storage {dtype}")
+                assert dE.item() == pytest.approx(e, rel=1e-11), (dtype, k, dE.item(), e)
+                continue
+            # BF16: sums in FP32, whose error scales with the sum of the
+            # terms' magnitudes (the same contraction over |D| and |G|), not
+            # with the signed result where the terms cancel
+            Fa = np.zeros((n, n), order="F")
+            ea = oracle.rxchfne_omg2(k, ne, npb, np.abs(DA), np.abs(DB), np.abs(DP), np.abs(Gr), Fa)
+            err = np.abs((got - F0[k]) - (F - F0[k]))
+            assert np.all(err <= 1e-5 * Fa), (k, float((err / Fa).max()))
+            assert abs(dE.item() - e) <= 1e-5 * ea, (k, dE.item(), e, ea)
+
+
 def _run_list3(torch, S, cls, idx):
     n = len(idx)
     d_idx = torch.from_numpy(np.ascontiguousarray(idx)).cuda()
