@@ -572,17 +572,21 @@ __global__ void __launch_bounds__(1024) k_assemble(i64 ne, i64 np, int nelec, do
 // density D = prefac sum_{k=first..last} C(:,k) C(:,k)^T.  Matrices are
 // column-major n x n.  The parallel cyclic Jacobi method pairs the indices
 // round-robin (n/2 disjoint rotations per step) and sweeps until the
-// off-diagonal norm is below 1e-15 of the diagonal's.
+// off-diagonal norm is below 1e-15 of the diagonal's, starting from the
+// identity or, within an SCF, from the previous call's eigenvectors.
 // ------------------------------------------------------------------
 constexpr int RH_MAXN = 128;
 constexpr int RH_BS = 256;
 
-NRX_DI void jacobi(int n, double *a, double *v, double *red)
+// a (symmetric) is rotated to diagonal form; v accumulates the rotations,
+// from the identity when `identity` is set, else from the v given
+NRX_DI void jacobi(int n, double *a, double *v, double *red, bool identity = true)
 {
     __shared__ double rc[RH_MAXN / 2], rs[RH_MAXN / 2];
     __shared__ int rp[RH_MAXN / 2], rq[RH_MAXN / 2];
     const int m = n + (n & 1), half = m / 2;
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) v[t] = (t % n == t / n) ? 1.0 : 0.0;
+    if (identity)
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) v[t] = (t % n == t / n) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off = 0.0, dia = 0.0;
@@ -687,33 +691,80 @@ __global__ void __launch_bounds__(RH_BS) k_orth(int n, const double *__restrict_
 }
 
 // ROOTHAN with X = S^-1/2: C, eigenvalues and the density of orbitals
-// first..last (0-based, inclusive); work: 2 n^2 doubles (LDS when small)
+// first..last (0-based, inclusive).  Vw (n^2, may be null) keeps the
+// eigenvectors in the orthogonal basis between the calls of an SCF on one
+// basis: when *vflag is set, F' is first rotated into that basis
+// (Vw^T F' Vw, nearly diagonal once the SCF settles) and the rotations start
+// from Vw, so a sweep or two meet the same convergence test as a start from
+// the identity; the eigenvectors found go back to Vw and the flag is set.
+// Work: 4 n^2 doubles in LDS (a, v and copies of F and X) when gwork is
+// null, else a and v in gwork (2 n^2) and F, X read where they lie.
 __global__ void __launch_bounds__(RH_BS) k_roothan(int n, const double *__restrict__ X, const double *__restrict__ F,
                                                    int first, int last, double prefac, double *__restrict__ C,
                                                    double *__restrict__ ev, double *__restrict__ D,
-                                                   double *__restrict__ gwork)
+                                                   double *__restrict__ gwork, double *__restrict__ Vw,
+                                                   double *__restrict__ vflag)
 {
     extern __shared__ double lds[];
     __shared__ double red[16];
     __shared__ int ord[RH_MAXN];
-    double *a = gwork ? gwork : lds, *v = a + n * n;
+    const int nn = n * n;
+    double *a = gwork ? gwork : lds, *v = a + nn;
+    const double *Fm = F, *Xm = X;
+    double *fl = gwork ? nullptr : lds + 2 * nn;
+    if (!gwork) {
+        double *xl = lds + 3 * nn;
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            fl[t] = F[t];
+            xl[t] = X[t];
+        }
+        __syncthreads();
+        Fm = fl;
+        Xm = xl;
+    }
     // v = F X, a = X^T v
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += F[k * n + i] * X[j * n + k];
+        for (int k = 0; k < n; ++k) x += Fm[k * n + i] * Xm[j * n + k];
         v[t] = x;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += X[i * n + k] * v[j * n + k];
+        for (int k = 0; k < n; ++k) x += Xm[i * n + k] * v[j * n + k];
         a[t] = x;
     }
     __syncthreads();
+    const bool warm = Vw && *vflag != 0.0;
+    if (warm) {
+        // a <- W^T a W with W = Vw (staged in F's place when F is in LDS)
+        const double *W = Vw;
+        if (fl) {
+            for (int t = threadIdx.x; t < nn; t += blockDim.x) fl[t] = Vw[t];
+            __syncthreads();
+            W = fl;
+        }
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            const int i = t % n, j = t / n;
+            double x = 0.0;
+            for (int k = 0; k < n; ++k) x += a[k * n + i] * W[j * n + k];
+            v[t] = x;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            const int i = t % n, j = t / n;
+            double x = 0.0;
+            for (int k = 0; k < n; ++k) x += W[i * n + k] * v[j * n + k];
+            a[t] = x;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) v[t] = W[t];
+        __syncthreads();
+    }
     // symmetrise against rounding before the rotations
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         if (i < j) {
             const double m = 0.5 * (a[t] + a[i * n + j]);
@@ -722,17 +773,21 @@ __global__ void __launch_bounds__(RH_BS) k_roothan(int n, const double *__restri
         }
     }
     __syncthreads();
-    jacobi(n, a, v, red);
+    jacobi(n, a, v, red, !warm);
     eig_order(n, a, ord);
     for (int k = threadIdx.x; k < n; k += blockDim.x) ev[k] = a[ord[k] * n + ord[k]];
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, k = t / n, kk = ord[k];
         double x = 0.0;
-        for (int l = 0; l < n; ++l) x += X[l * n + i] * v[kk * n + l];
+        for (int l = 0; l < n; ++l) x += Xm[l * n + i] * v[kk * n + l];
         C[t] = x;
     }
+    if (Vw) {
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) Vw[t] = v[t];
+        if (threadIdx.x == 0) *vflag = 1.0;
+    }
     __syncthreads();
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
         for (int k = first; k <= last; ++k) x += C[k * n + i] * C[k * n + j];
@@ -740,7 +795,10 @@ __global__ void __launch_bounds__(RH_BS) k_roothan(int n, const double *__restri
     }
 }
 
+// LDS of k_orth (2 n^2 doubles) and of k_roothan (4 n^2), or 0: work in
+// global memory
 inline size_t roothan_lds(int n) { return 2 * (size_t)n * n * 8 <= 60 * 1024 ? 2 * (size_t)n * n * 8 : 0; }
+inline size_t roothan_lds4(int n) { return 4 * (size_t)n * n * 8 <= 60 * 1024 ? 4 * (size_t)n * n * 8 : 0; }
 
 // DENDIF (xcscf.f:719-738) and COPYDEN: out = max |D - D0|, then D0 = D
 __global__ void __launch_bounds__(256) k_dendif(i64 n, const double *__restrict__ D, double *__restrict__ D0,
@@ -762,30 +820,55 @@ __global__ void __launch_bounds__(256) k_dendif(i64 n, const double *__restrict_
 // ------------------------------------------------------------------
 constexpr int DIIS_MAX = 8;
 
+// LDS of k_diis: F, D, S, X and two work matrices (6 n^2 doubles), or 0:
+// the work matrices w1, w2 in global memory and the inputs read where they lie
+inline size_t diis_lds(int n) { return 6 * (size_t)n * n * 8 <= 60 * 1024 ? 6 * (size_t)n * n * 8 : 0; }
+
 __global__ void __launch_bounds__(256) k_diis(int n, const double *__restrict__ F, const double *__restrict__ D,
                                               const double *__restrict__ S, const double *__restrict__ X,
                                               double *__restrict__ Fh, double *__restrict__ Eh, double *__restrict__ B,
                                               double *__restrict__ hist, double *__restrict__ Fout,
-                                              double *__restrict__ errout, double *__restrict__ w1,
-                                              double *__restrict__ w2)
+                                              double *__restrict__ errout, double *__restrict__ w1g,
+                                              double *__restrict__ w2g, int staged)
 {
+    extern __shared__ double lds[];
     __shared__ double red[16];
     __shared__ double cvec[DIIS_MAX + 1];
-    const int nn = n * n;
+    __shared__ double dots[DIIS_MAX][4];
+    __shared__ double M[DIIS_MAX + 1][DIIS_MAX + 2];
+    const int nn = n * n, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int cnt = (int)hist[0], slot = (int)hist[1];
     double *Fs = Fh + (i64)slot * nn, *Es = Eh + (i64)slot * nn;
+    const double *Fm = F, *Dm = D, *Sm = S, *Xm = X;
+    double *w1 = w1g, *w2 = w2g;
+    if (staged) {
+        double *f = lds, *d = lds + nn, *sm = lds + 2 * nn, *xm = lds + 3 * nn;
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            f[t] = F[t];
+            d[t] = D[t];
+            sm[t] = S[t];
+            xm[t] = X[t];
+        }
+        __syncthreads();
+        Fm = f;
+        Dm = d;
+        Sm = sm;
+        Xm = xm;
+        w1 = lds + 4 * nn;
+        w2 = lds + 5 * nn;
+    }
     // w1 = F D, w2 = w1 S (= F D S)
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += F[k * n + i] * D[j * n + k];
+        for (int k = 0; k < n; ++k) x += Fm[k * n + i] * Dm[j * n + k];
         w1[t] = x;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += w1[k * n + i] * S[j * n + k];
+        for (int k = 0; k < n; ++k) x += w1[k * n + i] * Sm[j * n + k];
         w2[t] = x;
     }
     __syncthreads();
@@ -799,7 +882,7 @@ __global__ void __launch_bounds__(256) k_diis(int n, const double *__restrict__ 
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += w1[k * n + i] * X[j * n + k];
+        for (int k = 0; k < n; ++k) x += w1[k * n + i] * Xm[j * n + k];
         w2[t] = x;
     }
     __syncthreads();
@@ -807,29 +890,42 @@ __global__ void __launch_bounds__(256) k_diis(int n, const double *__restrict__ 
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         const int i = t % n, j = t / n;
         double x = 0.0;
-        for (int k = 0; k < n; ++k) x += X[i * n + k] * w2[j * n + k];
+        for (int k = 0; k < n; ++k) x += Xm[i * n + k] * w2[j * n + k];
         Es[t] = x;
-        Fs[t] = F[t];
+        w1[t] = x; // Es again, where this block reads it fastest
+        Fs[t] = Fm[t];
         em = fmax(em, fabs(x));
     }
     em = block_max(em, red);
     cnt = cnt < DIIS_MAX ? cnt + 1 : DIIS_MAX;
-    // row `slot` of the error overlaps
-    for (int k = 0; k < cnt; ++k) {
-        const double *Ek = Eh + (i64)k * nn;
-        double x = 0.0;
-        for (int t = threadIdx.x; t < nn; t += blockDim.x) x += Es[t] * Ek[t];
-        x = block_sum(x, red);
-        if (threadIdx.x == 0) {
-            B[slot * DIIS_MAX + k] = x;
-            B[k * DIIS_MAX + slot] = x;
-        }
+    // row `slot` of the error overlaps: every dot in one pass, per-wave
+    // partials summed in a fixed order (the slot's own error from w1)
+    double x[DIIS_MAX];
+#pragma unroll
+    for (int k = 0; k < DIIS_MAX; ++k) x[k] = 0.0;
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+        const double e = w1[t];
+#pragma unroll
+        for (int k = 0; k < DIIS_MAX; ++k)
+            if (k < cnt) x[k] += e * (k == slot ? e : Eh[(i64)k * nn + t]);
+    }
+#pragma unroll
+    for (int k = 0; k < DIIS_MAX; ++k) {
+        const double v = wsum(x[k]);
+        if (lane == 0 && k < cnt) dots[k][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+        const int k = threadIdx.x, nw = (blockDim.x + 63) >> 6;
+        double v = 0.0;
+        for (int q = 0; q < nw; ++q) v += dots[k][q];
+        B[slot * DIIS_MAX + k] = v;
+        B[k * DIIS_MAX + slot] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         // solve [B 1; 1 0][c; l] = [0; 1] by Gaussian elimination with
         // partial pivoting (cnt + 1 unknowns), scaled by the diagonal
-        double M[DIIS_MAX + 1][DIIS_MAX + 2];
         const int m = cnt + 1;
         double sc = 0.0;
         for (int i = 0; i < cnt; ++i) sc = fmax(sc, fabs(B[i * DIIS_MAX + i]));
@@ -849,9 +945,9 @@ __global__ void __launch_bounds__(256) k_diis(int n, const double *__restrict__ 
                 break;
             }
             for (int j = 0; j <= m; ++j) {
-                const double x = M[c][j];
+                const double t = M[c][j];
                 M[c][j] = M[piv][j];
-                M[piv][j] = x;
+                M[piv][j] = t;
             }
             for (int r = 0; r < m; ++r)
                 if (r != c) {
@@ -866,9 +962,9 @@ __global__ void __launch_bounds__(256) k_diis(int n, const double *__restrict__ 
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nn; t += blockDim.x) {
-        double x = 0.0;
-        for (int k = 0; k < cnt; ++k) x += cvec[k] * Fh[(i64)k * nn + t];
-        Fout[t] = x;
+        double v = 0.0;
+        for (int k = 0; k < cnt; ++k) v += cvec[k] * (k == slot ? Fm[t] : Fh[(i64)k * nn + t]);
+        Fout[t] = v;
     }
 }
 
@@ -944,6 +1040,9 @@ struct Xchf {
     double *FEx = nullptr, *CE = nullptr, *CP = nullptr, *evE = nullptr, *evP = nullptr, *comps = nullptr;
     double *stat = nullptr, *rwork = nullptr;
     double *Fh = nullptr, *Eh = nullptr, *Bd = nullptr, *hist = nullptr, *dw1 = nullptr, *dw2 = nullptr;
+    // eigenvectors of the last ROOTHAN per basis (orthogonal basis) and
+    // their valid flags [electronic, nuclear]
+    double *VwE = nullptr, *VwP = nullptr, *vflag = nullptr;
     double *hstat = nullptr; // pinned
     static constexpr int NGRAPH = 6;
     hipGraphExec_t graph[NGRAPH] = {};
@@ -1037,8 +1136,12 @@ enum G3Pass {
     G3_INC = 2,     // the reference rows plus the increment T(DE - DEref, DE + DEref)
 };
 
-// phase A (depends on DE): dtype = storage of GAM_3 read
-int phase_a(Xchf *x, const double *DE, int dtype, int pass = G3_FULL)
+// phase A (depends on DE): dtype = storage of GAM_3 read.  Everything but
+// the GM3 pass (GAM_ep/GAM_1 rows, GAM_ee, the GM2 slabs and the DE side of
+// the 5-particle term) goes first, then the GM3 pass, in one stream: run
+// beside the pass, which takes every CU, the small kernels waited for CUs
+// and lengthened the iteration.
+int phase_a_rest(Xchf *x, const double *DE)
 {
     hipStream_t st = x->st;
     const i64 ne = x->ne, np = x->np, np2 = np * np, ne4 = ne * ne * ne * ne;
@@ -1050,30 +1153,9 @@ int phase_a(Xchf *x, const double *DE, int dtype, int pass = G3_FULL)
         if (rc) return rc;
     }
     if (x->nelec > 2) {
-        const void *g = dtype == NRX_F64 ? (const void *)x->G3 : dtype == NRX_F32 ? x->G3f : x->G3h;
-        if (!g) return fail(NEORX_E_ARG, "GAM_3 is not held in storage type %d (see neorx_xchf_create store)", dtype);
-        const i64 nblk = np2 * ne;
-        int rc = 0;
-        if (pass == G3_INC) {
-            hipLaunchKernelGGL(k_inc_dens, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref, x->DXi,
-                               x->DYi);
-            rc = nrx_eg3_slabs2(ne, nblk, x->DXi, x->DYi, nullptr, g, dtype, x->part3, st);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_inc_rows, dim3((unsigned)nblk), dim3(64), 0, st, ne, nblk, DE, x->part3ref, x->part3);
-            HIPCHK(hipGetLastError());
-        } else {
-            rc = nrx_eg3_slabs(ne, nblk, DE, g, dtype, x->part3, st);
-            if (rc) return rc;
-            if (pass == G3_REFRESH) {
-                hipLaunchKernelGGL(k_copy, dim3(nblocks(nblk * (ne + 1), 256)), dim3(256), 0, st, nblk * (ne + 1),
-                                   x->part3, x->part3ref);
-                hipLaunchKernelGGL(k_copy, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref);
-                HIPCHK(hipGetLastError());
-            }
-        }
         // 5-particle term, DE side: A (both bras), HA (one bra), Bbar = adj(A),
         // GPbar = Bbar carried back, FP4 = G2s . GPbar
-        rc = both_bras(x, DE, x->GeeU, x->HA4, x->A4);
+        int rc = both_bras(x, DE, x->GeeU, x->HA4, x->A4);
         if (rc) return rc;
         hipLaunchKernelGGL(k_g4_traces, dim3(grid_blocks(4 * ne * ne, 256)), dim3(256), 0, st, ne, x->A4, x->R4);
         hipLaunchKernelGGL(k_g4_adjoint, dim3(grid_blocks(ne4, 256)), dim3(256), 0, st, ne, x->A4, x->R4, x->Bb4);
@@ -1089,6 +1171,37 @@ int phase_a(Xchf *x, const double *DE, int dtype, int pass = G3_FULL)
                            x->GPb4, x->rows);
         hipLaunchKernelGGL(k_sum_rows, dim3(nblocks(np2, 256)), dim3(256), 0, st, np2, x->nch_fp, x->rows, x->FP4);
         HIPCHK(hipGetLastError());
+    }
+    return NEORX_OK;
+}
+
+int phase_a(Xchf *x, const double *DE, int dtype, int pass = G3_FULL)
+{
+    hipStream_t st = x->st;
+    const i64 ne = x->ne, np = x->np, np2 = np * np;
+    if (x->nelec <= 2) return phase_a_rest(x, DE);
+    const void *g = dtype == NRX_F64 ? (const void *)x->G3 : dtype == NRX_F32 ? x->G3f : x->G3h;
+    if (!g) return fail(NEORX_E_ARG, "GAM_3 is not held in storage type %d (see neorx_xchf_create store)", dtype);
+    int rc = phase_a_rest(x, DE);
+    if (rc) return rc;
+    // the GM3 pass
+    const i64 nblk = np2 * ne;
+    if (pass == G3_INC) {
+        hipLaunchKernelGGL(k_inc_dens, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref, x->DXi,
+                           x->DYi);
+        rc = nrx_eg3_slabs2(ne, nblk, x->DXi, x->DYi, nullptr, g, dtype, x->part3, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_inc_rows, dim3((unsigned)nblk), dim3(64), 0, st, ne, nblk, DE, x->part3ref, x->part3);
+        HIPCHK(hipGetLastError());
+    } else {
+        rc = nrx_eg3_slabs(ne, nblk, DE, g, dtype, x->part3, st);
+        if (rc) return rc;
+        if (pass == G3_REFRESH) {
+            hipLaunchKernelGGL(k_copy, dim3(nblocks(nblk * (ne + 1), 256)), dim3(256), 0, st, nblk * (ne + 1),
+                               x->part3, x->part3ref);
+            hipLaunchKernelGGL(k_copy, dim3(nblocks(ne * ne, 256)), dim3(256), 0, st, ne * ne, DE, x->DEref);
+            HIPCHK(hipGetLastError());
+        }
     }
     return NEORX_OK;
 }
@@ -1312,17 +1425,20 @@ int scf_state(Xchf *x)
     rc |= x->alloc(x->hist, 2);
     rc |= x->alloc(x->dw1, ne2);
     rc |= x->alloc(x->dw2, ne2);
+    rc |= x->alloc(x->VwE, ne2);
+    rc |= x->alloc(x->VwP, np2);
+    rc |= x->alloc(x->vflag, 2);
     if (rc) return rc;
     HIPCHK(hipHostMalloc((void **)&x->hstat, 64 * sizeof(double), hipHostMallocDefault));
     return NEORX_OK;
 }
 
 int launch_roothan(Xchf *x, int n, const double *X, const double *F, int first, int last, double prefac, double *C,
-                   double *ev, double *D)
+                   double *ev, double *D, double *Vw, double *vflag)
 {
-    const size_t lds = roothan_lds(n);
+    const size_t lds = roothan_lds4(n);
     hipLaunchKernelGGL(k_roothan, dim3(1), dim3(RH_BS), lds, x->st, n, X, F, first, last, prefac, C, ev, D,
-                       lds ? nullptr : x->rwork);
+                       lds ? nullptr : x->rwork, Vw, vflag);
     HIPCHK(hipGetLastError());
     return NEORX_OK;
 }
@@ -1337,17 +1453,18 @@ int scf_iteration(Xchf *x, int dtype, bool diis, int nucst, int pass)
     const double pre = x->nelec > 1 ? 2.0 : 1.0;
     int rc = phase_a(x, x->DE, dtype, pass);
     if (!rc) rc = phase_b(x, x->DE, x->DP, x->FEx, x->FP, x->stat + 17);
-    if (!rc) rc = launch_roothan(x, np, x->Xp, x->FP, nucst - 1, nucst - 1, 1.0, x->CP, x->evP, x->DP);
+    if (!rc) rc = launch_roothan(x, np, x->Xp, x->FP, nucst - 1, nucst - 1, 1.0, x->CP, x->evP, x->DP, x->VwP, x->vflag + 1);
     if (!rc) rc = phase_b(x, x->DE, x->DP, x->FE, x->FEx, x->stat + 3);
     if (rc) return rc;
     const double *F = x->FE;
     if (diis) {
-        hipLaunchKernelGGL(k_diis, dim3(1), dim3(256), 0, st, ne, x->FE, x->DE, x->Se, x->Xe, x->Fh, x->Eh, x->Bd,
-                           x->hist, x->FEx, x->stat + 2, x->dw1, x->dw2);
+        const size_t dl = diis_lds(ne);
+        hipLaunchKernelGGL(k_diis, dim3(1), dim3(256), dl, st, ne, x->FE, x->DE, x->Se, x->Xe, x->Fh, x->Eh, x->Bd,
+                           x->hist, x->FEx, x->stat + 2, x->dw1, x->dw2, dl ? 1 : 0);
         HIPCHK(hipGetLastError());
         F = x->FEx;
     }
-    rc = launch_roothan(x, ne, x->Xe, F, 0, nocc - 1, pre, x->CE, x->evE, x->DE);
+    rc = launch_roothan(x, ne, x->Xe, F, 0, nocc - 1, pre, x->CE, x->evE, x->DE, x->VwE, x->vflag);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dendif, dim3(1), dim3(256), 0, st, (i64)ne * ne, x->DE, x->DE0, x->stat);
     hipLaunchKernelGGL(k_dendif, dim3(1), dim3(256), 0, st, (i64)np * np, x->DP, x->DP0, x->stat + 1);
@@ -1484,11 +1601,11 @@ extern "C" int neorx_roothan_dev(int64_t n, const double *d_s, const double *d_f
     if (!d_s || !d_f || !d_c || !d_ev || !d_d) return fail(NEORX_E_ARG, "null argument");
     hipStream_t st = (hipStream_t)stream;
     double *w = nullptr;
-    const size_t lds = roothan_lds((int)n);
+    const size_t lds = roothan_lds((int)n), lds4 = roothan_lds4((int)n);
     HIPCHK(hipMallocAsync((void **)&w, (size_t)(3 * n * n) * 8, st));
     hipLaunchKernelGGL(k_orth, dim3(1), dim3(RH_BS), lds, st, (int)n, d_s, w, lds ? nullptr : w + n * n);
-    hipLaunchKernelGGL(k_roothan, dim3(1), dim3(RH_BS), lds, st, (int)n, w, d_f, (int)kstart - 1, (int)klast - 1, prefac,
-                       d_c, d_ev, d_d, lds ? nullptr : w + n * n);
+    hipLaunchKernelGGL(k_roothan, dim3(1), dim3(RH_BS), lds4, st, (int)n, w, d_f, (int)kstart - 1, (int)klast - 1,
+                       prefac, d_c, d_ev, d_d, lds4 ? nullptr : w + n * n, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     (void)hipFreeAsync(w, st);
     if (e != hipSuccess) return fail(NEORX_E_HIP, "ROOTHAN launch failed: %s", hipGetErrorString(e));
@@ -1519,6 +1636,9 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
     const int nocc = x->nelec > 1 ? x->nelec / 2 : 1;
     const double pre = x->nelec > 1 ? 2.0 : 1.0;
     const i64 ne2 = (i64)ne * ne, np2 = (i64)np * np;
+    // every SCF starts its diagonalisations from the identity
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (i64)2, 0.0, x->vflag);
+    HIPCHK(hipGetLastError());
     // initial electronic density: read_elec_density (GAM_IO.f:247-330) from
     // the guess orbitals, or guess_elec (xcscf.f, ROOTHAN on GAM_ecore)
     if (ce_guess) {
@@ -1539,7 +1659,7 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
         for (int i = 0; i < ne; ++i)
             for (int j = 0; j < ne; ++j) f[j * ne + i] = h[i * ne + j];
         HIPCHK(hipMemcpy(x->FEx, f.data(), ne2 * 8, hipMemcpyHostToDevice));
-        rc = launch_roothan(x, ne, x->Xe, x->FEx, 0, nocc - 1, pre, x->CE, x->evE, x->DE);
+        rc = launch_roothan(x, ne, x->Xe, x->FEx, 0, nocc - 1, pre, x->CE, x->evE, x->DE, x->VwE, x->vflag);
         if (rc) return rc;
     }
     // guess_prot (xcscf.f): ROOTHAN on GAM_pcore, orbital NUCST occupied
@@ -1549,7 +1669,8 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
         for (int i = 0; i < np; ++i)
             for (int j = 0; j < np; ++j) f[j * np + i] = h[i * np + j];
         HIPCHK(hipMemcpy(x->FEx, f.data(), np2 * 8, hipMemcpyHostToDevice));
-        rc = launch_roothan(x, np, x->Xp, x->FEx, (int)nucst - 1, (int)nucst - 1, 1.0, x->CP, x->evP, x->DP);
+        rc = launch_roothan(x, np, x->Xp, x->FEx, (int)nucst - 1, (int)nucst - 1, 1.0, x->CP, x->evP, x->DP, x->VwP,
+                            x->vflag + 1);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, ne2, 0.0, x->DE0);

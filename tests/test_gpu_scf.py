@@ -193,9 +193,10 @@ def test_gemm_f64(torch_dev, M, N, K, ta, tb):
     assert np.max(np.abs(got - ref)) <= 1e-13 * K * np.max(np.abs(ref)), np.max(np.abs(got - ref))
 
 
-@pytest.mark.parametrize("n", [1, 7, 24, 70])
+@pytest.mark.parametrize("n", [1, 7, 24, 50, 70])
 def test_roothan(torch_dev, n):
-    """ROOTHAN on the device (Jacobi, LDS for n <= 63, global memory above)
+    """ROOTHAN on the device (Jacobi; S^-1/2 in LDS for n <= 62, the
+    diagonalisation in LDS for n <= 43, global memory above)
     against the oracle's restatement: eigenvalues, the density, and the
     orbitals up to their sign; F C = S C e and C^T S C = 1."""
     torch = torch_dev
