@@ -76,6 +76,7 @@ int neorx_system_destroy(int64_t handle);
  *   cls 4 (e-e-p)   : RXCHFne_xcalc_GAM2_MD, rxchfne/rxchfne_gam_2_OMP.f:2 -> ansE1, ansE2
  *   cls 9 (e-e-p)   : RXCHF_xcalc_GAM2_MD, rxchf/rxchf_gam_2_OMP.f:546 (with neorx_prim_list3_dev)
  *                     -> ansE1, ansS, ansE2
+ *   cls 10 (e-e-e-p): RXCHF_xcalc_GAM3_MD, rxchf/rxchf_gam_3_OMP.f:617 -> ans1, ans2
  * d_idx holds 0-based primitive indices per tuple, 2*(n+1) int64 each for n
  * electrons (n = cls, 2 for cls 4), in the order (ie1..ie_n, ip, je1..je_n, jp). */
 int neorx_prim_list_dev(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d_idx, double *d_ansE,
@@ -280,6 +281,38 @@ int neorx_rxchf_fbe_omg2(int64_t nebf, int64_t npbf, const double *dae, const do
 int neorx_rxchf_omg2_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
                          const double *d_dp, const double *d_gm2_1, const double *d_gm2_2, const double *d_gm2s,
                          double *d_f, double *d_sf, double *d_es, void *stream);
+/* The RXCHF e-e-e-p tuple RXCHF_xcalc_GAM3_MD (rxchf/rxchf_gam_3_OMP.f:617)
+ * is class 10 of neorx_prim_list3_dev: outputs (ans1, ans2).
+ * RXCHF_GAM3_IC1, source/rxchf/rxchf_gam3_IC1_OMP.f:2: GM3_1ICR
+ * ((x123 + x132)/2 of the contracted ans1) and GM3_2ICR (the contracted ans2) */
+int neorx_rxchf_gam3_ic1(int64_t handle, double *gm3_1, double *gm3_2);
+int neorx_rxchf_gam3_ic1_dev(int64_t handle, double *d_gm3_1, double *d_gm3_2, void *stream);
+/* RXCHF_FAE_OMG3 / _FP_OMG3 / _FBE_OMG3, source/rxchf/rxchf_omg3.f:2, :263,
+ * :486: F accumulates, e overwritten */
+int neorx_rxchf_fae_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                         const double *gm3_1, const double *gm3_2, double *fae, double *e);
+int neorx_rxchf_fp_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                        const double *gm3_1, const double *gm3_2, double *fp, double *e);
+int neorx_rxchf_fbe_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                         const double *gm3_1, const double *gm3_2, double *fbe, double *e);
+/* the same on device arrays: which 0 FAE, 1 FP, 2 FBE; *d_e = E.  Stream-
+ * ordered scratch (3 npbf^2 nebf^4 doubles); not synchronised. */
+int neorx_rxchf_omg3_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dae, const double *d_dbe,
+                         const double *d_dp, const double *d_gm3_1, const double *d_gm3_2, double *d_f, double *d_e,
+                         void *stream);
+/* RXCHF_FP_OMG1 / RXCHF_FBE_OMG1, source/rxchf/rxchf_omg1.f:2, :84, over the
+ * GAM_1 and GAM_1s arrays (the contents of GAM_1.ufm / GAM_1s.ufm, which the
+ * reference reads itself): F accumulates, SF (SP / SBE) overwritten, e and s
+ * overwritten.  RXCHF_GAM1_OMP_MD (rxchf/rxchf_gam_1_OMP.f:2) computes what
+ * RXCHFmult_GAM1_MD computes: neorx_rxchfmult_gam1. */
+int neorx_rxchf_fp_omg1(int64_t nebf, int64_t npbf, const double *dbe, const double *dp, const double *gam1,
+                        const double *gam1s, double *fp, double *sp, double *e, double *s);
+int neorx_rxchf_fbe_omg1(int64_t nebf, int64_t npbf, const double *dbe, const double *dp, const double *gam1,
+                         const double *gam1s, double *fbe, double *sbe, double *e, double *s);
+/* the same on device arrays: which 1 FP, 2 FBE; d_es[0] = E, d_es[1] = S */
+int neorx_rxchf_omg1_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dbe, const double *d_dp,
+                         const double *d_gam1, const double *d_gam1s, double *d_f, double *d_sf, double *d_es,
+                         void *stream);
 
 /* ---- direct-SCF routines (reference semantics) -------------------------- */
 /* GAM2_DSCF, source/gam2_DSCF_OMP.f:2: contracted e-e-p values rebuilt per

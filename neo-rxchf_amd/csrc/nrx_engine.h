@@ -1379,6 +1379,48 @@ NRX_HD void gam3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2,
         ans = 2.0 * gVEE * 0.5 + gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
 }
 
+// RXCHF_xcalc_GAM3_MD (rxchf/rxchf_gam_3_OMP.f:617-1032): e1 the special
+// electron, e2 and e3 regular, p the proton; only the two-geminal terms, two
+// outputs.  One configuration per (k,l), g(e1,p) = gk, g(e2,p) = gl, e3
+// unlinked, carries every 4-particle term:
+//   ans1 = gVEEg1 = <(e1,p) overlap, g = gk + gl> x <1/r23>
+//   ans2 = gHEg + gVEEg2 + gVEEg3 = hcore(e3) yy - <1/r(e3,p)> + <1/r13> + <1/r23>
+// (the reference's gVEEg3 puts gk on e2 and gl on e1; summed over all (k,l)
+// with the symmetric weights bcoef(k) bcoef(l) it equals <1/r23> of this
+// configuration).
+template <int NL, int NBC>
+NRX_HD void rxchf3_tuple(const SysParams &S, const PairRec *e1, const PairRec *e2, const PairRec *e3,
+                         const PairRec *p, double &ans1, double &ans2)
+{
+    Sys s23 = mksys(S, 2, e2, e3, 0, 0, 0.0, 0, 0);
+    double vee23 = coul<NL, NBC>(s23, 0, 1);
+    underflow(vee23);
+    Sys s3 = mksys(S, 1, e3, 0, 0, 0, 0, 0, 0);
+    double hcore = kin<NL>(s3, 0, -1, 0.0, 1.0) - nuc_sum<NL, NBC>(s3, 0, S.nat, S.cat, S.zan, false);
+    underflow(hcore);
+    double gHEg = 0, gVEEg1 = 0, gVEE23 = 0;
+    for (int ik = 0; ik < S.ngtg; ++ik)
+        for (int il = 0; il < S.ngtg; ++il) {
+            double gk = S.gamma[ik], gl = S.gamma[il];
+            double bb = S.bcoef[ik] * S.bcoef[il];
+            Sys s4 = mksys(S, 4, e1, e2, e3, p, gk, gl, 0.0);
+            double yy;
+            double zz = coul<NL, NBC>(s4, 2, 3, PI, &yy);
+            underflow(yy);
+            underflow(zz);
+            gHEg += bb * ((hcore * yy) + (zz * (-1.0)));
+            gVEE23 += bb * (coul<NL, NBC>(s4, 0, 2) + coul<NL, NBC>(s4, 1, 2));
+            if (il < ik) continue;
+            double w = (il == ik) ? bb : 2.0 * bb;
+            Sys s2 = mksys(S, 2, e1, p, 0, 0, gk + gl, 0, 0);
+            double xx = ovl<NL>(s2);
+            underflow(xx);
+            gVEEg1 += w * (xx * vee23);
+        }
+    ans1 = gVEEg1;
+    ans2 = gHEg + gVEE23;
+}
+
 // RXCHFne_xcalc_GAM2_MD (rxchfne/rxchfne_gam_2_OMP.f:2-530): the RXCHF-ne
 // e-e-p tuple, two outputs summed over all (k,l):
 //   ansE1 = -<1/r(e2,p)> with g(e1,p) = gamma_k + gamma_l  +  <1/r12> with the same geminal
@@ -1584,13 +1626,15 @@ template <> struct Tier<5> : Tier<2> {};
 template <> struct Tier<6> : Tier<3> {};
 template <> struct Tier<7> : Tier<3> {};
 template <> struct Tier<8> : Tier<1> {};
-// class 9: the RXCHF (NBE = 1) e-e-p tuple (kinetic terms as class 2)
+// class 9: the RXCHF (NBE = 1) e-e-p tuple (kinetic terms as class 2);
+// class 10: the RXCHF e-e-e-p tuple (as class 3)
 template <> struct Tier<9> : Tier<2> {};
+template <> struct Tier<10> : Tier<3> {};
 
 // electrons per tuple of a class
 template <int CLS> constexpr int nel()
 {
-    return (CLS == 4 || CLS == 5 || CLS == 9) ? 2 : (CLS == 6 || CLS == 7) ? 3 : CLS == 8 ? 1 : CLS;
+    return (CLS == 4 || CLS == 5 || CLS == 9) ? 2 : (CLS == 6 || CLS == 7 || CLS == 10) ? 3 : CLS == 8 ? 1 : CLS;
 }
 
 // q[0..CLS-1] electron pairs, q[CLS] proton pair
@@ -1614,6 +1658,8 @@ NRX_HD void tuple_eval(const SysParams &S, const PairRec *const *q, double &e, d
     } else if constexpr (CLS == 9) {
         double x;
         rxchf2_tuple<NL, NBC>(S, q[0], q[1], q[2], e, x, s);
+    } else if constexpr (CLS == 10) {
+        rxchf3_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s);
     } else if constexpr (CLS == 6) {
         double x;
         gam3int_tuple<NL, NBC>(S, q[0], q[1], q[2], q[3], e, s, x);
@@ -1657,8 +1703,8 @@ NRX_HD int tuple_L(const PairRec *const *q, int n)
     return L;
 }
 
-// the capacity tier a tuple runs in.  The e-e-e-p tuples of classes 3 and 7
-// also apply a one-particle kinetic operator to e3 (their hcore term), whose
+// the capacity tier a tuple runs in.  The e-e-e-p tuples of classes 3, 7 and
+// 10 also apply a one-particle kinetic operator to e3 (their hcore term), whose
 // polynomial needs e3's own largest per-direction degree + 2.
 template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
 {
@@ -1672,7 +1718,7 @@ template <int CLS> NRX_HD int tuple_tier(const PairRec *const *q)
         }
     if (lc > NRX_LMAX) return T::N;
     int deg = tuple_degree(q, nel<CLS>() + 1, T::KIN), L = tuple_L(q, nel<CLS>() + 1);
-    if constexpr (CLS == 3 || CLS == 7) {
+    if constexpr (CLS == 3 || CLS == 7 || CLS == 10) {
         NRX_UNROLL for (int d = 0; d < 3; ++d) {
             int k = q[2]->la[d] + q[2]->lb[d] + 2;
             deg = k > deg ? k : deg;

@@ -517,7 +517,7 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
 {
     System *s = get_sys(handle);
     if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
-    if (cls < 1 || cls > 9) return fail(NEORX_E_ARG, "class %lld not in 1..9", (long long)cls);
+    if (cls < 1 || cls > 10) return fail(NEORX_E_ARG, "class %lld not in 1..10", (long long)cls);
     if (ntup < 0 || (ntup > 0 && (!d_idx || !d_ansE))) return fail(NEORX_E_ARG, "bad tuple list arguments");
     if (ntup == 0) return NEORX_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -530,6 +530,7 @@ static int prim_list(int64_t handle, int64_t cls, int64_t ntup, const int64_t *d
     case 6: return launch_prims<6, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 7: return launch_prims<7, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     case 9: return launch_prims<9, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
+    case 10: return launch_prims<10, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     default: return launch_prims<8, SRC_LIST>(s->d, 0, ntup, d_idx, d_ansE, d_ansS, st, d_ansX);
     }
 }
@@ -2973,8 +2974,9 @@ __global__ void __launch_bounds__(64 * RX_WAVES) k_rxne_pass(i64 ne, int nsplit,
 // Weighted column sums of a row-major nrow x ncol matrix P, first stage:
 // block (cb, g) sums the rows of group g (rows g*rpg .. +rpg) for the 64
 // columns of block cb, each row q weighted by DP(pp) with pp = q / per
-// (DP column-major npbf x npbf, pp = ip*npbf + jp); 4 waves split the rows
-// and meet in LDS in a fixed order.  out1[g][c].
+// (DP column-major npbf x npbf, pp = ip*npbf + jp; weight 1 when DP is
+// null); 4 waves split the rows and meet in LDS in a fixed order.
+// out1[g][c].
 constexpr int WCS_GROUPS = 32;
 __global__ void __launch_bounds__(256) k_wcolsum1(i64 nrow, i64 ncol, i64 per, i64 np, const double *__restrict__ DP,
                                                   const double *__restrict__ P, double *__restrict__ out1)
@@ -2987,7 +2989,7 @@ __global__ void __launch_bounds__(256) k_wcolsum1(i64 nrow, i64 ncol, i64 per, i
     if (c < ncol)
         for (i64 q = q0 + w; q < q1; q += 4) {
             const i64 pp = q / per;
-            acc += DP[(pp % np) * np + pp / np] * P[q * ncol + c];
+            acc += (DP ? DP[(pp % np) * np + pp / np] : 1.0) * P[q * ncol + c];
         }
     red[w][lane] = acc;
     __syncthreads();
@@ -3518,6 +3520,353 @@ extern "C" int neorx_rxchf_fbe_omg2(int64_t nebf, int64_t npbf, const double *da
                                     double *sbe, double *e, double *s)
 {
     return rxchf_omg2_host(2, nebf, npbf, dae, dbe, dp, gm2_1, gm2_2, gm2s, fbe, sbe, e, s);
+}
+
+// ------------------------------------------------------------------
+// RXCHF e-e-e-p tensors (rxchf/rxchf_gam3_IC1_OMP.f) and their contractions
+// (rxchf_omg3.f), and the GAM_1 contractions (rxchf_omg1.f)
+// ------------------------------------------------------------------
+// RXCHF_GAM3_IC1's symmetrisation (rxchf_gam3_IC1_OMP.f:211-238) of one
+// (ip,jp) block: GM3_1 = (x123 + x132)/2 of the contracted ans1 (pairs 2 and
+// 3 exchanged), GM3_2 = the contracted ans2 as is;
+// t = ((((i1 ne + j1) ne + i2) ne + j2) ne + i3) ne + j3
+__global__ void __launch_bounds__(256) k_rxchf3_symm(i64 ne, i64 blk, const double *__restrict__ X1,
+                                                     const double *__restrict__ X2, double *__restrict__ G1,
+                                                     double *__restrict__ G2)
+{
+    const i64 ne2 = ne * ne;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < blk; t += (i64)gridDim.x * blockDim.x) {
+        const i64 p3 = t % ne2, p2 = (t / ne2) % ne2, p1 = t / (ne2 * ne2);
+        G1[t] = (X1[t] + X1[(p1 * ne2 + p3) * ne2 + p2]) * 0.5;
+        G2[t] = X2[t];
+    }
+}
+
+extern "C" int neorx_rxchf_gam3_ic1_dev(int64_t handle, double *d_gm3_1, double *d_gm3_2, void *stream)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    if (!d_gm3_1 || !d_gm3_2) return fail(NEORX_E_ARG, "null output");
+    if (s->d.nebf1 != s->d.nebf) return fail(NEORX_E_ARG, "RXCHF_GAM3_IC1 takes a one-basis handle");
+    const i64 ne = s->d.nebf, np = s->d.npbf, ne5 = ne * ne * ne * ne * ne, blk = ne5 * ne;
+    hipStream_t st = (hipStream_t)stream;
+    const i64 cap = slab_cap(s, 3);
+    // one (ip,jp) block of unsymmetrised values at a time
+    DevBuf X1, X2, Y1, Y2;
+    int rc = X1.alloc(blk * 8);
+    if (!rc) rc = X2.alloc(blk * 8);
+    if (!rc) rc = Y1.alloc(cap * 8);
+    if (!rc) rc = Y2.alloc(cap * 8);
+    if (rc) return rc;
+    for (i64 b = 0; b < np * np; ++b) {
+        i64 c2;
+        for (auto &sl : make_slabs(s, 3, b * ne, (b + 1) * ne, c2)) {
+            const i64 R0 = slab_start(s, 3, sl.first), R1 = slab_start(s, 3, sl.second);
+            // primitive values of the slab: ans1, ans2
+            rc = launch_prims<10, SRC_ORIG>(s->d, R0, R1 - R0, nullptr, Y1.d(), Y2.d(), st);
+            if (rc) return rc;
+            const i64 n = (sl.second - sl.first) * ne5;
+            hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y1.d(),
+                               X1.d(), b * ne);
+            hipLaunchKernelGGL(k_contract3, dim3(nblocks(n, 256)), dim3(256), 0, st, s->d, sl.first, n, R0, Y2.d(),
+                               X2.d(), b * ne);
+            HIPCHK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_rxchf3_symm, dim3(grid_blocks(blk, 256)), dim3(256), 0, st, ne, blk, X1.d(), X2.d(),
+                           d_gm3_1 + b * blk, d_gm3_2 + b * blk);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchf_gam3_ic1(int64_t handle, double *gm3_1, double *gm3_2)
+{
+    System *s = get_sys(handle);
+    if (!s) return fail(NEORX_E_ARG, "unknown system handle %lld", (long long)handle);
+    const i64 ne = s->d.nebf, np = s->d.npbf, n = ne * ne * ne * ne * ne * ne * np * np;
+    DevBuf G1, G2;
+    int rc = G1.alloc(n * 8);
+    if (!rc) rc = G2.alloc(n * 8);
+    if (!rc) rc = neorx_rxchf_gam3_ic1_dev(handle, G1.d(), G2.d(), 0);
+    if (!rc) rc = copy_out(gm3_1, G1.d(), n);
+    if (!rc) rc = copy_out(gm3_2, G2.d(), n);
+    if (!rc) {
+        res_keep(G1, gm3_1, n);
+        res_keep(G2, gm3_2, n);
+    }
+    return rc;
+}
+
+// RXCHF_FAE_OMG3 / _FP_OMG3 / _FBE_OMG3 (rxchf_omg3.f:2-708).  Thread per
+// (pq, i1, j1, i2, j2); over (i3, j3), with
+//   val1 = GM3_1(i1 j1 i2 j2 i3 j3) - GM3_1(i1 j1 i2 j3 i3 j2)/2          (:215)
+//   val2 = (GM3_2(i1 j2 i2 j1 i3 j3) + GM3_2(i1 i2 j2 j1 i3 j3))
+//          - (GM3_2(i1 j3 i2 j1 i3 j2) + GM3_2(i1 j3 j2 j1 i3 i2))/2       (:239-240)
+//   val3 = (GM3_2(i1 j3 i3 j1 i2 j2) + GM3_2(i1 j3 i3 j1 j2 i2))
+//          - (GM3_2(i1 j2 i3 j1 i2 j3) + GM3_2(i1 i2 i3 j1 j2 j3))/2       (:241-242)
+//   v2p  = GM3_2(i1 j2 i2 j1 i3 j3) - GM3_2(i1 j3 i2 j1 i3 j2)/2           (:466)
+// it sums a = DAE(i3,j3) (2 val1 - (val2 + val3)/2) (FAE, :245-247),
+// w = DAE(i3,j3) (val1 - v2p)/2 (FP and FBE, :469-475, :692-698) and
+// v = DAE(i3,j3) (val1 - val2/2)/2 (FAE's energy, :249-251), and stores, per
+// row q = (pq, i1, j1) and column c = (i2, j2):
+//   P[q][c] = DP(pq) DBE(i1,j1) a,  W[q][c] = DAE(i2,j2) w,
+//   V[q][c] = DP(pq) DBE(i1,j1) DAE(i2,j2) v.
+__global__ void __launch_bounds__(256) k_rxchf3_terms(i64 ne, i64 np, const double *__restrict__ DAE,
+                                                      const double *__restrict__ DBE, const double *__restrict__ DP,
+                                                      const double *__restrict__ G1, const double *__restrict__ G2,
+                                                      double *__restrict__ P, double *__restrict__ W,
+                                                      double *__restrict__ V)
+{
+    const i64 ne2 = ne * ne, ne4 = ne2 * ne2;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np * np * ne4) return;
+    const i64 pq = t / ne4, r = t % ne4, i1 = r / (ne2 * ne), j1 = (r / ne2) % ne, i2 = (r / ne) % ne, j2 = r % ne;
+    const double *g1 = G1 + pq * ne4 * ne2, *g2 = G2 + pq * ne4 * ne2;
+    auto X = [ne](i64 a1, i64 b1, i64 a2, i64 b2, i64 a3, i64 b3) {
+        return ((((a1 * ne + b1) * ne + a2) * ne + b2) * ne + a3) * ne + b3;
+    };
+    double a = 0.0, w = 0.0, v = 0.0;
+    for (i64 i3 = 0; i3 < ne; ++i3)
+        for (i64 j3 = 0; j3 < ne; ++j3) {
+            const double d3 = DAE[j3 * ne + i3];
+            const double val1 = g1[X(i1, j1, i2, j2, i3, j3)] - 0.5 * g1[X(i1, j1, i2, j3, i3, j2)];
+            const double g213 = g2[X(i1, j2, i2, j1, i3, j3)], g312 = g2[X(i1, j3, i2, j1, i3, j2)];
+            const double val2 = (g213 + g2[X(i1, i2, j2, j1, i3, j3)]) - 0.5 * (g312 + g2[X(i1, j3, j2, j1, i3, i2)]);
+            const double val3 = (g2[X(i1, j3, i3, j1, i2, j2)] + g2[X(i1, j3, i3, j1, j2, i2)]) -
+                                0.5 * (g2[X(i1, j2, i3, j1, i2, j3)] + g2[X(i1, i2, i3, j1, j2, j3)]);
+            const double v2p = g213 - 0.5 * g312;
+            a += d3 * (2.0 * val1 - 0.5 * (val2 + val3));
+            w += d3 * (0.5 * (val1 - v2p));
+            v += d3 * (0.5 * (val1 - 0.5 * val2));
+        }
+    const double dpb = DP[(pq % np) * np + pq / np] * DBE[j1 * ne + i1], d2 = DAE[j2 * ne + i2];
+    P[t] = dpb * a;
+    W[t] = d2 * w;
+    V[t] = dpb * d2 * v;
+}
+
+// wave per row q: R[q] = sum_c W[q][c], Ev[q] = sum_c V[q][c]
+__global__ void __launch_bounds__(64) k_rxchf3_rows(i64 ncol, const double *__restrict__ W,
+                                                    const double *__restrict__ V, double *__restrict__ R,
+                                                    double *__restrict__ Ev)
+{
+    const i64 q = blockIdx.x;
+    double x = 0.0, y = 0.0;
+    for (i64 c = threadIdx.x; c < ncol; c += 64) {
+        x += W[q * ncol + c];
+        y += V[q * ncol + c];
+    }
+    x = wave_sum(x);
+    y = wave_sum(y);
+    if (threadIdx.x == 0) {
+        R[q] = x;
+        Ev[q] = y;
+    }
+}
+
+// which 1: FP(pq) += sum_r DBE(r) R[pq][r]; which 2: FBE(r) += sum_pq DP(pq)
+// R[pq][r]; E = sum_q Ev[q] for which 0, else sum_pq DP sum_r DBE R
+__global__ void __launch_bounds__(64) k_rxchf3_fin(int which, i64 ne, i64 np, const double *__restrict__ DBE,
+                                                   const double *__restrict__ DP, const double *__restrict__ R,
+                                                   const double *__restrict__ Ev, double *__restrict__ F,
+                                                   double *__restrict__ E)
+{
+    const i64 ne2 = ne * ne, np2 = np * np;
+    if (which == 1)
+        for (i64 pq = threadIdx.x; pq < np2; pq += 64) {
+            double f = 0.0;
+            for (i64 r = 0; r < ne2; ++r) f += DBE[(r % ne) * ne + r / ne] * R[pq * ne2 + r];
+            F[(pq % np) * np + pq / np] += f;
+        }
+    else if (which == 2)
+        for (i64 r = threadIdx.x; r < ne2; r += 64) {
+            double f = 0.0;
+            for (i64 pq = 0; pq < np2; ++pq) f += DP[(pq % np) * np + pq / np] * R[pq * ne2 + r];
+            F[(r % ne) * ne + r / ne] += f;
+        }
+    double e = 0.0;
+    for (i64 q = threadIdx.x; q < np2 * ne2; q += 64) {
+        const i64 pq = q / ne2, r = q % ne2;
+        e += which == 0 ? Ev[q] : DP[(pq % np) * np + pq / np] * DBE[(r % ne) * ne + r / ne] * R[q];
+    }
+    e = wave_sum(e);
+    if (threadIdx.x == 0) *E = e;
+}
+
+extern "C" int neorx_rxchf_omg3_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dae,
+                                    const double *d_dbe, const double *d_dp, const double *d_gm3_1,
+                                    const double *d_gm3_2, double *d_f, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf, ne2 = ne * ne;
+    if (ne <= 0 || np <= 0 || which < 0 || which > 2) return fail(NEORX_E_ARG, "RXCHF OMG3: bad arguments");
+    if (!d_dae || !d_dbe || !d_dp || !d_gm3_1 || !d_gm3_2 || !d_f || !d_e) return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const i64 nq = np * np * ne2, n5 = nq * ne2;
+    StreamBuf w;
+    rc = w.alloc((size_t)(3 * n5 + 2 * nq + WCS_GROUPS * ne2) * 8, st);
+    if (rc) return rc;
+    double *P = (double *)w.p, *W = P + n5, *V = W + n5, *R = V + n5, *Ev = R + nq, *o1 = Ev + nq;
+    hipLaunchKernelGGL(k_rxchf3_terms, dim3(nblocks(n5, 256)), dim3(256), 0, st, ne, np, d_dae, d_dbe, d_dp, d_gm3_1,
+                       d_gm3_2, P, W, V);
+    hipLaunchKernelGGL(k_rxchf3_rows, dim3((unsigned)nq), dim3(64), 0, st, ne2, W, V, R, Ev);
+    if (which == 0) {
+        hipLaunchKernelGGL(k_wcolsum1, dim3((unsigned)((ne2 + 63) / 64), WCS_GROUPS), dim3(256), 0, st, nq, ne2, (i64)1,
+                           np, (const double *)nullptr, P, o1);
+        hipLaunchKernelGGL(k_wcolsum2, dim3(nblocks(ne2, 256)), dim3(256), 0, st, ne, o1, d_f);
+    }
+    hipLaunchKernelGGL(k_rxchf3_fin, dim3(1), dim3(64), 0, st, (int)which, ne, np, d_dbe, d_dp, R, Ev, d_f, d_e);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int rxchf_omg3_host(int which, i64 ne, i64 np, const double *dae, const double *dbe, const double *dp,
+                           const double *g1, const double *g2, double *f, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    const i64 ng3 = ne * ne * ne * ne * ne * ne * np * np, nf = which == 1 ? np * np : ne * ne;
+    DevBuf dA, dB, dP, d1, d2, dF, dE;
+    rc = h2d(dA, dae, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(d1, g1, ng3);
+    if (!rc) rc = h2d(d2, g2, ng3);
+    if (!rc) rc = dF.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dF.p, f, nf * 8, hipMemcpyHostToDevice));
+    rc = neorx_rxchf_omg3_dev(which, ne, np, dA.d(), dB.d(), dP.d(), d1.d(), d2.d(), dF.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchf_fae_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                    const double *gm3_1, const double *gm3_2, double *fae, double *e)
+{
+    return rxchf_omg3_host(0, nebf, npbf, dae, dbe, dp, gm3_1, gm3_2, fae, e);
+}
+extern "C" int neorx_rxchf_fp_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                   const double *gm3_1, const double *gm3_2, double *fp, double *e)
+{
+    return rxchf_omg3_host(1, nebf, npbf, dae, dbe, dp, gm3_1, gm3_2, fp, e);
+}
+extern "C" int neorx_rxchf_fbe_omg3(int64_t nebf, int64_t npbf, const double *dae, const double *dbe, const double *dp,
+                                    const double *gm3_1, const double *gm3_2, double *fbe, double *e)
+{
+    return rxchf_omg3_host(2, nebf, npbf, dae, dbe, dp, gm3_1, gm3_2, fbe, e);
+}
+
+// RXCHF_FP_OMG1 / RXCHF_FBE_OMG1 (rxchf_omg1.f:2-165, per entry :60-66,
+// :142-148) over GAM_1 / GAM_1s (index ((ip np + jp) ne + i1) ne + j1):
+// which 1: FP(ip,jp) += sum DBE(i1,j1) GAM_1, SP = sum DBE GAM_1s;
+// which 2: FBE(i1,j1) += sum DP(ip,jp) GAM_1, SBE = sum DP GAM_1s;
+// es[0] = sum DP DBE GAM_1, es[1] = sum DP DBE GAM_1s.  One block.
+__global__ void __launch_bounds__(64) k_rxchf_omg1(int which, i64 ne, i64 np, const double *__restrict__ DBE,
+                                                   const double *__restrict__ DP, const double *__restrict__ G,
+                                                   const double *__restrict__ Gs, double *__restrict__ F,
+                                                   double *__restrict__ SF, double *__restrict__ es)
+{
+    const i64 ne2 = ne * ne, np2 = np * np;
+    if (which == 1)
+        for (i64 pq = threadIdx.x; pq < np2; pq += 64) {
+            double f = 0.0, sf = 0.0;
+            for (i64 r = 0; r < ne2; ++r) {
+                const double d = DBE[(r % ne) * ne + r / ne];
+                f += d * G[pq * ne2 + r];
+                sf += d * Gs[pq * ne2 + r];
+            }
+            const i64 at = (pq % np) * np + pq / np;
+            F[at] += f;
+            SF[at] = sf;
+        }
+    else
+        for (i64 r = threadIdx.x; r < ne2; r += 64) {
+            double f = 0.0, sf = 0.0;
+            for (i64 pq = 0; pq < np2; ++pq) {
+                const double d = DP[(pq % np) * np + pq / np];
+                f += d * G[pq * ne2 + r];
+                sf += d * Gs[pq * ne2 + r];
+            }
+            const i64 at = (r % ne) * ne + r / ne;
+            F[at] += f;
+            SF[at] = sf;
+        }
+    double e = 0.0, sv = 0.0;
+    for (i64 q = threadIdx.x; q < np2 * ne2; q += 64) {
+        const i64 pq = q / ne2, r = q % ne2;
+        const double d = DP[(pq % np) * np + pq / np] * DBE[(r % ne) * ne + r / ne];
+        e += d * G[q];
+        sv += d * Gs[q];
+    }
+    e = wave_sum(e);
+    sv = wave_sum(sv);
+    if (threadIdx.x == 0) {
+        es[0] = e;
+        es[1] = sv;
+    }
+}
+
+extern "C" int neorx_rxchf_omg1_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dbe,
+                                    const double *d_dp, const double *d_gam1, const double *d_gam1s, double *d_f,
+                                    double *d_sf, double *d_es, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (nebf <= 0 || npbf <= 0 || (which != 1 && which != 2)) return fail(NEORX_E_ARG, "RXCHF OMG1: bad arguments");
+    if (!d_dbe || !d_dp || !d_gam1 || !d_gam1s || !d_f || !d_sf || !d_es) return fail(NEORX_E_ARG, "null argument");
+    hipLaunchKernelGGL(k_rxchf_omg1, dim3(1), dim3(64), 0, (hipStream_t)stream, (int)which, nebf, npbf, d_dbe, d_dp,
+                       d_gam1, d_gam1s, d_f, d_sf, d_es);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int rxchf_omg1_host(int which, i64 ne, i64 np, const double *dbe, const double *dp, const double *g1,
+                           const double *g1s, double *f, double *sf, double *e, double *sv)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    const i64 ng1 = ne * ne * np * np, nf = which == 1 ? np * np : ne * ne;
+    DevBuf dB, dP, d1, d1s, dF, dSF, dE;
+    rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(d1, g1, ng1);
+    if (!rc) rc = h2d(d1s, g1s, ng1);
+    if (!rc) rc = dF.alloc(nf * 8);
+    if (!rc) rc = dSF.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(16);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dF.p, f, nf * 8, hipMemcpyHostToDevice));
+    rc = neorx_rxchf_omg1_dev(which, ne, np, dB.d(), dP.d(), d1.d(), d1s.d(), dF.d(), dSF.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    double es[2];
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sf, dSF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e = es[0];
+    *sv = es[1];
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxchf_fp_omg1(int64_t nebf, int64_t npbf, const double *dbe, const double *dp,
+                                   const double *gam1, const double *gam1s, double *fp, double *sp, double *e,
+                                   double *s)
+{
+    return rxchf_omg1_host(1, nebf, npbf, dbe, dp, gam1, gam1s, fp, sp, e, s);
+}
+extern "C" int neorx_rxchf_fbe_omg1(int64_t nebf, int64_t npbf, const double *dbe, const double *dp,
+                                    const double *gam1, const double *gam1s, double *fbe, double *sbe, double *e,
+                                    double *s)
+{
+    return rxchf_omg1_host(2, nebf, npbf, dbe, dp, gam1, gam1s, fbe, sbe, e, s);
 }
 
 // ------------------------------------------------------------------

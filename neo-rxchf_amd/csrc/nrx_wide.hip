@@ -90,6 +90,7 @@ int nrx_wide_prims(int cls, int src, const DevSys &d, i64 start, const i64 *idx,
     case 7: return launch_wide_src<7>(src, d, start, idx, oe, os, ox, oy, list, counts, wbin, n, st);
     case 8: return launch_wide_src<8>(src, d, start, idx, oe, os, ox, oy, list, counts, wbin, n, st);
     case 9: return launch_wide_src<9>(src, d, start, idx, oe, os, ox, oy, list, counts, wbin, n, st);
+    case 10: return launch_wide_src<10>(src, d, start, idx, oe, os, ox, oy, list, counts, wbin, n, st);
     default: return fail(NEORX_E_ARG, "wide tuples: unknown class %d", cls);
     }
 }

@@ -172,6 +172,42 @@ module neorx_capi
       real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
       integer(c_int) :: neorx_rxchf_fbe_omg2
     end function
+    function neorx_rxchf_gam3_ic1(h, g1, g2) bind(C, name="neorx_rxchf_gam3_ic1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: h
+      real(c_double) :: g1(*), g2(*)
+      integer(c_int) :: neorx_rxchf_gam3_ic1
+    end function
+    function neorx_rxchf_fae_omg3(ne, np, da, db, dp, g1, g2, f, e) bind(C, name="neorx_rxchf_fae_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), f(*), e
+      integer(c_int) :: neorx_rxchf_fae_omg3
+    end function
+    function neorx_rxchf_fp_omg3(ne, np, da, db, dp, g1, g2, f, e) bind(C, name="neorx_rxchf_fp_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), f(*), e
+      integer(c_int) :: neorx_rxchf_fp_omg3
+    end function
+    function neorx_rxchf_fbe_omg3(ne, np, da, db, dp, g1, g2, f, e) bind(C, name="neorx_rxchf_fbe_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: da(*), db(*), dp(*), g1(*), g2(*), f(*), e
+      integer(c_int) :: neorx_rxchf_fbe_omg3
+    end function
+    function neorx_rxchf_fp_omg1(ne, np, db, dp, g, gs, f, sf, e, sv) bind(C, name="neorx_rxchf_fp_omg1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: db(*), dp(*), g(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxchf_fp_omg1
+    end function
+    function neorx_rxchf_fbe_omg1(ne, np, db, dp, g, gs, f, sf, e, sv) bind(C, name="neorx_rxchf_fbe_omg1")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: db(*), dp(*), g(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxchf_fbe_omg1
+    end function
     function neorx_rxchfne_gam2_ic1(h, g, gx) bind(C, name="neorx_rxchfne_gam2_ic1")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: h
@@ -777,6 +813,99 @@ subroutine RXCHF_FBE_OMG2(Nchunks, nebf, npbf, ng2, DAE, DBE, DP, GM2_1ICR, GM2_
   real(c_double) :: FBE(nebf, nebf), SBE(nebf, nebf), E_BE_OMG2, S_BE_OMG2
   call neorx_check(neorx_rxchf_fbe_omg2(nebf, npbf, DAE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FBE, SBE, E_BE_OMG2, &
                    S_BE_OMG2), 'RXCHF_FBE_OMG2')
+end subroutine
+
+! RXCHF_GAM1_OMP_MD, source/rxchf/rxchf_gam_1_OMP.f:2-397: the same
+! computation as RXCHFmult_GAM1_MD (GAM_1.ufm / GAM_1s.ufm)
+subroutine RXCHF_GAM1_OMP_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                             AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  call RXCHFmult_GAM1_MD(nebf, npebf, npbf, ng1, ng1prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                         AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC)
+end subroutine
+
+! RXCHF_GAM3_IC1, source/rxchf/rxchf_gam3_IC1_OMP.f:2-283
+subroutine RXCHF_GAM3_IC1(Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, &
+                          KPESTR, KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, &
+                          GM3_1ICR, GM3_2ICR)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npebf, npbf, ng3, ng3prm, nat, ngtg1
+  real(c_double) :: pmass, cat(3, nat), zan(nat), bcoef1(ngtg1), gamma1(ngtg1)
+  integer(c_int64_t) :: KPESTR(nebf), KPEEND(nebf), AMPEB2C(npebf), ELCAM(npebf, 3), NUCAM(npbf, 3)
+  real(c_double) :: AGEBFCC(npebf), AGNBFCC(npbf), ELCEX(npebf), NUCEX(npbf), ELCBFC(npebf, 3), NUCBFC(npbf, 3)
+  real(c_double) :: GM3_1ICR(ng3), GM3_2ICR(ng3)
+  integer(c_int64_t) :: h
+  call neorx_check(neorx_system_create(nebf, npebf, npbf, nat, ngtg1, pmass, cat, zan, bcoef1, gamma1, KPESTR, &
+                   KPEEND, AMPEB2C, AGEBFCC, AGNBFCC, ELCEX, NUCEX, ELCAM, NUCAM, ELCBFC, NUCBFC, h), 'RXCHF_GAM3_IC1')
+  call neorx_check(neorx_rxchf_gam3_ic1(h, GM3_1ICR, GM3_2ICR), 'RXCHF_GAM3_IC1')
+  call neorx_check(neorx_system_destroy(h), 'RXCHF_GAM3_IC1')
+end subroutine
+
+! RXCHF_FAE_OMG3, source/rxchf/rxchf_omg3.f:2-105
+subroutine RXCHF_FAE_OMG3(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FAE, E_AE_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3_1ICR(ng3), GM3_2ICR(ng3)
+  real(c_double) :: FAE(nebf, nebf), E_AE_OMG3
+  call neorx_check(neorx_rxchf_fae_omg3(nebf, npbf, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FAE, E_AE_OMG3), &
+                   'RXCHF_FAE_OMG3')
+end subroutine
+
+! RXCHF_FP_OMG3, source/rxchf/rxchf_omg3.f:263-366
+subroutine RXCHF_FP_OMG3(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FP, E_P_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3_1ICR(ng3), GM3_2ICR(ng3)
+  real(c_double) :: FP(npbf, npbf), E_P_OMG3
+  call neorx_check(neorx_rxchf_fp_omg3(nebf, npbf, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FP, E_P_OMG3), 'RXCHF_FP_OMG3')
+end subroutine
+
+! RXCHF_FBE_OMG3, source/rxchf/rxchf_omg3.f:486-589
+subroutine RXCHF_FBE_OMG3(Nchunks, nebf, npbf, ng3, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FBE, E_BE_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf), GM3_1ICR(ng3), GM3_2ICR(ng3)
+  real(c_double) :: FBE(nebf, nebf), E_BE_OMG3
+  call neorx_check(neorx_rxchf_fbe_omg3(nebf, npbf, DAE, DBE, DP, GM3_1ICR, GM3_2ICR, FBE, E_BE_OMG3), &
+                   'RXCHF_FBE_OMG3')
+end subroutine
+
+! RXCHF_FP_OMG1, source/rxchf/rxchf_omg1.f:2-82: reads GAM_1.ufm / GAM_1s.ufm
+subroutine RXCHF_FP_OMG1(nebf, npbf, ng1, DBE, DP, FP, SP, E_P_OMG1, S_P_OMG1)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npbf, ng1
+  real(c_double) :: DBE(nebf, nebf), DP(npbf, npbf), FP(npbf, npbf), SP(npbf, npbf), E_P_OMG1, S_P_OMG1
+  real(c_double), allocatable :: g(:), gs(:)
+  allocate(g(ng1), gs(ng1))
+  call ufm_read(801, 'GAM_1.ufm', ng1, g)
+  call ufm_read(802, 'GAM_1s.ufm', ng1, gs)
+  call neorx_check(neorx_rxchf_fp_omg1(nebf, npbf, DBE, DP, g, gs, FP, SP, E_P_OMG1, S_P_OMG1), 'RXCHF_FP_OMG1')
+  deallocate(g, gs)
+end subroutine
+
+! RXCHF_FBE_OMG1, source/rxchf/rxchf_omg1.f:84-165: reads GAM_1.ufm / GAM_1s.ufm
+subroutine RXCHF_FBE_OMG1(nebf, npbf, ng1, DBE, DP, FBE, SBE, E_BE_OMG1, S_BE_OMG1)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: nebf, npbf, ng1
+  real(c_double) :: DBE(nebf, nebf), DP(npbf, npbf), FBE(nebf, nebf), SBE(nebf, nebf), E_BE_OMG1, S_BE_OMG1
+  real(c_double), allocatable :: g(:), gs(:)
+  allocate(g(ng1), gs(ng1))
+  call ufm_read(801, 'GAM_1.ufm', ng1, g)
+  call ufm_read(802, 'GAM_1s.ufm', ng1, gs)
+  call neorx_check(neorx_rxchf_fbe_omg1(nebf, npbf, DBE, DP, g, gs, FBE, SBE, E_BE_OMG1, S_BE_OMG1), &
+                   'RXCHF_FBE_OMG1')
+  deallocate(g, gs)
 end subroutine
 
 ! RXCHFne_FAE_OMG2, source/rxchfne/rxchfne_omg2.f:2-116

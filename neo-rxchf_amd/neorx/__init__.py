@@ -54,6 +54,11 @@ def load():
     _sig("neorx_rxchf_gam2_ic1", [_I64, _P, _P, _P])
     for nm in ("neorx_rxchf_fae_omg2", "neorx_rxchf_fp_omg2", "neorx_rxchf_fbe_omg2"):
         _sig(nm, [_I64, _I64] + [_P] * 10)
+    _sig("neorx_rxchf_gam3_ic1", [_I64, _P, _P])
+    for nm in ("neorx_rxchf_fae_omg3", "neorx_rxchf_fp_omg3", "neorx_rxchf_fbe_omg3"):
+        _sig(nm, [_I64, _I64] + [_P] * 7)
+    for nm in ("neorx_rxchf_fp_omg1", "neorx_rxchf_fbe_omg1"):
+        _sig(nm, [_I64, _I64] + [_P] * 8)
     _sig("neorx_transfer_bytes", [_P, _P])
     _sig("neorx_system_create", [_I64, _I64, _I64, _I64, _I64, _D] + [_P] * 15 + [_P])
     _sig("neorx_system_destroy", [_I64])
@@ -300,6 +305,15 @@ class System:
         _check(_lib.neorx_rxchf_gam2_ic1(self.handle, _p(g1), _p(g2), _p(gs)))
         return g1, g2, gs
 
+    def RXCHF_GAM3_IC1(self):
+        """RXCHF_GAM3_IC1 (source/rxchf/rxchf_gam3_IC1_OMP.f:2): (GM3_1ICR,
+        GM3_2ICR)."""
+        ne, npb = self.s["nebf"], self.s["npbf"]
+        g1 = _np.zeros(ne ** 6 * npb * npb)
+        g2 = _np.zeros_like(g1)
+        _check(_lib.neorx_rxchf_gam3_ic1(self.handle, _p(g1), _p(g2)))
+        return g1, g2
+
     def RXCHFne_GAM2_IC1(self):
         """RXCHFne_GAM2_IC1 (source/rxchfne/rxchfne_gam2_IC1_OMP.f:2):
         returns (GM2ICR, GM2exICR)."""
@@ -358,6 +372,29 @@ def RXCHF_OMG2(which, nebf, npbf, DAE, DBE, DP, G1, G2, Gs, F, SF):
     e, sv = _np.zeros(1), _np.zeros(1)
     _check(fn(nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(G1)),
               _p(_np.ascontiguousarray(G2)), _p(_np.ascontiguousarray(Gs)), _p(F), _p(SF), _p(e), _p(sv)))
+    return float(e[0]), float(sv[0])
+
+
+def RXCHF_OMG3(which, nebf, npbf, DAE, DBE, DP, G1, G2, F):
+    """RXCHF_FAE_OMG3 / _FP_OMG3 / _FBE_OMG3 (source/rxchf/rxchf_omg3.f:2,
+    :263, :486) for which = 0, 1, 2: F accumulates in place; returns E."""
+    load()
+    fn = (_lib.neorx_rxchf_fae_omg3, _lib.neorx_rxchf_fp_omg3, _lib.neorx_rxchf_fbe_omg3)[which]
+    e = _np.zeros(1)
+    _check(fn(nebf, npbf, _p(_fort(DAE)), _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(G1)),
+              _p(_np.ascontiguousarray(G2)), _p(F), _p(e)))
+    return float(e[0])
+
+
+def RXCHF_OMG1(which, nebf, npbf, DBE, DP, G1, G1s, F, SF):
+    """RXCHF_FP_OMG1 / RXCHF_FBE_OMG1 (source/rxchf/rxchf_omg1.f:2, :84) for
+    which = 1, 2 over the GAM_1 / GAM_1s arrays: F accumulates, SF is
+    overwritten; returns (E, S)."""
+    load()
+    fn = _lib.neorx_rxchf_fp_omg1 if which == 1 else _lib.neorx_rxchf_fbe_omg1
+    e, sv = _np.zeros(1), _np.zeros(1)
+    _check(fn(nebf, npbf, _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(G1)),
+              _p(_np.ascontiguousarray(G1s)), _p(F), _p(SF), _p(e), _p(sv)))
     return float(e[0]), float(sv[0])
 
 

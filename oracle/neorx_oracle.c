@@ -986,6 +986,51 @@ static void gam3xchf_prim(const sys_t *S, const prim_t *b, const prim_t *k, doub
     *ans = gHEg + gVEEg1 * 0.5 + 4.0 * gVEEg2 * 0.5;
 }
 
+/* RXCHF_xcalc_GAM3_MD (rxchf/rxchf_gam_3_OMP.f:617-1032): e1 the special
+ * electron, e2 and e3 regular, p the proton; the (k,l) terms only, two
+ * outputs (:1010-1011):
+ *   ans1 = gVEEg1: G2_MD_xggs(e1, p; gk, gl) x gfvee(e2, e3)       (:892-920)
+ *   ans2 = gHEg + gVEEg2 + gVEEg3 with
+ *     gHEg   = hcore(e3) G3_MD_xggs(e1, e2, p; g(e1,p) gk, g(e2,p) gl)
+ *              - G4_MD_xgVepg(e1, e2, e3, p; gk, gl)                 (:766-889)
+ *     gVEEg2 = G4_MD_xgVeeg(e1, e3, e2, p; gk, gl)                   (:922-945)
+ *     gVEEg3 = G4_MD_xgVeeg(e2, e3, e1, p; gk, gl)                   (:956-984)
+ * accumulated with bcoef(k) bcoef(l) (:999-1002) */
+static void rxchf3_prim(const sys_t *S, const prim_t *b, const prim_t *k, double *ans1, double *ans2)
+{
+    double gHEg = 0, gVEEg1 = 0, gVEEg2 = 0, gVEEg3 = 0;
+    for (i64 ik = 0; ik < S->ngtg; ++ik)
+        for (i64 il = 0; il < S->ngtg; ++il) {
+            double gk = S->gamma[ik], gl = S->gamma[il];
+            double xke = g1(OP_KIN, &b[2], &k[2], NULL, 1.0);
+            double vc = 0.0;
+            for (i64 c = 0; c < S->nat; ++c) vc += S->zan[c] * g1(OP_NUC, &b[2], &k[2], S->cat + 3 * c, 1.0);
+            double xx = xke + vc * (-1.0);
+            double yy = g3(OP_OVL, &b[0], &b[1], &b[3], &k[0], &k[1], &k[3], 0, gk, 0, 0, 0, gl, NULL, 1.0);
+            double zz = g4(1, b, k, gk, gl);
+            underflow(&xx);
+            underflow(&yy);
+            underflow(&zz);
+            double xgHEg = (xx * yy) + (zz * (-1.0));
+            xx = g2(OP_OVL, &b[0], &b[3], &k[0], &k[3], gk, gl, NULL, 1.0);
+            yy = gfvee(&b[1], &b[2], &k[1], &k[2]);
+            underflow(&xx);
+            underflow(&yy);
+            double xgVEEg1 = xx * yy;
+            prim_t b2[4] = {b[0], b[2], b[1], b[3]}, k2[4] = {k[0], k[2], k[1], k[3]};
+            double xgVEEg2 = g4(0, b2, k2, gk, gl);
+            prim_t b3[4] = {b[1], b[2], b[0], b[3]}, k3[4] = {k[1], k[2], k[0], k[3]};
+            double xgVEEg3 = g4(0, b3, k3, gk, gl);
+            double bb = S->bcoef[ik] * S->bcoef[il];
+            gHEg += bb * xgHEg;
+            gVEEg1 += bb * xgVEEg1;
+            gVEEg2 += bb * xgVEEg2;
+            gVEEg3 += bb * xgVEEg3;
+        }
+    *ans1 = gVEEg1;
+    *ans2 = gHEg + gVEEg2 + gVEEg3;
+}
+
 /* ------------------------------------------------------------------ */
 /* Basis access (Fortran arrays of the reference)                      */
 /* ------------------------------------------------------------------ */
@@ -1040,7 +1085,7 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
 {
     /* cls 4: the RXCHF-ne e-e-p tuple; cls 5, 6: the RXCHFmult e-e-p and
      * e-e-e-p tuples (primitive indices over one combined list) */
-    int nel = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls;
+    int nel = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7 || cls == 10) ? 3 : cls == 8 ? 1 : (int)cls;
     int w = 2 * (nel + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
@@ -1070,6 +1115,8 @@ static void prim_list_impl(const basis_t *Bp, const sys_t *Sp, i64 cls, i64 ntup
             gam1_prim_impl(Sp, 1, &b[0], &b[1], &k[0], &k[1], &e, &s);
         else if (cls == 9) /* (ansE1, ansS, ansE2) */
             rxchf2_prim(Sp, b, k, &e, &e2, &s);
+        else if (cls == 10) /* (ans1, ans2) */
+            rxchf3_prim(Sp, b, k, &e, &s);
         else
             gam3_prim(Sp, b, k, &e);
         outE[t] = e;
@@ -2947,3 +2994,151 @@ void orc_rxchf_omg2(i64 which, i64 nebf, i64 npbf, const double *DAE, const doub
 #undef RXB
 #undef RXP
 
+/* RXCHF_GAM3_IC1 (rxchf_gam3_IC1_OMP.f:2-283) with
+ * RXCHF_contract_omega3_conv (rxchf_gam3_CONV_OMP.f:479-713): contracted ans1
+ * averaged over the exchange of pairs 2 and 3, (x123 + x132)/2 (GM3_1,
+ * :211-217), and contracted ans2 as is (GM3_2, :235-238) */
+void orc_rxchf_gam3_ic1(BAS_ARGS, SYS_ARGS, double *GM3_1, double *GM3_2)
+{
+    BAS_INIT;
+    SYS_INIT;
+    i64 ne = nebf, np = npbf, ng3 = ne * ne * ne * ne * ne * ne * np * np;
+    double *X1 = (double *)malloc(sizeof(double) * ng3);
+#pragma omp parallel for collapse(3) schedule(dynamic, 1)
+    for (i64 ip = 0; ip < np; ++ip)
+        for (i64 jp = 0; jp < np; ++jp)
+            for (i64 i1 = 0; i1 < ne; ++i1)
+                for (i64 j1 = 0; j1 < ne; ++j1)
+                    for (i64 i2 = 0; i2 < ne; ++i2)
+                        for (i64 j2 = 0; j2 < ne; ++j2)
+                            for (i64 i3 = 0; i3 < ne; ++i3)
+                                for (i64 j3 = 0; j3 < ne; ++j3) {
+                                    double o1 = 0.0, o2 = 0.0;
+                                    for (i64 a1 = kpestr[i1] - 1; a1 < kpeend[i1]; ++a1)
+                                    for (i64 c1 = kpestr[j1] - 1; c1 < kpeend[j1]; ++c1)
+                                    for (i64 a2 = kpestr[i2] - 1; a2 < kpeend[i2]; ++a2)
+                                    for (i64 c2 = kpestr[j2] - 1; c2 < kpeend[j2]; ++c2)
+                                    for (i64 a3 = kpestr[i3] - 1; a3 < kpeend[i3]; ++a3)
+                                    for (i64 c3 = kpestr[j3] - 1; c3 < kpeend[j3]; ++c3) {
+                                        prim_t b[4] = {eprim(&B, a1), eprim(&B, a2), eprim(&B, a3), pprim(&B, ip)};
+                                        prim_t k[4] = {eprim(&B, c1), eprim(&B, c2), eprim(&B, c3), pprim(&B, jp)};
+                                        double v1, v2;
+                                        rxchf3_prim(&S, b, k, &v1, &v2);
+                                        double cf = agnbfcc[ip] * agnbfcc[jp] * agebfcc[a1] * agebfcc[c1] *
+                                                    agebfcc[a2] * agebfcc[c2] * agebfcc[a3] * agebfcc[c3];
+                                        o1 += v1 * cf;
+                                        o2 += v2 * cf;
+                                    }
+                                    i64 ia = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3);
+                                    X1[ia] = o1;
+                                    GM3_2[ia] = o2;
+                                }
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1)
+    for (i64 i2 = 0; i2 < ne; ++i2)
+    for (i64 j2 = 0; j2 < ne; ++j2)
+    for (i64 i3 = 0; i3 < ne; ++i3)
+    for (i64 j3 = 0; j3 < ne; ++j3) {
+        i64 a = ix3(ne, np, ip, jp, i1, j1, i2, j2, i3, j3), c = ix3(ne, np, ip, jp, i1, j1, i3, j3, i2, j2);
+        GM3_1[a] = (X1[a] + X1[c]) * 0.5;
+    }
+    free(X1);
+}
+
+#define RXA(i, j) DAE[(j) * ne + (i)]
+#define RXB(i, j) DBE[(j) * ne + (i)]
+#define RXP(i, j) DP[(j) * np + (i)]
+#define G31(a1, b1, a2, b2, a3, b3) GM3_1[ix3(ne, np, ip, jp, a1, b1, a2, b2, a3, b3)]
+#define G32(a1, b1, a2, b2, a3, b3) GM3_2[ix3(ne, np, ip, jp, a1, b1, a2, b2, a3, b3)]
+/* RXCHF_FAE_OMG3 / _FP_OMG3 / _FBE_OMG3 (rxchf_omg3.f:2-708); which = 0, 1, 2
+ * selects the routine.  Per entry (ip,jp,i1,j1,i2,j2,i3,j3):
+ *   val1 = GM3_1(i1 j1 i2 j2 i3 j3) - GM3_1(i1 j1 i2 j3 i3 j2)/2
+ * (:215, :465, :688).
+ * FAE (:226-251): val2 = (GM3_2(i1 j2 i2 j1 i3 j3) + GM3_2(i1 i2 j2 j1 i3 j3))
+ *                        - (GM3_2(i1 j3 i2 j1 i3 j2) + GM3_2(i1 j3 j2 j1 i3 i2))/2,
+ *   val3 = (GM3_2(i1 j3 i3 j1 i2 j2) + GM3_2(i1 j3 i3 j1 j2 i2))
+ *          - (GM3_2(i1 j2 i3 j1 i2 j3) + GM3_2(i1 i2 i3 j1 j2 j3))/2,
+ *   FAE(i2,j2) += DAE(i3,j3) DBE(i1,j1) DP(ip,jp) (2 val1 - (val2 + val3)/2),
+ *   E += DP DBE(i1,j1) DAE(i2,j2) DAE(i3,j3) (val1 - val2/2) / 2;
+ * FP, FBE (:466-475, :689-698): val2 = GM3_2(i1 j2 i2 j1 i3 j3)
+ *   - GM3_2(i1 j3 i2 j1 i3 j2)/2, w = DAE(i2,j2) DAE(i3,j3) (val1 - val2) / 2,
+ *   FP(ip,jp) += DBE(i1,j1) w, FBE(i1,j1) += DP(ip,jp) w, E += DP DBE w.
+ * F accumulates (add2fock); E is overwritten. */
+void orc_rxchf_omg3(i64 which, i64 nebf, i64 npbf, const double *DAE, const double *DBE, const double *DP,
+                    const double *GM3_1, const double *GM3_2, double *F, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1)
+    for (i64 i2 = 0; i2 < ne; ++i2)
+    for (i64 j2 = 0; j2 < ne; ++j2)
+    for (i64 i3 = 0; i3 < ne; ++i3)
+    for (i64 j3 = 0; j3 < ne; ++j3) {
+        double val1 = G31(i1, j1, i2, j2, i3, j3) - 0.5 * G31(i1, j1, i2, j3, i3, j2);
+        if (which == 0) {
+            double val2 = (G32(i1, j2, i2, j1, i3, j3) + G32(i1, i2, j2, j1, i3, j3)) -
+                          0.5 * (G32(i1, j3, i2, j1, i3, j2) + G32(i1, j3, j2, j1, i3, i2));
+            double val3 = (G32(i1, j3, i3, j1, i2, j2) + G32(i1, j3, i3, j1, j2, i2)) -
+                          0.5 * (G32(i1, j2, i3, j1, i2, j3) + G32(i1, i2, i3, j1, j2, j3));
+            xf[j2 * ne + i2] += RXA(i3, j3) * RXB(i1, j1) * RXP(ip, jp) * (2.0 * val1 - 0.5 * (val2 + val3));
+            e += 0.5 * RXP(ip, jp) * RXB(i1, j1) * RXA(i2, j2) * RXA(i3, j3) * (val1 - 0.5 * val2);
+        } else {
+            double val2 = G32(i1, j2, i2, j1, i3, j3) - 0.5 * G32(i1, j3, i2, j1, i3, j2);
+            double w = 0.5 * RXA(i2, j2) * RXA(i3, j3) * (val1 - val2);
+            if (which == 1)
+                xf[jp * np + ip] += RXB(i1, j1) * w;
+            else
+                xf[j1 * ne + i1] += RXP(ip, jp) * w;
+            e += RXP(ip, jp) * RXB(i1, j1) * w;
+        }
+    }
+    for (i64 k = 0; k < nf; ++k) F[k] += xf[k];
+    *E = e;
+    free(xf);
+}
+#undef G31
+#undef G32
+
+/* RXCHF_FP_OMG1 / RXCHF_FBE_OMG1 (rxchf_omg1.f:2-165, per entry :60-66 and
+ * :142-148) over the GAM_1 and
+ * GAM_1s arrays (pack_4D(nebf,nebf,npbf, je1,ie1,jp,ip) order, as the .ufm
+ * files hold them); which = 1 (FP) or 2 (FBE).  F accumulates (add2fock), SF
+ * is overwritten with the GAM_1s contraction, E and S overwritten. */
+void orc_rxchf_omg1(i64 which, i64 nebf, i64 npbf, const double *DBE, const double *DP, const double *G1,
+                    const double *G1s, double *F, double *SF, double *E, double *Sv)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double));
+    double e = 0.0, sv = 0.0;
+    for (i64 k = 0; k < nf; ++k) SF[k] = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1) {
+        i64 ia = ((ip * np + jp) * ne + i1) * ne + j1;
+        if (which == 1) {
+            xf[jp * np + ip] += RXB(i1, j1) * G1[ia];
+            SF[jp * np + ip] += RXB(i1, j1) * G1s[ia];
+        } else {
+            xf[j1 * ne + i1] += RXP(ip, jp) * G1[ia];
+            SF[j1 * ne + i1] += RXP(ip, jp) * G1s[ia];
+        }
+        e += RXP(ip, jp) * RXB(i1, j1) * G1[ia];
+        sv += RXP(ip, jp) * RXB(i1, j1) * G1s[ia];
+    }
+    for (i64 k = 0; k < nf; ++k) F[k] += xf[k];
+    *E = e;
+    *Sv = sv;
+    free(xf);
+}
+#undef RXA
+#undef RXB
+#undef RXP

@@ -204,12 +204,13 @@ def ref_prim_list(s, cls, idx):
 def ref_prim_list3(s, cls, idx):
     """As ref_prim_list; cls 5: RXCHFmult_GAM2_MD (ansE1, ansS, ansE2),
     cls 6: RXCHFmult_INT_GAM3_MD, over a combined one-basis dict; cls 9:
-    RXCHF_xcalc_GAM2_MD (ansE1, ansS, ansE2)."""
+    RXCHF_xcalc_GAM2_MD (ansE1, ansS, ansE2); cls 10: RXCHF_xcalc_GAM3_MD
+    (ans1, ans2 as E and S)."""
     R = ref()
     fn = {1: R.xcalc_gam1_md_, 2: R.xcalc_gam2_md_, 3: R.xcalc_gam3_md_, 4: R.rxchfne_xcalc_gam2_md_,
           5: R.rxchfmult_gam2_md_, 6: R.rxchfmult_int_gam3_md_, 7: R.rxchfmult_xchf_gam3_md_,
-          8: R.rxchfmult_xcalc_gam1_md_, 9: R.rxchf_xcalc_gam2_md_}[cls]
-    nel = {4: 2, 5: 2, 6: 3, 7: 3, 8: 1, 9: 2}.get(cls, cls)
+          8: R.rxchfmult_xcalc_gam1_md_, 9: R.rxchf_xcalc_gam2_md_, 10: R.rxchf_xcalc_gam3_md_}[cls]
+    nel = {4: 2, 5: 2, 6: 3, 7: 3, 8: 1, 9: 2, 10: 3}.get(cls, cls)
     am_e, am_p = np.asarray(s["elcam"]), np.asarray(s["nucam"])
     c_e, c_p = np.asarray(s["elcbfc"]), np.asarray(s["nucbfc"])
     cat = np.ascontiguousarray(s["cat"])
@@ -1122,3 +1123,75 @@ def ref_rxchf_omg2(which, ne, npb, DAE, DBE, DP, G1, G2, Gs, F, SF):
        _p(np.ascontiguousarray(G2)), _p(np.ascontiguousarray(Gs)), _p(F), _p(SF), C.byref(e), C.byref(sv))
     return e.value, sv.value
 
+
+def rxchf_gam3_ic1(s):
+    """RXCHF_GAM3_IC1 restated: (GM3_1ICR, GM3_2ICR)."""
+    ne, npb = s["nebf"], s["npbf"]
+    g1 = np.zeros(ne ** 6 * npb * npb)
+    g2 = np.zeros_like(g1)
+    ba, k1 = _bas(s)
+    sa, k2 = _sysargs(s)
+    lib().orc_rxchf_gam3_ic1(*ba, *sa, _p(g1), _p(g2))
+    return g1, g2
+
+
+def rxchf_omg3(which, ne, npb, DAE, DBE, DP, G1, G2, F):
+    """which: 0 RXCHF_FAE_OMG3, 1 RXCHF_FP_OMG3, 2 RXCHF_FBE_OMG3; F
+    accumulates in place; returns E."""
+    e = np.zeros(1)
+    lib().orc_rxchf_omg3(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAE), _f(DBE), _f(DP),
+                         _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G2)), _p(F), _p(e))
+    return float(e[0])
+
+
+def rxchf_omg1(which, ne, npb, DBE, DP, G1, G1s, F, SF):
+    """which: 1 RXCHF_FP_OMG1, 2 RXCHF_FBE_OMG1 over the GAM_1 / GAM_1s
+    arrays; F accumulates, SF is overwritten; returns (E, S)."""
+    e, sv = np.zeros(1), np.zeros(1)
+    lib().orc_rxchf_omg1(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DBE), _f(DP),
+                         _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G1s)), _p(F), _p(SF), _p(e), _p(sv))
+    return float(e[0]), float(sv[0])
+
+
+def ref_rxchf_gam3_ic1(s):
+    """RXCHF_GAM3_IC1 (source/rxchf/rxchf_gam3_IC1_OMP.f:2) of the reference."""
+    ne, npb = s["nebf"], s["npbf"]
+    ng3 = ne ** 6 * npb * npb
+    g1, g2 = np.zeros(ng3), np.zeros(ng3)
+    keep = _ref_basis_args(s)
+    cat, zan, b, gm = (np.ascontiguousarray(s[k]) for k in ("cat", "zan", "bcoef", "gamma"))
+    ref().rxchf_gam3_ic1_(_i8(1), _i8(ne), _i8(s["npebf"]), _i8(npb), _i8(ng3), _i8(0), _i8(s["nat"]),
+                          _i8(s["ngtg"]), _d(s["pmass"]), _p(cat), _p(zan), _p(b), _p(gm), *[_p(a) for a in keep],
+                          _p(g1), _p(g2))
+    return g1, g2
+
+
+def ref_rxchf_omg3(which, ne, npb, DAE, DBE, DP, G1, G2, F):
+    """RXCHF_FAE_OMG3 / _FP_OMG3 / _FBE_OMG3 (source/rxchf/rxchf_omg3.f:2,
+    :263, :486) of the reference; returns E."""
+    e = C.c_double()
+    ng3 = ne ** 6 * npb * npb
+    R = ref()
+    fn = (R.rxchf_fae_omg3_, R.rxchf_fp_omg3_, R.rxchf_fbe_omg3_)[which]
+    fn(_i8(1), _i8(ne), _i8(npb), _i8(ng3), _f(DAE), _f(DBE), _f(DP), _p(np.ascontiguousarray(G1)),
+       _p(np.ascontiguousarray(G2)), _p(F), C.byref(e))
+    return e.value
+
+
+def ref_rxchf_omg1(which, ne, npb, DBE, DP, G1, G1s, F, SF, workdir):
+    """RXCHF_FP_OMG1 / RXCHF_FBE_OMG1 (source/rxchf/rxchf_omg1.f:2, :84) of
+    the reference: they read GAM_1.ufm and GAM_1s.ufm from the working
+    directory (8-byte records, the flang build), written here into workdir
+    first; returns (E, S)."""
+    np.ascontiguousarray(G1, dtype=np.float64).tofile(os.path.join(workdir, "GAM_1.ufm"))
+    np.ascontiguousarray(G1s, dtype=np.float64).tofile(os.path.join(workdir, "GAM_1s.ufm"))
+    e, sv = C.c_double(), C.c_double()
+    R = ref()
+    fn = R.rxchf_fp_omg1_ if which == 1 else R.rxchf_fbe_omg1_
+    cwd = os.getcwd()
+    os.chdir(workdir)
+    try:
+        fn(_i8(ne), _i8(npb), _i8(len(G1)), _f(DBE), _f(DP), _p(F), _p(SF), C.byref(e), C.byref(sv))
+    finally:
+        os.chdir(cwd)
+    return e.value, sv.value

@@ -72,6 +72,7 @@ static void eval_tuple(const SysParams &S, int cls, const PairRec *q, int tier, 
     else if (cls == 7) eval_one<7>(S, qp, tier, e, s, x);
     else if (cls == 8) eval_one<8>(S, qp, tier, e, s, x);
     else if (cls == 9) eval_one<9>(S, qp, tier, e, s, x);
+    else if (cls == 10) eval_one<10>(S, qp, tier, e, s, x);
     else eval_one<3>(S, qp, tier, e, s, x);
 }
 
@@ -82,7 +83,7 @@ extern "C" void eng_prim_list(i64 npebf, i64 npbf, const double *elcex, const do
                               double *outX, i64 tier)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls, w = 2 * (ne + 1);
+    int ne = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7 || cls == 10) ? 3 : cls == 8 ? 1 : (int)cls, w = 2 * (ne + 1);
 #pragma omp parallel for schedule(dynamic, 4)
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];
@@ -136,7 +137,7 @@ extern "C" double eng_flops_per_tuple(i64 npebf, i64 npbf, const double *elcex, 
                                       const double *gamma, i64 cls, i64 ntup, const i64 *idx)
 {
     SysParams S{(int)nat, (int)ngtg, pmass, cat, zan, bcoef, gamma, host_boys_table()};
-    int ne = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7) ? 3 : cls == 8 ? 1 : (int)cls, w = 2 * (ne + 1);
+    int ne = (cls == 4 || cls == 5 || cls == 9) ? 2 : (cls == 6 || cls == 7 || cls == 10) ? 3 : cls == 8 ? 1 : (int)cls, w = 2 * (ne + 1);
     nrx_flops = 0.0;
     for (i64 t = 0; t < ntup; ++t) {
         PairRec q[4];

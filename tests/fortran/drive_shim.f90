@@ -234,6 +234,30 @@ program drive_shim
   se = 0.0d0
   call RXCHF_FBE_OMG2(one, nebf, npbf, ng2, dae, dbe, dp, gm2, gx1, gx2, fe, se, e2, s2)
   write(11) fe, se, e2, s2
+  ! RXCHF e-e-e-p and GAM_1 pieces (xcneo_HYBRID.f:756, :907; rxchf_fock.f:501,
+  ! :586, :671, :577, :662)
+  call RXCHF_GAM3_IC1(one, nebf, npebf, npbf, ng3, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, kpestr, &
+                      kpeend, ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc, gm3, gx3)
+  write(11) gm3, gx3
+  fe = 0.0d0
+  call RXCHF_FAE_OMG3(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gx3, fe, e3)
+  write(11) fe, e3
+  fp = 0.0d0
+  call RXCHF_FP_OMG3(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gx3, fp, e3)
+  write(11) fp, e3
+  fe = 0.0d0
+  call RXCHF_FBE_OMG3(one, nebf, npbf, ng3, dae, dbe, dp, gm3, gx3, fe, e3)
+  write(11) fe, e3
+  call RXCHF_GAM1_OMP_MD(nebf, npebf, npbf, ng1, zero, nat, ngtg, pmass, cat, zan, bcoef, gamma, &
+                         ampeb2c, agebfcc, agnbfcc, elcex, nucex, elcam, nucam, elcbfc, nucbfc)
+  fp = 0.0d0
+  sp = 0.0d0
+  call RXCHF_FP_OMG1(nebf, npbf, ng1, dbe, dp, fp, sp, e2, s2)
+  write(11) fp, sp, e2, s2
+  fe = 0.0d0
+  se = 0.0d0
+  call RXCHF_FBE_OMG1(nebf, npbf, ng1, dbe, dp, fe, se, e2, s2)
+  write(11) fe, se, e2, s2
   close(11)
 contains
   subroutine read_ufm(name, n, a)

@@ -241,3 +241,19 @@ def test_engine_rxchf_tuples_vs_oracle(lmax, shells):
             check_close(got[k], ref[k], atol_rel=1e-12 if lmax == 2 else 1e-11,
                         what=f"class 9 out{k} tier {tier} lmax {lmax}")
 
+
+
+@pytest.mark.parametrize("lmax,shells", [(2, (0, 1, 2)), (3, (0, 3))])
+def test_engine_rxchf3_tuples_vs_oracle(lmax, shells):
+    """RXCHF_xcalc_GAM3_MD (class 10: ans1, ans2), every capacity tier (tier
+    0) and the general one (tier 1); the l <= 3 build with f functions."""
+    from _util import engine_prim_list3
+
+    s = synth.small(seed=150 + lmax, e_shells=shells, p_shells=(0, 1))
+    idx = random_tuples(s, 3, 60 if lmax == 2 else 24, seed=6)
+    ref = oracle.prim_list3(s, 10, idx)
+    for tier in (0, 1):
+        got = engine_prim_list3(s, 10, idx, tier, lmax=lmax)
+        for k in range(2):
+            check_close(got[k], ref[k], atol_rel=1e-12 if lmax == 2 else 1e-11,
+                        what=f"class 10 out{k} tier {tier} lmax {lmax}")

@@ -78,14 +78,14 @@ def test_f_tuples_vs_reference_golden(torch_dev, name):
     S.close()
 
 
-@pytest.mark.parametrize("cls,n", [(1, 1024), (2, 512), (3, 128), (4, 256), (8, 512), (9, 512)])
+@pytest.mark.parametrize("cls,n", [(1, 1024), (2, 512), (3, 128), (4, 256), (8, 512), (9, 512), (10, 128)])
 def test_f_prim_lists_vs_oracle(torch_dev, cls, n):
     """Random tuples over s/p/d/f electrons and an s/p/f nucleus (the RXCHF-ne
-    class 4, the RXCHFmult e-p class 8 and the RXCHF class 9 too)."""
+    class 4, the RXCHFmult e-p class 8 and the RXCHF classes 9 and 10 too)."""
     s = synth.small(seed=110 + cls, n_e_centres=2, e_shells=(0, 1, 3), p_shells=(0, 3))
     print(f"case f prim_list cls={cls} n={n}", flush=True)
     S = neorx.System(s)
-    idx = random_tuples(s, {4: 2, 8: 1, 9: 2}.get(cls, cls), n, seed=cls)
+    idx = random_tuples(s, {4: 2, 8: 1, 9: 2, 10: 3}.get(cls, cls), n, seed=cls)
     got = _run(torch_dev, S, cls, idx)
     ref = oracle.prim_list3(s, cls, idx)
     check_close(got[0], ref[0], atol_rel=F_ATOL, what=f"f cls{cls} out0")

@@ -103,7 +103,10 @@ def test_fortran_dropin_matches_oracle(recl):
              ng3, ng3, ng3,  # RXCHFmult_GAM3_IC1ex
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHFmult_FAE/FP/FBE_OMG3ex
              ng2, ng2, ng2,  # RXCHF_GAM2_IC1: GM2_1ICR, GM2_2ICR, GM2sICR
-             ne * ne, ne * ne, 1, 1, npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1]  # RXCHF_FAE/FP/FBE_OMG2
+             ne * ne, ne * ne, 1, 1, npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1,  # RXCHF_FAE/FP/FBE_OMG2
+             ng3, ng3,  # RXCHF_GAM3_IC1: GM3_1ICR, GM3_2ICR
+             ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHF_FAE/FP/FBE_OMG3
+             npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1]  # RXCHF_FP/FBE_OMG1
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -239,5 +242,25 @@ def test_fortran_dropin_matches_oracle(recl):
         at = 105 + 4 * which
         check_close(parts[at], F.ravel(order="F"), what=f"RXCHF omg2 {which} F")
         check_close(parts[at + 1], SF.ravel(order="F"), what=f"RXCHF omg2 {which} S")
+        assert parts[at + 2][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+        assert parts[at + 3][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)
+    # RXCHF e-e-e-p and GAM_1 pieces
+    r3 = oracle.rxchf_gam3_ic1(s)
+    check_close(parts[117], r3[0], what="RXCHF_GAM3_IC1 GM3_1ICR")
+    check_close(parts[118], r3[1], what="RXCHF_GAM3_IC1 GM3_2ICR")
+    for which in range(3):
+        n = npb if which == 1 else ne
+        F = np.zeros((n, n), order="F")
+        e = oracle.rxchf_omg3(which, ne, npb, DAE, DBE, DP, *r3, F)
+        check_close(parts[119 + 2 * which], F.ravel(order="F"), what=f"RXCHF omg3 {which}")
+        assert parts[120 + 2 * which][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    rg1, rg1s = oracle.rxchfmult_gam1(s)
+    for k, which in enumerate((1, 2)):
+        n = npb if which == 1 else ne
+        F, SF = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        e, sv = oracle.rxchf_omg1(which, ne, npb, DBE, DP, rg1, rg1s, F, SF)
+        at = 125 + 4 * k
+        check_close(parts[at], F.ravel(order="F"), what=f"RXCHF omg1 {which} F")
+        check_close(parts[at + 1], SF.ravel(order="F"), what=f"RXCHF omg1 {which} S")
         assert parts[at + 2][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
         assert parts[at + 3][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)

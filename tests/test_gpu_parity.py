@@ -747,6 +747,59 @@ def test_rxchf(torch_dev):
     neorx.release_resident()
 
 
+def test_rxchf3(torch_dev):
+    """RXCHF e-e-e-p side (source/rxchf/): class-10 tuples
+    (RXCHF_xcalc_GAM3_MD), RXCHF_GAM3_IC1, RXCHF_FAE/FP/FBE_OMG3 and
+    RXCHF_FP/FBE_OMG1 through the C ABI against the oracle (itself checked
+    against the reference's routines in tests/test_oracle.py); the
+    contractions over random tensors, so that every index permutation of
+    their formulas reads distinct values."""
+    torch = torch_dev
+    s = synth.small(seed=152, e_shells=(0, 1, 2), p_shells=(0, 1, 2))
+    S = neorx.System(s)
+    idx = random_tuples(s, 3, 512, seed=21)
+    print("case rxchf3 cls10 n=512", flush=True)
+    got = _run_list3(torch, S, 10, idx)
+    ref = oracle.prim_list3(s, 10, idx)
+    check_close(got[0], ref[0], what="cls10 ans1")
+    check_close(got[1], ref[1], what="cls10 ans2")
+    S.close()
+    t = synth.small(seed=153, n_e_centres=3, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    print(f"case rxchf3 builder nebf={t['nebf']} npbf={t['npbf']}", flush=True)
+    T = neorx.System(t)
+    g = T.RXCHF_GAM3_IC1()
+    T.close()
+    r = oracle.rxchf_gam3_ic1(t)
+    check_close(g[0], r[0], what="RXCHF_GAM3_IC1 GM3_1ICR")
+    check_close(g[1], r[1], what="RXCHF_GAM3_IC1 GM3_2ICR")
+    ne, npb = 4, 2
+    rng = np.random.default_rng(29)
+    G1, G2 = rng.standard_normal(ne ** 6 * npb * npb), rng.standard_normal(ne ** 6 * npb * npb)
+    DA, DB, DP = synth.density(ne, 2.0, 1), synth.density(ne, 1.0, 2), synth.density(npb, 1.0, 3)
+    for w in range(3):
+        print(f"case rxchf3 omg3 which={w}", flush=True)
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        eg = neorx.RXCHF_OMG3(w, ne, npb, DA, DB, DP, G1, G2, A)
+        eo = oracle.rxchf_omg3(w, ne, npb, DA, DB, DP, G1, G2, B)
+        check_close(A - F0, B - F0, what=f"RXCHF OMG3 {w} F")
+        assert eg == pytest.approx(eo, rel=1e-11), (w, eg, eo)
+    H1, H1s = rng.standard_normal(ne * ne * npb * npb), rng.standard_normal(ne * ne * npb * npb)
+    for w in (1, 2):
+        print(f"case rxchf omg1 which={w}", flush=True)
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        SA, SB = np.asfortranarray(rng.standard_normal((n, n))), np.zeros((n, n), order="F")
+        eg = neorx.RXCHF_OMG1(w, ne, npb, DB, DP, H1, H1s, A, SA)
+        eo = oracle.rxchf_omg1(w, ne, npb, DB, DP, H1, H1s, B, SB)
+        check_close(A - F0, B - F0, what=f"RXCHF OMG1 {w} F")
+        check_close(SA, SB, what=f"RXCHF OMG1 {w} S")
+        assert eg[0] == pytest.approx(eo[0], rel=1e-11) and eg[1] == pytest.approx(eo[1], rel=1e-11)
+    neorx.release_resident()
+
+
 def test_rxchfmult_xchf_side(torch_dev):
     """RXCHFmult's XCHF-side builders over one basis: class-8 and class-7
     tuples, RXCHFmult_GAM1_MD, RXCHFmult_GAM3_XCHF and RXCHFmult_GAM4_XCHF."""

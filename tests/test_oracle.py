@@ -575,3 +575,45 @@ def test_oracle_rxchf_vs_reference():
         check_close(C2 - S0, D - S0, what=f"RXCHF OMG2 {w} S")
         assert eo[0] == pytest.approx(er[0], rel=1e-12) and eo[1] == pytest.approx(er[1], rel=1e-12)
 
+
+@needs_ref
+def test_oracle_rxchf3_vs_reference(tmp_path):
+    """RXCHF (one special electron): the oracle's RXCHF_xcalc_GAM3_MD (class
+    10), RXCHF_GAM3_IC1, RXCHF_FAE/FP/FBE_OMG3 and RXCHF_FP/FBE_OMG1 against
+    the reference's own routines.  The contractions run over random tensors
+    (every index permutation of the formulas touches distinct values), OMG1
+    over GAM_1 files written for the reference to read."""
+    s = synth.small(seed=142, e_shells=(0, 1), p_shells=(0, 1))
+    idx = random_tuples(s, 3, 8, seed=10)
+    ref = oracle.ref_prim_list3(s, 10, idx)
+    got = oracle.prim_list3(s, 10, idx)
+    check_close(got[0], ref[0], what="RXCHF_xcalc_GAM3_MD ans1")
+    check_close(got[1], ref[1], what="RXCHF_xcalc_GAM3_MD ans2")
+    t = synth.small(seed=143, n_e_centres=2, e_shells=(0,), contract=(1, 2), p_shells=(0, 1), ngtg=2)
+    g = oracle.rxchf_gam3_ic1(t)
+    rg = oracle.ref_rxchf_gam3_ic1(t)
+    check_close(g[0], rg[0], what="RXCHF_GAM3_IC1 GM3_1ICR")
+    check_close(g[1], rg[1], what="RXCHF_GAM3_IC1 GM3_2ICR")
+    ne, npb = 3, 2
+    rng = np.random.default_rng(4)
+    G1, G2 = rng.standard_normal(ne ** 6 * npb * npb), rng.standard_normal(ne ** 6 * npb * npb)
+    DA, DB, DP = synth.density(ne, 2.0, 5), synth.density(ne, 1.0, 6), synth.density(npb, 1.0, 7)
+    for w in range(3):
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        eo = oracle.rxchf_omg3(w, ne, npb, DA, DB, DP, G1, G2, A)
+        er = oracle.ref_rxchf_omg3(w, ne, npb, DA, DB, DP, G1, G2, B)
+        check_close(A - F0, B - F0, what=f"RXCHF OMG3 {w} F")
+        assert eo == pytest.approx(er, rel=1e-12)
+    H1, H1s = rng.standard_normal(ne * ne * npb * npb), rng.standard_normal(ne * ne * npb * npb)
+    for w in (1, 2):
+        n = npb if w == 1 else ne
+        F0 = np.asfortranarray(rng.standard_normal((n, n)))
+        A, B = F0.copy(order="F"), F0.copy(order="F")
+        SA, SB = np.asfortranarray(rng.standard_normal((n, n))), np.zeros((n, n), order="F")
+        eo = oracle.rxchf_omg1(w, ne, npb, DB, DP, H1, H1s, A, SA)
+        er = oracle.ref_rxchf_omg1(w, ne, npb, DB, DP, H1, H1s, B, SB, str(tmp_path))
+        check_close(A - F0, B - F0, what=f"RXCHF OMG1 {w} F")
+        check_close(SA, SB, what=f"RXCHF OMG1 {w} S")
+        assert eo[0] == pytest.approx(er[0], rel=1e-12) and eo[1] == pytest.approx(er[1], rel=1e-12)
