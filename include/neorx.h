@@ -309,6 +309,41 @@ int neorx_rxchf_fp_omg1(int64_t nebf, int64_t npbf, const double *dbe, const dou
                         const double *gam1s, double *fp, double *sp, double *e, double *s);
 int neorx_rxchf_fbe_omg1(int64_t nebf, int64_t npbf, const double *dbe, const double *dp, const double *gam1,
                          const double *gam1s, double *fbe, double *sbe, double *e, double *s);
+/* RXCUHF (open-shell regular electrons): RXCUHF_FE_OMG2 / _FP_OMG2 /
+ * _FBE_OMG2, source/rxchf/rxcuhf_omg2.f:2, :253, :469, over the RXCHF_GAM2_IC1
+ * tensors; F, FAbetE and the S matrices accumulate, e and s overwritten.  FE
+ * takes DAbetE as the reference does and does not read it. */
+int neorx_rxcuhf_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet, const double *datot,
+                         const double *dbe, const double *dp, const double *gm2_1, const double *gm2_2,
+                         const double *gm2s, double *faalp, double *fabet, double *saalp, double *e, double *s);
+int neorx_rxcuhf_fp_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,
+                         const double *dp, const double *gm2_1, const double *gm2_2, const double *gm2s, double *fp,
+                         double *sp, double *e, double *s);
+int neorx_rxcuhf_fbe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot, const double *dbe,
+                          const double *dp, const double *gm2_1, const double *gm2_2, const double *gm2s, double *fbe,
+                          double *sbe, double *e, double *s);
+/* device arrays: which 0 FE (d_f = FAalpE, d_f2 = FAbetE, d_sf = SAalpE),
+ * 1 FP, 2 FBE; d_es[0] = E, d_es[1] = S */
+int neorx_rxcuhf_omg2_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_daalp, const double *d_datot,
+                          const double *d_dbe, const double *d_dp, const double *d_gm2_1, const double *d_gm2_2,
+                          const double *d_gm2s, double *d_f, double *d_f2, double *d_sf, double *d_es, void *stream);
+/* RXCUHF_FE_OMG3 / _FP_OMG3 / _FBE_OMG3, source/rxchf/rxcuhf_omg3.f:2, :270,
+ * :499, over the RXCHF_GAM3_IC1 tensors: F (and FAbetE) accumulate, e
+ * overwritten */
+int neorx_rxcuhf_fe_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet, const double *datot,
+                         const double *dbe, const double *dp, const double *gm3_1, const double *gm3_2, double *faalp,
+                         double *fabet, double *e);
+int neorx_rxcuhf_fp_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet, const double *datot,
+                         const double *dbe, const double *dp, const double *gm3_1, const double *gm3_2, double *fp,
+                         double *e);
+int neorx_rxcuhf_fbe_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet, const double *datot,
+                          const double *dbe, const double *dp, const double *gm3_1, const double *gm3_2, double *fbe,
+                          double *e);
+/* device arrays: which 0 FE (d_f = FAalpE, d_f2 = FAbetE), 1 FP, 2 FBE;
+ * stream-ordered scratch (3 npbf^2 nebf^4 doubles); not synchronised */
+int neorx_rxcuhf_omg3_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_daalp, const double *d_dabet,
+                          const double *d_datot, const double *d_dbe, const double *d_dp, const double *d_gm3_1,
+                          const double *d_gm3_2, double *d_f, double *d_f2, double *d_e, void *stream);
 /* the same on device arrays: which 1 FP, 2 FBE; d_es[0] = E, d_es[1] = S */
 int neorx_rxchf_omg1_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_dbe, const double *d_dp,
                          const double *d_gam1, const double *d_gam1s, double *d_f, double *d_sf, double *d_es,

@@ -3378,7 +3378,8 @@ __global__ void __launch_bounds__(256) k_rxchf_cols(i64 ne, i64 np, const double
     cols[2 * t + 1] = cs;
 }
 
-// which: 0 FAE/SAE, 1 FP/SP, 2 FBE/SBE; F, SF accumulate; es[0] = E, es[1] = S
+// which: 0 FAE/SAE, 1 FP/SP, 2 FBE/SBE, 3 the energies only; F, SF
+// accumulate; es[0] = E, es[1] = S
 __global__ void __launch_bounds__(64) k_rxchf_finish(int which, i64 ne, i64 np, const double *__restrict__ DBE,
                                                      const double *__restrict__ DP, const double *__restrict__ rows,
                                                      const double *__restrict__ cols, double *__restrict__ F,
@@ -3409,7 +3410,7 @@ __global__ void __launch_bounds__(64) k_rxchf_finish(int which, i64 ne, i64 np, 
             F[jp * np + ip] += f;
             SF[jp * np + ip] += sf;
         }
-    } else {
+    } else if (which == 2) {
         for (i64 r = threadIdx.x; r < ne2; r += 64) {
             double f = 0.0, sf = 0.0;
             for (i64 pq = 0; pq < np2; ++pq) {
@@ -3666,7 +3667,8 @@ __global__ void __launch_bounds__(64) k_rxchf3_rows(i64 ncol, const double *__re
 }
 
 // which 1: FP(pq) += sum_r DBE(r) R[pq][r]; which 2: FBE(r) += sum_pq DP(pq)
-// R[pq][r]; E = sum_q Ev[q] for which 0, else sum_pq DP sum_r DBE R
+// R[pq][r]; E = sum_q Ev[q] for which 0, else (1, 2, and 3: the energy
+// only) sum_pq DP sum_r DBE R
 __global__ void __launch_bounds__(64) k_rxchf3_fin(int which, i64 ne, i64 np, const double *__restrict__ DBE,
                                                    const double *__restrict__ DP, const double *__restrict__ R,
                                                    const double *__restrict__ Ev, double *__restrict__ F,
@@ -3867,6 +3869,310 @@ extern "C" int neorx_rxchf_fbe_omg1(int64_t nebf, int64_t npbf, const double *db
                                     double *s)
 {
     return rxchf_omg1_host(2, nebf, npbf, dbe, dp, gam1, gam1s, fbe, sbe, e, s);
+}
+
+// ------------------------------------------------------------------
+// RXCUHF: open-shell regular electrons (rxchf/rxcuhf_omg2.f, rxcuhf_omg3.f)
+// ------------------------------------------------------------------
+// RXCUHF_FE/FP/FBE_OMG2 (rxcuhf_omg2.f:2-694).  Per entry of block pq:
+// val1 = GM2_1(i1 j1 i2 j2), val2 = GM2_2(i1 j2 i2 j1), Sval = GM2s(i1 j2 i2 j1)
+// (:219-243, :438-454, :664-680).  Rows (pq, i1, j1), in k_rxchf_finish's
+// layout: r = sum_c DAtot(c) val1 - DAalp(c) val2, rs = -sum_c DAalp(c) Sval;
+// columns (pq, i2, j2): DBE-weighted sums of val1 - val2, Sval and val1.
+__global__ void __launch_bounds__(64) k_rxcuhf2_rows(i64 ne, i64 np, const double *__restrict__ Dal,
+                                                     const double *__restrict__ Dto, const double *__restrict__ G1,
+                                                     const double *__restrict__ G2, const double *__restrict__ GS,
+                                                     double *__restrict__ rows)
+{
+    const i64 ne2 = ne * ne, row = blockIdx.x; // (pq, i1, j1)
+    if (row >= np * np * ne2) return;
+    const i64 pq = row / ne2, i1 = (row / ne) % ne, j1 = row % ne;
+    const double *g1 = G1 + row * ne2, *g2 = G2 + pq * ne2 * ne2, *gs = GS + pq * ne2 * ne2;
+    double r1 = 0.0, rs = 0.0;
+    for (i64 c = threadIdx.x; c < ne2; c += 64) {
+        const i64 i2 = c / ne, j2 = c % ne;
+        const i64 x21 = ((i1 * ne + j2) * ne + i2) * ne + j1;
+        const double dal = Dal[j2 * ne + i2];
+        r1 += Dto[j2 * ne + i2] * g1[c] - dal * g2[x21];
+        rs += -dal * gs[x21];
+    }
+    r1 = wave_sum(r1);
+    rs = wave_sum(rs);
+    if (threadIdx.x == 0) {
+        rows[2 * row] = r1;
+        rows[2 * row + 1] = rs;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rxcuhf2_cols(i64 ne, i64 np, const double *__restrict__ DBE,
+                                                      const double *__restrict__ G1, const double *__restrict__ G2,
+                                                      const double *__restrict__ GS, double *__restrict__ cols)
+{
+    const i64 ne2 = ne * ne;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; // (pq, i2, j2)
+    if (t >= np * np * ne2) return;
+    const i64 pq = t / ne2, c = t % ne2, i2 = c / ne, j2 = c % ne;
+    const double *g1 = G1 + pq * ne2 * ne2, *g2 = G2 + pq * ne2 * ne2, *gs = GS + pq * ne2 * ne2;
+    double ca = 0.0, cs = 0.0, cb = 0.0;
+    for (i64 i1 = 0; i1 < ne; ++i1)
+        for (i64 j1 = 0; j1 < ne; ++j1) {
+            const i64 x12 = (i1 * ne + j1) * ne2 + c, x21 = ((i1 * ne + j2) * ne + i2) * ne + j1;
+            const double dbe = DBE[j1 * ne + i1];
+            ca += dbe * (g1[x12] - g2[x21]);
+            cs += dbe * gs[x21];
+            cb += dbe * g1[x12];
+        }
+    cols[3 * t] = ca;
+    cols[3 * t + 1] = cs;
+    cols[3 * t + 2] = cb;
+}
+
+// FE: FAalp(i2,j2) += sum_pq DP ca, SAalp += sum_pq DP cs, FAbet += sum_pq DP cb
+__global__ void __launch_bounds__(64) k_rxcuhf2_fe(i64 ne, i64 np, const double *__restrict__ DP,
+                                                   const double *__restrict__ cols, double *__restrict__ Fa,
+                                                   double *__restrict__ Fb, double *__restrict__ Sa)
+{
+    const i64 ne2 = ne * ne, np2 = np * np;
+    for (i64 c = threadIdx.x; c < ne2; c += 64) {
+        double a = 0.0, sa = 0.0, b = 0.0;
+        for (i64 pq = 0; pq < np2; ++pq) {
+            const double dp = DP[(pq % np) * np + pq / np];
+            a += dp * cols[3 * (pq * ne2 + c)];
+            sa += dp * cols[3 * (pq * ne2 + c) + 1];
+            b += dp * cols[3 * (pq * ne2 + c) + 2];
+        }
+        const i64 at = (c % ne) * ne + c / ne;
+        Fa[at] += a;
+        Sa[at] += sa;
+        Fb[at] += b;
+    }
+}
+
+extern "C" int neorx_rxcuhf_omg2_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_daalp,
+                                     const double *d_datot, const double *d_dbe, const double *d_dp,
+                                     const double *d_gm2_1, const double *d_gm2_2, const double *d_gm2s, double *d_f,
+                                     double *d_f2, double *d_sf, double *d_es, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf;
+    if (ne <= 0 || np <= 0 || which < 0 || which > 2) return fail(NEORX_E_ARG, "RXCUHF OMG2: bad arguments");
+    if (!d_daalp || !d_datot || !d_dbe || !d_dp || !d_gm2_1 || !d_gm2_2 || !d_gm2s || !d_f || !d_sf || !d_es ||
+        (which == 0 && !d_f2))
+        return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const i64 nrow = np * np * ne * ne;
+    StreamBuf w;
+    rc = w.alloc((size_t)(5 * nrow) * 8, st);
+    if (rc) return rc;
+    double *rows = (double *)w.p, *cols = rows + 2 * nrow;
+    hipLaunchKernelGGL(k_rxcuhf2_rows, dim3((unsigned)nrow), dim3(64), 0, st, ne, np, d_daalp, d_datot, d_gm2_1,
+                       d_gm2_2, d_gm2s, rows);
+    if (which == 0) {
+        hipLaunchKernelGGL(k_rxcuhf2_cols, dim3(nblocks(nrow, 256)), dim3(256), 0, st, ne, np, d_dbe, d_gm2_1, d_gm2_2,
+                           d_gm2s, cols);
+        hipLaunchKernelGGL(k_rxcuhf2_fe, dim3(1), dim3(64), 0, st, ne, np, d_dp, cols, d_f, d_f2, d_sf);
+    }
+    hipLaunchKernelGGL(k_rxchf_finish, dim3(1), dim3(64), 0, st, which == 0 ? 3 : (int)which, ne, np, d_dbe, d_dp,
+                       rows, cols, d_f, d_sf, d_es);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int rxcuhf_omg2_host(int which, i64 ne, i64 np, const double *dal, const double *dto, const double *dbe,
+                            const double *dp, const double *g1, const double *g2, const double *gs, double *f,
+                            double *f2, double *sf, double *e, double *sv)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    const i64 ng2 = ne * ne * ne * ne * np * np, nf = which == 1 ? np * np : ne * ne;
+    DevBuf dL, dT, dB, dP, d1, d2, dS, dF, dF2, dSF, dE;
+    rc = h2d(dL, dal, ne * ne);
+    if (!rc) rc = h2d(dT, dto, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(d1, g1, ng2);
+    if (!rc) rc = h2d(d2, g2, ng2);
+    if (!rc) rc = h2d(dS, gs, ng2);
+    if (!rc) rc = dF.alloc(nf * 8);
+    if (!rc) rc = dF2.alloc(nf * 8);
+    if (!rc) rc = dSF.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(16);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dF.p, f, nf * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dSF.p, sf, nf * 8, hipMemcpyHostToDevice));
+    if (which == 0) HIPCHK(hipMemcpy(dF2.p, f2, nf * 8, hipMemcpyHostToDevice));
+    rc = neorx_rxcuhf_omg2_dev(which, ne, np, dL.d(), dT.d(), dB.d(), dP.d(), d1.d(), d2.d(), dS.d(), dF.d(), dF2.d(),
+                               dSF.d(), dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    double es[2];
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sf, dSF.p, nf * 8, hipMemcpyDeviceToHost));
+    if (which == 0) HIPCHK(hipMemcpy(f2, dF2.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(es, dE.p, 16, hipMemcpyDeviceToHost));
+    *e = es[0];
+    *sv = es[1];
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxcuhf_fe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet,
+                                    const double *datot, const double *dbe, const double *dp, const double *gm2_1,
+                                    const double *gm2_2, const double *gm2s, double *faalp, double *fabet,
+                                    double *saalp, double *e, double *s)
+{
+    (void)dabet; // RXCUHF_FE_OMG2 takes DAbetE and does not read it
+    if (!fabet) return fail(NEORX_E_ARG, "null argument");
+    return rxcuhf_omg2_host(0, nebf, npbf, daalp, datot, dbe, dp, gm2_1, gm2_2, gm2s, faalp, fabet, saalp, e, s);
+}
+extern "C" int neorx_rxcuhf_fp_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
+                                    const double *dbe, const double *dp, const double *gm2_1, const double *gm2_2,
+                                    const double *gm2s, double *fp, double *sp, double *e, double *s)
+{
+    return rxcuhf_omg2_host(1, nebf, npbf, daalp, datot, dbe, dp, gm2_1, gm2_2, gm2s, fp, nullptr, sp, e, s);
+}
+extern "C" int neorx_rxcuhf_fbe_omg2(int64_t nebf, int64_t npbf, const double *daalp, const double *datot,
+                                     const double *dbe, const double *dp, const double *gm2_1, const double *gm2_2,
+                                     const double *gm2s, double *fbe, double *sbe, double *e, double *s)
+{
+    return rxcuhf_omg2_host(2, nebf, npbf, daalp, datot, dbe, dp, gm2_1, gm2_2, gm2s, fbe, nullptr, sbe, e, s);
+}
+
+// RXCUHF_FE/FP/FBE_OMG3 (rxcuhf_omg3.f:2-728).  Thread per (pq, i1, j1, i2, j2);
+// over (i3, j3), with val1a = GM3_1(i1 j1 i2 j2 i3 j3), val1b = GM3_1(i1 j1 i2 j3
+// i3 j2), g213 = GM3_2(i1 j2 i2 j1 i3 j3), g312 = GM3_2(i1 j3 i2 j1 i3 j2) and the
+// FE-only pair averages val2a..val2d (:233-251):
+//   fa = DAtot3 val1a - DAalp3 val1b - DAtot3 val2a + DAalp3 val2b
+//        - DAalp3 val2c + DAalp3 val2d                         (FAalpE, :240-246)
+//   fb = DAtot3 val1a - DAbet3 val1b - DAalp3 val2c          (FAbetE, :247-250)
+//   w  = DAtot2 DAtot3 val1a / 2 - (DAalp2 DAalp3 + DAbet2 DAbet3) val1b / 2
+//        - DAalp2 DAtot3 g213 + DAalp2 DAalp3 g312            (FP, FBE, E: :476-489)
+// stores Pa = DBE DP fa, Pb = DBE DP fb (sums over (i3,j3)) and W = w.
+__global__ void __launch_bounds__(256) k_rxcuhf3_terms(int fe, i64 ne, i64 np, const double *__restrict__ Dal,
+                                                       const double *__restrict__ Dbt, const double *__restrict__ Dto,
+                                                       const double *__restrict__ DBE, const double *__restrict__ DP,
+                                                       const double *__restrict__ G1, const double *__restrict__ G2,
+                                                       double *__restrict__ Pa, double *__restrict__ Pb,
+                                                       double *__restrict__ W)
+{
+    const i64 ne2 = ne * ne, ne4 = ne2 * ne2;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np * np * ne4) return;
+    const i64 pq = t / ne4, r = t % ne4, i1 = r / (ne2 * ne), j1 = (r / ne2) % ne, i2 = (r / ne) % ne, j2 = r % ne;
+    const double *g1 = G1 + pq * ne4 * ne2, *g2 = G2 + pq * ne4 * ne2;
+    auto X = [ne](i64 a1, i64 b1, i64 a2, i64 b2, i64 a3, i64 b3) {
+        return ((((a1 * ne + b1) * ne + a2) * ne + b2) * ne + a3) * ne + b3;
+    };
+    const double al2 = Dal[j2 * ne + i2], bt2 = Dbt[j2 * ne + i2], to2 = Dto[j2 * ne + i2];
+    double fa = 0.0, fb = 0.0, w = 0.0;
+    for (i64 i3 = 0; i3 < ne; ++i3)
+        for (i64 j3 = 0; j3 < ne; ++j3) {
+            const double al3 = Dal[j3 * ne + i3], bt3 = Dbt[j3 * ne + i3], to3 = Dto[j3 * ne + i3];
+            const double val1a = g1[X(i1, j1, i2, j2, i3, j3)], val1b = g1[X(i1, j1, i2, j3, i3, j2)];
+            const double g213 = g2[X(i1, j2, i2, j1, i3, j3)], g312 = g2[X(i1, j3, i2, j1, i3, j2)];
+            w += 0.5 * to2 * to3 * val1a - 0.5 * (al2 * al3 + bt2 * bt3) * val1b - al2 * to3 * g213 +
+                 al2 * al3 * g312;
+            if (fe) {
+                const double val2a = (g213 + g2[X(i1, i2, j2, j1, i3, j3)]) * 0.5;
+                const double val2b = (g312 + g2[X(i1, j3, j2, j1, i3, i2)]) * 0.5;
+                const double val2c = (g2[X(i1, j3, i3, j1, i2, j2)] + g2[X(i1, j3, i3, j1, j2, i2)]) * 0.5;
+                const double val2d = (g2[X(i1, j2, i3, j1, i2, j3)] + g2[X(i1, i2, i3, j1, j2, j3)]) * 0.5;
+                fa += to3 * val1a - al3 * val1b - to3 * val2a + al3 * val2b - al3 * val2c + al3 * val2d;
+                fb += to3 * val1a - bt3 * val1b - al3 * val2c;
+            }
+        }
+    const double f = DBE[j1 * ne + i1] * DP[(pq % np) * np + pq / np];
+    Pa[t] = f * fa;
+    Pb[t] = f * fb;
+    W[t] = w;
+}
+
+extern "C" int neorx_rxcuhf_omg3_dev(int64_t which, int64_t nebf, int64_t npbf, const double *d_daalp,
+                                     const double *d_dabet, const double *d_datot, const double *d_dbe,
+                                     const double *d_dp, const double *d_gm3_1, const double *d_gm3_2, double *d_f,
+                                     double *d_f2, double *d_e, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    const i64 ne = nebf, np = npbf, ne2 = ne * ne;
+    if (ne <= 0 || np <= 0 || which < 0 || which > 2) return fail(NEORX_E_ARG, "RXCUHF OMG3: bad arguments");
+    if (!d_daalp || !d_dabet || !d_datot || !d_dbe || !d_dp || !d_gm3_1 || !d_gm3_2 || !d_f || !d_e ||
+        (which == 0 && !d_f2))
+        return fail(NEORX_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const i64 nq = np * np * ne2, n5 = nq * ne2;
+    StreamBuf w;
+    rc = w.alloc((size_t)(3 * n5 + 2 * nq + WCS_GROUPS * ne2) * 8, st);
+    if (rc) return rc;
+    double *Pa = (double *)w.p, *Pb = Pa + n5, *W = Pb + n5, *R = W + n5, *Ev = R + nq, *o1 = Ev + nq;
+    hipLaunchKernelGGL(k_rxcuhf3_terms, dim3(nblocks(n5, 256)), dim3(256), 0, st, (int)(which == 0), ne, np, d_daalp,
+                       d_dabet, d_datot, d_dbe, d_dp, d_gm3_1, d_gm3_2, Pa, Pb, W);
+    hipLaunchKernelGGL(k_rxchf3_rows, dim3((unsigned)nq), dim3(64), 0, st, ne2, W, W, R, Ev);
+    if (which == 0) {
+        const dim3 g1((unsigned)((ne2 + 63) / 64), WCS_GROUPS);
+        hipLaunchKernelGGL(k_wcolsum1, g1, dim3(256), 0, st, nq, ne2, (i64)1, np, (const double *)nullptr, Pa, o1);
+        hipLaunchKernelGGL(k_wcolsum2, dim3(nblocks(ne2, 256)), dim3(256), 0, st, ne, o1, d_f);
+        hipLaunchKernelGGL(k_wcolsum1, g1, dim3(256), 0, st, nq, ne2, (i64)1, np, (const double *)nullptr, Pb, o1);
+        hipLaunchKernelGGL(k_wcolsum2, dim3(nblocks(ne2, 256)), dim3(256), 0, st, ne, o1, d_f2);
+    }
+    // which 3: the energy from the rows only (FE's energy is FP's)
+    hipLaunchKernelGGL(k_rxchf3_fin, dim3(1), dim3(64), 0, st, which == 0 ? 3 : (int)which, ne, np, d_dbe, d_dp, R, Ev,
+                       d_f, d_e);
+    HIPCHK(hipGetLastError());
+    return NEORX_OK;
+}
+
+static int rxcuhf_omg3_host(int which, i64 ne, i64 np, const double *dal, const double *dbt, const double *dto,
+                            const double *dbe, const double *dp, const double *g1, const double *g2, double *f,
+                            double *f2, double *e)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (ne <= 0 || np <= 0) return fail(NEORX_E_ARG, "bad sizes");
+    const i64 ng3 = ne * ne * ne * ne * ne * ne * np * np, nf = which == 1 ? np * np : ne * ne;
+    DevBuf dL, dBt, dT, dB, dP, d1, d2, dF, dF2, dE;
+    rc = h2d(dL, dal, ne * ne);
+    if (!rc) rc = h2d(dBt, dbt, ne * ne);
+    if (!rc) rc = h2d(dT, dto, ne * ne);
+    if (!rc) rc = h2d(dB, dbe, ne * ne);
+    if (!rc) rc = h2d(dP, dp, np * np);
+    if (!rc) rc = h2d(d1, g1, ng3);
+    if (!rc) rc = h2d(d2, g2, ng3);
+    if (!rc) rc = dF.alloc(nf * 8);
+    if (!rc) rc = dF2.alloc(nf * 8);
+    if (!rc) rc = dE.alloc(8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dF.p, f, nf * 8, hipMemcpyHostToDevice));
+    if (which == 0) HIPCHK(hipMemcpy(dF2.p, f2, nf * 8, hipMemcpyHostToDevice));
+    rc = neorx_rxcuhf_omg3_dev(which, ne, np, dL.d(), dBt.d(), dT.d(), dB.d(), dP.d(), d1.d(), d2.d(), dF.d(), dF2.d(),
+                               dE.d(), 0);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(f, dF.p, nf * 8, hipMemcpyDeviceToHost));
+    if (which == 0) HIPCHK(hipMemcpy(f2, dF2.p, nf * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(e, dE.p, 8, hipMemcpyDeviceToHost));
+    return NEORX_OK;
+}
+
+extern "C" int neorx_rxcuhf_fe_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet,
+                                    const double *datot, const double *dbe, const double *dp, const double *gm3_1,
+                                    const double *gm3_2, double *faalp, double *fabet, double *e)
+{
+    if (!fabet) return fail(NEORX_E_ARG, "null argument");
+    return rxcuhf_omg3_host(0, nebf, npbf, daalp, dabet, datot, dbe, dp, gm3_1, gm3_2, faalp, fabet, e);
+}
+extern "C" int neorx_rxcuhf_fp_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet,
+                                    const double *datot, const double *dbe, const double *dp, const double *gm3_1,
+                                    const double *gm3_2, double *fp, double *e)
+{
+    return rxcuhf_omg3_host(1, nebf, npbf, daalp, dabet, datot, dbe, dp, gm3_1, gm3_2, fp, nullptr, e);
+}
+extern "C" int neorx_rxcuhf_fbe_omg3(int64_t nebf, int64_t npbf, const double *daalp, const double *dabet,
+                                     const double *datot, const double *dbe, const double *dp, const double *gm3_1,
+                                     const double *gm3_2, double *fbe, double *e)
+{
+    return rxcuhf_omg3_host(2, nebf, npbf, daalp, dabet, datot, dbe, dp, gm3_1, gm3_2, fbe, nullptr, e);
 }
 
 // ------------------------------------------------------------------

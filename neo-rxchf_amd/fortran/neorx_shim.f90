@@ -208,6 +208,44 @@ module neorx_capi
       real(c_double) :: db(*), dp(*), g(*), gs(*), f(*), sf(*), e, sv
       integer(c_int) :: neorx_rxchf_fbe_omg1
     end function
+    function neorx_rxcuhf_fe_omg2(ne, np, dl, dbt, dt, db, dp, g1, g2, gs, fa, fb, sa, e, sv) &
+        bind(C, name="neorx_rxcuhf_fe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dbt(*), dt(*), db(*), dp(*), g1(*), g2(*), gs(*), fa(*), fb(*), sa(*), e, sv
+      integer(c_int) :: neorx_rxcuhf_fe_omg2
+    end function
+    function neorx_rxcuhf_fp_omg2(ne, np, dl, dt, db, dp, g1, g2, gs, f, sf, e, sv) bind(C, name="neorx_rxcuhf_fp_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dt(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxcuhf_fp_omg2
+    end function
+    function neorx_rxcuhf_fbe_omg2(ne, np, dl, dt, db, dp, g1, g2, gs, f, sf, e, sv) &
+        bind(C, name="neorx_rxcuhf_fbe_omg2")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dt(*), db(*), dp(*), g1(*), g2(*), gs(*), f(*), sf(*), e, sv
+      integer(c_int) :: neorx_rxcuhf_fbe_omg2
+    end function
+    function neorx_rxcuhf_fe_omg3(ne, np, dl, dbt, dt, db, dp, g1, g2, fa, fb, e) bind(C, name="neorx_rxcuhf_fe_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dbt(*), dt(*), db(*), dp(*), g1(*), g2(*), fa(*), fb(*), e
+      integer(c_int) :: neorx_rxcuhf_fe_omg3
+    end function
+    function neorx_rxcuhf_fp_omg3(ne, np, dl, dbt, dt, db, dp, g1, g2, f, e) bind(C, name="neorx_rxcuhf_fp_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dbt(*), dt(*), db(*), dp(*), g1(*), g2(*), f(*), e
+      integer(c_int) :: neorx_rxcuhf_fp_omg3
+    end function
+    function neorx_rxcuhf_fbe_omg3(ne, np, dl, dbt, dt, db, dp, g1, g2, f, e) bind(C, name="neorx_rxcuhf_fbe_omg3")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: ne, np
+      real(c_double) :: dl(*), dbt(*), dt(*), db(*), dp(*), g1(*), g2(*), f(*), e
+      integer(c_int) :: neorx_rxcuhf_fbe_omg3
+    end function
     function neorx_rxchfne_gam2_ic1(h, g, gx) bind(C, name="neorx_rxchfne_gam2_ic1")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: h
@@ -906,6 +944,78 @@ subroutine RXCHF_FBE_OMG1(nebf, npbf, ng1, DBE, DP, FBE, SBE, E_BE_OMG1, S_BE_OM
   call neorx_check(neorx_rxchf_fbe_omg1(nebf, npbf, DBE, DP, g, gs, FBE, SBE, E_BE_OMG1, S_BE_OMG1), &
                    'RXCHF_FBE_OMG1')
   deallocate(g, gs)
+end subroutine
+
+! RXCUHF_FE_OMG2, source/rxchf/rxcuhf_omg2.f:2-126
+subroutine RXCUHF_FE_OMG2(Nchunks, nebf, npbf, ng2, DAalpE, DAbetE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, &
+                          FAalpE, FAbetE, SAalpE, E_AE_OMG2, S_AE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2
+  real(c_double) :: DAalpE(nebf, nebf), DAbetE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2)
+  real(c_double) :: FAalpE(nebf, nebf), FAbetE(nebf, nebf), SAalpE(nebf, nebf), E_AE_OMG2, S_AE_OMG2
+  call neorx_check(neorx_rxcuhf_fe_omg2(nebf, npbf, DAalpE, DAbetE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, &
+                   FAalpE, FAbetE, SAalpE, E_AE_OMG2, S_AE_OMG2), 'RXCUHF_FE_OMG2')
+end subroutine
+
+! RXCUHF_FP_OMG2, source/rxchf/rxcuhf_omg2.f:253-352
+subroutine RXCUHF_FP_OMG2(Nchunks, nebf, npbf, ng2, NAalpE, NBE, DAalpE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, &
+                          GM2sICR, FP, SP, E_P_OMG2, S_P_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAalpE, NBE
+  real(c_double) :: DAalpE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2), FP(npbf, npbf), SP(npbf, npbf), E_P_OMG2, S_P_OMG2
+  call neorx_check(neorx_rxcuhf_fp_omg2(nebf, npbf, DAalpE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FP, SP, &
+                   E_P_OMG2, S_P_OMG2), 'RXCUHF_FP_OMG2')
+end subroutine
+
+! RXCUHF_FBE_OMG2, source/rxchf/rxcuhf_omg2.f:469-576
+subroutine RXCUHF_FBE_OMG2(Nchunks, nebf, npbf, ng2, NAalpE, NBE, DAalpE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, &
+                           GM2sICR, FBE, SBE, E_BE_OMG2, S_BE_OMG2)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng2, NAalpE, NBE
+  real(c_double) :: DAalpE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM2_1ICR(ng2), GM2_2ICR(ng2), GM2sICR(ng2), FBE(nebf, nebf), SBE(nebf, nebf), E_BE_OMG2, S_BE_OMG2
+  call neorx_check(neorx_rxcuhf_fbe_omg2(nebf, npbf, DAalpE, DAtotE, DBE, DP, GM2_1ICR, GM2_2ICR, GM2sICR, FBE, SBE, &
+                   E_BE_OMG2, S_BE_OMG2), 'RXCUHF_FBE_OMG2')
+end subroutine
+
+! RXCUHF_FE_OMG3, source/rxchf/rxcuhf_omg3.f:2-107
+subroutine RXCUHF_FE_OMG3(Nchunks, nebf, npbf, ng3, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FAalpE, &
+                          FAbetE, E_AE_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAalpE(nebf, nebf), DAbetE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM3_1ICR(ng3), GM3_2ICR(ng3), FAalpE(nebf, nebf), FAbetE(nebf, nebf), E_AE_OMG3
+  call neorx_check(neorx_rxcuhf_fe_omg3(nebf, npbf, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FAalpE, &
+                   FAbetE, E_AE_OMG3), 'RXCUHF_FE_OMG3')
+end subroutine
+
+! RXCUHF_FP_OMG3, source/rxchf/rxcuhf_omg3.f:270-371
+subroutine RXCUHF_FP_OMG3(Nchunks, nebf, npbf, ng3, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FP, E_P_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAalpE(nebf, nebf), DAbetE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM3_1ICR(ng3), GM3_2ICR(ng3), FP(npbf, npbf), E_P_OMG3
+  call neorx_check(neorx_rxcuhf_fp_omg3(nebf, npbf, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FP, &
+                   E_P_OMG3), 'RXCUHF_FP_OMG3')
+end subroutine
+
+! RXCUHF_FBE_OMG3, source/rxchf/rxcuhf_omg3.f:499-600
+subroutine RXCUHF_FBE_OMG3(Nchunks, nebf, npbf, ng3, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FBE, &
+                           E_BE_OMG3)
+  use neorx_capi
+  implicit none
+  integer(c_int64_t) :: Nchunks, nebf, npbf, ng3
+  real(c_double) :: DAalpE(nebf, nebf), DAbetE(nebf, nebf), DAtotE(nebf, nebf), DBE(nebf, nebf), DP(npbf, npbf)
+  real(c_double) :: GM3_1ICR(ng3), GM3_2ICR(ng3), FBE(nebf, nebf), E_BE_OMG3
+  call neorx_check(neorx_rxcuhf_fbe_omg3(nebf, npbf, DAalpE, DAbetE, DAtotE, DBE, DP, GM3_1ICR, GM3_2ICR, FBE, &
+                   E_BE_OMG3), 'RXCUHF_FBE_OMG3')
 end subroutine
 
 ! RXCHFne_FAE_OMG2, source/rxchfne/rxchfne_omg2.f:2-116

@@ -59,6 +59,12 @@ def load():
         _sig(nm, [_I64, _I64] + [_P] * 7)
     for nm in ("neorx_rxchf_fp_omg1", "neorx_rxchf_fbe_omg1"):
         _sig(nm, [_I64, _I64] + [_P] * 8)
+    _sig("neorx_rxcuhf_fe_omg2", [_I64, _I64] + [_P] * 13)
+    for nm in ("neorx_rxcuhf_fp_omg2", "neorx_rxcuhf_fbe_omg2"):
+        _sig(nm, [_I64, _I64] + [_P] * 11)
+    _sig("neorx_rxcuhf_fe_omg3", [_I64, _I64] + [_P] * 10)
+    for nm in ("neorx_rxcuhf_fp_omg3", "neorx_rxcuhf_fbe_omg3"):
+        _sig(nm, [_I64, _I64] + [_P] * 9)
     _sig("neorx_transfer_bytes", [_P, _P])
     _sig("neorx_system_create", [_I64, _I64, _I64, _I64, _I64, _D] + [_P] * 15 + [_P])
     _sig("neorx_system_destroy", [_I64])
@@ -396,6 +402,39 @@ def RXCHF_OMG1(which, nebf, npbf, DBE, DP, G1, G1s, F, SF):
     _check(fn(nebf, npbf, _p(_fort(DBE)), _p(_fort(DP)), _p(_np.ascontiguousarray(G1)),
               _p(_np.ascontiguousarray(G1s)), _p(F), _p(SF), _p(e), _p(sv)))
     return float(e[0]), float(sv[0])
+
+
+def RXCUHF_OMG2(which, nebf, npbf, DAalp, DAbet, DAtot, DBE, DP, G1, G2, Gs, F, SF, F2=None):
+    """RXCUHF_FE_OMG2 / _FP_OMG2 / _FBE_OMG2 (source/rxchf/rxcuhf_omg2.f:2,
+    :253, :469) for which = 0, 1, 2 (FE: F = FAalpE, F2 = FAbetE, SF =
+    SAalpE); F, F2, SF accumulate in place; returns (E, S)."""
+    load()
+    e, sv = _np.zeros(1), _np.zeros(1)
+    g = [_p(_np.ascontiguousarray(x)) for x in (G1, G2, Gs)]
+    if which == 0:
+        _check(_lib.neorx_rxcuhf_fe_omg2(nebf, npbf, _p(_fort(DAalp)), _p(_fort(DAbet)), _p(_fort(DAtot)),
+                                         _p(_fort(DBE)), _p(_fort(DP)), *g, _p(F), _p(F2), _p(SF), _p(e), _p(sv)))
+    else:
+        fn = _lib.neorx_rxcuhf_fp_omg2 if which == 1 else _lib.neorx_rxcuhf_fbe_omg2
+        _check(fn(nebf, npbf, _p(_fort(DAalp)), _p(_fort(DAtot)), _p(_fort(DBE)), _p(_fort(DP)), *g, _p(F), _p(SF),
+                  _p(e), _p(sv)))
+    return float(e[0]), float(sv[0])
+
+
+def RXCUHF_OMG3(which, nebf, npbf, DAalp, DAbet, DAtot, DBE, DP, G1, G2, F, F2=None):
+    """RXCUHF_FE_OMG3 / _FP_OMG3 / _FBE_OMG3 (source/rxchf/rxcuhf_omg3.f:2,
+    :270, :499) for which = 0, 1, 2 (FE: F = FAalpE, F2 = FAbetE); F and F2
+    accumulate in place; returns E."""
+    load()
+    e = _np.zeros(1)
+    a = [_p(_fort(x)) for x in (DAalp, DAbet, DAtot, DBE, DP)] + [_p(_np.ascontiguousarray(G1)),
+                                                                  _p(_np.ascontiguousarray(G2))]
+    if which == 0:
+        _check(_lib.neorx_rxcuhf_fe_omg3(nebf, npbf, *a, _p(F), _p(F2), _p(e)))
+    else:
+        fn = _lib.neorx_rxcuhf_fp_omg3 if which == 1 else _lib.neorx_rxcuhf_fbe_omg3
+        _check(fn(nebf, npbf, *a, _p(F), _p(e)))
+    return float(e[0])
 
 
 def transfer_bytes():

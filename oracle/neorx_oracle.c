@@ -3142,3 +3142,133 @@ void orc_rxchf_omg1(i64 which, i64 nebf, i64 npbf, const double *DBE, const doub
 #undef RXA
 #undef RXB
 #undef RXP
+
+/* RXCUHF (open-shell regular electrons, one special electron) */
+#define RXL(i, j) Dal[(j) * ne + (i)]
+#define RXBt(i, j) Dbt[(j) * ne + (i)]
+#define RXT(i, j) Dto[(j) * ne + (i)]
+#define RXB(i, j) DBE[(j) * ne + (i)]
+#define RXP(i, j) DP[(j) * np + (i)]
+/* RXCUHF_FE_OMG2 / _FP_OMG2 / _FBE_OMG2 (rxchf/rxcuhf_omg2.f:2-694); which =
+ * 0, 1, 2.  Per entry (:219-243, :438-454, :664-680): val1 = GM2_1(i1 j1 i2 j2),
+ * val2 = GM2_2(i1 j2 i2 j1), Sval = GM2s(i1 j2 i2 j1);
+ *   FE:  FAalp(i2,j2) += DP DBE (val1 - val2), SAalp += DP DBE Sval,
+ *        FAbet(i2,j2) += DP DBE val1;
+ *   u = DAtot(i2,j2) val1 - DAalp(i2,j2) val2, us = -DAalp(i2,j2) Sval:
+ *   FP(ip,jp) += DBE u, SP += DBE us;  FBE(i1,j1) += DP u, SBE += DP us;
+ *   E = sum DP DBE u, S = sum DP DBE us.
+ * F, F2 (FAbet, which 0), SF accumulate (add2fock); E, S overwritten. */
+void orc_rxcuhf_omg2(i64 which, i64 nebf, i64 npbf, const double *Dal, const double *Dto, const double *DBE,
+                     const double *DP, const double *GM2_1, const double *GM2_2, const double *GM2s, double *F,
+                     double *F2, double *SF, double *E, double *Sv)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double)), *xs = (double *)calloc(nf, sizeof(double));
+    double *xb = (double *)calloc(nf, sizeof(double));
+    double e = 0.0, sv = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1)
+    for (i64 i2 = 0; i2 < ne; ++i2)
+    for (i64 j2 = 0; j2 < ne; ++j2) {
+        double val1 = GM2_1[ix2(ne, np, ip, jp, i1, j1, i2, j2)];
+        double val2 = GM2_2[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+        double sval = GM2s[ix2(ne, np, ip, jp, i1, j2, i2, j1)];
+        double u = RXT(i2, j2) * val1 - RXL(i2, j2) * val2, us = -RXL(i2, j2) * sval;
+        if (which == 0) {
+            xf[j2 * ne + i2] += RXP(ip, jp) * RXB(i1, j1) * (val1 - val2);
+            xs[j2 * ne + i2] += RXP(ip, jp) * RXB(i1, j1) * sval;
+            xb[j2 * ne + i2] += RXP(ip, jp) * RXB(i1, j1) * val1;
+        } else if (which == 1) {
+            xf[jp * np + ip] += RXB(i1, j1) * u;
+            xs[jp * np + ip] += RXB(i1, j1) * us;
+        } else {
+            xf[j1 * ne + i1] += RXP(ip, jp) * u;
+            xs[j1 * ne + i1] += RXP(ip, jp) * us;
+        }
+        e += RXP(ip, jp) * RXB(i1, j1) * u;
+        sv += RXP(ip, jp) * RXB(i1, j1) * us;
+    }
+    for (i64 k = 0; k < nf; ++k) {
+        F[k] += xf[k];
+        SF[k] += xs[k];
+        if (which == 0) F2[k] += xb[k];
+    }
+    *E = e;
+    *Sv = sv;
+    free(xf);
+    free(xs);
+    free(xb);
+}
+
+#define G31(a1, b1, a2, b2, a3, b3) GM3_1[ix3(ne, np, ip, jp, a1, b1, a2, b2, a3, b3)]
+#define G32(a1, b1, a2, b2, a3, b3) GM3_2[ix3(ne, np, ip, jp, a1, b1, a2, b2, a3, b3)]
+/* RXCUHF_FE_OMG3 / _FP_OMG3 / _FBE_OMG3 (rxchf/rxcuhf_omg3.f:2-728); which =
+ * 0, 1, 2.  Per entry (ip,jp,i1,j1,i2,j2,i3,j3), with
+ *   val1a = GM3_1(i1 j1 i2 j2 i3 j3), val1b = GM3_1(i1 j1 i2 j3 i3 j2),
+ *   g213 = GM3_2(i1 j2 i2 j1 i3 j3), g312 = GM3_2(i1 j3 i2 j1 i3 j2),
+ *   w = DAtot2 DAtot3 val1a / 2 - (DAalp2 DAalp3 + DAbet2 DAbet3) val1b / 2
+ *       - DAalp2 DAtot3 g213 + DAalp2 DAalp3 g312        (:253-259, :476-489, :706-719)
+ * (densities at (i2,j2) and (i3,j3)): FP(ip,jp) += DBE w, FBE(i1,j1) += DP w,
+ * E = sum DP DBE w for all three routines; FE (:233-251):
+ *   val2a = (g213 + GM3_2(i1 i2 j2 j1 i3 j3))/2, val2b = (g312 + GM3_2(i1 j3 j2 j1 i3 i2))/2,
+ *   val2c = (GM3_2(i1 j3 i3 j1 i2 j2) + GM3_2(i1 j3 i3 j1 j2 i2))/2,
+ *   val2d = (GM3_2(i1 j2 i3 j1 i2 j3) + GM3_2(i1 i2 i3 j1 j2 j3))/2,
+ *   FAalp(i2,j2) += DBE DP (DAtot3 val1a - DAalp3 val1b - DAtot3 val2a + DAalp3 val2b
+ *                           - DAalp3 val2c + DAalp3 val2d),
+ *   FAbet(i2,j2) += DBE DP (DAtot3 val1a - DAbet3 val1b - DAalp3 val2c).
+ * F, F2 (FAbet, which 0) accumulate; E overwritten. */
+void orc_rxcuhf_omg3(i64 which, i64 nebf, i64 npbf, const double *Dal, const double *Dbt, const double *Dto,
+                     const double *DBE, const double *DP, const double *GM3_1, const double *GM3_2, double *F,
+                     double *F2, double *E)
+{
+    i64 ne = nebf, np = npbf;
+    i64 nf = which == 1 ? np * np : ne * ne;
+    double *xf = (double *)calloc(nf, sizeof(double)), *xb = (double *)calloc(nf, sizeof(double));
+    double e = 0.0;
+    for (i64 ip = 0; ip < np; ++ip)
+    for (i64 jp = 0; jp < np; ++jp)
+    for (i64 i1 = 0; i1 < ne; ++i1)
+    for (i64 j1 = 0; j1 < ne; ++j1)
+    for (i64 i2 = 0; i2 < ne; ++i2)
+    for (i64 j2 = 0; j2 < ne; ++j2)
+    for (i64 i3 = 0; i3 < ne; ++i3)
+    for (i64 j3 = 0; j3 < ne; ++j3) {
+        double val1a = G31(i1, j1, i2, j2, i3, j3), val1b = G31(i1, j1, i2, j3, i3, j2);
+        double g213 = G32(i1, j2, i2, j1, i3, j3), g312 = G32(i1, j3, i2, j1, i3, j2);
+        double w = 0.5 * RXT(i2, j2) * RXT(i3, j3) * val1a -
+                   0.5 * (RXL(i2, j2) * RXL(i3, j3) + RXBt(i2, j2) * RXBt(i3, j3)) * val1b -
+                   RXL(i2, j2) * RXT(i3, j3) * g213 + RXL(i2, j2) * RXL(i3, j3) * g312;
+        if (which == 0) {
+            double val2a = (g213 + G32(i1, i2, j2, j1, i3, j3)) * 0.5;
+            double val2b = (g312 + G32(i1, j3, j2, j1, i3, i2)) * 0.5;
+            double val2c = (G32(i1, j3, i3, j1, i2, j2) + G32(i1, j3, i3, j1, j2, i2)) * 0.5;
+            double val2d = (G32(i1, j2, i3, j1, i2, j3) + G32(i1, i2, i3, j1, j2, j3)) * 0.5;
+            double f = RXB(i1, j1) * RXP(ip, jp);
+            xf[j2 * ne + i2] += f * (RXT(i3, j3) * val1a - RXL(i3, j3) * val1b - RXT(i3, j3) * val2a +
+                                     RXL(i3, j3) * val2b - RXL(i3, j3) * val2c + RXL(i3, j3) * val2d);
+            xb[j2 * ne + i2] += f * (RXT(i3, j3) * val1a - RXBt(i3, j3) * val1b - RXL(i3, j3) * val2c);
+        } else if (which == 1) {
+            xf[jp * np + ip] += RXB(i1, j1) * w;
+        } else {
+            xf[j1 * ne + i1] += RXP(ip, jp) * w;
+        }
+        e += RXP(ip, jp) * RXB(i1, j1) * w;
+    }
+    for (i64 k = 0; k < nf; ++k) {
+        F[k] += xf[k];
+        if (which == 0) F2[k] += xb[k];
+    }
+    *E = e;
+    free(xf);
+    free(xb);
+}
+#undef G31
+#undef G32
+#undef RXL
+#undef RXBt
+#undef RXT
+#undef RXB
+#undef RXP

@@ -1195,3 +1195,55 @@ def ref_rxchf_omg1(which, ne, npb, DBE, DP, G1, G1s, F, SF, workdir):
     finally:
         os.chdir(cwd)
     return e.value, sv.value
+
+
+def rxcuhf_omg2(which, ne, npb, DAalp, DAtot, DBE, DP, G1, G2, Gs, F, SF, F2=None):
+    """which: 0 RXCUHF_FE_OMG2 (F = FAalpE, F2 = FAbetE, SF = SAalpE),
+    1 RXCUHF_FP_OMG2, 2 RXCUHF_FBE_OMG2; F, F2, SF accumulate; returns (E, S)."""
+    e, sv = np.zeros(1), np.zeros(1)
+    lib().orc_rxcuhf_omg2(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAalp), _f(DAtot), _f(DBE), _f(DP),
+                          _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G2)), _p(np.ascontiguousarray(Gs)),
+                          _p(F), None if F2 is None else _p(F2), _p(SF), _p(e), _p(sv))
+    return float(e[0]), float(sv[0])
+
+
+def rxcuhf_omg3(which, ne, npb, DAalp, DAbet, DAtot, DBE, DP, G1, G2, F, F2=None):
+    """which: 0 RXCUHF_FE_OMG3 (F = FAalpE, F2 = FAbetE), 1 RXCUHF_FP_OMG3,
+    2 RXCUHF_FBE_OMG3; F, F2 accumulate; returns E."""
+    e = np.zeros(1)
+    lib().orc_rxcuhf_omg3(C.c_int64(which), C.c_int64(ne), C.c_int64(npb), _f(DAalp), _f(DAbet), _f(DAtot), _f(DBE),
+                          _f(DP), _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G2)), _p(F),
+                          None if F2 is None else _p(F2), _p(e))
+    return float(e[0])
+
+
+def ref_rxcuhf_omg2(which, ne, npb, DAalp, DAbet, DAtot, DBE, DP, G1, G2, Gs, F, SF, F2=None):
+    """RXCUHF_FE_OMG2 / _FP_OMG2 / _FBE_OMG2 (source/rxchf/rxcuhf_omg2.f:2,
+    :253, :469) of the reference; returns (E, S)."""
+    e, sv = C.c_double(), C.c_double()
+    ng2 = ne ** 4 * npb * npb
+    R = ref()
+    g = [_p(np.ascontiguousarray(x)) for x in (G1, G2, Gs)]
+    if which == 0:
+        R.rxcuhf_fe_omg2_(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _f(DAalp), _f(DAbet), _f(DAtot), _f(DBE), _f(DP), *g,
+                          _p(F), _p(F2), _p(SF), C.byref(e), C.byref(sv))
+    else:
+        fn = R.rxcuhf_fp_omg2_ if which == 1 else R.rxcuhf_fbe_omg2_
+        fn(_i8(1), _i8(ne), _i8(npb), _i8(ng2), _i8(1), _i8(1), _f(DAalp), _f(DAtot), _f(DBE), _f(DP), *g, _p(F),
+           _p(SF), C.byref(e), C.byref(sv))
+    return e.value, sv.value
+
+
+def ref_rxcuhf_omg3(which, ne, npb, DAalp, DAbet, DAtot, DBE, DP, G1, G2, F, F2=None):
+    """RXCUHF_FE_OMG3 / _FP_OMG3 / _FBE_OMG3 (source/rxchf/rxcuhf_omg3.f:2,
+    :270, :499) of the reference; returns E."""
+    e = C.c_double()
+    ng3 = ne ** 6 * npb * npb
+    R = ref()
+    a = [_i8(1), _i8(ne), _i8(npb), _i8(ng3), _f(DAalp), _f(DAbet), _f(DAtot), _f(DBE), _f(DP),
+         _p(np.ascontiguousarray(G1)), _p(np.ascontiguousarray(G2))]
+    if which == 0:
+        R.rxcuhf_fe_omg3_(*a, _p(F), _p(F2), C.byref(e))
+    else:
+        (R.rxcuhf_fp_omg3_ if which == 1 else R.rxcuhf_fbe_omg3_)(*a, _p(F), C.byref(e))
+    return e.value

@@ -23,7 +23,7 @@ program drive_shim
   real(c_double), allocatable :: elcbfc(:), nucbfc(:), de(:,:), dp(:,:), dae(:,:), dbe(:,:)
   integer(c_int64_t), allocatable :: kpestr(:), kpeend(:), ampeb2c(:), elcam(:), nucam(:)
   real(c_double), allocatable :: gm2(:), gm2s(:), gm3(:), gm4(:), fe(:,:), se(:,:), fp(:,:), sp(:,:), g1(:)
-  real(c_double), allocatable :: gx1(:), gx2(:), gx3(:)
+  real(c_double), allocatable :: gx1(:), gx2(:), gx3(:), fb(:,:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
@@ -258,6 +258,32 @@ program drive_shim
   se = 0.0d0
   call RXCHF_FBE_OMG1(nebf, npbf, ng1, dbe, dp, fe, se, e2, s2)
   write(11) fe, se, e2, s2
+  ! RXCUHF (rxcuhf_fock.f:368, :463, :555, :376, :470, :562) over the RXCHF
+  ! tensors: alpha = dae, beta = dbe, total = de, special-electron density dbe
+  allocate(fb(nebf, nebf))
+  fe = 0.0d0
+  fb = 0.0d0
+  se = 0.0d0
+  call RXCUHF_FE_OMG2(one, nebf, npbf, ng2, dae, dbe, de, dbe, dp, gm2, gx1, gx2, fe, fb, se, e2, s2)
+  write(11) fe, fb, se, e2, s2
+  fp = 0.0d0
+  sp = 0.0d0
+  call RXCUHF_FP_OMG2(one, nebf, npbf, ng2, nae, nbe, dae, de, dbe, dp, gm2, gx1, gx2, fp, sp, e2, s2)
+  write(11) fp, sp, e2, s2
+  fe = 0.0d0
+  se = 0.0d0
+  call RXCUHF_FBE_OMG2(one, nebf, npbf, ng2, nae, nbe, dae, de, dbe, dp, gm2, gx1, gx2, fe, se, e2, s2)
+  write(11) fe, se, e2, s2
+  fe = 0.0d0
+  fb = 0.0d0
+  call RXCUHF_FE_OMG3(one, nebf, npbf, ng3, dae, dbe, de, dbe, dp, gm3, gx3, fe, fb, e3)
+  write(11) fe, fb, e3
+  fp = 0.0d0
+  call RXCUHF_FP_OMG3(one, nebf, npbf, ng3, dae, dbe, de, dbe, dp, gm3, gx3, fp, e3)
+  write(11) fp, e3
+  fe = 0.0d0
+  call RXCUHF_FBE_OMG3(one, nebf, npbf, ng3, dae, dbe, de, dbe, dp, gm3, gx3, fe, e3)
+  write(11) fe, e3
   close(11)
 contains
   subroutine read_ufm(name, n, a)

@@ -106,7 +106,9 @@ def test_fortran_dropin_matches_oracle(recl):
              ne * ne, ne * ne, 1, 1, npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1,  # RXCHF_FAE/FP/FBE_OMG2
              ng3, ng3,  # RXCHF_GAM3_IC1: GM3_1ICR, GM3_2ICR
              ne * ne, 1, npb * npb, 1, ne * ne, 1,  # RXCHF_FAE/FP/FBE_OMG3
-             npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1]  # RXCHF_FP/FBE_OMG1
+             npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1,  # RXCHF_FP/FBE_OMG1
+             ne * ne, ne * ne, ne * ne, 1, 1, npb * npb, npb * npb, 1, 1, ne * ne, ne * ne, 1, 1,  # RXCUHF OMG2
+             ne * ne, ne * ne, 1, npb * npb, 1, ne * ne, 1]  # RXCUHF OMG3
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
     og2, og2s = oracle.gam2_ic1(s)
@@ -264,3 +266,31 @@ def test_fortran_dropin_matches_oracle(recl):
         check_close(parts[at + 1], SF.ravel(order="F"), what=f"RXCHF omg1 {which} S")
         assert parts[at + 2][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
         assert parts[at + 3][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)
+    # RXCUHF over the RXCHF tensors: alpha DAE, beta DBE, total DE, special DBE
+    r2 = oracle.rxchf_gam2_ic1(s)
+    F, F2, SF = (np.zeros((ne, ne), order="F") for _ in range(3))
+    e, sv = oracle.rxcuhf_omg2(0, ne, npb, DAE, DE, DBE, DP, *r2, F, SF, F2)
+    check_close(parts[133], F.ravel(order="F"), what="RXCUHF_FE_OMG2 FAalpE")
+    check_close(parts[134], F2.ravel(order="F"), what="RXCUHF_FE_OMG2 FAbetE")
+    check_close(parts[135], SF.ravel(order="F"), what="RXCUHF_FE_OMG2 SAalpE")
+    assert parts[136][0] == pytest.approx(e, rel=1e-11, abs=1e-14) and parts[137][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)
+    for k, which in enumerate((1, 2)):
+        n = npb if which == 1 else ne
+        F, SF = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        e, sv = oracle.rxcuhf_omg2(which, ne, npb, DAE, DE, DBE, DP, *r2, F, SF)
+        at = 138 + 4 * k
+        check_close(parts[at], F.ravel(order="F"), what=f"RXCUHF omg2 {which} F")
+        check_close(parts[at + 1], SF.ravel(order="F"), what=f"RXCUHF omg2 {which} S")
+        assert parts[at + 2][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+        assert parts[at + 3][0] == pytest.approx(sv, rel=1e-11, abs=1e-14)
+    F, F2 = np.zeros((ne, ne), order="F"), np.zeros((ne, ne), order="F")
+    e = oracle.rxcuhf_omg3(0, ne, npb, DAE, DBE, DE, DBE, DP, *r3, F, F2)
+    check_close(parts[146], F.ravel(order="F"), what="RXCUHF_FE_OMG3 FAalpE")
+    check_close(parts[147], F2.ravel(order="F"), what="RXCUHF_FE_OMG3 FAbetE")
+    assert parts[148][0] == pytest.approx(e, rel=1e-11, abs=1e-14)
+    for k, which in enumerate((1, 2)):
+        n = npb if which == 1 else ne
+        F = np.zeros((n, n), order="F")
+        e = oracle.rxcuhf_omg3(which, ne, npb, DAE, DBE, DE, DBE, DP, *r3, F)
+        check_close(parts[149 + 2 * k], F.ravel(order="F"), what=f"RXCUHF omg3 {which}")
+        assert parts[150 + 2 * k][0] == pytest.approx(e, rel=1e-11, abs=1e-14)

@@ -800,6 +800,41 @@ def test_rxchf3(torch_dev):
     neorx.release_resident()
 
 
+def test_rxcuhf(torch_dev):
+    """RXCUHF (open-shell regular electrons, source/rxchf/rxcuhf_omg{2,3}.f):
+    RXCUHF_FE/FP/FBE_OMG2 and _OMG3 through the C ABI against the oracle
+    (itself checked against the reference's routines in tests/test_oracle.py),
+    over random tensors with different alpha and beta densities."""
+    ne, npb = 4, 2
+    rng = np.random.default_rng(31)
+    Dal, Dbt, DB, DP = (synth.density(n, t, sd) for n, t, sd in ((ne, 2.0, 11), (ne, 1.0, 12), (ne, 1.0, 13),
+                                                                  (npb, 1.0, 14)))
+    Dto = Dal + Dbt
+    G2a = [rng.standard_normal(ne ** 4 * npb * npb) for _ in range(3)]
+    G3a = [rng.standard_normal(ne ** 6 * npb * npb) for _ in range(2)]
+    for w in range(3):
+        print(f"case rxcuhf omg2/omg3 which={w}", flush=True)
+        n = npb if w == 1 else ne
+        F0, S0, B0 = (np.asfortranarray(rng.standard_normal((n, n))) for _ in range(3))
+        A, SA, BA = F0.copy(order="F"), S0.copy(order="F"), B0.copy(order="F")
+        R, SR, BR = F0.copy(order="F"), S0.copy(order="F"), B0.copy(order="F")
+        eg = neorx.RXCUHF_OMG2(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G2a, A, SA, BA)
+        eo = oracle.rxcuhf_omg2(w, ne, npb, Dal, Dto, DB, DP, *G2a, R, SR, BR)
+        check_close(A - F0, R - F0, what=f"RXCUHF OMG2 {w} F")
+        check_close(SA - S0, SR - S0, what=f"RXCUHF OMG2 {w} S")
+        if w == 0:
+            check_close(BA - B0, BR - B0, what="RXCUHF OMG2 FAbetE")
+        assert eg[0] == pytest.approx(eo[0], rel=1e-11) and eg[1] == pytest.approx(eo[1], rel=1e-11)
+        A, BA, R, BR = F0.copy(order="F"), B0.copy(order="F"), F0.copy(order="F"), B0.copy(order="F")
+        eg = neorx.RXCUHF_OMG3(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G3a, A, BA)
+        eo = oracle.rxcuhf_omg3(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G3a, R, BR)
+        check_close(A - F0, R - F0, what=f"RXCUHF OMG3 {w} F")
+        if w == 0:
+            check_close(BA - B0, BR - B0, what="RXCUHF OMG3 FAbetE")
+        assert eg == pytest.approx(eo, rel=1e-11)
+    neorx.release_resident()
+
+
 def test_rxchfmult_xchf_side(torch_dev):
     """RXCHFmult's XCHF-side builders over one basis: class-8 and class-7
     tuples, RXCHFmult_GAM1_MD, RXCHFmult_GAM3_XCHF and RXCHFmult_GAM4_XCHF."""

@@ -617,3 +617,37 @@ def test_oracle_rxchf3_vs_reference(tmp_path):
         check_close(A - F0, B - F0, what=f"RXCHF OMG1 {w} F")
         check_close(SA, SB, what=f"RXCHF OMG1 {w} S")
         assert eo[0] == pytest.approx(er[0], rel=1e-12) and eo[1] == pytest.approx(er[1], rel=1e-12)
+
+
+@needs_ref
+def test_oracle_rxcuhf_vs_reference():
+    """RXCUHF (open-shell regular electrons, one special electron): the
+    oracle's RXCUHF_FE/FP/FBE_OMG2 and _OMG3 against the reference's routines
+    over random tensors and densities (alpha and beta densities differ)."""
+    ne, npb = 3, 2
+    rng = np.random.default_rng(8)
+    Dal, Dbt, DB, DP = (synth.density(n, t, sd) for n, t, sd in ((ne, 2.0, 11), (ne, 1.0, 12), (ne, 1.0, 13),
+                                                                  (npb, 1.0, 14)))
+    Dto = Dal + Dbt
+    G2a = [rng.standard_normal(ne ** 4 * npb * npb) for _ in range(3)]
+    G3a = [rng.standard_normal(ne ** 6 * npb * npb) for _ in range(2)]
+    for w in range(3):
+        n = npb if w == 1 else ne
+        F0, S0, B0 = (np.asfortranarray(rng.standard_normal((n, n))) for _ in range(3))
+        A, SA, BA = F0.copy(order="F"), S0.copy(order="F"), B0.copy(order="F")
+        R, SR, BR = F0.copy(order="F"), S0.copy(order="F"), B0.copy(order="F")
+        eo = oracle.rxcuhf_omg2(w, ne, npb, Dal, Dto, DB, DP, *G2a, A, SA, BA)
+        er = oracle.ref_rxcuhf_omg2(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G2a, R, SR, BR)
+        check_close(A - F0, R - F0, what=f"RXCUHF OMG2 {w} F")
+        check_close(SA - S0, SR - S0, what=f"RXCUHF OMG2 {w} S")
+        if w == 0:
+            check_close(BA - B0, BR - B0, what="RXCUHF OMG2 FAbetE")
+        assert eo[0] == pytest.approx(er[0], rel=1e-12) and eo[1] == pytest.approx(er[1], rel=1e-12)
+        F0, B0 = (np.asfortranarray(rng.standard_normal((n, n))) for _ in range(2))
+        A, BA, R, BR = F0.copy(order="F"), B0.copy(order="F"), F0.copy(order="F"), B0.copy(order="F")
+        eo = oracle.rxcuhf_omg3(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G3a, A, BA)
+        er = oracle.ref_rxcuhf_omg3(w, ne, npb, Dal, Dbt, Dto, DB, DP, *G3a, R, BR)
+        check_close(A - F0, R - F0, what=f"RXCUHF OMG3 {w} F")
+        if w == 0:
+            check_close(BA - B0, BR - B0, what="RXCUHF OMG3 FAbetE")
+        assert eo == pytest.approx(er, rel=1e-12)
