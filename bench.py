@@ -285,6 +285,61 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     }
 
 
+def n5_bench(np, torch, neorx, synth, n=64, workers=16, maxit=80):
+    """Config 5 at reduced basis sizes: n independent NEO-XCHF RHF instances
+    (neorx.synth.config5: proton scanned over n points in [-0.6, 0.6] bohr
+    along the bond, 4 to 12 electronic and 1 or 4 nuclear functions per
+    instance, 4 electrons).  Every instance's integrals are built on the
+    device (GM3 in FP64 and BF16); then all SCFs run with DIIS and graphs,
+    FP64 and incremental mixed precision, one after another and concurrently
+    (neorx.xchf.scf_batch: `workers` host threads, one stream per instance).
+    Reports instances/s per way and the largest per-instance |E_mixed -
+    E_FP64|."""
+    from neorx import xchf
+
+    systems = [synth.config5(k, n) for k in range(n)]
+    t0 = time.perf_counter()
+    hs = [neorx.System(s) for s in systems]
+    xs = [xchf.XCHF(h, s["nelec"], store=("f64", "bf16")) for h, s in zip(hs, systems)]
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    kw = dict(diis=True, graph=True, maxit=maxit, sw_hi=1e-3, sw_lo=1e-6)
+    res = {}
+    for mode in ("f64", "inc"):
+        xchf.scf_batch(xs, workers, mode=mode, **kw)  # graph captures
+        t0 = time.perf_counter()
+        seq = [x.scf(mode=mode, **kw) for x in xs]
+        t_seq = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        con = xchf.scf_batch(xs, workers, mode=mode, **kw)
+        t_con = time.perf_counter() - t0
+        same = all(a["comps"]["E_TOTAL"] == b["comps"]["E_TOTAL"] for a, b in zip(seq, con))
+        res[mode] = dict(seq=t_seq, con=t_con, energies=[r["comps"]["E_TOTAL"] for r in con],
+                         iters=sum(r["niter"] for r in con), same=same)
+        print(f"# C5 {mode}: {n} instances sequential {t_seq:.3f} s, concurrent ({workers} threads) {t_con:.3f} s, "
+              f"{res[mode]['iters']} iterations", file=sys.stderr, flush=True)
+    for x in xs:
+        x.close()
+    for h in hs:
+        h.close()
+    de = max(abs(a - b) for a, b in zip(res["inc"]["energies"], res["f64"]["energies"]))
+    return {
+        "workload": (f"{n} NEO-XCHF RHF instances, proton scanned over [-0.6, 0.6] bohr, nebf "
+                     f"{min(s['nebf'] for s in systems)}-{max(s['nebf'] for s in systems)}, npbf "
+                     f"{min(s['npbf'] for s in systems)}-{max(s['npbf'] for s in systems)}, 4 electrons (basis sizes "
+                     "reduced from config 5's 20-60 / 6-16)"),
+        "integral_build_s": t_build,
+        "instances_per_s_fp64_sequential": n / res["f64"]["seq"],
+        "instances_per_s_fp64_concurrent": n / res["f64"]["con"],
+        "instances_per_s_mixed_sequential": n / res["inc"]["seq"],
+        "instances_per_s_mixed_concurrent": n / res["inc"]["con"],
+        "concurrent_equals_sequential": res["f64"]["same"] and res["inc"]["same"],
+        "max_dE_mixed_vs_fp64_Eh": de,
+        "energy_range_fp64_Eh": [min(res["f64"]["energies"]), max(res["f64"]["energies"])],
+        "workers": workers,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +354,8 @@ def main():
                     help="SCF s/iter of the C1 system, FP64 vs mixed precision (builds C1's integrals first)")
     ap.add_argument("--c3", type=str, default="on", choices=("off", "on"),
                     help="config 3: RXCHF-ne contractions at 48/12 functions, FP64 vs FP32 vs BF16 storage")
+    ap.add_argument("--n5", type=str, default="on", choices=("off", "on"),
+                    help="config 5 at reduced basis sizes: 64 SCF instances, sequential vs concurrent")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -408,6 +465,11 @@ def main():
         print(f"# C3 GM2 built in {c3['gam2_ic1_build_s']:.1f} s; pass {c3['pass_ms_fp64']:.3f} / "
               f"{c3['pass_ms_fp32']:.3f} / {c3['pass_ms_bf16']:.3f} ms", file=sys.stderr, flush=True)
 
+    n5 = None
+    if args.n5 == "on":
+        print("# C5 batched instances (reduced basis sizes)", file=sys.stderr, flush=True)
+        n5 = n5_bench(np, torch, neorx, synth)
+
     scf = None
     if args.scf == "c1":
         print("# C1 SCF (FP64 vs mixed precision)", file=sys.stderr, flush=True)
@@ -432,7 +494,7 @@ def main():
                                "geminal; all e-p tuples + seeded samples of e-e-p and e-e-e-p tuples",
                    "e_p_tuples": n1, "e_e_p_tuples": ns, "e_e_e_p_tuples": ns, "device": arch},
         "per_class_integrals_per_s": {"e-p": rates[1], "e-e-p": rates[2], "e-e-e-p": rates[3]},
-        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock, "scf": scf, "c3_rxchfne": c3,
+        "roofline": roof, "cpu_baseline": cpu, "fock_build": fock, "scf": scf, "c3_rxchfne": c3, "c5_batched": n5,
     }
     print(json.dumps(out))
 

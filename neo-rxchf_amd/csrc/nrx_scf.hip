@@ -1246,6 +1246,14 @@ int phase_b(Xchf *x, const double *DE, const double *DP, double *FE, double *FP,
 // ------------------------------------------------------------------
 namespace {
 
+// Blocking copy ordered on stream st (after the work queued there).
+int copy_on(hipStream_t st, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NEORX_OK;
+}
+
 int xchf_build(Xchf *x, i64 handle, i64 store)
 {
     System *s = x->s;
@@ -1269,8 +1277,10 @@ int xchf_build(Xchf *x, i64 handle, i64 store)
         const i64 nat = s->d.nat;
         std::vector<double> cat(3 * std::max<i64>(nat, 1)), zan(std::max<i64>(nat, 1));
         if (nat > 0) {
-            HIPCHK(hipMemcpy(cat.data(), s->d.cat, 3 * nat * 8, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(zan.data(), s->d.zan, nat * 8, hipMemcpyDeviceToHost));
+            rc = copy_on(x->st, cat.data(), s->d.cat, 3 * nat * 8, hipMemcpyDeviceToHost);
+            if (rc) return rc;
+            rc = copy_on(x->st, zan.data(), s->d.zan, nat * 8, hipMemcpyDeviceToHost);
+            if (rc) return rc;
         }
         double e = 0.0;
         for (i64 i = 1; i < nat; ++i)
@@ -1497,9 +1507,14 @@ extern "C" int neorx_xchf_create(int64_t handle, int64_t nelec, int64_t store, i
     x->ne = s->d.nebf;
     x->np = s->d.npbf;
     x->nelec = (int)nelec;
-    if (hipStreamCreate(&x->st) != hipSuccess) {
+    // Non-blocking: a context's stream takes no implicit dependency on the
+    // legacy stream, so another thread's legacy-stream work cannot
+    // invalidate a graph capture on it (SCFs of several contexts run
+    // concurrently from host threads).  Every host copy of the context's
+    // buffers is therefore ordered on this stream (copy_on).
+    if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
         delete x;
-        return fail(NEORX_E_HIP, "hipStreamCreate failed");
+        return fail(NEORX_E_HIP, "hipStreamCreateWithFlags failed");
     }
     rc = xchf_build(x, handle, store ? store : 1);
     if (rc) {
@@ -1534,14 +1549,15 @@ extern "C" int neorx_xchf_integrals(int64_t ctx, double *enuc, double *se, doubl
     Xchf *x = get_x(ctx);
     if (!x) return fail(NEORX_E_ARG, "unknown XCHF context %lld", (long long)ctx);
     const i64 ne2 = x->ne * x->ne, np2 = x->np * x->np;
+    int rc = NEORX_OK;
     HIPCHK(hipStreamSynchronize(x->st));
     if (enuc) *enuc = x->enuc;
-    if (se) HIPCHK(hipMemcpy(se, x->Se, ne2 * 8, hipMemcpyDeviceToHost));
-    if (sp) HIPCHK(hipMemcpy(sp, x->Sp, np2 * 8, hipMemcpyDeviceToHost));
-    if (gam_ecore) HIPCHK(hipMemcpy(gam_ecore, x->He, ne2 * 8, hipMemcpyDeviceToHost));
-    if (gam_pcore) HIPCHK(hipMemcpy(gam_pcore, x->Hp, np2 * 8, hipMemcpyDeviceToHost));
-    if (gam_ep) HIPCHK(hipMemcpy(gam_ep, x->Gep, ne2 * np2 * 8, hipMemcpyDeviceToHost));
-    if (gam_ee) HIPCHK(hipMemcpy(gam_ee, x->Gee, ne2 * ne2 * 8, hipMemcpyDeviceToHost));
+    if (se && (rc = copy_on(x->st, se, x->Se, ne2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (sp && (rc = copy_on(x->st, sp, x->Sp, np2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (gam_ecore && (rc = copy_on(x->st, gam_ecore, x->He, ne2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (gam_pcore && (rc = copy_on(x->st, gam_pcore, x->Hp, np2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (gam_ep && (rc = copy_on(x->st, gam_ep, x->Gep, ne2 * np2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (gam_ee && (rc = copy_on(x->st, gam_ee, x->Gee, ne2 * ne2 * 8, hipMemcpyDeviceToHost))) return rc;
     return NEORX_OK;
 }
 
@@ -1655,20 +1671,24 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
     } else {
         // GAM_ecore as a column-major Fock matrix: fock(i,j) = GAM_ecore[i][j]
         std::vector<double> h(ne2), f(ne2);
-        HIPCHK(hipMemcpy(h.data(), x->He, ne2 * 8, hipMemcpyDeviceToHost));
+        rc = copy_on(st, h.data(), x->He, ne2 * 8, hipMemcpyDeviceToHost);
+        if (rc) return rc;
         for (int i = 0; i < ne; ++i)
             for (int j = 0; j < ne; ++j) f[j * ne + i] = h[i * ne + j];
-        HIPCHK(hipMemcpy(x->FEx, f.data(), ne2 * 8, hipMemcpyHostToDevice));
+        rc = copy_on(st, x->FEx, f.data(), ne2 * 8, hipMemcpyHostToDevice);
+        if (rc) return rc;
         rc = launch_roothan(x, ne, x->Xe, x->FEx, 0, nocc - 1, pre, x->CE, x->evE, x->DE, x->VwE, x->vflag);
         if (rc) return rc;
     }
     // guess_prot (xcscf.f): ROOTHAN on GAM_pcore, orbital NUCST occupied
     {
         std::vector<double> h(np2), f(np2);
-        HIPCHK(hipMemcpy(h.data(), x->Hp, np2 * 8, hipMemcpyDeviceToHost));
+        rc = copy_on(st, h.data(), x->Hp, np2 * 8, hipMemcpyDeviceToHost);
+        if (rc) return rc;
         for (int i = 0; i < np; ++i)
             for (int j = 0; j < np; ++j) f[j * np + i] = h[i * np + j];
-        HIPCHK(hipMemcpy(x->FEx, f.data(), np2 * 8, hipMemcpyHostToDevice));
+        rc = copy_on(st, x->FEx, f.data(), np2 * 8, hipMemcpyHostToDevice);
+        if (rc) return rc;
         rc = launch_roothan(x, np, x->Xp, x->FEx, (int)nucst - 1, (int)nucst - 1, 1.0, x->CP, x->evP, x->DP, x->VwP,
                             x->vflag + 1);
         if (rc) return rc;
@@ -1767,10 +1787,10 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
     if (niter) *niter = it - 1;
     HIPCHK(hipStreamSynchronize(st));
     if (comps) std::copy(x->hstat + 3, x->hstat + 17, comps);
-    if (ce) HIPCHK(hipMemcpy(ce, x->CE, ne2 * 8, hipMemcpyDeviceToHost));
-    if (cp) HIPCHK(hipMemcpy(cp, x->CP, np2 * 8, hipMemcpyDeviceToHost));
-    if (ev_e) HIPCHK(hipMemcpy(ev_e, x->evE, ne * 8, hipMemcpyDeviceToHost));
-    if (ev_p) HIPCHK(hipMemcpy(ev_p, x->evP, np * 8, hipMemcpyDeviceToHost));
+    if (ce && (rc = copy_on(x->st, ce, x->CE, ne2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (cp && (rc = copy_on(x->st, cp, x->CP, np2 * 8, hipMemcpyDeviceToHost))) return rc;
+    if (ev_e && (rc = copy_on(x->st, ev_e, x->evE, ne * 8, hipMemcpyDeviceToHost))) return rc;
+    if (ev_p && (rc = copy_on(x->st, ev_p, x->evP, np * 8, hipMemcpyDeviceToHost))) return rc;
     if (!conv) return fail(NEORX_E_ARG, "SCF did not converge in %lld iterations", (long long)maxit);
     return NEORX_OK;
 }

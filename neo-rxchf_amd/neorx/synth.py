@@ -195,6 +195,46 @@ def config3(seed=3):
     return sysd
 
 
+def config5(k, n=64, seed=5):
+    """Instance k of n of the batched many-instance NEO-XCHF RHF workload, at
+    reduced basis sizes: a linear frame with classical nuclei of charge 2 at
+    z = 0 and charge 1 at z = 3.0 bohr, 4 electrons, and the quantum proton's
+    basis centre scanned over n points in [-0.6, 0.6] bohr about z = 1.5
+    along the bond.  Uneven sizes per instance (drawn from the seed and k):
+    on each classical centre 2 or 3 s functions with even-tempered exponents
+    (0.35, 1.4, 5.6, each scaled by a factor in [0.9, 1.1]) and optionally a
+    p shell (exponent 0.9, same scaling), 4 to 12 functions of one primitive
+    each; nuclear basis an s shell and optionally a p shell (1 or 4
+    functions), exponents in [8, 16]; a 3-term geminal shared by all
+    instances."""
+    rng = np.random.default_rng(seed * 1000 + k)
+    zc = [np.array([0.0, 0.0, 0.0]), np.array([0.0, 0.0, 3.0])]
+    zp = np.array([0.0, 0.0, 1.5 - 0.6 + 1.2 * k / max(1, n - 1)])
+    prims, contr = [], []
+    for c in zc:
+        ns = int(rng.integers(2, 4))
+        f = rng.uniform(0.9, 1.1)
+        for a in (0.35, 1.4, 5.6)[:ns]:
+            prims.append(((0, 0, 0), a * f, c, 1.0))
+            contr.append(1)
+        if rng.random() < 0.5:
+            for l3 in _SHELL[1]:
+                prims.append((l3, 0.9 * f, c, 1.0))
+                contr.append(1)
+    nuc = []
+    for l in [0] + ([1] if rng.random() < 0.5 else []):
+        a = float(rng.uniform(8.0, 16.0))
+        for l3 in _SHELL[l]:
+            nuc.append((l3, a, zp))
+    g_rng = np.random.default_rng(seed)
+    b = g_rng.uniform(0.1, 0.5, 3)
+    g = _logu(g_rng, 1.0, 30.0, 3)
+    sysd = _finish(prims, contr, nuc, zc, [2.0, 1.0], b, g)
+    sysd["nelec"] = 4
+    sysd["name"] = f"C5 instance {k}: proton at z = {zp[2]:.4f} bohr"
+    return sysd
+
+
 def small(seed=0, n_e_centres=2, e_shells=(0, 1), contract=(1, 2), p_shells=(0, 1), ngtg=3, nat=2, maxl=None):
     """Small random system for parity tests: s/p/d/f shells on random
     centres, contracted functions of 1-2 primitives."""

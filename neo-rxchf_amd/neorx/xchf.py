@@ -274,6 +274,20 @@ class XCHF:
                     trace=tr[:n, :8], niter=n)
 
 
+def scf_batch(contexts, workers=16, **kw):
+    """The SCF of several XCHF contexts at once: up to ``workers`` host
+    threads, each driving one context's own stream and captured graphs
+    (capture is per thread), so the small kernels of many instances run on
+    the GPU side by side.  Keyword arguments go to every XCHF.scf call;
+    returns their results in input order.  Contexts made with autotune=True
+    are tuned when they are built, never during the SCF, so concurrent runs
+    share nothing but read-only tables."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        return list(ex.map(lambda c: c.scf(**kw), contexts))
+
+
 def roothan_dev(n, d_s, d_f, kstart, klast, prefac, d_c, d_ev, d_d, stream=0):
     """ROOTHAN (xcscf.f:766) on device arrays: orbitals kstart..klast (1-based)
     occupied with weight prefac."""
