@@ -2321,141 +2321,170 @@ __global__ void __launch_bounds__(256) k_uhf_slab(i64 ne, i64 np, int flags, con
     nblock = nrem * ne;
     const T *Gb = G + pp * nblock;
     const T *Gsb = Gs ? Gs + pp * nblock : nullptr;
-    for (int k = threadIdx.x; k < 2 * ne + 2; k += blockDim.x) sm[k] = 0.0;
-    __syncthreads();
+    // Fixed-order sums throughout (no floating-point atomics): the slab is
+    // walked one ket index j0 of the first electron at a time; the row
+    // entries sm[j0], sm[ne + j0] are block sums over the other indices
+    // (wave_sum, then the four waves' slots in order).
+    __shared__ double wslot[4][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const i64 ninner = nrem / ne;
     double pe = 0.0, ps = 0.0;
 #define DA_(i, j) DA[(j) * ne + (i)]
 #define DB_(i, j) DB[(j) * ne + (i)]
-    for (i64 r0 = threadIdx.x; r0 < nrem; r0 += blockDim.x) {
-        i64 ii[N], jj[N], r = r0;
-        for (int k = N - 1; k >= 1; --k) {
-            jj[k] = r % ne;
-            r /= ne;
-            ii[k] = r % ne;
-            r /= ne;
-        }
-        jj[0] = r;
-        ii[0] = a;
-        // read with ket of electron k = jj[perm[k]]
-        auto rd = [&](const T *B, const int *perm) -> double {
-            i64 off = 0;
-            for (int k = 0; k < N; ++k) off = off * ne2 + ii[k] * ne + jj[perm[k]];
-            return (double)B[off];
-        };
-        double t = 0.0, ts = 0.0;
-        if constexpr (N == 2) {
-            const int p12[2] = {0, 1}, p21[2] = {1, 0};
-            double o12 = rd(Gb, p12), o21 = rd(Gb, p21), so12 = rd(Gsb, p12), so21 = rd(Gsb, p21);
-            if constexpr (FAE) {
-                double c1 = (flags & U_AA) ? DA_(ii[1], jj[1]) : 0.0, c2 = (flags & U_AB) ? DB_(ii[1], jj[1]) : 0.0;
-                t = c1 * (o12 - o21) + c2 * o12;
-                ts = c1 * (so12 - so21) + c2 * so12;
-            } else {
+    for (i64 j0 = 0; j0 < ne; ++j0) {
+        double fe = 0.0, fs = 0.0;
+        for (i64 r0 = threadIdx.x; r0 < ninner; r0 += blockDim.x) {
+            i64 ii[N], jj[N], r = r0;
+            for (int k = N - 1; k >= 1; --k) {
+                jj[k] = r % ne;
+                r /= ne;
+                ii[k] = r % ne;
+                r /= ne;
+            }
+            jj[0] = j0;
+            ii[0] = a;
+            // read with ket of electron k = jj[perm[k]]
+            auto rd = [&](const T *B, const int *perm) -> double {
+                i64 off = 0;
+                for (int k = 0; k < N; ++k) off = off * ne2 + ii[k] * ne + jj[perm[k]];
+                return (double)B[off];
+            };
+            double t = 0.0, ts = 0.0;
+            if constexpr (N == 2) {
+                const int p12[2] = {0, 1}, p21[2] = {1, 0};
+                double o12 = rd(Gb, p12), o21 = rd(Gb, p21), so12 = rd(Gsb, p12), so21 = rd(Gsb, p21);
+                if constexpr (FAE) {
+                    double c1 = (flags & U_AA) ? DA_(ii[1], jj[1]) : 0.0, c2 = (flags & U_AB) ? DB_(ii[1], jj[1]) : 0.0;
+                    t = c1 * (o12 - o21) + c2 * o12;
+                    ts = c1 * (so12 - so21) + c2 * so12;
+                } else {
+                    double a1 = DA_(ii[0], jj[0]), b1 = DB_(ii[0], jj[0]), a2 = DA_(ii[1], jj[1]), b2 = DB_(ii[1], jj[1]);
+                    double c1 = (flags & U_AB) ? a1 * b2 + b1 * a2 : 0.0;
+                    double c2 = (flags & U_AA) ? a1 * a2 : 0.0, c3 = (flags & U_BB) ? b1 * b2 : 0.0;
+                    t = c1 * o12 + (c2 + c3) * (o12 - o21);
+                    ts = c1 * so12 + (c2 + c3) * (so12 - so21);
+                }
+            } else if constexpr (N == 3) {
+                const int p123[3] = {0, 1, 2}, p213[3] = {1, 0, 2}, p312[3] = {2, 0, 1}, p132[3] = {0, 2, 1},
+                          p231[3] = {1, 2, 0}, p321[3] = {2, 1, 0};
+                double g123 = rd(Gb, p123), g213 = rd(Gb, p213), g312 = rd(Gb, p312), g132 = rd(Gb, p132),
+                       g231 = rd(Gb, p231), g321 = rd(Gb, p321);
+                double o6 = g123 - g213 + g312 - g132 + g231 - g321;
+                double o23 = g123 - g132, o12 = g123 - g213, o13 = g123 - g321;
                 double a1 = DA_(ii[0], jj[0]), b1 = DB_(ii[0], jj[0]), a2 = DA_(ii[1], jj[1]), b2 = DB_(ii[1], jj[1]);
-                double c1 = (flags & U_AB) ? a1 * b2 + b1 * a2 : 0.0;
-                double c2 = (flags & U_AA) ? a1 * a2 : 0.0, c3 = (flags & U_BB) ? b1 * b2 : 0.0;
-                t = c1 * o12 + (c2 + c3) * (o12 - o21);
-                ts = c1 * so12 + (c2 + c3) * (so12 - so21);
-            }
-        } else if constexpr (N == 3) {
-            const int p123[3] = {0, 1, 2}, p213[3] = {1, 0, 2}, p312[3] = {2, 0, 1}, p132[3] = {0, 2, 1},
-                      p231[3] = {1, 2, 0}, p321[3] = {2, 1, 0};
-            double g123 = rd(Gb, p123), g213 = rd(Gb, p213), g312 = rd(Gb, p312), g132 = rd(Gb, p132),
-                   g231 = rd(Gb, p231), g321 = rd(Gb, p321);
-            double o6 = g123 - g213 + g312 - g132 + g231 - g321;
-            double o23 = g123 - g132, o12 = g123 - g213, o13 = g123 - g321;
-            double a1 = DA_(ii[0], jj[0]), b1 = DB_(ii[0], jj[0]), a2 = DA_(ii[1], jj[1]), b2 = DB_(ii[1], jj[1]);
-            double a3 = DA_(ii[2], jj[2]), b3 = DB_(ii[2], jj[2]);
-            if constexpr (FAE) {
-                // FAE_OMG3 flags: U_AAA = LAA (NAE>=3), U_BBB = LBB (NBE>=2), U_AAB = LAB
-                double xaa = (flags & U_AAA) ? a2 * a3 * o6 : 0.0;
-                double xbb = (flags & U_BBB) ? b2 * b3 * o23 : 0.0;
-                double xab = (flags & U_AAB) ? a2 * b3 * o12 : 0.0;
-                double xba = (flags & U_AAB) ? b2 * a3 * o13 : 0.0;
-                t = xaa + xbb + xab + xba;
-            } else {
-                double caaa = (flags & U_AAA) ? a1 * a2 * a3 : 0.0, cbbb = (flags & U_BBB) ? b1 * b2 * b3 : 0.0;
-                double caab = 0, caba = 0, cbaa = 0, cbba = 0, cbab = 0, cabb = 0;
-                if (flags & U_AAB) {
-                    caab = a1 * a2 * b3;
-                    caba = a1 * b2 * a3;
-                    cbaa = b1 * a2 * a3;
+                double a3 = DA_(ii[2], jj[2]), b3 = DB_(ii[2], jj[2]);
+                if constexpr (FAE) {
+                    // FAE_OMG3 flags: U_AAA = LAA (NAE>=3), U_BBB = LBB (NBE>=2), U_AAB = LAB
+                    double xaa = (flags & U_AAA) ? a2 * a3 * o6 : 0.0;
+                    double xbb = (flags & U_BBB) ? b2 * b3 * o23 : 0.0;
+                    double xab = (flags & U_AAB) ? a2 * b3 * o12 : 0.0;
+                    double xba = (flags & U_AAB) ? b2 * a3 * o13 : 0.0;
+                    t = xaa + xbb + xab + xba;
+                } else {
+                    double caaa = (flags & U_AAA) ? a1 * a2 * a3 : 0.0, cbbb = (flags & U_BBB) ? b1 * b2 * b3 : 0.0;
+                    double caab = 0, caba = 0, cbaa = 0, cbba = 0, cbab = 0, cabb = 0;
+                    if (flags & U_AAB) {
+                        caab = a1 * a2 * b3;
+                        caba = a1 * b2 * a3;
+                        cbaa = b1 * a2 * a3;
+                    }
+                    if (flags & U_BBA) {
+                        cbba = b1 * b2 * a3;
+                        cbab = b1 * a2 * b3;
+                        cabb = a1 * b2 * b3;
+                    }
+                    t = (caaa + cbbb) * o6 + (caab + cbba) * o12 + (caba + cbab) * o13 + (cabb + cbaa) * o23;
                 }
-                if (flags & U_BBA) {
-                    cbba = b1 * b2 * a3;
-                    cbab = b1 * a2 * b3;
-                    cabb = a1 * b2 * b3;
+            } else {
+                double o[24];
+                for (int p = 0; p < 24; ++p) o[p] = rd(Gb, c_omg4[p]);
+                double anti = 0.0;
+                for (int p = 0; p < 24; ++p) anti += c_anti[p] * o[p];
+                // 0 OMG 1 23 2 12 3 12_23 4 13_23 5 13 6 24 7 12_24 8 12_14 9 14 10 34 11 12_34 12 13_34 13 14_34
+                // 14 13_24 15 14_23 16 23_34 17 24_34
+                double aab = o[0] - o[1] - o[2] + o[3] + o[4] - o[5];
+                double aba = o[0] - o[6] - o[2] + o[7] + o[8] - o[9];
+                double baa = o[0] - o[10] - o[5] + o[12] + o[13] - o[9];
+                double abb = o[0] - o[10] - o[2] + o[11];
+                double bab = o[0] - o[6] - o[5] + o[14];
+                double bba = o[0] - o[1] - o[9] + o[15];
+                double bbb = o[0] - o[10] - o[1] + o[16] + o[17] - o[6];
+                double A[4], B[4];
+                for (int k = 0; k < 4; ++k) {
+                    A[k] = DA_(ii[k], jj[k]);
+                    B[k] = DB_(ii[k], jj[k]);
                 }
-                t = (caaa + cbbb) * o6 + (caab + cbba) * o12 + (caba + cbab) * o13 + (cabb + cbaa) * o23;
-            }
-        } else {
-            double o[24];
-            for (int p = 0; p < 24; ++p) o[p] = rd(Gb, c_omg4[p]);
-            double anti = 0.0;
-            for (int p = 0; p < 24; ++p) anti += c_anti[p] * o[p];
-            // 0 OMG 1 23 2 12 3 12_23 4 13_23 5 13 6 24 7 12_24 8 12_14 9 14 10 34 11 12_34 12 13_34 13 14_34
-            // 14 13_24 15 14_23 16 23_34 17 24_34
-            double aab = o[0] - o[1] - o[2] + o[3] + o[4] - o[5];
-            double aba = o[0] - o[6] - o[2] + o[7] + o[8] - o[9];
-            double baa = o[0] - o[10] - o[5] + o[12] + o[13] - o[9];
-            double abb = o[0] - o[10] - o[2] + o[11];
-            double bab = o[0] - o[6] - o[5] + o[14];
-            double bba = o[0] - o[1] - o[9] + o[15];
-            double bbb = o[0] - o[10] - o[1] + o[16] + o[17] - o[6];
-            double A[4], B[4];
-            for (int k = 0; k < 4; ++k) {
-                A[k] = DA_(ii[k], jj[k]);
-                B[k] = DB_(ii[k], jj[k]);
+                if constexpr (FAE) {
+                    double x = 0.0;
+                    x += (flags & U_AAA) ? A[1] * A[2] * A[3] * anti : 0.0;
+                    x += (flags & U_AAB) ? A[1] * A[2] * B[3] * aab : 0.0;
+                    x += (flags & U_AAB) ? A[1] * B[2] * A[3] * aba : 0.0;
+                    x += (flags & U_AAB) ? B[1] * A[2] * A[3] * baa : 0.0;
+                    x += (flags & U_BBA) ? A[1] * B[2] * B[3] * abb : 0.0;
+                    x += (flags & U_BBA) ? B[1] * A[2] * B[3] * bab : 0.0;
+                    x += (flags & U_BBA) ? B[1] * B[2] * A[3] * bba : 0.0;
+                    x += (flags & U_BBB) ? B[1] * B[2] * B[3] * bbb : 0.0;
+                    t = x;
+                } else {
+                    double x = 0.0;
+                    x += (flags & U_4A) ? A[0] * A[1] * A[2] * A[3] * anti : 0.0;
+                    x += (flags & U_3A1B) ? A[0] * A[1] * A[2] * B[3] * aab : 0.0;
+                    x += (flags & U_3A1B) ? A[0] * A[1] * B[2] * A[3] * aba : 0.0;
+                    x += (flags & U_3A1B) ? A[0] * B[1] * A[2] * A[3] * baa : 0.0;
+                    x += (flags & U_2A2B) ? A[0] * A[1] * B[2] * B[3] * abb : 0.0;
+                    x += (flags & U_2A2B) ? A[0] * B[1] * A[2] * B[3] * bab : 0.0;
+                    x += (flags & U_2A2B) ? A[0] * B[1] * B[2] * A[3] * bba : 0.0;
+                    x += (flags & U_1A3B) ? A[0] * B[1] * B[2] * B[3] * bbb : 0.0;
+                    x += (flags & U_4B) ? B[0] * B[1] * B[2] * B[3] * anti : 0.0;
+                    x += (flags & U_1A3B) ? B[0] * B[1] * B[2] * A[3] * aab : 0.0;
+                    x += (flags & U_1A3B) ? B[0] * B[1] * A[2] * B[3] * aba : 0.0;
+                    x += (flags & U_1A3B) ? B[0] * A[1] * B[2] * B[3] * baa : 0.0;
+                    x += (flags & U_2A2B) ? B[0] * B[1] * A[2] * A[3] * abb : 0.0;
+                    x += (flags & U_2A2B) ? B[0] * A[1] * B[2] * A[3] * bab : 0.0;
+                    x += (flags & U_2A2B) ? B[0] * A[1] * A[2] * B[3] * bba : 0.0;
+                    x += (flags & U_3A1B) ? B[0] * A[1] * A[2] * A[3] * bbb : 0.0;
+                    t = x;
+                }
             }
             if constexpr (FAE) {
-                double x = 0.0;
-                x += (flags & U_AAA) ? A[1] * A[2] * A[3] * anti : 0.0;
-                x += (flags & U_AAB) ? A[1] * A[2] * B[3] * aab : 0.0;
-                x += (flags & U_AAB) ? A[1] * B[2] * A[3] * aba : 0.0;
-                x += (flags & U_AAB) ? B[1] * A[2] * A[3] * baa : 0.0;
-                x += (flags & U_BBA) ? A[1] * B[2] * B[3] * abb : 0.0;
-                x += (flags & U_BBA) ? B[1] * A[2] * B[3] * bab : 0.0;
-                x += (flags & U_BBA) ? B[1] * B[2] * A[3] * bba : 0.0;
-                x += (flags & U_BBB) ? B[1] * B[2] * B[3] * bbb : 0.0;
-                t = x;
+                fe += (double)N * t;
+                fs += 2.0 * ts;
+                pe += DA_(a, jj[0]) * t;
+                ps += DA_(a, jj[0]) * ts;
             } else {
-                double x = 0.0;
-                x += (flags & U_4A) ? A[0] * A[1] * A[2] * A[3] * anti : 0.0;
-                x += (flags & U_3A1B) ? A[0] * A[1] * A[2] * B[3] * aab : 0.0;
-                x += (flags & U_3A1B) ? A[0] * A[1] * B[2] * A[3] * aba : 0.0;
-                x += (flags & U_3A1B) ? A[0] * B[1] * A[2] * A[3] * baa : 0.0;
-                x += (flags & U_2A2B) ? A[0] * A[1] * B[2] * B[3] * abb : 0.0;
-                x += (flags & U_2A2B) ? A[0] * B[1] * A[2] * B[3] * bab : 0.0;
-                x += (flags & U_2A2B) ? A[0] * B[1] * B[2] * A[3] * bba : 0.0;
-                x += (flags & U_1A3B) ? A[0] * B[1] * B[2] * B[3] * bbb : 0.0;
-                x += (flags & U_4B) ? B[0] * B[1] * B[2] * B[3] * anti : 0.0;
-                x += (flags & U_1A3B) ? B[0] * B[1] * B[2] * A[3] * aab : 0.0;
-                x += (flags & U_1A3B) ? B[0] * B[1] * A[2] * B[3] * aba : 0.0;
-                x += (flags & U_1A3B) ? B[0] * A[1] * B[2] * B[3] * baa : 0.0;
-                x += (flags & U_2A2B) ? B[0] * B[1] * A[2] * A[3] * abb : 0.0;
-                x += (flags & U_2A2B) ? B[0] * A[1] * B[2] * A[3] * bab : 0.0;
-                x += (flags & U_2A2B) ? B[0] * A[1] * A[2] * B[3] * bba : 0.0;
-                x += (flags & U_3A1B) ? B[0] * A[1] * A[2] * A[3] * bbb : 0.0;
-                t = x;
+                pe += t;
+                ps += ts;
             }
         }
         if constexpr (FAE) {
-            atomicAdd(&sm[jj[0]], (double)N * t);
-            if (N == 2) atomicAdd(&sm[ne + jj[0]], 2.0 * ts);
-            pe += DA_(a, jj[0]) * t;
-            ps += DA_(a, jj[0]) * ts;
-        } else {
-            pe += t;
-            ps += ts;
+            fe = wave_sum(fe);
+            fs = wave_sum(fs);
+            if (lane == 0) {
+                wslot[wv][0] = fe;
+                wslot[wv][1] = fs;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                sm[j0] = ((wslot[0][0] + wslot[1][0]) + wslot[2][0]) + wslot[3][0];
+                sm[ne + j0] = N == 2 ? ((wslot[0][1] + wslot[1][1]) + wslot[2][1]) + wslot[3][1] : 0.0;
+            }
+            __syncthreads();
+        } else if (threadIdx.x < 2) {
+            sm[threadIdx.x * ne + j0] = 0.0;
         }
     }
 #undef DA_
 #undef DB_
     pe = wave_sum(pe);
     ps = wave_sum(ps);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&sm[2 * ne], pe);
-        atomicAdd(&sm[2 * ne + 1], ps);
+    if (lane == 0) {
+        wslot[wv][0] = pe;
+        wslot[wv][1] = ps;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sm[2 * ne] = ((wslot[0][0] + wslot[1][0]) + wslot[2][0]) + wslot[3][0];
+        sm[2 * ne + 1] = ((wslot[0][1] + wslot[1][1]) + wslot[2][1]) + wslot[3][1];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 2 * ne + 2; k += blockDim.x) part[blk * (2 * ne + 2) + k] = sm[k];

@@ -954,3 +954,40 @@ def test_rxchfmult_tiers_lone_angular_momentum(torch_dev):
     ref = oracle.prim_list3(s, 7, idx)[0].reshape(-1, 6).mean(axis=1)
     check_close(got, ref, what="lone-am cls7 six-permutation average")
     S.close()
+
+
+def test_uhf_contractions_reproducible(torch_dev):
+    """FAE_OMG2/3 and UFP_OMG2/3 at nebf 8 (slab rows longer than a
+    workgroup, so the row sums cross waves): equal to the oracle, and two
+    calls on the same inputs agree bit for bit (fixed-order sums)."""
+    print("case uhf reproducible nebf=8 npbf=2", flush=True)
+    rng = np.random.default_rng(88)
+    ne, npb, nae, nbe = 8, 2, 3, 2
+    DAE, DBE = synth.density(ne, float(nae), 15), synth.density(ne, float(nbe), 16)
+    DP = synth.density(npb, 1.0, 17)
+    g2, g2s = rng.standard_normal(ne ** 4 * npb * npb), rng.standard_normal(ne ** 4 * npb * npb)
+    g3 = rng.standard_normal(ne ** 6 * npb * npb)
+    a = (nae, nbe, ne, npb, DAE, DBE, DP)
+    for gfn, ofn, n in ((neorx.FAE_OMG2, oracle.fae_omg2, ne), (neorx.UFP_OMG2, oracle.ufp_omg2, npb)):
+        runs = []
+        for _ in range(2):
+            Fg = np.zeros((n, n), order="F")
+            Sg, eg, sg = gfn(*a, g2, g2s, Fg)
+            runs.append((Fg, Sg, eg, sg))
+        Fo, So = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        eo, so = ofn(*a, g2, g2s, Fo, So)
+        check_close(runs[0][0], Fo, what=gfn.__name__)
+        check_close(runs[0][1], So, what=gfn.__name__ + " S")
+        assert runs[0][2] == pytest.approx(eo, rel=1e-11, abs=1e-14)
+        assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+        assert runs[0][2] == runs[1][2] and runs[0][3] == runs[1][3]
+    for gfn, ofn, n in ((neorx.FAE_OMG3, oracle.fae_omg3, ne), (neorx.UFP_OMG3, oracle.ufp_omg3, npb)):
+        runs = []
+        for _ in range(2):
+            Fg = np.zeros((n, n), order="F")
+            runs.append((Fg, gfn(*a, g3, Fg)))
+        Fo = np.zeros((n, n), order="F")
+        eo = ofn(*a, g3, Fo)
+        check_close(runs[0][0], Fo, what=gfn.__name__)
+        assert runs[0][1] == pytest.approx(eo, rel=1e-11, abs=1e-14)
+        assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
