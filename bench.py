@@ -226,14 +226,15 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     """NEO-XCHF RHF SCF of the C1 system (neorx.synth.config1: C/N classical,
     quantum proton, 24 electronic functions from 36 s/p primitives, 8 nuclear
     s/p/d primitives, 3-term geminal, 14 electrons) on the device: every
-    integral built on the GPU (GM3 = 1.22e10 entries, kept in FP64 and BF16),
-    then the SCF with DIIS, each iteration one replay of a captured hipGraph:
+    integral built on the GPU (GM3 = 1.22e10 entries, kept in FP64, FP32 and
+    BF16: 171 GB), then the SCF with DIIS, each iteration one replay of a
+    captured hipGraph:
       * FP64: GAM_3 read in FP64 every iteration;
       * mixed (incremental): GAM_3 read in BF16 (sums over its stored
-        entries in FP32) until the DIIS error < 1e-3, then an FP64 build kept
-        as the reference and BF16-stored increments T(DE - DEref, DE + DEref)
-        from it, a new FP64 reference if the DIIS error reaches 1e-6 before
-        convergence, and a last FP64 build that confirms convergence.
+        entries in FP32) until the DIIS error < 1e-3, then a build from the
+        FP32 copy kept as the reference and BF16-stored increments
+        T(DE - DEref, DE + DEref) from it, an FP64 reference once the DIIS
+        error reaches 1e-6, and a last FP64 build that confirms convergence.
     Each mode runs once untimed (graph capture) and once timed; SCF s/iter =
     the timed run's wall time (host clock, first graph launch to the last
     read-back) / its iterations.  The EG3 slab depth is autotuned per storage
@@ -243,7 +244,7 @@ def scf_bench(np, torch, neorx, synth, maxit=80):
     sysd = synth.config1()
     S = neorx.System(sysd)
     t0 = time.perf_counter()
-    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "bf16"), autotune=True)
+    X = xchf.XCHF(S, sysd["nelec"], store=("f64", "f32", "bf16"), autotune=True)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     ne, npe, npb = sysd["nebf"], sysd["npebf"], sysd["npbf"]

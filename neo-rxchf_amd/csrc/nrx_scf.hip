@@ -1703,12 +1703,13 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
     // precision of GAM_3 per iteration: mode 0 FP64; mode 1 FP32 until the
     // switch signal (DIIS error with DIIS, else max|dDE|) < sw_lo; mode 2
     // BF16 until < sw_hi, then FP32 until < sw_lo; FP64 from then on.
-    // Mode 3 (incremental): BF16 builds until the signal < sw_hi; then an FP64
-    // build whose rows become the reference, and BF16-stored increments from
-    // it (T(DE - DEref, DE + DEref), whose storage error shrinks with the
-    // distance to the reference); a new FP64 reference once the signal <
-    // sw_lo, and a last FP64 build when the increments meet the convergence
-    // test.  Every mode converges only on an FP64 build.
+    // Mode 3 (incremental): BF16 builds until the signal < sw_hi; then a
+    // reference build (from the FP32 copy if one is held, else FP64) whose
+    // rows are kept, and BF16-stored increments from it (T(DE - DEref,
+    // DE + DEref), whose storage error shrinks with the distance to the
+    // reference); an FP64 reference once the signal < sw_lo, and a last FP64
+    // build when the increments meet the convergence test.  Every mode
+    // converges only on an FP64 build.
     int dtype = mode == 0 ? NRX_F64 : mode == 1 ? NRX_F32 : NRX_BF16;
     int pass = G3_FULL;
     double ref_sig = 0.0; // the switch signal when the last reference was asked for
@@ -1772,8 +1773,15 @@ extern "C" int neorx_xchf_scf(int64_t ctx, const double *ce_guess, int64_t nucst
             const bool want_ref = pass == G3_INC ? (crit || (ref_sig >= sw_lo && sig < sw_lo))
                                                  : (pass == G3_FULL && dtype == NRX_BF16 && (sig < sw_hi || crit));
             if (want_ref) {
+                // the first reference (leaving the full BF16 builds above
+                // sw_hi) reads the FP32 copy when one is held: its rounding
+                // (2^-24 relative) is far below the DIIS error at that point.
+                // Every later reference, and so every build that may
+                // converge, reads FP64: increments follow any reference, and
+                // an increment meeting the criteria asks for a reference.
+                const bool f32_ref = pass == G3_FULL && !crit && x->G3f;
                 ref_sig = sig;
-                dtype = NRX_F64;
+                dtype = f32_ref ? NRX_F32 : NRX_F64;
                 pass = G3_REFRESH;
             } else if (pass == G3_REFRESH || pass == G3_INC) {
                 dtype = NRX_BF16;

@@ -256,7 +256,8 @@ class XCHF:
         (0 FP64, 1 FP32, 2 BF16), ms, GAM_3 pass (0 full, 1 full kept as the
         reference, 2 increment)].  mode: "f64"; "f32" (FP32 then FP64);
         "bf16" (BF16, FP32, FP64); "inc" (BF16, then BF16-stored increments
-        from FP64 reference builds)."""
+        from reference builds: the first from the FP32 copy when the context
+        holds one, the later ones and the converging build from FP64)."""
         ne, npb = self.ne, self.np
         m = {"f64": 0, "f32": 1, "bf16": 2, "inc": 3}[mode]
         ce0 = None if ce_guess is None else np.asfortranarray(ce_guess, dtype=np.float64)

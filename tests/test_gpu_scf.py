@@ -261,23 +261,28 @@ def test_scf_golden_minis(torch_dev, name):
     _check_golden(g, r)
 
 
-@pytest.mark.parametrize("mode", ["f32", "bf16", "inc"])
+@pytest.mark.parametrize("mode", ["f32", "bf16", "inc", "inc-f64ref"])
 def test_scf_mixed_precision(torch_dev, mode):
     """Mixed precision: GAM_3 read in FP32 (or BF16, then FP32) until the DIIS
     error falls below 1e-5, then FP64; or (inc) BF16 builds, then BF16-stored
-    increments from FP64 reference builds.  Every mode ends on an FP64 build,
-    so the final energy is the FP64 one."""
+    increments from reference builds: the first reference from the FP32 copy
+    when the context holds one (inc), from FP64 when it does not
+    (inc-f64ref), every later one from FP64.  Every mode ends on an FP64
+    build, so the final energy is the FP64 one."""
     print(f"case scf mixed {mode}", flush=True)
-    g, r = _run("c1mini_sp", mode=mode, diis=True, store=("f64", "f32", "bf16"))
+    store = ("f64", "bf16") if mode == "inc-f64ref" else ("f64", "f32", "bf16")
+    g, r = _run("c1mini_sp", mode=mode.split("-")[0], diis=True, store=store)
     prec, kind = r["trace"][:, 5], r["trace"][:, 7]
-    assert prec[0] == {"f32": 1, "bf16": 2, "inc": 2}[mode] and prec[-1] == 0, prec
-    if mode == "inc":
+    assert prec[0] == {"f32": 1, "bf16": 2, "inc": 2, "inc-f64ref": 2}[mode] and prec[-1] == 0, prec
+    if mode.startswith("inc"):
         # BF16 builds first, increments in between, an FP64 reference last
         assert kind[0] == 0 and kind[-1] == 1 and (kind == 2).any(), kind
-        assert np.all(prec[kind == 2] == 2) and np.all(prec[kind == 1] == 0), (prec, kind)
+        refs = prec[kind == 1]
+        assert np.all(prec[kind == 2] == 2) and refs[-1] == 0, (prec, kind)
+        assert refs[0] == (1 if mode == "inc" else 0) and np.all(refs[1:] == 0), (prec, kind)
     # the first build reads GAM_3 in the reduced storage: E_GAM3 (and the
     # total) of the initial guess move by the storage rounding, the rest not
-    tol = {"f32": 1e-6, "bf16": 1e-2, "inc": 1e-2}[mode]
+    tol = {"f32": 1e-6, "bf16": 1e-2, "inc": 1e-2, "inc-f64ref": 1e-2}[mode]
     assert abs(r["comps_init"]["E_GAM3"] - g["initial"]["E_GAM3"]) <= tol * abs(g["initial"]["E_GAM3"]) + PRINT_TOL
     _check_golden(g, r, skip=("E_GAM3", "E_TOTAL"))
 
