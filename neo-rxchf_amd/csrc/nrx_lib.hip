@@ -4757,42 +4757,53 @@ __global__ void __launch_bounds__(256) k_mult_weights(int k, i64 nb, i64 nc, con
     W[c] = w;
 }
 
-__global__ void k_fill0(i64 n, double *__restrict__ p)
-{
-    i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = 0.0;
-}
-
 static const i64 MULT_ROWS = 64; // rows of G per block of k_mult_rows
 
-// block = (run of MULT_ROWS rows, column tile of ntile); lane per column
+// block = (run of MULT_ROWS rows, column tile of ntile); lane per column.
+// Fixed-order sums: a row's sum over the tile's columns goes to
+// rowpart[tile][row] (wave_sum, then the waves' slots in order), a column's
+// sum over the block's rows to hhpart[row run][column]; k_sum_parts adds the
+// partials in order.
 __global__ void __launch_bounds__(256) k_mult_rows(i64 na, i64 np, i64 nrows, i64 ncols, i64 ntile,
                                                    const double *__restrict__ DAE, const double *__restrict__ DP,
                                                    const double *__restrict__ W, const double *__restrict__ G,
-                                                   double *__restrict__ rowsum, double *__restrict__ HH)
+                                                   double *__restrict__ rowpart, double *__restrict__ hhpart)
 {
-    __shared__ double rs[MULT_ROWS];
-    i64 c = ((i64)blockIdx.x % ntile) * blockDim.x + threadIdx.x;
-    i64 R0 = ((i64)blockIdx.x / ntile) * MULT_ROWS, nr = std::min<i64>(MULT_ROWS, nrows - R0);
-    if (threadIdx.x < MULT_ROWS) rs[threadIdx.x] = 0.0;
-    __syncthreads();
+    __shared__ double rsw[4][MULT_ROWS];
+    const i64 tile = (i64)blockIdx.x % ntile, grp = (i64)blockIdx.x / ntile;
+    i64 c = tile * blockDim.x + threadIdx.x;
+    i64 R0 = grp * MULT_ROWS, nr = std::min<i64>(MULT_ROWS, nrows - R0);
     bool on = c < ncols;
     double w = (on && W) ? W[c] : 0.0, h = 0.0;
-    const bool lead = (threadIdx.x & 63) == 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     i64 na2 = na * na;
     for (i64 k = 0; k < nr; ++k) {
         i64 R = R0 + k, pp = R / na2, a = R % na2;
         double g = on ? G[R * ncols + c] : 0.0;
         double wr = DP[(pp % np) * np + pp / np] * DAE[(a % na) * na + a / na];
         h += wr * g;
-        if (rowsum) {
+        if (rowpart) {
             double v = wave_sum(w * g);
-            if (lead) atomicAdd(&rs[k], v);
+            if (lane == 0) rsw[wv][k] = v;
         }
     }
     __syncthreads();
-    if (rowsum && threadIdx.x < nr) atomicAdd(&rowsum[R0 + threadIdx.x], rs[threadIdx.x]);
-    if (HH && on) atomicAdd(&HH[c], h);
+    if (rowpart && threadIdx.x < nr) {
+        double t = 0.0;
+        for (int q = 0; q < nw; ++q) t += rsw[q][threadIdx.x];
+        rowpart[tile * nrows + R0 + threadIdx.x] = t;
+    }
+    if (hhpart && on) hhpart[grp * ncols + c] = h;
+}
+
+// out[i] = sum over p of part[p][i], p in order
+__global__ void k_sum_parts(i64 n, i64 nparts, const double *__restrict__ part, double *__restrict__ out)
+{
+    i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (i64 p = 0; p < nparts; ++p) t += part[p * n + i];
+    out[i] = t;
 }
 
 // FAE(i1,j1) += sum_pp DP rowsum;  FP(ip,jp) += sum_a DAE rowsum;
@@ -4859,23 +4870,25 @@ static int mult_omg_core(int mode, i64 na, i64 nb, i64 np, const double *d_dae, 
     int k = mode == 4 ? 2 : mode;
     i64 nrows = np * np * na * na, ncols = 1;
     for (int i = 0; i < k; ++i) ncols *= nb * nb;
+    unsigned nt = (unsigned)std::min<i64>(256, ((ncols + 63) / 64) * 64);
+    const i64 ntile = (ncols + nt - 1) / nt, ngrp = (nrows + MULT_ROWS - 1) / MULT_ROWS, nblk = ntile * ngrp;
+    if (nblk > 0x7fffffff) return fail(NEORX_E_UNSUPPORTED, "tensor too large for one launch");
+    // rowsum, W, HH, then the partials of the row and column sums
     StreamBuf ws;
-    int rc = ws.alloc((nrows + 2 * ncols) * 8, st);
+    int rc = ws.alloc((nrows + 2 * ncols + ntile * nrows + (d_fbe ? ngrp * ncols : 0)) * 8, st);
     if (rc) return rc;
     double *rowsum = (double *)ws.p, *W = rowsum + nrows, *HH = d_fbe ? W + ncols : nullptr;
-    hipLaunchKernelGGL(k_fill0, dim3(nblocks(nrows, 256)), dim3(256), 0, st, nrows, rowsum);
-    if (HH) hipLaunchKernelGGL(k_fill0, dim3(nblocks(ncols, 256)), dim3(256), 0, st, ncols, HH);
+    double *rowpart = W + 2 * ncols, *hhpart = d_fbe ? rowpart + ntile * nrows : nullptr;
     hipLaunchKernelGGL(k_mult_weights, dim3(nblocks(ncols, 256)), dim3(256), 0, st, mode, nb, ncols, d_dbe, W);
     HIPCHK(hipGetLastError());
-    unsigned nt = (unsigned)std::min<i64>(256, ((ncols + 63) / 64) * 64);
-    i64 ntile = (ncols + nt - 1) / nt, nblk = ntile * ((nrows + MULT_ROWS - 1) / MULT_ROWS);
-    if (nblk > 0x7fffffff) return fail(NEORX_E_UNSUPPORTED, "tensor too large for one launch");
     bool sep = K != G;
     hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae, d_dp, W,
-                       G, rowsum, sep ? nullptr : HH);
+                       G, rowpart, sep ? nullptr : hhpart);
     if (sep && HH)
         hipLaunchKernelGGL(k_mult_rows, dim3((unsigned)nblk), dim3(nt), 0, st, na, np, nrows, ncols, ntile, d_dae,
-                           d_dp, nullptr, K, nullptr, HH);
+                           d_dp, nullptr, K, nullptr, hhpart);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblocks(nrows, 256)), dim3(256), 0, st, nrows, ntile, rowpart, rowsum);
+    if (HH) hipLaunchKernelGGL(k_sum_parts, dim3(nblocks(ncols, 256)), dim3(256), 0, st, ncols, ngrp, hhpart, HH);
     HIPCHK(hipGetLastError());
     i64 nf = std::max(std::max(na * na, np * np), nb * nb);
     hipLaunchKernelGGL(k_mult_finish, dim3(nblocks(nf, 256)), dim3(256), 0, st, mode, na, nb, np, d_dae, d_dbe, d_dp,

@@ -705,6 +705,10 @@ def test_rxchfmult(torch_dev):
             eo = oracle.rxchfmult_omg(order, which, na, nb, npb, DA, DB, DP, G, B)
             check_close(A - F0, B - F0, what=f"RXCHFmult omg{order} {which}")
             assert eg == pytest.approx(eo, rel=1e-11, abs=1e-14)
+            # fixed-order sums: a second call agrees bit for bit
+            A2 = F0.copy(order="F")
+            assert neorx.RXCHFmult_OMG(which, order, na, nb, npb, DA, DB, DP, G, A2) == eg
+            assert np.array_equal(A2, A)
 
 
 def test_rxchf(torch_dev):
