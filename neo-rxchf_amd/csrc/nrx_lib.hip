@@ -2759,42 +2759,49 @@ extern "C" int neorx_rxchfne_gam2_ic1(int64_t handle, double *gm2icr, double *gm
 //   v12(i2,j2) = sum_{i1 j1} DBE(i1,j1) G(i1 j1 i2 j2)
 //   v21(i2,j2) = sum_{i1 j1} DBE(i1,j1) Gx(i1 j2 i2 j1)
 //   R(i1,j1)   = sum_{i2 j2} DT(i2,j2) G(i1 j1 i2 j2) - DX(i2,j2) Gx(i1 j2 i2 j1)
+//                (k_rxne_slab_rows)
 // (Gx, DX absent in the closed-shell routines).  part[pp] = [v12, v21, R].
 __global__ void __launch_bounds__(1024) k_rxne_slab(i64 ne, const double *__restrict__ DBE,
-                                                    const double *__restrict__ DT, const double *__restrict__ DX,
                                                     const double *__restrict__ G, const double *__restrict__ Gx,
                                                     double *__restrict__ part)
 {
-    extern __shared__ double R[]; // [ne^2]
     i64 pp = blockIdx.x, ne2 = ne * ne, ne4 = ne2 * ne2;
     const double *g = G + pp * ne4, *gx = Gx ? Gx + pp * ne4 : nullptr;
     double *out = part + pp * 3 * ne2;
-    for (i64 k = threadIdx.x; k < ne2; k += blockDim.x) R[k] = 0.0;
-    __syncthreads();
-    const bool lead = (threadIdx.x & 63) == 0;
-    for (i64 c0 = 0; c0 < ne2; c0 += blockDim.x) {
-        i64 c = c0 + threadIdx.x;
-        bool on = c < ne2;
-        i64 i2 = on ? c / ne : 0, j2 = on ? c % ne : 0;
-        double dt = on ? DT[j2 * ne + i2] : 0.0, dx = (on && DX) ? DX[j2 * ne + i2] : 0.0;
+    for (i64 c = threadIdx.x; c < ne2; c += blockDim.x) {
+        i64 i2 = c / ne, j2 = c % ne;
         double v12 = 0.0, v21 = 0.0;
         for (i64 i1 = 0; i1 < ne; ++i1)
             for (i64 j1 = 0; j1 < ne; ++j1) {
                 double d1 = DBE[j1 * ne + i1];
-                double a = on ? g[(i1 * ne + j1) * ne2 + c] : 0.0;
-                double b = (on && gx) ? gx[((i1 * ne + j2) * ne + i2) * ne + j1] : 0.0;
-                v12 += d1 * a;
-                v21 += d1 * b;
-                double r = wave_sum(dt * a - dx * b);
-                if (lead) atomicAdd(&R[i1 * ne + j1], r);
+                v12 += d1 * g[(i1 * ne + j1) * ne2 + c];
+                if (gx) v21 += d1 * gx[((i1 * ne + j2) * ne + i2) * ne + j1];
             }
-        if (on) {
-            out[c] = v12;
-            out[ne2 + c] = v21;
-        }
+        out[c] = v12;
+        out[ne2 + c] = v21;
     }
-    __syncthreads();
-    for (i64 k = threadIdx.x; k < ne2; k += blockDim.x) out[2 * ne2 + k] = R[k];
+}
+
+// R of k_rxne_slab's block pp: one wave per row r = (i1,j1), lanes over the
+// columns (i2,j2) in a fixed stride, then wave_sum.  Grid (np^2, ceil(ne^2/4)).
+__global__ void __launch_bounds__(256) k_rxne_slab_rows(i64 ne, const double *__restrict__ DT,
+                                                        const double *__restrict__ DX, const double *__restrict__ G,
+                                                        const double *__restrict__ Gx, double *__restrict__ part)
+{
+    const i64 pp = blockIdx.x, ne2 = ne * ne, ne4 = ne2 * ne2;
+    const i64 r = (i64)blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (r >= ne2) return; // whole waves leave together
+    const i64 i1 = r / ne, j1 = r % ne;
+    const double *g = G + pp * ne4 + r * ne2, *gx = Gx ? Gx + pp * ne4 : nullptr;
+    double acc = 0.0;
+    for (i64 c = threadIdx.x & 63; c < ne2; c += 64) {
+        const i64 i2 = c / ne, j2 = c % ne;
+        double v = DT[j2 * ne + i2] * g[c];
+        if (gx) v -= DX[j2 * ne + i2] * gx[((i1 * ne + j2) * ne + i2) * ne + j1];
+        acc += v;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[pp * 3 * ne2 + 2 * ne2 + r] = acc;
 }
 
 // mode 0: F1 += sum_pp DP (v12 - v21), F2 (may be null) += sum_pp DP v12;
@@ -2860,7 +2867,9 @@ static int rxne_contract(int mode, i64 ne, i64 np, const double *dbe, const doub
     if (!rc) rc = part.alloc(np * np * 3 * ne * ne * 8);
     if (rc) return rc;
     unsigned nt = (unsigned)std::min<i64>(((ne * ne + 63) / 64) * 64, 1024);
-    hipLaunchKernelGGL(k_rxne_slab, dim3((unsigned)(np * np)), dim3(nt), ne * ne * 8, 0, ne, dB.d(), dT.d(),
+    hipLaunchKernelGGL(k_rxne_slab, dim3((unsigned)(np * np)), dim3(nt), 0, 0, ne, dB.d(), dG.d(),
+                       gm2ex ? dGx.d() : nullptr, part.d());
+    hipLaunchKernelGGL(k_rxne_slab_rows, dim3((unsigned)(np * np), nblocks(ne * ne, 4)), dim3(256), 0, 0, ne, dT.d(),
                        dx ? dX.d() : nullptr, dG.d(), gm2ex ? dGx.d() : nullptr, part.d());
     hipLaunchKernelGGL(k_rxne_reduce, dim3(nblocks(std::max(ne * ne, np * np), 256)), dim3(256), 0, 0, ne, np, mode,
                        dP.d(), dB.d(), part.d(), dF1.d(), f2 ? dF2.d() : nullptr, dE.d());

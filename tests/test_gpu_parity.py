@@ -567,14 +567,18 @@ def test_rxchfne(torch_dev):
         assert eg == pytest.approx(eo, rel=1e-11, abs=1e-15)
         wu = {"fae": "fe", "fp": "fp", "fbe": "fbe"}[which]
         A, B = F0.copy(order="F"), F0.copy(order="F")
-        A2 = np.asfortranarray(rng.standard_normal((ne, ne)))
-        B2 = A2.copy(order="F")
+        F20 = np.asfortranarray(rng.standard_normal((ne, ne)))
+        A2, B2 = F20.copy(order="F"), F20.copy(order="F")
         eg = neorx.RXCUHFne_OMG2(wu, ne, npb, DAa, DA, DB, DP, og, ogx, A, A2)
         eo = oracle.rxcuhfne_omg2(k, ne, npb, DAa, DA, DB, DP, og, ogx, B, B2)
         check_close(A, B, what=f"RXCUHFne_{wu}_OMG2")
         if wu == "fe":
             check_close(A2, B2, what="RXCUHFne_FE_OMG2 beta")
         assert eg == pytest.approx(eo, rel=1e-11, abs=1e-15)
+        # fixed-order sums: a second call agrees bit for bit
+        A3, A4 = F0.copy(order="F"), F20.copy(order="F")
+        assert neorx.RXCUHFne_OMG2(wu, ne, npb, DAa, DA, DB, DP, og, ogx, A3, A4) == eg
+        assert np.array_equal(A3, A) and np.array_equal(A4, A2)
     gee = oracle.gam_ee(t)
     F0 = np.asfortranarray(rng.standard_normal((ne, ne)))
     A, B = F0.copy(order="F"), F0.copy(order="F")
