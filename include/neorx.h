@@ -521,7 +521,10 @@ int neorx_autotune_eg3_dev(int64_t nebf, int64_t npbf, int64_t dtype, const doub
  * store: bit mask of the storage types GAM_3 is kept in (1 FP64, 2 FP32,
  * 4 BF16); 0 means FP64; bit 8 also autotunes the EG3 contraction for each
  * stored type (neorx_autotune_eg3_dev).  The integrals are only built here:
- * these entry points read no .ufm file. */
+ * these entry points read no .ufm file.
+ * Each context owns a non-blocking stream; neorx_xchf_scf on different
+ * contexts may run from different host threads at once (graph capture is
+ * thread-local).  Calls on one context must not overlap. */
 int neorx_xchf_create(int64_t handle, int64_t nelec, int64_t store, int64_t *ctx);
 int neorx_xchf_destroy(int64_t ctx);
 /* Host copies of the context's integrals in the reference packings (any
@@ -570,10 +573,11 @@ int neorx_roothan_dev(int64_t n, const double *d_s, const double *d_f, int64_t k
  *   mode:     0 FP64 GAM_3; 1 FP32 until the switch signal < sw_lo, then
  *             FP64; 2 BF16 until < sw_hi, then FP32 until < sw_lo, then FP64
  *             (signal: the DIIS error max|X^T(FDS - SDF)X| with DIIS, else
- *             max|dDE|); 3 (incremental) BF16 until < sw_hi, then an FP64
- *             build taken as the reference and BF16-stored increments from
- *             it, a new FP64 reference once < sw_lo, and an FP64 build to
- *             confirm convergence; convergence is only declared on an FP64
+ *             max|dDE|); 3 (incremental) BF16 until < sw_hi, then a build
+ *             taken as the reference (from the FP32 copy if the context
+ *             holds one, else FP64) and BF16-stored increments from it, an
+ *             FP64 reference once < sw_lo, and an FP64 build to confirm
+ *             convergence; convergence is only declared on an FP64
  *             (non-incremental) build;
  *   flags:    bit 1: capture one iteration as a hipGraph and replay it;
  *             bit 2: DIIS;
