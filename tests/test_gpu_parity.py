@@ -598,14 +598,15 @@ def _round_storage(g, prec):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("ne,npb", [(36, 2), (7, 3), (24, 2), (32, 1)])
+@pytest.mark.parametrize("ne,npb", [(36, 2), (7, 3), (24, 2), (32, 1), (48, 1)])
 def test_rxchfne_one_pass(torch_dev, ne, npb):
     """neorx_rxchfne_omg2_dev: RXCHFne_FAE/FP/FBE_OMG2 from one pass over GM2
     stored in FP64, FP32 and BF16, against the oracle's three routines over
     the same rounded tensor.  The chunk widths differ: nebf 36 takes two
     chunks of 12 columns per lane (the second partial) and many row splits,
     nebf 7 one partial chunk of 8, nebf 24 one chunk of 9, nebf 32 one of 16
-    with a single nuclear function.  Bars: 1e-11 for FP64 and FP32 storage
+    with a single nuclear function, nebf 48 (rows of 2304 entries) one chunk
+    of 9 loads of four BF16 columns per lane.  Bars: 1e-11 for FP64 and FP32 storage
     (FP64 sums); for BF16, whose sums run in FP32, 1e-5 of the same
     contraction over absolute values, element by element."""
     torch = torch_dev
