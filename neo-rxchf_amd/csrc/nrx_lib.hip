@@ -3067,9 +3067,9 @@ __global__ void __launch_bounds__(64) k_rxne_e(i64 np, int nq, const double *__r
 // 64 KC V columns pad the ne^2 columns least (the larger on a tie)
 static int rxne_kc(i64 ne2, int V)
 {
-    static const int c1[] = {16, 12, 9, 8}, c2[] = {8, 6, 4}, c4[] = {9, 4, 3, 2};
+    static const int c1[] = {16, 12, 9, 8}, c2[] = {8, 6, 4}, c4[] = {4, 3, 2};
     const int *c = V == 1 ? c1 : V == 2 ? c2 : c4;
-    const int n = V == 2 ? 3 : 4;
+    const int n = V == 1 ? 4 : 3;
     int best = c[0];
     i64 pad = -1;
     for (int i = 0; i < n; ++i) {
@@ -3106,7 +3106,6 @@ static void rxne_pass_kernel(int kc, i64 ne, i64 np2, int nsplit, const double *
             NRX_RXNE_KC(2);
             NRX_RXNE_KC(3);
             NRX_RXNE_KC(4);
-            NRX_RXNE_KC(9);
         }
     }
 #undef NRX_RXNE_KC

@@ -605,8 +605,8 @@ def test_rxchfne_one_pass(torch_dev, ne, npb):
     the same rounded tensor.  The chunk widths differ: nebf 36 takes two
     chunks of 12 columns per lane (the second partial) and many row splits,
     nebf 7 one partial chunk of 8, nebf 24 one chunk of 9, nebf 32 one of 16
-    with a single nuclear function, nebf 48 (rows of 2304 entries) one chunk
-    of 9 loads of four BF16 columns per lane.  Bars: 1e-11 for FP64 and FP32 storage
+    with a single nuclear function, nebf 48 (rows of 2304 entries, the C3
+    size) three chunks of 3 loads of four BF16 columns per lane.  Bars: 1e-11 for FP64 and FP32 storage
     (FP64 sums); for BF16, whose sums run in FP32, 1e-5 of the same
     contraction over absolute values, element by element."""
     torch = torch_dev
